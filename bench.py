@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Throughput of the slab particle step on one MI355X.
+
+Workload (default): the liquid-vapour planar interface — periodic 400 sigma
+box, 4096 cells, 20M particles, n sigma^3 = 0.55 liquid slab in 0.02 vapour,
+T = 0.7, Enskog collisions plus the screened-Poisson mean field, FP32
+particles, FP64 field, dt = 0.02.  One step = collisions, field solve, kick,
+streaming, periodic wrap and the stable re-sort into cells.
+
+Prints one JSON line.  `value` is particle-steps per second with the
+particles resident in HBM.  `roofline` describes the dominant kernel class,
+timed with HIP events on the library's own stream; `cpu_baseline` is the CPU
+oracle timed on this host on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "monte-carlo-particle-method_amd")
+ORACLE = os.path.join(ROOT, "oracle")
+sys.path.insert(0, PKG)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--particles", type=int, default=20_000_000)
+    ap.add_argument("--cells", type=int, default=4096)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-particles", type=int, default=2_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU decomposition is a design, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+
+    import mcpm
+    import workloads as W
+
+    N = args.particles
+    p, arrs, meta = W.interface(N=N, n_cells=args.cells)
+    log(f"workload: interface N={N} cells={args.cells} dt={p.delta_t} FP32")
+    sim = mcpm.Simulation(p)
+    sim.upload(*arrs)
+    del arrs
+    sim.cell_update()
+    sim.init_omega_max(meta["n_gas"])
+    step = 0
+    t0 = time.time()
+    for _ in range(args.warmup):
+        st = sim.step(step)
+        step += 1
+    sim.sync()
+    log(f"warmup {args.warmup} steps in {time.time() - t0:.2f}s, last {st}")
+
+    sim.timing_enable(True)
+    sim.sync()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stats.append(sim.step(step))
+        step += 1
+    sim.sync()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = N * args.steps / elapsed
+    ms_step = 1000.0 * elapsed / args.steps
+    timers = {}
+    for name, k in (("collide_rounds", mcpm.TIMER_COLLIDE), ("sort", mcpm.TIMER_SORT),
+                    ("candidates", mcpm.TIMER_GEN), ("field", mcpm.TIMER_FIELD),
+                    ("step", mcpm.TIMER_STEP)):
+        ms, calls = sim.timing(k)
+        timers[name] = {"avg_ms": ms, "calls": calls}
+    sim.timing_enable(False)
+    acc = float(np.mean([s["accepted"] for s in stats]))
+    valid = float(np.mean([s["valid"] for s in stats]))
+    rounds = float(np.mean([s["rounds"] for s in stats]))
+    log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, {value:.4g} particle-steps/s")
+    log(f"phases: {json.dumps(timers)}; valid ops {valid:.0f}, accepted {acc:.0f}, rounds {rounds}")
+
+    # Dominant kernel class and its roofline.  Algorithmic bytes:
+    #  sort (push pass 1 + scatter pass): read 16 B + read 16 B + write 16 B per
+    #  particle (pass 1 reads the record, pass 2 reads it again and writes it);
+    #  collide rounds: per valid operation the two particle records are read
+    #  and written (4 x 16 B) plus the operation itself (5 x 4 B).
+    sort_ms = timers["sort"]["avg_ms"]
+    coll_ms = timers["collide_rounds"]["avg_ms"]
+    sort_bytes = 48.0 * N
+    coll_bytes = valid * (64.0 + 20.0)
+    if coll_ms >= sort_ms:
+        dom, dom_ms, dom_bytes = "collide_rounds (k_round)", coll_ms, coll_bytes
+    else:
+        dom, dom_ms, dom_bytes = "sort (k_push x2)", sort_ms, sort_bytes
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": None,
+                "step_alg_bytes_per_particle": 36,
+                "step_frac": round(value * 36.0 / 1e9 / HBM_PEAK_GBS, 4)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args)
+
+    out = {
+        "metric": "particle-steps/s on 1 MI355X (Enskog-Vlasov interface step)",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 particles / fp64 field and moments",
+        "data": "synthetic (numpy Philox initial state)",
+        "config": {"workload": "interface: periodic 400 sigma, 4096 cells, 20M particles, "
+                               "n=0.55/0.02, T=0.7, ESMC + screened-Poisson, dt=0.02",
+                   "particles": N, "cells": args.cells, "parallelism": "1 GPU",
+                   "valid_ops_per_step": valid, "accepted_per_step": acc,
+                   "reservation_rounds": rounds, "phases_ms": {k: v["avg_ms"] for k, v in timers.items()}},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(args):
+    """The CPU oracle (one thread) on the same workload at reduced N."""
+    sys.path.insert(0, ORACLE)
+    import oracle as orc
+    import workloads as W
+
+    n = args.cpu_particles
+    cells = max(64, int(round(args.cells * n / args.particles))) if n < args.particles else args.cells
+    cells = args.cells  # same mesh; fewer particles per cell
+    p, arrs, meta = W.interface(N=n, n_cells=cells)
+    o = orc.OracleSim(p)
+    o.upload(*arrs)
+    o.cell_update()
+    o.init_omega_max(meta["n_gas"])
+    o.step(0)  # one untimed step
+    t0 = time.perf_counter()
+    for s in range(1, 1 + args.cpu_steps):
+        o.step(s)
+    dt = time.perf_counter() - t0
+    v = n * args.cpu_steps / dt
+    log(f"cpu oracle: {args.cpu_steps} steps of {n} particles in {dt:.2f}s -> {v:.4g} particle-steps/s")
+    return {"value": v, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"interface workload at {n} particles, {cells} cells, {args.cpu_steps} steps "
+                      "after 1 untimed step (C++ oracle, g++ -O3, one thread)"}
+
+
+if __name__ == "__main__":
+    main()

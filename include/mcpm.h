@@ -1,0 +1,199 @@
+/*
+ * mcpm.h — C ABI of the MI355X particle Monte Carlo step (libmcpm.so).
+ *
+ * Plain C: POD structs, plain pointers and sizes, integer status codes.
+ * Each entry point replaces one per-step operator of the reference program
+ * mohsensadr/Monte-Carlo-Particle-Method; the comment above each names the
+ * reference function (prototype in src/cpp_headers.h, body file:line).
+ *
+ * Ownership and residency.  The reference keeps particles in caller-owned
+ * host arrays x2[], U1[], U2[], U3[] (main.cpp:104-165) and per-cell state
+ * in struct CELLS.  Here a context owns the device copy: particles are
+ * uploaded once, stay resident in HBM across steps and come back to the
+ * host only on request.  Particles are kept physically ordered by cell
+ * (a stable counting sort of the previous order), so the reference's
+ * cells[c].indices_inside[u] is simply start[c] + u.
+ *
+ * Threading: one context is one HIP stream; calls on one context must not
+ * overlap.  All calls return MCPM_OK or an error code; nothing exits the
+ * process (the reference calls exit(1)/exit(2) on the same conditions).
+ */
+#ifndef MCPM_H
+#define MCPM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define MCPM_OK            0
+#define MCPM_ERR_ARG       1  /* bad argument or unsupported parameter combination */
+#define MCPM_ERR_HIP       2  /* HIP runtime failure (no device, launch failure)    */
+#define MCPM_ERR_CAPACITY  3  /* more particles / candidates than the context holds */
+#define MCPM_ERR_DOMAIN    4  /* particle outside the domain: Main_Functions.cpp:2248-2263 */
+#define MCPM_ERR_NAN       5  /* non-finite velocity: Fokker_Planck.cpp:1137-1149    */
+#define MCPM_ERR_BC        6  /* >20 wall interactions in one step: Main_Functions.cpp:4958-4965 */
+#define MCPM_ERR_STATE     7  /* call out of order (e.g. step before upload)       */
+
+/* ---- enumerations ------------------------------------------------------ */
+enum mcpm_precision { MCPM_FP64 = 0, MCPM_FP32 = 1 };
+/* Boundary treatment along the slab axis (the reference's x2 / y axis).
+   DIFFUSE: BC_thermal_wall_3D (Main_Functions.cpp:4917-5248), walls at
+   wall_lo / wall_hi.  SPECULAR: BC_specular_reflection
+   (Main_Functions.cpp:6399-6525).  PERIODIC: a periodic box, which the
+   reference's 1D screened-Poisson path does not offer. */
+enum mcpm_boundary { MCPM_BC_DIFFUSE = 0, MCPM_BC_PERIODIC = 1, MCPM_BC_SPECULAR = 2 };
+/* Contact value Y(n): the reference's active polynomial
+   (Main_Functions.cpp:1906-1915) or Carnahan-Starling (the commented
+   alternative at Main_Functions.cpp:1912). */
+enum mcpm_ymodel { MCPM_Y_REFERENCE = 0, MCPM_Y_CARNAHAN_STARLING = 1 };
+/* Long-range attraction switch of velocity_update (Main_Functions.cpp:1612-1668). */
+enum mcpm_long_range { MCPM_LR_NONE = 0, MCPM_LR_SCREENED_POISSON = 3 };
+
+/* ---- parameters -------------------------------------------------------- */
+/* The subset of the reference's struct GAS / struct BOX that the slab step
+   reads (cpp_headers.h:145-269), in plain C types.  Lengths, masses and
+   temperatures are in the reference's own units. */
+typedef struct mcpm_params {
+  int32_t  precision;    /* enum mcpm_precision: particle storage type        */
+  int32_t  boundary;     /* enum mcpm_boundary                                 */
+  int32_t  y_model;      /* enum mcpm_ymodel                                   */
+  int32_t  long_range;   /* enum mcpm_long_range (gas.long_range)              */
+  int32_t  n_cells;      /* box.N[1]: cells along the slab axis                */
+  int32_t  reserved0;
+  int64_t  capacity;     /* maximum number of particles the context holds      */
+  uint64_t seed;         /* Philox key for every random draw                   */
+  double   length;       /* box.Len[1]                                          */
+  double   wall_lo;      /* lower diffuse wall (reference: sigma + 1e-15)      */
+  double   wall_hi;      /* upper diffuse wall (reference: Len - sigma - 1e-15)*/
+  double   T_wall_lo;    /* box.T_wall_1                                        */
+  double   T_wall_hi;    /* box.T_wall_2                                        */
+  double   U_wall_lo;    /* box.U_wall_1: tangential wall velocity              */
+  double   U_wall_hi;    /* box.U_wall_2                                        */
+  double   sigma;        /* gas.sigma: hard-sphere diameter                     */
+  double   mass;         /* gas.m                                               */
+  double   kb;           /* gas.kb                                              */
+  double   b;            /* gas.b = 2 pi sigma^3 / 3 (myIO.cpp:2828)            */
+  double   Fn;           /* gas.Fn: real molecules per simulated particle       */
+  double   cell_volume;  /* cells[c].volume (uniform mesh)                      */
+  double   n_ratio;      /* cells[c].n_ratio (1 unless zoomed)                  */
+  double   delta_t;      /* gas.delta_t                                         */
+  double   ast;          /* gas.ast: Yukawa amplitude (myIO.cpp:2891)           */
+  double   bst;          /* gas.bst: Yukawa screening lambda (myIO.cpp:2892)    */
+  double   eqdist;       /* gas.eqdist: Dirichlet quadrature reach / 3          */
+} mcpm_params;
+
+typedef struct mcpm_ctx mcpm_ctx;
+
+/* Per-step collision statistics (the reference prints the equivalent). */
+typedef struct mcpm_collision_stats {
+  int64_t candidates;    /* sum over cells of m_c                               */
+  int64_t valid;         /* candidates with a partner j != i                    */
+  int64_t accepted;      /* collisions performed                                */
+  int32_t rounds;        /* parallel rounds the reservation schedule took       */
+  int32_t reserved0;
+} mcpm_collision_stats;
+
+/* ---- lifetime ---------------------------------------------------------- */
+/* Create a context on the current HIP device.  No reference counterpart:
+   replaces the malloc of particle and cell arrays in main.cpp:74-165. */
+int mcpm_create(const mcpm_params *p, mcpm_ctx **out);
+int mcpm_destroy(mcpm_ctx *ctx);
+const char *mcpm_status_string(int status);
+/* Block until all work queued on the context's stream has finished. */
+int mcpm_sync(mcpm_ctx *ctx);
+
+/* ---- particle data ------------------------------------------------------ */
+/* Copy n particles from host SoA arrays (the reference's x2, U1, U2, U3,
+   main.cpp:104-165) into the context.  Any order; mcpm_cell_update sorts. */
+int mcpm_upload(mcpm_ctx *ctx, int64_t n, const double *y,
+                const double *u1, const double *u2, const double *u3);
+/* Copy the particles back in the current (cell-sorted) order. */
+int mcpm_download(mcpm_ctx *ctx, double *y, double *u1, double *u2, double *u3);
+int64_t mcpm_particle_count(const mcpm_ctx *ctx);
+
+/* ---- per-step operators -------------------------------------------------- */
+/* cell_update (cpp_headers.h:501; Main_Functions.cpp:2138-2749): bin every
+   particle into cell floor(y / dy), rebuild the per-cell index lists in
+   stable order, and set num_inside and n = n_ratio Fn N_c / V.  Here the
+   index lists become a physical stable counting sort of the particles. */
+int mcpm_cell_update(mcpm_ctx *ctx);
+
+/* Collision_ESMC_new3 (cpp_headers.h:365; collisions.cpp:583-720): Enskog
+   SMC collisions with the Y(n) contact factor.  `step` addresses the random
+   stream. */
+int mcpm_collide_esmc(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
+
+/* poisson_solver (cpp_headers.h:450; Main_Functions.cpp:3285-3581): the 1D
+   screened-Poisson mean field and the kick U2 += F2 dt.  This call only
+   solves for the per-cell force F2; the kick is applied by mcpm_stream_bc
+   (or mcpm_kick) in the same pass that moves the particles. */
+int mcpm_poisson_solver(mcpm_ctx *ctx);
+
+/* The velocity kick of poisson_solver (Main_Functions.cpp:3574-3577) alone,
+   U2 += F2[cell] dt, for callers that want the reference's operator split. */
+int mcpm_kick(mcpm_ctx *ctx);
+
+/* stream_particles + apply_BC (cpp_headers.h:495,498; Main_Functions.cpp:
+   1451-1521, 4917-5248; Fokker_Planck.cpp:948-969), followed by the binning
+   of cell_update: y += U2 dt, wall re-emission or periodic wrap, re-sort.
+   When `with_kick` is non-zero the pending F2 kick is applied first, fused
+   into the same pass. */
+int mcpm_stream_bc(mcpm_ctx *ctx, int64_t step, int32_t with_kick);
+
+/* One whole time step in the reference's order (main.cpp:389-669):
+   collisions, mean-field solve, kick + stream + boundary + re-sort. */
+int mcpm_step(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
+
+/* ---- cell state ---------------------------------------------------------- */
+/* omega_max initialisation of main.cpp:250-269: crm = 10 sqrt(kb T_c / m),
+   omega_max = 4 pi sigma^2 crm n_c dt, times Y(n_gas) for ESMC.  T_c is
+   taken from per-cell moments; `n_gas` is gas.n. */
+int mcpm_init_omega_max(mcpm_ctx *ctx, double n_gas, int32_t times_y);
+int mcpm_set_omega_max(mcpm_ctx *ctx, const double *omega_max);
+
+/* Cell arrays in cell order; any pointer may be NULL.  count = num_inside,
+   start = offset of the cell's first particle, n = number density,
+   omega_max, F2 = mean-field force per unit mass on the slab axis. */
+int mcpm_get_cells(mcpm_ctx *ctx, int32_t *count, int32_t *start, double *n,
+                   double *omega_max, double *F2);
+
+/* Per-cell moments of cell_update (Main_Functions.cpp:2461-2593), FP64,
+   two-pass: out[c*MCPM_NMOM + k] with k = 0..2 U_space, 3..8 PIJ (11,12,
+   13,22,23,33), 9..18 M3, 19..24 M4, 25..27 M5, 28 DM2, 29 T, 30 N_c. */
+#define MCPM_NMOM 31
+int mcpm_moments(mcpm_ctx *ctx, double *out);
+
+/* cell_poly_velocity (cpp_headers.h:470; Main_Functions.cpp:1918-1968):
+   accumulate the per-cell time-average sums.  mcpm_get_samples returns
+   out[c*MCPM_NSAMP + k]: 0 sum_weight, 1..3 sum_M, 4..9 sum_MM, 10..19
+   sum_MMM (m,n,l order of the reference), 20..25 sum_MMMM. */
+#define MCPM_NSAMP 26
+int mcpm_sample(mcpm_ctx *ctx);
+int mcpm_get_samples(mcpm_ctx *ctx, double *out);
+int mcpm_reset_samples(mcpm_ctx *ctx);
+
+/* increase_collision_rate (cpp_headers.h:505; Main_Functions.cpp:1906-1915),
+   evaluated on the host with the context's parameters. */
+double mcpm_contact_y(const mcpm_params *p, double n);
+
+/* ---- diagnostics --------------------------------------------------------- */
+/* Record the outcome of every valid collision candidate of the next
+   mcpm_collide_esmc call (for parity checks): priority, i, j, accepted. */
+int mcpm_debug_collision_log(mcpm_ctx *ctx, int32_t enable);
+int mcpm_get_collision_log(mcpm_ctx *ctx, int64_t cap, uint32_t *prio, uint32_t *i,
+                           uint32_t *j, uint8_t *accepted, int64_t *n_out);
+
+/* Average device duration (ms) of each kernel class over the calls since the
+   last reset, measured with HIP events on the context's stream.  `which`:
+   0 collide rounds, 1 sort (push+scatter), 2 candidate generation, 3 field,
+   4 whole step. */
+int mcpm_timing_enable(mcpm_ctx *ctx, int32_t enable);
+int mcpm_timing_get(mcpm_ctx *ctx, int32_t which, double *avg_ms, int64_t *calls);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MCPM_H */

@@ -1,0 +1,789 @@
+// mcpm_api.hip — host layer of libmcpm.so: the context, device memory,
+// launch sequencing and the extern "C" entry points declared in mcpm.h.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <algorithm>
+#include <utility>
+#include <vector>
+
+#include "mcpm_kernels.h"
+
+using namespace mcpm;
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+enum { T_COLLIDE = 0, T_SORT = 1, T_GEN = 2, T_FIELD = 3, T_STEP = 4, T_NCLASS = 5 };
+
+}  // namespace
+
+struct mcpm_ctx {
+  mcpm_params p;
+  KParams kp;
+  int device = 0;
+  hipStream_t s = nullptr;
+  int nc = 0;
+  int64_t n = 0;
+  bool sorted = false;
+  bool kick_pending = false;
+  size_t rec = 0;
+  void *P[2] = {nullptr, nullptr};
+  int cur = 0;
+  // sort
+  int64_t nchunks_cap = 0;
+  int seg = 32;
+  uint32_t *table = nullptr, *partial = nullptr;
+  // cells
+  uint32_t *cnt = nullptr, *start = nullptr, *mcand = nullptr, *cstart = nullptr;
+  double *nden = nullptr, *yc = nullptr, *omax = nullptr, *omega_new = nullptr, *F2 = nullptr;
+  double *moments = nullptr, *samples = nullptr;
+  uint32_t *counters = nullptr, *err = nullptr;
+  uint32_t *res[2] = {nullptr, nullptr};
+  // collisions
+  OpArrays ops{};
+  uint32_t *list[2] = {nullptr, nullptr};
+  int64_t op_cap = 0;
+  uint8_t *log_acc = nullptr;
+  bool debug_log = false;
+  int64_t last_ops = 0;
+  // field
+  FieldArrays fa{};
+  double *field_buf = nullptr;
+  // timing
+  bool timing = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[T_NCLASS];
+  std::vector<hipEvent_t> ev_pool;
+};
+
+namespace {
+
+int hip_status(hipError_t e) { return e == hipSuccess ? MCPM_OK : MCPM_ERR_HIP; }
+
+#define HIPCHK(expr)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (expr);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      fprintf(stderr, "mcpm: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(e_), \
+              __FILE__, __LINE__);                                                 \
+      return MCPM_ERR_HIP;                                                         \
+    }                                                                              \
+  } while (0)
+
+#define LAUNCHCHK()                                                                \
+  do {                                                                             \
+    hipError_t e_ = hipGetLastError();                                             \
+    if (e_ != hipSuccess) {                                                        \
+      fprintf(stderr, "mcpm: launch failed: %s (%s:%d)\n", hipGetErrorString(e_), \
+              __FILE__, __LINE__);                                                 \
+      return MCPM_ERR_HIP;                                                         \
+    }                                                                              \
+  } while (0)
+
+template <typename T>
+int dalloc(T **ptr, size_t count) {
+  *ptr = nullptr;
+  if (count == 0) count = 1;
+  HIPCHK(hipMalloc((void **)ptr, count * sizeof(T)));
+  return MCPM_OK;
+}
+
+hipEvent_t ev_get(mcpm_ctx *c) {
+  if (!c->ev_pool.empty()) {
+    hipEvent_t e = c->ev_pool.back();
+    c->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct Timer {
+  mcpm_ctx *c;
+  int cls;
+  hipEvent_t a = nullptr;
+  Timer(mcpm_ctx *c_, int cls_) : c(c_), cls(cls_) {
+    if (c->timing) {
+      a = ev_get(c);
+      if (a) hipEventRecord(a, c->s);
+    }
+  }
+  ~Timer() {
+    if (c->timing && a) {
+      hipEvent_t b = ev_get(c);
+      if (b) {
+        hipEventRecord(b, c->s);
+        c->ev[cls].push_back({a, b});
+      }
+    }
+  }
+};
+
+// Gauss-Legendre nodes and weights on [-1,1] by Newton iteration on P_n,
+// rounded to 8 decimals: the reference tabulates its 50-point rule with 8
+// decimals (Main_Functions.cpp:3328-3345) and parity needs the same rule.
+void gauss_legendre_8dec(int n, double *x, double *w) {
+  for (int i = 0; i < n; ++i) {
+    double z = cos(PI * (i + 0.75) / (n + 0.5));
+    double pp = 0.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int k = 1; k <= n; ++k) {
+        double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * k - 1.0) * z * p2 - (k - 1.0) * p3) / k;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      double z1 = z;
+      z = z1 - p1 / pp;
+      if (fabs(z - z1) < 1e-16) break;
+    }
+    // ascending order: node i counts from -1
+    x[n - 1 - i] = z;
+    w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+  }
+  for (int i = 0; i < n; ++i) {
+    x[i] = nearbyint(x[i] * 1e8) / 1e8;
+    w[i] = nearbyint(w[i] * 1e8) / 1e8;
+  }
+}
+
+// Factor the constant tridiagonal matrix of the 1D field solve.
+int build_field(mcpm_ctx *c) {
+  const int nc = c->nc;
+  const bool periodic = c->p.boundary == MCPM_BC_PERIODIC;
+  const int nf = periodic ? nc : nc + 1;
+  const double dy = c->kp.dy;
+  const double diag = 2.0 + pow(dy * c->p.bst, 2.0);  // Main_Functions.cpp:3375-3377
+  std::vector<double> a(nf, diag), lo(nf, -1.0), up(nf, -1.0);  // lo[k]=A[k+1][k], up[k]=A[k][k+1]
+  double vlast = 0.0, den = 1.0;
+  std::vector<double> z;
+  if (!periodic) {
+    a[0] = 1.0;
+    up[0] = 0.0;  // c[0] = 0 (:3448-3449)
+    a[nf - 1] = 1.0;
+    lo[nf - 2] = 0.0;  // b[nf-2] = 0 (:3452-3454)
+  } else {
+    // cyclic: corners A[0][n-1] = beta = -1, A[n-1][0] = alpha = -1
+    double gamma = -a[0], alpha = -1.0, beta = -1.0;
+    a[0] -= gamma;
+    a[nf - 1] -= alpha * beta / gamma;
+    vlast = beta / gamma;
+    (void)alpha;
+  }
+  std::vector<double> mL(nf, 0.0), d(nf), invd(nf), mU(nf, 0.0);
+  d[0] = a[0];
+  for (int k = 1; k < nf; ++k) {
+    double l = lo[k - 1] / d[k - 1];
+    d[k] = a[k] - up[k - 1] * l;
+    mL[k] = -l;
+  }
+  for (int k = 0; k < nf; ++k) {
+    invd[k] = 1.0 / d[k];
+    mU[k] = (k < nf - 1) ? -up[k] / d[k] : 0.0;
+  }
+  if (periodic) {
+    double gamma = -(2.0 + pow(dy * c->p.bst, 2.0));
+    double alpha = -1.0;
+    std::vector<double> u(nf, 0.0), yv(nf), zz(nf);
+    u[0] = gamma;
+    u[nf - 1] = alpha;
+    yv[0] = u[0];
+    for (int k = 1; k < nf; ++k) yv[k] = u[k] + mL[k] * yv[k - 1];
+    zz[nf - 1] = yv[nf - 1] * invd[nf - 1];
+    for (int k = nf - 2; k >= 0; --k) zz[k] = yv[k] * invd[k] + mU[k] * zz[k + 1];
+    z = zz;
+    den = 1.0 + zz[0] + vlast * zz[nf - 1];
+  }
+  const size_t nfa = (size_t)nf + 1;
+  int st = dalloc(&c->field_buf, nfa * 9 + 200);
+  if (st) return st;
+  double *b = c->field_buf;
+  FieldArrays &fa = c->fa;
+  fa.nj = b; b += nfa;
+  fa.rhs = b; b += nfa;
+  fa.tB = b; b += nfa;
+  fa.y = b; b += nfa;
+  fa.x = b; b += nfa;
+  fa.mL = b; b += nfa;
+  fa.invd = b; b += nfa;
+  fa.mU = b; b += nfa;
+  fa.sm_z = b; b += nfa;
+  fa.gl_r = b; b += 50;
+  fa.gl_dr = b; b += 50;
+  fa.gl_phi = b; b += 50;
+  fa.gl_dphi = b; b += 50;
+  fa.sm_vlast = vlast;
+  fa.sm_den = den;
+  HIPCHK(hipMemcpy(fa.mL, mL.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa.invd, invd.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa.mU, mU.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  if (periodic) HIPCHK(hipMemcpy(fa.sm_z, z.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  double xi[50], wi[50], r[50], dr[50], ph[50], dph[50];
+  gauss_legendre_8dec(50, xi, wi);
+  const double bb = 3.0 * c->p.eqdist, aa = 0.0, cc = 0.0, dd = PI;
+  for (int i = 0; i < 50; ++i) {
+    r[i] = (bb - aa) / 2.0 * xi[i] + (aa + bb) / 2.0;
+    dr[i] = wi[i] * (bb - aa) / 2.0;
+    ph[i] = (dd - cc) / 2.0 * xi[i] + (cc + dd) / 2.0;
+    dph[i] = wi[i] * (dd - cc) / 2.0;
+  }
+  HIPCHK(hipMemcpy(fa.gl_r, r, sizeof r, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa.gl_dr, dr, sizeof dr, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa.gl_phi, ph, sizeof ph, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa.gl_dphi, dph, sizeof dph, hipMemcpyHostToDevice));
+  return MCPM_OK;
+}
+
+int check_err(mcpm_ctx *c) {
+  uint32_t e = 0;
+  HIPCHK(hipMemcpyAsync(&e, c->err, sizeof e, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (e == 0) return MCPM_OK;
+  HIPCHK(hipMemsetAsync(c->err, 0, sizeof(uint32_t), c->s));
+  if (e & ERRBIT_DOMAIN) return MCPM_ERR_DOMAIN;
+  if (e & ERRBIT_NAN) return MCPM_ERR_NAN;
+  if (e & ERRBIT_BC) return MCPM_ERR_BC;
+  return MCPM_ERR_CAPACITY;
+}
+
+int chunk_for(int64_t n) {
+  // about 2048 chunks (one wavefront each), between 1024 and 32768 particles
+  int64_t ch = (n + 2047) / 2048;
+  ch = (ch + 255) / 256 * 256;
+  if (ch < 1024) ch = 1024;
+  if (ch > 32768) ch = 32768;
+  return (int)ch;
+}
+
+template <typename real>
+int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
+  if (c->n == 0) {
+    HIPCHK(hipMemsetAsync(c->cnt, 0, c->nc * sizeof(uint32_t), c->s));
+    HIPCHK(hipMemsetAsync(c->start, 0, (c->nc + 1) * sizeof(uint32_t), c->s));
+    c->sorted = true;
+    return MCPM_OK;
+  }
+  Timer tm(c, T_SORT);
+  const int nc = c->nc;
+  const int chunk = chunk_for(c->n);
+  const int64_t nch = (c->n + chunk - 1) / chunk;
+  if (nch > c->nchunks_cap) return MCPM_ERR_CAPACITY;
+  const int nseg = (int)((nch + c->seg - 1) / c->seg);
+  const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  const Rec<real> *in = (const Rec<real> *)c->P[c->cur];
+  Rec<real> *out = (Rec<real> *)c->P[1 - c->cur];
+  const size_t lds = (size_t)nc * sizeof(uint32_t);
+  hipLaunchKernelGGL((k_push<real, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
+                     nullptr, c->n, chunk, c->kp, c->F2, mode, slo, shi, c->table, nullptr,
+                     c->err);
+  LAUNCHCHK();
+  dim3 gs((nc + 255) / 256, nseg);
+  hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
+                     c->seg);
+  hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, c->s, c->partial, nseg, nc, c->cnt,
+                     c->start, c->nden, c->yc, c->kp);
+  hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
+                     c->seg);
+  hipLaunchKernelGGL((k_push<real, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
+                     c->n, chunk, c->kp, c->F2, mode, slo, shi, c->table, c->start, c->err);
+  LAUNCHCHK();
+  c->cur = 1 - c->cur;
+  c->sorted = true;
+  return MCPM_OK;
+}
+
+int run_sort(mcpm_ctx *c, int mode, int64_t step) {
+  return c->p.precision == MCPM_FP32 ? run_sort_t<float>(c, mode, step)
+                                     : run_sort_t<double>(c, mode, step);
+}
+
+int ensure_ops(mcpm_ctx *c, int64_t m) {
+  if (m <= c->op_cap) return MCPM_OK;
+  int64_t cap = m + m / 4 + 1024;
+  HIPCHK(hipStreamSynchronize(c->s));
+  hipFree(c->ops.i);
+  hipFree(c->list[0]);
+  hipFree(c->log_acc);
+  c->ops = OpArrays{};
+  c->list[0] = c->list[1] = nullptr;
+  c->log_acc = nullptr;
+  uint32_t *blk = nullptr;
+  int st = dalloc(&blk, (size_t)cap * 7);
+  if (st) {
+    c->op_cap = 0;
+    return st;
+  }
+  c->ops.i = blk;
+  c->ops.j = blk + cap;
+  c->ops.p = blk + 2 * cap;
+  c->ops.c = blk + 3 * cap;
+  c->ops.t = blk + 4 * cap;
+  c->list[0] = blk + 5 * cap;
+  c->list[1] = blk + 6 * cap;
+  st = dalloc(&c->log_acc, (size_t)cap);
+  if (st) return st;
+  c->op_cap = cap;
+  return MCPM_OK;
+}
+
+template <typename real>
+int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  const int nc = c->nc;
+  const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  Rec<real> *P = (Rec<real> *)c->P[c->cur];
+  hipLaunchKernelGGL(k_mcand, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->cnt, c->omax,
+                     c->mcand, nc, c->kp, slo, shi);
+  hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->mcand, c->cstart, nc);
+  hipLaunchKernelGGL(k_zero_u32, dim3((CNT_TOTAL + 255) / 256), dim3(256), 0, c->s,
+                     c->counters, (int)CNT_TOTAL);
+  LAUNCHCHK();
+  uint32_t M = 0;
+  HIPCHK(hipMemcpyAsync(&M, c->cstart + nc, sizeof M, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  int st = ensure_ops(c, (int64_t)M);
+  if (st) return st;
+  mcpm_u32x4 pk = mcpm_philox(0u, 0u, slo, mcpm_ctr3(MCPM_TAG_PRIO, shi, 0u), c->kp.key0,
+                              c->kp.key1);
+  {
+    Timer tm(c, T_GEN);
+    hipLaunchKernelGGL((k_gen<real>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
+                       c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->ops, c->res[0],
+                       c->counters);
+    LAUNCHCHK();
+  }
+  int rounds = 0;
+  {
+    Timer tm(c, T_COLLIDE);
+    unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(1, ((int64_t)M + 255) / 256), 4096);
+    uint8_t *lg = c->debug_log ? c->log_acc : nullptr;
+    int next_check = 12;
+    for (int r = 0; r < MAX_ROUNDS; ++r) {
+      hipLaunchKernelGGL((k_round<real>), dim3(grid), dim3(256), 0, c->s, r, P, c->ops,
+                         r == 0 ? (const uint32_t *)nullptr : (const uint32_t *)c->list[r & 1],
+                         c->list[(r + 1) & 1], c->res[r & 1], c->res[(r + 1) & 1], c->counters,
+                         c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, lg);
+      LAUNCHCHK();
+      rounds = r + 1;
+      if (rounds == next_check) {
+        uint32_t left = 0;
+        HIPCHK(hipMemcpyAsync(&left, c->counters + CNT_LIST + rounds, sizeof left,
+                              hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        if (left == 0) break;
+        next_check += 4;
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_omega_finish, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->omax,
+                     c->omega_new, nc);
+  LAUNCHCHK();
+  uint32_t hc[CNT_LIST + 2];
+  HIPCHK(hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  uint32_t left = 0;
+  HIPCHK(hipMemcpy(&left, c->counters + CNT_LIST + rounds, sizeof left, hipMemcpyDeviceToHost));
+  if (left != 0) return MCPM_ERR_CAPACITY;  // more rounds than MAX_ROUNDS
+  c->last_ops = hc[CNT_OPS];
+  if (stats) {
+    stats->candidates = M;
+    stats->valid = hc[CNT_OPS];
+    uint64_t acc;
+    memcpy(&acc, &hc[CNT_ACC], sizeof acc);
+    stats->accepted = (int64_t)acc;
+    stats->rounds = rounds;
+    stats->reserved0 = 0;
+  }
+  return MCPM_OK;
+}
+
+double contact_y_host(const mcpm_params *p, double n) { return contact_y(n, p->b, p->y_model); }
+
+}  // namespace
+
+extern "C" {
+
+const char *mcpm_status_string(int s) {
+  switch (s) {
+    case MCPM_OK: return "ok";
+    case MCPM_ERR_ARG: return "invalid argument";
+    case MCPM_ERR_HIP: return "HIP runtime error";
+    case MCPM_ERR_CAPACITY: return "capacity exceeded";
+    case MCPM_ERR_DOMAIN: return "particle outside the domain";
+    case MCPM_ERR_NAN: return "non-finite velocity";
+    case MCPM_ERR_BC: return "too many boundary interactions in one step";
+    case MCPM_ERR_STATE: return "call out of order";
+    default: return "unknown status";
+  }
+}
+
+int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
+  if (!p || !out) return MCPM_ERR_ARG;
+  *out = nullptr;
+  if (p->n_cells < 1 || p->n_cells > 16384) return MCPM_ERR_ARG;
+  if (p->capacity < 1 || p->capacity > 2000000000LL) return MCPM_ERR_ARG;
+  if (p->precision != MCPM_FP32 && p->precision != MCPM_FP64) return MCPM_ERR_ARG;
+  if (p->boundary < 0 || p->boundary > 2) return MCPM_ERR_ARG;
+  if (!(p->length > 0.0) || !(p->delta_t > 0.0) || !(p->cell_volume > 0.0)) return MCPM_ERR_ARG;
+  if (p->long_range == MCPM_LR_SCREENED_POISSON && !(p->bst > 0.0)) return MCPM_ERR_ARG;
+  if (p->long_range != MCPM_LR_NONE && p->long_range != MCPM_LR_SCREENED_POISSON)
+    return MCPM_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCPM_ERR_HIP;
+  mcpm_ctx *c = new mcpm_ctx();
+  c->p = *p;
+  c->nc = p->n_cells;
+  HIPCHK(hipGetDevice(&c->device));
+  HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
+  KParams &k = c->kp;
+  k.n_cells = p->n_cells;
+  k.boundary = p->boundary;
+  k.y_model = p->y_model;
+  k.reserved = 0;
+  k.dy = p->length / (double)p->n_cells;
+  k.length = p->length;
+  k.wall_lo = p->wall_lo;
+  k.wall_hi = p->wall_hi;
+  k.T_lo = p->T_wall_lo;
+  k.T_hi = p->T_wall_hi;
+  k.U_lo = p->U_wall_lo;
+  k.U_hi = p->U_wall_hi;
+  k.sigma = p->sigma;
+  k.mass = p->mass;
+  k.kb = p->kb;
+  k.b = p->b;
+  k.Fn = p->Fn;
+  k.volume = p->cell_volume;
+  k.n_ratio = p->n_ratio;
+  k.dt = p->delta_t;
+  k.coll_pref = 4.0 * PI * p->sigma * p->sigma;
+  k.key0 = (uint32_t)p->seed;
+  k.key1 = (uint32_t)(p->seed >> 32);
+  c->rec = 4 * (p->precision == MCPM_FP32 ? sizeof(float) : sizeof(double));
+  const int nc = c->nc;
+  int st = MCPM_OK;
+  auto fail = [&](int s2) {
+    mcpm_destroy(c);
+    return s2;
+  };
+  if ((st = dalloc((char **)&c->P[0], c->rec * (size_t)p->capacity))) return fail(st);
+  if ((st = dalloc((char **)&c->P[1], c->rec * (size_t)p->capacity))) return fail(st);
+  c->nchunks_cap = (p->capacity + 1023) / 1024;
+  if (c->nchunks_cap > 2100) c->nchunks_cap = 2100;
+  // chunk_for keeps the number of chunks at or below about 2048
+  if ((st = dalloc(&c->table, (size_t)c->nchunks_cap * nc))) return fail(st);
+  if ((st = dalloc(&c->partial, (size_t)((c->nchunks_cap + c->seg - 1) / c->seg + 1) * nc)))
+    return fail(st);
+  if ((st = dalloc(&c->cnt, nc))) return fail(st);
+  if ((st = dalloc(&c->start, nc + 1))) return fail(st);
+  if ((st = dalloc(&c->mcand, nc))) return fail(st);
+  if ((st = dalloc(&c->cstart, nc + 1))) return fail(st);
+  if ((st = dalloc(&c->nden, nc))) return fail(st);
+  if ((st = dalloc(&c->yc, nc))) return fail(st);
+  if ((st = dalloc(&c->omax, nc))) return fail(st);
+  if ((st = dalloc(&c->omega_new, nc))) return fail(st);
+  if ((st = dalloc(&c->F2, nc))) return fail(st);
+  if ((st = dalloc(&c->moments, (size_t)nc * MCPM_NMOM))) return fail(st);
+  if ((st = dalloc(&c->samples, (size_t)nc * MCPM_NSAMP))) return fail(st);
+  if ((st = dalloc(&c->counters, CNT_TOTAL))) return fail(st);
+  if ((st = dalloc(&c->err, 1))) return fail(st);
+  if ((st = dalloc(&c->res[0], (size_t)p->capacity))) return fail(st);
+  if ((st = dalloc(&c->res[1], (size_t)p->capacity))) return fail(st);
+  if (hipMemset(c->omax, 0, nc * sizeof(double)) != hipSuccess ||
+      hipMemset(c->omega_new, 0, nc * sizeof(double)) != hipSuccess ||
+      hipMemset(c->F2, 0, nc * sizeof(double)) != hipSuccess ||
+      hipMemset(c->samples, 0, (size_t)nc * MCPM_NSAMP * sizeof(double)) != hipSuccess ||
+      hipMemset(c->err, 0, sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(c->cnt, 0, nc * sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(c->start, 0, (nc + 1) * sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(c->res[0], 0xFF, (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(c->res[1], 0xFF, (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess)
+    return fail(MCPM_ERR_HIP);
+  if (p->long_range == MCPM_LR_SCREENED_POISSON)
+    if ((st = build_field(c))) return fail(st);
+  if (hipDeviceSynchronize() != hipSuccess) return fail(MCPM_ERR_HIP);
+  *out = c;
+  return MCPM_OK;
+}
+
+int mcpm_destroy(mcpm_ctx *c) {
+  if (!c) return MCPM_OK;
+  if (c->s) hipStreamSynchronize(c->s);
+  void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
+                  c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
+                  c->samples, c->counters, c->err, c->res[0], c->res[1], c->ops.i,
+                  c->log_acc, c->field_buf};
+  for (void *b : bufs)
+    if (b) hipFree(b);
+  for (int k = 0; k < T_NCLASS; ++k)
+    for (auto &pr : c->ev[k]) {
+      hipEventDestroy(pr.first);
+      hipEventDestroy(pr.second);
+    }
+  for (auto e : c->ev_pool) hipEventDestroy(e);
+  if (c->s) hipStreamDestroy(c->s);
+  delete c;
+  return MCPM_OK;
+}
+
+int mcpm_sync(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  return check_err(c);
+}
+
+int mcpm_upload(mcpm_ctx *c, int64_t n, const double *y, const double *u1, const double *u2,
+                const double *u3) {
+  if (!c || n < 0 || n > c->p.capacity) return n > 0 && c ? MCPM_ERR_CAPACITY : MCPM_ERR_ARG;
+  if (n > 0 && (!y || !u1 || !u2 || !u3)) return MCPM_ERR_ARG;
+  std::vector<char> host(c->rec * (size_t)n);
+  if (c->p.precision == MCPM_FP32) {
+    float *h = (float *)host.data();
+    for (int64_t k = 0; k < n; ++k) {
+      h[4 * k] = (float)y[k];
+      h[4 * k + 1] = (float)u1[k];
+      h[4 * k + 2] = (float)u2[k];
+      h[4 * k + 3] = (float)u3[k];
+    }
+  } else {
+    double *h = (double *)host.data();
+    for (int64_t k = 0; k < n; ++k) {
+      h[4 * k] = y[k];
+      h[4 * k + 1] = u1[k];
+      h[4 * k + 2] = u2[k];
+      h[4 * k + 3] = u3[k];
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (n > 0)
+    HIPCHK(hipMemcpy(c->P[c->cur], host.data(), host.size(), hipMemcpyHostToDevice));
+  c->n = n;
+  c->sorted = false;
+  c->kick_pending = false;
+  return MCPM_OK;
+}
+
+int mcpm_download(mcpm_ctx *c, double *y, double *u1, double *u2, double *u3) {
+  if (!c) return MCPM_ERR_ARG;
+  const int64_t n = c->n;
+  std::vector<char> host(c->rec * (size_t)(n > 0 ? n : 1));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (n > 0)
+    HIPCHK(hipMemcpy(host.data(), c->P[c->cur], c->rec * (size_t)n, hipMemcpyDeviceToHost));
+  for (int64_t k = 0; k < n; ++k) {
+    double v[4];
+    if (c->p.precision == MCPM_FP32) {
+      const float *h = (const float *)host.data() + 4 * k;
+      for (int q = 0; q < 4; ++q) v[q] = h[q];
+    } else {
+      const double *h = (const double *)host.data() + 4 * k;
+      for (int q = 0; q < 4; ++q) v[q] = h[q];
+    }
+    if (y) y[k] = v[0];
+    if (u1) u1[k] = v[1];
+    if (u2) u2[k] = v[2];
+    if (u3) u3[k] = v[3];
+  }
+  return MCPM_OK;
+}
+
+int64_t mcpm_particle_count(const mcpm_ctx *c) { return c ? c->n : -1; }
+
+int mcpm_cell_update(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  int st = run_sort(c, PUSH_CHECK, 0);
+  if (st) return st;
+  return check_err(c);
+}
+
+int mcpm_collide_esmc(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->n == 0) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return MCPM_OK;
+  }
+  return c->p.precision == MCPM_FP32 ? collide_t<float>(c, step, stats)
+                                     : collide_t<double>(c, step, stats);
+}
+
+int mcpm_poisson_solver(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->p.long_range != MCPM_LR_SCREENED_POISSON) return MCPM_ERR_ARG;
+  Timer tm(c, T_FIELD);
+  hipLaunchKernelGGL(k_field, dim3(1), dim3(1024), 0, c->s, c->nden, c->fa, c->F2, c->kp,
+                     c->p.ast, c->p.bst, c->p.eqdist);
+  LAUNCHCHK();
+  c->kick_pending = true;
+  return MCPM_OK;
+}
+
+int mcpm_kick(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (!c->kick_pending) return MCPM_OK;
+  int st = run_sort(c, PUSH_KICK, 0);
+  c->kick_pending = false;
+  return st;
+}
+
+int mcpm_stream_bc(mcpm_ctx *c, int64_t step, int32_t with_kick) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  int mode = PUSH_MOVE;
+  if (with_kick && c->kick_pending) mode |= PUSH_KICK;
+  int st = run_sort(c, mode, step);
+  c->kick_pending = false;
+  return st;
+}
+
+int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  Timer tm(c, T_STEP);
+  int st = mcpm_collide_esmc(c, step, stats);
+  if (st) return st;
+  if (c->p.long_range == MCPM_LR_SCREENED_POISSON) {
+    st = mcpm_poisson_solver(c);
+    if (st) return st;
+  }
+  return mcpm_stream_bc(c, step, 1);
+}
+
+int mcpm_moments(mcpm_ctx *c, double *out) {
+  if (!c || !out) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->p.precision == MCPM_FP32)
+    hipLaunchKernelGGL((k_moments<float>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
+  else
+    hipLaunchKernelGGL((k_moments<double>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<double> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
+  LAUNCHCHK();
+  HIPCHK(hipMemcpyAsync(out, c->moments, (size_t)c->nc * MCPM_NMOM * sizeof(double),
+                        hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return MCPM_OK;
+}
+
+int mcpm_sample(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->p.precision == MCPM_FP32)
+    hipLaunchKernelGGL((k_sample<float>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->samples);
+  else
+    hipLaunchKernelGGL((k_sample<double>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<double> *)c->P[c->cur], c->cnt, c->start, c->samples);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
+int mcpm_get_samples(mcpm_ctx *c, double *out) {
+  if (!c || !out) return MCPM_ERR_ARG;
+  HIPCHK(hipMemcpyAsync(out, c->samples, (size_t)c->nc * MCPM_NSAMP * sizeof(double),
+                        hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return MCPM_OK;
+}
+
+int mcpm_reset_samples(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  HIPCHK(hipMemsetAsync(c->samples, 0, (size_t)c->nc * MCPM_NSAMP * sizeof(double), c->s));
+  return MCPM_OK;
+}
+
+int mcpm_init_omega_max(mcpm_ctx *c, double n_gas, int32_t times_y) {
+  if (!c) return MCPM_ERR_ARG;
+  std::vector<double> mom((size_t)c->nc * MCPM_NMOM), nd(c->nc), om(c->nc);
+  int st = mcpm_moments(c, mom.data());
+  if (st) return st;
+  st = mcpm_get_cells(c, nullptr, nullptr, nd.data(), nullptr, nullptr);
+  if (st) return st;
+  const double y = times_y ? contact_y_host(&c->p, n_gas) : 1.0;
+  for (int k = 0; k < c->nc; ++k) {
+    double T = mom[(size_t)k * MCPM_NMOM + 29];
+    double crm = 10.0 * sqrt(c->p.kb * T / c->p.mass);               // main.cpp:259
+    double o = 4.0 * PI * c->p.sigma * c->p.sigma * crm * nd[k] * c->p.delta_t;  // :260
+    om[k] = o * y;                                                       // :268-269
+  }
+  return mcpm_set_omega_max(c, om.data());
+}
+
+int mcpm_set_omega_max(mcpm_ctx *c, const double *om) {
+  if (!c || !om) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(c->omax, om, c->nc * sizeof(double), hipMemcpyHostToDevice));
+  return MCPM_OK;
+}
+
+int mcpm_get_cells(mcpm_ctx *c, int32_t *count, int32_t *start, double *n, double *omega_max,
+                   double *F2) {
+  if (!c) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  const size_t nc = c->nc;
+  if (count) HIPCHK(hipMemcpy(count, c->cnt, nc * 4, hipMemcpyDeviceToHost));
+  if (start) HIPCHK(hipMemcpy(start, c->start, nc * 4, hipMemcpyDeviceToHost));
+  if (n) HIPCHK(hipMemcpy(n, c->nden, nc * 8, hipMemcpyDeviceToHost));
+  if (omega_max) HIPCHK(hipMemcpy(omega_max, c->omax, nc * 8, hipMemcpyDeviceToHost));
+  if (F2) HIPCHK(hipMemcpy(F2, c->F2, nc * 8, hipMemcpyDeviceToHost));
+  return MCPM_OK;
+}
+
+double mcpm_contact_y(const mcpm_params *p, double n) { return p ? contact_y_host(p, n) : 0.0; }
+
+int mcpm_debug_collision_log(mcpm_ctx *c, int32_t enable) {
+  if (!c) return MCPM_ERR_ARG;
+  c->debug_log = enable != 0;
+  return MCPM_OK;
+}
+
+int mcpm_get_collision_log(mcpm_ctx *c, int64_t cap, uint32_t *prio, uint32_t *i, uint32_t *j,
+                           uint8_t *accepted, int64_t *n_out) {
+  if (!c || !n_out) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  int64_t n = c->last_ops;
+  *n_out = n;
+  if (n > cap) return MCPM_ERR_CAPACITY;
+  if (n == 0) return MCPM_OK;
+  if (prio) HIPCHK(hipMemcpy(prio, c->ops.p, n * 4, hipMemcpyDeviceToHost));
+  if (i) HIPCHK(hipMemcpy(i, c->ops.i, n * 4, hipMemcpyDeviceToHost));
+  if (j) HIPCHK(hipMemcpy(j, c->ops.j, n * 4, hipMemcpyDeviceToHost));
+  if (accepted) HIPCHK(hipMemcpy(accepted, c->log_acc, n, hipMemcpyDeviceToHost));
+  return MCPM_OK;
+}
+
+int mcpm_timing_enable(mcpm_ctx *c, int32_t enable) {
+  if (!c) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int k = 0; k < T_NCLASS; ++k) {
+    for (auto &pr : c->ev[k]) {
+      c->ev_pool.push_back(pr.first);
+      c->ev_pool.push_back(pr.second);
+    }
+    c->ev[k].clear();
+  }
+  c->timing = enable != 0;
+  return MCPM_OK;
+}
+
+int mcpm_timing_get(mcpm_ctx *c, int32_t which, double *avg_ms, int64_t *calls) {
+  if (!c || which < 0 || which >= T_NCLASS) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  double tot = 0.0;
+  for (auto &pr : c->ev[which]) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    tot += ms;
+  }
+  int64_t k = (int64_t)c->ev[which].size();
+  if (avg_ms) *avg_ms = k ? tot / (double)k : 0.0;
+  if (calls) *calls = k;
+  return MCPM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// mcpm_device.h — device-side building blocks shared by the slab kernels:
+// particle record layout, cell binning, the contact value Y(n), launch
+// parameter block and wavefront helpers.  gfx950 only (64-lane wavefronts).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "mcpm.h"
+#include "mcpm_rng.h"
+
+#pragma clang fp contract(off)
+
+namespace mcpm {
+
+// One particle: slab coordinate y (the reference's x2) and the three
+// velocity components U1, U2, U3.  Packed so that a random gather of one
+// particle (the collision partner) touches one 16-byte (FP32) or 32-byte
+// (FP64) span instead of four separate arrays.
+template <typename real>
+struct alignas(4 * sizeof(real)) Rec {
+  real y, u1, u2, u3;
+};
+
+// Constants every slab kernel reads, passed by value.
+struct KParams {
+  int32_t n_cells;
+  int32_t boundary;
+  int32_t y_model;
+  int32_t reserved;
+  double dy, length, wall_lo, wall_hi;
+  double T_lo, T_hi, U_lo, U_hi;
+  double sigma, mass, kb, b, Fn, volume, n_ratio, dt;
+  double coll_pref;  // 4 pi sigma^2, multiplied left to right as in collisions.cpp:691
+  uint32_t key0, key1;  // Philox key from the seed
+};
+
+__host__ __device__ __forceinline__ double contact_y(double n, double b, int model) {
+  // increase_collision_rate, Main_Functions.cpp:1906-1915
+  double nb = n * b;
+  if (model == MCPM_Y_CARNAHAN_STARLING) {
+    double eta = nb / 4.0;
+    double om = 1.0 - eta;
+    return (1.0 - eta / 2.0) / (om * om * om);
+  }
+  return 1.0 + 0.625 * n * b + 0.2869 * (nb * nb) + 0.115 * (nb * nb * nb);
+}
+
+// Cell of slab coordinate y: floor(y / dy) (Main_Functions.cpp:2236), held
+// inside [0, n_cells).  Identical arithmetic in the oracle.
+__host__ __device__ __forceinline__ int32_t cell_of(double y, double dy, int32_t nc) {
+  double q = mcpm_floor(y / dy);
+  if (!(q >= 0.0)) return 0;  // also catches NaN
+  if (q >= (double)nc) return nc - 1;
+  return (int32_t)q;
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+__device__ __forceinline__ uint64_t lanemask_lt() {
+  return (1ull << lane_id()) - 1ull;
+}
+
+// Append to a global list with one atomic per wavefront.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
+  uint64_t m = __ballot(pred);
+  if (m == 0) return 0;
+  int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader);
+  return base + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+__device__ __forceinline__ void atomic_max_pos_double(double *addr, double v) {
+  // For non-negative doubles the IEEE bit pattern orders like the value.
+  if (v > 0.0)
+    atomicMax((unsigned long long *)addr, (unsigned long long)mcpm_dbits(v));
+}
+
+}  // namespace mcpm

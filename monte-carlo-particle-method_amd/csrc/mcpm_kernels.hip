@@ -1,0 +1,786 @@
+// mcpm_kernels.hip — HIP kernels of the 1D slab step for gfx950.
+//
+//   k_push<real, SCATTER>   kick + stream + boundary + cell key; per-chunk
+//                           LDS histogram (SCATTER=false) or stable scatter
+//                           into the other buffer (SCATTER=true)
+//   k_scan_*                per-cell column scans of the chunk histograms
+//   k_mcand, k_gen, k_round ESMC collisions by deterministic reservations
+//   k_field                 1D screened-Poisson solve in one workgroup
+//   k_moments, k_sample     per-cell FP64 moments / time-average sums
+//
+// Every kernel is written for 64-lane wavefronts.
+#include "mcpm_kernels.h"
+
+#pragma clang fp contract(off)
+
+namespace mcpm {
+
+// ---------------------------------------------------------------------------
+// kick + stream + boundary of one particle
+// ---------------------------------------------------------------------------
+// Position_FP_ideal_gas (Fokker_Planck.cpp:948-969) for the slab axis, the
+// kick U2 += F2 dt of poisson_solver (Main_Functions.cpp:3574-3577), and the
+// slab-axis part of BC_thermal_wall_3D (Main_Functions.cpp:4917-5248):
+// a particle past a wall is put on the wall, re-emitted with a half-range
+// Maxwellian at the wall temperature, and flies the time it spent past the
+// wall with the new velocity.  Draws: Philox(slot, bounce, step, WALL).
+template <typename real>
+__device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const KParams &kp,
+                                              const double *__restrict__ F2, int mode,
+                                              uint32_t step_lo, uint32_t step_hi,
+                                              uint32_t *err) {
+  double y = (double)r.y;
+  if (mode & PUSH_KICK) {
+    int32_t c = cell_of(y, kp.dy, kp.n_cells);
+    r.u2 = (real)((double)r.u2 + F2[c] * kp.dt);
+  }
+  if (!(mode & PUSH_MOVE)) return r;
+  double v = (double)r.u2;
+  y = y + v * kp.dt;
+  if (kp.boundary == MCPM_BC_PERIODIC) {
+    if (y < 0.0) y += kp.length;
+    else if (y >= kp.length) y -= kp.length;
+    real yr = (real)y;
+    if (!(yr < (real)kp.length)) yr = (real)0;
+    if (!(yr >= (real)0)) yr = (real)0;
+    r.y = yr;
+    return r;
+  }
+  if (kp.boundary == MCPM_BC_SPECULAR) {
+    // BC_specular_reflection (Main_Functions.cpp:6399-6525), slab axis.
+    int guard = 0;
+    while ((y < kp.wall_lo || y > kp.wall_hi) && guard < 20) {
+      if (y > kp.wall_hi) y = 2.0 * kp.wall_hi - y;
+      else y = 2.0 * kp.wall_lo - y;
+      v = -v;
+      ++guard;
+    }
+    if (guard >= 20) atomicOr(err, ERRBIT_BC);
+    r.y = (real)y;
+    r.u2 = (real)v;
+    return r;
+  }
+  // diffuse walls
+  double dt_left = kp.dt;
+  double v_prev = v;
+  uint32_t bounce = 0;
+  while (y < kp.wall_lo || y > kp.wall_hi) {
+    if (bounce >= 20) {  // "TOO MANY BC OPERATIONS", Main_Functions.cpp:4958-4965
+      atomicOr(err, ERRBIT_BC);
+      y = y < kp.wall_lo ? kp.wall_lo : kp.wall_hi;
+      break;
+    }
+    bool upper = y > kp.wall_hi;
+    double yw = upper ? kp.wall_hi : kp.wall_lo;
+    double T = upper ? kp.T_hi : kp.T_lo;
+    double Uw = upper ? kp.U_hi : kp.U_lo;
+    mcpm_u32x4 w = mcpm_philox((uint32_t)slot, bounce, step_lo,
+                               mcpm_ctr3(MCPM_TAG_WALL, step_hi, (uint32_t)(slot >> 32)),
+                               kp.key0, kp.key1);
+    double z0, z1;
+    mcpm_normal2(w.v[0], w.v[1], &z0, &z1);
+    double sd = __builtin_sqrt(kp.kb * T / kp.mass);
+    double vn = __builtin_sqrt(2.0 * kp.kb * T / kp.mass) *
+                __builtin_sqrt(-mcpm_log(mcpm_u01(w.v[2])));
+    double u1 = sd * z0 + Uw;
+    double u3 = sd * z1;
+    double u2 = upper ? -vn : vn;
+    double dtr = __builtin_fabs((y - yw) / v_prev);
+    if (dtr > dt_left) dtr = dt_left;
+    if (dt_left > 1e-15) y = yw + dtr * u2;
+    else y = yw;
+    dt_left = dt_left - dtr;
+    v_prev = u2;
+    r.u1 = (real)u1;
+    r.u2 = (real)u2;
+    r.u3 = (real)u3;
+    ++bounce;
+  }
+  r.y = (real)y;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Stable counting sort, pass 1 and pass 3.  One wavefront per chunk of
+// consecutive particles; the chunk's cell histogram lives in LDS.
+//   SCATTER=false: count particles per new cell -> table[chunk][cell]
+//   SCATTER=true : running[cell] = start[cell] + table[chunk][cell] (exclusive
+//                  over chunks) and each particle lands at running + rank, the
+//                  rank taken among equal keys in the wavefront with ballots.
+// Particles of a chunk are processed in index order, so equal keys keep their
+// relative order: the sort is stable.
+// ---------------------------------------------------------------------------
+template <typename real, bool SCATTER>
+__global__ __launch_bounds__(64) void k_push(const Rec<real> *__restrict__ in,
+                                             Rec<real> *__restrict__ out, int64_t n,
+                                             int32_t chunk, KParams kp,
+                                             const double *__restrict__ F2, int mode,
+                                             uint32_t step_lo, uint32_t step_hi,
+                                             uint32_t *__restrict__ table,
+                                             const uint32_t *__restrict__ start,
+                                             uint32_t *err) {
+  extern __shared__ uint32_t lds_cnt[];
+  const int nc = kp.n_cells;
+  const int lane = threadIdx.x;
+  const int64_t b = blockIdx.x;
+  const int64_t beg = b * (int64_t)chunk;
+  int64_t end = beg + chunk;
+  if (end > n) end = n;
+  uint32_t *trow = table + (size_t)b * (size_t)nc;
+  if (SCATTER) {
+    for (int c = lane; c < nc; c += 64) lds_cnt[c] = start[c] + trow[c];
+  } else {
+    for (int c = lane; c < nc; c += 64) lds_cnt[c] = 0u;
+  }
+  __syncthreads();
+  const uint64_t lt = lanemask_lt();
+  constexpr int U = 4;  // tiles loaded ahead
+  for (int64_t t0 = beg; t0 < end; t0 += 64 * U) {
+    Rec<real> rr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t p = t0 + u * 64 + lane;
+      if (p < end) rr[u] = in[p];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      int64_t p = t0 + u * 64 + lane;
+      bool valid = p < end;
+      int32_t key = 0;
+      Rec<real> r = rr[u];
+      if (valid) {
+        if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
+        if (mode & PUSH_CHECK) {
+          double yy = (double)r.y;
+          if (!(yy >= 0.0) || !(yy <= kp.length)) atomicOr(err, ERRBIT_DOMAIN);
+        }
+        r = push_one<real>(r, (uint64_t)p, kp, F2, mode, step_lo, step_hi, err);
+        key = cell_of((double)r.y, kp.dy, nc);
+      }
+      uint64_t active = __ballot(valid);
+      while (active) {
+        int leader = __ffsll((unsigned long long)active) - 1;
+        int lkey = __shfl(key, leader);
+        bool mine = valid && key == lkey;
+        uint64_t m = __ballot(mine);
+        if (SCATTER) {
+          if (mine) {
+            uint32_t dst = lds_cnt[lkey] + (uint32_t)__popcll(m & lt);
+            out[dst] = r;
+          }
+          __builtin_amdgcn_wave_barrier();
+          if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
+          __builtin_amdgcn_wave_barrier();
+        } else {
+          if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
+        }
+        active &= ~m;
+      }
+    }
+  }
+  if (!SCATTER) {
+    __syncthreads();
+    for (int c = lane; c < nc; c += 64) trow[c] = lds_cnt[c];
+  }
+}
+
+// Column scans of table[chunk][cell] in segments of SEG chunks.
+__global__ void k_scan_a(const uint32_t *__restrict__ table, uint32_t *__restrict__ partial,
+                         int nchunks, int nc, int seg) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = blockIdx.y;
+  if (c >= nc) return;
+  int b0 = s * seg, b1 = min(nchunks, b0 + seg);
+  uint32_t sum = 0;
+  for (int b = b0; b < b1; ++b) sum += table[(size_t)b * nc + c];
+  partial[(size_t)s * nc + c] = sum;
+}
+
+// One workgroup: per-cell exclusive scan over segments, cell counts, the
+// exclusive scan over cells (start), densities and contact values.
+__global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ partial, int nseg, int nc,
+                                                 uint32_t *__restrict__ cnt,
+                                                 uint32_t *__restrict__ start,
+                                                 double *__restrict__ nden,
+                                                 double *__restrict__ yc, KParams kp) {
+  __shared__ uint32_t sh[1024];
+  const int tid = threadIdx.x;
+  const int per = (nc + 1023) / 1024;
+  const int c0 = tid * per;
+  uint32_t tot = 0;
+  for (int k = 0; k < per; ++k) {
+    int c = c0 + k;
+    if (c >= nc) break;
+    uint32_t run = 0;
+    for (int s = 0; s < nseg; ++s) {
+      uint32_t v = partial[(size_t)s * nc + c];
+      partial[(size_t)s * nc + c] = run;
+      run += v;
+    }
+    cnt[c] = run;
+    tot += run;
+    double nd = kp.n_ratio * kp.Fn * (double)run / kp.volume;  // Main_Functions.cpp:2601
+    nden[c] = nd;
+    yc[c] = contact_y(nd, kp.b, kp.y_model);
+  }
+  sh[tid] = tot;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = (tid >= off) ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = (tid == 0) ? 0u : sh[tid - 1];
+  for (int k = 0; k < per; ++k) {
+    int c = c0 + k;
+    if (c >= nc) break;
+    start[c] = run;
+    run += cnt[c];
+  }
+  if (tid == 1023) start[nc] = sh[1023];
+}
+
+__global__ void k_scan_c(uint32_t *__restrict__ table, const uint32_t *__restrict__ partial,
+                         int nchunks, int nc, int seg) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int s = blockIdx.y;
+  if (c >= nc) return;
+  int b0 = s * seg, b1 = min(nchunks, b0 + seg);
+  uint32_t run = partial[(size_t)s * nc + c];
+  for (int b = b0; b < b1; ++b) {
+    uint32_t v = table[(size_t)b * nc + c];
+    table[(size_t)b * nc + c] = run;
+    run += v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// ESMC collisions (Collision_ESMC_new3, collisions.cpp:583-720)
+// ---------------------------------------------------------------------------
+// Candidate count per cell, collisions.cpp:617-629:
+//   m_c = floor(0.5 * omega_max_c * N_c + U)
+__global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restrict__ omax,
+                        uint32_t *__restrict__ mcand, int nc, KParams kp, uint32_t step_lo,
+                        uint32_t step_hi) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  mcpm_u32x4 w = mcpm_philox((uint32_t)c, 0u, step_lo, mcpm_ctr3(MCPM_TAG_MCAND, step_hi, 0u),
+                             kp.key0, kp.key1);
+  double m = mcpm_floor(0.5 * omax[c] * (1.0 * (double)cnt[c]) + mcpm_u01(w.v[0]));
+  mcand[c] = m > 0.0 ? (uint32_t)m : 0u;
+}
+
+// Exclusive scan of a per-cell u32 array (one workgroup); total in out[nc].
+__global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__ in,
+                                                    uint32_t *__restrict__ out, int nc) {
+  __shared__ uint32_t sh[1024];
+  const int tid = threadIdx.x;
+  const int per = (nc + 1023) / 1024;
+  const int c0 = tid * per;
+  uint32_t tot = 0;
+  for (int k = 0; k < per; ++k) {
+    int c = c0 + k;
+    if (c < nc) tot += in[c];
+  }
+  sh[tid] = tot;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    uint32_t v = (tid >= off) ? sh[tid - off] : 0u;
+    __syncthreads();
+    sh[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = (tid == 0) ? 0u : sh[tid - 1];
+  for (int k = 0; k < per; ++k) {
+    int c = c0 + k;
+    if (c < nc) {
+      out[c] = run;
+      run += in[c];
+    }
+  }
+  if (tid == 1023) out[nc] = sh[1023];
+}
+
+// The candidate draw: i in cell c, isotropic direction, the partner cell J
+// at y_i + sigma dir_y (collisions.cpp:648-671; 1D rule of new4,
+// collisions.cpp:784-792), j in J, j != i (collisions.cpp:675-679).
+template <typename real>
+__device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const Rec<real> *P,
+                                               const uint32_t *cnt, const uint32_t *start,
+                                               const KParams &kp, uint32_t step_lo,
+                                               uint32_t step_hi, uint32_t *oi, uint32_t *oj) {
+  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
+                             kp.key1);
+  uint32_t Nc = cnt[c];
+  uint32_t i = start[c] + mcpm_uint_below(w.v[0], Nc);
+  double d0, d1, d2;
+  mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
+  double yJ = (double)P[i].y + kp.sigma * d1;
+  int32_t J;
+  if (kp.boundary == MCPM_BC_PERIODIC) {
+    if (yJ < 0.0) yJ += kp.length;
+    else if (yJ >= kp.length) yJ -= kp.length;
+    J = cell_of(yJ, kp.dy, kp.n_cells);
+  } else {
+    double q = mcpm_floor(yJ / kp.dy);
+    if (!(q >= 0.0) || q >= (double)kp.n_cells) return false;
+    J = (int32_t)q;
+  }
+  uint32_t NJ = cnt[J];
+  if (NJ == 0) return false;
+  uint32_t j = start[J] + mcpm_uint_below(w.v[3], NJ);
+  if (j == i) return false;
+  *oi = i;
+  *oj = j;
+  return true;
+}
+
+// One workgroup per cell: draw the cell's m_c candidates, keep the valid
+// ones as operations, reserve both particles for the first round.
+template <typename real>
+__global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
+                                             const uint32_t *__restrict__ cnt,
+                                             const uint32_t *__restrict__ start,
+                                             const uint32_t *__restrict__ mcand,
+                                             const uint32_t *__restrict__ cstart, KParams kp,
+                                             uint32_t step_lo, uint32_t step_hi,
+                                             uint32_t prio_key, OpArrays ops,
+                                             uint32_t *__restrict__ res0,
+                                             uint32_t *__restrict__ counters) {
+  const uint32_t c = blockIdx.x;
+  const uint32_t m = mcand[c];
+  if (m == 0 || cnt[c] == 0) return;
+  const uint32_t k0 = cstart[c];
+  for (uint32_t t0 = 0; t0 < m; t0 += blockDim.x) {
+    uint32_t t = t0 + threadIdx.x;
+    uint32_t i = 0, j = 0;
+    bool ok = false;
+    if (t < m) ok = draw_candidate<real>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+    uint32_t pos = wave_append(&counters[CNT_OPS], ok);
+    if (ok) {
+      uint32_t pr = mcpm_priority(k0 + t, prio_key);
+      ops.i[pos] = i;
+      ops.j[pos] = j;
+      ops.p[pos] = pr;
+      ops.c[pos] = c;
+      ops.t[pos] = t;
+      atomicMin(&res0[i], pr);
+      atomicMin(&res0[j], pr);
+    }
+  }
+}
+
+// The collision itself, collisions.cpp:680-716: Y at the midpoint cell,
+// p = 4 pi sigma^2 (g.dir) Y n_J dt, omega_I = max(omega_I, p),
+// accept iff p / omega_max_I > U, hard-sphere exchange along dir.
+template <typename real>
+__device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t c, uint32_t t,
+                                          Rec<real> *P, const double *nden, const double *yc,
+                                          const double *omax, double *omega_new,
+                                          const KParams &kp, uint32_t step_lo,
+                                          uint32_t step_hi) {
+  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
+                             kp.key1);
+  mcpm_u32x4 w2 = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 1u), kp.key0,
+                              kp.key1);
+  double d0, d1, d2;
+  mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
+  Rec<real> a = P[i];
+  Rec<real> bb = P[j];
+  double yi = (double)a.y, yj = (double)bb.y;
+  double ymid;
+  if (kp.boundary == MCPM_BC_PERIODIC) {
+    double d = yj - yi;
+    if (d > 0.5 * kp.length) d -= kp.length;
+    else if (d < -0.5 * kp.length) d += kp.length;
+    ymid = yi + d / 2.0;
+    if (ymid < 0.0) ymid += kp.length;
+    else if (ymid >= kp.length) ymid -= kp.length;
+  } else {
+    ymid = (yi + yj) / 2.0;
+  }
+  int32_t cm = cell_of(ymid, kp.dy, kp.n_cells);
+  int32_t J = cell_of(yj, kp.dy, kp.n_cells);
+  double Ymid = yc[cm];
+  double g0 = (double)a.u1 - (double)bb.u1;
+  double g1 = (double)a.u2 - (double)bb.u2;
+  double g2 = (double)a.u3 - (double)bb.u3;
+  double gd = g0 * d0 + g1 * d1 + g2 * d2;
+  double pv = kp.coll_pref * gd * Ymid * nden[J] * kp.dt;
+  atomic_max_pos_double(&omega_new[c], pv);
+  if (pv / omax[c] > mcpm_u01(w2.v[0])) {
+    a.u1 = (real)((double)a.u1 - gd * d0);
+    a.u2 = (real)((double)a.u2 - gd * d1);
+    a.u3 = (real)((double)a.u3 - gd * d2);
+    bb.u1 = (real)((double)bb.u1 + gd * d0);
+    bb.u2 = (real)((double)bb.u2 + gd * d1);
+    bb.u3 = (real)((double)bb.u3 + gd * d2);
+    P[i] = a;
+    P[j] = bb;
+    return true;
+  }
+  return false;
+}
+
+// One round of deterministic reservations.  An operation commits when it
+// holds the smallest priority on both of its particles, which is exactly
+// when every earlier-priority operation sharing a particle has committed:
+// the result equals processing all operations one by one in priority order.
+// Losers reserve in the other array for the next round.  Each operation that
+// finds its own priority in the current array clears it, so the array is
+// back to all-ones when the round ends.
+template <typename real>
+__global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P, OpArrays ops,
+                                               const uint32_t *__restrict__ list_in,
+                                               uint32_t *__restrict__ list_out,
+                                               uint32_t *__restrict__ res_cur,
+                                               uint32_t *__restrict__ res_next,
+                                               uint32_t *__restrict__ counters,
+                                               const double *__restrict__ nden,
+                                               const double *__restrict__ yc,
+                                               const double *__restrict__ omax,
+                                               double *__restrict__ omega_new, KParams kp,
+                                               uint32_t step_lo, uint32_t step_hi,
+                                               uint8_t *__restrict__ log_acc) {
+  const uint32_t count = (r == 0) ? counters[CNT_OPS] : counters[CNT_LIST + r];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  uint32_t acc_local = 0;
+  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+    uint32_t idx = base + threadIdx.x;
+    bool live = idx < count;
+    uint32_t o = 0, i = 0, j = 0, pr = 0;
+    bool lose = false;
+    if (live) {
+      o = (r == 0) ? idx : list_in[idx];
+      i = ops.i[o];
+      j = ops.j[o];
+      pr = ops.p[o];
+      uint32_t ri = res_cur[i], rj = res_cur[j];
+      if (ri == pr && rj == pr) {
+        bool acc = commit_op<real>(i, j, ops.c[o], ops.t[o], P, nden, yc, omax, omega_new, kp,
+                                   step_lo, step_hi);
+        acc_local += acc ? 1u : 0u;
+        if (log_acc) log_acc[o] = acc ? 1 : 0;
+        res_cur[i] = 0xFFFFFFFFu;
+        res_cur[j] = 0xFFFFFFFFu;
+      } else {
+        if (ri == pr) res_cur[i] = 0xFFFFFFFFu;
+        if (rj == pr) res_cur[j] = 0xFFFFFFFFu;
+        atomicMin(&res_next[i], pr);
+        atomicMin(&res_next[j], pr);
+        lose = true;
+      }
+    }
+    uint32_t pos = wave_append(&counters[CNT_LIST + r + 1], lose);
+    if (lose) list_out[pos] = o;
+  }
+  // accepted count, one atomic per wavefront
+  for (int off = 32; off > 0; off >>= 1) acc_local += __shfl_down(acc_local, off);
+  if (lane_id() == 0 && acc_local) atomicAdd((unsigned long long *)&counters[CNT_ACC], (unsigned long long)acc_local);
+}
+
+__global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ omega_new, int nc) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  double o = omega_new[c];
+  if (o > omax[c]) omax[c] = o;  // collisions.cpp:714-716
+  omega_new[c] = 0.0;
+}
+
+__global__ void k_zero_u32(uint32_t *p, int n) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0u;
+}
+__global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
+// ---------------------------------------------------------------------------
+// 1D screened-Poisson mean field (poisson_solver, Main_Functions.cpp:3285-3581)
+// ---------------------------------------------------------------------------
+// x(y) solves (-d2/dy2 + bst^2) x = n on the faces, discretised as
+// (2 + (dy bst)^2) x_f - x_{f-1} - x_{f+1} = dy^2 n_f, with n_f the mean of
+// the two adjacent cells.  Walls: Dirichlet values from the truncated Yukawa
+// integral (50x50 Gauss-Legendre, the reference's 8-digit table).  Periodic:
+// cyclic system by Sherman-Morrison.  The constant tridiagonal matrix is
+// factored once on the host; each step runs the forward and backward
+// substitutions as block-wide scans of affine maps x -> A x + B.
+// F2_c = -(x_{c+1} - x_c) / dy * ast / m after subtracting the analytic core
+// part n_f (1 - (1 + bst sigma) e^{-bst sigma}) / bst^2 (:3539-3577).
+
+__device__ void block_affine_scan(const double *__restrict__ A, const double *__restrict__ B,
+                                  double *__restrict__ out, int n, bool reverse, double *sA,
+                                  double *sB) {
+  const int tid = threadIdx.x;
+  const int nt = blockDim.x;
+  const int E = (n + nt - 1) / nt;
+  const int s0 = tid * E;
+  const int s1 = min(n, s0 + E);
+  double a = 1.0, b = 0.0;
+  for (int s = s0; s < s1; ++s) {
+    int k = reverse ? (n - 1 - s) : s;
+    a = A[k] * a;
+    b = A[k] * b + B[k];
+  }
+  sA[tid] = a;
+  sB[tid] = b;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {
+    double pa = 1.0, pb = 0.0;
+    bool has = tid >= off;
+    if (has) {
+      pa = sA[tid - off];
+      pb = sB[tid - off];
+    }
+    __syncthreads();
+    if (has) {
+      double ca = sA[tid], cb = sB[tid];
+      sA[tid] = ca * pa;
+      sB[tid] = ca * pb + cb;
+    }
+    __syncthreads();
+  }
+  double v = (tid == 0) ? 0.0 : sB[tid - 1];
+  __syncthreads();
+  for (int s = s0; s < s1; ++s) {
+    int k = reverse ? (n - 1 - s) : s;
+    v = A[k] * v + B[k];
+    out[k] = v;
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_field(const double *__restrict__ nden,
+                                                FieldArrays fa, double *__restrict__ F2,
+                                                KParams kp, double ast, double bst,
+                                                double eqdist) {
+  __shared__ double sA[1024];
+  __shared__ double sB[1024];
+  __shared__ double red[2][1024];
+  const int tid = threadIdx.x;
+  const int nc = kp.n_cells;
+  const bool periodic = kp.boundary == MCPM_BC_PERIODIC;
+  const int nf = periodic ? nc : nc + 1;
+  const double dy = kp.dy;
+  // face densities (Main_Functions.cpp:3352-3372)
+  for (int f = tid; f < nf; f += blockDim.x) {
+    double nj;
+    if (periodic) nj = (nden[f] + nden[(f + nc - 1) % nc]) / 2.0;
+    else if (f == 0) nj = nden[0];
+    else if (f == nc) nj = nden[nc - 1];
+    else nj = (nden[f] + nden[f - 1]) / 2.0;
+    fa.nj[f] = nj;
+    fa.rhs[f] = nj * (dy * dy);
+  }
+  if (!periodic) {
+    // Dirichlet values: truncated Yukawa integrals at both walls
+    // (Main_Functions.cpp:3322-3442).
+    double s_lo = 0.0, s_hi = 0.0;
+    for (int q = tid; q < 2500; q += blockDim.x) {
+      int ir = q / 50, ip = q % 50;
+      double rr = fa.gl_r[ir], dr = fa.gl_dr[ir];
+      double ph = fa.gl_phi[ip], dph = fa.gl_dphi[ip];
+      double sp, cp;
+      mcpm_sincos2pi(ph / 6.283185307179586476925286766559, &sp, &cp);
+      double radial = mcpm_exp(-bst * rr) / 2.0 * rr * dr;
+      for (int side = 0; side < 2; ++side) {
+        double yf = (side == 0 ? 0.0 : (double)nc * dy) + rr * cp;
+        double qq = mcpm_floor(yf / dy);
+        double n;
+        if (qq < 0.0) n = nden[0];
+        else if (qq > (double)(nc - 1)) n = nden[nc - 1];
+        else n = nden[(int)qq];
+        double term = radial * n * sp * dph;
+        if (side == 0) s_lo += term; else s_hi += term;
+      }
+    }
+    red[0][tid] = s_lo;
+    red[1][tid] = s_hi;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if (tid < off) {
+        red[0][tid] += red[0][tid + off];
+        red[1][tid] += red[1][tid + off];
+      }
+      __syncthreads();
+    }
+    if (tid == 0) {
+      fa.rhs[0] = red[0][0];
+      fa.rhs[nf - 1] = red[1][0];
+    }
+    __syncthreads();
+  }
+  __syncthreads();
+  // forward: y_k = -l_k y_{k-1} + r_k
+  for (int f = tid; f < nf; f += blockDim.x) fa.tB[f] = fa.rhs[f];
+  __syncthreads();
+  block_affine_scan(fa.mL, fa.tB, fa.y, nf, false, sA, sB);
+  // backward: x_k = (y_k - c_k x_{k+1}) / d_k
+  for (int f = tid; f < nf; f += blockDim.x) fa.tB[f] = fa.y[f] * fa.invd[f];
+  __syncthreads();
+  block_affine_scan(fa.mU, fa.tB, fa.x, nf, true, sA, sB);
+  if (periodic) {
+    __shared__ double corr;
+    if (tid == 0) {
+      double vy = fa.x[0] + fa.sm_vlast * fa.x[nf - 1];
+      corr = vy / fa.sm_den;
+    }
+    __syncthreads();
+    for (int f = tid; f < nf; f += blockDim.x) fa.x[f] = fa.x[f] - corr * fa.sm_z[f];
+    __syncthreads();
+  }
+  // core subtraction and per-cell force
+  double core = 1.0 - (kp.sigma * bst + 1.0) * mcpm_exp(-bst * kp.sigma);
+  core = core / bst / bst;
+  for (int f = tid; f < nf; f += blockDim.x) fa.y[f] = fa.x[f] - fa.nj[f] * core;
+  __syncthreads();
+  for (int c = tid; c < nc; c += blockDim.x) {
+    int f1 = periodic ? (c + 1) % nc : c + 1;
+    double F = -(fa.y[f1] - fa.y[c]) / dy;
+    F2[c] = F / kp.mass * ast;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Per-cell moments (cell_update, Main_Functions.cpp:2461-2593) and the
+// time-average sums of cell_poly_velocity (Main_Functions.cpp:1918-1968).
+// One workgroup per cell; FP64 accumulation; fixed-order tree reductions.
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *sh /* [K][blockDim/64] */) {
+  const int lane = lane_id();
+  const int wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (lane == 0) sh[k * nw + wid] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double x = 0.0;
+    for (int w = 0; w < nw; ++w) x += sh[k * nw + w];
+    v[k] = x;
+  }
+  __syncthreads();
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P,
+                                                 const uint32_t *__restrict__ cnt,
+                                                 const uint32_t *__restrict__ start,
+                                                 double *__restrict__ out, KParams kp) {
+  __shared__ double sh[28 * 4];
+  const int c = blockIdx.x;
+  const uint32_t N = cnt[c];
+  const uint32_t s = start[c];
+  double m1[3] = {0.0, 0.0, 0.0};
+  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+    Rec<real> r = P[s + k];
+    m1[0] += (double)r.u1;
+    m1[1] += (double)r.u2;
+    m1[2] += (double)r.u3;
+  }
+  block_sum<3>(m1, sh);
+  double U0 = 0.0, U1 = 0.0, U2 = 0.0;
+  if (N > 0) {
+    U0 = m1[0] / (double)N;
+    U1 = m1[1] / (double)N;
+    U2 = m1[2] / (double)N;
+  }
+  double a[26];
+#pragma unroll
+  for (int q = 0; q < 26; ++q) a[q] = 0.0;
+  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+    Rec<real> r = P[s + k];
+    double p1 = (double)r.u1 - U0, p2 = (double)r.u2 - U1, p3 = (double)r.u3 - U2;
+    double e = p1 * p1 + p2 * p2 + p3 * p3;
+    a[0] += p1 * p1; a[1] += p1 * p2; a[2] += p1 * p3;
+    a[3] += p2 * p2; a[4] += p2 * p3; a[5] += p3 * p3;
+    a[6] += p1 * p1 * p1; a[7] += p1 * p1 * p2; a[8] += p1 * p1 * p3;
+    a[9] += p1 * p2 * p2; a[10] += p1 * p2 * p3; a[11] += p1 * p3 * p3;
+    a[12] += p2 * p2 * p2; a[13] += p2 * p2 * p3; a[14] += p2 * p3 * p3;
+    a[15] += p3 * p3 * p3;
+    a[16] += p1 * p1 * e; a[17] += p1 * p2 * e; a[18] += p1 * p3 * e;
+    a[19] += p2 * p2 * e; a[20] += p2 * p3 * e; a[21] += p3 * p3 * e;
+    a[22] += p1 * e * e; a[23] += p2 * e * e; a[24] += p3 * e * e;
+    a[25] += e;
+  }
+  block_sum<26>(a, sh);
+  if (threadIdx.x == 0) {
+    double *o = out + (size_t)c * MCPM_NMOM;
+    double w = (double)N;
+    o[0] = U0; o[1] = U1; o[2] = U2;
+    for (int q = 0; q < 26; ++q) o[3 + q] = (N > 0) ? a[q] / w : 0.0;
+    double DM2 = (N > 0) ? a[25] / w : 0.0;
+    o[29] = kp.mass * DM2 / (3.0 * kp.kb);
+    o[30] = w;
+  }
+}
+
+template <typename real>
+__global__ __launch_bounds__(256) void k_sample(const Rec<real> *__restrict__ P,
+                                                const uint32_t *__restrict__ cnt,
+                                                const uint32_t *__restrict__ start,
+                                                double *__restrict__ acc) {
+  __shared__ double sh[25 * 4];
+  const int c = blockIdx.x;
+  const uint32_t N = cnt[c];
+  const uint32_t s = start[c];
+  double a[25];
+#pragma unroll
+  for (int q = 0; q < 25; ++q) a[q] = 0.0;
+  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
+    Rec<real> r = P[s + k];
+    double u = (double)r.u1, v = (double)r.u2, w = (double)r.u3;
+    double e = u * u + v * v + w * w;
+    a[0] += u; a[1] += v; a[2] += w;
+    a[3] += u * u; a[4] += u * v; a[5] += u * w; a[6] += v * v; a[7] += v * w; a[8] += w * w;
+    // (m,n,l) for m=0..3, n=0..3-m, l=3-m-n (Main_Functions.cpp:1940-1948)
+    a[9] += w * w * w;  a[10] += v * w * w; a[11] += v * v * w; a[12] += v * v * v;
+    a[13] += u * w * w; a[14] += u * v * w; a[15] += u * v * v;
+    a[16] += u * u * w; a[17] += u * u * v; a[18] += u * u * u;
+    a[19] += u * u * e; a[20] += u * v * e; a[21] += u * w * e;
+    a[22] += v * v * e; a[23] += v * w * e; a[24] += w * w * e;
+  }
+  block_sum<25>(a, sh);
+  if (threadIdx.x == 0) {
+    double *o = acc + (size_t)c * MCPM_NSAMP;
+    o[0] += (double)N;
+    for (int q = 0; q < 25; ++q) o[1 + q] += a[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// explicit instantiations
+// ---------------------------------------------------------------------------
+#define MCPM_INST(real)                                                                    \
+  template __global__ void k_push<real, false>(const Rec<real> *, Rec<real> *, int64_t,    \
+                                               int32_t, KParams, const double *, int,      \
+                                               uint32_t, uint32_t, uint32_t *,             \
+                                               const uint32_t *, uint32_t *);              \
+  template __global__ void k_push<real, true>(const Rec<real> *, Rec<real> *, int64_t,     \
+                                              int32_t, KParams, const double *, int,       \
+                                              uint32_t, uint32_t, uint32_t *,              \
+                                              const uint32_t *, uint32_t *);               \
+  template __global__ void k_gen<real>(const Rec<real> *, const uint32_t *,                \
+                                       const uint32_t *, const uint32_t *,                 \
+                                       const uint32_t *, KParams, uint32_t, uint32_t,      \
+                                       uint32_t, OpArrays, uint32_t *, uint32_t *);        \
+  template __global__ void k_round<real>(int, Rec<real> *, OpArrays, const uint32_t *,     \
+                                         uint32_t *, uint32_t *, uint32_t *, uint32_t *,   \
+                                         const double *, const double *, const double *,   \
+                                         double *, KParams, uint32_t, uint32_t, uint8_t *); \
+  template __global__ void k_moments<real>(const Rec<real> *, const uint32_t *,            \
+                                           const uint32_t *, double *, KParams);           \
+  template __global__ void k_sample<real>(const Rec<real> *, const uint32_t *,             \
+                                          const uint32_t *, double *);
+
+MCPM_INST(float)
+MCPM_INST(double)
+
+}  // namespace mcpm

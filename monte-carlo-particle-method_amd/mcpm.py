@@ -1,0 +1,336 @@
+"""Python binding of libmcpm.so, the MI355X slab particle step.
+
+Mirrors the reference's per-step operator interface (src/cpp_headers.h):
+
+    ============================  ==========================================
+    reference (C++)               here
+    ============================  ==========================================
+    cell_update                   Simulation.cell_update()
+    Collision_ESMC_new3           Simulation.collide_esmc(step)
+    poisson_solver                Simulation.poisson_solver() (+ kick)
+    stream_particles + apply_BC   Simulation.stream_bc(step)
+    cell_poly_velocity            Simulation.sample()
+    increase_collision_rate       contact_y(params, n)
+    main.cpp:250-269 omega init   Simulation.init_omega_max(n_gas)
+    ============================  ==========================================
+
+The library is required: importing this module on a machine where
+libmcpm.so is missing or no GPU is visible raises, there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field, fields
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmcpm.so")
+
+FP64, FP32 = 0, 1
+BC_DIFFUSE, BC_PERIODIC, BC_SPECULAR = 0, 1, 2
+Y_REFERENCE, Y_CARNAHAN_STARLING = 0, 1
+LR_NONE, LR_SCREENED_POISSON = 0, 3
+NMOM, NSAMP = 31, 26
+
+STATUS = {
+    0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "capacity exceeded",
+    4: "particle outside the domain", 5: "non-finite velocity",
+    6: "too many boundary interactions in one step", 7: "call out of order",
+}
+
+
+class McpmError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: {STATUS.get(code, 'unknown')} (status {code})")
+        self.code = code
+
+
+class CParams(ctypes.Structure):
+    _fields_ = [
+        ("precision", ctypes.c_int32), ("boundary", ctypes.c_int32),
+        ("y_model", ctypes.c_int32), ("long_range", ctypes.c_int32),
+        ("n_cells", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("capacity", ctypes.c_int64), ("seed", ctypes.c_uint64),
+        ("length", ctypes.c_double), ("wall_lo", ctypes.c_double), ("wall_hi", ctypes.c_double),
+        ("T_wall_lo", ctypes.c_double), ("T_wall_hi", ctypes.c_double),
+        ("U_wall_lo", ctypes.c_double), ("U_wall_hi", ctypes.c_double),
+        ("sigma", ctypes.c_double), ("mass", ctypes.c_double), ("kb", ctypes.c_double),
+        ("b", ctypes.c_double), ("Fn", ctypes.c_double), ("cell_volume", ctypes.c_double),
+        ("n_ratio", ctypes.c_double), ("delta_t", ctypes.c_double),
+        ("ast", ctypes.c_double), ("bst", ctypes.c_double), ("eqdist", ctypes.c_double),
+    ]
+
+
+class CStats(ctypes.Structure):
+    _fields_ = [("candidates", ctypes.c_int64), ("valid", ctypes.c_int64),
+                ("accepted", ctypes.c_int64), ("rounds", ctypes.c_int32),
+                ("reserved0", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"candidates": self.candidates, "valid": self.valid,
+                "accepted": self.accepted, "rounds": self.rounds}
+
+
+@dataclass
+class Params:
+    """Slab parameters; names follow the reference's GAS/BOX fields."""
+    n_cells: int
+    length: float
+    capacity: int
+    precision: int = FP64
+    boundary: int = BC_DIFFUSE
+    y_model: int = Y_REFERENCE
+    long_range: int = LR_NONE
+    seed: int = 1
+    wall_lo: float = float("nan")
+    wall_hi: float = float("nan")
+    T_wall_lo: float = 1.0
+    T_wall_hi: float = 1.0
+    U_wall_lo: float = 0.0
+    U_wall_hi: float = 0.0
+    sigma: float = 1.0
+    mass: float = 1.0
+    kb: float = 1.0
+    b: float = float("nan")
+    Fn: float = 1.0
+    cell_volume: float = float("nan")
+    n_ratio: float = 1.0
+    delta_t: float = 0.01
+    ast: float = 0.0
+    bst: float = 1.0
+    eqdist: float = 1.0
+
+    def __post_init__(self):
+        if math.isnan(self.b):
+            self.b = 2.0 * math.pi * self.sigma ** 3 / 3.0          # myIO.cpp:2828
+        if math.isnan(self.wall_lo):
+            self.wall_lo = self.sigma + 1e-15                       # Main_Functions.cpp:5126
+        if math.isnan(self.wall_hi):
+            self.wall_hi = self.length - self.sigma - 1e-15         # Main_Functions.cpp:5088
+        if math.isnan(self.cell_volume):
+            self.cell_volume = self.length / self.n_cells
+
+    @property
+    def dy(self) -> float:
+        return self.length / self.n_cells
+
+    def to_c(self) -> CParams:
+        c = CParams()
+        for f in fields(self):
+            setattr(c, f.name, getattr(self, f.name))
+        c.reserved0 = 0
+        return c
+
+
+def _u8p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _u32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))
+
+
+def _i32p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+_lib = None
+
+
+def load_library() -> ctypes.CDLL:
+    """Load libmcpm.so; raise if it is missing (no silent fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"libmcpm.so not built: {LIB_PATH} (run __graft_entry__.build())")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, dpp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
+    u32pp = ctypes.POINTER(ctypes.c_uint32)
+    sig = {
+        "mcpm_create": (i32, [ctypes.POINTER(CParams), ctypes.POINTER(vp)]),
+        "mcpm_destroy": (i32, [vp]),
+        "mcpm_sync": (i32, [vp]),
+        "mcpm_upload": (i32, [vp, i64, dpp, dpp, dpp, dpp]),
+        "mcpm_download": (i32, [vp, dpp, dpp, dpp, dpp]),
+        "mcpm_particle_count": (i64, [vp]),
+        "mcpm_cell_update": (i32, [vp]),
+        "mcpm_collide_esmc": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_poisson_solver": (i32, [vp]),
+        "mcpm_kick": (i32, [vp]),
+        "mcpm_stream_bc": (i32, [vp, i64, i32]),
+        "mcpm_step": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_init_omega_max": (i32, [vp, ctypes.c_double, i32]),
+        "mcpm_set_omega_max": (i32, [vp, dpp]),
+        "mcpm_get_cells": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), dpp, dpp, dpp]),
+        "mcpm_moments": (i32, [vp, dpp]),
+        "mcpm_sample": (i32, [vp]),
+        "mcpm_get_samples": (i32, [vp, dpp]),
+        "mcpm_reset_samples": (i32, [vp]),
+        "mcpm_contact_y": (ctypes.c_double, [ctypes.POINTER(CParams), ctypes.c_double]),
+        "mcpm_debug_collision_log": (i32, [vp, i32]),
+        "mcpm_get_collision_log": (i32, [vp, i64, u32pp, u32pp, u32pp,
+                                         ctypes.POINTER(ctypes.c_uint8), ctypes.POINTER(i64)]),
+        "mcpm_timing_enable": (i32, [vp, i32]),
+        "mcpm_timing_get": (i32, [vp, i32, dpp, ctypes.POINTER(i64)]),
+        "mcpm_status_string": (ctypes.c_char_p, [i32]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def contact_y(p: Params, n: float) -> float:
+    """increase_collision_rate (Main_Functions.cpp:1906-1915)."""
+    lib = load_library()
+    cp = p.to_c()
+    return lib.mcpm_contact_y(ctypes.byref(cp), float(n))
+
+
+TIMER_COLLIDE, TIMER_SORT, TIMER_GEN, TIMER_FIELD, TIMER_STEP = 0, 1, 2, 3, 4
+
+
+class Simulation:
+    """A device-resident slab of particles driven step by step."""
+
+    def __init__(self, params: Params):
+        self.params = params
+        self._lib = load_library()
+        self._cp = params.to_c()
+        h = ctypes.c_void_p()
+        self._check(self._lib.mcpm_create(ctypes.byref(self._cp), ctypes.byref(h)), "mcpm_create")
+        self._h = h
+
+    # -- plumbing -----------------------------------------------------------
+    def _check(self, st: int, what: str):
+        if st != 0:
+            raise McpmError(st, what)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mcpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @property
+    def n_cells(self) -> int:
+        return self.params.n_cells
+
+    # -- data ---------------------------------------------------------------
+    def upload(self, y, u1, u2, u3):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (y, u1, u2, u3)]
+        n = arrs[0].shape[0]
+        self._check(self._lib.mcpm_upload(self._h, n, *[_dp(a) for a in arrs]), "mcpm_upload")
+
+    def download(self):
+        n = self._lib.mcpm_particle_count(self._h)
+        out = [np.empty(n, np.float64) for _ in range(4)]
+        self._check(self._lib.mcpm_download(self._h, *[_dp(a) for a in out]), "mcpm_download")
+        return tuple(out)
+
+    def cells(self):
+        nc = self.n_cells
+        cnt = np.empty(nc, np.int32)
+        st = np.empty(nc, np.int32)
+        n = np.empty(nc)
+        om = np.empty(nc)
+        f2 = np.empty(nc)
+        self._check(self._lib.mcpm_get_cells(self._h, _i32p(cnt), _i32p(st), _dp(n), _dp(om),
+                                             _dp(f2)), "mcpm_get_cells")
+        return {"count": cnt, "start": st, "n": n, "omega_max": om, "F2": f2}
+
+    # -- operators ----------------------------------------------------------
+    def cell_update(self):
+        self._check(self._lib.mcpm_cell_update(self._h), "mcpm_cell_update")
+
+    def collide_esmc(self, step: int) -> dict:
+        s = CStats()
+        self._check(self._lib.mcpm_collide_esmc(self._h, step, ctypes.byref(s)), "mcpm_collide_esmc")
+        return s.as_dict()
+
+    def poisson_solver(self):
+        self._check(self._lib.mcpm_poisson_solver(self._h), "mcpm_poisson_solver")
+
+    def kick(self):
+        self._check(self._lib.mcpm_kick(self._h), "mcpm_kick")
+
+    def stream_bc(self, step: int, with_kick: bool = True):
+        self._check(self._lib.mcpm_stream_bc(self._h, step, 1 if with_kick else 0), "mcpm_stream_bc")
+
+    def step(self, step: int) -> dict:
+        s = CStats()
+        self._check(self._lib.mcpm_step(self._h, step, ctypes.byref(s)), "mcpm_step")
+        return s.as_dict()
+
+    def sync(self):
+        self._check(self._lib.mcpm_sync(self._h), "mcpm_sync")
+
+    def init_omega_max(self, n_gas: float, times_y: bool = True):
+        self._check(self._lib.mcpm_init_omega_max(self._h, float(n_gas), 1 if times_y else 0),
+                    "mcpm_init_omega_max")
+
+    def set_omega_max(self, om):
+        om = np.ascontiguousarray(om, dtype=np.float64)
+        self._check(self._lib.mcpm_set_omega_max(self._h, _dp(om)), "mcpm_set_omega_max")
+
+    def moments(self) -> np.ndarray:
+        out = np.empty(self.n_cells * NMOM)
+        self._check(self._lib.mcpm_moments(self._h, _dp(out)), "mcpm_moments")
+        return out.reshape(self.n_cells, NMOM)
+
+    def sample(self):
+        self._check(self._lib.mcpm_sample(self._h), "mcpm_sample")
+
+    def samples(self) -> np.ndarray:
+        out = np.empty(self.n_cells * NSAMP)
+        self._check(self._lib.mcpm_get_samples(self._h, _dp(out)), "mcpm_get_samples")
+        return out.reshape(self.n_cells, NSAMP)
+
+    def reset_samples(self):
+        self._check(self._lib.mcpm_reset_samples(self._h), "mcpm_reset_samples")
+
+    # -- diagnostics --------------------------------------------------------
+    def debug_collision_log(self, enable: bool = True):
+        self._check(self._lib.mcpm_debug_collision_log(self._h, 1 if enable else 0), "debug_log")
+
+    def collision_log(self, cap: int):
+        pr = np.empty(cap, np.uint32)
+        i = np.empty(cap, np.uint32)
+        j = np.empty(cap, np.uint32)
+        a = np.empty(cap, np.uint8)
+        n = ctypes.c_int64()
+        self._check(self._lib.mcpm_get_collision_log(self._h, cap, _u32p(pr), _u32p(i), _u32p(j),
+                                                     _u8p(a), ctypes.byref(n)), "collision_log")
+        k = n.value
+        return pr[:k], i[:k], j[:k], a[:k]
+
+    def timing_enable(self, on: bool = True):
+        self._check(self._lib.mcpm_timing_enable(self._h, 1 if on else 0), "timing_enable")
+
+    def timing(self, which: int):
+        ms = ctypes.c_double()
+        calls = ctypes.c_int64()
+        self._check(self._lib.mcpm_timing_get(self._h, which, ctypes.byref(ms), ctypes.byref(calls)),
+                    "timing_get")
+        return ms.value, calls.value

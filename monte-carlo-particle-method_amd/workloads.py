@@ -1,0 +1,98 @@
+"""Synthetic workloads of the slab step: parameters plus initial particles.
+
+Three slab set-ups, sized by arguments so that the same code builds the full
+problem and the small versions the parity tests run:
+
+* heat_conduction: Enskog DSMC Fourier heat conduction between diffuse walls
+  (20 sigma slab, 200 cells, 100 particles per cell, n sigma^3 = 0.2, T0 = 1.5,
+  walls 1.0 / 2.0, dt = 0.1 mean collision time, FP64).
+* dense_slab: the same geometry refined (4000 cells, 2500 particles per cell,
+  10M particles) at n sigma^3 = 0.1 or 0.3, walls 1.0 / 1.5, dt = 0.05 mean
+  collision time, FP32 velocities.
+* interface: liquid-vapour planar interface in a periodic 400 sigma box with
+  4096 cells and 20M particles; n sigma^3 = 0.55 in the central third, 0.02
+  elsewhere, T = 0.7, attractive tail -eps (sigma/r)^6 through the
+  screened-Poisson mean field, dt = 0.02, FP32.
+
+Units are reduced: sigma = m = kB = eps = 1.  Initial positions are uniform
+in each density region and velocities Maxwellian (numpy Philox generator),
+the reference's `initialization` (Main_Functions.cpp:4657-4801) in spirit.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from mcpm import (BC_DIFFUSE, BC_PERIODIC, FP32, FP64, LR_NONE, LR_SCREENED_POISSON, Params,
+                  Y_REFERENCE)
+
+# Yukawa fit of the reference (myIO.cpp:2891-2892) in reduced units:
+# bst = 6.91304716e9 1/m * 3.405e-10 m, ast = -1.64835851e-28 J m / (eps sigma)
+# with eps = 119.8 kB, sigma = 3.405e-10 m.  It fits -4 eps (sigma/r)^6; the
+# interface problem's tail -eps (sigma/r)^6 scales ast by 1/4.
+YUKAWA_BST = 6.91304716e9 * 3.405e-10
+YUKAWA_AST_4EPS = -1.64835851e-28 / (119.8 * 1.380649e-23 * 3.405e-10)
+
+
+def contact_y_ref(n: float, sigma: float = 1.0) -> float:
+    b = 2.0 * math.pi * sigma ** 3 / 3.0
+    nb = n * b
+    return 1.0 + 0.625 * n * b + 0.2869 * nb * nb + 0.115 * nb * nb * nb
+
+
+def mean_collision_time(n: float, T: float, sigma: float = 1.0, m: float = 1.0) -> float:
+    """1 / nu with the Enskog frequency nu = 4 n sigma^2 Y(n) sqrt(pi kB T / m)."""
+    return 1.0 / (4.0 * n * sigma ** 2 * contact_y_ref(n, sigma) * math.sqrt(math.pi * T / m))
+
+
+def maxwellian(rng, n, T, m=1.0, dtype=np.float64):
+    s = math.sqrt(T / m)
+    return (rng.standard_normal(n) * s, rng.standard_normal(n) * s, rng.standard_normal(n) * s)
+
+
+def heat_conduction(n_cells=200, ppc=100, length=20.0, n_density=0.2, T0=1.5, T_lo=1.0,
+                    T_hi=2.0, dt_frac=0.1, precision=FP64, seed=1, walls_at_sigma=False):
+    N = n_cells * ppc
+    area = N / (n_density * length)
+    dy = length / n_cells
+    eps = 1e-15
+    lo, hi = (1.0 + eps, length - 1.0 - eps) if walls_at_sigma else (eps, length - eps)
+    p = Params(n_cells=n_cells, length=length, capacity=N, precision=precision,
+               boundary=BC_DIFFUSE, y_model=Y_REFERENCE, long_range=LR_NONE, seed=seed,
+               wall_lo=lo, wall_hi=hi, T_wall_lo=T_lo, T_wall_hi=T_hi,
+               cell_volume=dy * area, Fn=1.0,
+               delta_t=dt_frac * mean_collision_time(n_density, T0))
+    rng = np.random.Generator(np.random.Philox(seed))
+    y = lo + (hi - lo) * rng.random(N)
+    u1, u2, u3 = maxwellian(rng, N, T0)
+    return p, (y, u1, u2, u3), {"n_gas": n_density, "T0": T0, "N": N, "area": area}
+
+
+def dense_slab(n_density=0.3, n_cells=4000, ppc=2500, length=20.0, T0=1.25, T_lo=1.0,
+               T_hi=1.5, dt_frac=0.05, precision=FP32, seed=1):
+    p, arrs, meta = heat_conduction(n_cells=n_cells, ppc=ppc, length=length, n_density=n_density,
+                                    T0=T0, T_lo=T_lo, T_hi=T_hi, dt_frac=dt_frac,
+                                    precision=precision, seed=seed)
+    return p, arrs, meta
+
+
+def interface(N=20_000_000, n_cells=4096, length=400.0, n_liq=0.55, n_vap=0.02, T=0.7,
+              dt=0.02, precision=FP32, seed=1, capacity=None):
+    L3 = length / 3.0
+    area = N / (n_liq * L3 + n_vap * 2.0 * L3)
+    dy = length / n_cells
+    n_l = int(round(n_liq * L3 * area))
+    n_v = N - n_l
+    rng = np.random.Generator(np.random.Philox(seed))
+    yl = L3 + L3 * rng.random(n_l)
+    yv = rng.random(n_v) * (2.0 * L3)
+    yv = np.where(yv < L3, yv, yv + L3)
+    y = np.concatenate([yl, yv])
+    u1, u2, u3 = maxwellian(rng, N, T)
+    p = Params(n_cells=n_cells, length=length, capacity=capacity or N, precision=precision,
+               boundary=BC_PERIODIC, y_model=Y_REFERENCE, long_range=LR_SCREENED_POISSON,
+               seed=seed, cell_volume=dy * area, Fn=1.0, delta_t=dt,
+               ast=YUKAWA_AST_4EPS / 4.0, bst=YUKAWA_BST, eqdist=1.0)
+    n_gas = N / (length * area)
+    return p, (y, u1, u2, u3), {"n_gas": n_gas, "T0": T, "N": N, "area": area}

@@ -1,0 +1,706 @@
+/*
+ * mcpm_oracle.cpp — CPU oracle for the slab particle step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: the tests, the
+ * smoke check and the CPU-baseline leg of bench.py load liboracle.so and
+ * compare the GPU library against it.  Nothing in the product (libmcpm.so,
+ * its Python binding, the C++ host layer) links, loads or calls it.
+ *
+ * It restates, sequentially and in plain loops, the reference algorithms of
+ * mohsensadr/Monte-Carlo-Particle-Method for the 1D slab step:
+ *   cell_update binning + stable index lists   Main_Functions.cpp:2138-2376
+ *   cell_update moments                         Main_Functions.cpp:2461-2593
+ *   increase_collision_rate (Y)                 Main_Functions.cpp:1906-1915
+ *   Collision_ESMC_new3                         collisions.cpp:583-720
+ *     (1D partner-cell rule of new4,             collisions.cpp:784-792)
+ *   poisson_solver + Thompson_3d                Main_Functions.cpp:3285-3581,
+ *                                               Math_Func.cpp:28-52
+ *   Position_FP_ideal_gas (slab axis)           Fokker_Planck.cpp:948-969
+ *   BC_thermal_wall_3D (slab axis)              Main_Functions.cpp:4917-5248
+ *   BC_specular_reflection (slab axis)          Main_Functions.cpp:6399-6525
+ *   cell_poly_velocity                          Main_Functions.cpp:1918-1968
+ *   omega_max initialisation                    main.cpp:250-269
+ * with these deliberate, documented substitutions:
+ *   - random numbers: Philox4x32-10 addressed by (purpose, step, cell or
+ *     particle, draw) from mcpm_rng.h instead of random_device-seeded
+ *     mt19937 / rand();
+ *   - candidate order: new3 draws each candidate's cell with probability
+ *     proportional to the cell's remaining quota (collisions.cpp:631-646),
+ *     i.e. it processes a uniformly random interleaving of all cells'
+ *     candidates.  Here that interleaving is a keyed bijection of the
+ *     candidate index (mcpm_priority), sorted ascending, so it is a pure
+ *     function of (seed, step);
+ *   - particle order: the per-cell index lists become a physical stable
+ *     counting sort, so indices_inside[c][u] is start[c] + u;
+ *   - periodic boundaries (cyclic tridiagonal by Sherman-Morrison, wrapped
+ *     partner cell and minimum-image midpoint) for the periodic slab, which
+ *     the reference's 1D path does not offer.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "mcpm.h"
+#include "mcpm_rng.h"
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+double orc_y(double n, const mcpm_params &p) {
+  // increase_collision_rate, Main_Functions.cpp:1906-1915
+  double nb = n * p.b;
+  if (p.y_model == MCPM_Y_CARNAHAN_STARLING) {
+    double eta = nb / 4.0;
+    double om = 1.0 - eta;
+    return (1.0 - eta / 2.0) / (om * om * om);
+  }
+  return 1.0 + 0.625 * n * p.b + 0.2869 * (nb * nb) + 0.115 * (nb * nb * nb);
+}
+
+int orc_cell(double y, double dy, int nc) {
+  double q = mcpm_floor(y / dy);
+  if (!(q >= 0.0)) return 0;
+  if (q >= (double)nc) return nc - 1;
+  return (int)q;
+}
+
+struct Oracle {
+  mcpm_params p;
+  int nc = 0;
+  double dy = 0.0;
+  int64_t n = 0;
+  std::vector<double> rec;  // 4 per particle, stored through `real` rounding
+  std::vector<int> cnt, start;
+  std::vector<double> nden, yc, omax, F2, samples;
+  bool kick_pending = false;
+  bool sorted = false;
+  uint32_t err = 0;
+  // collision log of the last call, in processing order
+  std::vector<uint32_t> log_p, log_i, log_j;
+  std::vector<uint8_t> log_a;
+  int64_t last_cand = 0, last_acc = 0;
+};
+
+template <typename real>
+double rnd(double v) {
+  return (double)(real)v;
+}
+
+// Stable counting sort of the particles by cell (cell_update binning and
+// index lists, Main_Functions.cpp:2231-2376).
+void orc_sort(Oracle &o, bool check) {
+  const int nc = o.nc;
+  std::vector<int> key(o.n);
+  std::fill(o.cnt.begin(), o.cnt.end(), 0);
+  for (int64_t k = 0; k < o.n; ++k) {
+    double y = o.rec[4 * k];
+    if (check && (!(y >= 0.0) || !(y <= o.p.length))) o.err |= 1u;
+    key[k] = orc_cell(y, o.dy, nc);
+    o.cnt[key[k]]++;
+  }
+  int run = 0;
+  for (int c = 0; c < nc; ++c) {
+    o.start[c] = run;
+    run += o.cnt[c];
+  }
+  o.start[nc] = run;
+  std::vector<int> fill(o.start.begin(), o.start.end() - 1);
+  std::vector<double> out(o.rec.size());
+  for (int64_t k = 0; k < o.n; ++k) {
+    int d = fill[key[k]]++;
+    memcpy(&out[4 * (size_t)d], &o.rec[4 * (size_t)k], 4 * sizeof(double));
+  }
+  o.rec.swap(out);
+  for (int c = 0; c < nc; ++c) {
+    // n = n_ratio Fn weight / volume (Main_Functions.cpp:2601)
+    o.nden[c] = o.p.n_ratio * o.p.Fn * (double)o.cnt[c] / o.p.cell_volume;
+    o.yc[c] = orc_y(o.nden[c], o.p);
+  }
+  o.sorted = true;
+}
+
+// Kick, stream and boundary of one particle (see header list).
+template <typename real>
+void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, uint32_t shi) {
+  const mcpm_params &p = o.p;
+  double *r = &o.rec[4 * (size_t)slot];
+  double y = r[0];
+  if (kick) {
+    int c = orc_cell(y, o.dy, o.nc);
+    r[2] = rnd<real>(r[2] + o.F2[c] * p.delta_t);  // U2 += F2 dt (Main_Functions.cpp:3576)
+  }
+  if (!move) return;
+  double v = r[2];
+  y = y + v * p.delta_t;  // x2 += dt U2 (Fokker_Planck.cpp:963)
+  if (p.boundary == MCPM_BC_PERIODIC) {
+    if (y < 0.0) y += p.length;
+    else if (y >= p.length) y -= p.length;
+    real yr = (real)y;
+    if (!(yr < (real)p.length)) yr = (real)0;
+    if (!(yr >= (real)0)) yr = (real)0;
+    r[0] = (double)yr;
+    return;
+  }
+  if (p.boundary == MCPM_BC_SPECULAR) {
+    int guard = 0;
+    while ((y < p.wall_lo || y > p.wall_hi) && guard < 20) {
+      if (y > p.wall_hi) y = 2.0 * p.wall_hi - y;
+      else y = 2.0 * p.wall_lo - y;
+      v = -v;
+      ++guard;
+    }
+    if (guard >= 20) o.err |= 4u;
+    r[0] = rnd<real>(y);
+    r[2] = rnd<real>(v);
+    return;
+  }
+  // diffuse walls (BC_thermal_wall_3D, upper/lower wall branches :5044-5127)
+  double dt_left = p.delta_t;
+  double v_prev = v;
+  uint32_t bounce = 0;
+  while (y < p.wall_lo || y > p.wall_hi) {
+    if (bounce >= 20) {
+      o.err |= 4u;
+      y = y < p.wall_lo ? p.wall_lo : p.wall_hi;
+      break;
+    }
+    bool upper = y > p.wall_hi;
+    double yw = upper ? p.wall_hi : p.wall_lo;
+    double T = upper ? p.T_wall_hi : p.T_wall_lo;
+    double Uw = upper ? p.U_wall_hi : p.U_wall_lo;
+    uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
+    mcpm_u32x4 w = mcpm_philox((uint32_t)slot, bounce, slo,
+                               mcpm_ctr3(MCPM_TAG_WALL, shi, (uint32_t)((uint64_t)slot >> 32)),
+                               key0, key1);
+    double z0, z1;
+    mcpm_normal2(w.v[0], w.v[1], &z0, &z1);
+    double sd = sqrt(p.kb * T / p.mass);
+    double vn = sqrt(2.0 * p.kb * T / p.mass) * sqrt(-mcpm_log(mcpm_u01(w.v[2])));
+    double u1 = sd * z0 + Uw;
+    double u3 = sd * z1;
+    double u2 = upper ? -vn : vn;
+    double dtr = fabs((y - yw) / v_prev);
+    if (dtr > dt_left) dtr = dt_left;
+    if (dt_left > 1e-15) y = yw + dtr * u2;
+    else y = yw;
+    dt_left = dt_left - dtr;
+    v_prev = u2;
+    r[1] = rnd<real>(u1);
+    r[2] = rnd<real>(u2);
+    r[3] = rnd<real>(u3);
+    ++bounce;
+  }
+  r[0] = rnd<real>(y);
+}
+
+template <typename real>
+void orc_push(Oracle &o, bool kick, bool move, int64_t step) {
+  uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  for (int64_t k = 0; k < o.n; ++k) {
+    double *r = &o.rec[4 * (size_t)k];
+    if (r[1] != r[1] || r[2] != r[2] || r[3] != r[3]) o.err |= 2u;
+    orc_push_one<real>(o, k, kick, move, slo, shi);
+  }
+}
+
+// LSD radix sort of (priority, payload) pairs.
+void radix_sort_pairs(std::vector<uint64_t> &a) {
+  std::vector<uint64_t> tmp(a.size());
+  for (int shift = 32; shift < 64; shift += 8) {
+    size_t count[257] = {0};
+    for (uint64_t v : a) count[((v >> shift) & 0xFF) + 1]++;
+    for (int b = 0; b < 256; ++b) count[b + 1] += count[b];
+    for (uint64_t v : a) tmp[count[(v >> shift) & 0xFF]++] = v;
+    a.swap(tmp);
+  }
+}
+
+struct Cand {
+  uint32_t c, t, i, j;
+};
+
+// Collision_ESMC_new3 (collisions.cpp:583-720), candidates in priority order.
+template <typename real>
+void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
+  const mcpm_params &p = o.p;
+  const int nc = o.nc;
+  uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
+  // candidate counts (collisions.cpp:617-629)
+  std::vector<uint32_t> mc(nc), cs(nc + 1);
+  uint64_t M = 0;
+  for (int c = 0; c < nc; ++c) {
+    mcpm_u32x4 w = mcpm_philox((uint32_t)c, 0u, slo, mcpm_ctr3(MCPM_TAG_MCAND, shi, 0u), key0, key1);
+    double m = mcpm_floor(0.5 * o.omax[c] * (1.0 * (double)o.cnt[c]) + mcpm_u01(w.v[0]));
+    mc[c] = m > 0.0 ? (uint32_t)m : 0u;
+    cs[c] = (uint32_t)M;
+    M += mc[c];
+  }
+  cs[nc] = (uint32_t)M;
+  uint32_t pkey = mcpm_philox(0u, 0u, slo, mcpm_ctr3(MCPM_TAG_PRIO, shi, 0u), key0, key1).v[0];
+  std::vector<Cand> cand;
+  std::vector<uint64_t> order;
+  cand.reserve(M);
+  order.reserve(M);
+  for (int c = 0; c < nc; ++c) {
+    if (o.cnt[c] == 0) continue;
+    for (uint32_t t = 0; t < mc[c]; ++t) {
+      mcpm_u32x4 w = mcpm_philox(t, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_CAND, shi, 0u), key0, key1);
+      uint32_t i = (uint32_t)o.start[c] + mcpm_uint_below(w.v[0], (uint32_t)o.cnt[c]);
+      double d0, d1, d2;
+      mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
+      double yJ = o.rec[4 * (size_t)i] + p.sigma * d1;  // collisions.cpp:661-662
+      int J;
+      if (p.boundary == MCPM_BC_PERIODIC) {
+        if (yJ < 0.0) yJ += p.length;
+        else if (yJ >= p.length) yJ -= p.length;
+        J = orc_cell(yJ, o.dy, nc);
+      } else {
+        double q = mcpm_floor(yJ / o.dy);
+        if (!(q >= 0.0) || q >= (double)nc) continue;  // outside: no partner (:668-671)
+        J = (int)q;
+      }
+      if (o.cnt[J] == 0) continue;
+      uint32_t j = (uint32_t)o.start[J] + mcpm_uint_below(w.v[3], (uint32_t)o.cnt[J]);
+      if (i == j) continue;  // collisions.cpp:677-678
+      uint32_t pr = mcpm_priority(cs[c] + t, pkey);
+      order.push_back(((uint64_t)pr << 32) | (uint64_t)cand.size());
+      cand.push_back(Cand{(uint32_t)c, t, i, j});
+    }
+  }
+  radix_sort_pairs(order);
+  std::vector<double> omega(nc, 0.0);
+  o.log_p.resize(order.size());
+  o.log_i.resize(order.size());
+  o.log_j.resize(order.size());
+  o.log_a.resize(order.size());
+  int64_t acc = 0;
+  const double pref = 4.0 * PI * p.sigma * p.sigma;
+  for (size_t q = 0; q < order.size(); ++q) {
+    const Cand &cd = cand[order[q] & 0xFFFFFFFFu];
+    mcpm_u32x4 w = mcpm_philox(cd.t, cd.c, slo, mcpm_ctr3(MCPM_TAG_CAND, shi, 0u), key0, key1);
+    mcpm_u32x4 w2 = mcpm_philox(cd.t, cd.c, slo, mcpm_ctr3(MCPM_TAG_CAND, shi, 1u), key0, key1);
+    double d0, d1, d2;
+    mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
+    double *ri = &o.rec[4 * (size_t)cd.i];
+    double *rj = &o.rec[4 * (size_t)cd.j];
+    double yi = ri[0], yj = rj[0];
+    double ymid;
+    if (p.boundary == MCPM_BC_PERIODIC) {
+      double d = yj - yi;
+      if (d > 0.5 * p.length) d -= p.length;
+      else if (d < -0.5 * p.length) d += p.length;
+      ymid = yi + d / 2.0;
+      if (ymid < 0.0) ymid += p.length;
+      else if (ymid >= p.length) ymid -= p.length;
+    } else {
+      ymid = (yi + yj) / 2.0;  // collisions.cpp:682-683
+    }
+    int cm = orc_cell(ymid, o.dy, nc);
+    int J = orc_cell(yj, o.dy, nc);
+    double Ymid = o.yc[cm];  // collisions.cpp:687
+    double g0 = ri[1] - rj[1], g1 = ri[2] - rj[2], g2 = ri[3] - rj[3];
+    double gd = g0 * d0 + g1 * d1 + g2 * d2;
+    double pv = pref * gd * Ymid * o.nden[J] * p.delta_t;  // collisions.cpp:691
+    if (pv - omega[cd.c] > 0.0) omega[cd.c] = pv;          // :692-693
+    bool a = pv / o.omax[cd.c] > mcpm_u01(w2.v[0]);        // :694
+    if (a) {
+      ri[1] = rnd<real>(ri[1] - gd * d0);
+      ri[2] = rnd<real>(ri[2] - gd * d1);
+      ri[3] = rnd<real>(ri[3] - gd * d2);
+      rj[1] = rnd<real>(rj[1] + gd * d0);
+      rj[2] = rnd<real>(rj[2] + gd * d1);
+      rj[3] = rnd<real>(rj[3] + gd * d2);
+      ++acc;
+    }
+    o.log_p[q] = (uint32_t)(order[q] >> 32);
+    o.log_i[q] = cd.i;
+    o.log_j[q] = cd.j;
+    o.log_a[q] = a ? 1 : 0;
+  }
+  for (int c = 0; c < nc; ++c)
+    if (omega[c] > o.omax[c]) o.omax[c] = omega[c];  // collisions.cpp:714-716
+  o.last_cand = (int64_t)M;
+  o.last_acc = acc;
+  if (st) {
+    st->candidates = (int64_t)M;
+    st->valid = (int64_t)order.size();
+    st->accepted = acc;
+    st->rounds = 0;
+    st->reserved0 = 0;
+  }
+}
+
+// Thompson_3d (Math_Func.cpp:28-52): a main diagonal, b lower, c upper.
+void thomas(std::vector<double> a, const std::vector<double> &b, const std::vector<double> &c,
+            std::vector<double> &x, std::vector<double> rhs) {
+  int n = (int)a.size();
+  for (int i = 1; i < n; ++i) {
+    double F = (1.0 / a[i - 1]) * b[i - 1];
+    a[i] = a[i] - c[i - 1] * F;
+    rhs[i] = rhs[i] - F * rhs[i - 1];
+  }
+  x.assign(n, 0.0);
+  for (int i = n - 1; i >= 0; --i) {
+    double r = rhs[i];
+    if (i != n - 1) r = r - c[i] * x[i + 1];
+    x[i] = r / a[i];
+  }
+}
+
+void gl50(double *x, double *w) {
+  // 50-point Gauss-Legendre rule rounded to 8 decimals, the precision of the
+  // reference's table (Main_Functions.cpp:3328-3345).
+  const int n = 50;
+  for (int i = 0; i < n; ++i) {
+    double z = cos(PI * (i + 0.75) / (n + 0.5)), dp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p0 = 1.0, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        double p2 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      dp = n * (z * p1 - p0) / (z * z - 1.0);
+      double dz = p1 / dp;
+      z -= dz;
+      if (fabs(dz) < 1e-16) break;
+    }
+    x[n - 1 - i] = nearbyint(z * 1e8) / 1e8;
+    w[n - 1 - i] = nearbyint(2.0 / ((1.0 - z * z) * dp * dp) * 1e8) / 1e8;
+  }
+}
+
+// poisson_solver (Main_Functions.cpp:3285-3581) without the kick.
+void orc_field(Oracle &o) {
+  const mcpm_params &p = o.p;
+  const int nc = o.nc;
+  const double dy = o.dy;
+  const bool periodic = p.boundary == MCPM_BC_PERIODIC;
+  const int nf = periodic ? nc : nc + 1;
+  std::vector<double> nj(nf), a(nf), b(nf, -1.0), c(nf, -1.0), rhs(nf), x;
+  for (int i = 0; i < nf; ++i) {
+    if (periodic) nj[i] = (o.nden[i] + o.nden[(i + nc - 1) % nc]) / 2.0;
+    else if (i == 0) nj[i] = o.nden[0];
+    else if (i == nc) nj[i] = o.nden[nc - 1];
+    else nj[i] = (o.nden[i] + o.nden[i - 1]) / 2.0;
+    a[i] = 2.0 + pow(dy * p.bst, 2.0);
+    rhs[i] = nj[i] * pow(dy, 2.0);
+  }
+  if (!periodic) {
+    double xi[50], wi[50], rr[50], dr[50], ph[50], dph[50];
+    gl50(xi, wi);
+    double bb = 3.0 * p.eqdist, aa = 0.0, cc = 0.0, dd = PI;
+    for (int i = 0; i < 50; ++i) {
+      rr[i] = (bb - aa) / 2.0 * xi[i] + (aa + bb) / 2.0;
+      dr[i] = wi[i] * (bb - aa) / 2.0;
+      ph[i] = (dd - cc) / 2.0 * xi[i] + (cc + dd) / 2.0;
+      dph[i] = wi[i] * (dd - cc) / 2.0;
+    }
+    double V[2] = {0.0, 0.0};
+    for (int side = 0; side < 2; ++side) {
+      double f = side == 0 ? 0.0 : (double)nc;
+      for (int i = 0; i < 50; ++i) {
+        double radial = exp(-p.bst * rr[i]) / 2.0 * rr[i] * dr[i];
+        for (int ii = 0; ii < 50; ++ii) {
+          double y = f * dy + rr[i] * cos(ph[ii]);
+          double q = floor(y / dy);
+          double n;
+          if (q < 0.0) n = o.nden[0];
+          else if (q > (double)(nc - 1)) n = o.nden[nc - 1];
+          else n = o.nden[(int)q];
+          V[side] += radial * n * sin(ph[ii]) * dph[ii];
+        }
+      }
+    }
+    c[0] = 0.0;
+    a[0] = 1.0;
+    rhs[0] = V[0];
+    a[nf - 1] = 1.0;
+    rhs[nf - 1] = V[1];
+    b[nf - 2] = 0.0;
+    thomas(a, b, c, x, rhs);
+  } else {
+    // cyclic tridiagonal: Sherman-Morrison around Thompson_3d
+    double gamma = -a[0], alpha = -1.0, beta = -1.0;
+    std::vector<double> ap = a, u(nf, 0.0), z, y;
+    ap[0] = a[0] - gamma;
+    ap[nf - 1] = a[nf - 1] - alpha * beta / gamma;
+    thomas(ap, b, c, y, rhs);
+    u[0] = gamma;
+    u[nf - 1] = alpha;
+    thomas(ap, b, c, z, u);
+    double fact = (y[0] + beta * y[nf - 1] / gamma) / (1.0 + z[0] + beta * z[nf - 1] / gamma);
+    x.resize(nf);
+    for (int i = 0; i < nf; ++i) x[i] = y[i] - fact * z[i];
+  }
+  double core = 1.0 - (p.sigma * p.bst + 1.0) * exp(-p.bst * p.sigma);  // :3539-3543
+  core = core / p.bst / p.bst;
+  for (int i = 0; i < nf; ++i) x[i] = x[i] - nj[i] * core;
+  for (int cc = 0; cc < nc; ++cc) {
+    int f1 = periodic ? (cc + 1) % nc : cc + 1;
+    double F = -(x[f1] - x[cc]) / dy;  // :3571-3572
+    o.F2[cc] = F / p.mass * p.ast;
+  }
+  o.kick_pending = true;
+}
+
+void orc_moments(const Oracle &o, double *out) {
+  // two-pass moments of cell_update (Main_Functions.cpp:2483-2593)
+  for (int c = 0; c < o.nc; ++c) {
+    double *m = out + (size_t)c * MCPM_NMOM;
+    for (int k = 0; k < MCPM_NMOM; ++k) m[k] = 0.0;
+    int N = o.cnt[c];
+    double U[3] = {0, 0, 0};
+    for (int k = 0; k < N; ++k) {
+      const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      U[0] += r[1];
+      U[1] += r[2];
+      U[2] += r[3];
+    }
+    if (N > 0)
+      for (int q = 0; q < 3; ++q) U[q] /= (double)N;
+    double a[26] = {0};
+    for (int k = 0; k < N; ++k) {
+      const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      double p1 = r[1] - U[0], p2 = r[2] - U[1], p3 = r[3] - U[2];
+      double e = p1 * p1 + p2 * p2 + p3 * p3;
+      a[0] += p1 * p1; a[1] += p1 * p2; a[2] += p1 * p3;
+      a[3] += p2 * p2; a[4] += p2 * p3; a[5] += p3 * p3;
+      a[6] += p1 * p1 * p1; a[7] += p1 * p1 * p2; a[8] += p1 * p1 * p3;
+      a[9] += p1 * p2 * p2; a[10] += p1 * p2 * p3; a[11] += p1 * p3 * p3;
+      a[12] += p2 * p2 * p2; a[13] += p2 * p2 * p3; a[14] += p2 * p3 * p3;
+      a[15] += p3 * p3 * p3;
+      a[16] += p1 * p1 * e; a[17] += p1 * p2 * e; a[18] += p1 * p3 * e;
+      a[19] += p2 * p2 * e; a[20] += p2 * p3 * e; a[21] += p3 * p3 * e;
+      a[22] += p1 * e * e; a[23] += p2 * e * e; a[24] += p3 * e * e;
+      a[25] += e;
+    }
+    m[0] = U[0]; m[1] = U[1]; m[2] = U[2];
+    for (int q = 0; q < 26; ++q) m[3 + q] = N > 0 ? a[q] / (double)N : 0.0;
+    m[29] = o.p.mass * m[28] / (3.0 * o.p.kb);  // T = m DM2 / (3 kB)
+    m[30] = (double)N;
+  }
+}
+
+void orc_sample(Oracle &o) {
+  // cell_poly_velocity (Main_Functions.cpp:1918-1968)
+  for (int c = 0; c < o.nc; ++c) {
+    double *s = &o.samples[(size_t)c * MCPM_NSAMP];
+    for (int k = 0; k < o.cnt[c]; ++k) {
+      const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      double U1 = r[1], U2 = r[2], U3 = r[3];
+      s[0] += 1.0;
+      s[1] += U1; s[2] += U2; s[3] += U3;
+      s[4] += U1 * U1; s[5] += U1 * U2; s[6] += U1 * U3;
+      s[7] += U2 * U2; s[8] += U2 * U3; s[9] += U3 * U3;
+      int id = 0;
+      for (int mm = 0; mm < 4; ++mm)
+        for (int nn = 0; nn < 4 - mm; ++nn) {
+          int ll = 3 - mm - nn;
+          s[10 + id] += pow(U1, mm) * pow(U2, nn) * pow(U3, ll);
+          ++id;
+        }
+      double e = U1 * U1 + U2 * U2 + U3 * U3;
+      s[20] += U1 * U1 * e; s[21] += U1 * U2 * e; s[22] += U1 * U3 * e;
+      s[23] += U2 * U2 * e; s[24] += U2 * U3 * e; s[25] += U3 * U3 * e;
+    }
+  }
+}
+
+int take_err(Oracle &o) {
+  uint32_t e = o.err;
+  o.err = 0;
+  if (e & 1u) return MCPM_ERR_DOMAIN;
+  if (e & 2u) return MCPM_ERR_NAN;
+  if (e & 4u) return MCPM_ERR_BC;
+  return MCPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void *orc_create(const mcpm_params *p) {
+  if (!p || p->n_cells < 1) return nullptr;
+  Oracle *o = new Oracle();
+  o->p = *p;
+  o->nc = p->n_cells;
+  o->dy = p->length / (double)p->n_cells;
+  o->cnt.assign(o->nc, 0);
+  o->start.assign(o->nc + 1, 0);
+  o->nden.assign(o->nc, 0.0);
+  o->yc.assign(o->nc, 1.0);
+  o->omax.assign(o->nc, 0.0);
+  o->F2.assign(o->nc, 0.0);
+  o->samples.assign((size_t)o->nc * MCPM_NSAMP, 0.0);
+  return o;
+}
+
+void orc_destroy(void *h) { delete (Oracle *)h; }
+
+int orc_upload(void *h, int64_t n, const double *y, const double *u1, const double *u2,
+               const double *u3) {
+  Oracle &o = *(Oracle *)h;
+  o.n = n;
+  o.rec.assign(4 * (size_t)n, 0.0);
+  bool f32 = o.p.precision == MCPM_FP32;
+  for (int64_t k = 0; k < n; ++k) {
+    double v[4] = {y[k], u1[k], u2[k], u3[k]};
+    for (int q = 0; q < 4; ++q) o.rec[4 * k + q] = f32 ? (double)(float)v[q] : v[q];
+  }
+  o.sorted = false;
+  o.kick_pending = false;
+  return MCPM_OK;
+}
+
+int orc_download(void *h, double *y, double *u1, double *u2, double *u3) {
+  Oracle &o = *(Oracle *)h;
+  for (int64_t k = 0; k < o.n; ++k) {
+    if (y) y[k] = o.rec[4 * k];
+    if (u1) u1[k] = o.rec[4 * k + 1];
+    if (u2) u2[k] = o.rec[4 * k + 2];
+    if (u3) u3[k] = o.rec[4 * k + 3];
+  }
+  return MCPM_OK;
+}
+
+int orc_cell_update(void *h) {
+  Oracle &o = *(Oracle *)h;
+  orc_sort(o, true);
+  return take_err(o);
+}
+
+int orc_collide_esmc(void *h, int64_t step, mcpm_collision_stats *st) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  if (o.p.precision == MCPM_FP32) orc_collide<float>(o, step, st);
+  else orc_collide<double>(o, step, st);
+  return MCPM_OK;
+}
+
+int orc_poisson_solver(void *h) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  orc_field(o);
+  return MCPM_OK;
+}
+
+int orc_kick(void *h) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.kick_pending) return MCPM_OK;
+  if (o.p.precision == MCPM_FP32) orc_push<float>(o, true, false, 0);
+  else orc_push<double>(o, true, false, 0);
+  o.kick_pending = false;
+  orc_sort(o, false);
+  return take_err(o);
+}
+
+int orc_stream_bc(void *h, int64_t step, int32_t with_kick) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  bool kick = with_kick && o.kick_pending;
+  if (o.p.precision == MCPM_FP32) orc_push<float>(o, kick, true, step);
+  else orc_push<double>(o, kick, true, step);
+  o.kick_pending = false;
+  orc_sort(o, false);
+  return take_err(o);
+}
+
+int orc_step(void *h, int64_t step, mcpm_collision_stats *st) {
+  Oracle &o = *(Oracle *)h;
+  int r = orc_collide_esmc(h, step, st);
+  if (r) return r;
+  if (o.p.long_range == MCPM_LR_SCREENED_POISSON) orc_field(o);
+  return orc_stream_bc(h, step, 1);
+}
+
+int orc_moments(void *h, double *out) {
+  orc_moments(*(Oracle *)h, out);
+  return MCPM_OK;
+}
+
+int orc_sample(void *h) {
+  orc_sample(*(Oracle *)h);
+  return MCPM_OK;
+}
+
+int orc_get_samples(void *h, double *out) {
+  Oracle &o = *(Oracle *)h;
+  memcpy(out, o.samples.data(), o.samples.size() * sizeof(double));
+  return MCPM_OK;
+}
+
+int orc_set_omega_max(void *h, const double *om) {
+  Oracle &o = *(Oracle *)h;
+  for (int c = 0; c < o.nc; ++c) o.omax[c] = om[c];
+  return MCPM_OK;
+}
+
+int orc_init_omega_max(void *h, double n_gas, int32_t times_y) {
+  // main.cpp:250-269
+  Oracle &o = *(Oracle *)h;
+  std::vector<double> mom((size_t)o.nc * MCPM_NMOM);
+  orc_moments(o, mom.data());
+  double y = times_y ? orc_y(n_gas, o.p) : 1.0;
+  for (int c = 0; c < o.nc; ++c) {
+    double T = mom[(size_t)c * MCPM_NMOM + 29];
+    double crm = 10.0 * sqrt(o.p.kb * T / o.p.mass);
+    double om = 4.0 * PI * o.p.sigma * o.p.sigma * crm * o.nden[c] * o.p.delta_t;
+    o.omax[c] = om * y;
+  }
+  return MCPM_OK;
+}
+
+int orc_get_cells(void *h, int32_t *count, int32_t *start, double *n, double *omax, double *F2) {
+  Oracle &o = *(Oracle *)h;
+  for (int c = 0; c < o.nc; ++c) {
+    if (count) count[c] = o.cnt[c];
+    if (start) start[c] = o.start[c];
+    if (n) n[c] = o.nden[c];
+    if (omax) omax[c] = o.omax[c];
+    if (F2) F2[c] = o.F2[c];
+  }
+  return MCPM_OK;
+}
+
+int orc_get_collision_log(void *h, int64_t cap, uint32_t *prio, uint32_t *i, uint32_t *j,
+                          uint8_t *acc, int64_t *n_out) {
+  Oracle &o = *(Oracle *)h;
+  int64_t n = (int64_t)o.log_p.size();
+  *n_out = n;
+  if (n > cap) return MCPM_ERR_CAPACITY;
+  for (int64_t k = 0; k < n; ++k) {
+    if (prio) prio[k] = o.log_p[k];
+    if (i) i[k] = o.log_i[k];
+    if (j) j[k] = o.log_j[k];
+    if (acc) acc[k] = o.log_a[k];
+  }
+  return MCPM_OK;
+}
+
+/* Philox and deterministic-math probes for the known-answer tests. */
+void orc_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3, uint32_t k0, uint32_t k1,
+                uint32_t *out) {
+  mcpm_u32x4 r = mcpm_philox(c0, c1, c2, c3, k0, k1);
+  for (int q = 0; q < 4; ++q) out[q] = r.v[q];
+}
+double orc_det_log(double x) { return mcpm_log(x); }
+double orc_det_exp(double x) { return mcpm_exp(x); }
+void orc_det_sincos2pi(double u, double *s, double *c) { mcpm_sincos2pi(u, s, c); }
+uint32_t orc_priority(uint32_t k, uint32_t key) { return mcpm_priority(k, key); }
+double orc_contact_y(const mcpm_params *p, double n) { return orc_y(n, *p); }
+void orc_gl50(double *x, double *w) { gl50(x, w); }
+void orc_thomas(int n, const double *a, const double *b, const double *c, const double *rhs,
+                double *x) {
+  std::vector<double> A(a, a + n), B(b, b + n), C(c, c + n), R(rhs, rhs + n), X;
+  thomas(A, B, C, X, R);
+  for (int k = 0; k < n; ++k) x[k] = X[k];
+}
+
+}  // extern "C"
