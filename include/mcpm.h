@@ -51,6 +51,17 @@ enum mcpm_boundary { MCPM_BC_DIFFUSE = 0, MCPM_BC_PERIODIC = 1, MCPM_BC_SPECULAR
 enum mcpm_ymodel { MCPM_Y_REFERENCE = 0, MCPM_Y_CARNAHAN_STARLING = 1 };
 /* Long-range attraction switch of velocity_update (Main_Functions.cpp:1612-1668). */
 enum mcpm_long_range { MCPM_LR_NONE = 0, MCPM_LR_SCREENED_POISSON = 3 };
+/* Velocity model of velocity_update (gas.model, Main_Functions.cpp:1545-1600):
+   ESMC -> Collision_ESMC_new3, FP_DENSE / FP_IDEAL -> Velocity_FP_parallel
+   with or without the dense-gas Y(n) and closures, DSMC_VHS ->
+   Collision_DSMC_VHS, NONE -> free flight. */
+enum mcpm_model {
+  MCPM_MODEL_ESMC = 0,
+  MCPM_MODEL_FP_DENSE = 1,
+  MCPM_MODEL_FP_IDEAL = 2,
+  MCPM_MODEL_DSMC_VHS = 3,
+  MCPM_MODEL_NONE = 4
+};
 
 /* ---- parameters -------------------------------------------------------- */
 /* The subset of the reference's struct GAS / struct BOX that the slab step
@@ -62,7 +73,7 @@ typedef struct mcpm_params {
   int32_t  y_model;      /* enum mcpm_ymodel                                   */
   int32_t  long_range;   /* enum mcpm_long_range (gas.long_range)              */
   int32_t  n_cells;      /* box.N[1]: cells along the slab axis                */
-  int32_t  reserved0;
+  int32_t  model;        /* enum mcpm_model (gas.model)                        */
   int64_t  capacity;     /* maximum number of particles the context holds      */
   uint64_t seed;         /* Philox key for every random draw                   */
   double   length;       /* box.Len[1]                                          */
@@ -83,6 +94,10 @@ typedef struct mcpm_params {
   double   ast;          /* gas.ast: Yukawa amplitude (myIO.cpp:2891)           */
   double   bst;          /* gas.bst: Yukawa screening lambda (myIO.cpp:2892)    */
   double   eqdist;       /* gas.eqdist: Dirichlet quadrature reach / 3          */
+  double   mu;           /* gas.mu: viscosity at T0 (myIO.cpp:2829)             */
+  double   T0;           /* gas.T0: reference temperature of mu(T)              */
+  double   visp;         /* gas.visp: mu ~ T^visp (scale_mu, Main_Functions.cpp:1970) */
+  double   crref;        /* gas.crref: VHS reference relative speed             */
 } mcpm_params;
 
 typedef struct mcpm_ctx mcpm_ctx;
@@ -143,8 +158,11 @@ int mcpm_kick(mcpm_ctx *ctx);
    into the same pass. */
 int mcpm_stream_bc(mcpm_ctx *ctx, int64_t step, int32_t with_kick);
 
-/* One whole time step in the reference's order (main.cpp:389-669):
-   collisions, mean-field solve, kick + stream + boundary + re-sort. */
+/* One whole time step in the reference's order (main.cpp:389-669): the
+   velocity model of params.model (ESMC collisions, FP closures + velocity
+   update, DSMC collisions or nothing), mean-field solve, kick, the FP_dense
+   position drift (DFP_extra_streaming, Fokker_Planck.cpp:904-946), stream,
+   boundary and re-sort. */
 int mcpm_step(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
 
 /* ---- cell state ---------------------------------------------------------- */
@@ -162,9 +180,36 @@ int mcpm_get_cells(mcpm_ctx *ctx, int32_t *count, int32_t *start, double *n,
 
 /* Per-cell moments of cell_update (Main_Functions.cpp:2461-2593), FP64,
    two-pass: out[c*MCPM_NMOM + k] with k = 0..2 U_space, 3..8 PIJ (11,12,
-   13,22,23,33), 9..18 M3, 19..24 M4, 25..27 M5, 28 DM2, 29 T, 30 N_c. */
-#define MCPM_NMOM 31
+   13,22,23,33), 9..18 M3 (111,112,113,122,123,133,222,223,233,333),
+   19..24 M4, 25..27 M5, 28 DM2, 29 T, 30 N_c, 31..33 Mp (mean peculiar
+   velocity, zero up to round-off). */
+#define MCPM_NMOM 34
 int mcpm_moments(mcpm_ctx *ctx, double *out);
+
+/* ---- Fokker-Planck model ------------------------------------------------- */
+/* The closure part of cell_update for the FP models: Lambda and gamma_st
+   (Main_Functions.cpp:2602-2605, 2718-2728), alpha_calculation when model is
+   FP_DENSE and step > 0 (cpp_headers.h:541; Fokker_Planck.cpp:200-587),
+   c_gamma_calculation (cpp_headers.h:467; Fokker_Planck.cpp:590-804), both
+   with Gauss_Jordan (Math_Func.cpp:165-232), and phi = alpha Mp
+   (Main_Functions.cpp:2734-2746).  Needs sorted particles. */
+int mcpm_fp_closures(mcpm_ctx *ctx, int64_t step);
+/* random_generator + Velocity_FP_parallel (cpp_headers.h:362,632;
+   Fokker_Planck.cpp:20-145, 1065-1182): per-cell zero-mean unit-variance
+   Gaussian noise, exact Ornstein-Uhlenbeck step with the closures' drift,
+   per-cell kinetic-energy rescale.  Uses the last mcpm_fp_closures. */
+int mcpm_velocity_fp(mcpm_ctx *ctx, int64_t step);
+/* Closures in cell order: out[c*MCPM_NCLOS + k], k = 0 Lambda, 1..6 c_ij,
+   7..9 gamma_i, 10..18 alpha, 19..21 beta, 22 gamma_st, 23..25 phi,
+   26 div U, 27 tau = 2 mu(T) / (n kB T). */
+#define MCPM_NCLOS 28
+int mcpm_get_closures(mcpm_ctx *ctx, double *out);
+
+/* ---- plain DSMC ------------------------------------------------------------ */
+/* Collision_DSMC_VHS (cpp_headers.h:612; collisions.cpp:2063-2141): NTC
+   pairs inside each cell, VHS cross-section, isotropic scattering, per-cell
+   crm (kept in the omega_max array) <- max(1.1 crm_seen, 0.9 crm). */
+int mcpm_collide_dsmc_vhs(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
 
 /* cell_poly_velocity (cpp_headers.h:470; Main_Functions.cpp:1918-1968):
    accumulate the per-cell time-average sums.  mcpm_get_samples returns

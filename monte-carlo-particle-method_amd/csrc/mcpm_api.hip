@@ -49,6 +49,10 @@ struct mcpm_ctx {
   uint8_t *log_acc = nullptr;
   bool debug_log = false;
   int64_t last_ops = 0;
+  // Fokker-Planck / DSMC
+  FPParams fpp{};
+  double *clos = nullptr, *drift = nullptr;
+  bool have_closures = false;
   // field
   FieldArrays fa{};
   double *field_buf = nullptr;
@@ -277,9 +281,13 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   const Rec<real> *in = (const Rec<real> *)c->P[c->cur];
   Rec<real> *out = (Rec<real> *)c->P[1 - c->cur];
   const size_t lds = (size_t)nc * sizeof(uint32_t);
+  const double *drift = (c->p.model == MCPM_MODEL_FP_DENSE && c->have_closures &&
+                         (mode & PUSH_MOVE))
+                            ? c->drift
+                            : nullptr;
   hipLaunchKernelGGL((k_push<real, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
-                     nullptr, c->n, chunk, c->kp, c->F2, mode, slo, shi, c->table, nullptr,
-                     c->err);
+                     nullptr, c->n, chunk, c->kp, c->F2, drift, mode, slo, shi, c->table,
+                     nullptr, c->err);
   LAUNCHCHK();
   dim3 gs((nc + 255) / 256, nseg);
   hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
@@ -289,7 +297,8 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
                      c->seg);
   hipLaunchKernelGGL((k_push<real, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
-                     c->n, chunk, c->kp, c->F2, mode, slo, shi, c->table, c->start, c->err);
+                     c->n, chunk, c->kp, c->F2, drift, mode, slo, shi, c->table, c->start,
+                     c->err);
   LAUNCHCHK();
   c->cur = 1 - c->cur;
   c->sorted = true;
@@ -330,13 +339,13 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   return MCPM_OK;
 }
 
-template <typename real>
+template <typename real, int MODEL>
 int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   const int nc = c->nc;
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
   Rec<real> *P = (Rec<real> *)c->P[c->cur];
   hipLaunchKernelGGL(k_mcand, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->cnt, c->omax,
-                     c->mcand, nc, c->kp, slo, shi);
+                     c->nden, c->mcand, nc, c->kp, (int)MODEL, slo, shi);
   hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->mcand, c->cstart, nc);
   hipLaunchKernelGGL(k_zero_u32, dim3((CNT_TOTAL + 255) / 256), dim3(256), 0, c->s,
                      c->counters, (int)CNT_TOTAL);
@@ -350,7 +359,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
                               c->kp.key1);
   {
     Timer tm(c, T_GEN);
-    hipLaunchKernelGGL((k_gen<real>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
+    hipLaunchKernelGGL((k_gen<real, MODEL>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
                        c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->ops, c->res[0],
                        c->counters);
     LAUNCHCHK();
@@ -362,10 +371,10 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     uint8_t *lg = c->debug_log ? c->log_acc : nullptr;
     int next_check = 12;
     for (int r = 0; r < MAX_ROUNDS; ++r) {
-      hipLaunchKernelGGL((k_round<real>), dim3(grid), dim3(256), 0, c->s, r, P, c->ops,
+      hipLaunchKernelGGL((k_round<real, MODEL>), dim3(grid), dim3(256), 0, c->s, r, P, c->ops,
                          r == 0 ? (const uint32_t *)nullptr : (const uint32_t *)c->list[r & 1],
                          c->list[(r + 1) & 1], c->res[r & 1], c->res[(r + 1) & 1], c->counters,
-                         c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, lg);
+                         c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg);
       LAUNCHCHK();
       rounds = r + 1;
       if (rounds == next_check) {
@@ -379,7 +388,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     }
   }
   hipLaunchKernelGGL(k_omega_finish, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->omax,
-                     c->omega_new, nc);
+                     c->omega_new, c->cnt, nc, (int)MODEL);
   LAUNCHCHK();
   uint32_t hc[CNT_LIST + 2];
   HIPCHK(hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->s));
@@ -431,6 +440,7 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
   if (p->long_range == MCPM_LR_SCREENED_POISSON && !(p->bst > 0.0)) return MCPM_ERR_ARG;
   if (p->long_range != MCPM_LR_NONE && p->long_range != MCPM_LR_SCREENED_POISSON)
     return MCPM_ERR_ARG;
+  if (p->model < MCPM_MODEL_ESMC || p->model > MCPM_MODEL_NONE) return MCPM_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCPM_ERR_HIP;
   mcpm_ctx *c = new mcpm_ctx();
@@ -462,6 +472,12 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
   k.coll_pref = 4.0 * PI * p->sigma * p->sigma;
   k.key0 = (uint32_t)p->seed;
   k.key1 = (uint32_t)(p->seed >> 32);
+  c->fpp.model = p->model;
+  c->fpp.reserved = 0;
+  c->fpp.mu = p->mu;
+  c->fpp.T0 = p->T0;
+  c->fpp.visp = p->visp;
+  c->fpp.crref = p->crref;
   c->rec = 4 * (p->precision == MCPM_FP32 ? sizeof(float) : sizeof(double));
   const int nc = c->nc;
   int st = MCPM_OK;
@@ -489,6 +505,8 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
   if ((st = dalloc(&c->moments, (size_t)nc * MCPM_NMOM))) return fail(st);
   if ((st = dalloc(&c->samples, (size_t)nc * MCPM_NSAMP))) return fail(st);
   if ((st = dalloc(&c->counters, CNT_TOTAL))) return fail(st);
+  if ((st = dalloc(&c->clos, (size_t)nc * MCPM_NCLOS))) return fail(st);
+  if ((st = dalloc(&c->drift, (size_t)nc * DRIFT_N))) return fail(st);
   if ((st = dalloc(&c->err, 1))) return fail(st);
   if ((st = dalloc(&c->res[0], (size_t)p->capacity))) return fail(st);
   if ((st = dalloc(&c->res[1], (size_t)p->capacity))) return fail(st);
@@ -515,7 +533,7 @@ int mcpm_destroy(mcpm_ctx *c) {
   void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
                   c->samples, c->counters, c->err, c->res[0], c->res[1], c->ops.i,
-                  c->log_acc, c->field_buf};
+                  c->log_acc, c->field_buf, c->clos, c->drift};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < T_NCLASS; ++k)
@@ -606,8 +624,60 @@ int mcpm_collide_esmc(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     if (stats) memset(stats, 0, sizeof *stats);
     return MCPM_OK;
   }
-  return c->p.precision == MCPM_FP32 ? collide_t<float>(c, step, stats)
-                                     : collide_t<double>(c, step, stats);
+  return c->p.precision == MCPM_FP32 ? collide_t<float, COLL_ESMC>(c, step, stats)
+                                     : collide_t<double, COLL_ESMC>(c, step, stats);
+}
+
+int mcpm_collide_dsmc_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->n == 0) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return MCPM_OK;
+  }
+  return c->p.precision == MCPM_FP32 ? collide_t<float, COLL_DSMC>(c, step, stats)
+                                     : collide_t<double, COLL_DSMC>(c, step, stats);
+}
+
+int mcpm_fp_closures(mcpm_ctx *c, int64_t step) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->p.precision == MCPM_FP32)
+    hipLaunchKernelGGL((k_moments<float>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
+  else
+    hipLaunchKernelGGL((k_moments<double>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<double> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
+  hipLaunchKernelGGL(k_fp_closures, dim3((c->nc + 63) / 64), dim3(64), 0, c->s, c->moments,
+                     c->cnt, c->nden, c->yc, c->clos, c->drift, c->kp, c->fpp, step, c->err);
+  LAUNCHCHK();
+  c->have_closures = true;
+  return MCPM_OK;
+}
+
+int mcpm_velocity_fp(mcpm_ctx *c, int64_t step) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->have_closures) return MCPM_ERR_STATE;
+  const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  if (c->p.precision == MCPM_FP32)
+    hipLaunchKernelGGL((k_fp_velocity<float>), dim3(c->nc), dim3(256), 0, c->s,
+                       (Rec<float> *)c->P[c->cur], c->cnt, c->start, c->moments, c->clos, c->yc,
+                       c->kp, c->fpp, slo, shi, c->err);
+  else
+    hipLaunchKernelGGL((k_fp_velocity<double>), dim3(c->nc), dim3(256), 0, c->s,
+                       (Rec<double> *)c->P[c->cur], c->cnt, c->start, c->moments, c->clos, c->yc,
+                       c->kp, c->fpp, slo, shi, c->err);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
+int mcpm_get_closures(mcpm_ctx *c, double *out) {
+  if (!c || !out) return MCPM_ERR_ARG;
+  if (!c->have_closures) return MCPM_ERR_STATE;
+  HIPCHK(hipMemcpyAsync(out, c->clos, (size_t)c->nc * MCPM_NCLOS * sizeof(double),
+                        hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return MCPM_OK;
 }
 
 int mcpm_poisson_solver(mcpm_ctx *c) {
@@ -645,13 +715,27 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   if (!c) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
   Timer tm(c, T_STEP);
-  int st = mcpm_collide_esmc(c, step, stats);
+  int st = MCPM_OK;
+  if (stats) memset(stats, 0, sizeof *stats);
+  // velocity_update dispatch (Main_Functions.cpp:1545-1600)
+  switch (c->p.model) {
+    case MCPM_MODEL_ESMC: st = mcpm_collide_esmc(c, step, stats); break;
+    case MCPM_MODEL_FP_DENSE:
+    case MCPM_MODEL_FP_IDEAL:
+      st = mcpm_fp_closures(c, step);
+      if (!st) st = mcpm_velocity_fp(c, step);
+      break;
+    case MCPM_MODEL_DSMC_VHS: st = mcpm_collide_dsmc_vhs(c, step, stats); break;
+    default: break;
+  }
   if (st) return st;
   if (c->p.long_range == MCPM_LR_SCREENED_POISSON) {
     st = mcpm_poisson_solver(c);
     if (st) return st;
   }
-  return mcpm_stream_bc(c, step, 1);
+  st = mcpm_stream_bc(c, step, 1);
+  c->have_closures = false;
+  return st;
 }
 
 int mcpm_moments(mcpm_ctx *c, double *out) {

@@ -33,6 +33,25 @@ struct KParams {
   uint32_t key0, key1;  // Philox key from the seed
 };
 
+// Parameters of the Fokker-Planck and DSMC velocity models.
+struct FPParams {
+  int32_t model;
+  int32_t reserved;
+  double mu, T0, visp, crref;
+};
+
+// Per-cell constants of the FP_dense position drift, see k_fp_closures.
+constexpr int DRIFT_N = 12;
+
+// x^a through the deterministic log/exp (exact shortcuts for a = 0, 1/2, 1);
+// used for mu(T) (scale_mu) and the VHS cross-section.
+__host__ __device__ __forceinline__ double det_pow(double x, double a) {
+  if (a == 0.5) return __builtin_sqrt(x);
+  if (a == 1.0) return x;
+  if (a == 0.0) return 1.0;
+  return mcpm_exp(a * mcpm_log(x));
+}
+
 __host__ __device__ __forceinline__ double contact_y(double n, double b, int model) {
   // increase_collision_rate, Main_Functions.cpp:1906-1915
   double nb = n * b;

@@ -34,30 +34,35 @@ struct FieldArrays {
   double sm_vlast, sm_den;
 };
 
+// collision models sharing the reservation rounds
+enum : int { COLL_ESMC = 0, COLL_DSMC = 1 };
+
 template <typename real, bool SCATTER>
 __global__ void k_push(const Rec<real> *in, Rec<real> *out, int64_t n, int32_t chunk,
-                       KParams kp, const double *F2, int mode, uint32_t step_lo,
-                       uint32_t step_hi, uint32_t *table, const uint32_t *start,
-                       uint32_t *err);
+                       KParams kp, const double *F2, const double *drift, int mode,
+                       uint32_t step_lo, uint32_t step_hi, uint32_t *table,
+                       const uint32_t *start, uint32_t *err);
 __global__ void k_scan_a(const uint32_t *table, uint32_t *partial, int nchunks, int nc, int seg);
 __global__ void k_scan_b(uint32_t *partial, int nseg, int nc, uint32_t *cnt, uint32_t *start,
                          double *nden, double *yc, KParams kp);
 __global__ void k_scan_c(uint32_t *table, const uint32_t *partial, int nchunks, int nc, int seg);
-__global__ void k_mcand(const uint32_t *cnt, const double *omax, uint32_t *mcand, int nc,
-                        KParams kp, uint32_t step_lo, uint32_t step_hi);
+__global__ void k_mcand(const uint32_t *cnt, const double *omax, const double *nden,
+                        uint32_t *mcand, int nc, KParams kp, int model, uint32_t step_lo,
+                        uint32_t step_hi);
 __global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc);
-template <typename real>
+template <typename real, int MODEL>
 __global__ void k_gen(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
                       const uint32_t *mcand, const uint32_t *cstart, KParams kp,
                       uint32_t step_lo, uint32_t step_hi, uint32_t prio_key, OpArrays ops,
                       uint32_t *res0, uint32_t *counters);
-template <typename real>
+template <typename real, int MODEL>
 __global__ void k_round(int r, Rec<real> *P, OpArrays ops, const uint32_t *list_in,
                         uint32_t *list_out, uint32_t *res_cur, uint32_t *res_next,
                         uint32_t *counters, const double *nden, const double *yc,
                         const double *omax, double *omega_new, KParams kp, uint32_t step_lo,
-                        uint32_t step_hi, uint8_t *log_acc);
-__global__ void k_omega_finish(double *omax, double *omega_new, int nc);
+                        uint32_t step_hi, FPParams fpp, uint8_t *log_acc);
+__global__ void k_omega_finish(double *omax, double *omega_new, const uint32_t *cnt, int nc,
+                               int model);
 __global__ void k_zero_u32(uint32_t *p, int n);
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v);
 __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams kp, double ast,
@@ -68,5 +73,13 @@ __global__ void k_moments(const Rec<real> *P, const uint32_t *cnt, const uint32_
 template <typename real>
 __global__ void k_sample(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
                          double *acc);
+__global__ void k_fp_closures(const double *mom, const uint32_t *cnt, const double *nden,
+                              const double *yc, double *clos, double *drift, KParams kp,
+                              FPParams fpp, int64_t step, uint32_t *err);
+template <typename real>
+__global__ void k_fp_velocity(Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
+                              const double *mom, const double *clos, const double *yc,
+                              KParams kp, FPParams fpp, uint32_t slo, uint32_t shi,
+                              uint32_t *err);
 
 }  // namespace mcpm

@@ -26,7 +26,8 @@ namespace mcpm {
 // wall with the new velocity.  Draws: Philox(slot, bounce, step, WALL).
 template <typename real>
 __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const KParams &kp,
-                                              const double *__restrict__ F2, int mode,
+                                              const double *__restrict__ F2,
+                                              const double *__restrict__ drift, int mode,
                                               uint32_t step_lo, uint32_t step_hi,
                                               uint32_t *err) {
   double y = (double)r.y;
@@ -36,7 +37,24 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
   }
   if (!(mode & PUSH_MOVE)) return r;
   double v = (double)r.u2;
+  const double y_start = y;
+  bool drifted = false;
+  if (drift) {
+    // DFP_extra_streaming, slab axis (Fokker_Planck.cpp:932-937)
+    const double *d = drift + (size_t)cell_of(y, kp.dy, kp.n_cells) * DRIFT_N;
+    if (d[11] != 0.0) {
+      double a = (double)r.u1 - d[0], b = (double)r.u2 - d[1], c3 = (double)r.u3 - d[2];
+      double e = a * a + b * b + c3 * c3;
+      double dt = kp.dt;
+      y = y + dt * (a * d[4] + b * d[5] + c3 * d[6] + d[7] * (e - 3.0 * kp.kb * d[3] / kp.mass)) +
+          d[8] * (b * e - d[9]) * dt;
+      y = y - dt * d[10];
+      drifted = true;
+    }
+  }
   y = y + v * kp.dt;
+  // speed used for the time spent past a wall: the displacement this step
+  const double v_eff = drifted ? (y - y_start) / kp.dt : v;
   if (kp.boundary == MCPM_BC_PERIODIC) {
     if (y < 0.0) y += kp.length;
     else if (y >= kp.length) y -= kp.length;
@@ -62,7 +80,7 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
   }
   // diffuse walls
   double dt_left = kp.dt;
-  double v_prev = v;
+  double v_prev = v_eff;
   uint32_t bounce = 0;
   while (y < kp.wall_lo || y > kp.wall_hi) {
     if (bounce >= 20) {  // "TOO MANY BC OPERATIONS", Main_Functions.cpp:4958-4965
@@ -114,7 +132,8 @@ template <typename real, bool SCATTER>
 __global__ __launch_bounds__(64) void k_push(const Rec<real> *__restrict__ in,
                                              Rec<real> *__restrict__ out, int64_t n,
                                              int32_t chunk, KParams kp,
-                                             const double *__restrict__ F2, int mode,
+                                             const double *__restrict__ F2,
+                                             const double *__restrict__ drift, int mode,
                                              uint32_t step_lo, uint32_t step_hi,
                                              uint32_t *__restrict__ table,
                                              const uint32_t *__restrict__ start,
@@ -154,7 +173,7 @@ __global__ __launch_bounds__(64) void k_push(const Rec<real> *__restrict__ in,
           double yy = (double)r.y;
           if (!(yy >= 0.0) || !(yy <= kp.length)) atomicOr(err, ERRBIT_DOMAIN);
         }
-        r = push_one<real>(r, (uint64_t)p, kp, F2, mode, step_lo, step_hi, err);
+        r = push_one<real>(r, (uint64_t)p, kp, F2, drift, mode, step_lo, step_hi, err);
         key = cell_of((double)r.y, kp.dy, nc);
       }
       uint64_t active = __ballot(valid);
@@ -260,14 +279,27 @@ __global__ void k_scan_c(uint32_t *__restrict__ table, const uint32_t *__restric
 // ---------------------------------------------------------------------------
 // Candidate count per cell, collisions.cpp:617-629:
 //   m_c = floor(0.5 * omega_max_c * N_c + U)
+// DSMC (collisions.cpp:2087-2090): M_c = pi sigma^2 N_c n_c crm dt / 2 for
+// cells with two or more particles, crm kept in the omax array.
 __global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restrict__ omax,
-                        uint32_t *__restrict__ mcand, int nc, KParams kp, uint32_t step_lo,
-                        uint32_t step_hi) {
+                        const double *__restrict__ nden, uint32_t *__restrict__ mcand, int nc,
+                        KParams kp, int model, uint32_t step_lo, uint32_t step_hi) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   mcpm_u32x4 w = mcpm_philox((uint32_t)c, 0u, step_lo, mcpm_ctr3(MCPM_TAG_MCAND, step_hi, 0u),
                              kp.key0, kp.key1);
-  double m = mcpm_floor(0.5 * omax[c] * (1.0 * (double)cnt[c]) + mcpm_u01(w.v[0]));
+  double m;
+  if (model == COLL_DSMC) {
+    if (cnt[c] < 2) {
+      mcand[c] = 0u;
+      return;
+    }
+    double Mc = 3.14159265358979323846 * kp.sigma * kp.sigma * ((double)cnt[c] * nden[c]) *
+                omax[c] * kp.dt / 2.0;
+    m = mcpm_floor(Mc + mcpm_u01(w.v[0]));
+  } else {
+    m = mcpm_floor(0.5 * omax[c] * (1.0 * (double)cnt[c]) + mcpm_u01(w.v[0]));
+  }
   mcand[c] = m > 0.0 ? (uint32_t)m : 0u;
 }
 
@@ -336,9 +368,26 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const Rec
   return true;
 }
 
+// DSMC pair draw (collisions.cpp:2093-2098): two distinct particles of cell c.
+__device__ __forceinline__ void draw_dsmc_pair(uint32_t c, uint32_t t, const uint32_t *cnt,
+                                               const uint32_t *start, const KParams &kp,
+                                               uint32_t step_lo, uint32_t step_hi, uint32_t *oi,
+                                               uint32_t *oj) {
+  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
+                             kp.key1);
+  uint32_t Nc = cnt[c];
+  uint32_t li = mcpm_uint_below(w.v[0], Nc);
+  uint32_t lj = mcpm_uint_below(w.v[1], Nc - 1u);
+  if (lj >= li) lj += 1u;
+  *oi = start[c] + li;
+  *oj = start[c] + lj;
+}
+
 // One workgroup per cell: draw the cell's m_c candidates, keep the valid
-// ones as operations, reserve both particles for the first round.
-template <typename real>
+// ones as operations, reserve both particles for the first round.  ESMC
+// priorities are the keyed bijection of the candidate index; DSMC keeps the
+// reference's in-cell candidate order (cells do not interact).
+template <typename real, int MODEL>
 __global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
                                              const uint32_t *__restrict__ cnt,
                                              const uint32_t *__restrict__ start,
@@ -356,10 +405,17 @@ __global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
     uint32_t t = t0 + threadIdx.x;
     uint32_t i = 0, j = 0;
     bool ok = false;
-    if (t < m) ok = draw_candidate<real>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+    if (t < m) {
+      if (MODEL == COLL_DSMC) {
+        draw_dsmc_pair(c, t, cnt, start, kp, step_lo, step_hi, &i, &j);
+        ok = true;
+      } else {
+        ok = draw_candidate<real>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+      }
+    }
     uint32_t pos = wave_append(&counters[CNT_OPS], ok);
     if (ok) {
-      uint32_t pr = mcpm_priority(k0 + t, prio_key);
+      uint32_t pr = MODEL == COLL_DSMC ? k0 + t : mcpm_priority(k0 + t, prio_key);
       ops.i[pos] = i;
       ops.j[pos] = j;
       ops.p[pos] = pr;
@@ -423,6 +479,46 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t c, ui
   return false;
 }
 
+// Collision_DSMC_VHS commit (collisions.cpp:2099-2131): VHS cross-section
+// sigma_T ~ Vr^(2-2 visp) crref^(2 visp - 1), accept iff sigma_T / crm > U,
+// isotropic post-collision relative velocity about the centre of mass.
+template <typename real>
+__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t c, uint32_t t,
+                                            Rec<real> *P, const double *omax, double *omega_new,
+                                            const KParams &kp, const FPParams &fpp,
+                                            uint32_t step_lo, uint32_t step_hi) {
+  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
+                             kp.key1);
+  Rec<real> a = P[i];
+  Rec<real> bb = P[j];
+  double g0 = (double)a.u1 - (double)bb.u1;
+  double g1 = (double)a.u2 - (double)bb.u2;
+  double g2 = (double)a.u3 - (double)bb.u3;
+  double Vr = __builtin_sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+  double sig = det_pow(Vr, 2.0 - 2.0 * fpp.visp) * det_pow(fpp.crref, 2.0 * fpp.visp - 1.0);
+  atomic_max_pos_double(&omega_new[c], sig);
+  if (!(sig / omax[c] > mcpm_u01(w.v[2]))) return false;
+  mcpm_u32x4 w2 = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 1u), kp.key0,
+                              kp.key1);
+  double ct = 2.0 * mcpm_u01(w.v[3]) - 1.0;
+  double st = __builtin_sqrt(1.0 - ct * ct);
+  double sp, cp;
+  mcpm_sincos2pi(mcpm_u01(w2.v[0]), &sp, &cp);
+  double v0 = Vr * ct, v1 = Vr * st * cp, v2 = Vr * st * sp;
+  double c0 = 0.5 * ((double)a.u1 + (double)bb.u1);
+  double c1 = 0.5 * ((double)a.u2 + (double)bb.u2);
+  double c2 = 0.5 * ((double)a.u3 + (double)bb.u3);
+  a.u1 = (real)(c0 + 0.5 * v0);
+  a.u2 = (real)(c1 + 0.5 * v1);
+  a.u3 = (real)(c2 + 0.5 * v2);
+  bb.u1 = (real)(c0 - 0.5 * v0);
+  bb.u2 = (real)(c1 - 0.5 * v1);
+  bb.u3 = (real)(c2 - 0.5 * v2);
+  P[i] = a;
+  P[j] = bb;
+  return true;
+}
+
 // One round of deterministic reservations.  An operation commits when it
 // holds the smallest priority on both of its particles, which is exactly
 // when every earlier-priority operation sharing a particle has committed:
@@ -430,7 +526,7 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t c, ui
 // Losers reserve in the other array for the next round.  Each operation that
 // finds its own priority in the current array clears it, so the array is
 // back to all-ones when the round ends.
-template <typename real>
+template <typename real, int MODEL>
 __global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P, OpArrays ops,
                                                const uint32_t *__restrict__ list_in,
                                                uint32_t *__restrict__ list_out,
@@ -442,7 +538,7 @@ __global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P,
                                                const double *__restrict__ omax,
                                                double *__restrict__ omega_new, KParams kp,
                                                uint32_t step_lo, uint32_t step_hi,
-                                               uint8_t *__restrict__ log_acc) {
+                                               FPParams fpp, uint8_t *__restrict__ log_acc) {
   const uint32_t count = (r == 0) ? counters[CNT_OPS] : counters[CNT_LIST + r];
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t acc_local = 0;
@@ -458,8 +554,11 @@ __global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P,
       pr = ops.p[o];
       uint32_t ri = res_cur[i], rj = res_cur[j];
       if (ri == pr && rj == pr) {
-        bool acc = commit_op<real>(i, j, ops.c[o], ops.t[o], P, nden, yc, omax, omega_new, kp,
-                                   step_lo, step_hi);
+        bool acc = MODEL == COLL_DSMC
+                       ? commit_dsmc<real>(i, j, ops.c[o], ops.t[o], P, omax, omega_new, kp, fpp,
+                                           step_lo, step_hi)
+                       : commit_op<real>(i, j, ops.c[o], ops.t[o], P, nden, yc, omax, omega_new,
+                                         kp, step_lo, step_hi);
         acc_local += acc ? 1u : 0u;
         if (log_acc) log_acc[o] = acc ? 1 : 0;
         res_cur[i] = 0xFFFFFFFFu;
@@ -480,11 +579,20 @@ __global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P,
   if (lane_id() == 0 && acc_local) atomicAdd((unsigned long long *)&counters[CNT_ACC], (unsigned long long)acc_local);
 }
 
-__global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ omega_new, int nc) {
+__global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ omega_new,
+                               const uint32_t *__restrict__ cnt, int nc, int model) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   double o = omega_new[c];
-  if (o > omax[c]) omax[c] = o;  // collisions.cpp:714-716
+  if (model == COLL_DSMC) {
+    // crm <- max(1.1 crm_seen, 0.9 crm) (collisions.cpp:2136)
+    if (cnt[c] > 1) {
+      double a = 1.1 * o, b = 0.9 * omax[c];
+      omax[c] = a > b ? a : b;
+    }
+  } else if (o > omax[c]) {
+    omax[c] = o;  // collisions.cpp:714-716
+  }
   omega_new[c] = 0.0;
 }
 
@@ -674,7 +782,7 @@ __global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P
                                                  const uint32_t *__restrict__ cnt,
                                                  const uint32_t *__restrict__ start,
                                                  double *__restrict__ out, KParams kp) {
-  __shared__ double sh[28 * 4];
+  __shared__ double sh[29 * 4];
   const int c = blockIdx.x;
   const uint32_t N = cnt[c];
   const uint32_t s = start[c];
@@ -692,12 +800,13 @@ __global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P
     U1 = m1[1] / (double)N;
     U2 = m1[2] / (double)N;
   }
-  double a[26];
+  double a[29];
 #pragma unroll
-  for (int q = 0; q < 26; ++q) a[q] = 0.0;
+  for (int q = 0; q < 29; ++q) a[q] = 0.0;
   for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
     Rec<real> r = P[s + k];
     double p1 = (double)r.u1 - U0, p2 = (double)r.u2 - U1, p3 = (double)r.u3 - U2;
+    a[26] += p1; a[27] += p2; a[28] += p3;
     double e = p1 * p1 + p2 * p2 + p3 * p3;
     a[0] += p1 * p1; a[1] += p1 * p2; a[2] += p1 * p3;
     a[3] += p2 * p2; a[4] += p2 * p3; a[5] += p3 * p3;
@@ -710,7 +819,7 @@ __global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P
     a[22] += p1 * e * e; a[23] += p2 * e * e; a[24] += p3 * e * e;
     a[25] += e;
   }
-  block_sum<26>(a, sh);
+  block_sum<29>(a, sh);
   if (threadIdx.x == 0) {
     double *o = out + (size_t)c * MCPM_NMOM;
     double w = (double)N;
@@ -719,6 +828,7 @@ __global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P
     double DM2 = (N > 0) ? a[25] / w : 0.0;
     o[29] = kp.mass * DM2 / (3.0 * kp.kb);
     o[30] = w;
+    for (int q = 0; q < 3; ++q) o[31 + q] = (N > 0) ? a[26 + q] / w : 0.0;
   }
 }
 
@@ -758,23 +868,28 @@ __global__ __launch_bounds__(256) void k_sample(const Rec<real> *__restrict__ P,
 // ---------------------------------------------------------------------------
 // explicit instantiations
 // ---------------------------------------------------------------------------
+#define MCPM_INST_COLL(real, M)                                                            \
+  template __global__ void k_gen<real, M>(const Rec<real> *, const uint32_t *,             \
+                                          const uint32_t *, const uint32_t *,              \
+                                          const uint32_t *, KParams, uint32_t, uint32_t,   \
+                                          uint32_t, OpArrays, uint32_t *, uint32_t *);     \
+  template __global__ void k_round<real, M>(int, Rec<real> *, OpArrays, const uint32_t *,  \
+                                            uint32_t *, uint32_t *, uint32_t *,            \
+                                            uint32_t *, const double *, const double *,    \
+                                            const double *, double *, KParams, uint32_t,   \
+                                            uint32_t, FPParams, uint8_t *);
+
 #define MCPM_INST(real)                                                                    \
   template __global__ void k_push<real, false>(const Rec<real> *, Rec<real> *, int64_t,    \
-                                               int32_t, KParams, const double *, int,      \
-                                               uint32_t, uint32_t, uint32_t *,             \
-                                               const uint32_t *, uint32_t *);              \
+                                               int32_t, KParams, const double *,           \
+                                               const double *, int, uint32_t, uint32_t,    \
+                                               uint32_t *, const uint32_t *, uint32_t *);  \
   template __global__ void k_push<real, true>(const Rec<real> *, Rec<real> *, int64_t,     \
-                                              int32_t, KParams, const double *, int,       \
-                                              uint32_t, uint32_t, uint32_t *,              \
-                                              const uint32_t *, uint32_t *);               \
-  template __global__ void k_gen<real>(const Rec<real> *, const uint32_t *,                \
-                                       const uint32_t *, const uint32_t *,                 \
-                                       const uint32_t *, KParams, uint32_t, uint32_t,      \
-                                       uint32_t, OpArrays, uint32_t *, uint32_t *);        \
-  template __global__ void k_round<real>(int, Rec<real> *, OpArrays, const uint32_t *,     \
-                                         uint32_t *, uint32_t *, uint32_t *, uint32_t *,   \
-                                         const double *, const double *, const double *,   \
-                                         double *, KParams, uint32_t, uint32_t, uint8_t *); \
+                                              int32_t, KParams, const double *,            \
+                                              const double *, int, uint32_t, uint32_t,     \
+                                              uint32_t *, const uint32_t *, uint32_t *);   \
+  MCPM_INST_COLL(real, COLL_ESMC)                                                           \
+  MCPM_INST_COLL(real, COLL_DSMC)                                                           \
   template __global__ void k_moments<real>(const Rec<real> *, const uint32_t *,            \
                                            const uint32_t *, double *, KParams);           \
   template __global__ void k_sample<real>(const Rec<real> *, const uint32_t *,             \

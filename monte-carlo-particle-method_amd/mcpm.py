@@ -33,7 +33,8 @@ FP64, FP32 = 0, 1
 BC_DIFFUSE, BC_PERIODIC, BC_SPECULAR = 0, 1, 2
 Y_REFERENCE, Y_CARNAHAN_STARLING = 0, 1
 LR_NONE, LR_SCREENED_POISSON = 0, 3
-NMOM, NSAMP = 31, 26
+MODEL_ESMC, MODEL_FP_DENSE, MODEL_FP_IDEAL, MODEL_DSMC_VHS, MODEL_NONE = 0, 1, 2, 3, 4
+NMOM, NSAMP, NCLOS = 34, 26, 28
 
 STATUS = {
     0: "ok", 1: "invalid argument", 2: "HIP runtime error", 3: "capacity exceeded",
@@ -52,7 +53,7 @@ class CParams(ctypes.Structure):
     _fields_ = [
         ("precision", ctypes.c_int32), ("boundary", ctypes.c_int32),
         ("y_model", ctypes.c_int32), ("long_range", ctypes.c_int32),
-        ("n_cells", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+        ("n_cells", ctypes.c_int32), ("model", ctypes.c_int32),
         ("capacity", ctypes.c_int64), ("seed", ctypes.c_uint64),
         ("length", ctypes.c_double), ("wall_lo", ctypes.c_double), ("wall_hi", ctypes.c_double),
         ("T_wall_lo", ctypes.c_double), ("T_wall_hi", ctypes.c_double),
@@ -61,6 +62,8 @@ class CParams(ctypes.Structure):
         ("b", ctypes.c_double), ("Fn", ctypes.c_double), ("cell_volume", ctypes.c_double),
         ("n_ratio", ctypes.c_double), ("delta_t", ctypes.c_double),
         ("ast", ctypes.c_double), ("bst", ctypes.c_double), ("eqdist", ctypes.c_double),
+        ("mu", ctypes.c_double), ("T0", ctypes.c_double), ("visp", ctypes.c_double),
+        ("crref", ctypes.c_double),
     ]
 
 
@@ -84,6 +87,7 @@ class Params:
     boundary: int = BC_DIFFUSE
     y_model: int = Y_REFERENCE
     long_range: int = LR_NONE
+    model: int = MODEL_ESMC
     seed: int = 1
     wall_lo: float = float("nan")
     wall_hi: float = float("nan")
@@ -102,6 +106,10 @@ class Params:
     ast: float = 0.0
     bst: float = 1.0
     eqdist: float = 1.0
+    mu: float = float("nan")
+    T0: float = 1.0
+    visp: float = 0.5
+    crref: float = 1.0
 
     def __post_init__(self):
         if math.isnan(self.b):
@@ -112,6 +120,9 @@ class Params:
             self.wall_hi = self.length - self.sigma - 1e-15         # Main_Functions.cpp:5088
         if math.isnan(self.cell_volume):
             self.cell_volume = self.length / self.n_cells
+        if math.isnan(self.mu):
+            # hard-sphere viscosity 5/(16 sigma^2) sqrt(m kB T0 / pi) (myIO.cpp:2829)
+            self.mu = 5.0 / (16.0 * self.sigma ** 2) * math.sqrt(self.mass * self.kb * self.T0 / math.pi)
 
     @property
     def dy(self) -> float:
@@ -121,7 +132,6 @@ class Params:
         c = CParams()
         for f in fields(self):
             setattr(c, f.name, getattr(self, f.name))
-        c.reserved0 = 0
         return c
 
 
@@ -181,6 +191,10 @@ def load_library() -> ctypes.CDLL:
         "mcpm_timing_enable": (i32, [vp, i32]),
         "mcpm_timing_get": (i32, [vp, i32, dpp, ctypes.POINTER(i64)]),
         "mcpm_status_string": (ctypes.c_char_p, [i32]),
+        "mcpm_fp_closures": (i32, [vp, i64]),
+        "mcpm_velocity_fp": (i32, [vp, i64]),
+        "mcpm_get_closures": (i32, [vp, dpp]),
+        "mcpm_collide_dsmc_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -285,6 +299,23 @@ class Simulation:
 
     def sync(self):
         self._check(self._lib.mcpm_sync(self._h), "mcpm_sync")
+
+    def fp_closures(self, step: int):
+        self._check(self._lib.mcpm_fp_closures(self._h, step), "mcpm_fp_closures")
+
+    def velocity_fp(self, step: int):
+        self._check(self._lib.mcpm_velocity_fp(self._h, step), "mcpm_velocity_fp")
+
+    def closures(self) -> np.ndarray:
+        out = np.empty(self.n_cells * NCLOS)
+        self._check(self._lib.mcpm_get_closures(self._h, _dp(out)), "mcpm_get_closures")
+        return out.reshape(self.n_cells, NCLOS)
+
+    def collide_dsmc_vhs(self, step: int) -> dict:
+        s = CStats()
+        self._check(self._lib.mcpm_collide_dsmc_vhs(self._h, step, ctypes.byref(s)),
+                    "mcpm_collide_dsmc_vhs")
+        return s.as_dict()
 
     def init_omega_max(self, n_gas: float, times_y: bool = True):
         self._check(self._lib.mcpm_init_omega_max(self._h, float(n_gas), 1 if times_y else 0),

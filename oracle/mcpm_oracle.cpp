@@ -84,6 +84,9 @@ struct Oracle {
   std::vector<uint32_t> log_p, log_i, log_j;
   std::vector<uint8_t> log_a;
   int64_t last_cand = 0, last_acc = 0;
+  // Fokker-Planck closures of the last orc_closures call
+  std::vector<double> mom, clos;
+  bool have_closures = false;
 };
 
 template <typename real>
@@ -136,7 +139,28 @@ void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, u
   }
   if (!move) return;
   double v = r[2];
+  const double y_start = y;
+  bool drifted = false;
+  if (p.model == MCPM_MODEL_FP_DENSE && o.have_closures) {
+    // DFP_extra_streaming, slab axis (Fokker_Planck.cpp:932-937)
+    int c = orc_cell(y, o.dy, o.nc);
+    const double *m = &o.mom[(size_t)c * MCPM_NMOM];
+    const double *cl = &o.clos[(size_t)c * MCPM_NCLOS];
+    if (o.cnt[c] > 1) {  // closures exist for cells with two or more particles
+      double e = (r[1] - m[0]) * (r[1] - m[0]) + (r[2] - m[1]) * (r[2] - m[1]) +
+                 (r[3] - m[2]) * (r[3] - m[2]);
+      double q1 = m[10] + m[15] + m[17];  // M3[1] + M3[6] + M3[8]
+      double dt = p.delta_t;
+      y = y + dt * ((r[1] - m[0]) * cl[13] + (r[2] - m[1]) * cl[14] + (r[3] - m[2]) * cl[15] +
+                    cl[20] * (e - 3.0 * p.kb * m[29] / p.mass)) +
+          cl[22] * ((r[2] - m[1]) * e - q1) * dt;
+      y = y - dt * cl[24];
+      drifted = true;
+    }
+  }
   y = y + v * p.delta_t;  // x2 += dt U2 (Fokker_Planck.cpp:963)
+  // speed used for the time spent past a wall: the displacement this step
+  const double v_eff = drifted ? (y - y_start) / p.delta_t : v;
   if (p.boundary == MCPM_BC_PERIODIC) {
     if (y < 0.0) y += p.length;
     else if (y >= p.length) y -= p.length;
@@ -161,7 +185,7 @@ void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, u
   }
   // diffuse walls (BC_thermal_wall_3D, upper/lower wall branches :5044-5127)
   double dt_left = p.delta_t;
-  double v_prev = v;
+  double v_prev = v_eff;
   uint32_t bounce = 0;
   while (y < p.wall_lo || y > p.wall_hi) {
     if (bounce >= 20) {
@@ -466,9 +490,13 @@ void orc_moments(const Oracle &o, double *out) {
     if (N > 0)
       for (int q = 0; q < 3; ++q) U[q] /= (double)N;
     double a[26] = {0};
+    double mp[3] = {0, 0, 0};
     for (int k = 0; k < N; ++k) {
       const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
       double p1 = r[1] - U[0], p2 = r[2] - U[1], p3 = r[3] - U[2];
+      mp[0] += p1;
+      mp[1] += p2;
+      mp[2] += p3;
       double e = p1 * p1 + p2 * p2 + p3 * p3;
       a[0] += p1 * p1; a[1] += p1 * p2; a[2] += p1 * p3;
       a[3] += p2 * p2; a[4] += p2 * p3; a[5] += p3 * p3;
@@ -485,6 +513,389 @@ void orc_moments(const Oracle &o, double *out) {
     for (int q = 0; q < 26; ++q) m[3 + q] = N > 0 ? a[q] / (double)N : 0.0;
     m[29] = o.p.mass * m[28] / (3.0 * o.p.kb);  // T = m DM2 / (3 kB)
     m[30] = (double)N;
+    for (int q = 0; q < 3; ++q) m[31 + q] = N > 0 ? mp[q] / (double)N : 0.0;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Fokker-Planck model
+// ------------------------------------------------------------------------
+// mu(T) = mu0 (T/T0)^visp, scale_mu (Main_Functions.cpp:1970-1973).
+double orc_pow(double x, double a) {
+  if (a == 0.5) return sqrt(x);
+  if (a == 1.0) return x;
+  if (a == 0.0) return 1.0;
+  return mcpm_exp(a * mcpm_log(x));
+}
+
+// Gauss_Jordan (Math_Func.cpp:165-232): forward elimination with scaled
+// partial pivoting (row scale = largest magnitude in the row), then back
+// substitution.  A is n x n row-major and is overwritten, as is rhs.
+void orc_gauss_jordan(double *A, double *rhs, int n, double *x) {
+  for (int i = 0; i < n - 1; ++i) {
+    int best = 0;
+    double bz = -1.0;
+    for (int j = i; j < n; ++j) {
+      const double *row = A + (size_t)j * n;
+      int l = 0;
+      double b = fabs(row[0]);
+      for (int k = 0; k < n; ++k)
+        if (fabs(b) < fabs(row[k])) {
+          b = fabs(row[k]);
+          l = k;
+        }
+      double z = fabs(row[i]) / fabs(row[l]);
+      if (j == i || fabs(bz) < fabs(z)) {
+        if (j == i || z > bz) {
+          bz = z;
+          best = j - i;
+        }
+      }
+    }
+    int k = i + best;
+    if (k != i) {
+      for (int j = 0; j < n; ++j) {
+        double t = A[(size_t)i * n + j];
+        A[(size_t)i * n + j] = A[(size_t)k * n + j];
+        A[(size_t)k * n + j] = t;
+      }
+      double t = rhs[i];
+      rhs[i] = rhs[k];
+      rhs[k] = t;
+    }
+    for (int j = i + 1; j < n; ++j) {
+      double F = (1.0 / A[(size_t)i * n + i]) * A[(size_t)j * n + i];
+      for (int kk = 0; kk < n; ++kk) A[(size_t)j * n + kk] = A[(size_t)j * n + kk] - F * A[(size_t)i * n + kk];
+      rhs[j] = rhs[j] - F * rhs[i];
+    }
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double a = 0.0;
+    for (int j = i + 1; j < n; ++j) a = a + A[(size_t)i * n + j] * x[j];
+    x[i] = (rhs[i] - a) / A[(size_t)i * n + i];
+  }
+}
+
+// Per-cell closures of cell_update and the FP model, written to
+// clos[c * MCPM_NCLOS + k]: 0 Lambda, 1-6 c_ij, 7-9 gamma_i, 10-18 alpha,
+// 19-21 beta, 22 gamma_st, 23-25 phi, 26 div U, 27 tau.
+void orc_closures(Oracle &o, int64_t step) {
+  const mcpm_params &p = o.p;
+  const int nc = o.nc;
+  std::vector<double> mom((size_t)nc * MCPM_NMOM);
+  orc_moments(o, mom.data());
+  o.mom = mom;
+  o.clos.assign((size_t)nc * MCPM_NCLOS, 0.0);
+  const bool dense = p.model == MCPM_MODEL_FP_DENSE;
+  const bool fp = dense || p.model == MCPM_MODEL_FP_IDEAL;
+  for (int c = 0; c < nc; ++c) {
+    const double *m = &mom[(size_t)c * MCPM_NMOM];
+    double *cl = &o.clos[(size_t)c * MCPM_NCLOS];
+    const double *P = m + 3, *M3 = m + 9, *M4 = m + 19, *M5 = m + 25;
+    const double DM2 = m[28], T = m[29], N = m[30];
+    const double n = o.nden[c];
+    if (!(N > 1e-14)) continue;
+    const double Q[3] = {M3[0] + M3[3] + M3[5], M3[1] + M3[6] + M3[8], M3[2] + M3[7] + M3[9]};
+    const double DM4 = M4[0] + M4[3] + M4[5];
+    const double Yn = orc_y(n, p);
+    // gamma_st (Main_Functions.cpp:2602-2605), Lambda (:2718-2728)
+    cl[22] = -0.001 * (n * p.b * Yn / (p.kb * T / p.mass));
+    double mu = p.mu * orc_pow(T / p.T0, p.visp);
+    double tau = 2.0 * mu / (n * p.kb * T);
+    double D3 = DM2 / 3.0;
+    double L = (P[0] - D3) * ((P[3] - D3) * (P[5] - D3) - P[4] * P[4]);
+    L = L - P[1] * (P[1] * (P[5] - D3) - P[4] * P[2]);
+    L = L + P[2] * (P[1] * P[4] - P[2] * (P[3] - D3));
+    double D2 = DM2 * DM2;
+    L = -L / (tau * (D2 * D2));
+    cl[0] = L;
+    cl[27] = tau;
+    if (!fp || o.cnt[c] < 2) continue;
+    // alpha_calculation (Fokker_Planck.cpp:200-587), FP_dense after step 0
+    if (dense && step > 0) {
+      double A[144] = {0}, rhs[12], x[12] = {0};
+      double trace = P[0] + P[3] + P[5];
+      double dev[6] = {P[0], P[1], P[2], P[3], P[4], P[5]};
+      dev[0] -= (1.0 / 3.0) * trace;
+      dev[3] -= (1.0 / 3.0) * trace;
+      dev[5] -= (1.0 / 3.0) * trace;
+      double a = Q[0], b = Q[1], cc = Q[2];
+      double d = DM4 - DM2 * DM2;
+#define AA(i, j) A[(i) * 12 + (j)]
+      AA(0, 0) = P[0]; AA(0, 1) = P[1]; AA(0, 2) = P[2]; AA(0, 9) = a;
+      AA(1, 0) = P[1]; AA(1, 1) = P[3]; AA(1, 2) = P[4]; AA(1, 9) = a;
+      AA(2, 0) = P[2]; AA(2, 1) = P[4]; AA(2, 2) = P[5]; AA(2, 9) = a;
+      AA(3, 3) = P[0]; AA(3, 4) = P[1]; AA(3, 5) = P[2]; AA(3, 10) = b;
+      AA(4, 3) = P[1]; AA(4, 4) = P[3]; AA(4, 5) = P[4]; AA(4, 10) = b;
+      AA(5, 3) = P[2]; AA(5, 4) = P[4]; AA(5, 5) = P[5]; AA(5, 10) = b;
+      AA(6, 6) = P[0]; AA(6, 7) = P[1]; AA(6, 8) = P[2]; AA(6, 11) = cc;
+      AA(7, 6) = P[1]; AA(7, 7) = P[3]; AA(7, 8) = P[4]; AA(7, 11) = cc;
+      AA(8, 6) = P[2]; AA(8, 7) = P[4]; AA(8, 8) = P[5]; AA(8, 11) = cc;
+      AA(9, 0) = a; AA(9, 1) = b; AA(9, 2) = cc; AA(9, 9) = d;
+      AA(10, 3) = a; AA(10, 4) = b; AA(10, 5) = cc; AA(10, 10) = d;
+      AA(11, 6) = a; AA(11, 7) = b; AA(11, 8) = cc; AA(11, 11) = d;
+#undef AA
+      double nbY = n * p.b * Yn;
+      double p_crr = 1.0 + nbY;
+      double mu_crr = 1.0 + 2.0 * nbY / 5.0;
+      double k_crr = 1.0 + 3.0 * nbY / 5.0;
+      double s2 = p.sigma * p.sigma;
+      double w = (4.0 / 9.0) * sqrt(PI) * Yn * (n * n) * (s2 * s2) * sqrt(p.mass * p.kb * T);
+      // div U from the neighbouring cells' mean slab velocity (:459-468)
+      double divU;
+      auto Uy = [&](int cc2) { return mom[(size_t)cc2 * MCPM_NMOM + 1]; };
+      if (p.boundary == MCPM_BC_PERIODIC) {
+        divU = (Uy((c + 1) % nc) - Uy((c + nc - 1) % nc)) / o.dy / 2.0;
+      } else if (c == 0) {
+        divU = (Uy(1 % nc) - Uy(0)) / o.dy;
+      } else if (c == nc - 1) {
+        divU = (Uy(c) - Uy(c - 1)) / o.dy;
+      } else {
+        divU = (Uy(c + 1) - Uy(c - 1)) / o.dy / 2.0;
+      }
+      cl[26] = divU;
+      double cmu = 3.0 * 1.002 / 5.0 * (nbY * nbY) / (1 + 2.0 * nbY / 5.0);
+      double ck = 1.002 * (nbY * nbY) / (1 + 3.0 * nbY / 5.0) / 5.0;
+      double gst = cl[22];
+      double bulk = w * divU / (p.mass * n);
+      rhs[0] = (mu_crr - 1.0) * dev[0] + (p_crr - 1.0) * trace / 3.0 - bulk + cmu * dev[0] - gst * M4[0];
+      rhs[1] = (mu_crr - 1.0) * dev[1] + cmu * dev[1] - gst * M4[1];
+      rhs[2] = (mu_crr - 1.0) * dev[2] + cmu * dev[2] - gst * M4[2];
+      rhs[3] = (mu_crr - 1.0) * dev[1] + cmu * dev[1] - gst * M4[1];
+      rhs[4] = (mu_crr - 1.0) * dev[3] + (p_crr - 1.0) * trace / 3.0 - bulk + cmu * dev[3] - gst * M4[3];
+      rhs[5] = (mu_crr - 1.0) * dev[4] + cmu * dev[4] - gst * M4[4];
+      rhs[6] = (mu_crr - 1.0) * dev[2] + cmu * dev[2] - gst * M4[2];
+      rhs[7] = (mu_crr - 1.0) * dev[4] + cmu * dev[4] - gst * M4[4];
+      rhs[8] = (mu_crr - 1.0) * dev[5] + (p_crr - 1.0) * trace / 3.0 - bulk + cmu * dev[5] - gst * M4[5];
+      rhs[9] = (k_crr - 1.0) * a + ck * a - gst * M5[0] + gst * (DM2 * Q[0]);
+      rhs[10] = (k_crr - 1.0) * b + ck * b - gst * M5[1] + gst * (DM2 * Q[1]);
+      rhs[11] = (k_crr - 1.0) * cc + ck * cc - gst * M5[2] + gst * (DM2 * Q[2]);
+      orc_gauss_jordan(A, rhs, 12, x);
+      for (int q = 0; q < 9; ++q) cl[10 + q] = x[q];
+      for (int q = 0; q < 3; ++q) cl[19 + q] = x[9 + q];
+      if (x[0] != x[0]) o.err |= 2u;  // Fokker_Planck.cpp:557-571
+    }
+    // c_gamma_calculation (Fokker_Planck.cpp:590-804)
+    {
+      double A[81] = {0}, rhs[9], x[9] = {0};
+#define AB(i, j) A[(i) * 9 + (j)]
+      AB(0, 0) = 2.0 * P[0]; AB(0, 1) = 2.0 * P[1]; AB(0, 2) = 2.0 * P[2]; AB(0, 6) = 2.0 * Q[0];
+      AB(1, 0) = P[1]; AB(1, 1) = P[0] + P[3]; AB(1, 2) = P[4]; AB(1, 3) = P[1]; AB(1, 4) = P[2];
+      AB(1, 6) = Q[1]; AB(1, 7) = Q[0];
+      AB(2, 0) = P[2]; AB(2, 1) = P[4]; AB(2, 2) = P[0] + P[5]; AB(2, 4) = P[1]; AB(2, 5) = P[2];
+      AB(2, 6) = Q[2]; AB(2, 8) = Q[0];
+      AB(3, 1) = 2.0 * P[1]; AB(3, 3) = 2.0 * P[3]; AB(3, 4) = 2.0 * P[4]; AB(3, 7) = 2.0 * Q[1];
+      AB(4, 1) = P[2]; AB(4, 2) = P[1]; AB(4, 3) = P[4]; AB(4, 4) = P[3] + P[5]; AB(4, 5) = P[4];
+      AB(4, 7) = Q[2]; AB(4, 8) = Q[1];
+      AB(5, 2) = 2.0 * P[2]; AB(5, 4) = 2.0 * P[4]; AB(5, 5) = 2.0 * P[5]; AB(5, 8) = 2.0 * Q[2];
+      AB(6, 0) = Q[0] + 2.0 * M3[0]; AB(6, 1) = Q[1] + 4.0 * M3[1]; AB(6, 2) = Q[2] + 4.0 * M3[2];
+      AB(6, 3) = 2.0 * M3[3]; AB(6, 4) = 4.0 * M3[4]; AB(6, 5) = 2.0 * M3[5];
+      AB(6, 6) = 2.0 * M4[0] - 2.0 * P[0] * DM2; AB(6, 7) = 2.0 * M4[1] - 2.0 * P[1] * DM2;
+      AB(6, 8) = 2.0 * M4[2] - 2.0 * P[2] * DM2;
+      AB(7, 0) = 2.0 * M3[1]; AB(7, 1) = Q[0] + 4.0 * M3[3]; AB(7, 2) = 4.0 * M3[4];
+      AB(7, 3) = Q[1] + 2.0 * M3[6]; AB(7, 4) = Q[2] + 4.0 * M3[7]; AB(7, 5) = 2.0 * M3[8];
+      AB(7, 6) = 2.0 * M4[1] - 2.0 * P[1] * DM2; AB(7, 7) = 2.0 * M4[3] - 2.0 * P[3] * DM2;
+      AB(7, 8) = 2.0 * M4[4] - 2.0 * P[4] * DM2;
+      AB(8, 0) = 2.0 * M3[2]; AB(8, 1) = 4.0 * M3[4]; AB(8, 2) = Q[0] + 4.0 * M3[5];
+      AB(8, 3) = 2.0 * M3[7]; AB(8, 4) = Q[1] + 4.0 * M3[8]; AB(8, 5) = Q[2] + 2.0 * M3[9];
+      AB(8, 6) = 2.0 * M4[2] - 2.0 * P[2] * DM2; AB(8, 7) = 2.0 * M4[4] - 2.0 * P[4] * DM2;
+      AB(8, 8) = 2.0 * M4[5] - 2.0 * P[5] * DM2;
+      AB(6, 6) += DM4 - DM2 * DM2;
+      AB(7, 7) += DM4 - DM2 * DM2;
+      AB(8, 8) += DM4 - DM2 * DM2;
+#undef AB
+      const double Pr = 2.0 / 3.0;
+      const double Yc = dense ? Yn : 1.0;
+      for (int q = 0; q < 6; ++q) rhs[q] = -2.0 * L * M4[q];
+      for (int q = 0; q < 3; ++q) rhs[6 + q] = Yc * (3.0 / tau - 2.0 * Pr / tau) * Q[q];
+      rhs[6] += L * (-3.0 * M5[0] + DM2 * Q[0] + 2.0 * (P[0] * Q[0] + P[1] * Q[1] + P[2] * Q[2]));
+      rhs[7] += L * (-3.0 * M5[1] + DM2 * Q[1] + 2.0 * (P[1] * Q[0] + P[3] * Q[1] + P[4] * Q[2]));
+      rhs[8] += L * (-3.0 * M5[2] + DM2 * Q[2] + 2.0 * (P[2] * Q[0] + P[4] * Q[1] + P[5] * Q[2]));
+      orc_gauss_jordan(A, rhs, 9, x);
+      for (int q = 0; q < 6; ++q) cl[1 + q] = x[q];
+      for (int q = 0; q < 3; ++q) cl[7 + q] = x[6 + q];
+    }
+    // phi = alpha . Mp (Main_Functions.cpp:2734-2746)
+    const double *al = cl + 10, *mp = m + 31;
+    cl[23] = al[0] * mp[0] + al[1] * mp[1] + al[2] * mp[2];
+    cl[24] = al[3] * mp[0] + al[4] * mp[1] + al[5] * mp[2];
+    cl[25] = al[6] * mp[0] + al[7] * mp[1] + al[8] * mp[2];
+  }
+  o.have_closures = true;
+}
+
+// random_generator (Fokker_Planck.cpp:20-145) with Philox normals: per cell
+// subtract the mean, divide by the rms (cells with more than one particle).
+void orc_noise(const Oracle &o, int64_t step, std::vector<double> &xi) {
+  const mcpm_params &p = o.p;
+  uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
+  xi.assign(3 * (size_t)o.n, 0.0);
+  for (int64_t k = 0; k < o.n; ++k) {
+    mcpm_u32x4 w = mcpm_philox((uint32_t)k, 0u, slo,
+                               mcpm_ctr3(MCPM_TAG_FPNOISE, shi, (uint32_t)((uint64_t)k >> 32)),
+                               key0, key1);
+    double z0, z1, z2, z3;
+    mcpm_normal2(w.v[0], w.v[1], &z0, &z1);
+    mcpm_normal2(w.v[2], w.v[3], &z2, &z3);
+    xi[3 * k] = z0;
+    xi[3 * k + 1] = z1;
+    xi[3 * k + 2] = z2;
+  }
+  for (int c = 0; c < o.nc; ++c) {
+    int N = o.cnt[c];
+    if (N < 2) continue;
+    double s[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+    for (int k = 0; k < N; ++k)
+      for (int q = 0; q < 3; ++q) s[q] += xi[3 * (size_t)(o.start[c] + k) + q];
+    for (int q = 0; q < 3; ++q) s[q] = s[q] / (1.0 * N);
+    for (int k = 0; k < N; ++k)
+      for (int q = 0; q < 3; ++q) {
+        double v = xi[3 * (size_t)(o.start[c] + k) + q] - s[q];
+        xi[3 * (size_t)(o.start[c] + k) + q] = v;
+        s2[q] += v * v;
+      }
+    for (int q = 0; q < 3; ++q) s2[q] = sqrt(s2[q] / (1.0 * N));
+    for (int k = 0; k < N; ++k)
+      for (int q = 0; q < 3; ++q) xi[3 * (size_t)(o.start[c] + k) + q] /= s2[q];
+  }
+}
+
+// Velocity_FP_parallel (Fokker_Planck.cpp:1065-1182).
+template <typename real>
+void orc_fp_velocity(Oracle &o, int64_t step) {
+  const mcpm_params &p = o.p;
+  const bool dense = p.model == MCPM_MODEL_FP_DENSE;
+  std::vector<double> xi;
+  orc_noise(o, step, xi);
+  for (int c = 0; c < o.nc; ++c) {
+    const int N = o.cnt[c];
+    if (N == 0) continue;
+    const double *m = &o.mom[(size_t)c * MCPM_NMOM];
+    const double *cl = &o.clos[(size_t)c * MCPM_NCLOS];
+    const double U[3] = {m[0], m[1], m[2]};
+    const double T = m[29];
+    if (N < 2) {
+      for (int k = 0; k < N; ++k) {
+        double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+        r[1] = rnd<real>(U[0]);
+        r[2] = rnd<real>(U[1]);
+        r[3] = rnd<real>(U[2]);
+      }
+      continue;
+    }
+    double tau = cl[27];
+    if (dense) tau = tau / o.yc[c];
+    // exp through the shared deterministic routine, as on the GPU
+    const double A1 = mcpm_exp(-p.delta_t / tau);
+    const double A3 = sqrt((1.0 - mcpm_exp(-2.0 * p.delta_t / tau)) * p.kb * T / (1.0 * p.mass));
+    const double A2 = (1.0 - mcpm_exp(-p.delta_t / tau)) * tau;
+    const double L = cl[0];
+    const double *cc = cl + 1, *g = cl + 7;
+    const double Q[3] = {m[9] + m[12] + m[14], m[10] + m[15] + m[17], m[11] + m[16] + m[18]};
+    const double e0 = 3.0 * p.kb * T / p.mass;
+    double frac1 = 0.0, frac2 = 0.0;
+    std::vector<double> mp(3 * (size_t)N);
+    for (int k = 0; k < N; ++k) {
+      const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      double a = r[1] - U[0], b = r[2] - U[1], d = r[3] - U[2];
+      frac1 += a * a + b * b + d * d;
+      double e = a * a + b * b + d * d;
+      double d0 = cc[0] * a + cc[1] * b + cc[2] * d;
+      d0 += g[0] * (e - e0);
+      d0 += L * (a * e - Q[0]);
+      double d1 = cc[1] * a + cc[3] * b + cc[4] * d;
+      d1 += g[1] * (e - e0);
+      d1 += L * (b * e - Q[1]);
+      double d2 = cc[2] * a + cc[4] * b + cc[5] * d;
+      d2 += g[2] * (e - e0);
+      d2 += L * (d * e - Q[2]);
+      const double *x = &xi[3 * (size_t)(o.start[c] + k)];
+      double n0 = A1 * a + A2 * d0 + A3 * x[0];
+      double n1 = A1 * b + A2 * d1 + A3 * x[1];
+      double n2 = A1 * d + A2 * d2 + A3 * x[2];
+      if (n1 != n1) o.err |= 2u;  // Fokker_Planck.cpp:1137-1149
+      mp[3 * k] = n0;
+      mp[3 * k + 1] = n1;
+      mp[3 * k + 2] = n2;
+      frac2 += n0 * n0 + n1 * n1 + n2 * n2;
+    }
+    double s = sqrt(fabs(frac1 / frac2));
+    for (int k = 0; k < N; ++k) {
+      double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      r[1] = rnd<real>(mp[3 * k] * s + U[0]);
+      r[2] = rnd<real>(mp[3 * k + 1] * s + U[1]);
+      r[3] = rnd<real>(mp[3 * k + 2] * s + U[2]);
+    }
+  }
+}
+
+// Collision_DSMC_VHS (collisions.cpp:2063-2141): pairs within a cell,
+// candidates in index order, VHS cross-section, isotropic scattering;
+// crm <- max(1.1 crm_observed, 0.9 crm) per cell.
+template <typename real>
+void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st) {
+  const mcpm_params &p = o.p;
+  uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
+  int64_t M = 0, valid = 0, acc = 0;
+  o.log_p.clear();
+  o.log_i.clear();
+  o.log_j.clear();
+  o.log_a.clear();
+  uint32_t kglob = 0;
+  for (int c = 0; c < o.nc; ++c) {
+    const int Nc = o.cnt[c];
+    if (Nc < 2) continue;
+    double Mc = PI * p.sigma * p.sigma * ((double)Nc * o.nden[c]) * o.omax[c] * p.delta_t / 2.0;
+    mcpm_u32x4 w0 = mcpm_philox((uint32_t)c, 0u, slo, mcpm_ctr3(MCPM_TAG_MCAND, shi, 0u), key0, key1);
+    double mf = mcpm_floor(Mc + mcpm_u01(w0.v[0]));
+    uint32_t m = mf > 0.0 ? (uint32_t)mf : 0u;
+    M += m;
+    double crm = 0.0;
+    for (uint32_t t = 0; t < m; ++t) {
+      mcpm_u32x4 w = mcpm_philox(t, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_DSMC, shi, 0u), key0, key1);
+      uint32_t li = mcpm_uint_below(w.v[0], (uint32_t)Nc);
+      uint32_t lj = mcpm_uint_below(w.v[1], (uint32_t)(Nc - 1));
+      if (lj >= li) lj += 1;
+      uint32_t i = (uint32_t)o.start[c] + li, j = (uint32_t)o.start[c] + lj;
+      double *ri = &o.rec[4 * (size_t)i], *rj = &o.rec[4 * (size_t)j];
+      double g0 = ri[1] - rj[1], g1 = ri[2] - rj[2], g2 = ri[3] - rj[3];
+      double Vr = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+      double sig = orc_pow(Vr, 2.0 - 2.0 * p.visp) * orc_pow(p.crref, 2.0 * p.visp - 1.0);
+      if (sig - crm > 0.0) crm = sig;
+      bool a = sig / o.omax[c] > mcpm_u01(w.v[2]);
+      if (a) {
+        double ct = 2.0 * mcpm_u01(w.v[3]) - 1.0;
+        double stt = sqrt(1.0 - ct * ct);
+        mcpm_u32x4 w2 = mcpm_philox(t, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_DSMC, shi, 1u), key0, key1);
+        double sp, cp;
+        mcpm_sincos2pi(mcpm_u01(w2.v[0]), &sp, &cp);
+        double v0 = Vr * ct, v1 = Vr * stt * cp, v2 = Vr * stt * sp;
+        double c0 = 0.5 * (ri[1] + rj[1]), c1 = 0.5 * (ri[2] + rj[2]), c2 = 0.5 * (ri[3] + rj[3]);
+        ri[1] = rnd<real>(c0 + 0.5 * v0);
+        ri[2] = rnd<real>(c1 + 0.5 * v1);
+        ri[3] = rnd<real>(c2 + 0.5 * v2);
+        rj[1] = rnd<real>(c0 - 0.5 * v0);
+        rj[2] = rnd<real>(c1 - 0.5 * v1);
+        rj[3] = rnd<real>(c2 - 0.5 * v2);
+        ++acc;
+      }
+      ++valid;
+      o.log_p.push_back(kglob + t);
+      o.log_i.push_back(i);
+      o.log_j.push_back(j);
+      o.log_a.push_back(a ? 1 : 0);
+    }
+    kglob += m;
+    o.omax[c] = std::max(1.1 * crm, 0.9 * o.omax[c]);
+  }
+  if (st) {
+    st->candidates = M;
+    st->valid = valid;
+    st->accepted = acc;
+    st->rounds = 0;
+    st->reserved0 = 0;
   }
 }
 
@@ -612,12 +1023,57 @@ int orc_stream_bc(void *h, int64_t step, int32_t with_kick) {
   return take_err(o);
 }
 
+int orc_fp_closures(void *h, int64_t step) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  orc_closures(o, step);
+  return take_err(o);
+}
+
+int orc_velocity_fp(void *h, int64_t step) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.have_closures) return MCPM_ERR_STATE;
+  if (o.p.precision == MCPM_FP32) orc_fp_velocity<float>(o, step);
+  else orc_fp_velocity<double>(o, step);
+  return take_err(o);
+}
+
+int orc_get_closures(void *h, double *out) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.have_closures) return MCPM_ERR_STATE;
+  memcpy(out, o.clos.data(), o.clos.size() * sizeof(double));
+  return MCPM_OK;
+}
+
+int orc_collide_dsmc_vhs(void *h, int64_t step, mcpm_collision_stats *st) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  if (o.p.precision == MCPM_FP32) orc_dsmc<float>(o, step, st);
+  else orc_dsmc<double>(o, step, st);
+  return MCPM_OK;
+}
+
+// velocity_update dispatch (Main_Functions.cpp:1545-1600) + long range +
+// stream and boundary, in the order of main.cpp:389-669.
 int orc_step(void *h, int64_t step, mcpm_collision_stats *st) {
   Oracle &o = *(Oracle *)h;
-  int r = orc_collide_esmc(h, step, st);
+  int r = MCPM_OK;
+  if (st) memset(st, 0, sizeof *st);
+  switch (o.p.model) {
+    case MCPM_MODEL_ESMC: r = orc_collide_esmc(h, step, st); break;
+    case MCPM_MODEL_FP_DENSE:
+    case MCPM_MODEL_FP_IDEAL:
+      r = orc_fp_closures(h, step);
+      if (!r) r = orc_velocity_fp(h, step);
+      break;
+    case MCPM_MODEL_DSMC_VHS: r = orc_collide_dsmc_vhs(h, step, st); break;
+    default: break;
+  }
   if (r) return r;
   if (o.p.long_range == MCPM_LR_SCREENED_POISSON) orc_field(o);
-  return orc_stream_bc(h, step, 1);
+  r = orc_stream_bc(h, step, 1);
+  o.have_closures = false;
+  return r;
 }
 
 int orc_moments(void *h, double *out) {
@@ -696,6 +1152,11 @@ void orc_det_sincos2pi(double u, double *s, double *c) { mcpm_sincos2pi(u, s, c)
 uint32_t orc_priority(uint32_t k, uint32_t key) { return mcpm_priority(k, key); }
 double orc_contact_y(const mcpm_params *p, double n) { return orc_y(n, *p); }
 void orc_gl50(double *x, double *w) { gl50(x, w); }
+void orc_gauss_jordan(int n, const double *A, const double *rhs, double *x) {
+  std::vector<double> a(A, A + (size_t)n * n), r(rhs, rhs + n);
+  for (int k = 0; k < n; ++k) x[k] = 0.0;
+  orc_gauss_jordan(a.data(), r.data(), n, x);
+}
 void orc_thomas(int n, const double *a, const double *b, const double *c, const double *rhs,
                 double *x) {
   std::vector<double> A(a, a + n), B(b, b + n), C(c, c + n), R(rhs, rhs + n), X;

@@ -268,3 +268,95 @@ def test_moments_and_samples_match_numpy():
         np.testing.assert_allclose(s[c, 1:4], U.sum(axis=1), rtol=1e-12)
         np.testing.assert_allclose(s[c, 10], (U[2] ** 3).sum(), rtol=1e-12)   # (0,0,3)
         np.testing.assert_allclose(s[c, 19], (U[0] ** 3).sum(), rtol=1e-12)   # (3,0,0)
+
+
+# ---------------------------------------------------------------- Fokker-Planck
+def test_gauss_jordan_matches_numpy_with_pivoting():
+    rng = np.random.default_rng(31)
+    for n in (9, 12):
+        for _ in range(20):
+            A = rng.normal(size=(n, n))
+            A[0, 0] = 1e-12  # forces a row exchange at the first column
+            rhs = rng.normal(size=n)
+            np.testing.assert_allclose(orc.gauss_jordan(A, rhs), np.linalg.solve(A, rhs),
+                                       rtol=1e-9, atol=1e-10)
+
+
+def _fp_case(model, seed=41, **kw):
+    p, arrs, meta = W.heat_conduction(seed=seed, **kw)
+    p.model = model
+    return p, arrs, meta
+
+
+@pytest.mark.parametrize("model", [mcpm.MODEL_FP_DENSE, mcpm.MODEL_FP_IDEAL])
+def test_fp_velocity_conserves_cell_momentum_and_energy(model):
+    p, arrs, meta = _fp_case(model)
+    o = _sim(p, arrs)
+    for step in (0, 3):   # step 0: no alpha; step > 0: alpha_calculation active
+        m0 = o.moments()
+        o.fp_closures(step)
+        o.velocity_fp(step)
+        m1 = o.moments()
+        sel = m0[:, 30] > 1
+        np.testing.assert_allclose(m1[sel, :3], m0[sel, :3], rtol=0, atol=1e-12)
+        np.testing.assert_allclose(m1[sel, 28], m0[sel, 28], rtol=1e-12)
+        # the velocities did change
+        assert np.abs(m1[sel, 3] - m0[sel, 3]).max() > 1e-3
+
+
+def test_alpha_closure_equilibrium_value():
+    """Uniform Maxwellian gas at rest: the alpha system reduces to
+    alpha_ii = n b Y (1 + 0.005) (pressure correction p_crr - 1 = n b Y plus the
+    gamma_st M4 term with M4_ii = 5 (kT/m)^2), off-diagonals and beta ~ 0."""
+    n0, T = 0.3, 1.0
+    p, arrs, meta = W.interface(N=400_000, n_cells=4, length=4.0, n_liq=n0, n_vap=n0, T=T,
+                                dt=0.01, precision=mcpm.FP64, seed=43)
+    p.long_range = mcpm.LR_NONE
+    p.model = mcpm.MODEL_FP_DENSE
+    o = _sim(p, arrs)
+    o.fp_closures(1)
+    cl = o.closures()
+    nbY = n0 * p.b * W.contact_y_ref(n0)
+    for c in range(4):
+        alpha = cl[c, 10:19].reshape(3, 3)
+        np.testing.assert_allclose(np.diag(alpha), 1.005 * nbY, rtol=0.03)
+        assert np.abs(alpha - np.diag(np.diag(alpha))).max() < 0.05 * nbY
+
+
+def test_fp_isothermal_gas_stays_at_wall_temperature():
+    p, arrs, meta = W.heat_conduction(n_cells=50, ppc=400, length=5.0, T0=1.2, T_lo=1.2,
+                                      T_hi=1.2, seed=44)
+    p.model = mcpm.MODEL_FP_IDEAL
+    o = _sim(p, arrs)
+    for s in range(30):
+        o.step(s)
+    m = o.moments()
+    T = (m[:, 29] * m[:, 30]).sum() / m[:, 30].sum()
+    assert T == pytest.approx(1.2, rel=0.03)
+
+
+# ---------------------------------------------------------------- plain DSMC
+def test_dsmc_vhs_conserves_and_relaxes_anisotropy():
+    """Hard-sphere DSMC (visp = 1/2): momentum and energy conserved per
+    collision; an anisotropic temperature relaxes towards isotropy."""
+    p, arrs, meta = W.interface(N=100_000, n_cells=50, length=10.0, n_liq=0.1, n_vap=0.1, T=1.0,
+                                dt=0.05, precision=mcpm.FP64, seed=45)
+    p.long_range = mcpm.LR_NONE
+    p.model = mcpm.MODEL_DSMC_VHS
+    y, u1, u2, u3 = arrs
+    u1 = u1 * 1.5
+    o = _sim(p, (y, u1, u2, u3))
+    o.init_omega_max(meta["n_gas"], times_y=False)
+    _, a1, a2, a3 = o.download()
+    P0 = np.array([a1.sum(), a2.sum(), a3.sum()])
+    E0 = (a1 ** 2 + a2 ** 2 + a3 ** 2).sum()
+    an0 = (a1 ** 2).mean() - (a2 ** 2).mean()
+    tot = 0
+    for s in range(10):
+        tot += o.collide_dsmc_vhs(s)["accepted"]
+    _, a1, a2, a3 = o.download()
+    assert tot > 1000
+    assert np.abs(np.array([a1.sum(), a2.sum(), a3.sum()]) - P0).max() < 1e-9
+    assert abs((a1 ** 2 + a2 ** 2 + a3 ** 2).sum() - E0) < 1e-10 * E0
+    an1 = (a1 ** 2).mean() - (a2 ** 2).mean()
+    assert an1 < 0.8 * an0
