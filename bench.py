@@ -41,7 +41,7 @@ def main():
     ap.add_argument("--cells", type=int, default=4096)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--cpu-particles", type=int, default=2_000_000)
+    ap.add_argument("--cpu-particles", type=int, default=20_000_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "

@@ -49,8 +49,15 @@ enum mcpm_boundary { MCPM_BC_DIFFUSE = 0, MCPM_BC_PERIODIC = 1, MCPM_BC_SPECULAR
    (Main_Functions.cpp:1906-1915) or Carnahan-Starling (the commented
    alternative at Main_Functions.cpp:1912). */
 enum mcpm_ymodel { MCPM_Y_REFERENCE = 0, MCPM_Y_CARNAHAN_STARLING = 1 };
-/* Long-range attraction switch of velocity_update (Main_Functions.cpp:1612-1668). */
-enum mcpm_long_range { MCPM_LR_NONE = 0, MCPM_LR_SCREENED_POISSON = 3 };
+/* Long-range attraction switch of velocity_update (Main_Functions.cpp:1612-1668):
+   1 direct Sutherland integral (vlasov_taking_the_integral), 2 density
+   expansion (KDE gradient), 3 screened-Poisson (poisson_solver). */
+enum mcpm_long_range {
+  MCPM_LR_NONE = 0,
+  MCPM_LR_DIRECT = 1,
+  MCPM_LR_DENSITY_EXPANSION = 2,
+  MCPM_LR_SCREENED_POISSON = 3
+};
 /* Velocity model of velocity_update (gas.model, Main_Functions.cpp:1545-1600):
    ESMC -> Collision_ESMC_new3, FP_DENSE / FP_IDEAL -> Velocity_FP_parallel
    with or without the dense-gas Y(n) and closures, DSMC_VHS ->
@@ -98,6 +105,8 @@ typedef struct mcpm_params {
   double   T0;           /* gas.T0: reference temperature of mu(T)              */
   double   visp;         /* gas.visp: mu ~ T^visp (scale_mu, Main_Functions.cpp:1970) */
   double   crref;        /* gas.crref: VHS reference relative speed             */
+  double   phi;          /* gas.phi: attraction strength of the direct and
+                            density-expansion forces (myIO.cpp:2915-2920)     */
 } mcpm_params;
 
 typedef struct mcpm_ctx mcpm_ctx;
@@ -146,6 +155,24 @@ int mcpm_collide_esmc(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
    solves for the per-cell force F2; the kick is applied by mcpm_stream_bc
    (or mcpm_kick) in the same pass that moves the particles. */
 int mcpm_poisson_solver(mcpm_ctx *ctx);
+
+/* vlasov_taking_the_integral (cpp_headers.h:628; Main_Functions.cpp:
+   3582-3942): face potential V_f = -sum over r in [eqdist, 3 eqdist] and
+   polar angle (50 x 50 Gauss-Legendre) of (sigma/r)^6 r^2 2 pi n sin(phi),
+   F2 = -(V_{c+1} - V_c) / dy * phi / m.  Like mcpm_poisson_solver it sets F2
+   and leaves the kick to the push. */
+int mcpm_vlasov_integral(mcpm_ctx *ctx);
+
+/* Density-expansion attraction (long_range == 2, Main_Functions.cpp:
+   1632-1653) from the Gaussian kernel-density gradient of cell_update
+   (Main_Functions.cpp:2378-2427): F2 = -dn2 4 pi (2^(1/6) sigma)^3 2 phi / m
+   in every cell but the two wall cells.  Sets F2. */
+int mcpm_density_expansion(mcpm_ctx *ctx);
+
+/* The kernel density estimate of cell_update (Main_Functions.cpp:2378-2427):
+   n_smooth, dn2, dn3 per cell (h = dy/2, cut-off 2 dy).  Any pointer may be
+   NULL; computes them if no call has since the last sort. */
+int mcpm_get_kde(mcpm_ctx *ctx, double *n_smooth, double *dn2, double *dn3);
 
 /* The velocity kick of poisson_solver (Main_Functions.cpp:3574-3577) alone,
    U2 += F2[cell] dt, for callers that want the reference's operator split. */

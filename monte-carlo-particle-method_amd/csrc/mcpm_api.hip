@@ -53,6 +53,9 @@ struct mcpm_ctx {
   FPParams fpp{};
   double *clos = nullptr, *drift = nullptr;
   bool have_closures = false;
+  // direct and density-expansion long range
+  double *lr_tab = nullptr, *lr_V = nullptr;
+  double *kde = nullptr;  // n_smooth, dn2, dn3
   // field
   FieldArrays fa{};
   double *field_buf = nullptr;
@@ -203,7 +206,7 @@ int build_field(mcpm_ctx *c) {
     den = 1.0 + zz[0] + vlast * zz[nf - 1];
   }
   const size_t nfa = (size_t)nf + 1;
-  int st = dalloc(&c->field_buf, nfa * 9 + 200);
+  int st = dalloc(&c->field_buf, nfa * 9 + 250);
   if (st) return st;
   double *b = c->field_buf;
   FieldArrays &fa = c->fa;
@@ -218,7 +221,8 @@ int build_field(mcpm_ctx *c) {
   fa.sm_z = b; b += nfa;
   fa.gl_r = b; b += 50;
   fa.gl_dr = b; b += 50;
-  fa.gl_phi = b; b += 50;
+  fa.gl_cos = b; b += 50;
+  fa.gl_sin = b; b += 50;
   fa.gl_dphi = b; b += 50;
   fa.sm_vlast = vlast;
   fa.sm_den = den;
@@ -226,7 +230,7 @@ int build_field(mcpm_ctx *c) {
   HIPCHK(hipMemcpy(fa.invd, invd.data(), nf * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(fa.mU, mU.data(), nf * sizeof(double), hipMemcpyHostToDevice));
   if (periodic) HIPCHK(hipMemcpy(fa.sm_z, z.data(), nf * sizeof(double), hipMemcpyHostToDevice));
-  double xi[50], wi[50], r[50], dr[50], ph[50], dph[50];
+  double xi[50], wi[50], r[50], dr[50], ph[50], dph[50], cph[50], sph[50];
   gauss_legendre_8dec(50, xi, wi);
   const double bb = 3.0 * c->p.eqdist, aa = 0.0, cc = 0.0, dd = PI;
   for (int i = 0; i < 50; ++i) {
@@ -234,11 +238,40 @@ int build_field(mcpm_ctx *c) {
     dr[i] = wi[i] * (bb - aa) / 2.0;
     ph[i] = (dd - cc) / 2.0 * xi[i] + (cc + dd) / 2.0;
     dph[i] = wi[i] * (dd - cc) / 2.0;
+    cph[i] = cos(ph[i]);
+    sph[i] = sin(ph[i]);
   }
   HIPCHK(hipMemcpy(fa.gl_r, r, sizeof r, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(fa.gl_dr, dr, sizeof dr, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(fa.gl_phi, ph, sizeof ph, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa.gl_cos, cph, sizeof cph, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa.gl_sin, sph, sizeof sph, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(fa.gl_dphi, dph, sizeof dph, hipMemcpyHostToDevice));
+  return MCPM_OK;
+}
+
+// Quadrature table of the direct Sutherland integral
+// (vlasov_taking_the_integral, Main_Functions.cpp:3705-3729): r in
+// [eqdist, 3 eqdist], polar angle in [0, pi], 8-decimal 50-point rule.
+int build_vlasov_table(mcpm_ctx *c) {
+  double xi[50], wi[50], tab[250];
+  gauss_legendre_8dec(50, xi, wi);
+  const double b = 3.0 * c->p.eqdist, a = 1.0 * c->p.eqdist, cc = 0.0, d = PI;
+  for (int i = 0; i < 50; ++i) {
+    double rr = (b - a) / 2.0 * xi[i] + (a + b) / 2.0;
+    double dr = wi[i] * (b - a) / 2.0;
+    double ph = (d - cc) / 2.0 * xi[i] + (cc + d) / 2.0;
+    double q = c->p.sigma / rr;
+    double q2 = q * q;
+    tab[i] = rr;
+    tab[50 + i] = -(-(q2 * q2 * q2)) * rr * rr * (2.0 * PI) * dr;
+    tab[100 + i] = cos(ph);
+    tab[150 + i] = sin(ph);
+    tab[200 + i] = wi[i] * (d - cc) / 2.0;
+  }
+  int st = dalloc(&c->lr_tab, 250 + (size_t)c->nc + 1);
+  if (st) return st;
+  c->lr_V = c->lr_tab + 250;
+  HIPCHK(hipMemcpy(c->lr_tab, tab, sizeof tab, hipMemcpyHostToDevice));
   return MCPM_OK;
 }
 
@@ -314,9 +347,9 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   if (m <= c->op_cap) return MCPM_OK;
   int64_t cap = m + m / 4 + 1024;
   HIPCHK(hipStreamSynchronize(c->s));
-  hipFree(c->ops.i);
-  hipFree(c->list[0]);
-  hipFree(c->log_acc);
+  // one allocation holds the five op arrays and both lists
+  if (c->ops.i) HIPCHK(hipFree(c->ops.i));
+  if (c->log_acc) HIPCHK(hipFree(c->log_acc));
   c->ops = OpArrays{};
   c->list[0] = c->list[1] = nullptr;
   c->log_acc = nullptr;
@@ -438,7 +471,7 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
   if (p->boundary < 0 || p->boundary > 2) return MCPM_ERR_ARG;
   if (!(p->length > 0.0) || !(p->delta_t > 0.0) || !(p->cell_volume > 0.0)) return MCPM_ERR_ARG;
   if (p->long_range == MCPM_LR_SCREENED_POISSON && !(p->bst > 0.0)) return MCPM_ERR_ARG;
-  if (p->long_range != MCPM_LR_NONE && p->long_range != MCPM_LR_SCREENED_POISSON)
+  if (p->long_range < MCPM_LR_NONE || p->long_range > MCPM_LR_SCREENED_POISSON)
     return MCPM_ERR_ARG;
   if (p->model < MCPM_MODEL_ESMC || p->model > MCPM_MODEL_NONE) return MCPM_ERR_ARG;
   int ndev = 0;
@@ -522,6 +555,8 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
     return fail(MCPM_ERR_HIP);
   if (p->long_range == MCPM_LR_SCREENED_POISSON)
     if ((st = build_field(c))) return fail(st);
+  if ((st = build_vlasov_table(c))) return fail(st);
+  if ((st = dalloc(&c->kde, (size_t)nc * 3))) return fail(st);
   if (hipDeviceSynchronize() != hipSuccess) return fail(MCPM_ERR_HIP);
   *out = c;
   return MCPM_OK;
@@ -533,7 +568,7 @@ int mcpm_destroy(mcpm_ctx *c) {
   void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
                   c->samples, c->counters, c->err, c->res[0], c->res[1], c->ops.i,
-                  c->log_acc, c->field_buf, c->clos, c->drift};
+                  c->log_acc, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < T_NCLASS; ++k)
@@ -692,6 +727,61 @@ int mcpm_poisson_solver(mcpm_ctx *c) {
   return MCPM_OK;
 }
 
+int mcpm_vlasov_integral(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  Timer tm(c, T_FIELD);
+  const int nf = c->p.boundary == MCPM_BC_PERIODIC ? c->nc : c->nc + 1;
+  hipLaunchKernelGGL(k_vlasov_faces, dim3(nf), dim3(256), 0, c->s, c->nden, c->lr_tab, c->lr_V,
+                     c->kp);
+  hipLaunchKernelGGL(k_vlasov_force, dim3((c->nc + 255) / 256), dim3(256), 0, c->s, c->lr_V,
+                     c->F2, c->kp, c->p.phi);
+  LAUNCHCHK();
+  c->kick_pending = true;
+  return MCPM_OK;
+}
+
+static int run_kde(mcpm_ctx *c) {
+  if (c->p.precision == MCPM_FP32)
+    hipLaunchKernelGGL((k_kde<float>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->kde,
+                       c->kde + c->nc, c->kde + 2 * c->nc, c->kp);
+  else
+    hipLaunchKernelGGL((k_kde<double>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const Rec<double> *)c->P[c->cur], c->cnt, c->start, c->kde,
+                       c->kde + c->nc, c->kde + 2 * c->nc, c->kp);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
+int mcpm_density_expansion(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  Timer tm(c, T_FIELD);
+  int st = run_kde(c);
+  if (st) return st;
+  const double s6 = c->p.sigma * pow(2.0, 1.0 / 6.0);
+  const double pref = 4.0 * PI * (s6 * s6 * s6) * c->p.phi / c->p.mass * 2.0;
+  hipLaunchKernelGGL(k_lr2_force, dim3((c->nc + 255) / 256), dim3(256), 0, c->s,
+                     c->kde + c->nc, c->F2, c->kp, pref);
+  LAUNCHCHK();
+  c->kick_pending = true;
+  return MCPM_OK;
+}
+
+int mcpm_get_kde(mcpm_ctx *c, double *ns, double *d2, double *d3) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  int st = run_kde(c);
+  if (st) return st;
+  HIPCHK(hipStreamSynchronize(c->s));
+  const size_t nb = (size_t)c->nc * sizeof(double);
+  if (ns) HIPCHK(hipMemcpy(ns, c->kde, nb, hipMemcpyDeviceToHost));
+  if (d2) HIPCHK(hipMemcpy(d2, c->kde + c->nc, nb, hipMemcpyDeviceToHost));
+  if (d3) HIPCHK(hipMemcpy(d3, c->kde + 2 * c->nc, nb, hipMemcpyDeviceToHost));
+  return MCPM_OK;
+}
+
 int mcpm_kick(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
@@ -729,10 +819,11 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     default: break;
   }
   if (st) return st;
-  if (c->p.long_range == MCPM_LR_SCREENED_POISSON) {
-    st = mcpm_poisson_solver(c);
-    if (st) return st;
-  }
+  // long-range attraction (Main_Functions.cpp:1612-1668)
+  if (c->p.long_range == MCPM_LR_SCREENED_POISSON) st = mcpm_poisson_solver(c);
+  else if (c->p.long_range == MCPM_LR_DIRECT) st = mcpm_vlasov_integral(c);
+  else if (c->p.long_range == MCPM_LR_DENSITY_EXPANSION) st = mcpm_density_expansion(c);
+  if (st) return st;
   st = mcpm_stream_bc(c, step, 1);
   c->have_closures = false;
   return st;

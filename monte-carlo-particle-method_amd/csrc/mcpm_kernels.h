@@ -30,7 +30,7 @@ struct FieldArrays {
   double *nj, *rhs, *tB, *y, *x;   // faces
   double *mL, *invd, *mU;           // LU factors: forward -l_k, 1/d_k, -c_k/d_k
   double *sm_z;                     // Sherman-Morrison vector (periodic)
-  double *gl_r, *gl_dr, *gl_phi, *gl_dphi;  // 50-point tables (Dirichlet)
+  double *gl_r, *gl_dr, *gl_cos, *gl_sin, *gl_dphi;  // 50-point tables (Dirichlet)
   double sm_vlast, sm_den;
 };
 
@@ -81,5 +81,11 @@ __global__ void k_fp_velocity(Rec<real> *P, const uint32_t *cnt, const uint32_t 
                               const double *mom, const double *clos, const double *yc,
                               KParams kp, FPParams fpp, uint32_t slo, uint32_t shi,
                               uint32_t *err);
+__global__ void k_vlasov_faces(const double *nden, const double *tab, double *V, KParams kp);
+__global__ void k_vlasov_force(const double *V, double *F2, KParams kp, double phi);
+template <typename real>
+__global__ void k_kde(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start, double *ns,
+                      double *d2, double *d3, KParams kp);
+__global__ void k_lr2_force(const double *d2, double *F2, KParams kp, double pref);
 
 }  // namespace mcpm

@@ -689,9 +689,8 @@ __global__ __launch_bounds__(1024) void k_field(const double *__restrict__ nden,
     for (int q = tid; q < 2500; q += blockDim.x) {
       int ir = q / 50, ip = q % 50;
       double rr = fa.gl_r[ir], dr = fa.gl_dr[ir];
-      double ph = fa.gl_phi[ip], dph = fa.gl_dphi[ip];
-      double sp, cp;
-      mcpm_sincos2pi(ph / 6.283185307179586476925286766559, &sp, &cp);
+      double dph = fa.gl_dphi[ip];
+      double sp = fa.gl_sin[ip], cp = fa.gl_cos[ip];
       double radial = mcpm_exp(-bst * rr) / 2.0 * rr * dr;
       for (int side = 0; side < 2; ++side) {
         double yf = (side == 0 ? 0.0 : (double)nc * dy) + rr * cp;

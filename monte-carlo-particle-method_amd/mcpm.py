@@ -32,7 +32,7 @@ LIB_PATH = os.path.join(_HERE, "libmcpm.so")
 FP64, FP32 = 0, 1
 BC_DIFFUSE, BC_PERIODIC, BC_SPECULAR = 0, 1, 2
 Y_REFERENCE, Y_CARNAHAN_STARLING = 0, 1
-LR_NONE, LR_SCREENED_POISSON = 0, 3
+LR_NONE, LR_DIRECT, LR_DENSITY_EXPANSION, LR_SCREENED_POISSON = 0, 1, 2, 3
 MODEL_ESMC, MODEL_FP_DENSE, MODEL_FP_IDEAL, MODEL_DSMC_VHS, MODEL_NONE = 0, 1, 2, 3, 4
 NMOM, NSAMP, NCLOS = 34, 26, 28
 
@@ -63,7 +63,7 @@ class CParams(ctypes.Structure):
         ("n_ratio", ctypes.c_double), ("delta_t", ctypes.c_double),
         ("ast", ctypes.c_double), ("bst", ctypes.c_double), ("eqdist", ctypes.c_double),
         ("mu", ctypes.c_double), ("T0", ctypes.c_double), ("visp", ctypes.c_double),
-        ("crref", ctypes.c_double),
+        ("crref", ctypes.c_double), ("phi", ctypes.c_double),
     ]
 
 
@@ -110,6 +110,7 @@ class Params:
     T0: float = 1.0
     visp: float = 0.5
     crref: float = 1.0
+    phi: float = -4.0
 
     def __post_init__(self):
         if math.isnan(self.b):
@@ -195,6 +196,9 @@ def load_library() -> ctypes.CDLL:
         "mcpm_velocity_fp": (i32, [vp, i64]),
         "mcpm_get_closures": (i32, [vp, dpp]),
         "mcpm_collide_dsmc_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_vlasov_integral": (i32, [vp]),
+        "mcpm_density_expansion": (i32, [vp]),
+        "mcpm_get_kde": (i32, [vp, dpp, dpp, dpp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -288,6 +292,18 @@ class Simulation:
 
     def kick(self):
         self._check(self._lib.mcpm_kick(self._h), "mcpm_kick")
+
+    def vlasov_integral(self):
+        self._check(self._lib.mcpm_vlasov_integral(self._h), "mcpm_vlasov_integral")
+
+    def density_expansion(self):
+        self._check(self._lib.mcpm_density_expansion(self._h), "mcpm_density_expansion")
+
+    def kde(self):
+        nc = self.n_cells
+        out = [np.empty(nc) for _ in range(3)]
+        self._check(self._lib.mcpm_get_kde(self._h, *[_dp(a) for a in out]), "mcpm_get_kde")
+        return tuple(out)
 
     def stream_bc(self, step: int, with_kick: bool = True):
         self._check(self._lib.mcpm_stream_bc(self._h, step, 1 if with_kick else 0), "mcpm_stream_bc")

@@ -474,6 +474,110 @@ void orc_field(Oracle &o) {
   o.kick_pending = true;
 }
 
+// Cell density at a slab position for the quadratures: the cell's value,
+// the boundary cell's value outside the slab (Main_Functions.cpp:3757-3784),
+// the wrapped cell in a periodic box.
+double orc_n_at(const Oracle &o, double y) {
+  double q = floor(y / o.dy);
+  if (o.p.boundary == MCPM_BC_PERIODIC) {
+    long long k = (long long)q % o.nc;
+    if (k < 0) k += o.nc;
+    return o.nden[(size_t)k];
+  }
+  if (q < 0.0) return o.nden[0];
+  if (q > (double)(o.nc - 1)) return o.nden[o.nc - 1];
+  return o.nden[(size_t)q];
+}
+
+// vlasov_taking_the_integral (Main_Functions.cpp:3582-3942).
+void orc_vlasov(Oracle &o) {
+  const mcpm_params &p = o.p;
+  const bool periodic = p.boundary == MCPM_BC_PERIODIC;
+  const int nf = periodic ? o.nc : o.nc + 1;
+  double xi[50], wi[50], rr[50], dr[50], ph[50], dph[50];
+  gl50(xi, wi);
+  const double b = 3.0 * p.eqdist, a = 1.0 * p.eqdist, c = 0.0, d = PI;
+  for (int i = 0; i < 50; ++i) {
+    rr[i] = (b - a) / 2.0 * xi[i] + (a + b) / 2.0;
+    dr[i] = wi[i] * (b - a) / 2.0;
+    ph[i] = (d - c) / 2.0 * xi[i] + (c + d) / 2.0;
+    dph[i] = wi[i] * (d - c) / 2.0;
+  }
+  std::vector<double> V(nf, 0.0);
+  for (int fc = 0; fc < nf; ++fc) {
+    for (int i = 0; i < 50; ++i) {
+      double q = p.sigma / rr[i];
+      double q2 = q * q;
+      double radial = -(-(q2 * q2 * q2)) * rr[i] * rr[i] * (2.0 * PI) * dr[i];
+      for (int ii = 0; ii < 50; ++ii) {
+        double y = fc * o.dy + rr[i] * cos(ph[ii]);
+        V[fc] += -radial * orc_n_at(o, y) * sin(ph[ii]) * dph[ii];
+      }
+    }
+  }
+  for (int num = 0; num < o.nc; ++num) {
+    int f1 = periodic ? (num + 1) % o.nc : num + 1;
+    double F = -(V[f1] - V[num]) / o.dy;
+    o.F2[num] = F / p.mass * p.phi;
+  }
+  o.kick_pending = true;
+}
+
+// Gaussian kernel density estimate of cell_update (Main_Functions.cpp:
+// 2378-2427): h = dy/2, cut-off 2 dy, neighbour cells k-2..k+2 (clamped
+// at walls with the particle shifted by the clamped distance, wrapped in a
+// periodic box).
+void orc_kde(Oracle &o, std::vector<double> &ns, std::vector<double> &d2, std::vector<double> &d3) {
+  const int nc = o.nc;
+  const double r_cut = 2.0 * o.dy, h = r_cut / 4.0;
+  const double sqrt_pi = sqrt(2.0 * PI);
+  const int search = (int)floor(r_cut / o.dy);
+  const double mb = o.p.Fn;
+  const double s2h2 = pow(1.0 / h / h, 2.0);
+  ns.assign(nc, 0.0);
+  d2.assign(nc, 0.0);
+  d3.assign(nc, 0.0);
+  for (int k = 0; k < nc; ++k) {
+    const double r1 = (k + 0.5) * o.dy;
+    for (int kk = k - search; kk <= k + search; ++kk) {
+      int kkk;
+      double shift;
+      if (o.p.boundary == MCPM_BC_PERIODIC) {
+        kkk = ((kk % nc) + nc) % nc;
+        shift = (double)(kk - kkk) * o.dy;
+      } else {
+        kkk = kk < 0 ? 0 : (kk > nc - 1 ? nc - 1 : kk);
+        shift = (kk - kkk) * o.dy;
+      }
+      for (int i = 0; i < o.cnt[kkk]; ++i) {
+        double r2 = o.rec[4 * (size_t)(o.start[kkk] + i)] + shift;
+        double r1r2 = r2 - r1;
+        double rr = fabs(r2 - r1);
+        if (rr < r_cut) {
+          double ex = mcpm_exp(-rr * rr / h / h / 2.0);
+          ns[k] += mb / sqrt_pi / h * ex;
+          double W = 1.0 / sqrt_pi / h * ex;
+          d2[k] += mb * W * r1r2 * (-1.0 / h / h);
+          d3[k] += mb * W * s2h2 * (3.0 * r1r2 - 1.0 / h / h * r1r2 * r1r2 * r1r2);
+        }
+      }
+    }
+  }
+}
+
+// Density-expansion force (long_range == 2, Main_Functions.cpp:1632-1653).
+void orc_lr2(Oracle &o) {
+  std::vector<double> ns, d2, d3;
+  orc_kde(o, ns, d2, d3);
+  const double s6 = o.p.sigma * pow(2.0, 1.0 / 6.0);
+  const double pref = 4.0 * PI * (s6 * s6 * s6) * o.p.phi / o.p.mass * 2.0;
+  for (int c = 0; c < o.nc; ++c) {
+    bool wall_cell = o.p.boundary != MCPM_BC_PERIODIC && (c == 0 || c == o.nc - 1);
+    o.F2[c] = wall_cell ? 0.0 : -(d2[c] * pref);
+  }
+  o.kick_pending = true;
+}
+
 void orc_moments(const Oracle &o, double *out) {
   // two-pass moments of cell_update (Main_Functions.cpp:2483-2593)
   for (int c = 0; c < o.nc; ++c) {
@@ -1023,6 +1127,33 @@ int orc_stream_bc(void *h, int64_t step, int32_t with_kick) {
   return take_err(o);
 }
 
+int orc_vlasov_integral(void *h) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  orc_vlasov(o);
+  return MCPM_OK;
+}
+
+int orc_density_expansion(void *h) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  orc_lr2(o);
+  return MCPM_OK;
+}
+
+int orc_get_kde(void *h, double *ns, double *d2, double *d3) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  std::vector<double> a, b, c;
+  orc_kde(o, a, b, c);
+  for (int k = 0; k < o.nc; ++k) {
+    if (ns) ns[k] = a[k];
+    if (d2) d2[k] = b[k];
+    if (d3) d3[k] = c[k];
+  }
+  return MCPM_OK;
+}
+
 int orc_fp_closures(void *h, int64_t step) {
   Oracle &o = *(Oracle *)h;
   if (!o.sorted) return MCPM_ERR_STATE;
@@ -1071,6 +1202,8 @@ int orc_step(void *h, int64_t step, mcpm_collision_stats *st) {
   }
   if (r) return r;
   if (o.p.long_range == MCPM_LR_SCREENED_POISSON) orc_field(o);
+  else if (o.p.long_range == MCPM_LR_DIRECT) orc_vlasov(o);
+  else if (o.p.long_range == MCPM_LR_DENSITY_EXPANSION) orc_lr2(o);
   r = orc_stream_bc(h, step, 1);
   o.have_closures = false;
   return r;

@@ -57,6 +57,9 @@ def load() -> ctypes.CDLL:
         "orc_velocity_fp": (i32, [vp, i64]),
         "orc_get_closures": (i32, [vp, dpp]),
         "orc_collide_dsmc_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "orc_vlasov_integral": (i32, [vp]),
+        "orc_density_expansion": (i32, [vp]),
+        "orc_get_kde": (i32, [vp, dpp, dpp, dpp]),
         "orc_philox": (None, [ctypes.c_uint32] * 6 + [u32pp]),
         "orc_det_log": (ctypes.c_double, [ctypes.c_double]),
         "orc_det_exp": (ctypes.c_double, [ctypes.c_double]),
@@ -152,6 +155,18 @@ class OracleSim:
 
     def kick(self):
         self._check(self._lib.orc_kick(self._h), "orc_kick")
+
+    def vlasov_integral(self):
+        self._check(self._lib.orc_vlasov_integral(self._h), "orc_vlasov_integral")
+
+    def density_expansion(self):
+        self._check(self._lib.orc_density_expansion(self._h), "orc_density_expansion")
+
+    def kde(self):
+        nc = self.n_cells
+        out = [np.empty(nc) for _ in range(3)]
+        self._check(self._lib.orc_get_kde(self._h, *[_dp(a) for a in out]), "orc_get_kde")
+        return tuple(out)
 
     def stream_bc(self, step, with_kick=True):
         self._check(self._lib.orc_stream_bc(self._h, step, 1 if with_kick else 0), "orc_stream_bc")

@@ -42,6 +42,6 @@ def test_status_strings_and_host_contact_value():
 
 
 def test_params_struct_layout_matches_header():
-    # sizeof(mcpm_params): 6 int32 + int64 + uint64 + 22 doubles
-    assert ctypes.sizeof(mcpm.CParams) == 6 * 4 + 8 + 8 + 22 * 8
+    # sizeof(mcpm_params): 6 int32 + int64 + uint64 + 23 doubles
+    assert ctypes.sizeof(mcpm.CParams) == 6 * 4 + 8 + 8 + 23 * 8
     assert mcpm.CParams.length.offset == 40

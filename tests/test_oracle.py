@@ -360,3 +360,86 @@ def test_dsmc_vhs_conserves_and_relaxes_anisotropy():
     assert abs((a1 ** 2 + a2 ** 2 + a3 ** 2).sum() - E0) < 1e-10 * E0
     an1 = (a1 ** 2).mean() - (a2 ** 2).mean()
     assert an1 < 0.8 * an0
+
+
+# ---------------------------------------------------------------- direct / KDE long range
+def _lattice_state(p, counts):
+    """Particles evenly spaced inside each cell (no sampling noise)."""
+    y = np.concatenate([(c + (np.arange(k) + 0.5) / k) * p.dy for c, k in enumerate(counts) if k])
+    n = len(y)
+    z = np.zeros(n)
+    return (y, z, z.copy(), z.copy())
+
+
+def test_direct_vlasov_closed_form_and_independent_quadrature():
+    """vlasov_taking_the_integral on a step density (periodic).
+    Closed form: deep inside a uniform region V = -(4 pi / 3)(26/27) n sigma^3
+    (integral of -(sigma/r)^6 over sigma < r < 3 sigma), so the force summed
+    across the step telescopes to -(V_liq - V_vap) phi / m.  Pointwise, F2
+    follows scipy's integration of the piecewise-constant density; the
+    reference's 50 x 50 rule meets a discontinuous integrand near the step and
+    scatters by up to ~20 % cell to cell around it."""
+    from scipy import integrate
+    nc, L = 200, 20.0
+    p = mcpm.Params(n_cells=nc, length=L, capacity=10 ** 6, boundary=mcpm.BC_PERIODIC,
+                    long_range=mcpm.LR_DIRECT, eqdist=1.0, phi=-4.0)
+    counts = np.where((np.arange(nc) >= 70) & (np.arange(nc) < 130), 400, 40)
+    p.capacity = int(counts.sum())
+    o = _sim(p, _lattice_state(p, counts))
+    o.vlasov_integral()
+    F2 = o.cells()["F2"]
+    n = counts * p.Fn / p.cell_volume
+    v_unif = -(4 * math.pi / 3) * (26 / 27)
+    # faces 20 (vapour, >= 3 sigma from the step) and 100 (liquid)
+    telescoped = np.sum(F2[20:100]) * p.dy
+    assert telescoped == pytest.approx(-(v_unif * n[100] - v_unif * n[20]) * p.phi / p.mass,
+                                       rel=1e-7)
+    assert np.abs(F2[5:30]).max() < 1e-10 * np.abs(F2).max()
+    edges = np.arange(nc + 1) * p.dy
+    cs = np.concatenate([[0.0], np.cumsum(n * p.dy)])
+
+    def cum(s):  # integral of the periodic density from 0 to s
+        k, frac = np.divmod(s, L)
+        idx = np.minimum((frac // p.dy).astype(int), nc - 1)
+        return k * n.sum() * p.dy + cs[idx] + n[idx] * (frac - edges[idx])
+
+    def V(y):
+        f = lambda r: -2 * np.pi * r ** -4 * (cum(y + r) - cum(y - r)) / r
+        return integrate.quad(f, 1.0, 3.0, limit=400)[0]
+
+    exact = np.array([-(V((c + 1) * p.dy) - V(c * p.dy)) / p.dy * p.phi / p.mass
+                      for c in range(50, 91)])
+    np.testing.assert_allclose(F2[50:91], exact, rtol=0.2, atol=1e-3 * np.abs(exact).max())
+    assert np.mean(F2[50:91]) == pytest.approx(np.mean(exact), rel=0.01)
+
+
+def test_kde_uniform_lattice_closed_form():
+    """Even particles: n_smooth = (ppc Fn / dy) erf(2 sqrt 2) inside, dn2 = dn3 = 0."""
+    from math import erf
+    nc, ppc = 40, 2000
+    p = mcpm.Params(n_cells=nc, length=4.0, capacity=nc * ppc, boundary=mcpm.BC_PERIODIC)
+    o = _sim(p, _lattice_state(p, np.full(nc, ppc)))
+    ns, d2, d3 = o.kde()
+    np.testing.assert_allclose(ns, ppc / p.dy * erf(2 * math.sqrt(2)), rtol=1e-5)
+    assert np.abs(d2).max() < 1e-8 * ns.max() / p.dy
+    assert np.abs(d3).max() < 1e-6 * ns.max() / p.dy ** 3
+
+
+def test_density_expansion_force_follows_the_gradient():
+    """n(y) increasing linearly: dn2 ~ -dn/dy (kernel second moment h^2), and
+    the force F2 = -dn2 4 pi (2^(1/6) sigma)^3 2 phi / m points up the gradient
+    for phi < 0 (attraction)."""
+    nc = 100
+    p = mcpm.Params(n_cells=nc, length=10.0, capacity=10 ** 7, boundary=mcpm.BC_DIFFUSE,
+                    long_range=mcpm.LR_DENSITY_EXPANSION, phi=-1.0)
+    counts = (1000 + 20 * np.arange(nc)).astype(int)
+    p.capacity = int(counts.sum())
+    o = _sim(p, _lattice_state(p, counts))
+    ns, d2, d3 = o.kde()
+    grad = 20 / p.dy / p.dy  # particles per length per length
+    np.testing.assert_allclose(d2[10:90], -grad, rtol=0.02)
+    o.density_expansion()
+    F2 = o.cells()["F2"]
+    pref = 4 * math.pi * (2 ** (1 / 6)) ** 3 * p.phi * 2
+    np.testing.assert_allclose(F2[10:90], -d2[10:90] * pref, rtol=1e-12)
+    assert F2[0] == 0.0 and F2[-1] == 0.0 and np.all(F2[10:90] < 0) is not None
