@@ -98,11 +98,11 @@ def main():
     #  sort (push pass 1 + scatter pass): read 16 B + read 16 B + write 16 B per
     #  particle (pass 1 reads the record, pass 2 reads it again and writes it);
     #  collide rounds: per valid operation the two particle records are read
-    #  and written (4 x 16 B) plus the operation itself (5 x 4 B).
+    #  and written once (4 x 16 B) plus the 16-byte operation record.
     sort_ms = timers["sort"]["avg_ms"]
     coll_ms = timers["collide_rounds"]["avg_ms"]
     sort_bytes = 48.0 * N
-    coll_bytes = valid * (64.0 + 20.0)
+    coll_bytes = valid * (64.0 + 16.0)
     if coll_ms >= sort_ms:
         dom, dom_ms, dom_bytes = "collide_rounds (k_round)", coll_ms, coll_bytes
     else:

@@ -43,12 +43,11 @@ struct mcpm_ctx {
   uint32_t *counters = nullptr, *err = nullptr;
   uint32_t *res[2] = {nullptr, nullptr};
   // collisions
-  OpArrays ops{};
-  uint32_t *list[2] = {nullptr, nullptr};
+  uint4 *opA = nullptr, *opB = nullptr;  // one allocation: ping-pong op lists
+  OpLog log{};                           // one allocation: debug log
   int64_t op_cap = 0;
-  uint8_t *log_acc = nullptr;
   bool debug_log = false;
-  int64_t last_ops = 0;
+  int64_t last_ops = 0, last_log = 0;
   // Fokker-Planck / DSMC
   FPParams fpp{};
   double *clos = nullptr, *drift = nullptr;
@@ -347,27 +346,23 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   if (m <= c->op_cap) return MCPM_OK;
   int64_t cap = m + m / 4 + 1024;
   HIPCHK(hipStreamSynchronize(c->s));
-  // one allocation holds the five op arrays and both lists
-  if (c->ops.i) HIPCHK(hipFree(c->ops.i));
-  if (c->log_acc) HIPCHK(hipFree(c->log_acc));
-  c->ops = OpArrays{};
-  c->list[0] = c->list[1] = nullptr;
-  c->log_acc = nullptr;
-  uint32_t *blk = nullptr;
-  int st = dalloc(&blk, (size_t)cap * 7);
-  if (st) {
-    c->op_cap = 0;
-    return st;
-  }
-  c->ops.i = blk;
-  c->ops.j = blk + cap;
-  c->ops.p = blk + 2 * cap;
-  c->ops.c = blk + 3 * cap;
-  c->ops.t = blk + 4 * cap;
-  c->list[0] = blk + 5 * cap;
-  c->list[1] = blk + 6 * cap;
-  st = dalloc(&c->log_acc, (size_t)cap);
+  if (c->opA) HIPCHK(hipFree(c->opA));
+  if (c->log.p) HIPCHK(hipFree(c->log.p));
+  c->opA = c->opB = nullptr;
+  c->log = OpLog{};
+  c->op_cap = 0;
+  uint4 *blk = nullptr;
+  int st = dalloc(&blk, (size_t)cap * 2);
   if (st) return st;
+  c->opA = blk;
+  c->opB = blk + cap;
+  uint32_t *lg = nullptr;
+  st = dalloc(&lg, (size_t)cap * 3 + ((size_t)cap + 3) / 4);
+  if (st) return st;
+  c->log.p = lg;
+  c->log.i = lg + cap;
+  c->log.j = lg + 2 * cap;
+  c->log.a = (uint8_t *)(lg + 3 * cap);
   c->op_cap = cap;
   return MCPM_OK;
 }
@@ -393,20 +388,22 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   {
     Timer tm(c, T_GEN);
     hipLaunchKernelGGL((k_gen<real, MODEL>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
-                       c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->ops, c->res[0],
+                       c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->opA, c->res[0],
                        c->counters);
     LAUNCHCHK();
   }
   int rounds = 0;
   {
     Timer tm(c, T_COLLIDE);
-    unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(1, ((int64_t)M + 255) / 256), 4096);
-    uint8_t *lg = c->debug_log ? c->log_acc : nullptr;
+    // a multiple of 8 blocks so that every XCD gets the same share
+    unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256), 4096);
+    grid = (grid + 7u) / 8u * 8u;
+    OpLog lg = c->debug_log ? c->log : OpLog{};
     int next_check = 12;
     for (int r = 0; r < MAX_ROUNDS; ++r) {
-      hipLaunchKernelGGL((k_round<real, MODEL>), dim3(grid), dim3(256), 0, c->s, r, P, c->ops,
-                         r == 0 ? (const uint32_t *)nullptr : (const uint32_t *)c->list[r & 1],
-                         c->list[(r + 1) & 1], c->res[r & 1], c->res[(r + 1) & 1], c->counters,
+      hipLaunchKernelGGL((k_round<real, MODEL>), dim3(grid), dim3(256), 0, c->s, r, M, P,
+                         (const uint4 *)((r & 1) ? c->opB : c->opA), (r & 1) ? c->opA : c->opB,
+                         c->res[r & 1], c->res[(r + 1) & 1], c->counters,
                          c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg);
       LAUNCHCHK();
       rounds = r + 1;
@@ -430,6 +427,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   HIPCHK(hipMemcpy(&left, c->counters + CNT_LIST + rounds, sizeof left, hipMemcpyDeviceToHost));
   if (left != 0) return MCPM_ERR_CAPACITY;  // more rounds than MAX_ROUNDS
   c->last_ops = hc[CNT_OPS];
+  c->last_log = hc[CNT_LOG];
   if (stats) {
     stats->candidates = M;
     stats->valid = hc[CNT_OPS];
@@ -567,8 +565,8 @@ int mcpm_destroy(mcpm_ctx *c) {
   if (c->s) hipStreamSynchronize(c->s);
   void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
-                  c->samples, c->counters, c->err, c->res[0], c->res[1], c->ops.i,
-                  c->log_acc, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde};
+                  c->samples, c->counters, c->err, c->res[0], c->res[1], c->opA,
+                  c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < T_NCLASS; ++k)
@@ -921,14 +919,14 @@ int mcpm_get_collision_log(mcpm_ctx *c, int64_t cap, uint32_t *prio, uint32_t *i
                            uint8_t *accepted, int64_t *n_out) {
   if (!c || !n_out) return MCPM_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->s));
-  int64_t n = c->last_ops;
+  int64_t n = c->last_log;
   *n_out = n;
   if (n > cap) return MCPM_ERR_CAPACITY;
   if (n == 0) return MCPM_OK;
-  if (prio) HIPCHK(hipMemcpy(prio, c->ops.p, n * 4, hipMemcpyDeviceToHost));
-  if (i) HIPCHK(hipMemcpy(i, c->ops.i, n * 4, hipMemcpyDeviceToHost));
-  if (j) HIPCHK(hipMemcpy(j, c->ops.j, n * 4, hipMemcpyDeviceToHost));
-  if (accepted) HIPCHK(hipMemcpy(accepted, c->log_acc, n, hipMemcpyDeviceToHost));
+  if (prio) HIPCHK(hipMemcpy(prio, c->log.p, n * 4, hipMemcpyDeviceToHost));
+  if (i) HIPCHK(hipMemcpy(i, c->log.i, n * 4, hipMemcpyDeviceToHost));
+  if (j) HIPCHK(hipMemcpy(j, c->log.j, n * 4, hipMemcpyDeviceToHost));
+  if (accepted) HIPCHK(hipMemcpy(accepted, c->log.a, n, hipMemcpyDeviceToHost));
   return MCPM_OK;
 }
 

@@ -89,8 +89,10 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double *addr, double v) {
-  // For non-negative doubles the IEEE bit pattern orders like the value.
-  if (v > 0.0)
+  // For non-negative doubles the IEEE bit pattern orders like the value.  The
+  // plain read filters most calls: the stored maximum only grows, so a stale
+  // read is never above the true value and cannot skip a needed update.
+  if (v > 0.0 && v > *(volatile double *)addr)
     atomicMax((unsigned long long *)addr, (unsigned long long)mcpm_dbits(v));
 }
 

@@ -15,14 +15,20 @@ enum : uint32_t { ERRBIT_DOMAIN = 1u, ERRBIT_NAN = 2u, ERRBIT_BC = 4u, ERRBIT_CA
 enum : int {
   CNT_OPS = 0,     // valid candidate operations of the current step
   CNT_ACC = 2,     // accepted collisions (u64, 8-byte aligned)
-  CNT_LIST = 4,    // CNT_LIST + r: length of the list processed by round r (r >= 1)
+  CNT_LOG = 4,     // entries in the debug collision log
+  CNT_LIST = 6,    // CNT_LIST + r: length of the list processed by round r (r >= 1)
   MAX_ROUNDS = 512,
   CNT_TOTAL = CNT_LIST + MAX_ROUNDS + 2
 };
 
-// Candidate operations in structure-of-arrays form.
-struct OpArrays {
-  uint32_t *i, *j, *p, *c, *t;
+// Candidate operation {i, j, priority, t} as one uint4; i = OP_NONE marks a
+// candidate without a partner.
+constexpr uint32_t OP_NONE = 0xFFFFFFFFu;
+
+// Debug log of committed operations (null pointers when disabled).
+struct OpLog {
+  uint32_t *p, *i, *j;
+  uint8_t *a;
 };
 
 // Work arrays and host-prepared factors of the 1D field solve.
@@ -53,14 +59,14 @@ __global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc);
 template <typename real, int MODEL>
 __global__ void k_gen(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
                       const uint32_t *mcand, const uint32_t *cstart, KParams kp,
-                      uint32_t step_lo, uint32_t step_hi, uint32_t prio_key, OpArrays ops,
+                      uint32_t step_lo, uint32_t step_hi, uint32_t prio_key, uint4 *ops,
                       uint32_t *res0, uint32_t *counters);
 template <typename real, int MODEL>
-__global__ void k_round(int r, Rec<real> *P, OpArrays ops, const uint32_t *list_in,
-                        uint32_t *list_out, uint32_t *res_cur, uint32_t *res_next,
-                        uint32_t *counters, const double *nden, const double *yc,
-                        const double *omax, double *omega_new, KParams kp, uint32_t step_lo,
-                        uint32_t step_hi, FPParams fpp, uint8_t *log_acc);
+__global__ void k_round(int r, uint32_t count0, Rec<real> *P, const uint4 *in, uint4 *out,
+                        uint32_t *res_cur, uint32_t *res_next, uint32_t *counters,
+                        const double *nden, const double *yc, const double *omax,
+                        double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,
+                        FPParams fpp, OpLog log);
 __global__ void k_omega_finish(double *omax, double *omega_new, const uint32_t *cnt, int nc,
                                int model);
 __global__ void k_zero_u32(uint32_t *p, int n);

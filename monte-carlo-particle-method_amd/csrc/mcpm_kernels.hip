@@ -383,10 +383,20 @@ __device__ __forceinline__ void draw_dsmc_pair(uint32_t c, uint32_t t, const uin
   *oj = start[c] + lj;
 }
 
-// One workgroup per cell: draw the cell's m_c candidates, keep the valid
-// ones as operations, reserve both particles for the first round.  ESMC
-// priorities are the keyed bijection of the candidate index; DSMC keeps the
-// reference's in-cell candidate order (cells do not interact).
+// Hardware places workgroup b on XCD b % 8.  Renumber blocks so that each
+// XCD runs a contiguous range of logical blocks: neighbouring cells (and the
+// operations drawn in them) then share one XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t per = nb / 8u;
+  if (per == 0u || b >= per * 8u) return b;
+  return (b % 8u) * per + b / 8u;
+}
+
+// One workgroup per cell: draw the cell's m_c candidates and store each at
+// its candidate index as {i, j, priority, t} (invalid ones as OP_NONE), so
+// that the first round streams operations in cell order; reserve both
+// particles.  ESMC priorities are the keyed bijection of the candidate
+// index; DSMC keeps the reference's in-cell candidate order.
 template <typename real, int MODEL>
 __global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
                                              const uint32_t *__restrict__ cnt,
@@ -394,18 +404,18 @@ __global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
                                              const uint32_t *__restrict__ mcand,
                                              const uint32_t *__restrict__ cstart, KParams kp,
                                              uint32_t step_lo, uint32_t step_hi,
-                                             uint32_t prio_key, OpArrays ops,
+                                             uint32_t prio_key, uint4 *__restrict__ ops,
                                              uint32_t *__restrict__ res0,
                                              uint32_t *__restrict__ counters) {
-  const uint32_t c = blockIdx.x;
+  const uint32_t c = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t m = mcand[c];
-  if (m == 0 || cnt[c] == 0) return;
+  if (m == 0) return;
   const uint32_t k0 = cstart[c];
-  for (uint32_t t0 = 0; t0 < m; t0 += blockDim.x) {
-    uint32_t t = t0 + threadIdx.x;
+  uint32_t nvalid = 0;
+  for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
     uint32_t i = 0, j = 0;
     bool ok = false;
-    if (t < m) {
+    if (cnt[c] > 0) {
       if (MODEL == COLL_DSMC) {
         draw_dsmc_pair(c, t, cnt, start, kp, step_lo, step_hi, &i, &j);
         ok = true;
@@ -413,38 +423,42 @@ __global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
         ok = draw_candidate<real>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
       }
     }
-    uint32_t pos = wave_append(&counters[CNT_OPS], ok);
+    uint4 rec;
     if (ok) {
       uint32_t pr = MODEL == COLL_DSMC ? k0 + t : mcpm_priority(k0 + t, prio_key);
-      ops.i[pos] = i;
-      ops.j[pos] = j;
-      ops.p[pos] = pr;
-      ops.c[pos] = c;
-      ops.t[pos] = t;
+      rec = make_uint4(i, j, pr, t);
       atomicMin(&res0[i], pr);
       atomicMin(&res0[j], pr);
+      ++nvalid;
+    } else {
+      rec = make_uint4(OP_NONE, 0u, 0u, 0u);
     }
+    ops[k0 + t] = rec;
   }
+  for (int off = 32; off > 0; off >>= 1) nvalid += __shfl_down(nvalid, off);
+  if (lane_id() == 0 && nvalid) atomicAdd(&counters[CNT_OPS], nvalid);
 }
 
 // The collision itself, collisions.cpp:680-716: Y at the midpoint cell,
 // p = 4 pi sigma^2 (g.dir) Y n_J dt, omega_I = max(omega_I, p),
 // accept iff p / omega_max_I > U, hard-sphere exchange along dir.
+// The cell I of the candidate is the cell of particle i (i was drawn in it).
 template <typename real>
-__device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t c, uint32_t t,
-                                          Rec<real> *P, const double *nden, const double *yc,
+__device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, Rec<real> *P,
+                                          const double *nden, const double *yc,
                                           const double *omax, double *omega_new,
                                           const KParams &kp, uint32_t step_lo,
                                           uint32_t step_hi) {
+  Rec<real> a = P[i];
+  Rec<real> bb = P[j];
+  double yi = (double)a.y, yj = (double)bb.y;
+  const uint32_t c = (uint32_t)cell_of(yi, kp.dy, kp.n_cells);
   mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
                              kp.key1);
   mcpm_u32x4 w2 = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 1u), kp.key0,
                               kp.key1);
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-  Rec<real> a = P[i];
-  Rec<real> bb = P[j];
-  double yi = (double)a.y, yj = (double)bb.y;
   double ymid;
   if (kp.boundary == MCPM_BC_PERIODIC) {
     double d = yj - yi;
@@ -483,14 +497,15 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t c, ui
 // sigma_T ~ Vr^(2-2 visp) crref^(2 visp - 1), accept iff sigma_T / crm > U,
 // isotropic post-collision relative velocity about the centre of mass.
 template <typename real>
-__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t c, uint32_t t,
-                                            Rec<real> *P, const double *omax, double *omega_new,
+__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, Rec<real> *P,
+                                            const double *omax, double *omega_new,
                                             const KParams &kp, const FPParams &fpp,
                                             uint32_t step_lo, uint32_t step_hi) {
-  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
-                             kp.key1);
   Rec<real> a = P[i];
   Rec<real> bb = P[j];
+  const uint32_t c = (uint32_t)cell_of((double)a.y, kp.dy, kp.n_cells);
+  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
+                             kp.key1);
   double g0 = (double)a.u1 - (double)bb.u1;
   double g1 = (double)a.u2 - (double)bb.u2;
   double g2 = (double)a.u3 - (double)bb.u3;
@@ -523,13 +538,14 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t c, 
 // holds the smallest priority on both of its particles, which is exactly
 // when every earlier-priority operation sharing a particle has committed:
 // the result equals processing all operations one by one in priority order.
-// Losers reserve in the other array for the next round.  Each operation that
-// finds its own priority in the current array clears it, so the array is
-// back to all-ones when the round ends.
+// Losers reserve in the other array for the next round and are compacted
+// into the next operation list.  Each operation that finds its own priority
+// in the current array clears it, so the array is back to all-ones when the
+// round ends.
 template <typename real, int MODEL>
-__global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P, OpArrays ops,
-                                               const uint32_t *__restrict__ list_in,
-                                               uint32_t *__restrict__ list_out,
+__global__ __launch_bounds__(256) void k_round(int r, uint32_t count0, Rec<real> *__restrict__ P,
+                                               const uint4 *__restrict__ in,
+                                               uint4 *__restrict__ out,
                                                uint32_t *__restrict__ res_cur,
                                                uint32_t *__restrict__ res_next,
                                                uint32_t *__restrict__ counters,
@@ -538,29 +554,33 @@ __global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P,
                                                const double *__restrict__ omax,
                                                double *__restrict__ omega_new, KParams kp,
                                                uint32_t step_lo, uint32_t step_hi,
-                                               FPParams fpp, uint8_t *__restrict__ log_acc) {
-  const uint32_t count = (r == 0) ? counters[CNT_OPS] : counters[CNT_LIST + r];
+                                               FPParams fpp, OpLog log) {
+  const uint32_t count = (r == 0) ? count0 : counters[CNT_LIST + r];
   const uint32_t stride = gridDim.x * blockDim.x;
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   uint32_t acc_local = 0;
-  for (uint32_t base = blockIdx.x * blockDim.x; base < count; base += stride) {
+  for (uint32_t base = lb * blockDim.x; base < count; base += stride) {
     uint32_t idx = base + threadIdx.x;
-    bool live = idx < count;
-    uint32_t o = 0, i = 0, j = 0, pr = 0;
+    uint4 rec = make_uint4(OP_NONE, 0u, 0u, 0u);
+    if (idx < count) rec = in[idx];
     bool lose = false;
-    if (live) {
-      o = (r == 0) ? idx : list_in[idx];
-      i = ops.i[o];
-      j = ops.j[o];
-      pr = ops.p[o];
+    if (rec.x != OP_NONE) {
+      const uint32_t i = rec.x, j = rec.y, pr = rec.z;
       uint32_t ri = res_cur[i], rj = res_cur[j];
       if (ri == pr && rj == pr) {
         bool acc = MODEL == COLL_DSMC
-                       ? commit_dsmc<real>(i, j, ops.c[o], ops.t[o], P, omax, omega_new, kp, fpp,
-                                           step_lo, step_hi)
-                       : commit_op<real>(i, j, ops.c[o], ops.t[o], P, nden, yc, omax, omega_new,
-                                         kp, step_lo, step_hi);
+                       ? commit_dsmc<real>(i, j, rec.w, P, omax, omega_new, kp, fpp, step_lo,
+                                           step_hi)
+                       : commit_op<real>(i, j, rec.w, P, nden, yc, omax, omega_new, kp, step_lo,
+                                         step_hi);
         acc_local += acc ? 1u : 0u;
-        if (log_acc) log_acc[o] = acc ? 1 : 0;
+        if (log.p) {
+          uint32_t q = atomicAdd(&counters[CNT_LOG], 1u);
+          log.p[q] = pr;
+          log.i[q] = i;
+          log.j[q] = j;
+          log.a[q] = acc ? 1 : 0;
+        }
         res_cur[i] = 0xFFFFFFFFu;
         res_cur[j] = 0xFFFFFFFFu;
       } else {
@@ -572,7 +592,7 @@ __global__ __launch_bounds__(256) void k_round(int r, Rec<real> *__restrict__ P,
       }
     }
     uint32_t pos = wave_append(&counters[CNT_LIST + r + 1], lose);
-    if (lose) list_out[pos] = o;
+    if (lose) out[pos] = rec;
   }
   // accepted count, one atomic per wavefront
   for (int off = 32; off > 0; off >>= 1) acc_local += __shfl_down(acc_local, off);
@@ -871,12 +891,12 @@ __global__ __launch_bounds__(256) void k_sample(const Rec<real> *__restrict__ P,
   template __global__ void k_gen<real, M>(const Rec<real> *, const uint32_t *,             \
                                           const uint32_t *, const uint32_t *,              \
                                           const uint32_t *, KParams, uint32_t, uint32_t,   \
-                                          uint32_t, OpArrays, uint32_t *, uint32_t *);     \
-  template __global__ void k_round<real, M>(int, Rec<real> *, OpArrays, const uint32_t *,  \
-                                            uint32_t *, uint32_t *, uint32_t *,            \
-                                            uint32_t *, const double *, const double *,    \
-                                            const double *, double *, KParams, uint32_t,   \
-                                            uint32_t, FPParams, uint8_t *);
+                                          uint32_t, uint4 *, uint32_t *, uint32_t *);      \
+  template __global__ void k_round<real, M>(int, uint32_t, Rec<real> *, const uint4 *,     \
+                                            uint4 *, uint32_t *, uint32_t *, uint32_t *,   \
+                                            const double *, const double *, const double *, \
+                                            double *, KParams, uint32_t, uint32_t,         \
+                                            FPParams, OpLog);
 
 #define MCPM_INST(real)                                                                    \
   template __global__ void k_push<real, false>(const Rec<real> *, Rec<real> *, int64_t,    \
