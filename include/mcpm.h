@@ -107,6 +107,14 @@ typedef struct mcpm_params {
   double   crref;        /* gas.crref: VHS reference relative speed             */
   double   phi;          /* gas.phi: attraction strength of the direct and
                             density-expansion forces (myIO.cpp:2915-2920)     */
+  /* The plane (problem injection, myIO.cpp:2729-2761; FEM.cpp): dims = 2
+     adds the x axis.  Then n_cells, length and boundary describe y (box.N[1],
+     box.Len[1]) and the fields below describe x.  0 is read as 1. */
+  int32_t  dims;         /* 1: slab (the default), 2: plane                     */
+  int32_t  n_cells_x;    /* box.N[0]; 2D only                                   */
+  int32_t  boundary_x;   /* enum mcpm_boundary along x; 2D only                 */
+  int32_t  reserved1;
+  double   length_x;     /* box.Len[0]; 2D only                                 */
 } mcpm_params;
 
 typedef struct mcpm_ctx mcpm_ctx;
@@ -137,6 +145,13 @@ int mcpm_upload(mcpm_ctx *ctx, int64_t n, const double *y,
 /* Copy the particles back in the current (cell-sorted) order. */
 int mcpm_download(mcpm_ctx *ctx, double *y, double *u1, double *u2, double *u3);
 int64_t mcpm_particle_count(const mcpm_ctx *ctx);
+/* The plane's particle data: the reference's x1, x2, U1, U2, U3
+   (main.cpp:104-165).  mcpm_upload / mcpm_download serve the slab; on a
+   plane context mcpm_upload is MCPM_ERR_ARG and mcpm_download leaves x out. */
+int mcpm_upload_xy(mcpm_ctx *ctx, int64_t n, const double *x, const double *y,
+                   const double *u1, const double *u2, const double *u3);
+int mcpm_download_xy(mcpm_ctx *ctx, double *x, double *y,
+                     double *u1, double *u2, double *u3);
 
 /* ---- per-step operators -------------------------------------------------- */
 /* cell_update (cpp_headers.h:501; Main_Functions.cpp:2138-2749): bin every
@@ -153,15 +168,36 @@ int mcpm_collide_esmc(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
 /* poisson_solver (cpp_headers.h:450; Main_Functions.cpp:3285-3581): the 1D
    screened-Poisson mean field and the kick U2 += F2 dt.  This call only
    solves for the per-cell force F2; the kick is applied by mcpm_stream_bc
-   (or mcpm_kick) in the same pass that moves the particles. */
+   (or mcpm_kick) in the same pass that moves the particles.
+   On a plane context this is compute_SP_2D (cpp_headers.h:398; FEM.cpp:
+   249-610): Q1 finite elements on the (N0+1)(N1+1) nodes, stiffness plus
+   lambda^2 V mass matrix, right-hand side V M n_n from the nodal density,
+   Dirichlet values on the walls from the 50^3-point Yukawa Green's-function
+   quadrature (FEM.cpp:196-247, 344-448), core subtraction with its 5-point
+   correction (FEM.cpp:497-554); the kick U -= grad(Q1 interpolant) ast / m
+   dt (FEM.cpp:568-582) is again left to the push.  The reference factors a
+   dense matrix with PARDISO; here conjugate gradients on the lifted system
+   solve the same equations. */
 int mcpm_poisson_solver(mcpm_ctx *ctx);
 
 /* vlasov_taking_the_integral (cpp_headers.h:628; Main_Functions.cpp:
    3582-3942): face potential V_f = -sum over r in [eqdist, 3 eqdist] and
    polar angle (50 x 50 Gauss-Legendre) of (sigma/r)^6 r^2 2 pi n sin(phi),
    F2 = -(V_{c+1} - V_c) / dy * phi / m.  Like mcpm_poisson_solver it sets F2
-   and leaves the kick to the push. */
+   and leaves the kick to the push.
+   On a plane context this is compute_Vlasov_2D (cpp_headers.h:390; FEM.cpp:
+   613-726): at every node the 100^3-point Gauss-Legendre sum over r in
+   [sigma, 3 sigma], azimuth and polar angle of (sigma/r)^6 r^2 n, and the
+   kick U -= grad(Q1 interpolant) phi / m dt. */
 int mcpm_vlasov_integral(mcpm_ctx *ctx);
+
+/* The plane's node arrays, (N0+1)(N1+1) doubles each, node (i, j) at
+   j (N0+1) + i (cells[c].id, myIO.cpp:1447-1451); either pointer may be NULL.
+   n_node: the nodal density deposit of cell_update (Main_Functions.cpp:
+   2268-2364): Fn / V per particle in the node's dual cell, doubled on each
+   wall the node lies on; computed by this call.  potential: the nodal
+   field x of the last mcpm_poisson_solver / mcpm_vlasov_integral. */
+int mcpm_get_nodes(mcpm_ctx *ctx, double *n_node, double *potential);
 
 /* Density-expansion attraction (long_range == 2, Main_Functions.cpp:
    1632-1653) from the Gaussian kernel-density gradient of cell_update

@@ -58,6 +58,13 @@ struct mcpm_ctx {
   // field
   FieldArrays fa{};
   double *field_buf = nullptr;
+  // the plane: nodal density, nodal field, CG work arrays, quadrature tables
+  FemParams fem{};
+  uint32_t *q4 = nullptr, *fem_iters = nullptr;
+  double *fem_buf = nullptr;  // n_node, pot, r, p, q, then the two tables
+  double *n_node = nullptr, *pot = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr;
+  double *tab_sp = nullptr, *tab_vl = nullptr;
+  double kick_coef = 0.0;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[T_NCLASS];
@@ -87,6 +94,15 @@ int hip_status(hipError_t e) { return e == hipSuccess ? MCPM_OK : MCPM_ERR_HIP; 
       return MCPM_ERR_HIP;                                                         \
     }                                                                              \
   } while (0)
+
+// Call FN<R>(args) with R the context's particle record: the slab's Rec or
+// the plane's Rec2, FP32 or FP64.
+#define MCPM_DISPATCH(c, FN, ...)                                                    \
+  ((c)->p.dims == 2                                                                  \
+       ? ((c)->p.precision == MCPM_FP32 ? FN<Rec2<float>>(__VA_ARGS__)                \
+                                        : FN<Rec2<double>>(__VA_ARGS__))              \
+       : ((c)->p.precision == MCPM_FP32 ? FN<Rec<float>>(__VA_ARGS__)                 \
+                                        : FN<Rec<double>>(__VA_ARGS__)))
 
 template <typename T>
 int dalloc(T **ptr, size_t count) {
@@ -128,29 +144,40 @@ struct Timer {
   }
 };
 
-// Gauss-Legendre nodes and weights on [-1,1] by Newton iteration on P_n,
-// rounded to 8 decimals: the reference tabulates its 50-point rule with 8
-// decimals (Main_Functions.cpp:3328-3345) and parity needs the same rule.
-void gauss_legendre_8dec(int n, double *x, double *w) {
+// Gauss-Legendre nodes and weights on [-1,1], ascending, by Newton
+// iteration on P_n (full double precision: the reference's 100-point table
+// of compute_Vlasov_2D, FEM.cpp:656-657).
+void gauss_legendre(int n, double *x, double *w) {
+  // long double Newton iteration: the weights near the interval ends are
+  // sensitive to the last bits of the nodes
   for (int i = 0; i < n; ++i) {
-    double z = cos(PI * (i + 0.75) / (n + 0.5));
-    double pp = 0.0;
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+    long double pnm1 = 0.0L;
     for (int it = 0; it < 100; ++it) {
-      double p1 = 1.0, p2 = 0.0;
+      long double p1 = 1.0L, p2 = 0.0L;
       for (int k = 1; k <= n; ++k) {
-        double p3 = p2;
+        long double p3 = p2;
         p2 = p1;
-        p1 = ((2.0 * k - 1.0) * z * p2 - (k - 1.0) * p3) / k;
+        p1 = ((2.0L * k - 1.0L) * z * p2 - (k - 1.0L) * p3) / k;
       }
-      pp = n * (z * p1 - p2) / (z * z - 1.0);
-      double z1 = z;
+      // p1 = P_n(z), p2 = P_{n-1}(z)
+      long double pp = n * (z * p1 - p2) / (z * z - 1.0L);
+      long double z1 = z;
       z = z1 - p1 / pp;
-      if (fabs(z - z1) < 1e-16) break;
+      pnm1 = p2;
+      if (fabsl(z - z1) < 1e-19L) break;
     }
-    // ascending order: node i counts from -1
-    x[n - 1 - i] = z;
-    w[n - 1 - i] = 2.0 / ((1.0 - z * z) * pp * pp);
+    // ascending order: node i counts from -1; w = 2 (1 - z^2) / (n P_{n-1})^2
+    x[n - 1 - i] = (double)z;
+    w[n - 1 - i] = (double)(2.0L * ((1.0L - z) * (1.0L + z)) / ((n * pnm1) * (n * pnm1)));
   }
+}
+
+// The same rule rounded to 8 decimals: the reference tabulates its 50-point
+// rule so (Main_Functions.cpp:3328-3345, FEM.cpp:366-380) and parity needs
+// the same numbers.
+void gauss_legendre_8dec(int n, double *x, double *w) {
+  gauss_legendre(n, x, w);
   for (int i = 0; i < n; ++i) {
     x[i] = nearbyint(x[i] * 1e8) / 1e8;
     w[i] = nearbyint(w[i] * 1e8) / 1e8;
@@ -274,6 +301,71 @@ int build_vlasov_table(mcpm_ctx *c) {
   return MCPM_OK;
 }
 
+// The plane's field set-up: deposit and solver constants, node arrays and
+// the quadrature tables of the wall Green's function (50^3 points, r in
+// [0, 3 sigma], FEM.cpp:344-394) and of the direct integral (100^3 points,
+// r in [sigma, 3 sigma], FEM.cpp:640-672).
+int build_fem(mcpm_ctx *c) {
+  const mcpm_params &p = c->p;
+  FemParams &f = c->fem;
+  f.nx = p.n_cells_x;
+  f.ny = p.n_cells;
+  f.per_x = p.boundary_x == MCPM_BC_PERIODIC;
+  f.per_y = p.boundary == MCPM_BC_PERIODIC;
+  f.fnv = p.Fn / p.cell_volume;
+  f.lam2V = pow(p.bst, 2.0) * p.cell_volume;
+  f.volume = p.cell_volume;
+  // FEM.cpp:497-502
+  const double ls = p.sigma * p.bst;
+  double d = 1.0 - (ls + 1.0) * exp(-p.bst * p.sigma);
+  f.core0 = p.bst > 0.0 ? d / p.bst / p.bst : 0.0;
+  double d2 = (-1.0 / 6.0 * pow(p.bst, 3) * pow(p.sigma, 3) - 1.0 / 2.0 * pow(p.bst, 2) * pow(p.sigma, 2) -
+               p.bst * p.sigma + exp(p.bst * p.sigma) - 1) *
+              exp(-p.bst * p.sigma) / pow(p.bst, 4);
+  f.core2 = p.bst > 0.0 ? d2 / pow(c->kp.dx, 2) : 0.0;
+  f.tol2 = 1e-26;
+  f.maxit = 20000;
+  f.nq = 0;
+  const size_t nn = (size_t)(f.nx + 1) * (f.ny + 1);
+  int st = dalloc(&c->fem_buf, nn * 5 + 2 * QT_N * 128);
+  if (st) return st;
+  c->n_node = c->fem_buf;
+  c->pot = c->n_node + nn;
+  c->cg_r = c->pot + nn;
+  c->cg_p = c->cg_r + nn;
+  c->cg_q = c->cg_p + nn;
+  c->tab_sp = c->cg_q + nn;
+  c->tab_vl = c->tab_sp + QT_N * 128;
+  if ((st = dalloc(&c->q4, (size_t)c->nc * 4 + 2))) return st;
+  c->fem_iters = c->q4 + (size_t)c->nc * 4;
+  HIPCHK(hipMemset(c->fem_buf, 0, nn * 5 * sizeof(double)));
+  for (int which = 0; which < 2; ++which) {
+    const int nq = which == 0 ? 50 : 100;
+    std::vector<double> xi(nq), wi(nq), t((size_t)QT_N * nq);
+    if (which == 0) gauss_legendre_8dec(nq, xi.data(), wi.data());
+    else gauss_legendre(nq, xi.data(), wi.data());
+    const double aa = which == 0 ? 0.0 : p.sigma, bb = 3.0 * p.sigma;
+    const double cc = 0.0, dd = 2 * PI, ee = 0.0, ff = PI;
+    for (int k = 0; k < nq; ++k) {
+      const double rr = (bb - aa) / 2.0 * xi[k] + (aa + bb) / 2.0;
+      const double dr = wi[k] * (bb - aa) / 2.0;
+      const double ph = (dd - cc) / 2.0 * xi[k] + (cc + dd) / 2.0;
+      const double th = (ff - ee) / 2.0 * xi[k] + (ff + ee) / 2.0;
+      t[QT_R * nq + k] = rr;
+      t[QT_RAD * nq + k] = which == 0 ? exp(-p.bst * rr) / (4.0 * PI) * rr * dr      // FEM.cpp:216
+                                      : pow(p.sigma / rr, 6.0) * rr * rr * dr;      // FEM.cpp:166
+      t[QT_CPH * nq + k] = cos(ph);
+      t[QT_SPH * nq + k] = sin(ph);
+      t[QT_DPH * nq + k] = wi[k] * (dd - cc) / 2.0;
+      t[QT_STH * nq + k] = sin(th);
+      t[QT_DTH * nq + k] = wi[k] * (ff - ee) / 2.0;
+    }
+    HIPCHK(hipMemcpy(which == 0 ? c->tab_sp : c->tab_vl, t.data(), t.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+  }
+  return MCPM_OK;
+}
+
 int check_err(mcpm_ctx *c) {
   uint32_t e = 0;
   HIPCHK(hipMemcpyAsync(&e, c->err, sizeof e, hipMemcpyDeviceToHost, c->s));
@@ -295,7 +387,7 @@ int chunk_for(int64_t n) {
   return (int)ch;
 }
 
-template <typename real>
+template <typename R>
 int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   if (c->n == 0) {
     HIPCHK(hipMemsetAsync(c->cnt, 0, c->nc * sizeof(uint32_t), c->s));
@@ -310,15 +402,16 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   if (nch > c->nchunks_cap) return MCPM_ERR_CAPACITY;
   const int nseg = (int)((nch + c->seg - 1) / c->seg);
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
-  const Rec<real> *in = (const Rec<real> *)c->P[c->cur];
-  Rec<real> *out = (Rec<real> *)c->P[1 - c->cur];
+  const R *in = (const R *)c->P[c->cur];
+  R *out = (R *)c->P[1 - c->cur];
+  const Kick kick{c->F2, c->pot, c->kick_coef};
   const size_t lds = (size_t)nc * sizeof(uint32_t);
   const double *drift = (c->p.model == MCPM_MODEL_FP_DENSE && c->have_closures &&
                          (mode & PUSH_MOVE))
                             ? c->drift
                             : nullptr;
-  hipLaunchKernelGGL((k_push<real, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
-                     nullptr, c->n, chunk, c->kp, c->F2, drift, mode, slo, shi, c->table,
+  hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
+                     nullptr, c->n, chunk, c->kp, kick, drift, mode, slo, shi, c->table,
                      nullptr, c->err);
   LAUNCHCHK();
   dim3 gs((nc + 255) / 256, nseg);
@@ -328,8 +421,8 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                      c->start, c->nden, c->yc, c->kp);
   hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
                      c->seg);
-  hipLaunchKernelGGL((k_push<real, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
-                     c->n, chunk, c->kp, c->F2, drift, mode, slo, shi, c->table, c->start,
+  hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
+                     c->n, chunk, c->kp, kick, drift, mode, slo, shi, c->table, c->start,
                      c->err);
   LAUNCHCHK();
   c->cur = 1 - c->cur;
@@ -338,8 +431,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
 }
 
 int run_sort(mcpm_ctx *c, int mode, int64_t step) {
-  return c->p.precision == MCPM_FP32 ? run_sort_t<float>(c, mode, step)
-                                     : run_sort_t<double>(c, mode, step);
+  return MCPM_DISPATCH(c, run_sort_t, c, mode, step);
 }
 
 int ensure_ops(mcpm_ctx *c, int64_t m) {
@@ -367,11 +459,11 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   return MCPM_OK;
 }
 
-template <typename real, int MODEL>
+template <typename R, int MODEL>
 int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   const int nc = c->nc;
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
-  Rec<real> *P = (Rec<real> *)c->P[c->cur];
+  R *P = (R *)c->P[c->cur];
   hipLaunchKernelGGL(k_mcand, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->cnt, c->omax,
                      c->nden, c->mcand, nc, c->kp, (int)MODEL, slo, shi);
   hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->mcand, c->cstart, nc);
@@ -387,7 +479,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
                               c->kp.key1);
   {
     Timer tm(c, T_GEN);
-    hipLaunchKernelGGL((k_gen<real, MODEL>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
+    hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
                        c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->opA, c->res[0],
                        c->counters);
     LAUNCHCHK();
@@ -401,7 +493,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     OpLog lg = c->debug_log ? c->log : OpLog{};
     int next_check = 12;
     for (int r = 0; r < MAX_ROUNDS; ++r) {
-      hipLaunchKernelGGL((k_round<real, MODEL>), dim3(grid), dim3(256), 0, c->s, r, M, P,
+      hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, M, P,
                          (const uint4 *)((r & 1) ? c->opB : c->opA), (r & 1) ? c->opA : c->opB,
                          c->res[r & 1], c->res[(r + 1) & 1], c->counters,
                          c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg);
@@ -440,6 +532,80 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   return MCPM_OK;
 }
 
+template <typename R>
+int collide_esmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  return collide_t<R, COLL_ESMC>(c, step, stats);
+}
+template <typename R>
+int collide_dsmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  return collide_t<R, COLL_DSMC>(c, step, stats);
+}
+
+template <typename R>
+int launch_moments(mcpm_ctx *c) {
+  hipLaunchKernelGGL((k_moments<R>), dim3(c->nc), dim3(256), 0, c->s, (const R *)c->P[c->cur],
+                     c->cnt, c->start, c->moments, c->kp);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
+template <typename R>
+int launch_sample(mcpm_ctx *c) {
+  hipLaunchKernelGGL((k_sample<R>), dim3(c->nc), dim3(256), 0, c->s, (const R *)c->P[c->cur],
+                     c->cnt, c->start, c->samples);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
+// Nodal density of the plane (cell_update's deposit) from the sorted particles.
+template <typename R>
+int launch_nodes(mcpm_ctx *c) {
+  if constexpr (sizeof(R) == 8 * sizeof(typename R::real_type)) {
+    hipLaunchKernelGGL((k_quadrants<R>), dim3(c->nc), dim3(256), 0, c->s,
+                       (const R *)c->P[c->cur], c->cnt, c->start, c->q4, c->kp);
+    const int nn = (c->fem.nx + 1) * (c->fem.ny + 1);
+    hipLaunchKernelGGL(k_nodes, dim3((nn + 255) / 256), dim3(256), 0, c->s, c->q4, c->n_node,
+                       c->fem);
+    LAUNCHCHK();
+    return MCPM_OK;
+  } else {
+    return MCPM_ERR_ARG;
+  }
+}
+
+// compute_SP_2D (FEM.cpp:249-610) up to the nodal field; the kick is the push's.
+int sp_2d(mcpm_ctx *c) {
+  int st = MCPM_DISPATCH(c, launch_nodes, c);
+  if (st) return st;
+  FemParams f = c->fem;
+  const int nn = (f.nx + 1) * (f.ny + 1);
+  if (!f.per_x) {
+    f.nq = 50;
+    const int nb = 2 * (f.nx + 1) + 2 * (f.ny - 1);
+    hipLaunchKernelGGL(k_node_integral, dim3(nb), dim3(256), 0, c->s, c->nden, c->tab_sp, c->pot,
+                       c->kp, f, 1);
+  }
+  hipLaunchKernelGGL(k_fem_cg, dim3(1), dim3(1024), 0, c->s, c->n_node, c->pot, c->cg_r,
+                     c->cg_p, c->cg_q, f, c->fem_iters);
+  hipLaunchKernelGGL(k_fem_core, dim3((nn + 255) / 256), dim3(256), 0, c->s, c->n_node, c->pot,
+                     f);
+  LAUNCHCHK();
+  c->kick_coef = c->p.ast;
+  return MCPM_OK;
+}
+
+// compute_Vlasov_2D (FEM.cpp:613-726) up to the nodal field.
+int vlasov_2d(mcpm_ctx *c) {
+  FemParams f = c->fem;
+  f.nq = 100;
+  const int nn = (f.nx + 1) * (f.ny + 1);
+  hipLaunchKernelGGL(k_node_integral, dim3(nn), dim3(256), 0, c->s, c->nden, c->tab_vl, c->pot,
+                     c->kp, f, 0);
+  LAUNCHCHK();
+  c->kick_coef = c->p.phi;
+  return MCPM_OK;
+}
+
 double contact_y_host(const mcpm_params *p, double n) { return contact_y(n, p->b, p->y_model); }
 
 }  // namespace
@@ -460,10 +626,28 @@ const char *mcpm_status_string(int s) {
   }
 }
 
-int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
-  if (!p || !out) return MCPM_ERR_ARG;
+int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
+  if (!p_in || !out) return MCPM_ERR_ARG;
   *out = nullptr;
-  if (p->n_cells < 1 || p->n_cells > 16384) return MCPM_ERR_ARG;
+  mcpm_params pv = *p_in;
+  if (pv.dims == 0) pv.dims = 1;
+  const mcpm_params *p = &pv;
+  if (p->dims != 1 && p->dims != 2) return MCPM_ERR_ARG;
+  const int64_t nx = p->dims == 2 ? p->n_cells_x : 1;
+  if (p->n_cells < 1 || nx < 1 || nx * p->n_cells > 16384) return MCPM_ERR_ARG;
+  if (p->dims == 2) {
+    // the plane: specular walls or periodic axes, hard-sphere or DSMC
+    // collisions, direct or screened-Poisson attraction (FEM.cpp)
+    if (!(p->length_x > 0.0)) return MCPM_ERR_ARG;
+    if (p->boundary == MCPM_BC_DIFFUSE || p->boundary_x == MCPM_BC_DIFFUSE) return MCPM_ERR_ARG;
+    if (p->boundary < 0 || p->boundary > 2 || p->boundary_x < 0 || p->boundary_x > 2)
+      return MCPM_ERR_ARG;
+    if (p->model == MCPM_MODEL_FP_DENSE || p->model == MCPM_MODEL_FP_IDEAL) return MCPM_ERR_ARG;
+    if (p->long_range == MCPM_LR_DENSITY_EXPANSION) return MCPM_ERR_ARG;
+    if (p->long_range != MCPM_LR_NONE && (nx < 2 || p->n_cells < 2)) return MCPM_ERR_ARG;
+    if (p->long_range == MCPM_LR_SCREENED_POISSON && p->boundary != p->boundary_x)
+      return MCPM_ERR_ARG;
+  }
   if (p->capacity < 1 || p->capacity > 2000000000LL) return MCPM_ERR_ARG;
   if (p->precision != MCPM_FP32 && p->precision != MCPM_FP64) return MCPM_ERR_ARG;
   if (p->boundary < 0 || p->boundary > 2) return MCPM_ERR_ARG;
@@ -476,14 +660,20 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCPM_ERR_HIP;
   mcpm_ctx *c = new mcpm_ctx();
   c->p = *p;
-  c->nc = p->n_cells;
+  c->nc = (int)(nx * p->n_cells);
   HIPCHK(hipGetDevice(&c->device));
   HIPCHK(hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking));
   KParams &k = c->kp;
-  k.n_cells = p->n_cells;
+  k.n_cells = c->nc;
   k.boundary = p->boundary;
   k.y_model = p->y_model;
+  k.dims = p->dims;
+  k.nx = (int32_t)nx;
+  k.ny = p->n_cells;
+  k.boundary_x = p->dims == 2 ? p->boundary_x : MCPM_BC_PERIODIC;
   k.reserved = 0;
+  k.length_x = p->dims == 2 ? p->length_x : 1.0;
+  k.dx = k.length_x / (double)nx;
   k.dy = p->length / (double)p->n_cells;
   k.length = p->length;
   k.wall_lo = p->wall_lo;
@@ -509,7 +699,7 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
   c->fpp.T0 = p->T0;
   c->fpp.visp = p->visp;
   c->fpp.crref = p->crref;
-  c->rec = 4 * (p->precision == MCPM_FP32 ? sizeof(float) : sizeof(double));
+  c->rec = (p->dims == 2 ? 8 : 4) * (p->precision == MCPM_FP32 ? sizeof(float) : sizeof(double));
   const int nc = c->nc;
   int st = MCPM_OK;
   auto fail = [&](int s2) {
@@ -551,9 +741,13 @@ int mcpm_create(const mcpm_params *p, mcpm_ctx **out) {
       hipMemset(c->res[0], 0xFF, (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->res[1], 0xFF, (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess)
     return fail(MCPM_ERR_HIP);
-  if (p->long_range == MCPM_LR_SCREENED_POISSON)
-    if ((st = build_field(c))) return fail(st);
-  if ((st = build_vlasov_table(c))) return fail(st);
+  if (p->dims == 2) {
+    if ((st = build_fem(c))) return fail(st);
+  } else {
+    if (p->long_range == MCPM_LR_SCREENED_POISSON)
+      if ((st = build_field(c))) return fail(st);
+    if ((st = build_vlasov_table(c))) return fail(st);
+  }
   if ((st = dalloc(&c->kde, (size_t)nc * 3))) return fail(st);
   if (hipDeviceSynchronize() != hipSuccess) return fail(MCPM_ERR_HIP);
   *out = c;
@@ -566,7 +760,8 @@ int mcpm_destroy(mcpm_ctx *c) {
   void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
                   c->samples, c->counters, c->err, c->res[0], c->res[1], c->opA,
-                  c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde};
+                  c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde,
+                  c->fem_buf, c->q4};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < T_NCLASS; ++k)
@@ -586,28 +781,36 @@ int mcpm_sync(mcpm_ctx *c) {
   return check_err(c);
 }
 
-int mcpm_upload(mcpm_ctx *c, int64_t n, const double *y, const double *u1, const double *u2,
-                const double *u3) {
-  if (!c || n < 0 || n > c->p.capacity) return n > 0 && c ? MCPM_ERR_CAPACITY : MCPM_ERR_ARG;
-  if (n > 0 && (!y || !u1 || !u2 || !u3)) return MCPM_ERR_ARG;
-  std::vector<char> host(c->rec * (size_t)n);
-  if (c->p.precision == MCPM_FP32) {
-    float *h = (float *)host.data();
-    for (int64_t k = 0; k < n; ++k) {
-      h[4 * k] = (float)y[k];
-      h[4 * k + 1] = (float)u1[k];
-      h[4 * k + 2] = (float)u2[k];
-      h[4 * k + 3] = (float)u3[k];
-    }
-  } else {
-    double *h = (double *)host.data();
-    for (int64_t k = 0; k < n; ++k) {
-      h[4 * k] = y[k];
-      h[4 * k + 1] = u1[k];
-      h[4 * k + 2] = u2[k];
-      h[4 * k + 3] = u3[k];
-    }
+}  // extern "C"
+
+namespace {
+// Pack host SoA arrays into the context's records: (y, u1, u2, u3) for the
+// slab, (x, y, u1, u2, u3, 0, 0, 0) for the plane.
+template <typename real>
+void pack(std::vector<char> &host, int64_t n, int stride, const double *const *src, int nsrc) {
+  real *h = (real *)host.data();
+  for (int64_t k = 0; k < n; ++k) {
+    for (int q = 0; q < nsrc; ++q) h[stride * k + q] = (real)src[q][k];
+    for (int q = nsrc; q < stride; ++q) h[stride * k + q] = (real)0;
   }
+}
+template <typename real>
+void unpack(const std::vector<char> &host, int64_t n, int stride, double *const *dst, int ndst) {
+  const real *h = (const real *)host.data();
+  for (int64_t k = 0; k < n; ++k)
+    for (int q = 0; q < ndst; ++q)
+      if (dst[q]) dst[q][k] = (double)h[stride * k + q];
+}
+
+int upload_any(mcpm_ctx *c, int64_t n, const double *const *src, int nsrc) {
+  if (n < 0) return MCPM_ERR_ARG;
+  if (n > c->p.capacity) return MCPM_ERR_CAPACITY;
+  for (int q = 0; q < nsrc; ++q)
+    if (n > 0 && !src[q]) return MCPM_ERR_ARG;
+  std::vector<char> host(c->rec * (size_t)n);
+  const int stride = c->p.dims == 2 ? 8 : 4;
+  if (c->p.precision == MCPM_FP32) pack<float>(host, n, stride, src, nsrc);
+  else pack<double>(host, n, stride, src, nsrc);
   HIPCHK(hipStreamSynchronize(c->s));
   if (n > 0)
     HIPCHK(hipMemcpy(c->P[c->cur], host.data(), host.size(), hipMemcpyHostToDevice));
@@ -617,28 +820,49 @@ int mcpm_upload(mcpm_ctx *c, int64_t n, const double *y, const double *u1, const
   return MCPM_OK;
 }
 
-int mcpm_download(mcpm_ctx *c, double *y, double *u1, double *u2, double *u3) {
-  if (!c) return MCPM_ERR_ARG;
+int download_any(mcpm_ctx *c, double *const *dst, int ndst) {
   const int64_t n = c->n;
   std::vector<char> host(c->rec * (size_t)(n > 0 ? n : 1));
   HIPCHK(hipStreamSynchronize(c->s));
   if (n > 0)
     HIPCHK(hipMemcpy(host.data(), c->P[c->cur], c->rec * (size_t)n, hipMemcpyDeviceToHost));
-  for (int64_t k = 0; k < n; ++k) {
-    double v[4];
-    if (c->p.precision == MCPM_FP32) {
-      const float *h = (const float *)host.data() + 4 * k;
-      for (int q = 0; q < 4; ++q) v[q] = h[q];
-    } else {
-      const double *h = (const double *)host.data() + 4 * k;
-      for (int q = 0; q < 4; ++q) v[q] = h[q];
-    }
-    if (y) y[k] = v[0];
-    if (u1) u1[k] = v[1];
-    if (u2) u2[k] = v[2];
-    if (u3) u3[k] = v[3];
-  }
+  const int stride = c->p.dims == 2 ? 8 : 4;
+  if (c->p.precision == MCPM_FP32) unpack<float>(host, n, stride, dst, ndst);
+  else unpack<double>(host, n, stride, dst, ndst);
   return MCPM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int mcpm_upload(mcpm_ctx *c, int64_t n, const double *y, const double *u1, const double *u2,
+                const double *u3) {
+  if (!c || c->p.dims == 2) return MCPM_ERR_ARG;
+  const double *src[4] = {y, u1, u2, u3};
+  return upload_any(c, n, src, 4);
+}
+
+int mcpm_upload_xy(mcpm_ctx *c, int64_t n, const double *x, const double *y, const double *u1,
+                   const double *u2, const double *u3) {
+  if (!c || c->p.dims != 2) return MCPM_ERR_ARG;
+  const double *src[5] = {x, y, u1, u2, u3};
+  return upload_any(c, n, src, 5);
+}
+
+int mcpm_download(mcpm_ctx *c, double *y, double *u1, double *u2, double *u3) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims == 2) {
+    double *dst[5] = {nullptr, y, u1, u2, u3};
+    return download_any(c, dst, 5);
+  }
+  double *dst[4] = {y, u1, u2, u3};
+  return download_any(c, dst, 4);
+}
+
+int mcpm_download_xy(mcpm_ctx *c, double *x, double *y, double *u1, double *u2, double *u3) {
+  if (!c || c->p.dims != 2) return MCPM_ERR_ARG;
+  double *dst[5] = {x, y, u1, u2, u3};
+  return download_any(c, dst, 5);
 }
 
 int64_t mcpm_particle_count(const mcpm_ctx *c) { return c ? c->n : -1; }
@@ -657,8 +881,7 @@ int mcpm_collide_esmc(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     if (stats) memset(stats, 0, sizeof *stats);
     return MCPM_OK;
   }
-  return c->p.precision == MCPM_FP32 ? collide_t<float, COLL_ESMC>(c, step, stats)
-                                     : collide_t<double, COLL_ESMC>(c, step, stats);
+  return MCPM_DISPATCH(c, collide_esmc_t, c, step, stats);
 }
 
 int mcpm_collide_dsmc_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
@@ -668,19 +891,15 @@ int mcpm_collide_dsmc_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats
     if (stats) memset(stats, 0, sizeof *stats);
     return MCPM_OK;
   }
-  return c->p.precision == MCPM_FP32 ? collide_t<float, COLL_DSMC>(c, step, stats)
-                                     : collide_t<double, COLL_DSMC>(c, step, stats);
+  return MCPM_DISPATCH(c, collide_dsmc_t, c, step, stats);
 }
 
 int mcpm_fp_closures(mcpm_ctx *c, int64_t step) {
   if (!c) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
-  if (c->p.precision == MCPM_FP32)
-    hipLaunchKernelGGL((k_moments<float>), dim3(c->nc), dim3(256), 0, c->s,
-                       (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
-  else
-    hipLaunchKernelGGL((k_moments<double>), dim3(c->nc), dim3(256), 0, c->s,
-                       (const Rec<double> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
+  if (c->p.dims != 1) return MCPM_ERR_ARG;
+  int st = MCPM_DISPATCH(c, launch_moments, c);
+  if (st) return st;
   hipLaunchKernelGGL(k_fp_closures, dim3((c->nc + 63) / 64), dim3(64), 0, c->s, c->moments,
                      c->cnt, c->nden, c->yc, c->clos, c->drift, c->kp, c->fpp, step, c->err);
   LAUNCHCHK();
@@ -690,7 +909,7 @@ int mcpm_fp_closures(mcpm_ctx *c, int64_t step) {
 
 int mcpm_velocity_fp(mcpm_ctx *c, int64_t step) {
   if (!c) return MCPM_ERR_ARG;
-  if (!c->have_closures) return MCPM_ERR_STATE;
+  if (!c->have_closures || c->p.dims != 1) return MCPM_ERR_STATE;
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
   if (c->p.precision == MCPM_FP32)
     hipLaunchKernelGGL((k_fp_velocity<float>), dim3(c->nc), dim3(256), 0, c->s,
@@ -718,6 +937,12 @@ int mcpm_poisson_solver(mcpm_ctx *c) {
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->p.long_range != MCPM_LR_SCREENED_POISSON) return MCPM_ERR_ARG;
   Timer tm(c, T_FIELD);
+  if (c->p.dims == 2) {
+    int st = sp_2d(c);
+    if (st) return st;
+    c->kick_pending = true;
+    return MCPM_OK;
+  }
   hipLaunchKernelGGL(k_field, dim3(1), dim3(1024), 0, c->s, c->nden, c->fa, c->F2, c->kp,
                      c->p.ast, c->p.bst, c->p.eqdist);
   LAUNCHCHK();
@@ -729,6 +954,12 @@ int mcpm_vlasov_integral(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
   Timer tm(c, T_FIELD);
+  if (c->p.dims == 2) {
+    int st = vlasov_2d(c);
+    if (st) return st;
+    c->kick_pending = true;
+    return MCPM_OK;
+  }
   const int nf = c->p.boundary == MCPM_BC_PERIODIC ? c->nc : c->nc + 1;
   hipLaunchKernelGGL(k_vlasov_faces, dim3(nf), dim3(256), 0, c->s, c->nden, c->lr_tab, c->lr_V,
                      c->kp);
@@ -740,6 +971,7 @@ int mcpm_vlasov_integral(mcpm_ctx *c) {
 }
 
 static int run_kde(mcpm_ctx *c) {
+  if (c->p.dims != 1) return MCPM_ERR_ARG;
   if (c->p.precision == MCPM_FP32)
     hipLaunchKernelGGL((k_kde<float>), dim3(c->nc), dim3(256), 0, c->s,
                        (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->kde,
@@ -830,13 +1062,8 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
 int mcpm_moments(mcpm_ctx *c, double *out) {
   if (!c || !out) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
-  if (c->p.precision == MCPM_FP32)
-    hipLaunchKernelGGL((k_moments<float>), dim3(c->nc), dim3(256), 0, c->s,
-                       (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
-  else
-    hipLaunchKernelGGL((k_moments<double>), dim3(c->nc), dim3(256), 0, c->s,
-                       (const Rec<double> *)c->P[c->cur], c->cnt, c->start, c->moments, c->kp);
-  LAUNCHCHK();
+  int st = MCPM_DISPATCH(c, launch_moments, c);
+  if (st) return st;
   HIPCHK(hipMemcpyAsync(out, c->moments, (size_t)c->nc * MCPM_NMOM * sizeof(double),
                         hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
@@ -846,13 +1073,22 @@ int mcpm_moments(mcpm_ctx *c, double *out) {
 int mcpm_sample(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
-  if (c->p.precision == MCPM_FP32)
-    hipLaunchKernelGGL((k_sample<float>), dim3(c->nc), dim3(256), 0, c->s,
-                       (const Rec<float> *)c->P[c->cur], c->cnt, c->start, c->samples);
-  else
-    hipLaunchKernelGGL((k_sample<double>), dim3(c->nc), dim3(256), 0, c->s,
-                       (const Rec<double> *)c->P[c->cur], c->cnt, c->start, c->samples);
-  LAUNCHCHK();
+  return MCPM_DISPATCH(c, launch_sample, c);
+}
+
+int mcpm_get_nodes(mcpm_ctx *c, double *n_node, double *potential) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims != 2) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  const size_t nn = (size_t)(c->fem.nx + 1) * (c->fem.ny + 1);
+  if (n_node) {
+    int st = MCPM_DISPATCH(c, launch_nodes, c);
+    if (st) return st;
+    HIPCHK(hipMemcpyAsync(n_node, c->n_node, nn * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  }
+  if (potential)
+    HIPCHK(hipMemcpyAsync(potential, c->pot, nn * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
   return MCPM_OK;
 }
 

@@ -17,15 +17,37 @@ namespace mcpm {
 // (FP64) span instead of four separate arrays.
 template <typename real>
 struct alignas(4 * sizeof(real)) Rec {
+  using real_type = real;
   real y, u1, u2, u3;
 };
 
-// Constants every slab kernel reads, passed by value.
+// One particle of the 2D problems (the reference's x1, x2, U1, U2, U3),
+// padded to 8 components so that a record is one aligned 32/64-byte span.
+template <typename real>
+struct alignas(8 * sizeof(real)) Rec2 {
+  using real_type = real;
+  real x, y, u1, u2, u3, p0, p1, p2;
+};
+
+// What the push's kick reads: the per-cell slab force F2 (1D) or the nodal
+// potential and its coefficient (2D, compute_SP_2D / compute_Vlasov_2D).
+struct Kick {
+  const double *F2;
+  const double *xnode;
+  double coef;
+};
+
+// Constants every kernel reads, passed by value.  n_cells is the total
+// number of cells (nx * ny); 1D slabs have nx = 1.
 struct KParams {
   int32_t n_cells;
-  int32_t boundary;
+  int32_t boundary;     // along y
   int32_t y_model;
+  int32_t dims;
+  int32_t nx, ny;       // cells along x (1 in 1D) and y
+  int32_t boundary_x;   // along x (2D)
   int32_t reserved;
+  double dx, length_x;  // 2D
   double dy, length, wall_lo, wall_hi;
   double T_lo, T_hi, U_lo, U_hi;
   double sigma, mass, kb, b, Fn, volume, n_ratio, dt;

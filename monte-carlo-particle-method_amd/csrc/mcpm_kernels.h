@@ -43,9 +43,10 @@ struct FieldArrays {
 // collision models sharing the reservation rounds
 enum : int { COLL_ESMC = 0, COLL_DSMC = 1 };
 
-template <typename real, bool SCATTER>
-__global__ void k_push(const Rec<real> *in, Rec<real> *out, int64_t n, int32_t chunk,
-                       KParams kp, const double *F2, const double *drift, int mode,
+// R is Rec<float|double> (1D slab) or Rec2<float|double> (2D plane).
+template <typename R, bool SCATTER>
+__global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp, Kick kick,
+                       const double *drift, int mode,
                        uint32_t step_lo, uint32_t step_hi, uint32_t *table,
                        const uint32_t *start, uint32_t *err);
 __global__ void k_scan_a(const uint32_t *table, uint32_t *partial, int nchunks, int nc, int seg);
@@ -56,13 +57,13 @@ __global__ void k_mcand(const uint32_t *cnt, const double *omax, const double *n
                         uint32_t *mcand, int nc, KParams kp, int model, uint32_t step_lo,
                         uint32_t step_hi);
 __global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc);
-template <typename real, int MODEL>
-__global__ void k_gen(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
+template <typename R, int MODEL>
+__global__ void k_gen(const R *P, const uint32_t *cnt, const uint32_t *start,
                       const uint32_t *mcand, const uint32_t *cstart, KParams kp,
                       uint32_t step_lo, uint32_t step_hi, uint32_t prio_key, uint4 *ops,
                       uint32_t *res0, uint32_t *counters);
-template <typename real, int MODEL>
-__global__ void k_round(int r, uint32_t count0, Rec<real> *P, const uint4 *in, uint4 *out,
+template <typename R, int MODEL>
+__global__ void k_round(int r, uint32_t count0, R *P, const uint4 *in, uint4 *out,
                         uint32_t *res_cur, uint32_t *res_next, uint32_t *counters,
                         const double *nden, const double *yc, const double *omax,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,
@@ -73,11 +74,11 @@ __global__ void k_zero_u32(uint32_t *p, int n);
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v);
 __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams kp, double ast,
                         double bst, double eqdist);
-template <typename real>
-__global__ void k_moments(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
+template <typename R>
+__global__ void k_moments(const R *P, const uint32_t *cnt, const uint32_t *start,
                           double *out, KParams kp);
-template <typename real>
-__global__ void k_sample(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
+template <typename R>
+__global__ void k_sample(const R *P, const uint32_t *cnt, const uint32_t *start,
                          double *acc);
 __global__ void k_fp_closures(const double *mom, const uint32_t *cnt, const double *nden,
                               const double *yc, double *clos, double *drift, KParams kp,
@@ -93,5 +94,32 @@ template <typename real>
 __global__ void k_kde(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start, double *ns,
                       double *d2, double *d3, KParams kp);
 __global__ void k_lr2_force(const double *d2, double *F2, KParams kp, double pref);
+
+// ---- the plane's mean field (mcpm_fem.hip) ----------------------------------
+// Constants of the nodal deposit and the Q1 screened-Poisson solve.
+struct FemParams {
+  int32_t nx, ny;          // cells per axis; nodes (nx+1)(ny+1), node (i,j) at j(nx+1)+i
+  int32_t per_x, per_y;    // periodic axes (1) or specular walls (0)
+  double fnv;              // Fn / V: one particle's share of a node density
+  double lam2V;            // lambda^2 V, the mass-matrix factor (FEM.cpp:301)
+  double volume;           // V, the right-hand side factor (FEM.cpp:303)
+  double core0, core2;     // core subtraction constants dummy, dummy2 (FEM.cpp:497-502)
+  double tol2;             // CG stop: |r|^2 <= tol2 |b|^2
+  int32_t maxit;
+  int32_t nq;              // quadrature points per dimension of the node integral
+};
+// Quadrature table layout of k_node_integral, nq entries each: r, radial
+// weight, cos(azimuth), sin(azimuth), azimuth weight, sin(polar), polar weight.
+enum : int { QT_R = 0, QT_RAD = 1, QT_CPH = 2, QT_SPH = 3, QT_DPH = 4, QT_STH = 5, QT_DTH = 6,
+             QT_N = 7 };
+template <typename R>
+__global__ void k_quadrants(const R *P, const uint32_t *cnt, const uint32_t *start, uint32_t *q4,
+                            KParams kp);
+__global__ void k_nodes(const uint32_t *q4, double *n_node, FemParams f);
+__global__ void k_node_integral(const double *nden, const double *tab, double *pot, KParams kp,
+                                FemParams f, int boundary_only);
+__global__ void k_fem_cg(const double *n_node, double *pot, double *r, double *p, double *q,
+                         FemParams f, uint32_t *iters);
+__global__ void k_fem_core(const double *n_node, double *pot, FemParams f);
 
 }  // namespace mcpm

@@ -10,6 +10,7 @@
 //
 // Every kernel is written for 64-lane wavefronts.
 #include "mcpm_kernels.h"
+#include "mcpm_geom.h"
 
 #pragma clang fp contract(off)
 
@@ -26,14 +27,14 @@ namespace mcpm {
 // wall with the new velocity.  Draws: Philox(slot, bounce, step, WALL).
 template <typename real>
 __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const KParams &kp,
-                                              const double *__restrict__ F2,
+                                              const Kick &kick,
                                               const double *__restrict__ drift, int mode,
                                               uint32_t step_lo, uint32_t step_hi,
                                               uint32_t *err) {
   double y = (double)r.y;
   if (mode & PUSH_KICK) {
     int32_t c = cell_of(y, kp.dy, kp.n_cells);
-    r.u2 = (real)((double)r.u2 + F2[c] * kp.dt);
+    r.u2 = (real)((double)r.u2 + kick.F2[c] * kp.dt);
   }
   if (!(mode & PUSH_MOVE)) return r;
   double v = (double)r.u2;
@@ -118,6 +119,89 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
   return r;
 }
 
+// Periodic wrap of one coordinate into [0, L) (stored type rounding included).
+template <typename real>
+__device__ __forceinline__ real wrap_store(double v, double L) {
+  if (v < 0.0) v += L;
+  else if (v >= L) v -= L;
+  real vr = (real)v;
+  if (!(vr < (real)L)) vr = (real)0;
+  if (!(vr >= (real)0)) vr = (real)0;
+  return vr;
+}
+
+// Derivatives of the bilinear shape functions (dNxi, dNeta, FEM.cpp:22-49)
+// for the nodes 0 (lower left), 1 (lower right), 2 (upper left), 3.
+__device__ __forceinline__ double dn_xi(int i, double xi, double eta) {
+  (void)xi;
+  return i == 0 ? eta - 1.0 : (i == 1 ? 1.0 - eta : (i == 2 ? -eta : eta));
+}
+__device__ __forceinline__ double dn_eta(int i, double xi, double eta) {
+  (void)eta;
+  return i == 0 ? xi - 1.0 : (i == 1 ? -xi : (i == 2 ? -xi + 1 : xi));
+}
+
+// 2D push: the FEM force kick of compute_SP_2D / compute_Vlasov_2D at the
+// particle (FEM.cpp:568-582, 709-721: U -= grad(Q1 interpolant) coef / m dt),
+// free flight in x and y, specular walls at 0 and L or periodic wrap per axis.
+template <typename real>
+__device__ __forceinline__ Rec2<real> push_one(Rec2<real> r, uint64_t slot, const KParams &kp,
+                                               const Kick &kick,
+                                               const double *__restrict__ drift, int mode,
+                                               uint32_t step_lo, uint32_t step_hi,
+                                               uint32_t *err) {
+  (void)slot; (void)drift; (void)step_lo; (void)step_hi;
+  if (mode & PUSH_KICK) {
+    const int32_t cx = cell_of((double)r.x, kp.dx, kp.nx);
+    const int32_t cy = cell_of((double)r.y, kp.dy, kp.ny);
+    // node coordinates as make_cells builds them (myIO.cpp:1455-1479)
+    const double x0 = (double)cx * kp.dx, x1 = (double)(cx + 1) * kp.dx;
+    const double y0 = (double)cy * kp.dy, y1 = (double)(cy + 1) * kp.dy;
+    const double dxc = x1 - x0, dyc = y1 - y0;
+    const double xn0 = 0.5 * (x0 + x1) - dxc / 2.0;
+    const double yn0 = 0.5 * (y0 + y1) - dyc / 2.0;
+    const double xi = ((double)r.x - xn0) / dxc;
+    const double eta = ((double)r.y - yn0) / dyc;
+    const int32_t w = kp.nx + 1;
+    const int32_t id[4] = {cy * w + cx, cy * w + cx + 1, (cy + 1) * w + cx, (cy + 1) * w + cx + 1};
+    double F1 = 0.0, F2 = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      F1 += dn_xi(q, xi, eta) * kick.xnode[id[q]];
+      F2 += dn_eta(q, xi, eta) * kick.xnode[id[q]];
+    }
+    F1 = F1 / kp.mass * kick.coef / dxc;
+    F2 = F2 / kp.mass * kick.coef / dyc;
+    r.u1 = (real)((double)r.u1 - F1 * kp.dt);
+    r.u2 = (real)((double)r.u2 - F2 * kp.dt);
+  }
+  if (!(mode & PUSH_MOVE)) return r;
+  // Position_FP_ideal_gas for both axes (Fokker_Planck.cpp:948-969)
+  double x = (double)r.x + (double)r.u1 * kp.dt;
+  double y = (double)r.y + (double)r.u2 * kp.dt;
+  const bool px = kp.boundary_x == MCPM_BC_PERIODIC, py = kp.boundary == MCPM_BC_PERIODIC;
+  double u1 = (double)r.u1, u2 = (double)r.u2;
+  if (!(px && py)) {
+    if (!specular_plane(&x, &u1, !px, kp.length_x, &y, &u2, !py, kp.length))
+      atomicOr(err, ERRBIT_BC);
+    r.u1 = (real)u1;
+    r.u2 = (real)u2;
+  }
+  r.x = px ? wrap_store<real>(x, kp.length_x) : (real)x;
+  r.y = py ? wrap_store<real>(y, kp.length) : (real)y;
+  return r;
+}
+
+template <typename real>
+__device__ __forceinline__ bool in_domain(const Rec<real> &r, const KParams &kp) {
+  double yy = (double)r.y;
+  return yy >= 0.0 && yy <= kp.length;
+}
+template <typename real>
+__device__ __forceinline__ bool in_domain(const Rec2<real> &r, const KParams &kp) {
+  double xx = (double)r.x, yy = (double)r.y;
+  return yy >= 0.0 && yy <= kp.length && xx >= 0.0 && xx <= kp.length_x;
+}
+
 // ---------------------------------------------------------------------------
 // Stable counting sort, pass 1 and pass 3.  One wavefront per chunk of
 // consecutive particles; the chunk's cell histogram lives in LDS.
@@ -128,11 +212,9 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
 // Particles of a chunk are processed in index order, so equal keys keep their
 // relative order: the sort is stable.
 // ---------------------------------------------------------------------------
-template <typename real, bool SCATTER>
-__global__ __launch_bounds__(64) void k_push(const Rec<real> *__restrict__ in,
-                                             Rec<real> *__restrict__ out, int64_t n,
-                                             int32_t chunk, KParams kp,
-                                             const double *__restrict__ F2,
+template <typename R, bool SCATTER>
+__global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__restrict__ out,
+                                             int64_t n, int32_t chunk, KParams kp, Kick kick,
                                              const double *__restrict__ drift, int mode,
                                              uint32_t step_lo, uint32_t step_hi,
                                              uint32_t *__restrict__ table,
@@ -155,7 +237,7 @@ __global__ __launch_bounds__(64) void k_push(const Rec<real> *__restrict__ in,
   const uint64_t lt = lanemask_lt();
   constexpr int U = 4;  // tiles loaded ahead
   for (int64_t t0 = beg; t0 < end; t0 += 64 * U) {
-    Rec<real> rr[U];
+    R rr[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       int64_t p = t0 + u * 64 + lane;
@@ -166,15 +248,12 @@ __global__ __launch_bounds__(64) void k_push(const Rec<real> *__restrict__ in,
       int64_t p = t0 + u * 64 + lane;
       bool valid = p < end;
       int32_t key = 0;
-      Rec<real> r = rr[u];
+      R r = rr[u];
       if (valid) {
         if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
-        if (mode & PUSH_CHECK) {
-          double yy = (double)r.y;
-          if (!(yy >= 0.0) || !(yy <= kp.length)) atomicOr(err, ERRBIT_DOMAIN);
-        }
-        r = push_one<real>(r, (uint64_t)p, kp, F2, drift, mode, step_lo, step_hi, err);
-        key = cell_of((double)r.y, kp.dy, nc);
+        if ((mode & PUSH_CHECK) && !in_domain(r, kp)) atomicOr(err, ERRBIT_DOMAIN);
+        r = push_one(r, (uint64_t)p, kp, kick, drift, mode, step_lo, step_hi, err);
+        key = cell_key(r, kp);
       }
       uint64_t active = __ballot(valid);
       while (active) {
@@ -337,8 +416,8 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__
 // The candidate draw: i in cell c, isotropic direction, the partner cell J
 // at y_i + sigma dir_y (collisions.cpp:648-671; 1D rule of new4,
 // collisions.cpp:784-792), j in J, j != i (collisions.cpp:675-679).
-template <typename real>
-__device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const Rec<real> *P,
+template <typename R>
+__device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *P,
                                                const uint32_t *cnt, const uint32_t *start,
                                                const KParams &kp, uint32_t step_lo,
                                                uint32_t step_hi, uint32_t *oi, uint32_t *oj) {
@@ -348,17 +427,8 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const Rec
   uint32_t i = start[c] + mcpm_uint_below(w.v[0], Nc);
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-  double yJ = (double)P[i].y + kp.sigma * d1;
   int32_t J;
-  if (kp.boundary == MCPM_BC_PERIODIC) {
-    if (yJ < 0.0) yJ += kp.length;
-    else if (yJ >= kp.length) yJ -= kp.length;
-    J = cell_of(yJ, kp.dy, kp.n_cells);
-  } else {
-    double q = mcpm_floor(yJ / kp.dy);
-    if (!(q >= 0.0) || q >= (double)kp.n_cells) return false;
-    J = (int32_t)q;
-  }
+  if (!partner_cell(P[i], d0, d1, kp, &J)) return false;
   uint32_t NJ = cnt[J];
   if (NJ == 0) return false;
   uint32_t j = start[J] + mcpm_uint_below(w.v[3], NJ);
@@ -397,8 +467,8 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // that the first round streams operations in cell order; reserve both
 // particles.  ESMC priorities are the keyed bijection of the candidate
 // index; DSMC keeps the reference's in-cell candidate order.
-template <typename real, int MODEL>
-__global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
+template <typename R, int MODEL>
+__global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              const uint32_t *__restrict__ cnt,
                                              const uint32_t *__restrict__ start,
                                              const uint32_t *__restrict__ mcand,
@@ -420,7 +490,7 @@ __global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
         draw_dsmc_pair(c, t, cnt, start, kp, step_lo, step_hi, &i, &j);
         ok = true;
       } else {
-        ok = draw_candidate<real>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+        ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
       }
     }
     uint4 rec;
@@ -443,35 +513,24 @@ __global__ __launch_bounds__(256) void k_gen(const Rec<real> *__restrict__ P,
 // p = 4 pi sigma^2 (g.dir) Y n_J dt, omega_I = max(omega_I, p),
 // accept iff p / omega_max_I > U, hard-sphere exchange along dir.
 // The cell I of the candidate is the cell of particle i (i was drawn in it).
-template <typename real>
-__device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, Rec<real> *P,
+template <typename R>
+__device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R *P,
                                           const double *nden, const double *yc,
                                           const double *omax, double *omega_new,
                                           const KParams &kp, uint32_t step_lo,
                                           uint32_t step_hi) {
-  Rec<real> a = P[i];
-  Rec<real> bb = P[j];
-  double yi = (double)a.y, yj = (double)bb.y;
-  const uint32_t c = (uint32_t)cell_of(yi, kp.dy, kp.n_cells);
+  using T = typename R::real_type;
+  R a = P[i];
+  R bb = P[j];
+  const uint32_t c = (uint32_t)cell_key(a, kp);
   mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
                              kp.key1);
   mcpm_u32x4 w2 = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 1u), kp.key0,
                               kp.key1);
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-  double ymid;
-  if (kp.boundary == MCPM_BC_PERIODIC) {
-    double d = yj - yi;
-    if (d > 0.5 * kp.length) d -= kp.length;
-    else if (d < -0.5 * kp.length) d += kp.length;
-    ymid = yi + d / 2.0;
-    if (ymid < 0.0) ymid += kp.length;
-    else if (ymid >= kp.length) ymid -= kp.length;
-  } else {
-    ymid = (yi + yj) / 2.0;
-  }
-  int32_t cm = cell_of(ymid, kp.dy, kp.n_cells);
-  int32_t J = cell_of(yj, kp.dy, kp.n_cells);
+  int32_t cm = mid_cell(a, bb, kp);
+  int32_t J = cell_key(bb, kp);
   double Ymid = yc[cm];
   double g0 = (double)a.u1 - (double)bb.u1;
   double g1 = (double)a.u2 - (double)bb.u2;
@@ -480,12 +539,12 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, Re
   double pv = kp.coll_pref * gd * Ymid * nden[J] * kp.dt;
   atomic_max_pos_double(&omega_new[c], pv);
   if (pv / omax[c] > mcpm_u01(w2.v[0])) {
-    a.u1 = (real)((double)a.u1 - gd * d0);
-    a.u2 = (real)((double)a.u2 - gd * d1);
-    a.u3 = (real)((double)a.u3 - gd * d2);
-    bb.u1 = (real)((double)bb.u1 + gd * d0);
-    bb.u2 = (real)((double)bb.u2 + gd * d1);
-    bb.u3 = (real)((double)bb.u3 + gd * d2);
+    a.u1 = (T)((double)a.u1 - gd * d0);
+    a.u2 = (T)((double)a.u2 - gd * d1);
+    a.u3 = (T)((double)a.u3 - gd * d2);
+    bb.u1 = (T)((double)bb.u1 + gd * d0);
+    bb.u2 = (T)((double)bb.u2 + gd * d1);
+    bb.u3 = (T)((double)bb.u3 + gd * d2);
     P[i] = a;
     P[j] = bb;
     return true;
@@ -496,14 +555,15 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, Re
 // Collision_DSMC_VHS commit (collisions.cpp:2099-2131): VHS cross-section
 // sigma_T ~ Vr^(2-2 visp) crref^(2 visp - 1), accept iff sigma_T / crm > U,
 // isotropic post-collision relative velocity about the centre of mass.
-template <typename real>
-__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, Rec<real> *P,
+template <typename R>
+__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, R *P,
                                             const double *omax, double *omega_new,
                                             const KParams &kp, const FPParams &fpp,
                                             uint32_t step_lo, uint32_t step_hi) {
-  Rec<real> a = P[i];
-  Rec<real> bb = P[j];
-  const uint32_t c = (uint32_t)cell_of((double)a.y, kp.dy, kp.n_cells);
+  using real = typename R::real_type;
+  R a = P[i];
+  R bb = P[j];
+  const uint32_t c = (uint32_t)cell_key(a, kp);
   mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
                              kp.key1);
   double g0 = (double)a.u1 - (double)bb.u1;
@@ -542,8 +602,8 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, 
 // into the next operation list.  Each operation that finds its own priority
 // in the current array clears it, so the array is back to all-ones when the
 // round ends.
-template <typename real, int MODEL>
-__global__ __launch_bounds__(256) void k_round(int r, uint32_t count0, Rec<real> *__restrict__ P,
+template <typename R, int MODEL>
+__global__ __launch_bounds__(256) void k_round(int r, uint32_t count0, R *__restrict__ P,
                                                const uint4 *__restrict__ in,
                                                uint4 *__restrict__ out,
                                                uint32_t *__restrict__ res_cur,
@@ -569,10 +629,10 @@ __global__ __launch_bounds__(256) void k_round(int r, uint32_t count0, Rec<real>
       uint32_t ri = res_cur[i], rj = res_cur[j];
       if (ri == pr && rj == pr) {
         bool acc = MODEL == COLL_DSMC
-                       ? commit_dsmc<real>(i, j, rec.w, P, omax, omega_new, kp, fpp, step_lo,
-                                           step_hi)
-                       : commit_op<real>(i, j, rec.w, P, nden, yc, omax, omega_new, kp, step_lo,
-                                         step_hi);
+                       ? commit_dsmc<R>(i, j, rec.w, P, omax, omega_new, kp, fpp, step_lo,
+                                        step_hi)
+                       : commit_op<R>(i, j, rec.w, P, nden, yc, omax, omega_new, kp, step_lo,
+                                      step_hi);
         acc_local += acc ? 1u : 0u;
         if (log.p) {
           uint32_t q = atomicAdd(&counters[CNT_LOG], 1u);
@@ -796,8 +856,8 @@ __device__ __forceinline__ void block_sum(double (&v)[K], double *sh /* [K][bloc
   __syncthreads();
 }
 
-template <typename real>
-__global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P,
+template <typename R>
+__global__ __launch_bounds__(256) void k_moments(const R *__restrict__ P,
                                                  const uint32_t *__restrict__ cnt,
                                                  const uint32_t *__restrict__ start,
                                                  double *__restrict__ out, KParams kp) {
@@ -807,7 +867,7 @@ __global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P
   const uint32_t s = start[c];
   double m1[3] = {0.0, 0.0, 0.0};
   for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-    Rec<real> r = P[s + k];
+    R r = P[s + k];
     m1[0] += (double)r.u1;
     m1[1] += (double)r.u2;
     m1[2] += (double)r.u3;
@@ -823,7 +883,7 @@ __global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P
 #pragma unroll
   for (int q = 0; q < 29; ++q) a[q] = 0.0;
   for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-    Rec<real> r = P[s + k];
+    R r = P[s + k];
     double p1 = (double)r.u1 - U0, p2 = (double)r.u2 - U1, p3 = (double)r.u3 - U2;
     a[26] += p1; a[27] += p2; a[28] += p3;
     double e = p1 * p1 + p2 * p2 + p3 * p3;
@@ -851,8 +911,8 @@ __global__ __launch_bounds__(256) void k_moments(const Rec<real> *__restrict__ P
   }
 }
 
-template <typename real>
-__global__ __launch_bounds__(256) void k_sample(const Rec<real> *__restrict__ P,
+template <typename R>
+__global__ __launch_bounds__(256) void k_sample(const R *__restrict__ P,
                                                 const uint32_t *__restrict__ cnt,
                                                 const uint32_t *__restrict__ start,
                                                 double *__restrict__ acc) {
@@ -864,7 +924,7 @@ __global__ __launch_bounds__(256) void k_sample(const Rec<real> *__restrict__ P,
 #pragma unroll
   for (int q = 0; q < 25; ++q) a[q] = 0.0;
   for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-    Rec<real> r = P[s + k];
+    R r = P[s + k];
     double u = (double)r.u1, v = (double)r.u2, w = (double)r.u3;
     double e = u * u + v * v + w * w;
     a[0] += u; a[1] += v; a[2] += w;
@@ -885,36 +945,36 @@ __global__ __launch_bounds__(256) void k_sample(const Rec<real> *__restrict__ P,
 }
 
 // ---------------------------------------------------------------------------
-// explicit instantiations
+// explicit instantiations: 1D (Rec) and 2D (Rec2) records, FP32 and FP64
 // ---------------------------------------------------------------------------
-#define MCPM_INST_COLL(real, M)                                                            \
-  template __global__ void k_gen<real, M>(const Rec<real> *, const uint32_t *,             \
-                                          const uint32_t *, const uint32_t *,              \
-                                          const uint32_t *, KParams, uint32_t, uint32_t,   \
-                                          uint32_t, uint4 *, uint32_t *, uint32_t *);      \
-  template __global__ void k_round<real, M>(int, uint32_t, Rec<real> *, const uint4 *,     \
-                                            uint4 *, uint32_t *, uint32_t *, uint32_t *,   \
-                                            const double *, const double *, const double *, \
-                                            double *, KParams, uint32_t, uint32_t,         \
-                                            FPParams, OpLog);
+#define MCPM_INST_COLL(R, M)                                                               \
+  template __global__ void k_gen<R, M>(const R *, const uint32_t *, const uint32_t *,      \
+                                       const uint32_t *, const uint32_t *, KParams,        \
+                                       uint32_t, uint32_t, uint32_t, uint4 *, uint32_t *,  \
+                                       uint32_t *);                                        \
+  template __global__ void k_round<R, M>(int, uint32_t, R *, const uint4 *, uint4 *,       \
+                                         uint32_t *, uint32_t *, uint32_t *,               \
+                                         const double *, const double *, const double *,   \
+                                         double *, KParams, uint32_t, uint32_t, FPParams,  \
+                                         OpLog);
 
-#define MCPM_INST(real)                                                                    \
-  template __global__ void k_push<real, false>(const Rec<real> *, Rec<real> *, int64_t,    \
-                                               int32_t, KParams, const double *,           \
-                                               const double *, int, uint32_t, uint32_t,    \
-                                               uint32_t *, const uint32_t *, uint32_t *);  \
-  template __global__ void k_push<real, true>(const Rec<real> *, Rec<real> *, int64_t,     \
-                                              int32_t, KParams, const double *,            \
-                                              const double *, int, uint32_t, uint32_t,     \
-                                              uint32_t *, const uint32_t *, uint32_t *);   \
-  MCPM_INST_COLL(real, COLL_ESMC)                                                           \
-  MCPM_INST_COLL(real, COLL_DSMC)                                                           \
-  template __global__ void k_moments<real>(const Rec<real> *, const uint32_t *,            \
-                                           const uint32_t *, double *, KParams);           \
-  template __global__ void k_sample<real>(const Rec<real> *, const uint32_t *,             \
-                                          const uint32_t *, double *);
+#define MCPM_INST(R)                                                                       \
+  template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
+                                            Kick, const double *, int, uint32_t, uint32_t, \
+                                            uint32_t *, const uint32_t *, uint32_t *);     \
+  template __global__ void k_push<R, true>(const R *, R *, int64_t, int32_t, KParams,      \
+                                           Kick, const double *, int, uint32_t, uint32_t,  \
+                                           uint32_t *, const uint32_t *, uint32_t *);      \
+  MCPM_INST_COLL(R, COLL_ESMC)                                                             \
+  MCPM_INST_COLL(R, COLL_DSMC)                                                             \
+  template __global__ void k_moments<R>(const R *, const uint32_t *, const uint32_t *,     \
+                                        double *, KParams);                                \
+  template __global__ void k_sample<R>(const R *, const uint32_t *, const uint32_t *,      \
+                                       double *);
 
-MCPM_INST(float)
-MCPM_INST(double)
+MCPM_INST(Rec<float>)
+MCPM_INST(Rec<double>)
+MCPM_INST(Rec2<float>)
+MCPM_INST(Rec2<double>)
 
 }  // namespace mcpm

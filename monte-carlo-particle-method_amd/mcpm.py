@@ -64,6 +64,9 @@ class CParams(ctypes.Structure):
         ("ast", ctypes.c_double), ("bst", ctypes.c_double), ("eqdist", ctypes.c_double),
         ("mu", ctypes.c_double), ("T0", ctypes.c_double), ("visp", ctypes.c_double),
         ("crref", ctypes.c_double), ("phi", ctypes.c_double),
+        ("dims", ctypes.c_int32), ("n_cells_x", ctypes.c_int32),
+        ("boundary_x", ctypes.c_int32), ("reserved1", ctypes.c_int32),
+        ("length_x", ctypes.c_double),
     ]
 
 
@@ -79,7 +82,9 @@ class CStats(ctypes.Structure):
 
 @dataclass
 class Params:
-    """Slab parameters; names follow the reference's GAS/BOX fields."""
+    """Slab (dims = 1) or plane (dims = 2) parameters; names follow the
+    reference's GAS/BOX fields.  On the plane n_cells, length and boundary
+    describe y and n_cells_x, length_x, boundary_x describe x."""
     n_cells: int
     length: float
     capacity: int
@@ -111,6 +116,11 @@ class Params:
     visp: float = 0.5
     crref: float = 1.0
     phi: float = -4.0
+    dims: int = 1
+    n_cells_x: int = 1
+    boundary_x: int = BC_SPECULAR
+    reserved1: int = 0
+    length_x: float = 1.0
 
     def __post_init__(self):
         if math.isnan(self.b):
@@ -120,7 +130,12 @@ class Params:
         if math.isnan(self.wall_hi):
             self.wall_hi = self.length - self.sigma - 1e-15         # Main_Functions.cpp:5088
         if math.isnan(self.cell_volume):
-            self.cell_volume = self.length / self.n_cells
+            if self.dims == 2:
+                # dx dy dz with unit depth: the injection problem scales Len[0]
+                # and Len[1] by sigma but keeps Len[2] = 1 (myIO.cpp:2741-2761)
+                self.cell_volume = (self.length_x / self.n_cells_x) * (self.length / self.n_cells) * 1.0
+            else:
+                self.cell_volume = self.length / self.n_cells
         if math.isnan(self.mu):
             # hard-sphere viscosity 5/(16 sigma^2) sqrt(m kB T0 / pi) (myIO.cpp:2829)
             self.mu = 5.0 / (16.0 * self.sigma ** 2) * math.sqrt(self.mass * self.kb * self.T0 / math.pi)
@@ -128,6 +143,18 @@ class Params:
     @property
     def dy(self) -> float:
         return self.length / self.n_cells
+
+    @property
+    def dx(self) -> float:
+        return self.length_x / self.n_cells_x
+
+    @property
+    def total_cells(self) -> int:
+        return self.n_cells * (self.n_cells_x if self.dims == 2 else 1)
+
+    @property
+    def n_nodes(self) -> int:
+        return (self.n_cells_x + 1) * (self.n_cells + 1)
 
     def to_c(self) -> CParams:
         c = CParams()
@@ -171,6 +198,9 @@ def load_library() -> ctypes.CDLL:
         "mcpm_sync": (i32, [vp]),
         "mcpm_upload": (i32, [vp, i64, dpp, dpp, dpp, dpp]),
         "mcpm_download": (i32, [vp, dpp, dpp, dpp, dpp]),
+        "mcpm_upload_xy": (i32, [vp, i64, dpp, dpp, dpp, dpp, dpp]),
+        "mcpm_download_xy": (i32, [vp, dpp, dpp, dpp, dpp, dpp]),
+        "mcpm_get_nodes": (i32, [vp, dpp, dpp]),
         "mcpm_particle_count": (i64, [vp]),
         "mcpm_cell_update": (i32, [vp]),
         "mcpm_collide_esmc": (i32, [vp, i64, ctypes.POINTER(CStats)]),
@@ -253,7 +283,8 @@ class Simulation:
 
     @property
     def n_cells(self) -> int:
-        return self.params.n_cells
+        """Total number of cells (N0 N1 on the plane)."""
+        return self.params.total_cells
 
     # -- data ---------------------------------------------------------------
     def upload(self, y, u1, u2, u3):
@@ -266,6 +297,26 @@ class Simulation:
         out = [np.empty(n, np.float64) for _ in range(4)]
         self._check(self._lib.mcpm_download(self._h, *[_dp(a) for a in out]), "mcpm_download")
         return tuple(out)
+
+    def upload_xy(self, x, y, u1, u2, u3):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u1, u2, u3)]
+        n = arrs[0].shape[0]
+        self._check(self._lib.mcpm_upload_xy(self._h, n, *[_dp(a) for a in arrs]), "mcpm_upload_xy")
+
+    def download_xy(self):
+        n = self._lib.mcpm_particle_count(self._h)
+        out = [np.empty(n, np.float64) for _ in range(5)]
+        self._check(self._lib.mcpm_download_xy(self._h, *[_dp(a) for a in out]), "mcpm_download_xy")
+        return tuple(out)
+
+    def nodes(self):
+        """(n_node, potential) on the plane's (N0+1)(N1+1) nodes, shape (N1+1, N0+1)."""
+        nn = self.params.n_nodes
+        nd = np.empty(nn)
+        pot = np.empty(nn)
+        self._check(self._lib.mcpm_get_nodes(self._h, _dp(nd), _dp(pot)), "mcpm_get_nodes")
+        shape = (self.params.n_cells + 1, self.params.n_cells_x + 1)
+        return nd.reshape(shape), pot.reshape(shape)
 
     def cells(self):
         nc = self.n_cells

@@ -13,6 +13,11 @@ problem and the small versions the parity tests run:
   4096 cells and 20M particles; n sigma^3 = 0.55 in the central third, 0.02
   elsewhere, T = 0.7, attractive tail -eps (sigma/r)^6 through the
   screened-Poisson mean field, dt = 0.02, FP32.
+* plane: the 2D problems (the reference's `injection`, myIO.cpp:2729-2761):
+  liquid discs in vapour on an Lx x Ly plane of unit depth, specular walls
+  or periodic axes, direct or screened-Poisson attraction.  `lattice=True`
+  places four particles per cell at the quarter points instead, an exactly
+  uniform density for closed-form checks.
 
 Units are reduced: sigma = m = kB = eps = 1.  Initial positions are uniform
 in each density region and velocities Maxwellian (numpy Philox generator),
@@ -24,8 +29,8 @@ import math
 
 import numpy as np
 
-from mcpm import (BC_DIFFUSE, BC_PERIODIC, FP32, FP64, LR_NONE, LR_SCREENED_POISSON, Params,
-                  Y_REFERENCE)
+from mcpm import (BC_DIFFUSE, BC_PERIODIC, BC_SPECULAR, FP32, FP64, LR_DIRECT, LR_NONE,
+                  LR_SCREENED_POISSON, MODEL_ESMC, Params, Y_REFERENCE)
 
 # Yukawa fit of the reference (myIO.cpp:2891-2892) in reduced units:
 # bst = 6.91304716e9 1/m * 3.405e-10 m, ast = -1.64835851e-28 J m / (eps sigma)
@@ -96,3 +101,79 @@ def interface(N=20_000_000, n_cells=4096, length=400.0, n_liq=0.55, n_vap=0.02, 
                ast=YUKAWA_AST_4EPS / 4.0, bst=YUKAWA_BST, eqdist=1.0)
     n_gas = N / (length * area)
     return p, (y, u1, u2, u3), {"n_gas": n_gas, "T0": T, "N": N, "area": area}
+
+
+def plane(nx=40, ny=40, Lx=40.0, Ly=40.0, ppc=20, n_vap=0.1, n_liq=0.55, drops=((20.0, 20.0, 10.0),),
+          T=0.835, dt=0.01, precision=FP64, seed=1, boundary=BC_SPECULAR, long_range=LR_DIRECT,
+          model=MODEL_ESMC, lattice=False, capacity=None, phi=-4.0):
+    """Droplets of density n_liq in vapour n_vap; `drops` lists (x, y, radius)."""
+    rng = np.random.Generator(np.random.Philox(seed))
+    dx, dy = Lx / nx, Ly / ny
+    if lattice:
+        j, k = np.meshgrid(np.arange(nx), np.arange(ny), indexing="xy")
+        xs, ys = [], []
+        for fx in (0.25, 0.75):
+            for fy in (0.25, 0.75):
+                xs.append(((j + fx) * dx).ravel())
+                ys.append(((k + fy) * dy).ravel())
+        x = np.concatenate(xs)
+        y = np.concatenate(ys)
+        n_bg = x.size
+        Fn = n_vap * Lx * Ly * 1.0 / n_bg
+    else:
+        n_bg = ppc * nx * ny
+        Fn = n_vap * Lx * Ly * 1.0 / n_bg           # myIO.cpp:2745
+        x = rng.random(n_bg) * Lx
+        y = rng.random(n_bg) * Ly
+        for (cx, cy, r) in drops:
+            nd = int(math.floor((n_liq - n_vap) * math.pi * r * r / Fn))   # myIO.cpp:2748
+            rad = r * np.sqrt(rng.random(nd))
+            ang = 2.0 * math.pi * rng.random(nd)
+            xd = np.clip(cx + rad * np.cos(ang), 0.0, np.nextafter(Lx, 0))
+            yd = np.clip(cy + rad * np.sin(ang), 0.0, np.nextafter(Ly, 0))
+            x = np.concatenate([x, xd])
+            y = np.concatenate([y, yd])
+    N = x.size
+    u1, u2, u3 = maxwellian(rng, N, T)
+    p = Params(n_cells=ny, length=Ly, capacity=capacity or N, precision=precision,
+               boundary=boundary, y_model=Y_REFERENCE, long_range=long_range, model=model,
+               seed=seed, Fn=Fn, delta_t=dt, ast=YUKAWA_AST_4EPS / 4.0, bst=YUKAWA_BST,
+               eqdist=1.0, phi=phi, dims=2, n_cells_x=nx, length_x=Lx, boundary_x=boundary)
+    n_gas = N * Fn / (Lx * Ly)
+    return p, (x, y, u1, u2, u3), {"n_gas": n_gas, "T0": T, "N": N, "Fn": Fn}
+
+
+def droplets(nx=512, ny=512, Lx=128.0, Ly=128.0, N=64_000_000, n_drops=20, r_lo=5.0, r_hi=15.0,
+             n_liq=0.55, n_vap=0.02, T=0.75, dt=0.01, precision=FP32, seed=1,
+             long_range=LR_SCREENED_POISSON, capacity=None):
+    """Periodic plane with random liquid discs: n_drops discs, radii uniform in
+    [r_lo, r_hi], at n_liq in vapour n_vap, about N particles in all."""
+    rng = np.random.Generator(np.random.Philox(seed + 1000))
+    drops = [(rng.random() * Lx, rng.random() * Ly, r_lo + (r_hi - r_lo) * rng.random())
+             for _ in range(n_drops)]
+    # liquid fraction of the particles from the disc areas (overlaps ignored)
+    a_liq = sum(math.pi * r * r for (_, _, r) in drops)
+    mass = n_vap * Lx * Ly + (n_liq - n_vap) * a_liq
+    ppc = max(1, int(round(N * (n_vap * Lx * Ly / mass) / (nx * ny))))
+    p, arrs, meta = plane(nx=nx, ny=ny, Lx=Lx, Ly=Ly, ppc=ppc, n_vap=n_vap, n_liq=n_liq,
+                          drops=[], T=T, dt=dt, precision=precision, seed=seed,
+                          boundary=BC_PERIODIC, long_range=long_range, capacity=capacity)
+    x, y, u1, u2, u3 = arrs
+    rng2 = np.random.Generator(np.random.Philox(seed + 2000))
+    xs, ys = [x], [y]
+    for (cx, cy, r) in drops:
+        nd = int(math.floor((n_liq - n_vap) * math.pi * r * r / p.Fn))
+        rad = r * np.sqrt(rng2.random(nd))
+        ang = 2.0 * math.pi * rng2.random(nd)
+        xs.append(np.mod(cx + rad * np.cos(ang), Lx))
+        ys.append(np.mod(cy + rad * np.sin(ang), Ly))
+    x = np.concatenate(xs)
+    y = np.concatenate(ys)
+    x = np.where(x < Lx, x, 0.0)
+    y = np.where(y < Ly, y, 0.0)
+    Nt = x.size
+    v1, v2, v3 = maxwellian(rng2, Nt, T)
+    if capacity is None or capacity < Nt:
+        p.capacity = Nt
+    meta.update({"N": Nt, "n_gas": Nt * p.Fn / (Lx * Ly), "drops": drops})
+    return p, (x, y, v1, v2, v3), meta

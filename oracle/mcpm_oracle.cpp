@@ -35,6 +35,16 @@
  *   - periodic boundaries (cyclic tridiagonal by Sherman-Morrison, wrapped
  *     partner cell and minimum-image midpoint) for the periodic slab, which
  *     the reference's 1D path does not offer.
+ *
+ * The plane (problem injection) adds:
+ *   cell_update 2D binning and nodal deposit   Main_Functions.cpp:2231-2364
+ *   Collision_ESMC_new3 2D partner cell        collisions.cpp:658-687
+ *   BC_specular_reflection (x and y)           Main_Functions.cpp:6399-6525
+ *   compute_SP_2D                              FEM.cpp:249-610
+ *     (assembled matrix with Dirichlet rows, solved here by banded LU
+ *      instead of PARDISO, FEM.cpp:450-458)
+ *   compute_Vlasov_2D                          FEM.cpp:613-726
+ * and, as an extension, periodic planes (dense LU, wrapped neighbours).
  */
 #include <math.h>
 #include <stdint.h>
@@ -69,12 +79,22 @@ int orc_cell(double y, double dy, int nc) {
   return (int)q;
 }
 
+double dn_xi(int i, double xi, double eta);
+double dn_eta(int i, double xi, double eta);
+
 struct Oracle {
   mcpm_params p;
   int nc = 0;
   double dy = 0.0;
   int64_t n = 0;
-  std::vector<double> rec;  // 4 per particle, stored through `real` rounding
+  // record layout: slab (y, u1, u2, u3), plane (x, y, u1, u2, u3)
+  int dims = 1, nx = 1, ny = 1, S = 4, U = 1;
+  double dx = 1.0;
+  std::vector<double> rec;  // S per particle, stored through `real` rounding
+  // the plane: per-cell corner copies of the nodal density (cells[c].n_n)
+  // and the nodal field of the last solve
+  std::vector<double> n_n, pot;
+  double kick_coef = 0.0;
   std::vector<int> cnt, start;
   std::vector<double> nden, yc, omax, F2, samples;
   bool kick_pending = false;
@@ -94,6 +114,68 @@ double rnd(double v) {
   return (double)(real)v;
 }
 
+// Cell of particle k: floor(y/dy) on the slab, k N0 + j on the plane
+// (Main_Functions.cpp:2233-2263).
+int orc_key(const Oracle &o, int64_t k) {
+  const double *r = &o.rec[(size_t)o.S * k];
+  if (o.dims == 2) return orc_cell(r[1], o.dy, o.ny) * o.nx + orc_cell(r[0], o.dx, o.nx);
+  return orc_cell(r[0], o.dy, o.nc);
+}
+
+// One coordinate of the point r_i + sigma dir: its cell on a periodic axis
+// after wrapping; on a walled axis false when outside (collisions.cpp:668-671).
+bool orc_axis(double v, double h, double L, int n, bool periodic, int *out) {
+  if (periodic) {
+    if (v < 0.0) v += L;
+    else if (v >= L) v -= L;
+    *out = orc_cell(v, h, n);
+    return true;
+  }
+  double q = mcpm_floor(v / h);
+  if (!(q >= 0.0) || q >= (double)n) return false;
+  *out = (int)q;
+  return true;
+}
+
+bool orc_partner(const Oracle &o, int64_t i, double d0, double d1, int *J) {
+  const double *r = &o.rec[(size_t)o.S * i];
+  const mcpm_params &p = o.p;
+  if (o.dims == 2) {  // J = ind[1] N0 + ind[0] (collisions.cpp:660-666)
+    int jx, jy;
+    if (!orc_axis(r[0] + p.sigma * d0, o.dx, p.length_x, o.nx, p.boundary_x == MCPM_BC_PERIODIC,
+                  &jx))
+      return false;
+    if (!orc_axis(r[1] + p.sigma * d1, o.dy, p.length, o.ny, p.boundary == MCPM_BC_PERIODIC, &jy))
+      return false;
+    *J = jy * o.nx + jx;
+    return true;
+  }
+  return orc_axis(r[0] + p.sigma * d1, o.dy, p.length, o.nc, p.boundary == MCPM_BC_PERIODIC, J);
+}
+
+double orc_mid_axis(double a, double b, double L, bool periodic) {
+  if (!periodic) return (a + b) / 2.0;  // collisions.cpp:682-683
+  double d = b - a;
+  if (d > 0.5 * L) d -= L;
+  else if (d < -0.5 * L) d += L;
+  double m = a + d / 2.0;
+  if (m < 0.0) m += L;
+  else if (m >= L) m -= L;
+  return m;
+}
+
+int orc_mid(const Oracle &o, int64_t i, int64_t j) {
+  const double *ri = &o.rec[(size_t)o.S * i], *rj = &o.rec[(size_t)o.S * j];
+  const mcpm_params &p = o.p;
+  if (o.dims == 2) {
+    double xm = orc_mid_axis(ri[0], rj[0], p.length_x, p.boundary_x == MCPM_BC_PERIODIC);
+    double ym = orc_mid_axis(ri[1], rj[1], p.length, p.boundary == MCPM_BC_PERIODIC);
+    return orc_cell(ym, o.dy, o.ny) * o.nx + orc_cell(xm, o.dx, o.nx);  // IJmid (:686)
+  }
+  return orc_cell(orc_mid_axis(ri[0], rj[0], p.length, p.boundary == MCPM_BC_PERIODIC), o.dy,
+                  o.nc);
+}
+
 // Stable counting sort of the particles by cell (cell_update binning and
 // index lists, Main_Functions.cpp:2231-2376).
 void orc_sort(Oracle &o, bool check) {
@@ -101,9 +183,11 @@ void orc_sort(Oracle &o, bool check) {
   std::vector<int> key(o.n);
   std::fill(o.cnt.begin(), o.cnt.end(), 0);
   for (int64_t k = 0; k < o.n; ++k) {
-    double y = o.rec[4 * k];
+    const double *r = &o.rec[(size_t)o.S * k];
+    double y = o.dims == 2 ? r[1] : r[0];
     if (check && (!(y >= 0.0) || !(y <= o.p.length))) o.err |= 1u;
-    key[k] = orc_cell(y, o.dy, nc);
+    if (check && o.dims == 2 && (!(r[0] >= 0.0) || !(r[0] <= o.p.length_x))) o.err |= 1u;
+    key[k] = orc_key(o, k);
     o.cnt[key[k]]++;
   }
   int run = 0;
@@ -116,7 +200,7 @@ void orc_sort(Oracle &o, bool check) {
   std::vector<double> out(o.rec.size());
   for (int64_t k = 0; k < o.n; ++k) {
     int d = fill[key[k]]++;
-    memcpy(&out[4 * (size_t)d], &o.rec[4 * (size_t)k], 4 * sizeof(double));
+    memcpy(&out[(size_t)o.S * d], &o.rec[(size_t)o.S * k], o.S * sizeof(double));
   }
   o.rec.swap(out);
   for (int c = 0; c < nc; ++c) {
@@ -222,13 +306,109 @@ void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, u
   r[0] = rnd<real>(y);
 }
 
+// One coordinate's periodic wrap, stored through `real` (the value must stay
+// below L after rounding).
+template <typename real>
+double wrap_real(double v, double L) {
+  if (v < 0.0) v += L;
+  else if (v >= L) v -= L;
+  real vr = (real)v;
+  if (!(vr < (real)L)) vr = (real)0;
+  if (!(vr >= (real)0)) vr = (real)0;
+  return (double)vr;
+}
+
+// The plane's kick and move for particle k:
+//   kick: U -= grad(Q1 interpolant of the nodal field) coef / m dt
+//         (FEM.cpp:568-582, 709-721; coef = ast or phi)
+//   move: x += U1 dt, y += U2 dt (Position_FP_ideal_gas, Fokker_Planck.cpp:948-969)
+//         then BC_specular_reflection on the walled axes
+//         (Main_Functions.cpp:6399-6525) or a periodic wrap.
+// Force per unit mass times -1 at (x, y) in cell num: the gradient of the
+// bilinear interpolant of the nodal field pot, times coef / m
+// (FEM.cpp:568-580).
+void fem_force(const Oracle &o, int num, double x, double y, const double *pot, double coef,
+               double *F1o, double *F2o) {
+  const mcpm_params &p = o.p;
+  {
+    const int j = num % o.nx, kk = num / o.nx;
+    // cells[num] geometry (myIO.cpp:1455-1479)
+    const double d0 = (double)j * o.dx, d1 = (double)(j + 1) * o.dx;
+    const double d2 = (double)kk * o.dy, d3 = (double)(kk + 1) * o.dy;
+    const double cdx = d1 - d0, cdy = d3 - d2;
+    const double ccx = 0.5 * (d0 + d1), ccy = 0.5 * (d2 + d3);
+    const double xn0 = ccx - cdx / 2.0, yn0 = ccy - cdy / 2.0;
+    const int id[4] = {kk * (o.nx + 1) + j, kk * (o.nx + 1) + j + 1, (kk + 1) * (o.nx + 1) + j,
+                       (kk + 1) * (o.nx + 1) + j + 1};
+    const double xi = (x - xn0) / cdx, eta = (y - yn0) / cdy;
+    double F1 = 0.0, F2 = 0.0;
+    for (int q = 0; q < 4; ++q) {
+      F1 += dn_xi(q, xi, eta) * pot[id[q]];
+      F2 += dn_eta(q, xi, eta) * pot[id[q]];
+    }
+    *F1o = F1 / p.mass * coef / cdx;
+    *F2o = F2 / p.mass * coef / cdy;
+  }
+}
+
+template <typename real>
+void orc_push2_one(Oracle &o, int64_t k, bool kick, bool move) {
+  const mcpm_params &p = o.p;
+  double *r = &o.rec[(size_t)o.S * k];
+  if (kick) {
+    double F1, F2;
+    fem_force(o, orc_key(o, k), r[0], r[1], o.pot.data(), o.kick_coef, &F1, &F2);
+    r[2] = rnd<real>(r[2] - F1 * p.delta_t);
+    r[3] = rnd<real>(r[3] - F2 * p.delta_t);
+  }
+  if (!move) return;
+  double x = r[0] + r[2] * p.delta_t, y = r[1] + r[3] * p.delta_t;
+  double u1 = r[2], u2 = r[3];
+  const bool wx = p.boundary_x != MCPM_BC_PERIODIC, wy = p.boundary != MCPM_BC_PERIODIC;
+  if (wx || wy) {
+    const double Lx = p.length_x, Ly = p.length;
+    const double tol1 = 1e-16, tol2 = -1e-16;
+    bool done = false;
+    for (int pass = 0; pass < 20 && !done; ++pass) {
+      if (wx) {
+        if (x - Lx > tol2) {
+          u1 = -u1;
+          x = x - 2.0 * fabs(x - Lx);
+        } else if (x < tol1) {
+          u1 = -u1;
+          x = x + 2.0 * fabs(x);
+        }
+      }
+      if (wy) {
+        if (y - Ly > tol2) {
+          u2 = -u2;
+          y = y - 2.0 * fabs(y - Ly);
+        } else if (y < tol1) {
+          u2 = -u2;
+          y = y + 2.0 * fabs(y);
+        }
+      }
+      done = true;
+      if (wx && (x - Lx > 0.0 || x < 0.0)) done = false;
+      if (wy && (y - Ly > 0.0 || y < 0.0)) done = false;
+    }
+    if (!done) o.err |= 4u;
+  }
+  r[2] = rnd<real>(u1);
+  r[3] = rnd<real>(u2);
+  r[0] = wx ? rnd<real>(x) : wrap_real<real>(x, p.length_x);
+  r[1] = wy ? rnd<real>(y) : wrap_real<real>(y, p.length);
+}
+
 template <typename real>
 void orc_push(Oracle &o, bool kick, bool move, int64_t step) {
   uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  const int U = o.U;
   for (int64_t k = 0; k < o.n; ++k) {
-    double *r = &o.rec[4 * (size_t)k];
-    if (r[1] != r[1] || r[2] != r[2] || r[3] != r[3]) o.err |= 2u;
-    orc_push_one<real>(o, k, kick, move, slo, shi);
+    double *r = &o.rec[(size_t)o.S * k];
+    if (r[U] != r[U] || r[U + 1] != r[U + 1] || r[U + 2] != r[U + 2]) o.err |= 2u;
+    if (o.dims == 2) orc_push2_one<real>(o, k, kick, move);
+    else orc_push_one<real>(o, k, kick, move, slo, shi);
   }
 }
 
@@ -278,17 +458,8 @@ void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
       uint32_t i = (uint32_t)o.start[c] + mcpm_uint_below(w.v[0], (uint32_t)o.cnt[c]);
       double d0, d1, d2;
       mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-      double yJ = o.rec[4 * (size_t)i] + p.sigma * d1;  // collisions.cpp:661-662
-      int J;
-      if (p.boundary == MCPM_BC_PERIODIC) {
-        if (yJ < 0.0) yJ += p.length;
-        else if (yJ >= p.length) yJ -= p.length;
-        J = orc_cell(yJ, o.dy, nc);
-      } else {
-        double q = mcpm_floor(yJ / o.dy);
-        if (!(q >= 0.0) || q >= (double)nc) continue;  // outside: no partner (:668-671)
-        J = (int)q;
-      }
+      int J;  // cell at r_i + sigma dir (collisions.cpp:658-671)
+      if (!orc_partner(o, i, d0, d1, &J)) continue;
       if (o.cnt[J] == 0) continue;
       uint32_t j = (uint32_t)o.start[J] + mcpm_uint_below(w.v[3], (uint32_t)o.cnt[J]);
       if (i == j) continue;  // collisions.cpp:677-678
@@ -311,22 +482,10 @@ void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
     mcpm_u32x4 w2 = mcpm_philox(cd.t, cd.c, slo, mcpm_ctr3(MCPM_TAG_CAND, shi, 1u), key0, key1);
     double d0, d1, d2;
     mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-    double *ri = &o.rec[4 * (size_t)cd.i];
-    double *rj = &o.rec[4 * (size_t)cd.j];
-    double yi = ri[0], yj = rj[0];
-    double ymid;
-    if (p.boundary == MCPM_BC_PERIODIC) {
-      double d = yj - yi;
-      if (d > 0.5 * p.length) d -= p.length;
-      else if (d < -0.5 * p.length) d += p.length;
-      ymid = yi + d / 2.0;
-      if (ymid < 0.0) ymid += p.length;
-      else if (ymid >= p.length) ymid -= p.length;
-    } else {
-      ymid = (yi + yj) / 2.0;  // collisions.cpp:682-683
-    }
-    int cm = orc_cell(ymid, o.dy, nc);
-    int J = orc_cell(yj, o.dy, nc);
+    double *ri = &o.rec[(size_t)o.S * cd.i] + o.U - 1;  // ri[1..3] = U1..U3
+    double *rj = &o.rec[(size_t)o.S * cd.j] + o.U - 1;
+    int cm = orc_mid(o, cd.i, cd.j);
+    int J = orc_key(o, cd.j);
     double Ymid = o.yc[cm];  // collisions.cpp:687
     double g0 = ri[1] - rj[1], g1 = ri[2] - rj[2], g2 = ri[3] - rj[3];
     double gd = g0 * d0 + g1 * d1 + g2 * d2;
@@ -586,7 +745,7 @@ void orc_moments(const Oracle &o, double *out) {
     int N = o.cnt[c];
     double U[3] = {0, 0, 0};
     for (int k = 0; k < N; ++k) {
-      const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
       U[0] += r[1];
       U[1] += r[2];
       U[2] += r[3];
@@ -596,7 +755,7 @@ void orc_moments(const Oracle &o, double *out) {
     double a[26] = {0};
     double mp[3] = {0, 0, 0};
     for (int k = 0; k < N; ++k) {
-      const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
       double p1 = r[1] - U[0], p2 = r[2] - U[1], p3 = r[3] - U[2];
       mp[0] += p1;
       mp[1] += p2;
@@ -963,7 +1122,7 @@ void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st) {
       uint32_t lj = mcpm_uint_below(w.v[1], (uint32_t)(Nc - 1));
       if (lj >= li) lj += 1;
       uint32_t i = (uint32_t)o.start[c] + li, j = (uint32_t)o.start[c] + lj;
-      double *ri = &o.rec[4 * (size_t)i], *rj = &o.rec[4 * (size_t)j];
+      double *ri = &o.rec[(size_t)o.S * i] + o.U - 1, *rj = &o.rec[(size_t)o.S * j] + o.U - 1;
       double g0 = ri[1] - rj[1], g1 = ri[2] - rj[2], g2 = ri[3] - rj[3];
       double Vr = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
       double sig = orc_pow(Vr, 2.0 - 2.0 * p.visp) * orc_pow(p.crref, 2.0 * p.visp - 1.0);
@@ -1008,7 +1167,7 @@ void orc_sample(Oracle &o) {
   for (int c = 0; c < o.nc; ++c) {
     double *s = &o.samples[(size_t)c * MCPM_NSAMP];
     for (int k = 0; k < o.cnt[c]; ++k) {
-      const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
+      const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
       double U1 = r[1], U2 = r[2], U3 = r[3];
       s[0] += 1.0;
       s[1] += U1; s[2] += U2; s[3] += U3;
@@ -1028,6 +1187,426 @@ void orc_sample(Oracle &o) {
   }
 }
 
+// ------------------------------------------------------------------------
+// The plane's mean field (FEM.cpp)
+
+// Derivatives of the bilinear shape functions on the unit cell (FEM.cpp:22-49).
+double dn_xi(int i, double xi, double eta) {
+  (void)xi;
+  if (i == 0) return eta - 1.0;
+  if (i == 1) return 1.0 - eta;
+  if (i == 2) return -eta;
+  return eta;
+}
+double dn_eta(int i, double xi, double eta) {
+  (void)eta;
+  if (i == 0) return xi - 1.0;
+  if (i == 1) return -xi;
+  if (i == 2) return -xi + 1;
+  return xi;
+}
+
+// Full-precision Gauss-Legendre rule (nodes ascending), the reference's
+// 100-point table of compute_Vlasov_2D (FEM.cpp:656-657).
+void gl_full(int n, double *x, double *w) {
+  // Newton iteration in long double, so the double nodes and weights are
+  // correctly rounded (the weights are sensitive to the last bits of the
+  // nodes near the ends of the interval)
+  for (int i = 0; i < n; ++i) {
+    long double z = cosl(3.14159265358979323846264338327950288L * (i + 0.75L) / (n + 0.5L));
+    long double pm1 = 0.0L;
+    for (int it = 0; it < 100; ++it) {
+      long double p0 = 1.0L, p1 = z;
+      for (int k = 2; k <= n; ++k) {
+        long double p2 = ((2.0L * k - 1.0L) * z * p1 - (k - 1.0L) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      long double dp = n * (z * p1 - p0) / (z * z - 1.0L);
+      long double dz = p1 / dp;
+      z -= dz;
+      pm1 = p0;
+      if (fabsl(dz) < 1e-19L) break;
+    }
+    x[n - 1 - i] = (double)z;
+    w[n - 1 - i] = (double)(2.0L * ((1.0L - z) * (1.0L + z)) / ((n * pm1) * (n * pm1)));
+  }
+}
+
+// Geometry of cell (j, k) as make_cells computes it (myIO.cpp:1455-1479).
+struct CellGeo {
+  double x_n[4], y_n[4], dx, dy;
+  int id[4];
+};
+CellGeo cell_geo(const Oracle &o, int j, int k) {
+  CellGeo g;
+  const double d0 = (double)j * o.dx, d1 = (double)(j + 1) * o.dx;
+  const double d2 = (double)k * o.dy, d3 = (double)(k + 1) * o.dy;
+  g.dx = d1 - d0;
+  g.dy = d3 - d2;
+  const double cx = 0.5 * (d0 + d1), cy = 0.5 * (d2 + d3);
+  g.x_n[0] = cx - g.dx / 2.0; g.x_n[1] = cx + g.dx / 2.0;
+  g.x_n[2] = cx - g.dx / 2.0; g.x_n[3] = cx + g.dx / 2.0;
+  g.y_n[0] = cy - g.dy / 2.0; g.y_n[1] = cy - g.dy / 2.0;
+  g.y_n[2] = cy + g.dy / 2.0; g.y_n[3] = cy + g.dy / 2.0;
+  g.id[0] = k * (o.nx + 1) + j;
+  g.id[1] = k * (o.nx + 1) + j + 1;
+  g.id[2] = (k + 1) * (o.nx + 1) + j;
+  g.id[3] = (k + 1) * (o.nx + 1) + j + 1;
+  return g;
+}
+
+// Nodal density deposit of cell_update (Main_Functions.cpp:2268-2364): each
+// particle credits Fn/V to the corner of its own cell nearest to it and to
+// the same node's copies in the three neighbouring cells; afterwards the
+// copies on each wall are doubled.  Counts are kept as integers and scaled
+// once (the reference adds Fn/V one particle at a time).  Periodic axes
+// (an extension) wrap the neighbours and skip the doubling.
+void orc_deposit(Oracle &o) {
+  const int nx = o.nx, ny = o.ny;
+  const bool px = o.p.boundary_x == MCPM_BC_PERIODIC, py = o.p.boundary == MCPM_BC_PERIODIC;
+  std::vector<int64_t> k4((size_t)4 * nx * ny, 0);
+  auto credit = [&](int j, int k, int corner) {
+    if (px) j = (j + nx) % nx;
+    else if (j < 0 || j > nx - 1) return;
+    if (py) k = (k + ny) % ny;
+    else if (k < 0 || k > ny - 1) return;
+    k4[(size_t)4 * (k * nx + j) + corner]++;
+  };
+  for (int64_t i = 0; i < o.n; ++i) {
+    const int num = orc_key(o, i);
+    const int j = num % nx, k = num / nx;
+    const double *r = &o.rec[(size_t)o.S * i];
+    const double ccx = 0.5 * ((double)j * o.dx + (double)(j + 1) * o.dx);
+    const double ccy = 0.5 * ((double)k * o.dy + (double)(k + 1) * o.dy);
+    if (r[0] < ccx) {
+      if (r[1] < ccy) {
+        credit(j, k, 0); credit(j - 1, k, 1); credit(j, k - 1, 2); credit(j - 1, k - 1, 3);
+      } else {
+        credit(j, k, 2); credit(j - 1, k, 3); credit(j, k + 1, 0); credit(j - 1, k + 1, 1);
+      }
+    } else {
+      if (r[1] < ccy) {
+        credit(j, k, 1); credit(j + 1, k, 0); credit(j, k - 1, 3); credit(j + 1, k - 1, 2);
+      } else {
+        credit(j, k, 3); credit(j + 1, k, 2); credit(j, k + 1, 1); credit(j + 1, k + 1, 0);
+      }
+    }
+  }
+  const double fnv = o.p.Fn / o.p.cell_volume;
+  o.n_n.assign((size_t)4 * nx * ny, 0.0);
+  for (size_t q = 0; q < k4.size(); ++q) o.n_n[q] = (double)k4[q] * fnv;
+  if (!py) {
+    for (int jj = 0; jj < nx; ++jj) {
+      o.n_n[4 * (size_t)jj + 0] *= 2.0;
+      o.n_n[4 * (size_t)jj + 1] *= 2.0;
+      o.n_n[4 * (size_t)((ny - 1) * nx + jj) + 2] *= 2.0;
+      o.n_n[4 * (size_t)((ny - 1) * nx + jj) + 3] *= 2.0;
+    }
+  }
+  if (!px) {
+    for (int kk = 0; kk < ny; ++kk) {
+      o.n_n[4 * (size_t)(kk * nx) + 0] *= 2.0;
+      o.n_n[4 * (size_t)(kk * nx) + 2] *= 2.0;
+      o.n_n[4 * (size_t)(kk * nx + nx - 1) + 1] *= 2.0;
+      o.n_n[4 * (size_t)(kk * nx + nx - 1) + 3] *= 2.0;
+    }
+  }
+}
+
+// The node value seen by node (i, j): corner k of its owner cell (ii, jj).
+void owner(const Oracle &o, int i, int j, int *ii, int *jj, int *k) {
+  *ii = i < o.nx ? i : i - 1;
+  *jj = j < o.ny ? j : j - 1;
+  *k = (i == o.nx ? 1 : 0) + (j == o.ny ? 2 : 0);
+}
+
+std::vector<double> node_values(const Oracle &o) {
+  std::vector<double> v((size_t)(o.nx + 1) * (o.ny + 1));
+  for (int j = 0; j <= o.ny; ++j)
+    for (int i = 0; i <= o.nx; ++i) {
+      int ii, jj, k;
+      owner(o, i, j, &ii, &jj, &k);
+      v[(size_t)j * (o.nx + 1) + i] = o.n_n[4 * (size_t)(jj * o.nx + ii) + k];
+    }
+  return v;
+}
+
+// Quadrature sum around (xx, yy) (Compute_SP_using_Green_Function,
+// FEM.cpp:196-247, and vlasov_taking_the_integral_2D, FEM.cpp:157-195):
+// rad[i] is the radial factor with its weight; cells clamp to the mesh.
+double node_integral(const Oracle &o, double xx, double yy, int nq, const double *rr,
+                     const double *rad, const double *phi, const double *dphi,
+                     const double *theta, const double *dtheta) {
+  const bool px = o.p.boundary_x == MCPM_BC_PERIODIC, py = o.p.boundary == MCPM_BC_PERIODIC;
+  // cos(phi), sin(phi), sin(theta) tabulated: the same libm values the
+  // reference evaluates inside the loops
+  std::vector<double> cph(nq), sph(nq), sth(nq);
+  for (int k = 0; k < nq; ++k) {
+    cph[k] = cos(phi[k]);
+    sph[k] = sin(phi[k]);
+    sth[k] = sin(theta[k]);
+  }
+  double V = 0.0;
+  for (int i = 0; i < nq; ++i) {
+    for (int ii = 0; ii < nq; ++ii) {
+      for (int iii = 0; iii < nq; ++iii) {
+        double x = xx + rr[i] * cph[ii] * sth[iii];
+        double y = yy + rr[i] * sph[ii] * sth[iii];
+        int j, k;
+        if (px) {
+          if (x < 0.0) x += o.p.length_x;
+          else if (x >= o.p.length_x) x -= o.p.length_x;
+          j = orc_cell(x, o.dx, o.nx);
+        } else {
+          j = (int)floor(x / o.dx);
+          if (j < 0) j = 0;
+          else if (j > o.nx - 1) j = o.nx - 1;
+        }
+        if (py) {
+          if (y < 0.0) y += o.p.length;
+          else if (y >= o.p.length) y -= o.p.length;
+          k = orc_cell(y, o.dy, o.ny);
+        } else {
+          k = (int)floor(y / o.dy);
+          if (k < 0) k = 0;
+          else if (k > o.ny - 1) k = o.ny - 1;
+        }
+        const double n = o.nden[k * o.nx + j];
+        V += rad[i] * n * dphi[ii] * sth[iii] * dtheta[iii];
+      }
+    }
+  }
+  return V;
+}
+
+struct Quad {
+  int nq;
+  std::vector<double> rr, rad, phi, dphi, theta, dtheta;
+};
+
+// r in [a, 3 sigma], azimuth in [0, 2 pi], polar angle in [0, pi].
+Quad make_quad(const Oracle &o, bool yukawa) {
+  Quad q;
+  q.nq = yukawa ? 50 : 100;
+  std::vector<double> xi(q.nq), wi(q.nq);
+  if (yukawa) gl50(xi.data(), wi.data());
+  else gl_full(q.nq, xi.data(), wi.data());
+  const double aa = yukawa ? 0.0 : o.p.sigma, bb = 3.0 * o.p.sigma;
+  const double cc = 0.0, dd = 2 * PI, ee = 0.0, ff = PI;
+  for (int i = 0; i < q.nq; ++i) {
+    const double r = (bb - aa) / 2.0 * xi[i] + (aa + bb) / 2.0;
+    const double dr = wi[i] * (bb - aa) / 2.0;
+    q.rr.push_back(r);
+    if (yukawa) q.rad.push_back(exp(-o.p.bst * r) / (4.0 * PI) * r * dr);   // FEM.cpp:216
+    else q.rad.push_back(pow(o.p.sigma / r, 6.0) * r * r * dr);            // FEM.cpp:166
+    q.phi.push_back((dd - cc) / 2.0 * xi[i] + (cc + dd) / 2.0);
+    q.dphi.push_back(wi[i] * (dd - cc) / 2.0);
+    q.theta.push_back((ff - ee) / 2.0 * xi[i] + (ff + ee) / 2.0);
+    q.dtheta.push_back(wi[i] * (ff - ee) / 2.0);
+  }
+  return q;
+}
+
+double quad_at(const Oracle &o, const Quad &q, double x, double y) {
+  return node_integral(o, x, y, q.nq, q.rr.data(), q.rad.data(), q.phi.data(), q.dphi.data(),
+                       q.theta.data(), q.dtheta.data());
+}
+
+// Banded LU without pivoting; A holds row r's entries A[r][c] at
+// band[r * (2h+1) + (c - r + h)].
+void band_solve(std::vector<double> &band, int n, int h, std::vector<double> &b) {
+  const int w = 2 * h + 1;
+  auto at = [&](int r, int c) -> double & { return band[(size_t)r * w + (c - r + h)]; };
+  for (int k = 0; k < n; ++k) {
+    const double piv = at(k, k);
+    for (int r = k + 1; r <= std::min(n - 1, k + h); ++r) {
+      const double l = at(r, k) / piv;
+      if (l == 0.0) continue;
+      for (int c = k; c <= std::min(n - 1, k + h); ++c) at(r, c) -= l * at(k, c);
+      b[r] -= l * b[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    double t = b[k];
+    for (int c = k + 1; c <= std::min(n - 1, k + h); ++c) t -= at(k, c) * b[c];
+    b[k] = t / at(k, k);
+  }
+}
+
+// Dense Gaussian elimination with partial pivoting (periodic planes).
+void dense_solve(std::vector<double> &A, int n, std::vector<double> &b) {
+  for (int k = 0; k < n; ++k) {
+    int pr = k;
+    for (int r = k + 1; r < n; ++r)
+      if (fabs(A[(size_t)r * n + k]) > fabs(A[(size_t)pr * n + k])) pr = r;
+    if (pr != k) {
+      for (int c = 0; c < n; ++c) std::swap(A[(size_t)k * n + c], A[(size_t)pr * n + c]);
+      std::swap(b[k], b[pr]);
+    }
+    for (int r = k + 1; r < n; ++r) {
+      const double l = A[(size_t)r * n + k] / A[(size_t)k * n + k];
+      if (l == 0.0) continue;
+      for (int c = k; c < n; ++c) A[(size_t)r * n + c] -= l * A[(size_t)k * n + c];
+      b[r] -= l * b[k];
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    double t = b[k];
+    for (int c = k + 1; c < n; ++c) t -= A[(size_t)k * n + c] * b[c];
+    b[k] = t / A[(size_t)k * n + k];
+  }
+}
+
+// compute_SP_2D (FEM.cpp:249-554) up to the nodal field.
+void orc_sp2d(Oracle &o) {
+  orc_deposit(o);
+  const mcpm_params &p = o.p;
+  const int nx = o.nx, ny = o.ny, nn = (nx + 1) * (ny + 1);
+  const double dudv[4][4] = {{2.0 / 3.0, -1.0 / 6.0, -1.0 / 6.0, -1.0 / 3.0},
+                             {-1.0 / 6.0, 2.0 / 3.0, -1.0 / 3.0, -1.0 / 6.0},
+                             {-1.0 / 6.0, -1.0 / 3.0, 2.0 / 3.0, -1.0 / 6.0},
+                             {-1.0 / 3.0, -1.0 / 6.0, -1.0 / 6.0, 2.0 / 3.0}};  // FEM.cpp:73-88
+  const double uv[4][4] = {{1.0 / 9.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 36.0},
+                           {1.0 / 18.0, 1.0 / 9.0, 1.0 / 36.0, 1.0 / 18.0},
+                           {1.0 / 18.0, 1.0 / 36.0, 1.0 / 9.0, 1.0 / 18.0},
+                           {1.0 / 36.0, 1.0 / 18.0, 1.0 / 18.0, 1.0 / 9.0}};  // FEM.cpp:93-108
+  const double lambda2 = pow(p.bst, 2.0);
+  const double V = p.cell_volume;
+  const bool periodic = p.boundary == MCPM_BC_PERIODIC;
+  std::vector<double> x(nn, 0.0);
+  if (!periodic) {
+    const int h = nx + 2, w = 2 * h + 1;
+    std::vector<double> band((size_t)nn * w, 0.0), b(nn, 0.0);
+    auto A = [&](int r, int c) -> double & { return band[(size_t)r * w + (c - r + h)]; };
+    for (int k = 0; k < ny; ++k)
+      for (int j = 0; j < nx; ++j) {  // FEM.cpp:283-305
+        const int num = k * nx + j;
+        const CellGeo g = cell_geo(o, j, k);
+        double v[4];
+        for (int a = 0; a < 4; ++a) {
+          v[a] = 0.0;
+          for (int c = 0; c < 4; ++c) v[a] += o.n_n[4 * (size_t)num + c] * uv[a][c];
+        }
+        for (int a = 0; a < 4; ++a) {
+          for (int c = 0; c < 4; ++c) {
+            A(g.id[a], g.id[c]) += dudv[a][c];
+            A(g.id[a], g.id[c]) += lambda2 * uv[a][c] * V;
+          }
+          b[g.id[a]] += v[a] * V;
+        }
+      }
+    // Dirichlet rows from the Green's function, in the reference's wall order
+    // (lower, upper, left, right; FEM.cpp:396-448)
+    const Quad q = make_quad(o, true);
+    auto dirichlet = [&](int j, int k, int corner) {
+      const CellGeo g = cell_geo(o, j, k);
+      const int r = g.id[corner];
+      for (int c = std::max(0, r - h); c <= std::min(nn - 1, r + h); ++c) A(r, c) = 0.0;
+      A(r, r) = 1.0;
+      b[r] = quad_at(o, q, g.x_n[corner], g.y_n[corner]);
+    };
+    for (int jj = 0; jj < nx; ++jj) { dirichlet(jj, 0, 0); dirichlet(jj, 0, 1); }
+    for (int jj = 0; jj < nx; ++jj) { dirichlet(jj, ny - 1, 2); dirichlet(jj, ny - 1, 3); }
+    for (int jj = 0; jj < ny; ++jj) { dirichlet(0, jj, 0); dirichlet(0, jj, 2); }
+    for (int jj = 0; jj < ny; ++jj) { dirichlet(nx - 1, jj, 1); dirichlet(nx - 1, jj, 3); }
+    band_solve(band, nn, h, b);
+    x = b;
+  } else {
+    // periodic plane: unique nodes (i mod nx, j mod ny)
+    const int nu = nx * ny;
+    std::vector<double> A((size_t)nu * nu, 0.0), b(nu, 0.0);
+    auto uid = [&](int id) {
+      const int i = id % (nx + 1), j = id / (nx + 1);
+      return (j % ny) * nx + (i % nx);
+    };
+    for (int k = 0; k < ny; ++k)
+      for (int j = 0; j < nx; ++j) {
+        const int num = k * nx + j;
+        const CellGeo g = cell_geo(o, j, k);
+        double v[4];
+        for (int a = 0; a < 4; ++a) {
+          v[a] = 0.0;
+          for (int c = 0; c < 4; ++c) v[a] += o.n_n[4 * (size_t)num + c] * uv[a][c];
+        }
+        for (int a = 0; a < 4; ++a) {
+          for (int c = 0; c < 4; ++c) {
+            A[(size_t)uid(g.id[a]) * nu + uid(g.id[c])] += dudv[a][c];
+            A[(size_t)uid(g.id[a]) * nu + uid(g.id[c])] += lambda2 * uv[a][c] * V;
+          }
+          b[uid(g.id[a])] += v[a] * V;
+        }
+      }
+    dense_solve(A, nu, b);
+    for (int id = 0; id < nn; ++id) x[id] = b[uid(id)];
+  }
+  // core subtraction (FEM.cpp:497-554)
+  double dummy = 1.0 - (p.sigma * p.bst + 1.0) * exp(-p.bst * p.sigma);
+  dummy = dummy / p.bst / p.bst;
+  double dummy2 = (-1.0 / 6.0 * pow(p.bst, 3) * pow(p.sigma, 3) -
+                   1.0 / 2.0 * pow(p.bst, 2) * pow(p.sigma, 2) - p.bst * p.sigma +
+                   exp(p.bst * p.sigma) - 1) *
+                  exp(-p.bst * p.sigma) / pow(p.bst, 4);
+  dummy2 = dummy2 / pow(o.dx, 2);
+  if (!periodic) {
+    for (int j = 0; j < ny + 1; ++j)
+      for (int i = 0; i < nx + 1; ++i) {
+        const int id = j * (nx + 1) + i;
+        int ii = i, jj = j, k = 0;
+        int ipm[4] = {0, -1, 0, 0}, jpm[4] = {0, 0, 0, -1}, kpm[4] = {1, 0, 2, 0};
+        if (i == nx) { ii = i - 1; jj = j; k = 1; kpm[0] = k; }
+        if (j == ny) { ii = i; jj = j - 1; k = 2; kpm[2] = k; }
+        if (i == nx && j == ny) { ii = i - 1; jj = j - 1; k = 3; kpm[0] = k; kpm[2] = k; }
+        if (i == 0) { ipm[1] = 0; jpm[1] = 0; kpm[1] = 0; }
+        if (j == 0) { ipm[3] = 0; jpm[3] = 0; kpm[3] = 0; }
+        int num = jj * nx + ii;
+        const double d2 = o.n_n[4 * (size_t)num + k] * dummy;
+        double d3 = -4.0 * o.n_n[4 * (size_t)num + k] * dummy2;
+        for (int ll = 0; ll < 4; ++ll) {
+          num = (jj + jpm[ll]) * nx + ii + ipm[ll];
+          d3 += o.n_n[4 * (size_t)num + kpm[ll]] * dummy2;
+        }
+        x[id] += -(d2 + d3);
+      }
+  } else {
+    const std::vector<double> nv = node_values(o);
+    auto nat = [&](int i, int j) {
+      i = (i % nx + nx) % nx;
+      j = (j % ny + ny) % ny;
+      return nv[(size_t)j * (nx + 1) + i];
+    };
+    for (int j = 0; j < ny + 1; ++j)
+      for (int i = 0; i < nx + 1; ++i) {
+        const double nc = nat(i, j);
+        const double d2 = nc * dummy;
+        double d3 = -4.0 * nc * dummy2;
+        d3 += nat(i + 1, j) * dummy2;
+        d3 += nat(i - 1, j) * dummy2;
+        d3 += nat(i, j + 1) * dummy2;
+        d3 += nat(i, j - 1) * dummy2;
+        x[(size_t)j * (nx + 1) + i] += -(d2 + d3);
+      }
+  }
+  o.pot = x;
+  o.kick_coef = p.ast;
+  o.kick_pending = true;
+}
+
+// compute_Vlasov_2D (FEM.cpp:613-706) up to the nodal field.
+void orc_vlasov2d(Oracle &o) {
+  const Quad q = make_quad(o, false);
+  o.pot.assign((size_t)(o.nx + 1) * (o.ny + 1), 0.0);
+  for (int j = 0; j < o.ny + 1; ++j)
+    for (int i = 0; i < o.nx + 1; ++i) {
+      int ii, jj, k;
+      owner(o, i, j, &ii, &jj, &k);
+      const CellGeo g = cell_geo(o, ii, jj);
+      o.pot[(size_t)j * (o.nx + 1) + i] = quad_at(o, q, g.x_n[k], g.y_n[k]);
+    }
+  o.kick_coef = o.p.phi;
+  o.kick_pending = true;
+}
+
 int take_err(Oracle &o) {
   uint32_t e = o.err;
   o.err = 0;
@@ -1045,7 +1624,22 @@ void *orc_create(const mcpm_params *p) {
   if (!p || p->n_cells < 1) return nullptr;
   Oracle *o = new Oracle();
   o->p = *p;
-  o->nc = p->n_cells;
+  if (o->p.dims == 0) o->p.dims = 1;
+  o->dims = o->p.dims;
+  if (o->dims == 2) {
+    if (p->n_cells_x < 1) {
+      delete o;
+      return nullptr;
+    }
+    o->nx = p->n_cells_x;
+    o->ny = p->n_cells;
+    o->dx = p->length_x / (double)p->n_cells_x;
+    o->S = 5;
+    o->U = 2;
+  } else {
+    o->ny = p->n_cells;
+  }
+  o->nc = o->nx * o->ny;
   o->dy = p->length / (double)p->n_cells;
   o->cnt.assign(o->nc, 0);
   o->start.assign(o->nc + 1, 0);
@@ -1062,6 +1656,7 @@ void orc_destroy(void *h) { delete (Oracle *)h; }
 int orc_upload(void *h, int64_t n, const double *y, const double *u1, const double *u2,
                const double *u3) {
   Oracle &o = *(Oracle *)h;
+  if (o.dims != 1) return MCPM_ERR_ARG;
   o.n = n;
   o.rec.assign(4 * (size_t)n, 0.0);
   bool f32 = o.p.precision == MCPM_FP32;
@@ -1074,13 +1669,69 @@ int orc_upload(void *h, int64_t n, const double *y, const double *u1, const doub
   return MCPM_OK;
 }
 
+int orc_upload_xy(void *h, int64_t n, const double *x, const double *y, const double *u1,
+                  const double *u2, const double *u3) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 2) return MCPM_ERR_ARG;
+  o.n = n;
+  o.rec.assign(5 * (size_t)n, 0.0);
+  bool f32 = o.p.precision == MCPM_FP32;
+  for (int64_t k = 0; k < n; ++k) {
+    double v[5] = {x[k], y[k], u1[k], u2[k], u3[k]};
+    for (int q = 0; q < 5; ++q) o.rec[5 * k + q] = f32 ? (double)(float)v[q] : v[q];
+  }
+  o.sorted = false;
+  o.kick_pending = false;
+  return MCPM_OK;
+}
+
+int orc_download_xy(void *h, double *x, double *y, double *u1, double *u2, double *u3) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 2) return MCPM_ERR_ARG;
+  double *dst[5] = {x, y, u1, u2, u3};
+  for (int64_t k = 0; k < o.n; ++k)
+    for (int q = 0; q < 5; ++q)
+      if (dst[q]) dst[q][k] = o.rec[5 * k + q];
+  return MCPM_OK;
+}
+
+int orc_get_nodes(void *h, double *n_node, double *pot) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 2 || !o.sorted) return MCPM_ERR_ARG;
+  const size_t nn = (size_t)(o.nx + 1) * (o.ny + 1);
+  if (n_node) {
+    orc_deposit(o);
+    std::vector<double> v = node_values(o);
+    memcpy(n_node, v.data(), nn * sizeof(double));
+  }
+  if (pot) {
+    if (o.pot.size() != nn) o.pot.assign(nn, 0.0);
+    memcpy(pot, o.pot.data(), nn * sizeof(double));
+  }
+  return MCPM_OK;
+}
+
+void orc_gl_full(int n, double *x, double *w) { gl_full(n, x, w); }
+
+/* Probe of the plane's force interpolation at (x, y) for a given nodal field. */
+int orc_fem_force(void *h, double x, double y, const double *pot, double coef, double *F1,
+                  double *F2) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 2) return MCPM_ERR_ARG;
+  const int num = orc_cell(y, o.dy, o.ny) * o.nx + orc_cell(x, o.dx, o.nx);
+  fem_force(o, num, x, y, pot, coef, F1, F2);
+  return MCPM_OK;
+}
+
 int orc_download(void *h, double *y, double *u1, double *u2, double *u3) {
   Oracle &o = *(Oracle *)h;
+  const int y0 = o.dims == 2 ? 1 : 0;
   for (int64_t k = 0; k < o.n; ++k) {
-    if (y) y[k] = o.rec[4 * k];
-    if (u1) u1[k] = o.rec[4 * k + 1];
-    if (u2) u2[k] = o.rec[4 * k + 2];
-    if (u3) u3[k] = o.rec[4 * k + 3];
+    const double *r = &o.rec[(size_t)o.S * k] + y0;
+    if (y) y[k] = r[0];
+    if (u1) u1[k] = r[1];
+    if (u2) u2[k] = r[2];
+    if (u3) u3[k] = r[3];
   }
   return MCPM_OK;
 }
@@ -1102,6 +1753,10 @@ int orc_collide_esmc(void *h, int64_t step, mcpm_collision_stats *st) {
 int orc_poisson_solver(void *h) {
   Oracle &o = *(Oracle *)h;
   if (!o.sorted) return MCPM_ERR_STATE;
+  if (o.dims == 2) {
+    orc_sp2d(o);
+    return MCPM_OK;
+  }
   orc_field(o);
   return MCPM_OK;
 }
@@ -1130,6 +1785,10 @@ int orc_stream_bc(void *h, int64_t step, int32_t with_kick) {
 int orc_vlasov_integral(void *h) {
   Oracle &o = *(Oracle *)h;
   if (!o.sorted) return MCPM_ERR_STATE;
+  if (o.dims == 2) {
+    orc_vlasov2d(o);
+    return MCPM_OK;
+  }
   orc_vlasov(o);
   return MCPM_OK;
 }
@@ -1201,9 +1860,10 @@ int orc_step(void *h, int64_t step, mcpm_collision_stats *st) {
     default: break;
   }
   if (r) return r;
-  if (o.p.long_range == MCPM_LR_SCREENED_POISSON) orc_field(o);
-  else if (o.p.long_range == MCPM_LR_DIRECT) orc_vlasov(o);
+  if (o.p.long_range == MCPM_LR_SCREENED_POISSON) r = orc_poisson_solver(h);
+  else if (o.p.long_range == MCPM_LR_DIRECT) r = orc_vlasov_integral(h);
   else if (o.p.long_range == MCPM_LR_DENSITY_EXPANSION) orc_lr2(o);
+  if (r) return r;
   r = orc_stream_bc(h, step, 1);
   o.have_closures = false;
   return r;

@@ -39,6 +39,12 @@ def load() -> ctypes.CDLL:
         "orc_destroy": (None, [vp]),
         "orc_upload": (i32, [vp, i64, dpp, dpp, dpp, dpp]),
         "orc_download": (i32, [vp, dpp, dpp, dpp, dpp]),
+        "orc_upload_xy": (i32, [vp, i64, dpp, dpp, dpp, dpp, dpp]),
+        "orc_download_xy": (i32, [vp, dpp, dpp, dpp, dpp, dpp]),
+        "orc_get_nodes": (i32, [vp, dpp, dpp]),
+        "orc_gl_full": (None, [i32, dpp, dpp]),
+        "orc_fem_force": (i32, [vp, ctypes.c_double, ctypes.c_double, dpp, ctypes.c_double,
+                                dpp, dpp]),
         "orc_cell_update": (i32, [vp]),
         "orc_collide_esmc": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_poisson_solver": (i32, [vp]),
@@ -119,7 +125,7 @@ class OracleSim:
 
     @property
     def n_cells(self):
-        return self.params.n_cells
+        return self.params.total_cells
 
     def upload(self, y, u1, u2, u3):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (y, u1, u2, u3)]
@@ -130,6 +136,34 @@ class OracleSim:
         out = [np.empty(self._n) for _ in range(4)]
         self._check(self._lib.orc_download(self._h, *[_dp(a) for a in out]), "orc_download")
         return tuple(out)
+
+    def upload_xy(self, x, y, u1, u2, u3):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, y, u1, u2, u3)]
+        self._n = arrs[0].shape[0]
+        self._check(self._lib.orc_upload_xy(self._h, self._n, *[_dp(a) for a in arrs]),
+                    "orc_upload_xy")
+
+    def download_xy(self):
+        out = [np.empty(self._n) for _ in range(5)]
+        self._check(self._lib.orc_download_xy(self._h, *[_dp(a) for a in out]), "orc_download_xy")
+        return tuple(out)
+
+    def nodes(self):
+        nn = self.params.n_nodes
+        nd = np.empty(nn)
+        pot = np.empty(nn)
+        self._check(self._lib.orc_get_nodes(self._h, _dp(nd), _dp(pot)), "orc_get_nodes")
+        shape = (self.params.n_cells + 1, self.params.n_cells_x + 1)
+        return nd.reshape(shape), pot.reshape(shape)
+
+    def fem_force(self, x, y, pot, coef):
+        """Probe: force interpolation of the plane at (x, y) from nodal field pot."""
+        pot = np.ascontiguousarray(pot, dtype=np.float64).ravel()
+        f1 = ctypes.c_double()
+        f2 = ctypes.c_double()
+        self._check(self._lib.orc_fem_force(self._h, float(x), float(y), _dp(pot), float(coef),
+                                            ctypes.byref(f1), ctypes.byref(f2)), "orc_fem_force")
+        return f1.value, f2.value
 
     def cells(self):
         nc = self.n_cells
@@ -266,6 +300,13 @@ def gl50():
     x = np.zeros(50)
     w = np.zeros(50)
     load().orc_gl50(_dp(x), _dp(w))
+    return x, w
+
+
+def gl_full(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    load().orc_gl_full(int(n), _dp(x), _dp(w))
     return x, w
 
 

@@ -41,7 +41,20 @@ def test_status_strings_and_host_contact_value():
     assert abs(mcpm.contact_y(p, 0.2) - 1.3205854) < 1e-6
 
 
-def test_params_struct_layout_matches_header():
-    # sizeof(mcpm_params): 6 int32 + int64 + uint64 + 23 doubles
-    assert ctypes.sizeof(mcpm.CParams) == 6 * 4 + 8 + 8 + 23 * 8
-    assert mcpm.CParams.length.offset == 40
+def test_params_struct_layout_matches_header(tmp_path):
+    """Every field offset of the ctypes mirror equals the C compiler's
+    offsetof on include/mcpm.h, and so does the struct size."""
+    fields = [f[0] for f in mcpm.CParams._fields_]
+    src = tmp_path / "probe.c"
+    body = "".join(f'  printf("%s %zu\\n", "{f}", offsetof(mcpm_params, {f}));\n' for f in fields)
+    src.write_text("#include <stddef.h>\n#include <stdio.h>\n#include \"mcpm.h\"\n"
+                   "int main(void) {\n" + body +
+                   '  printf("sizeof %zu\\n", sizeof(mcpm_params));\n  return 0;\n}\n')
+    exe = tmp_path / "probe"
+    inc = os.path.join(ROOT, "include")
+    subprocess.run(["gcc", "-std=c99", "-I", inc, str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    got = dict(line.split() for line in out.strip().splitlines())
+    for f in fields:
+        assert getattr(mcpm.CParams, f).offset == int(got[f]), f
+    assert ctypes.sizeof(mcpm.CParams) == int(got["sizeof"])
