@@ -43,7 +43,9 @@ struct mcpm_ctx {
   uint32_t *counters = nullptr, *err = nullptr;
   uint32_t *res[2] = {nullptr, nullptr};
   // collisions
-  uint4 *opA = nullptr, *opB = nullptr;  // one allocation: ping-pong op lists
+  uint4 *opA = nullptr, *opB = nullptr;  // one allocation: ping-pong op lists,
+  uint4 *opBand = nullptr;               // then the priority-band lists
+  int nbands = 8;
   OpLog log{};                           // one allocation: debug log
   int64_t op_cap = 0;
   bool debug_log = false;
@@ -440,14 +442,15 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   HIPCHK(hipStreamSynchronize(c->s));
   if (c->opA) HIPCHK(hipFree(c->opA));
   if (c->log.p) HIPCHK(hipFree(c->log.p));
-  c->opA = c->opB = nullptr;
+  c->opA = c->opB = c->opBand = nullptr;
   c->log = OpLog{};
   c->op_cap = 0;
   uint4 *blk = nullptr;
-  int st = dalloc(&blk, (size_t)cap * 2);
+  int st = dalloc(&blk, (size_t)cap * 3);
   if (st) return st;
   c->opA = blk;
   c->opB = blk + cap;
+  c->opBand = blk + 2 * cap;
   uint32_t *lg = nullptr;
   st = dalloc(&lg, (size_t)cap * 3 + ((size_t)cap + 3) / 4);
   if (st) return st;
@@ -477,24 +480,32 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   if (st) return st;
   mcpm_u32x4 pk = mcpm_philox(0u, 0u, slo, mcpm_ctr3(MCPM_TAG_PRIO, shi, 0u), c->kp.key0,
                               c->kp.key1);
+  // ESMC priorities spread over 32 bits and enter the rounds in bands; DSMC
+  // operations of different cells never conflict, one band keeps every cell
+  // busy from the first round
+  const int nbands = MODEL == COLL_ESMC ? std::max(1, std::min<int>(c->nbands, MAX_BANDS)) : 1;
+  const uint32_t band_mul = MODEL == COLL_ESMC ? (uint32_t)nbands : 0u;
+  // a multiple of 8 blocks so that every XCD gets the same share
+  unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256), 4096);
+  grid = (grid + 7u) / 8u * 8u;
   {
     Timer tm(c, T_GEN);
     hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
-                       c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->opA, c->res[0],
-                       c->counters);
+                       c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->opA, c->counters,
+                       band_mul, nbands);
+    hipLaunchKernelGGL(k_partition, dim3(grid), dim3(256), 0, c->s, (const uint4 *)c->opA, M,
+                       c->opBand, c->res[0], c->counters, band_mul, nbands);
     LAUNCHCHK();
   }
   int rounds = 0;
   {
     Timer tm(c, T_COLLIDE);
-    // a multiple of 8 blocks so that every XCD gets the same share
-    unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256), 4096);
-    grid = (grid + 7u) / 8u * 8u;
     OpLog lg = c->debug_log ? c->log : OpLog{};
-    int next_check = 12;
+    int next_check = std::max(12, nbands + 2);
     for (int r = 0; r < MAX_ROUNDS; ++r) {
-      hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, M, P,
-                         (const uint4 *)((r & 1) ? c->opB : c->opA), (r & 1) ? c->opA : c->opB,
+      hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, P,
+                         (const uint4 *)((r & 1) ? c->opB : c->opA),
+                         (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
                          c->res[r & 1], c->res[(r + 1) & 1], c->counters,
                          c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg);
       LAUNCHCHK();

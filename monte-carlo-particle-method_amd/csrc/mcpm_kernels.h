@@ -16,9 +16,12 @@ enum : int {
   CNT_OPS = 0,     // valid candidate operations of the current step
   CNT_ACC = 2,     // accepted collisions (u64, 8-byte aligned)
   CNT_LOG = 4,     // entries in the debug collision log
-  CNT_LIST = 6,    // CNT_LIST + r: length of the list processed by round r (r >= 1)
+  CNT_LIST = 6,    // CNT_LIST + r: losers of round r - 1 carried into round r (r >= 1)
   MAX_ROUNDS = 512,
-  CNT_TOTAL = CNT_LIST + MAX_ROUNDS + 2
+  MAX_BANDS = 16,
+  CNT_BAND = CNT_LIST + MAX_ROUNDS + 2,  // operations per priority band
+  CNT_FILL = CNT_BAND + MAX_BANDS,       // partition fill pointers per band
+  CNT_TOTAL = CNT_FILL + MAX_BANDS
 };
 
 // Candidate operation {i, j, priority, t} as one uint4; i = OP_NONE marks a
@@ -57,13 +60,21 @@ __global__ void k_mcand(const uint32_t *cnt, const double *omax, const double *n
                         uint32_t *mcand, int nc, KParams kp, int model, uint32_t step_lo,
                         uint32_t step_hi);
 __global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc);
+// Priority band of an operation: the top bits of the priority scaled to
+// nbands (band_mul = nbands for priorities spread over 32 bits, 0 for one band).
+__device__ __forceinline__ uint32_t op_band(uint32_t pr, uint32_t band_mul, int nbands) {
+  uint32_t b = (uint32_t)(((uint64_t)pr * band_mul) >> 32);
+  return b < (uint32_t)nbands ? b : (uint32_t)nbands - 1u;
+}
 template <typename R, int MODEL>
 __global__ void k_gen(const R *P, const uint32_t *cnt, const uint32_t *start,
                       const uint32_t *mcand, const uint32_t *cstart, KParams kp,
                       uint32_t step_lo, uint32_t step_hi, uint32_t prio_key, uint4 *ops,
-                      uint32_t *res0, uint32_t *counters);
+                      uint32_t *counters, uint32_t band_mul, int nbands);
+__global__ void k_partition(const uint4 *ops, uint32_t M, uint4 *bands, uint32_t *res0,
+                            uint32_t *counters, uint32_t band_mul, int nbands);
 template <typename R, int MODEL>
-__global__ void k_round(int r, uint32_t count0, R *P, const uint4 *in, uint4 *out,
+__global__ void k_round(int r, int nbands, R *P, const uint4 *in, const uint4 *bands, uint4 *out,
                         uint32_t *res_cur, uint32_t *res_next, uint32_t *counters,
                         const double *nden, const double *yc, const double *omax,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,

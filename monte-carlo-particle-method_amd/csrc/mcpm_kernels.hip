@@ -463,10 +463,10 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 }
 
 // One workgroup per cell: draw the cell's m_c candidates and store each at
-// its candidate index as {i, j, priority, t} (invalid ones as OP_NONE), so
-// that the first round streams operations in cell order; reserve both
-// particles.  ESMC priorities are the keyed bijection of the candidate
-// index; DSMC keeps the reference's in-cell candidate order.
+// its candidate index as {i, j, priority, t} (invalid ones as OP_NONE), in
+// cell order; count the valid ones per priority band.  ESMC priorities are
+// the keyed bijection of the candidate index; DSMC keeps the reference's
+// in-cell candidate order.
 template <typename R, int MODEL>
 __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              const uint32_t *__restrict__ cnt,
@@ -475,11 +475,14 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              const uint32_t *__restrict__ cstart, KParams kp,
                                              uint32_t step_lo, uint32_t step_hi,
                                              uint32_t prio_key, uint4 *__restrict__ ops,
-                                             uint32_t *__restrict__ res0,
-                                             uint32_t *__restrict__ counters) {
+                                             uint32_t *__restrict__ counters,
+                                             uint32_t band_mul, int nbands) {
+  __shared__ uint32_t sband[MAX_BANDS];
   const uint32_t c = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t m = mcand[c];
   if (m == 0) return;
+  if (threadIdx.x < MAX_BANDS) sband[threadIdx.x] = 0;
+  __syncthreads();
   const uint32_t k0 = cstart[c];
   uint32_t nvalid = 0;
   for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
@@ -497,8 +500,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
     if (ok) {
       uint32_t pr = MODEL == COLL_DSMC ? k0 + t : mcpm_priority(k0 + t, prio_key);
       rec = make_uint4(i, j, pr, t);
-      atomicMin(&res0[i], pr);
-      atomicMin(&res0[j], pr);
+      atomicAdd(&sband[op_band(pr, band_mul, nbands)], 1u);
       ++nvalid;
     } else {
       rec = make_uint4(OP_NONE, 0u, 0u, 0u);
@@ -507,6 +509,52 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
   }
   for (int off = 32; off > 0; off >>= 1) nvalid += __shfl_down(nvalid, off);
   if (lane_id() == 0 && nvalid) atomicAdd(&counters[CNT_OPS], nvalid);
+  __syncthreads();
+  if ((int)threadIdx.x < nbands && sband[threadIdx.x])
+    atomicAdd(&counters[CNT_BAND + threadIdx.x], sband[threadIdx.x]);
+}
+
+// Scatter the valid operations into one list per priority band (band b at
+// offset sum of the earlier bands' counts; the order inside a band does not
+// matter) and reserve the particles of band 0 for the first round.
+__global__ __launch_bounds__(256) void k_partition(const uint4 *__restrict__ ops, uint32_t M,
+                                                   uint4 *__restrict__ bands,
+                                                   uint32_t *__restrict__ res0,
+                                                   uint32_t *__restrict__ counters,
+                                                   uint32_t band_mul, int nbands) {
+  __shared__ uint32_t sbase[MAX_BANDS], scount[MAX_BANDS], sstart[MAX_BANDS];
+  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int b = 0; b < nbands; ++b) {
+      sstart[b] = run;
+      run += counters[CNT_BAND + b];
+    }
+  }
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t base = lb * blockDim.x; base < M; base += stride) {
+    if (threadIdx.x < MAX_BANDS) scount[threadIdx.x] = 0;
+    __syncthreads();
+    const uint32_t idx = base + threadIdx.x;
+    uint4 rec = make_uint4(OP_NONE, 0u, 0u, 0u);
+    if (idx < M) rec = ops[idx];
+    uint32_t b = 0, rank = 0;
+    const bool valid = rec.x != OP_NONE;
+    if (valid) {
+      b = op_band(rec.z, band_mul, nbands);
+      rank = atomicAdd(&scount[b], 1u);
+      if (b == 0) {
+        atomicMin(&res0[rec.x], rec.z);
+        atomicMin(&res0[rec.y], rec.z);
+      }
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nbands && scount[threadIdx.x])
+      sbase[threadIdx.x] = atomicAdd(&counters[CNT_FILL + threadIdx.x], scount[threadIdx.x]);
+    __syncthreads();
+    if (valid) bands[sstart[b] + sbase[b] + rank] = rec;
+    __syncthreads();
+  }
 }
 
 // The collision itself, collisions.cpp:680-716: Y at the midpoint cell,
@@ -602,9 +650,15 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, 
 // into the next operation list.  Each operation that finds its own priority
 // in the current array clears it, so the array is back to all-ones when the
 // round ends.
+// Operations enter by priority band: round r takes the losers of round r-1
+// plus band r, and reserves band r+1 for round r+1.  Every operation of a
+// lower priority than an admitted one is admitted too, so the smallest
+// reservation among admitted operations is the smallest among all unresolved
+// ones; late operations no longer lose round after round to early ones.
 template <typename R, int MODEL>
-__global__ __launch_bounds__(256) void k_round(int r, uint32_t count0, R *__restrict__ P,
+__global__ __launch_bounds__(256) void k_round(int r, int nbands, R *__restrict__ P,
                                                const uint4 *__restrict__ in,
+                                               const uint4 *__restrict__ bands,
                                                uint4 *__restrict__ out,
                                                uint32_t *__restrict__ res_cur,
                                                uint32_t *__restrict__ res_next,
@@ -615,14 +669,24 @@ __global__ __launch_bounds__(256) void k_round(int r, uint32_t count0, R *__rest
                                                double *__restrict__ omega_new, KParams kp,
                                                uint32_t step_lo, uint32_t step_hi,
                                                FPParams fpp, OpLog log) {
-  const uint32_t count = (r == 0) ? count0 : counters[CNT_LIST + r];
+  const uint32_t countA = (r == 0) ? 0u : counters[CNT_LIST + r];
+  uint32_t bstart = 0, countB = 0, countC = 0;
+  for (int b = 0; b < nbands && b <= r + 1; ++b) {
+    const uint32_t nb_ = counters[CNT_BAND + b];
+    if (b < r) bstart += nb_;
+    else if (b == r) countB = nb_;
+    else countC = nb_;
+  }
+  const uint32_t count = countA + countB;
+  const uint4 *inB = bands + bstart;
   const uint32_t stride = gridDim.x * blockDim.x;
   const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
   uint32_t acc_local = 0;
   for (uint32_t base = lb * blockDim.x; base < count; base += stride) {
     uint32_t idx = base + threadIdx.x;
     uint4 rec = make_uint4(OP_NONE, 0u, 0u, 0u);
-    if (idx < count) rec = in[idx];
+    if (idx < countA) rec = in[idx];
+    else if (idx < count) rec = inB[idx - countA];
     bool lose = false;
     if (rec.x != OP_NONE) {
       const uint32_t i = rec.x, j = rec.y, pr = rec.z;
@@ -653,6 +717,13 @@ __global__ __launch_bounds__(256) void k_round(int r, uint32_t count0, R *__rest
     }
     uint32_t pos = wave_append(&counters[CNT_LIST + r + 1], lose);
     if (lose) out[pos] = rec;
+  }
+  // admit band r+1: reserve its particles for the next round
+  const uint4 *inC = inB + countB;
+  for (uint32_t idx = lb * blockDim.x + threadIdx.x; idx < countC; idx += stride) {
+    const uint4 rec = inC[idx];
+    atomicMin(&res_next[rec.x], rec.z);
+    atomicMin(&res_next[rec.y], rec.z);
   }
   // accepted count, one atomic per wavefront
   for (int off = 32; off > 0; off >>= 1) acc_local += __shfl_down(acc_local, off);
@@ -951,8 +1022,9 @@ __global__ __launch_bounds__(256) void k_sample(const R *__restrict__ P,
   template __global__ void k_gen<R, M>(const R *, const uint32_t *, const uint32_t *,      \
                                        const uint32_t *, const uint32_t *, KParams,        \
                                        uint32_t, uint32_t, uint32_t, uint4 *, uint32_t *,  \
-                                       uint32_t *);                                        \
-  template __global__ void k_round<R, M>(int, uint32_t, R *, const uint4 *, uint4 *,       \
+                                       uint32_t, int);                                     \
+  template __global__ void k_round<R, M>(int, int, R *, const uint4 *, const uint4 *,      \
+                                         uint4 *,                                          \
                                          uint32_t *, uint32_t *, uint32_t *,               \
                                          const double *, const double *, const double *,   \
                                          double *, KParams, uint32_t, uint32_t, FPParams,  \
