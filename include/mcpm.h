@@ -301,6 +301,20 @@ int mcpm_get_collision_log(mcpm_ctx *ctx, int64_t cap, uint32_t *prio, uint32_t 
 int mcpm_timing_enable(mcpm_ctx *ctx, int32_t enable);
 int mcpm_timing_get(mcpm_ctx *ctx, int32_t which, double *avg_ms, int64_t *calls);
 
+/* ---- launch-shape knobs ---------------------------------------------------- */
+/* No counterpart in the reference (its loops are serial): the tile and
+   launch sizes of the GPU schedule, for an autotuner.  None of them changes
+   any result, only the speed.
+   MCPM_TUNE_BANDS       priority bands of the ESMC reservation rounds, 1..16
+   MCPM_TUNE_ROUND_GRID  most workgroups of one reservation round (multiple of 8)
+   MCPM_TUNE_SORT_CHUNK  particles per sort chunk (multiple of 256, 1024..32768;
+                         0 = about 2048 chunks) */
+#define MCPM_TUNE_BANDS      0
+#define MCPM_TUNE_ROUND_GRID 1
+#define MCPM_TUNE_SORT_CHUNK 2
+int mcpm_set_tuning(mcpm_ctx *ctx, int32_t knob, int64_t value);
+int mcpm_get_tuning(mcpm_ctx *ctx, int32_t knob, int64_t *value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -41,11 +41,14 @@ struct mcpm_ctx {
   double *nden = nullptr, *yc = nullptr, *omax = nullptr, *omega_new = nullptr, *F2 = nullptr;
   double *moments = nullptr, *samples = nullptr;
   uint32_t *counters = nullptr, *err = nullptr;
-  uint32_t *res[2] = {nullptr, nullptr};
+  uint32_t *res = nullptr;  // reservations, two interleaved arrays
   // collisions
   uint4 *opA = nullptr, *opB = nullptr;  // one allocation: ping-pong op lists,
   uint4 *opBand = nullptr;               // then the priority-band lists
-  int nbands = 8;
+  uint32_t *gen_table = nullptr;         // per-chunk list sizes of the candidate draw
+  int nbands = 12;
+  int round_grid = 2048;
+  int sort_chunk = 0;
   OpLog log{};                           // one allocation: debug log
   int64_t op_cap = 0;
   bool debug_log = false;
@@ -380,7 +383,8 @@ int check_err(mcpm_ctx *c) {
   return MCPM_ERR_CAPACITY;
 }
 
-int chunk_for(int64_t n) {
+int chunk_for(int64_t n, int fixed) {
+  if (fixed > 0) return fixed;
   // about 2048 chunks (one wavefront each), between 1024 and 32768 particles
   int64_t ch = (n + 2047) / 2048;
   ch = (ch + 255) / 256 * 256;
@@ -399,7 +403,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   }
   Timer tm(c, T_SORT);
   const int nc = c->nc;
-  const int chunk = chunk_for(c->n);
+  const int chunk = chunk_for(c->n, c->sort_chunk);
   const int64_t nch = (c->n + chunk - 1) / chunk;
   if (nch > c->nchunks_cap) return MCPM_ERR_CAPACITY;
   const int nseg = (int)((nch + c->seg - 1) / c->seg);
@@ -451,6 +455,9 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   c->opA = blk;
   c->opB = blk + cap;
   c->opBand = blk + 2 * cap;
+  if (c->gen_table) HIPCHK(hipFree(c->gen_table));
+  c->gen_table = nullptr;
+  if ((st = dalloc(&c->gen_table, ((size_t)cap / GEN_CHUNK + 2) * NLIST))) return st;
   uint32_t *lg = nullptr;
   st = dalloc(&lg, (size_t)cap * 3 + ((size_t)cap + 3) / 4);
   if (st) return st;
@@ -486,35 +493,49 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   const int nbands = MODEL == COLL_ESMC ? std::max(1, std::min<int>(c->nbands, MAX_BANDS)) : 1;
   const uint32_t band_mul = MODEL == COLL_ESMC ? (uint32_t)nbands : 0u;
   // a multiple of 8 blocks so that every XCD gets the same share
-  unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256), 4096);
+  unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256),
+                                              c->round_grid);
   grid = (grid + 7u) / 8u * 8u;
-  {
+  if (M > 0) {
     Timer tm(c, T_GEN);
-    hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nc), dim3(256), 0, c->s, P, c->cnt, c->start,
-                       c->mcand, c->cstart, c->kp, slo, shi, pk.v[0], c->opA, c->counters,
-                       band_mul, nbands);
-    hipLaunchKernelGGL(k_partition, dim3(grid), dim3(256), 0, c->s, (const uint4 *)c->opA, M,
-                       c->opBand, c->res[0], c->counters, band_mul, nbands);
+    const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
+    const int esmc = MODEL == COLL_ESMC;
+    hipLaunchKernelGGL(k_cand_count, dim3(nchunks), dim3(256), 0, c->s, M, pk.v[0], esmc,
+                       band_mul, nbands, c->gen_table);
+    hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
+                       nchunks, nbands * NREG, c->counters);
+    hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nchunks), dim3(256), 0, c->s, P, c->cnt, c->start,
+                       c->cstart, nc, M, c->kp, slo, shi, pk.v[0], band_mul, nbands,
+                       c->gen_table, c->opBand, c->res, c->counters);
     LAUNCHCHK();
   }
   int rounds = 0;
+  const uint32_t seg = (M + NREG - 1) / NREG;  // loser-list segment of one region
+  auto left_after = [&](int rr, uint32_t *left) -> int {
+    uint32_t v[NREG];
+    HIPCHK(hipMemcpyAsync(v, c->counters + CNT_LIST + rr * NREG, sizeof v, hipMemcpyDeviceToHost,
+                          c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    *left = 0;
+    for (int x = 0; x < NREG; ++x) *left += v[x];
+    return MCPM_OK;
+  };
   {
     Timer tm(c, T_COLLIDE);
     OpLog lg = c->debug_log ? c->log : OpLog{};
     int next_check = std::max(12, nbands + 2);
     for (int r = 0; r < MAX_ROUNDS; ++r) {
-      hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, P,
+      hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, P,
                          (const uint4 *)((r & 1) ? c->opB : c->opA),
                          (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
-                         c->res[r & 1], c->res[(r + 1) & 1], c->counters,
+                         c->res + (r & 1), c->res + ((r + 1) & 1), c->counters,
                          c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg);
       LAUNCHCHK();
       rounds = r + 1;
       if (rounds == next_check) {
         uint32_t left = 0;
-        HIPCHK(hipMemcpyAsync(&left, c->counters + CNT_LIST + rounds, sizeof left,
-                              hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
+        int st2 = left_after(rounds, &left);
+        if (st2) return st2;
         if (left == 0) break;
         next_check += 4;
       }
@@ -527,7 +548,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   HIPCHK(hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   uint32_t left = 0;
-  HIPCHK(hipMemcpy(&left, c->counters + CNT_LIST + rounds, sizeof left, hipMemcpyDeviceToHost));
+  if ((st = left_after(rounds, &left))) return st;
   if (left != 0) return MCPM_ERR_CAPACITY;  // more rounds than MAX_ROUNDS
   c->last_ops = hc[CNT_OPS];
   c->last_log = hc[CNT_LOG];
@@ -740,8 +761,7 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   if ((st = dalloc(&c->clos, (size_t)nc * MCPM_NCLOS))) return fail(st);
   if ((st = dalloc(&c->drift, (size_t)nc * DRIFT_N))) return fail(st);
   if ((st = dalloc(&c->err, 1))) return fail(st);
-  if ((st = dalloc(&c->res[0], (size_t)p->capacity))) return fail(st);
-  if ((st = dalloc(&c->res[1], (size_t)p->capacity))) return fail(st);
+  if ((st = dalloc(&c->res, 2 * (size_t)p->capacity))) return fail(st);
   if (hipMemset(c->omax, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->omega_new, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->F2, 0, nc * sizeof(double)) != hipSuccess ||
@@ -749,8 +769,7 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
       hipMemset(c->err, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->cnt, 0, nc * sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->start, 0, (nc + 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMemset(c->res[0], 0xFF, (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess ||
-      hipMemset(c->res[1], 0xFF, (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess)
+      hipMemset(c->res, 0xFF, 2 * (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess)
     return fail(MCPM_ERR_HIP);
   if (p->dims == 2) {
     if ((st = build_fem(c))) return fail(st);
@@ -770,8 +789,8 @@ int mcpm_destroy(mcpm_ctx *c) {
   if (c->s) hipStreamSynchronize(c->s);
   void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
-                  c->samples, c->counters, c->err, c->res[0], c->res[1], c->opA,
-                  c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde,
+                  c->samples, c->counters, c->err, c->res, c->opA,
+                  c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
                   c->fem_buf, c->q4};
   for (void *b : bufs)
     if (b) hipFree(b);
@@ -1175,6 +1194,38 @@ int mcpm_get_collision_log(mcpm_ctx *c, int64_t cap, uint32_t *prio, uint32_t *i
   if (j) HIPCHK(hipMemcpy(j, c->log.j, n * 4, hipMemcpyDeviceToHost));
   if (accepted) HIPCHK(hipMemcpy(accepted, c->log.a, n, hipMemcpyDeviceToHost));
   return MCPM_OK;
+}
+
+int mcpm_set_tuning(mcpm_ctx *c, int32_t knob, int64_t v) {
+  if (!c) return MCPM_ERR_ARG;
+  switch (knob) {
+    case MCPM_TUNE_BANDS:
+      if (v < 1 || v > MAX_BANDS) return MCPM_ERR_ARG;
+      c->nbands = (int)v;
+      return MCPM_OK;
+    case MCPM_TUNE_ROUND_GRID:
+      if (v < 8 || v > 65536 || v % 8) return MCPM_ERR_ARG;
+      c->round_grid = (int)v;
+      return MCPM_OK;
+    case MCPM_TUNE_SORT_CHUNK: {
+      if (v != 0 && (v < 1024 || v > 32768 || v % 256)) return MCPM_ERR_ARG;
+      // the chunk table holds nchunks_cap chunks
+      if (v != 0 && (c->p.capacity + v - 1) / v > c->nchunks_cap) return MCPM_ERR_ARG;
+      c->sort_chunk = (int)v;
+      return MCPM_OK;
+    }
+    default: return MCPM_ERR_ARG;
+  }
+}
+
+int mcpm_get_tuning(mcpm_ctx *c, int32_t knob, int64_t *v) {
+  if (!c || !v) return MCPM_ERR_ARG;
+  switch (knob) {
+    case MCPM_TUNE_BANDS: *v = c->nbands; return MCPM_OK;
+    case MCPM_TUNE_ROUND_GRID: *v = c->round_grid; return MCPM_OK;
+    case MCPM_TUNE_SORT_CHUNK: *v = c->sort_chunk; return MCPM_OK;
+    default: return MCPM_ERR_ARG;
+  }
 }
 
 int mcpm_timing_enable(mcpm_ctx *c, int32_t enable) {

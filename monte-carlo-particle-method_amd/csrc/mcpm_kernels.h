@@ -16,13 +16,20 @@ enum : int {
   CNT_OPS = 0,     // valid candidate operations of the current step
   CNT_ACC = 2,     // accepted collisions (u64, 8-byte aligned)
   CNT_LOG = 4,     // entries in the debug collision log
-  CNT_LIST = 6,    // CNT_LIST + r: losers of round r - 1 carried into round r (r >= 1)
+  NREG = 8,        // operation regions: contiguous candidate ranges, one per XCD
+  CNT_LIST = 8,    // CNT_LIST + r NREG + x: losers of round r - 1 in region x (r >= 1)
   MAX_ROUNDS = 512,
   MAX_BANDS = 16,
-  CNT_BAND = CNT_LIST + MAX_ROUNDS + 2,  // operations per priority band
-  CNT_FILL = CNT_BAND + MAX_BANDS,       // partition fill pointers per band
-  CNT_TOTAL = CNT_FILL + MAX_BANDS
+  CNT_BAND = CNT_LIST + (MAX_ROUNDS + 2) * NREG,  // [b NREG + x]: operations of band b, region x
+  CNT_FILL = CNT_BAND + MAX_BANDS * NREG,         // partition fill pointers, same layout
+  CNT_TOTAL = CNT_FILL + MAX_BANDS * NREG
 };
+
+// Region of candidate k of M: the candidates are in cell order, so a region
+// is a run of neighbouring cells.
+__host__ __device__ __forceinline__ uint32_t op_region(uint32_t k, uint32_t M) {
+  return (uint32_t)(((uint64_t)k * NREG) / (M ? M : 1u));
+}
 
 // Candidate operation {i, j, priority, t} as one uint4; i = OP_NONE marks a
 // candidate without a partner.
@@ -66,15 +73,23 @@ __device__ __forceinline__ uint32_t op_band(uint32_t pr, uint32_t band_mul, int 
   uint32_t b = (uint32_t)(((uint64_t)pr * band_mul) >> 32);
   return b < (uint32_t)nbands ? b : (uint32_t)nbands - 1u;
 }
+// Candidates are generated in chunks of GEN_CHUNK consecutive candidate
+// indices (cell order).  A candidate's list (priority band, region) depends
+// only on its index, so the lists are sized before anything is drawn.
+constexpr uint32_t GEN_CHUNK = 2048;
+constexpr int NLIST = MAX_BANDS * NREG;
+__global__ void k_cand_count(uint32_t M, uint32_t prio_key, int esmc, uint32_t band_mul,
+                             int nbands, uint32_t *table);
+__global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters);
 template <typename R, int MODEL>
 __global__ void k_gen(const R *P, const uint32_t *cnt, const uint32_t *start,
-                      const uint32_t *mcand, const uint32_t *cstart, KParams kp,
-                      uint32_t step_lo, uint32_t step_hi, uint32_t prio_key, uint4 *ops,
-                      uint32_t *counters, uint32_t band_mul, int nbands);
-__global__ void k_partition(const uint4 *ops, uint32_t M, uint4 *bands, uint32_t *res0,
-                            uint32_t *counters, uint32_t band_mul, int nbands);
+                      const uint32_t *cstart, int nc, uint32_t M, KParams kp, uint32_t step_lo,
+                      uint32_t step_hi, uint32_t prio_key, uint32_t band_mul, int nbands,
+                      const uint32_t *table, uint4 *bands, uint32_t *res0,
+                      uint32_t *counters);
 template <typename R, int MODEL>
-__global__ void k_round(int r, int nbands, R *P, const uint4 *in, const uint4 *bands, uint4 *out,
+__global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
+                        const uint4 *bands, uint4 *out,
                         uint32_t *res_cur, uint32_t *res_next, uint32_t *counters,
                         const double *nden, const double *yc, const double *omax,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,

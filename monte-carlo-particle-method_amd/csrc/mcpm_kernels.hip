@@ -424,7 +424,8 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *
   mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
                              kp.key1);
   uint32_t Nc = cnt[c];
-  uint32_t i = start[c] + mcpm_uint_below(w.v[0], Nc);
+  const uint32_t li = mcpm_uint_below(w.v[0], Nc);
+  uint32_t i = start[c] + li;
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
   int32_t J;
@@ -462,99 +463,144 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return (b % 8u) * per + b / 8u;
 }
 
-// One workgroup per cell: draw the cell's m_c candidates and store each at
-// its candidate index as {i, j, priority, t} (invalid ones as OP_NONE), in
-// cell order; count the valid ones per priority band.  ESMC priorities are
-// the keyed bijection of the candidate index; DSMC keeps the reference's
-// in-cell candidate order.
+// Priority of candidate k: ESMC the keyed bijection of the index (new3's
+// random interleave of all cells' candidates), DSMC the index itself (the
+// reference's in-cell order).
+__device__ __forceinline__ uint32_t cand_priority(uint32_t k, uint32_t key, int esmc) {
+  return esmc ? mcpm_priority(k, key) : k;
+}
+
+// Size of every (band, region) list per chunk of candidates.
+__global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_key, int esmc,
+                                                    uint32_t band_mul, int nbands,
+                                                    uint32_t *__restrict__ table) {
+  __shared__ uint32_t scount[NLIST];
+  const int nl = nbands * NREG;
+  if ((int)threadIdx.x < NLIST) scount[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t lo = blockIdx.x * GEN_CHUNK;
+  for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
+    const uint32_t k = lo + b0 + threadIdx.x;
+    const bool live = k < M;
+    const uint32_t q = live ? op_band(cand_priority(k, prio_key, esmc), band_mul, nbands) * NREG +
+                                  op_region(k, M)
+                            : 0u;
+    (void)wave_rank_add(q, live, scount);
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < nl) table[(size_t)blockIdx.x * NLIST + threadIdx.x] = scount[threadIdx.x];
+}
+
+// Exclusive scan of list q's chunk counts (one workgroup per list); the
+// list's total goes to the band counters.
+__global__ __launch_bounds__(1024) void k_cand_scan(uint32_t *__restrict__ table,
+                                                    uint32_t nchunks, int nl,
+                                                    uint32_t *__restrict__ counters) {
+  __shared__ uint32_t sw[16];
+  __shared__ uint32_t carry;
+  const int q = blockIdx.x;
+  if (q >= nl) return;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t b0 = 0; b0 < nchunks; b0 += blockDim.x) {
+    const uint32_t ch = b0 + threadIdx.x;
+    const uint32_t v = ch < nchunks ? table[(size_t)ch * NLIST + q] : 0u;
+    // inclusive scan in the wavefront
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o);
+      if ((int)lane_id() >= o) x += y;
+    }
+    if (lane_id() == 63) sw[threadIdx.x >> 6] = x;
+    __syncthreads();
+    uint32_t wbase = 0;
+    for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) wbase += sw[w];
+    const uint32_t base = carry;
+    if (ch < nchunks) table[(size_t)ch * NLIST + q] = base + wbase + x - v;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) carry = base + wbase + x;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) counters[CNT_BAND + q] = carry;
+}
+
+// Candidate generation, one chunk of GEN_CHUNK candidate indices per
+// workgroup (cell order, so a chunk covers one or a few cells).  Each
+// candidate k of cell c (t = k - cstart[c]) is drawn, given its priority
+// and written to its (band, region) list as {i, j, priority, t}, or as
+// OP_NONE when it has no partner; band 0 reserves its particles for the
+// first round.  Reservations of particle i live at res[2 i] (even rounds)
+// and res[2 i + 1] (odd rounds).
 template <typename R, int MODEL>
 __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              const uint32_t *__restrict__ cnt,
                                              const uint32_t *__restrict__ start,
-                                             const uint32_t *__restrict__ mcand,
-                                             const uint32_t *__restrict__ cstart, KParams kp,
-                                             uint32_t step_lo, uint32_t step_hi,
-                                             uint32_t prio_key, uint4 *__restrict__ ops,
-                                             uint32_t *__restrict__ counters,
-                                             uint32_t band_mul, int nbands) {
-  __shared__ uint32_t sband[MAX_BANDS];
-  const uint32_t c = xcd_block(blockIdx.x, gridDim.x);
-  const uint32_t m = mcand[c];
-  if (m == 0) return;
-  if (threadIdx.x < MAX_BANDS) sband[threadIdx.x] = 0;
+                                             const uint32_t *__restrict__ cstart, int nc,
+                                             uint32_t M, KParams kp, uint32_t step_lo,
+                                             uint32_t step_hi, uint32_t prio_key,
+                                             uint32_t band_mul, int nbands,
+                                             const uint32_t *__restrict__ table,
+                                             uint4 *__restrict__ bands,
+                                             uint32_t *__restrict__ res0,
+                                             uint32_t *__restrict__ counters) {
+  __shared__ uint32_t spos[NLIST];
+  __shared__ uint32_t scell[2];
+  const int nl = nbands * NREG;
+  const uint32_t lo = blockIdx.x * GEN_CHUNK;
+  if ((int)threadIdx.x < nl) {
+    // list start (band-major lists) + this chunk's offset inside the list
+    uint32_t run = 0;
+    for (int q = 0; q < (int)threadIdx.x; ++q) run += counters[CNT_BAND + q];
+    spos[threadIdx.x] = run + table[(size_t)blockIdx.x * NLIST + threadIdx.x];
+  }
+  if (threadIdx.x == 0) {
+    // first cell of the chunk: the last c with cstart[c] <= lo
+    int a = 0, b = nc;  // cstart[a] <= lo < cstart[b] (cstart[nc] = M > lo)
+    while (b - a > 1) {
+      const int mid = (a + b) >> 1;
+      if (cstart[mid] <= lo) a = mid;
+      else b = mid;
+    }
+    scell[0] = (uint32_t)a;
+  }
   __syncthreads();
-  const uint32_t k0 = cstart[c];
+  uint32_t c = scell[0];
   uint32_t nvalid = 0;
-  for (uint32_t t = threadIdx.x; t < m; t += blockDim.x) {
+  for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
+    const uint32_t k = lo + b0 + threadIdx.x;
+    const bool live = k < M;
+    if (live)
+      while (cstart[c + 1] <= k) ++c;  // empty cells have cstart[c] == cstart[c + 1]
     uint32_t i = 0, j = 0;
     bool ok = false;
-    if (cnt[c] > 0) {
-      if (MODEL == COLL_DSMC) {
-        draw_dsmc_pair(c, t, cnt, start, kp, step_lo, step_hi, &i, &j);
-        ok = true;
-      } else {
-        ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+    uint32_t pr = 0, q = 0;
+    if (live) {
+      const uint32_t t = k - cstart[c];
+      if (cnt[c] > 0) {
+        if (MODEL == COLL_DSMC) {
+          draw_dsmc_pair(c, t, cnt, start, kp, step_lo, step_hi, &i, &j);
+          ok = true;
+        } else {
+          ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+        }
       }
+      pr = cand_priority(k, prio_key, MODEL == COLL_ESMC);
+      const uint32_t band = op_band(pr, band_mul, nbands);
+      q = band * NREG + op_region(k, M);
+      if (ok) {
+        ++nvalid;
+        if (band == 0) {
+          atomicMin(&res0[2 * (size_t)i], pr);
+          atomicMin(&res0[2 * (size_t)j], pr);
+        }
+      }
+      (void)t;
     }
-    uint4 rec;
-    if (ok) {
-      uint32_t pr = MODEL == COLL_DSMC ? k0 + t : mcpm_priority(k0 + t, prio_key);
-      rec = make_uint4(i, j, pr, t);
-      atomicAdd(&sband[op_band(pr, band_mul, nbands)], 1u);
-      ++nvalid;
-    } else {
-      rec = make_uint4(OP_NONE, 0u, 0u, 0u);
-    }
-    ops[k0 + t] = rec;
+    const uint32_t pos = wave_rank_add(q, live, spos);
+    if (live) bands[pos] = ok ? make_uint4(i, j, pr, k - cstart[c]) : make_uint4(OP_NONE, 0u, 0u, 0u);
   }
   for (int off = 32; off > 0; off >>= 1) nvalid += __shfl_down(nvalid, off);
   if (lane_id() == 0 && nvalid) atomicAdd(&counters[CNT_OPS], nvalid);
-  __syncthreads();
-  if ((int)threadIdx.x < nbands && sband[threadIdx.x])
-    atomicAdd(&counters[CNT_BAND + threadIdx.x], sband[threadIdx.x]);
-}
-
-// Scatter the valid operations into one list per priority band (band b at
-// offset sum of the earlier bands' counts; the order inside a band does not
-// matter) and reserve the particles of band 0 for the first round.
-__global__ __launch_bounds__(256) void k_partition(const uint4 *__restrict__ ops, uint32_t M,
-                                                   uint4 *__restrict__ bands,
-                                                   uint32_t *__restrict__ res0,
-                                                   uint32_t *__restrict__ counters,
-                                                   uint32_t band_mul, int nbands) {
-  __shared__ uint32_t sbase[MAX_BANDS], scount[MAX_BANDS], sstart[MAX_BANDS];
-  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
-  if (threadIdx.x == 0) {
-    uint32_t run = 0;
-    for (int b = 0; b < nbands; ++b) {
-      sstart[b] = run;
-      run += counters[CNT_BAND + b];
-    }
-  }
-  const uint32_t stride = gridDim.x * blockDim.x;
-  for (uint32_t base = lb * blockDim.x; base < M; base += stride) {
-    if (threadIdx.x < MAX_BANDS) scount[threadIdx.x] = 0;
-    __syncthreads();
-    const uint32_t idx = base + threadIdx.x;
-    uint4 rec = make_uint4(OP_NONE, 0u, 0u, 0u);
-    if (idx < M) rec = ops[idx];
-    uint32_t b = 0, rank = 0;
-    const bool valid = rec.x != OP_NONE;
-    if (valid) {
-      b = op_band(rec.z, band_mul, nbands);
-      rank = atomicAdd(&scount[b], 1u);
-      if (b == 0) {
-        atomicMin(&res0[rec.x], rec.z);
-        atomicMin(&res0[rec.y], rec.z);
-      }
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < nbands && scount[threadIdx.x])
-      sbase[threadIdx.x] = atomicAdd(&counters[CNT_FILL + threadIdx.x], scount[threadIdx.x]);
-    __syncthreads();
-    if (valid) bands[sstart[b] + sbase[b] + rank] = rec;
-    __syncthreads();
-  }
 }
 
 // The collision itself, collisions.cpp:680-716: Y at the midpoint cell,
@@ -649,14 +695,20 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, 
 // Losers reserve in the other array for the next round and are compacted
 // into the next operation list.  Each operation that finds its own priority
 // in the current array clears it, so the array is back to all-ones when the
-// round ends.
+// round ends.  The two arrays are interleaved (particle i at res[2 i] and
+// res[2 i + 1]): a loser's read and its next reservation share a cache line.
 // Operations enter by priority band: round r takes the losers of round r-1
 // plus band r, and reserves band r+1 for round r+1.  Every operation of a
 // lower priority than an admitted one is admitted too, so the smallest
 // reservation among admitted operations is the smallest among all unresolved
 // ones; late operations no longer lose round after round to early ones.
+// Region x of the operations is processed by the workgroups with
+// blockIdx % 8 == x, which the hardware places on XCD x: each XCD keeps
+// working on the same run of cells, whose particles and reservations stay in
+// its L2.  Losers of region x are kept in segment x of the output list.
 template <typename R, int MODEL>
-__global__ __launch_bounds__(256) void k_round(int r, int nbands, R *__restrict__ P,
+__global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
+                                               R *__restrict__ P,
                                                const uint4 *__restrict__ in,
                                                const uint4 *__restrict__ bands,
                                                uint4 *__restrict__ out,
@@ -669,28 +721,40 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, R *__restrict_
                                                double *__restrict__ omega_new, KParams kp,
                                                uint32_t step_lo, uint32_t step_hi,
                                                FPParams fpp, OpLog log) {
-  const uint32_t countA = (r == 0) ? 0u : counters[CNT_LIST + r];
-  uint32_t bstart = 0, countB = 0, countC = 0;
-  for (int b = 0; b < nbands && b <= r + 1; ++b) {
-    const uint32_t nb_ = counters[CNT_BAND + b];
-    if (b < r) bstart += nb_;
-    else if (b == r) countB = nb_;
-    else countC = nb_;
+  __shared__ uint32_t sh[4];
+  const uint32_t x = blockIdx.x % NREG;
+  const uint32_t lb = blockIdx.x / NREG, nbx = gridDim.x / NREG;
+  if (threadIdx.x == 0) {
+    // band r and band r+1 of region x in the band-major lists
+    uint32_t run = 0, sB = 0, nB = 0, sC = 0, nC = 0;
+    for (int q = 0; q < nbands * NREG && q <= (r + 1) * NREG + (int)x; ++q) {
+      const uint32_t n = counters[CNT_BAND + q];
+      if (q == r * NREG + (int)x) { sB = run; nB = n; }
+      if (q == (r + 1) * NREG + (int)x) { sC = run; nC = n; }
+      run += n;
+    }
+    sh[0] = sB; sh[1] = nB; sh[2] = sC; sh[3] = nC;
   }
+  __syncthreads();
+  const uint32_t countA = (r == 0) ? 0u : counters[CNT_LIST + r * NREG + x];
+  const uint32_t countB = sh[1], countC = sh[3];
   const uint32_t count = countA + countB;
-  const uint4 *inB = bands + bstart;
-  const uint32_t stride = gridDim.x * blockDim.x;
-  const uint32_t lb = xcd_block(blockIdx.x, gridDim.x);
+  const uint4 *inA = in + (size_t)x * seg;
+  const uint4 *inB = bands + sh[0];
+  const uint4 *inC = bands + sh[2];
+  uint4 *outx = out + (size_t)x * seg;
+  uint32_t *next_count = &counters[CNT_LIST + (r + 1) * NREG + x];
+  const uint32_t stride = nbx * blockDim.x;
   uint32_t acc_local = 0;
   for (uint32_t base = lb * blockDim.x; base < count; base += stride) {
     uint32_t idx = base + threadIdx.x;
     uint4 rec = make_uint4(OP_NONE, 0u, 0u, 0u);
-    if (idx < countA) rec = in[idx];
+    if (idx < countA) rec = inA[idx];
     else if (idx < count) rec = inB[idx - countA];
     bool lose = false;
     if (rec.x != OP_NONE) {
       const uint32_t i = rec.x, j = rec.y, pr = rec.z;
-      uint32_t ri = res_cur[i], rj = res_cur[j];
+      uint32_t ri = res_cur[2 * (size_t)i], rj = res_cur[2 * (size_t)j];
       if (ri == pr && rj == pr) {
         bool acc = MODEL == COLL_DSMC
                        ? commit_dsmc<R>(i, j, rec.w, P, omax, omega_new, kp, fpp, step_lo,
@@ -705,25 +769,26 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, R *__restrict_
           log.j[q] = j;
           log.a[q] = acc ? 1 : 0;
         }
-        res_cur[i] = 0xFFFFFFFFu;
-        res_cur[j] = 0xFFFFFFFFu;
+        res_cur[2 * (size_t)i] = 0xFFFFFFFFu;
+        res_cur[2 * (size_t)j] = 0xFFFFFFFFu;
       } else {
-        if (ri == pr) res_cur[i] = 0xFFFFFFFFu;
-        if (rj == pr) res_cur[j] = 0xFFFFFFFFu;
-        atomicMin(&res_next[i], pr);
-        atomicMin(&res_next[j], pr);
+        if (ri == pr) res_cur[2 * (size_t)i] = 0xFFFFFFFFu;
+        if (rj == pr) res_cur[2 * (size_t)j] = 0xFFFFFFFFu;
+        atomicMin(&res_next[2 * (size_t)i], pr);
+        atomicMin(&res_next[2 * (size_t)j], pr);
         lose = true;
       }
     }
-    uint32_t pos = wave_append(&counters[CNT_LIST + r + 1], lose);
-    if (lose) out[pos] = rec;
+    uint32_t pos = wave_append(next_count, lose);
+    if (lose) outx[pos] = rec;
   }
-  // admit band r+1: reserve its particles for the next round
-  const uint4 *inC = inB + countB;
+  // admit band r+1: reserve its particles for the next round (the lists
+  // also hold the candidates without a partner, as OP_NONE)
   for (uint32_t idx = lb * blockDim.x + threadIdx.x; idx < countC; idx += stride) {
     const uint4 rec = inC[idx];
-    atomicMin(&res_next[rec.x], rec.z);
-    atomicMin(&res_next[rec.y], rec.z);
+    if (rec.x == OP_NONE) continue;
+    atomicMin(&res_next[2 * (size_t)rec.x], rec.z);
+    atomicMin(&res_next[2 * (size_t)rec.y], rec.z);
   }
   // accepted count, one atomic per wavefront
   for (int off = 32; off > 0; off >>= 1) acc_local += __shfl_down(acc_local, off);
@@ -1020,11 +1085,11 @@ __global__ __launch_bounds__(256) void k_sample(const R *__restrict__ P,
 // ---------------------------------------------------------------------------
 #define MCPM_INST_COLL(R, M)                                                               \
   template __global__ void k_gen<R, M>(const R *, const uint32_t *, const uint32_t *,      \
-                                       const uint32_t *, const uint32_t *, KParams,        \
-                                       uint32_t, uint32_t, uint32_t, uint4 *, uint32_t *,  \
-                                       uint32_t, int);                                     \
-  template __global__ void k_round<R, M>(int, int, R *, const uint4 *, const uint4 *,      \
-                                         uint4 *,                                          \
+                                       const uint32_t *, int, uint32_t, KParams, uint32_t, \
+                                       uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
+                                       uint4 *, uint32_t *, uint32_t *);                   \
+  template __global__ void k_round<R, M>(int, int, uint32_t, R *, const uint4 *,           \
+                                         const uint4 *, uint4 *,                           \
                                          uint32_t *, uint32_t *, uint32_t *,               \
                                          const double *, const double *, const double *,   \
                                          double *, KParams, uint32_t, uint32_t, FPParams,  \

@@ -229,6 +229,8 @@ def load_library() -> ctypes.CDLL:
         "mcpm_vlasov_integral": (i32, [vp]),
         "mcpm_density_expansion": (i32, [vp]),
         "mcpm_get_kde": (i32, [vp, dpp, dpp, dpp]),
+        "mcpm_set_tuning": (i32, [vp, i32, i64]),
+        "mcpm_get_tuning": (i32, [vp, i32, ctypes.POINTER(i64)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -246,6 +248,7 @@ def contact_y(p: Params, n: float) -> float:
 
 
 TIMER_COLLIDE, TIMER_SORT, TIMER_GEN, TIMER_FIELD, TIMER_STEP = 0, 1, 2, 3, 4
+TUNE_BANDS, TUNE_ROUND_GRID, TUNE_SORT_CHUNK = 0, 1, 2
 
 
 class Simulation:
@@ -422,6 +425,14 @@ class Simulation:
                                                      _u8p(a), ctypes.byref(n)), "collision_log")
         k = n.value
         return pr[:k], i[:k], j[:k], a[:k]
+
+    def set_tuning(self, knob: int, value: int):
+        self._check(self._lib.mcpm_set_tuning(self._h, knob, int(value)), "set_tuning")
+
+    def get_tuning(self, knob: int) -> int:
+        v = ctypes.c_int64()
+        self._check(self._lib.mcpm_get_tuning(self._h, knob, ctypes.byref(v)), "get_tuning")
+        return v.value
 
     def timing_enable(self, on: bool = True):
         self._check(self._lib.mcpm_timing_enable(self._h, 1 if on else 0), "timing_enable")
