@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-particles", type=int, default=20_000_000)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one device copy of 1 GiB (a known-size kernel for "
+                         "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -52,6 +55,15 @@ def main():
 
     import mcpm
     import workloads as W
+
+    if args.calibrate_copy:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("calibration copy: 1 GiB read + 1 GiB written")
+        del src, dst
 
     N = args.particles
     p, arrs, meta = W.interface(N=N, n_cells=args.cells)
@@ -69,7 +81,7 @@ def main():
     sim.sync()
     log(f"warmup {args.warmup} steps in {time.time() - t0:.2f}s, last {st}")
 
-    sim.timing_enable(True)
+    # the measured steps, no instrumentation inside
     sim.sync()
     stats = []
     t0 = time.perf_counter()
@@ -81,12 +93,21 @@ def main():
     elapsed = t1 - t0
     value = N * args.steps / elapsed
     ms_step = 1000.0 * elapsed / args.steps
+    # as many steps again with HIP-event timers on the library's stream, for
+    # the per-kernel figures (events around every launch cost a little time)
+    sim.timing_enable(True)
+    for _ in range(args.steps):
+        sim.step(step)
+        step += 1
+    sim.sync()
+    # HIP events on the library's stream: the collision rounds per k_round
+    # launch, the other classes per call; per-step figures = avg x calls / steps
     timers = {}
     for name, k in (("collide_rounds", mcpm.TIMER_COLLIDE), ("sort", mcpm.TIMER_SORT),
                     ("candidates", mcpm.TIMER_GEN), ("field", mcpm.TIMER_FIELD),
                     ("step", mcpm.TIMER_STEP)):
         ms, calls = sim.timing(k)
-        timers[name] = {"avg_ms": ms, "calls": calls}
+        timers[name] = {"avg_ms": ms, "calls": calls, "per_step_ms": ms * calls / args.steps}
     sim.timing_enable(False)
     acc = float(np.mean([s["accepted"] for s in stats]))
     valid = float(np.mean([s["valid"] for s in stats]))
@@ -94,23 +115,30 @@ def main():
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, {value:.4g} particle-steps/s")
     log(f"phases: {json.dumps(timers)}; valid ops {valid:.0f}, accepted {acc:.0f}, rounds {rounds}")
 
-    # Dominant kernel class and its roofline.  Algorithmic bytes:
+    # Dominant kernel and its roofline.  Algorithmic bytes:
     #  sort (push pass 1 + scatter pass): read 16 B + read 16 B + write 16 B per
     #  particle (pass 1 reads the record, pass 2 reads it again and writes it);
-    #  collide rounds: per valid operation the two particle records are read
-    #  and written once (4 x 16 B) plus the 16-byte operation record.
-    sort_ms = timers["sort"]["avg_ms"]
-    coll_ms = timers["collide_rounds"]["avg_ms"]
+    #  collision rounds (k_round, one launch per round): per valid operation
+    #  the two particle records are read and written once (4 x 16 B) plus the
+    #  16-byte operation record, summed over the step's launches.
+    sort_ms = timers["sort"]["per_step_ms"]
+    coll_ms = timers["collide_rounds"]["per_step_ms"]
     sort_bytes = 48.0 * N
     coll_bytes = valid * (64.0 + 16.0)
     if coll_ms >= sort_ms:
-        dom, dom_ms, dom_bytes = "collide_rounds (k_round)", coll_ms, coll_bytes
+        launches = timers["collide_rounds"]["calls"] / args.steps
+        dom, dom_ms, dom_bytes = "k_round", coll_ms, coll_bytes
     else:
-        dom, dom_ms, dom_bytes = "sort (k_push x2)", sort_ms, sort_bytes
+        launches = 2.0
+        dom, dom_ms, dom_bytes = "k_push (both passes)", sort_ms, sort_bytes
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = measured_traffic(dom, N, args.cells)
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": None,
+                "traffic": traffic,
+                "launches_per_step": launches,
+                "avg_launch_us": round(1000.0 * dom_ms / max(launches, 1.0), 2),
+                "alg_bytes_per_launch": round(dom_bytes / max(launches, 1.0)),
                 "step_alg_bytes_per_particle": 36,
                 "step_frac": round(value * 36.0 / 1e9 / HBM_PEAK_GBS, 4)}
 
@@ -135,11 +163,26 @@ def main():
                                "n=0.55/0.02, T=0.7, ESMC + screened-Poisson, dt=0.02",
                    "particles": N, "cells": args.cells, "parallelism": "1 GPU",
                    "valid_ops_per_step": valid, "accepted_per_step": acc,
-                   "reservation_rounds": rounds, "phases_ms": {k: v["avg_ms"] for k, v in timers.items()}},
+                   "reservation_rounds": rounds,
+                   "phases_ms_per_step": {k: v["per_step_ms"] for k, v in timers.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(kernel, particles, cells):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE + WRITE_SIZE, copy-calibrated)
+    from the committed rocprofv3 summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "interface_20M_hbm_traffic.md")
+    if particles != 20_000_000 or cells != 4096 or not os.path.exists(path):
+        return None
+    key = f"traffic_bytes_per_launch[{kernel}]:"
+    for line in open(path):
+        line = line.strip().strip("`")
+        if line.startswith(key):
+            return float(line.split(":", 1)[1])
+    return None
 
 
 def cpu_baseline(args):

@@ -24,9 +24,9 @@ import mcpm  # noqa: E402
 import workloads as W  # noqa: E402
 
 SPACE = {
-    mcpm.TUNE_BANDS: [4, 6, 8, 10, 12, 16],
-    mcpm.TUNE_ROUND_GRID: [256, 512, 1024, 2048, 4096],
-    mcpm.TUNE_SORT_CHUNK: [0, 4096, 8192, 16384],
+    mcpm.TUNE_BANDS: [8, 10, 12, 16],
+    mcpm.TUNE_ROUND_GRID: [1024, 2048, 4096],
+    mcpm.TUNE_SORT_CHUNK: [0, 1024, 2048, 4096, 8192],
 }
 NAMES = {mcpm.TUNE_BANDS: "bands", mcpm.TUNE_ROUND_GRID: "round_grid",
          mcpm.TUNE_SORT_CHUNK: "sort_chunk"}

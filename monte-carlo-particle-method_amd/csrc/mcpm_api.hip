@@ -47,6 +47,9 @@ struct mcpm_ctx {
   uint4 *opBand = nullptr;               // then the priority-band lists
   uint32_t *gen_table = nullptr;         // per-chunk list sizes of the candidate draw
   int nbands = 12;
+  // round termination: round r + 1 writes the losers left after round r into
+  // pinned host memory, which the host polls without touching the stream
+  uint32_t *hleft = nullptr, *dleft = nullptr;
   int round_grid = 2048;
   int sort_chunk = 0;
   OpLog log{};                           // one allocation: debug log
@@ -504,6 +507,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
                        band_mul, nbands, c->gen_table);
     hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
                        nchunks, nbands * NREG, c->counters);
+    hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
     hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nchunks), dim3(256), 0, c->s, P, c->cnt, c->start,
                        c->cstart, nc, M, c->kp, slo, shi, pk.v[0], band_mul, nbands,
                        c->gen_table, c->opBand, c->res, c->counters);
@@ -521,23 +525,50 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     return MCPM_OK;
   };
   {
-    Timer tm(c, T_COLLIDE);
     OpLog lg = c->debug_log ? c->log : OpLog{};
-    int next_check = std::max(12, nbands + 2);
-    for (int r = 0; r < MAX_ROUNDS; ++r) {
-      hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, P,
-                         (const uint4 *)((r & 1) ? c->opB : c->opA),
-                         (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
-                         c->res + (r & 1), c->res + ((r + 1) & 1), c->counters,
-                         c->nden, c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg);
+    // hleft[q] = losers after round q - 1, written by round q; unset = ~0
+    volatile uint32_t *hl = c->hleft;
+    for (int q = 0; q <= MAX_ROUNDS; ++q) hl[q] = 0xFFFFFFFFu;
+    constexpr int LAG = 6;  // rounds the host may run ahead of the last count seen
+    int polled = 1;         // next count to examine: losers after round polled - 1
+    bool done = false;
+    for (int r = 0; r < MAX_ROUNDS && !done; ++r) {
+      {
+        Timer tm(c, T_COLLIDE);
+        hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
+                           P, (const uint4 *)((r & 1) ? c->opB : c->opA),
+                           (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
+                           c->res + (r & 1), c->res + ((r + 1) & 1), c->counters, c->nden,
+                           c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg,
+                           c->dleft);
+      }
       LAUNCHCHK();
       rounds = r + 1;
-      if (rounds == next_check) {
-        uint32_t left = 0;
-        int st2 = left_after(rounds, &left);
-        if (st2) return st2;
-        if (left == 0) break;
-        next_check += 4;
+      // examine the counts published so far (round q publishes hleft[q]);
+      // wait only when the host is LAG rounds ahead
+      int spins = 0;
+      while (polled <= r) {
+        uint32_t v = hl[polled];
+        if (v == 0xFFFFFFFFu) {
+          if (r - polled < LAG) break;
+          // spin on the host word; ask the runtime only now and then whether
+          // the stream has stopped (an error, or every launched round ran)
+          if (++spins < 4096) {
+            __builtin_ia32_pause();
+            continue;
+          }
+          spins = 0;
+          hipError_t q = hipStreamQuery(c->s);
+          if (q != hipErrorNotReady && q != hipSuccess) return MCPM_ERR_HIP;
+          if (q == hipSuccess && hl[polled] == 0xFFFFFFFFu) return MCPM_ERR_HIP;
+          continue;
+        }
+        // every band admitted and processed (band b is processed in round b)
+        if (polled >= nbands && v == 0) {
+          done = true;
+          break;
+        }
+        ++polled;
       }
     }
   }
@@ -740,9 +771,14 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   };
   if ((st = dalloc((char **)&c->P[0], c->rec * (size_t)p->capacity))) return fail(st);
   if ((st = dalloc((char **)&c->P[1], c->rec * (size_t)p->capacity))) return fail(st);
+  // chunks of at least 1024 particles; the per-chunk histogram table is kept
+  // under 512 MB (chunk_for's automatic size makes about 2048 chunks)
   c->nchunks_cap = (p->capacity + 1023) / 1024;
-  if (c->nchunks_cap > 2100) c->nchunks_cap = 2100;
-  // chunk_for keeps the number of chunks at or below about 2048
+  {
+    const int64_t tab_cap = ((int64_t)512 << 20) / (4 * (int64_t)c->nc);
+    if (c->nchunks_cap > tab_cap) c->nchunks_cap = tab_cap;
+    if (c->nchunks_cap < 2100) c->nchunks_cap = 2100;
+  }
   if ((st = dalloc(&c->table, (size_t)c->nchunks_cap * nc))) return fail(st);
   if ((st = dalloc(&c->partial, (size_t)((c->nchunks_cap + c->seg - 1) / c->seg + 1) * nc)))
     return fail(st);
@@ -761,6 +797,10 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   if ((st = dalloc(&c->clos, (size_t)nc * MCPM_NCLOS))) return fail(st);
   if ((st = dalloc(&c->drift, (size_t)nc * DRIFT_N))) return fail(st);
   if ((st = dalloc(&c->err, 1))) return fail(st);
+  if (hipHostMalloc((void **)&c->hleft, (size_t)(MAX_ROUNDS + 2) * sizeof(uint32_t),
+                    hipHostMallocMapped) != hipSuccess ||
+      hipHostGetDevicePointer((void **)&c->dleft, c->hleft, 0) != hipSuccess)
+    return fail(MCPM_ERR_HIP);
   if ((st = dalloc(&c->res, 2 * (size_t)p->capacity))) return fail(st);
   if (hipMemset(c->omax, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->omega_new, 0, nc * sizeof(double)) != hipSuccess ||
@@ -800,6 +840,7 @@ int mcpm_destroy(mcpm_ctx *c) {
       hipEventDestroy(pr.second);
     }
   for (auto e : c->ev_pool) hipEventDestroy(e);
+  if (c->hleft) hipHostFree(c->hleft);
   if (c->s) hipStreamDestroy(c->s);
   delete c;
   return MCPM_OK;

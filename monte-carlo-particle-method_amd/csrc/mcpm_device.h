@@ -99,38 +99,6 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
   return (1ull << lane_id()) - 1ull;
 }
 
-// Append to a global list with one atomic per wavefront.
-__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool pred) {
-  uint64_t m = __ballot(pred);
-  if (m == 0) return 0;
-  int leader = __ffsll((unsigned long long)m) - 1;
-  uint32_t base = 0;
-  if ((int)lane_id() == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
-  base = __shfl(base, leader);
-  return base + (uint32_t)__popcll(m & lanemask_lt());
-}
-
-// Group the active lanes of a wavefront by key: each group adds its size to
-// cnt[key] with one LDS/global atomic, and every lane gets the counter's old
-// value plus its rank among the group's lanes (lower lanes first).  The
-// loop runs once per distinct key present in the wavefront.  Every lane of
-// the wavefront must call it (inactive ones with active = false).
-__device__ __forceinline__ uint32_t wave_rank_add(uint32_t key, bool active, uint32_t *cnt) {
-  uint64_t pending = __ballot(active);
-  uint32_t res = 0;
-  while (pending) {
-    const int leader = __ffsll((unsigned long long)pending) - 1;
-    const uint32_t k = __shfl(key, leader);
-    const uint64_t m = pending & __ballot(key == k);
-    uint32_t base = 0;
-    if ((int)lane_id() == leader) base = atomicAdd(&cnt[k], (uint32_t)__popcll(m));
-    base = __shfl(base, leader);
-    if ((m >> lane_id()) & 1ull) res = base + (uint32_t)__popcll(m & lanemask_lt());
-    pending &= ~m;
-  }
-  return res;
-}
-
 __device__ __forceinline__ void atomic_max_pos_double(double *addr, double v) {
   // For non-negative doubles the IEEE bit pattern orders like the value.  The
   // plain read filters most calls: the stored maximum only grows, so a stale

@@ -21,8 +21,8 @@ enum : int {
   MAX_ROUNDS = 512,
   MAX_BANDS = 16,
   CNT_BAND = CNT_LIST + (MAX_ROUNDS + 2) * NREG,  // [b NREG + x]: operations of band b, region x
-  CNT_FILL = CNT_BAND + MAX_BANDS * NREG,         // partition fill pointers, same layout
-  CNT_TOTAL = CNT_FILL + MAX_BANDS * NREG
+  CNT_BSTART = CNT_BAND + MAX_BANDS * NREG,       // start of list [b NREG + x] in the band lists
+  CNT_TOTAL = CNT_BSTART + MAX_BANDS * NREG + 1
 };
 
 // Region of candidate k of M: the candidates are in cell order, so a region
@@ -81,6 +81,7 @@ constexpr int NLIST = MAX_BANDS * NREG;
 __global__ void k_cand_count(uint32_t M, uint32_t prio_key, int esmc, uint32_t band_mul,
                              int nbands, uint32_t *table);
 __global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters);
+__global__ void k_list_starts(uint32_t *counters, int nl);
 template <typename R, int MODEL>
 __global__ void k_gen(const R *P, const uint32_t *cnt, const uint32_t *start,
                       const uint32_t *cstart, int nc, uint32_t M, KParams kp, uint32_t step_lo,
@@ -93,7 +94,7 @@ __global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
                         uint32_t *res_cur, uint32_t *res_next, uint32_t *counters,
                         const double *nden, const double *yc, const double *omax,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,
-                        FPParams fpp, OpLog log);
+                        FPParams fpp, OpLog log, uint32_t *host_left);
 __global__ void k_omega_finish(double *omax, double *omega_new, const uint32_t *cnt, int nc,
                                int model);
 __global__ void k_zero_u32(uint32_t *p, int n);

@@ -416,6 +416,7 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__
 // The candidate draw: i in cell c, isotropic direction, the partner cell J
 // at y_i + sigma dir_y (collisions.cpp:648-671; 1D rule of new4,
 // collisions.cpp:784-792), j in J, j != i (collisions.cpp:675-679).
+
 template <typename R>
 __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *P,
                                                const uint32_t *cnt, const uint32_t *start,
@@ -438,6 +439,7 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *
   *oj = j;
   return true;
 }
+
 
 // DSMC pair draw (collisions.cpp:2093-2098): two distinct particles of cell c.
 __device__ __forceinline__ void draw_dsmc_pair(uint32_t c, uint32_t t, const uint32_t *cnt,
@@ -463,6 +465,19 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
   return (b % 8u) * per + b / 8u;
 }
 
+// First candidate of region x: the smallest k with op_region(k, M) >= x.
+__device__ __forceinline__ uint32_t region_start(uint32_t x, uint32_t M) {
+  if (x >= NREG) return 0xFFFFFFFFu;
+  return (uint32_t)(((uint64_t)x * M + NREG - 1) / NREG);
+}
+// op_region(k, M) for k >= region_start(x0, M), walking up from region x0
+// (no division by M; a chunk usually stays in x0 or x0 + 1).
+__device__ __forceinline__ uint32_t region_from(uint32_t k, uint32_t x0, uint32_t M) {
+  uint32_t x = x0;
+  while (k >= region_start(x + 1u, M)) ++x;
+  return x;
+}
+
 // Priority of candidate k: ESMC the keyed bijection of the index (new3's
 // random interleave of all cells' candidates), DSMC the index itself (the
 // reference's in-cell order).
@@ -479,13 +494,13 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
   if ((int)threadIdx.x < NLIST) scount[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t lo = blockIdx.x * GEN_CHUNK;
+  const uint32_t x0 = op_region(lo, M);
   for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
     const uint32_t k = lo + b0 + threadIdx.x;
-    const bool live = k < M;
-    const uint32_t q = live ? op_band(cand_priority(k, prio_key, esmc), band_mul, nbands) * NREG +
-                                  op_region(k, M)
-                            : 0u;
-    (void)wave_rank_add(q, live, scount);
+    if (k >= M) break;
+    const uint32_t q = op_band(cand_priority(k, prio_key, esmc), band_mul, nbands) * NREG +
+                       region_from(k, x0, M);
+    atomicAdd(&scount[q], 1u);
   }
   __syncthreads();
   if ((int)threadIdx.x < nl) table[(size_t)blockIdx.x * NLIST + threadIdx.x] = scount[threadIdx.x];
@@ -524,6 +539,17 @@ __global__ __launch_bounds__(1024) void k_cand_scan(uint32_t *__restrict__ table
   if (threadIdx.x == 0) counters[CNT_BAND + q] = carry;
 }
 
+// Exclusive prefix of the list totals: where each (band, region) list starts.
+__global__ void k_list_starts(uint32_t *__restrict__ counters, int nl) {
+  if (threadIdx.x != 0) return;
+  uint32_t run = 0;
+  for (int q = 0; q < nl; ++q) {
+    counters[CNT_BSTART + q] = run;
+    run += counters[CNT_BAND + q];
+  }
+  counters[CNT_BSTART + nl] = run;
+}
+
 // Candidate generation, one chunk of GEN_CHUNK candidate indices per
 // workgroup (cell order, so a chunk covers one or a few cells).  Each
 // candidate k of cell c (t = k - cstart[c]) is drawn, given its priority
@@ -544,15 +570,12 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              uint32_t *__restrict__ res0,
                                              uint32_t *__restrict__ counters) {
   __shared__ uint32_t spos[NLIST];
-  __shared__ uint32_t scell[2];
+  __shared__ uint32_t scell[1];
   const int nl = nbands * NREG;
   const uint32_t lo = blockIdx.x * GEN_CHUNK;
-  if ((int)threadIdx.x < nl) {
-    // list start (band-major lists) + this chunk's offset inside the list
-    uint32_t run = 0;
-    for (int q = 0; q < (int)threadIdx.x; ++q) run += counters[CNT_BAND + q];
-    spos[threadIdx.x] = run + table[(size_t)blockIdx.x * NLIST + threadIdx.x];
-  }
+  if ((int)threadIdx.x < nl)  // list start + this chunk's offset inside the list
+    spos[threadIdx.x] = counters[CNT_BSTART + threadIdx.x] +
+                        table[(size_t)blockIdx.x * NLIST + threadIdx.x];
   if (threadIdx.x == 0) {
     // first cell of the chunk: the last c with cstart[c] <= lo
     int a = 0, b = nc;  // cstart[a] <= lo < cstart[b] (cstart[nc] = M > lo)
@@ -565,6 +588,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
   }
   __syncthreads();
   uint32_t c = scell[0];
+  const uint32_t x0 = op_region(lo, M);
   uint32_t nvalid = 0;
   for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
     const uint32_t k = lo + b0 + threadIdx.x;
@@ -586,7 +610,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
       }
       pr = cand_priority(k, prio_key, MODEL == COLL_ESMC);
       const uint32_t band = op_band(pr, band_mul, nbands);
-      q = band * NREG + op_region(k, M);
+      q = band * NREG + region_from(k, x0, M);
       if (ok) {
         ++nvalid;
         if (band == 0) {
@@ -596,8 +620,10 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
       }
       (void)t;
     }
-    const uint32_t pos = wave_rank_add(q, live, spos);
-    if (live) bands[pos] = ok ? make_uint4(i, j, pr, k - cstart[c]) : make_uint4(OP_NONE, 0u, 0u, 0u);
+    if (live) {
+      const uint32_t pos = atomicAdd(&spos[q], 1u);
+      bands[pos] = ok ? make_uint4(i, j, pr, k - cstart[c]) : make_uint4(OP_NONE, 0u, 0u, 0u);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) nvalid += __shfl_down(nvalid, off);
   if (lane_id() == 0 && nvalid) atomicAdd(&counters[CNT_OPS], nvalid);
@@ -608,14 +634,12 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
 // accept iff p / omega_max_I > U, hard-sphere exchange along dir.
 // The cell I of the candidate is the cell of particle i (i was drawn in it).
 template <typename R>
-__device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R *P,
+__device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R a, R bb, R *P,
                                           const double *nden, const double *yc,
                                           const double *omax, double *omega_new,
                                           const KParams &kp, uint32_t step_lo,
                                           uint32_t step_hi) {
   using T = typename R::real_type;
-  R a = P[i];
-  R bb = P[j];
   const uint32_t c = (uint32_t)cell_key(a, kp);
   mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
                              kp.key1);
@@ -650,13 +674,11 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R 
 // sigma_T ~ Vr^(2-2 visp) crref^(2 visp - 1), accept iff sigma_T / crm > U,
 // isotropic post-collision relative velocity about the centre of mass.
 template <typename R>
-__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, R *P,
+__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, R a, R bb, R *P,
                                             const double *omax, double *omega_new,
                                             const KParams &kp, const FPParams &fpp,
                                             uint32_t step_lo, uint32_t step_hi) {
   using real = typename R::real_type;
-  R a = P[i];
-  R bb = P[j];
   const uint32_t c = (uint32_t)cell_key(a, kp);
   mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
                              kp.key1);
@@ -720,47 +742,83 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
                                                const double *__restrict__ omax,
                                                double *__restrict__ omega_new, KParams kp,
                                                uint32_t step_lo, uint32_t step_hi,
-                                               FPParams fpp, OpLog log) {
-  __shared__ uint32_t sh[4];
+                                               FPParams fpp, OpLog log,
+                                               uint32_t *__restrict__ host_left) {
+  __shared__ uint32_t s_lose, s_base, s_acc;
+  // publish the losers left after round r - 1 (final now) to the host
+  if (blockIdx.x == 0 && threadIdx.x == 0 && r > 0) {
+    uint32_t left = 0;
+    for (int q = 0; q < NREG; ++q) left += counters[CNT_LIST + r * NREG + q];
+    __hip_atomic_store(&host_left[r], left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
   const uint32_t x = blockIdx.x % NREG;
   const uint32_t lb = blockIdx.x / NREG, nbx = gridDim.x / NREG;
-  if (threadIdx.x == 0) {
-    // band r and band r+1 of region x in the band-major lists
-    uint32_t run = 0, sB = 0, nB = 0, sC = 0, nC = 0;
-    for (int q = 0; q < nbands * NREG && q <= (r + 1) * NREG + (int)x; ++q) {
-      const uint32_t n = counters[CNT_BAND + q];
-      if (q == r * NREG + (int)x) { sB = run; nB = n; }
-      if (q == (r + 1) * NREG + (int)x) { sC = run; nC = n; }
-      run += n;
-    }
-    sh[0] = sB; sh[1] = nB; sh[2] = sC; sh[3] = nC;
-  }
-  __syncthreads();
+  // band r and band r+1 of region x in the band-major lists
+  const int qB = r * NREG + (int)x, qC = (r + 1) * NREG + (int)x;
+  const bool hasB = r < nbands, hasC = r + 1 < nbands;
+  const uint32_t countB = hasB ? counters[CNT_BAND + qB] : 0u;
+  const uint32_t countC = hasC ? counters[CNT_BAND + qC] : 0u;
+  const uint32_t sB = hasB ? counters[CNT_BSTART + qB] : 0u;
+  const uint32_t sC = hasC ? counters[CNT_BSTART + qC] : 0u;
   const uint32_t countA = (r == 0) ? 0u : counters[CNT_LIST + r * NREG + x];
-  const uint32_t countB = sh[1], countC = sh[3];
   const uint32_t count = countA + countB;
   const uint4 *inA = in + (size_t)x * seg;
-  const uint4 *inB = bands + sh[0];
-  const uint4 *inC = bands + sh[2];
+  const uint4 *inB = bands + sB;
+  const uint4 *inC = bands + sC;
   uint4 *outx = out + (size_t)x * seg;
   uint32_t *next_count = &counters[CNT_LIST + (r + 1) * NREG + x];
   const uint32_t stride = nbx * blockDim.x;
+  // admit band r+1 first: its reservations (no return value) overlap with
+  // the work below; the lists also hold partnerless candidates as OP_NONE
+  for (uint32_t idx = lb * blockDim.x + threadIdx.x; idx < countC; idx += stride) {
+    const uint4 rec = inC[idx];
+    if (rec.x == OP_NONE) continue;
+    atomicMin(&res_next[2 * (size_t)rec.x], rec.z);
+    atomicMin(&res_next[2 * (size_t)rec.y], rec.z);
+  }
+  // U operations per thread per tile: their loads are issued together
+  constexpr int U = 4;
+  const uint32_t tile = blockDim.x * U;
   uint32_t acc_local = 0;
-  for (uint32_t base = lb * blockDim.x; base < count; base += stride) {
-    uint32_t idx = base + threadIdx.x;
-    uint4 rec = make_uint4(OP_NONE, 0u, 0u, 0u);
-    if (idx < countA) rec = inA[idx];
-    else if (idx < count) rec = inB[idx - countA];
-    bool lose = false;
-    if (rec.x != OP_NONE) {
-      const uint32_t i = rec.x, j = rec.y, pr = rec.z;
-      uint32_t ri = res_cur[2 * (size_t)i], rj = res_cur[2 * (size_t)j];
-      if (ri == pr && rj == pr) {
+  for (uint32_t base = lb * tile; base < count; base += nbx * tile) {
+    uint4 rec[U];
+    uint32_t ri[U], rj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t idx = base + u * blockDim.x + threadIdx.x;
+      rec[u] = make_uint4(OP_NONE, 0u, 0u, 0u);
+      if (idx < countA) rec[u] = inA[idx];
+      else if (idx < count) rec[u] = inB[idx - countA];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ri[u] = rj[u] = 0u;
+      if (rec[u].x != OP_NONE) {
+        ri[u] = res_cur[2 * (size_t)rec[u].x];
+        rj[u] = res_cur[2 * (size_t)rec[u].y];
+      }
+    }
+    R pa[U], pb[U];
+    bool win[U], lose[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const bool valid = rec[u].x != OP_NONE;
+      win[u] = valid && ri[u] == rec[u].z && rj[u] == rec[u].z;
+      lose[u] = valid && !win[u];
+      if (win[u]) {
+        pa[u] = P[rec[u].x];
+        pb[u] = P[rec[u].y];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t i = rec[u].x, j = rec[u].y, pr = rec[u].z;
+      if (win[u]) {
         bool acc = MODEL == COLL_DSMC
-                       ? commit_dsmc<R>(i, j, rec.w, P, omax, omega_new, kp, fpp, step_lo,
-                                        step_hi)
-                       : commit_op<R>(i, j, rec.w, P, nden, yc, omax, omega_new, kp, step_lo,
-                                      step_hi);
+                       ? commit_dsmc<R>(i, j, rec[u].w, pa[u], pb[u], P, omax, omega_new, kp,
+                                        fpp, step_lo, step_hi)
+                       : commit_op<R>(i, j, rec[u].w, pa[u], pb[u], P, nden, yc, omax,
+                                      omega_new, kp, step_lo, step_hi);
         acc_local += acc ? 1u : 0u;
         if (log.p) {
           uint32_t q = atomicAdd(&counters[CNT_LOG], 1u);
@@ -771,28 +829,45 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
         }
         res_cur[2 * (size_t)i] = 0xFFFFFFFFu;
         res_cur[2 * (size_t)j] = 0xFFFFFFFFu;
-      } else {
-        if (ri == pr) res_cur[2 * (size_t)i] = 0xFFFFFFFFu;
-        if (rj == pr) res_cur[2 * (size_t)j] = 0xFFFFFFFFu;
+      } else if (lose[u]) {
+        if (ri[u] == pr) res_cur[2 * (size_t)i] = 0xFFFFFFFFu;
+        if (rj[u] == pr) res_cur[2 * (size_t)j] = 0xFFFFFFFFu;
         atomicMin(&res_next[2 * (size_t)i], pr);
         atomicMin(&res_next[2 * (size_t)j], pr);
-        lose = true;
       }
     }
-    uint32_t pos = wave_append(next_count, lose);
-    if (lose) outx[pos] = rec;
+    // compact the tile's losers: ranks in the wavefront, one LDS atomic per
+    // wavefront, one global atomic per workgroup
+    uint64_t lm[U];
+    uint32_t wtot = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      lm[u] = __ballot(lose[u]);
+      wtot += (uint32_t)__popcll(lm[u]);
+    }
+    if (threadIdx.x == 0) s_lose = 0;
+    __syncthreads();
+    uint32_t wbase = 0;
+    if (lane_id() == 0 && wtot) wbase = atomicAdd(&s_lose, wtot);
+    wbase = __shfl(wbase, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) s_base = s_lose ? atomicAdd(next_count, s_lose) : 0u;
+    __syncthreads();
+    uint32_t off = s_base + wbase;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (lose[u]) outx[off + (uint32_t)__popcll(lm[u] & lanemask_lt())] = rec[u];
+      off += (uint32_t)__popcll(lm[u]);
+    }
   }
-  // admit band r+1: reserve its particles for the next round (the lists
-  // also hold the candidates without a partner, as OP_NONE)
-  for (uint32_t idx = lb * blockDim.x + threadIdx.x; idx < countC; idx += stride) {
-    const uint4 rec = inC[idx];
-    if (rec.x == OP_NONE) continue;
-    atomicMin(&res_next[2 * (size_t)rec.x], rec.z);
-    atomicMin(&res_next[2 * (size_t)rec.y], rec.z);
-  }
-  // accepted count, one atomic per wavefront
+  // accepted count, one atomic per workgroup
   for (int off = 32; off > 0; off >>= 1) acc_local += __shfl_down(acc_local, off);
-  if (lane_id() == 0 && acc_local) atomicAdd((unsigned long long *)&counters[CNT_ACC], (unsigned long long)acc_local);
+  if (threadIdx.x == 0) s_acc = 0;
+  __syncthreads();
+  if (lane_id() == 0 && acc_local) atomicAdd(&s_acc, acc_local);
+  __syncthreads();
+  if (threadIdx.x == 0 && s_acc)
+    atomicAdd((unsigned long long *)&counters[CNT_ACC], (unsigned long long)s_acc);
 }
 
 __global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ omega_new,
@@ -1093,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_sample(const R *__restrict__ P,
                                          uint32_t *, uint32_t *, uint32_t *,               \
                                          const double *, const double *, const double *,   \
                                          double *, KParams, uint32_t, uint32_t, FPParams,  \
-                                         OpLog);
+                                         OpLog, uint32_t *);
 
 #define MCPM_INST(R)                                                                       \
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
