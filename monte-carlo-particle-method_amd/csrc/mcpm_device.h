@@ -100,10 +100,12 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double *addr, double v) {
-  // For non-negative doubles the IEEE bit pattern orders like the value.  The
-  // plain read filters most calls: the stored maximum only grows, so a stale
-  // read is never above the true value and cannot skip a needed update.
-  if (v > 0.0 && v > *(volatile double *)addr)
+  // For non-negative doubles the IEEE bit pattern orders like the value.  A
+  // relaxed device-scope read (served by L2) filters most calls: the stored
+  // maximum only grows, so a stale read is never above the true value and
+  // cannot skip a needed update.
+  if (v > 0.0 &&
+      v > __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
     atomicMax((unsigned long long *)addr, (unsigned long long)mcpm_dbits(v));
 }
 
