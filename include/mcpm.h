@@ -61,13 +61,19 @@ enum mcpm_long_range {
 /* Velocity model of velocity_update (gas.model, Main_Functions.cpp:1545-1600):
    ESMC -> Collision_ESMC_new3, FP_DENSE / FP_IDEAL -> Velocity_FP_parallel
    with or without the dense-gas Y(n) and closures, DSMC_VHS ->
-   Collision_DSMC_VHS, NONE -> free flight. */
+   Collision_DSMC_VHS, NONE -> free flight, DSMC_HS -> Collision_DSMC_HS,
+   CBA_VHS -> Collision_CBA_VHS, CBA_HS -> Collision_CBA_HS, FP_LINEAR ->
+   Velocity_FP_linear. */
 enum mcpm_model {
   MCPM_MODEL_ESMC = 0,
   MCPM_MODEL_FP_DENSE = 1,
   MCPM_MODEL_FP_IDEAL = 2,
   MCPM_MODEL_DSMC_VHS = 3,
-  MCPM_MODEL_NONE = 4
+  MCPM_MODEL_NONE = 4,
+  MCPM_MODEL_DSMC_HS = 5,
+  MCPM_MODEL_CBA_VHS = 6,
+  MCPM_MODEL_CBA_HS = 7,
+  MCPM_MODEL_FP_LINEAR = 8
 };
 
 /* ---- parameters -------------------------------------------------------- */
@@ -260,7 +266,11 @@ int mcpm_fp_closures(mcpm_ctx *ctx, int64_t step);
 /* random_generator + Velocity_FP_parallel (cpp_headers.h:362,632;
    Fokker_Planck.cpp:20-145, 1065-1182): per-cell zero-mean unit-variance
    Gaussian noise, exact Ornstein-Uhlenbeck step with the closures' drift,
-   per-cell kinetic-energy rescale.  Uses the last mcpm_fp_closures. */
+   per-cell kinetic-energy rescale.  Uses the last mcpm_fp_closures.  With
+   model FP_LINEAR this is Velocity_FP_linear (cpp_headers.h:517;
+   Fokker_Planck.cpp:1186-1287): the same noise and rescale, relaxation
+   time tau = 2 mu(T) / (n kB T) and no drift (of the closures only tau is
+   read). */
 int mcpm_velocity_fp(mcpm_ctx *ctx, int64_t step);
 /* Closures in cell order: out[c*MCPM_NCLOS + k], k = 0 Lambda, 1..6 c_ij,
    7..9 gamma_i, 10..18 alpha, 19..21 beta, 22 gamma_st, 23..25 phi,
@@ -273,6 +283,24 @@ int mcpm_get_closures(mcpm_ctx *ctx, double *out);
    pairs inside each cell, VHS cross-section, isotropic scattering, per-cell
    crm (kept in the omega_max array) <- max(1.1 crm_seen, 0.9 crm). */
 int mcpm_collide_dsmc_vhs(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
+/* Collision_DSMC_HS (cpp_headers.h:507; collisions.cpp:1989-2061): as
+   DSMC_VHS with the hard-sphere weight sigma_T = Vr, and crm updated in every
+   cell.  Cells with fewer than two particles draw no candidates. */
+int mcpm_collide_dsmc_hs(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
+/* Collision_CBA_VHS (cpp_headers.h:611; collisions.cpp:2144-2248): DSMC_VHS
+   with the candidate count scaled by the contact value Y(n) of the cell, and
+   each accepted pair displaced by +-sigma along the change of its relative
+   velocity (x1 and x2 of the reference; the slab coordinate is x2). */
+int mcpm_collide_cba_vhs(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
+/* Collision_CBA_HS (cpp_headers.h:610; collisions.cpp:2251-2346): per cell,
+   Vr_max = sqrt(pi kB T / m), M = Fn N^2 pi sigma^2 Vr_max dt / (2 V_c) times
+   (1 - b rho / 8) / (1 - b rho / 4)^3 (rho = n m) candidates, drawn with
+   replacement and run in order; an accepted pair faster than Vr_max raises
+   Vr_max and with it the number of candidates still to run.  Displacement
+   as CBA_VHS.  Cells run in parallel, the candidates of a cell one after
+   the other as in the reference.  MCPM_ERR_CAPACITY when a cell's bound
+   passes 2^26 candidates (non-finite or absurd velocities). */
+int mcpm_collide_cba_hs(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
 
 /* cell_poly_velocity (cpp_headers.h:470; Main_Functions.cpp:1918-1968):
    accumulate the per-cell time-average sums.  mcpm_get_samples returns

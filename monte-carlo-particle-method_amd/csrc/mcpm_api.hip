@@ -472,13 +472,16 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   return MCPM_OK;
 }
 
+// variant: for COLL_DSMC the mcpm_model run (DSMC_VHS, DSMC_HS, CBA_VHS).
 template <typename R, int MODEL>
-int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int variant) {
   const int nc = c->nc;
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
   R *P = (R *)c->P[c->cur];
+  FPParams fpp = c->fpp;
+  fpp.model = variant;
   hipLaunchKernelGGL(k_mcand, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->cnt, c->omax,
-                     c->nden, c->mcand, nc, c->kp, (int)MODEL, slo, shi);
+                     c->nden, c->yc, c->mcand, nc, c->kp, (int)MODEL, variant, slo, shi);
   hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->mcand, c->cstart, nc);
   hipLaunchKernelGGL(k_zero_u32, dim3((CNT_TOTAL + 255) / 256), dim3(256), 0, c->s,
                      c->counters, (int)CNT_TOTAL);
@@ -539,7 +542,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
                            P, (const uint4 *)((r & 1) ? c->opB : c->opA),
                            (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
                            c->res + (r & 1), c->res + ((r + 1) & 1), c->counters, c->nden,
-                           c->yc, c->omax, c->omega_new, c->kp, slo, shi, c->fpp, lg,
+                           c->yc, c->omax, c->omega_new, c->kp, slo, shi, fpp, lg,
                            c->dleft);
       }
       LAUNCHCHK();
@@ -573,7 +576,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     }
   }
   hipLaunchKernelGGL(k_omega_finish, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->omax,
-                     c->omega_new, c->cnt, nc, (int)MODEL);
+                     c->omega_new, c->cnt, nc, (int)MODEL, variant);
   LAUNCHCHK();
   uint32_t hc[CNT_LIST + 2];
   HIPCHK(hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->s));
@@ -597,11 +600,44 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
 
 template <typename R>
 int collide_esmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
-  return collide_t<R, COLL_ESMC>(c, step, stats);
+  return collide_t<R, COLL_ESMC>(c, step, stats, MCPM_MODEL_ESMC);
 }
 template <typename R>
-int collide_dsmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
-  return collide_t<R, COLL_DSMC>(c, step, stats);
+int collide_dsmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int variant) {
+  return collide_t<R, COLL_DSMC>(c, step, stats, variant);
+}
+
+template <typename R>
+int launch_moments(mcpm_ctx *c);
+
+// Collision_CBA_HS: moments (T), then one thread per cell.
+template <typename R>
+int collide_cba_hs_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  int st = launch_moments<R>(c);
+  if (st) return st;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->s, c->counters, (int)CNT_LIST);
+  Timer tm(c, T_COLLIDE);
+  hipLaunchKernelGGL((k_cba_hs<R>), dim3((c->nc + 63) / 64), dim3(64), 0, c->s,
+                     (R *)c->P[c->cur], c->cnt, c->start, c->moments, c->nden, c->kp, slo, shi,
+                     c->counters, c->err);
+  LAUNCHCHK();
+  if ((st = check_err(c))) return st;
+  uint32_t hc[CNT_LIST];
+  HIPCHK(hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  c->last_ops = 0;
+  c->last_log = 0;
+  if (stats) {
+    stats->candidates = hc[CNT_OPS];
+    stats->valid = hc[CNT_OPS];
+    uint64_t acc;
+    memcpy(&acc, &hc[CNT_ACC], sizeof acc);
+    stats->accepted = (int64_t)acc;
+    stats->rounds = 0;
+    stats->reserved0 = 0;
+  }
+  return MCPM_OK;
 }
 
 template <typename R>
@@ -705,7 +741,9 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
     if (p->boundary == MCPM_BC_DIFFUSE || p->boundary_x == MCPM_BC_DIFFUSE) return MCPM_ERR_ARG;
     if (p->boundary < 0 || p->boundary > 2 || p->boundary_x < 0 || p->boundary_x > 2)
       return MCPM_ERR_ARG;
-    if (p->model == MCPM_MODEL_FP_DENSE || p->model == MCPM_MODEL_FP_IDEAL) return MCPM_ERR_ARG;
+    if (p->model == MCPM_MODEL_FP_DENSE || p->model == MCPM_MODEL_FP_IDEAL ||
+        p->model == MCPM_MODEL_FP_LINEAR)
+      return MCPM_ERR_ARG;
     if (p->long_range == MCPM_LR_DENSITY_EXPANSION) return MCPM_ERR_ARG;
     if (p->long_range != MCPM_LR_NONE && (nx < 2 || p->n_cells < 2)) return MCPM_ERR_ARG;
     if (p->long_range == MCPM_LR_SCREENED_POISSON && p->boundary != p->boundary_x)
@@ -718,7 +756,7 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   if (p->long_range == MCPM_LR_SCREENED_POISSON && !(p->bst > 0.0)) return MCPM_ERR_ARG;
   if (p->long_range < MCPM_LR_NONE || p->long_range > MCPM_LR_SCREENED_POISSON)
     return MCPM_ERR_ARG;
-  if (p->model < MCPM_MODEL_ESMC || p->model > MCPM_MODEL_NONE) return MCPM_ERR_ARG;
+  if (p->model < MCPM_MODEL_ESMC || p->model > MCPM_MODEL_FP_LINEAR) return MCPM_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCPM_ERR_HIP;
   mcpm_ctx *c = new mcpm_ctx();
@@ -962,7 +1000,37 @@ int mcpm_collide_dsmc_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats
     if (stats) memset(stats, 0, sizeof *stats);
     return MCPM_OK;
   }
-  return MCPM_DISPATCH(c, collide_dsmc_t, c, step, stats);
+  return MCPM_DISPATCH(c, collide_dsmc_t, c, step, stats, (int)MCPM_MODEL_DSMC_VHS);
+}
+
+int mcpm_collide_dsmc_hs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->n == 0) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return MCPM_OK;
+  }
+  return MCPM_DISPATCH(c, collide_dsmc_t, c, step, stats, (int)MCPM_MODEL_DSMC_HS);
+}
+
+int mcpm_collide_cba_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->n == 0) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return MCPM_OK;
+  }
+  return MCPM_DISPATCH(c, collide_dsmc_t, c, step, stats, (int)MCPM_MODEL_CBA_VHS);
+}
+
+int mcpm_collide_cba_hs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->n == 0) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return MCPM_OK;
+  }
+  return MCPM_DISPATCH(c, collide_cba_hs_t, c, step, stats);
 }
 
 int mcpm_fp_closures(mcpm_ctx *c, int64_t step) {
@@ -1113,10 +1181,14 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     case MCPM_MODEL_ESMC: st = mcpm_collide_esmc(c, step, stats); break;
     case MCPM_MODEL_FP_DENSE:
     case MCPM_MODEL_FP_IDEAL:
+    case MCPM_MODEL_FP_LINEAR:
       st = mcpm_fp_closures(c, step);
       if (!st) st = mcpm_velocity_fp(c, step);
       break;
     case MCPM_MODEL_DSMC_VHS: st = mcpm_collide_dsmc_vhs(c, step, stats); break;
+    case MCPM_MODEL_DSMC_HS: st = mcpm_collide_dsmc_hs(c, step, stats); break;
+    case MCPM_MODEL_CBA_VHS: st = mcpm_collide_cba_vhs(c, step, stats); break;
+    case MCPM_MODEL_CBA_HS: st = mcpm_collide_cba_hs(c, step, stats); break;
     default: break;
   }
   if (st) return st;

@@ -18,6 +18,7 @@ namespace mcpm {
 template <typename real>
 struct alignas(4 * sizeof(real)) Rec {
   using real_type = real;
+  static constexpr int dims = 1;
   real y, u1, u2, u3;
 };
 
@@ -26,6 +27,7 @@ struct alignas(4 * sizeof(real)) Rec {
 template <typename real>
 struct alignas(8 * sizeof(real)) Rec2 {
   using real_type = real;
+  static constexpr int dims = 2;
   real x, y, u1, u2, u3, p0, p1, p2;
 };
 

@@ -299,9 +299,19 @@ __global__ __launch_bounds__(256) void k_fp_velocity(Rec<real> *__restrict__ P,
   const double *cc = cl + 1, *g = cl + 7;
   const double Q0 = m[9] + m[12] + m[14], Q1 = m[10] + m[15] + m[17], Q2 = m[11] + m[16] + m[18];
   const double e0 = 3.0 * kp.kb * T / kp.mass;
+  // Velocity_FP_linear (Fokker_Planck.cpp:1240-1264): the same step without drift
+  const bool linear = fpp.model == MCPM_MODEL_FP_LINEAR;
   auto update = [&](const Rec<real> &r, double &n0, double &n1, double &n2, uint32_t k) {
     double a = (double)r.u1 - U0, b = (double)r.u2 - U1, d = (double)r.u3 - U2;
     double e = a * a + b * b + d * d;
+    if (linear) {
+      double z[3];
+      fp_noise(s + k, kp, slo, shi, z);
+      n0 = A1 * a + A3 * ((z[0] - mean0) / rms0);
+      n1 = A1 * b + A3 * ((z[1] - mean1) / rms1);
+      n2 = A1 * d + A3 * ((z[2] - mean2) / rms2);
+      return e;
+    }
     double d0 = cc[0] * a + cc[1] * b + cc[2] * d;
     d0 += g[0] * (e - e0);
     d0 += L * (a * e - Q0);

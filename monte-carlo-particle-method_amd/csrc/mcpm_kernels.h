@@ -63,9 +63,10 @@ __global__ void k_scan_a(const uint32_t *table, uint32_t *partial, int nchunks, 
 __global__ void k_scan_b(uint32_t *partial, int nseg, int nc, uint32_t *cnt, uint32_t *start,
                          double *nden, double *yc, KParams kp);
 __global__ void k_scan_c(uint32_t *table, const uint32_t *partial, int nchunks, int nc, int seg);
+// variant: the mcpm_model of a COLL_DSMC collision (DSMC_VHS, DSMC_HS, CBA_VHS).
 __global__ void k_mcand(const uint32_t *cnt, const double *omax, const double *nden,
-                        uint32_t *mcand, int nc, KParams kp, int model, uint32_t step_lo,
-                        uint32_t step_hi);
+                        const double *yc, uint32_t *mcand, int nc, KParams kp, int model,
+                        int variant, uint32_t step_lo, uint32_t step_hi);
 __global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc);
 // Priority band of an operation: the top bits of the priority scaled to
 // nbands (band_mul = nbands for priorities spread over 32 bits, 0 for one band).
@@ -96,7 +97,13 @@ __global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,
                         FPParams fpp, OpLog log, uint32_t *host_left);
 __global__ void k_omega_finish(double *omax, double *omega_new, const uint32_t *cnt, int nc,
-                               int model);
+                               int model, int variant);
+// Most candidates one cell may run in one Collision_CBA_HS call.
+constexpr uint32_t CBA_MAX_CAND = 1u << 26;
+template <typename R>
+__global__ void k_cba_hs(R *P, const uint32_t *cnt, const uint32_t *start, const double *mom,
+                         const double *nden, KParams kp, uint32_t step_lo, uint32_t step_hi,
+                         uint32_t *counters, uint32_t *err);
 __global__ void k_zero_u32(uint32_t *p, int n);
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v);
 __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams kp, double ast,

@@ -359,10 +359,12 @@ __global__ void k_scan_c(uint32_t *__restrict__ table, const uint32_t *__restric
 // Candidate count per cell, collisions.cpp:617-629:
 //   m_c = floor(0.5 * omega_max_c * N_c + U)
 // DSMC (collisions.cpp:2087-2090): M_c = pi sigma^2 N_c n_c crm dt / 2 for
-// cells with two or more particles, crm kept in the omax array.
+// cells with two or more particles, crm kept in the omax array; CBA_VHS
+// multiplies by the contact value Y(n_c) (collisions.cpp:2171-2172).
 __global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restrict__ omax,
-                        const double *__restrict__ nden, uint32_t *__restrict__ mcand, int nc,
-                        KParams kp, int model, uint32_t step_lo, uint32_t step_hi) {
+                        const double *__restrict__ nden, const double *__restrict__ yc,
+                        uint32_t *__restrict__ mcand, int nc, KParams kp, int model, int variant,
+                        uint32_t step_lo, uint32_t step_hi) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   mcpm_u32x4 w = mcpm_philox((uint32_t)c, 0u, step_lo, mcpm_ctr3(MCPM_TAG_MCAND, step_hi, 0u),
@@ -375,6 +377,7 @@ __global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restri
     }
     double Mc = 3.14159265358979323846 * kp.sigma * kp.sigma * ((double)cnt[c] * nden[c]) *
                 omax[c] * kp.dt / 2.0;
+    if (variant == MCPM_MODEL_CBA_VHS) Mc = Mc * yc[c];
     m = mcpm_floor(Mc + mcpm_u01(w.v[0]));
   } else {
     m = mcpm_floor(0.5 * omax[c] * (1.0 * (double)cnt[c]) + mcpm_u01(w.v[0]));
@@ -442,11 +445,13 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *
 
 
 // DSMC pair draw (collisions.cpp:2093-2098): two distinct particles of cell c.
-__device__ __forceinline__ void draw_dsmc_pair(uint32_t c, uint32_t t, const uint32_t *cnt,
+// The draws of candidate k (global index) are keyed by (k, c): the commit
+// finds both in the operation record, whatever the particles' positions.
+__device__ __forceinline__ void draw_dsmc_pair(uint32_t c, uint32_t k, const uint32_t *cnt,
                                                const uint32_t *start, const KParams &kp,
                                                uint32_t step_lo, uint32_t step_hi, uint32_t *oi,
                                                uint32_t *oj) {
-  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
+  mcpm_u32x4 w = mcpm_philox(k, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
                              kp.key1);
   uint32_t Nc = cnt[c];
   uint32_t li = mcpm_uint_below(w.v[0], Nc);
@@ -602,7 +607,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
       const uint32_t t = k - cstart[c];
       if (cnt[c] > 0) {
         if (MODEL == COLL_DSMC) {
-          draw_dsmc_pair(c, t, cnt, start, kp, step_lo, step_hi, &i, &j);
+          draw_dsmc_pair(c, k, cnt, start, kp, step_lo, step_hi, &i, &j);
           ok = true;
         } else {
           ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
@@ -622,7 +627,9 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
     }
     if (live) {
       const uint32_t pos = atomicAdd(&spos[q], 1u);
-      bands[pos] = ok ? make_uint4(i, j, pr, k - cstart[c]) : make_uint4(OP_NONE, 0u, 0u, 0u);
+      // the fourth word: the candidate's index in its cell (ESMC) or its cell (DSMC)
+      const uint32_t w4 = MODEL == COLL_DSMC ? c : k - cstart[c];
+      bands[pos] = ok ? make_uint4(i, j, pr, w4) : make_uint4(OP_NONE, 0u, 0u, 0u);
     }
   }
   for (int off = 32; off > 0; off >>= 1) nvalid += __shfl_down(nvalid, off);
@@ -670,26 +677,53 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R 
   return false;
 }
 
+// The CBA displacement (collisions.cpp:2221-2238, 2325-2340): both particles
+// move by -+sigma along the change of their relative velocity, read back from
+// the stored (rounded) post-collision velocities.  The reference divides by
+// the norm of that change unguarded; a pair whose stored relative velocity
+// did not change is left in place instead of moved to NaN.
+template <typename R>
+__device__ __forceinline__ void cba_displace(R &a, R &bb, double pre0, double pre1, double pre2,
+                                             double sigma) {
+  using real = typename R::real_type;
+  const double e0 = ((double)a.u1 - (double)bb.u1) - pre0;
+  const double e1 = ((double)a.u2 - (double)bb.u2) - pre1;
+  const double e2 = ((double)a.u3 - (double)bb.u3) - pre2;
+  const double nrm = __builtin_sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  if (!(nrm > 0.0)) return;
+  const double d0 = e0 * (sigma / nrm), d1 = e1 * (sigma / nrm);
+  if constexpr (R::dims == 2) {
+    a.x = (real)((double)a.x + d0);
+    bb.x = (real)((double)bb.x - d0);
+  }
+  a.y = (real)((double)a.y + d1);
+  bb.y = (real)((double)bb.y - d1);
+}
+
 // Collision_DSMC_VHS commit (collisions.cpp:2099-2131): VHS cross-section
 // sigma_T ~ Vr^(2-2 visp) crref^(2 visp - 1), accept iff sigma_T / crm > U,
 // isotropic post-collision relative velocity about the centre of mass.
+// fpp.model selects the variant: DSMC_HS weighs by sigma_T = Vr
+// (collisions.cpp:2026), CBA_VHS adds the displacement.  Candidate k of
+// cell c (both from the operation record).
 template <typename R>
-__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, R a, R bb, R *P,
-                                            const double *omax, double *omega_new,
+__device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t k, uint32_t c, R a,
+                                            R bb, R *P, const double *omax, double *omega_new,
                                             const KParams &kp, const FPParams &fpp,
                                             uint32_t step_lo, uint32_t step_hi) {
   using real = typename R::real_type;
-  const uint32_t c = (uint32_t)cell_key(a, kp);
-  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
+  mcpm_u32x4 w = mcpm_philox(k, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 0u), kp.key0,
                              kp.key1);
   double g0 = (double)a.u1 - (double)bb.u1;
   double g1 = (double)a.u2 - (double)bb.u2;
   double g2 = (double)a.u3 - (double)bb.u3;
   double Vr = __builtin_sqrt(g0 * g0 + g1 * g1 + g2 * g2);
-  double sig = det_pow(Vr, 2.0 - 2.0 * fpp.visp) * det_pow(fpp.crref, 2.0 * fpp.visp - 1.0);
+  double sig = fpp.model == MCPM_MODEL_DSMC_HS
+                   ? Vr
+                   : det_pow(Vr, 2.0 - 2.0 * fpp.visp) * det_pow(fpp.crref, 2.0 * fpp.visp - 1.0);
   atomic_max_pos_double(&omega_new[c], sig);
   if (!(sig / omax[c] > mcpm_u01(w.v[2]))) return false;
-  mcpm_u32x4 w2 = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 1u), kp.key0,
+  mcpm_u32x4 w2 = mcpm_philox(k, c, step_lo, mcpm_ctr3(MCPM_TAG_DSMC, step_hi, 1u), kp.key0,
                               kp.key1);
   double ct = 2.0 * mcpm_u01(w.v[3]) - 1.0;
   double st = __builtin_sqrt(1.0 - ct * ct);
@@ -705,6 +739,7 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t t, 
   bb.u1 = (real)(c0 - 0.5 * v0);
   bb.u2 = (real)(c1 - 0.5 * v1);
   bb.u3 = (real)(c2 - 0.5 * v2);
+  if (fpp.model == MCPM_MODEL_CBA_VHS) cba_displace(a, bb, g0, g1, g2, kp.sigma);
   P[i] = a;
   P[j] = bb;
   return true;
@@ -815,8 +850,8 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
       const uint32_t i = rec[u].x, j = rec[u].y, pr = rec[u].z;
       if (win[u]) {
         bool acc = MODEL == COLL_DSMC
-                       ? commit_dsmc<R>(i, j, rec[u].w, pa[u], pb[u], P, omax, omega_new, kp,
-                                        fpp, step_lo, step_hi)
+                       ? commit_dsmc<R>(i, j, pr, rec[u].w, pa[u], pb[u], P, omax, omega_new,
+                                        kp, fpp, step_lo, step_hi)
                        : commit_op<R>(i, j, rec[u].w, pa[u], pb[u], P, nden, yc, omax,
                                       omega_new, kp, step_lo, step_hi);
         acc_local += acc ? 1u : 0u;
@@ -871,13 +906,16 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
 }
 
 __global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ omega_new,
-                               const uint32_t *__restrict__ cnt, int nc, int model) {
+                               const uint32_t *__restrict__ cnt, int nc, int model,
+                               int variant) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
   double o = omega_new[c];
   if (model == COLL_DSMC) {
-    // crm <- max(1.1 crm_seen, 0.9 crm) (collisions.cpp:2136)
-    if (cnt[c] > 1) {
+    // crm <- max(1.1 crm_seen, 0.9 crm) (collisions.cpp:2136), in cells with
+    // two or more particles for DSMC_VHS, in every cell for DSMC_HS and
+    // CBA_VHS (collisions.cpp:2059, 2247)
+    if (cnt[c] > 1 || variant != MCPM_MODEL_DSMC_VHS) {
       double a = 1.1 * o, b = 0.9 * omax[c];
       omax[c] = a > b ? a : b;
     }
@@ -1158,6 +1196,94 @@ __global__ __launch_bounds__(256) void k_sample(const R *__restrict__ P,
 // ---------------------------------------------------------------------------
 // explicit instantiations: 1D (Rec) and 2D (Rec2) records, FP32 and FP64
 // ---------------------------------------------------------------------------
+// Collision_CBA_HS (collisions.cpp:2251-2346), one thread per cell.  The
+// number of candidates grows whenever an accepted pair is faster than
+// Vr_max, so the candidates of a cell run one after the other, as in the
+// reference; cells are independent (each touches only its own particles).
+// Candidate t of cell c draws (i, j, r, alpha2) from one Philox block keyed
+// (t, c); i and j are drawn with replacement (collisions.cpp:2282-2283) and
+// r serves both the acceptance and the polar angle (collisions.cpp:2298).
+// T is the cell temperature of the moments, n the cell number density.  A
+// cell whose bound passes CBA_MAX_CAND (non-finite or absurd velocities)
+// stops there and raises the capacity error.
+template <typename R>
+__global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
+                                               const uint32_t *__restrict__ cnt,
+                                               const uint32_t *__restrict__ start,
+                                               const double *__restrict__ mom,
+                                               const double *__restrict__ nden, KParams kp,
+                                               uint32_t step_lo, uint32_t step_hi,
+                                               uint32_t *__restrict__ counters,
+                                               uint32_t *__restrict__ err) {
+  using real = typename R::real_type;
+  constexpr double PI = 3.14159265358979323846;
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t ncand = 0, nacc = 0;
+  const uint32_t N = c < kp.n_cells ? cnt[c] : 0u;
+  if (N >= 2) {
+    // a single particle has T = 0 and so no candidates
+    const uint32_t s = start[c];
+    const double T = mom[(size_t)c * MCPM_NMOM + 29];
+    const double rho = nden[c] * kp.mass;
+    double vmax = __builtin_sqrt(PI * kp.kb * T / kp.mass);
+    const double q = 1.0 - kp.b * rho / 4.0;
+    const double Y = (1.0 - kp.b * rho / 8.0) / (q * q * q);
+    auto mcand = [&](double v) {
+      return kp.Fn * (double)N * (double)N * PI * kp.sigma * kp.sigma * v * kp.dt /
+             (2.0 * kp.volume) * Y;
+    };
+    double lim = mcpm_floor(mcand(vmax));
+    for (uint32_t t = 0; (double)t < lim; ++t) {
+      if (!(lim <= (double)CBA_MAX_CAND)) {
+        atomicOr(err, ERRBIT_CAPACITY);
+        break;
+      }
+      const mcpm_u32x4 w = mcpm_philox(t, (uint32_t)c, step_lo,
+                                       mcpm_ctr3(MCPM_TAG_CBA, step_hi, 0u), kp.key0, kp.key1);
+      const uint32_t i = s + mcpm_uint_below(w.v[0], N);
+      const uint32_t j = s + mcpm_uint_below(w.v[1], N);
+      const double r = mcpm_u01(w.v[2]);
+      ++ncand;
+      R a = P[i], bb = P[j];
+      const double g0 = (double)a.u1 - (double)bb.u1;
+      const double g1 = (double)a.u2 - (double)bb.u2;
+      const double g2 = (double)a.u3 - (double)bb.u3;
+      const double Vr = __builtin_sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+      if (!(Vr / vmax > r)) continue;
+      if (Vr - vmax > 1e-14) {
+        vmax = Vr;
+        lim = mcpm_floor(mcand(vmax));
+      }
+      const double ct = 1.0 - 2.0 * r;
+      const double st = __builtin_sqrt(1.0 - ct * ct);
+      double sp, cp;
+      mcpm_sincos2pi(mcpm_u01(w.v[3]), &sp, &cp);
+      const double v2 = Vr * st * cp, v1 = Vr * st * sp, v0 = Vr * ct;
+      const double c0 = 0.5 * ((double)a.u1 + (double)bb.u1);
+      const double c1 = 0.5 * ((double)a.u2 + (double)bb.u2);
+      const double c2 = 0.5 * ((double)a.u3 + (double)bb.u3);
+      a.u1 = (real)(c0 + 0.5 * v0);
+      a.u2 = (real)(c1 + 0.5 * v1);
+      a.u3 = (real)(c2 + 0.5 * v2);
+      bb.u1 = (real)(c0 - 0.5 * v0);
+      bb.u2 = (real)(c1 - 0.5 * v1);
+      bb.u3 = (real)(c2 - 0.5 * v2);
+      cba_displace(a, bb, g0, g1, g2, kp.sigma);
+      P[i] = a;
+      P[j] = bb;
+      ++nacc;
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    ncand += __shfl_down(ncand, off);
+    nacc += __shfl_down(nacc, off);
+  }
+  if (lane_id() == 0 && ncand) {
+    atomicAdd(&counters[CNT_OPS], ncand);
+    atomicAdd((unsigned long long *)&counters[CNT_ACC], (unsigned long long)nacc);
+  }
+}
+
 #define MCPM_INST_COLL(R, M)                                                               \
   template __global__ void k_gen<R, M>(const R *, const uint32_t *, const uint32_t *,      \
                                        const uint32_t *, int, uint32_t, KParams, uint32_t, \
@@ -1179,6 +1305,9 @@ __global__ __launch_bounds__(256) void k_sample(const R *__restrict__ P,
                                            uint32_t *, const uint32_t *, uint32_t *);      \
   MCPM_INST_COLL(R, COLL_ESMC)                                                             \
   MCPM_INST_COLL(R, COLL_DSMC)                                                             \
+  template __global__ void k_cba_hs<R>(R *, const uint32_t *, const uint32_t *,            \
+                                       const double *, const double *, KParams, uint32_t,  \
+                                       uint32_t, uint32_t *, uint32_t *);                  \
   template __global__ void k_moments<R>(const R *, const uint32_t *, const uint32_t *,     \
                                         double *, KParams);                                \
   template __global__ void k_sample<R>(const R *, const uint32_t *, const uint32_t *,      \

@@ -34,6 +34,7 @@ BC_DIFFUSE, BC_PERIODIC, BC_SPECULAR = 0, 1, 2
 Y_REFERENCE, Y_CARNAHAN_STARLING = 0, 1
 LR_NONE, LR_DIRECT, LR_DENSITY_EXPANSION, LR_SCREENED_POISSON = 0, 1, 2, 3
 MODEL_ESMC, MODEL_FP_DENSE, MODEL_FP_IDEAL, MODEL_DSMC_VHS, MODEL_NONE = 0, 1, 2, 3, 4
+MODEL_DSMC_HS, MODEL_CBA_VHS, MODEL_CBA_HS, MODEL_FP_LINEAR = 5, 6, 7, 8
 NMOM, NSAMP, NCLOS = 34, 26, 28
 
 STATUS = {
@@ -226,6 +227,9 @@ def load_library() -> ctypes.CDLL:
         "mcpm_velocity_fp": (i32, [vp, i64]),
         "mcpm_get_closures": (i32, [vp, dpp]),
         "mcpm_collide_dsmc_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_collide_dsmc_hs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_collide_cba_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_collide_cba_hs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "mcpm_vlasov_integral": (i32, [vp]),
         "mcpm_density_expansion": (i32, [vp]),
         "mcpm_get_kde": (i32, [vp, dpp, dpp, dpp]),
@@ -381,11 +385,26 @@ class Simulation:
         self._check(self._lib.mcpm_get_closures(self._h, _dp(out)), "mcpm_get_closures")
         return out.reshape(self.n_cells, NCLOS)
 
-    def collide_dsmc_vhs(self, step: int) -> dict:
+    def _collide(self, name: str, step: int) -> dict:
         s = CStats()
-        self._check(self._lib.mcpm_collide_dsmc_vhs(self._h, step, ctypes.byref(s)),
-                    "mcpm_collide_dsmc_vhs")
+        self._check(getattr(self._lib, name)(self._h, step, ctypes.byref(s)), name)
         return s.as_dict()
+
+    def collide_dsmc_vhs(self, step: int) -> dict:
+        """Collision_DSMC_VHS (collisions.cpp:2063-2141)."""
+        return self._collide("mcpm_collide_dsmc_vhs", step)
+
+    def collide_dsmc_hs(self, step: int) -> dict:
+        """Collision_DSMC_HS (collisions.cpp:1989-2061)."""
+        return self._collide("mcpm_collide_dsmc_hs", step)
+
+    def collide_cba_vhs(self, step: int) -> dict:
+        """Collision_CBA_VHS (collisions.cpp:2144-2248)."""
+        return self._collide("mcpm_collide_cba_vhs", step)
+
+    def collide_cba_hs(self, step: int) -> dict:
+        """Collision_CBA_HS (collisions.cpp:2251-2346)."""
+        return self._collide("mcpm_collide_cba_hs", step)
 
     def init_omega_max(self, n_gas: float, times_y: bool = True):
         self._check(self._lib.mcpm_init_omega_max(self._h, float(n_gas), 1 if times_y else 0),
@@ -393,6 +412,8 @@ class Simulation:
 
     def set_omega_max(self, om):
         om = np.ascontiguousarray(om, dtype=np.float64)
+        if om.size != self.n_cells:
+            raise ValueError(f"omega_max needs {self.n_cells} values, got {om.size}")
         self._check(self._lib.mcpm_set_omega_max(self._h, _dp(om)), "mcpm_set_omega_max")
 
     def moments(self) -> np.ndarray:

@@ -20,6 +20,10 @@
  *   BC_specular_reflection (slab axis)          Main_Functions.cpp:6399-6525
  *   cell_poly_velocity                          Main_Functions.cpp:1918-1968
  *   omega_max initialisation                    main.cpp:250-269
+ *   Collision_DSMC_VHS / _HS                    collisions.cpp:2063-2141, 1989-2061
+ *   Collision_CBA_VHS / _HS                     collisions.cpp:2144-2248, 2251-2346
+ *   random_generator + Velocity_FP_parallel /   Fokker_Planck.cpp:20-145,
+ *     Velocity_FP_linear                        1065-1182, 1186-1287
  * with these deliberate, documented substitutions:
  *   - random numbers: Philox4x32-10 addressed by (purpose, step, cell or
  *     particle, draw) from mcpm_rng.h instead of random_device-seeded
@@ -1024,11 +1028,14 @@ void orc_noise(const Oracle &o, int64_t step, std::vector<double> &xi) {
   }
 }
 
-// Velocity_FP_parallel (Fokker_Planck.cpp:1065-1182).
+// Velocity_FP_parallel (Fokker_Planck.cpp:1065-1182); with model FP_LINEAR
+// Velocity_FP_linear (Fokker_Planck.cpp:1186-1287): tau = 2 mu / (n kB T)
+// (Fokker_Planck.cpp:1237-1238, Y = 1), no drift, the same rescale.
 template <typename real>
 void orc_fp_velocity(Oracle &o, int64_t step) {
   const mcpm_params &p = o.p;
   const bool dense = p.model == MCPM_MODEL_FP_DENSE;
+  const bool linear = p.model == MCPM_MODEL_FP_LINEAR;
   std::vector<double> xi;
   orc_noise(o, step, xi);
   for (int c = 0; c < o.nc; ++c) {
@@ -1077,6 +1084,12 @@ void orc_fp_velocity(Oracle &o, int64_t step) {
       double n0 = A1 * a + A2 * d0 + A3 * x[0];
       double n1 = A1 * b + A2 * d1 + A3 * x[1];
       double n2 = A1 * d + A2 * d2 + A3 * x[2];
+      if (linear) {
+        // Velocity_FP_linear: Mp = A1 Mp + A3 xi (Fokker_Planck.cpp:1262-1264)
+        n0 = A1 * a + A3 * x[0];
+        n1 = A1 * b + A3 * x[1];
+        n2 = A1 * d + A3 * x[2];
+      }
       if (n1 != n1) o.err |= 2u;  // Fokker_Planck.cpp:1137-1149
       mp[3 * k] = n0;
       mp[3 * k + 1] = n1;
@@ -1093,11 +1106,42 @@ void orc_fp_velocity(Oracle &o, int64_t step) {
   }
 }
 
+// The CBA displacement (collisions.cpp:2221-2238): the pair moves by -+sigma
+// along the change of its relative velocity, read back from the stored
+// velocities; x1 is the plane's x, x2 the slab or plane y.  A pair whose
+// stored relative velocity did not change stays in place (the reference
+// would divide 0 by 0).
+template <typename real>
+void orc_cba_displace(Oracle &o, uint32_t i, uint32_t j, const double pre[3]) {
+  double *pi = &o.rec[(size_t)o.S * i], *pj = &o.rec[(size_t)o.S * j];
+  const double *ui = pi + o.U - 1, *uj = pj + o.U - 1;
+  double post[3] = {ui[1] - uj[1], ui[2] - uj[2], ui[3] - uj[3]};
+  double e0 = post[0] - pre[0], e1 = post[1] - pre[1], e2 = post[2] - pre[2];
+  double nrm = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+  if (!(nrm > 0.0)) return;
+  double d0 = e0 * (o.p.sigma / nrm), d1 = e1 * (o.p.sigma / nrm);
+  if (o.dims == 2) {
+    pi[0] = rnd<real>(pi[0] + d0);
+    pj[0] = rnd<real>(pj[0] - d0);
+    pi[1] = rnd<real>(pi[1] + d1);
+    pj[1] = rnd<real>(pj[1] - d1);
+  } else {
+    pi[0] = rnd<real>(pi[0] + d1);
+    pj[0] = rnd<real>(pj[0] - d1);
+  }
+}
+
 // Collision_DSMC_VHS (collisions.cpp:2063-2141): pairs within a cell,
 // candidates in index order, VHS cross-section, isotropic scattering;
-// crm <- max(1.1 crm_observed, 0.9 crm) per cell.
+// crm <- max(1.1 crm_observed, 0.9 crm) per cell with two or more
+// particles.  variant DSMC_HS (collisions.cpp:1989-2061): sigma_T = Vr and
+// crm updated in every cell; CBA_VHS (collisions.cpp:2144-2248): the count
+// times Y(n_c), the displacement, crm updated in every cell.  Cells with
+// fewer than two particles draw nothing (the reference's pair loop would not
+// end for one particle).  Candidate k (global, cell order) of cell c draws
+// from Philox blocks keyed (k, c).
 template <typename real>
-void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st) {
+void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st, int variant) {
   const mcpm_params &p = o.p;
   uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
   uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
@@ -1109,15 +1153,20 @@ void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st) {
   uint32_t kglob = 0;
   for (int c = 0; c < o.nc; ++c) {
     const int Nc = o.cnt[c];
-    if (Nc < 2) continue;
+    if (Nc < 2) {
+      if (variant != MCPM_MODEL_DSMC_VHS) o.omax[c] = std::max(0.0, 0.9 * o.omax[c]);
+      continue;
+    }
     double Mc = PI * p.sigma * p.sigma * ((double)Nc * o.nden[c]) * o.omax[c] * p.delta_t / 2.0;
+    if (variant == MCPM_MODEL_CBA_VHS) Mc = Mc * o.yc[c];
     mcpm_u32x4 w0 = mcpm_philox((uint32_t)c, 0u, slo, mcpm_ctr3(MCPM_TAG_MCAND, shi, 0u), key0, key1);
     double mf = mcpm_floor(Mc + mcpm_u01(w0.v[0]));
     uint32_t m = mf > 0.0 ? (uint32_t)mf : 0u;
     M += m;
     double crm = 0.0;
     for (uint32_t t = 0; t < m; ++t) {
-      mcpm_u32x4 w = mcpm_philox(t, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_DSMC, shi, 0u), key0, key1);
+      const uint32_t k = kglob + t;
+      mcpm_u32x4 w = mcpm_philox(k, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_DSMC, shi, 0u), key0, key1);
       uint32_t li = mcpm_uint_below(w.v[0], (uint32_t)Nc);
       uint32_t lj = mcpm_uint_below(w.v[1], (uint32_t)(Nc - 1));
       if (lj >= li) lj += 1;
@@ -1125,13 +1174,15 @@ void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st) {
       double *ri = &o.rec[(size_t)o.S * i] + o.U - 1, *rj = &o.rec[(size_t)o.S * j] + o.U - 1;
       double g0 = ri[1] - rj[1], g1 = ri[2] - rj[2], g2 = ri[3] - rj[3];
       double Vr = sqrt(g0 * g0 + g1 * g1 + g2 * g2);
-      double sig = orc_pow(Vr, 2.0 - 2.0 * p.visp) * orc_pow(p.crref, 2.0 * p.visp - 1.0);
+      double sig = variant == MCPM_MODEL_DSMC_HS
+                       ? Vr
+                       : orc_pow(Vr, 2.0 - 2.0 * p.visp) * orc_pow(p.crref, 2.0 * p.visp - 1.0);
       if (sig - crm > 0.0) crm = sig;
       bool a = sig / o.omax[c] > mcpm_u01(w.v[2]);
       if (a) {
         double ct = 2.0 * mcpm_u01(w.v[3]) - 1.0;
         double stt = sqrt(1.0 - ct * ct);
-        mcpm_u32x4 w2 = mcpm_philox(t, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_DSMC, shi, 1u), key0, key1);
+        mcpm_u32x4 w2 = mcpm_philox(k, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_DSMC, shi, 1u), key0, key1);
         double sp, cp;
         mcpm_sincos2pi(mcpm_u01(w2.v[0]), &sp, &cp);
         double v0 = Vr * ct, v1 = Vr * stt * cp, v2 = Vr * stt * sp;
@@ -1142,10 +1193,14 @@ void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st) {
         rj[1] = rnd<real>(c0 - 0.5 * v0);
         rj[2] = rnd<real>(c1 - 0.5 * v1);
         rj[3] = rnd<real>(c2 - 0.5 * v2);
+        if (variant == MCPM_MODEL_CBA_VHS) {
+          const double pre[3] = {g0, g1, g2};
+          orc_cba_displace<real>(o, i, j, pre);
+        }
         ++acc;
       }
       ++valid;
-      o.log_p.push_back(kglob + t);
+      o.log_p.push_back(k);
       o.log_i.push_back(i);
       o.log_j.push_back(j);
       o.log_a.push_back(a ? 1 : 0);
@@ -1156,6 +1211,74 @@ void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st) {
   if (st) {
     st->candidates = M;
     st->valid = valid;
+    st->accepted = acc;
+    st->rounds = 0;
+    st->reserved0 = 0;
+  }
+}
+
+// Collision_CBA_HS (collisions.cpp:2251-2346), cell after cell, candidate
+// after candidate: Vr_max = sqrt(pi kB T / m), M = Fn N^2 pi sigma^2 Vr_max
+// dt / (2 V_c) Y with Y = (1 - b rho / 8) / (1 - b rho / 4)^3, rho = n m; the
+// loop bound floor(M) is re-read after every candidate, and an accepted pair
+// faster than Vr_max (by more than 1e-14) raises Vr_max and M.  Candidate t
+// of cell c draws (i, j, r, alpha2) from one Philox block keyed (t, c), i and
+// j with replacement; r is both the acceptance draw and alpha1.
+template <typename real>
+void orc_cba_hs(Oracle &o, int64_t step, mcpm_collision_stats *st) {
+  const mcpm_params &p = o.p;
+  uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
+  std::vector<double> mom((size_t)o.nc * MCPM_NMOM);
+  orc_moments(o, mom.data());
+  int64_t M = 0, acc = 0;
+  for (int c = 0; c < o.nc; ++c) {
+    const int Nc = o.cnt[c];
+    if (Nc < 2) continue;
+    const double T = mom[(size_t)c * MCPM_NMOM + 29];
+    const double rho = o.nden[c] * p.mass;
+    double vmax = sqrt(PI * p.kb * T / p.mass);
+    const double Vc = p.cell_volume;
+    double Mc = p.Fn * Nc * Nc * PI * p.sigma * p.sigma * vmax * p.delta_t / (2.0 * Vc);
+    const double q = 1.0 - p.b * rho / 4.0;
+    const double Y = (1.0 - p.b * rho / 8.0) / (q * q * q);
+    Mc = Mc * Y;
+    for (uint32_t t = 0; t < mcpm_floor(Mc); ++t) {
+      mcpm_u32x4 w = mcpm_philox(t, (uint32_t)c, slo, mcpm_ctr3(MCPM_TAG_CBA, shi, 0u), key0, key1);
+      uint32_t i = (uint32_t)o.start[c] + mcpm_uint_below(w.v[0], (uint32_t)Nc);
+      uint32_t j = (uint32_t)o.start[c] + mcpm_uint_below(w.v[1], (uint32_t)Nc);
+      double r = mcpm_u01(w.v[2]);
+      ++M;
+      double *ri = &o.rec[(size_t)o.S * i] + o.U - 1, *rj = &o.rec[(size_t)o.S * j] + o.U - 1;
+      double Vr = sqrt((ri[1] - rj[1]) * (ri[1] - rj[1]) + (ri[2] - rj[2]) * (ri[2] - rj[2]) +
+                       (ri[3] - rj[3]) * (ri[3] - rj[3]));
+      if (!(Vr / vmax > r)) continue;
+      if (Vr - vmax > 1e-14) {
+        vmax = Vr;
+        Mc = p.Fn * Nc * Nc * PI * p.sigma * p.sigma * vmax * p.delta_t / (2.0 * Vc);
+        Mc = Mc * Y;
+      }
+      const double pre[3] = {ri[1] - rj[1], ri[2] - rj[2], ri[3] - rj[3]};
+      double alpha2 = mcpm_u01(w.v[3]);
+      double qq = 1.0 - 2 * r;
+      double ct = qq, stt = sqrt(1.0 - qq * qq);
+      double sp, cp;
+      mcpm_sincos2pi(alpha2, &sp, &cp);
+      double v2 = Vr * stt * cp, v1 = Vr * stt * sp, v0 = Vr * ct;
+      double c0 = 0.5 * (ri[1] + rj[1]), c1 = 0.5 * (ri[2] + rj[2]), c2 = 0.5 * (ri[3] + rj[3]);
+      ri[1] = rnd<real>(c0 + 0.5 * v0);
+      ri[2] = rnd<real>(c1 + 0.5 * v1);
+      ri[3] = rnd<real>(c2 + 0.5 * v2);
+      rj[1] = rnd<real>(c0 - 0.5 * v0);
+      rj[2] = rnd<real>(c1 - 0.5 * v1);
+      rj[3] = rnd<real>(c2 - 0.5 * v2);
+      orc_cba_displace<real>(o, i, j, pre);
+      ++acc;
+    }
+  }
+  if (st) {
+    st->candidates = M;
+    st->valid = M;
     st->accepted = acc;
     st->rounds = 0;
     st->reserved0 = 0;
@@ -1835,11 +1958,31 @@ int orc_get_closures(void *h, double *out) {
   return MCPM_OK;
 }
 
-int orc_collide_dsmc_vhs(void *h, int64_t step, mcpm_collision_stats *st) {
+static int orc_dsmc_any(void *h, int64_t step, mcpm_collision_stats *st, int variant) {
   Oracle &o = *(Oracle *)h;
   if (!o.sorted) return MCPM_ERR_STATE;
-  if (o.p.precision == MCPM_FP32) orc_dsmc<float>(o, step, st);
-  else orc_dsmc<double>(o, step, st);
+  if (o.p.precision == MCPM_FP32) orc_dsmc<float>(o, step, st, variant);
+  else orc_dsmc<double>(o, step, st, variant);
+  return MCPM_OK;
+}
+
+int orc_collide_dsmc_vhs(void *h, int64_t step, mcpm_collision_stats *st) {
+  return orc_dsmc_any(h, step, st, MCPM_MODEL_DSMC_VHS);
+}
+
+int orc_collide_dsmc_hs(void *h, int64_t step, mcpm_collision_stats *st) {
+  return orc_dsmc_any(h, step, st, MCPM_MODEL_DSMC_HS);
+}
+
+int orc_collide_cba_vhs(void *h, int64_t step, mcpm_collision_stats *st) {
+  return orc_dsmc_any(h, step, st, MCPM_MODEL_CBA_VHS);
+}
+
+int orc_collide_cba_hs(void *h, int64_t step, mcpm_collision_stats *st) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  if (o.p.precision == MCPM_FP32) orc_cba_hs<float>(o, step, st);
+  else orc_cba_hs<double>(o, step, st);
   return MCPM_OK;
 }
 
@@ -1853,10 +1996,14 @@ int orc_step(void *h, int64_t step, mcpm_collision_stats *st) {
     case MCPM_MODEL_ESMC: r = orc_collide_esmc(h, step, st); break;
     case MCPM_MODEL_FP_DENSE:
     case MCPM_MODEL_FP_IDEAL:
+    case MCPM_MODEL_FP_LINEAR:
       r = orc_fp_closures(h, step);
       if (!r) r = orc_velocity_fp(h, step);
       break;
     case MCPM_MODEL_DSMC_VHS: r = orc_collide_dsmc_vhs(h, step, st); break;
+    case MCPM_MODEL_DSMC_HS: r = orc_collide_dsmc_hs(h, step, st); break;
+    case MCPM_MODEL_CBA_VHS: r = orc_collide_cba_vhs(h, step, st); break;
+    case MCPM_MODEL_CBA_HS: r = orc_collide_cba_hs(h, step, st); break;
     default: break;
   }
   if (r) return r;

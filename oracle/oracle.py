@@ -63,6 +63,9 @@ def load() -> ctypes.CDLL:
         "orc_velocity_fp": (i32, [vp, i64]),
         "orc_get_closures": (i32, [vp, dpp]),
         "orc_collide_dsmc_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "orc_collide_dsmc_hs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "orc_collide_cba_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "orc_collide_cba_hs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_vlasov_integral": (i32, [vp]),
         "orc_density_expansion": (i32, [vp]),
         "orc_get_kde": (i32, [vp, dpp, dpp, dpp]),
@@ -224,11 +227,22 @@ class OracleSim:
         self._check(self._lib.orc_get_closures(self._h, _dp(out)), "orc_get_closures")
         return out.reshape(self.n_cells, NCLOS)
 
-    def collide_dsmc_vhs(self, step):
+    def _collide(self, name, step):
         s = CStats()
-        self._check(self._lib.orc_collide_dsmc_vhs(self._h, step, ctypes.byref(s)),
-                    "orc_collide_dsmc_vhs")
+        self._check(getattr(self._lib, name)(self._h, step, ctypes.byref(s)), name)
         return s.as_dict()
+
+    def collide_dsmc_vhs(self, step):
+        return self._collide("orc_collide_dsmc_vhs", step)
+
+    def collide_dsmc_hs(self, step):
+        return self._collide("orc_collide_dsmc_hs", step)
+
+    def collide_cba_vhs(self, step):
+        return self._collide("orc_collide_cba_vhs", step)
+
+    def collide_cba_hs(self, step):
+        return self._collide("orc_collide_cba_hs", step)
 
     def init_omega_max(self, n_gas, times_y=True):
         self._check(self._lib.orc_init_omega_max(self._h, float(n_gas), 1 if times_y else 0),
@@ -236,6 +250,8 @@ class OracleSim:
 
     def set_omega_max(self, om):
         om = np.ascontiguousarray(om, dtype=np.float64)
+        if om.size != self.n_cells:
+            raise ValueError(f"omega_max needs {self.n_cells} values, got {om.size}")
         self._check(self._lib.orc_set_omega_max(self._h, _dp(om)), "orc_set_omega_max")
 
     def moments(self):
