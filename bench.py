@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-particles", type=int, default=20_000_000)
+    ap.add_argument("--pressure", action="store_true",
+                    help="sample measure_pressure's face sums in every step (the reference "
+                         "does once step > times * after)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (a known-size kernel for "
                          "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
@@ -73,6 +76,8 @@ def main():
     del arrs
     sim.cell_update()
     sim.init_omega_max(meta["n_gas"])
+    if args.pressure:
+        sim.measure_pressure(True)
     step = 0
     t0 = time.time()
     for _ in range(args.warmup):
@@ -162,6 +167,7 @@ def main():
         "config": {"workload": "interface: periodic 400 sigma, 4096 cells, 20M particles, "
                                "n=0.55/0.02, T=0.7, ESMC + screened-Poisson, dt=0.02",
                    "particles": N, "cells": args.cells, "parallelism": "1 GPU",
+                   "pressure_sampling": bool(args.pressure),
                    "valid_ops_per_step": valid, "accepted_per_step": acc,
                    "reservation_rounds": rounds,
                    "phases_ms_per_step": {k: v["per_step_ms"] for k, v in timers.items()}},

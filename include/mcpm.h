@@ -311,6 +311,20 @@ int mcpm_sample(mcpm_ctx *ctx);
 int mcpm_get_samples(mcpm_ctx *ctx, double *out);
 int mcpm_reset_samples(mcpm_ctx *ctx);
 
+/* measure_pressure (cpp_headers.h:388; Main_Functions.cpp:13-34, called
+   after streaming at main.cpp:490-492): while enabled, every moving push
+   (mcpm_stream_bc, mcpm_step) adds Fn m |U2| to p_m of each cell whose upper
+   face lies between the particle's cell before streaming (after the FP drift)
+   and its cell after the boundary.  On a periodic slab the faces of the
+   unwrapped flight are taken.  Slab only.  p_m accumulates on the device
+   until mcpm_reset_samples. */
+int mcpm_measure_pressure(mcpm_ctx *ctx, int32_t enable);
+/* The pressure columns of PostProc (myIO.cpp:743-744, 855, 889-890): p_m,
+   p_rep = n kB T (1 + n b Y(n)) and ptot = p_rep - 2 b n^2 phi, the last two
+   from the current particles (cell T of the moments, cell n).  Arrays of
+   n_cells doubles; any of them may be null.  Slab only. */
+int mcpm_get_pressure(mcpm_ctx *ctx, double *p_m, double *p_rep, double *p_tot);
+
 /* increase_collision_rate (cpp_headers.h:505; Main_Functions.cpp:1906-1915),
    evaluated on the host with the context's parameters. */
 double mcpm_contact_y(const mcpm_params *p, double n);

@@ -40,6 +40,9 @@ struct mcpm_ctx {
   uint32_t *cnt = nullptr, *start = nullptr, *mcand = nullptr, *cstart = nullptr;
   double *nden = nullptr, *yc = nullptr, *omax = nullptr, *omega_new = nullptr, *F2 = nullptr;
   double *moments = nullptr, *samples = nullptr;
+  // measure_pressure face sums, then the two PostProc pressure columns (3 x nc)
+  double *press = nullptr;
+  bool pressure_on = false;
   uint32_t *counters = nullptr, *err = nullptr;
   uint32_t *res = nullptr;  // reservations, two interleaved arrays
   // collisions
@@ -413,7 +416,10 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
   const R *in = (const R *)c->P[c->cur];
   R *out = (R *)c->P[1 - c->cur];
-  const Kick kick{c->F2, c->pot, c->kick_coef};
+  const Kick kick{c->F2, c->pot, c->kick_coef, nullptr};
+  // the face sums are taken once, in the scatter pass, of a moving push
+  Kick kick_scatter = kick;
+  if (c->pressure_on && (mode & PUSH_MOVE)) kick_scatter.pm = c->press;
   const size_t lds = (size_t)nc * sizeof(uint32_t);
   const double *drift = (c->p.model == MCPM_MODEL_FP_DENSE && c->have_closures &&
                          (mode & PUSH_MOVE))
@@ -431,7 +437,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
                      c->seg);
   hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
-                     c->n, chunk, c->kp, kick, drift, mode, slo, shi, c->table, c->start,
+                     c->n, chunk, c->kp, kick_scatter, drift, mode, slo, shi, c->table, c->start,
                      c->err);
   LAUNCHCHK();
   c->cur = 1 - c->cur;
@@ -831,6 +837,7 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   if ((st = dalloc(&c->F2, nc))) return fail(st);
   if ((st = dalloc(&c->moments, (size_t)nc * MCPM_NMOM))) return fail(st);
   if ((st = dalloc(&c->samples, (size_t)nc * MCPM_NSAMP))) return fail(st);
+  if ((st = dalloc(&c->press, (size_t)nc * 3))) return fail(st);
   if ((st = dalloc(&c->counters, CNT_TOTAL))) return fail(st);
   if ((st = dalloc(&c->clos, (size_t)nc * MCPM_NCLOS))) return fail(st);
   if ((st = dalloc(&c->drift, (size_t)nc * DRIFT_N))) return fail(st);
@@ -844,6 +851,7 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
       hipMemset(c->omega_new, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->F2, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->samples, 0, (size_t)nc * MCPM_NSAMP * sizeof(double)) != hipSuccess ||
+      hipMemset(c->press, 0, (size_t)nc * 3 * sizeof(double)) != hipSuccess ||
       hipMemset(c->err, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->cnt, 0, nc * sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->start, 0, (nc + 1) * sizeof(uint32_t)) != hipSuccess ||
@@ -867,7 +875,7 @@ int mcpm_destroy(mcpm_ctx *c) {
   if (c->s) hipStreamSynchronize(c->s);
   void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
-                  c->samples, c->counters, c->err, c->res, c->opA,
+                  c->samples, c->press, c->counters, c->err, c->res, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
                   c->fem_buf, c->q4};
   for (void *b : bufs)
@@ -1246,6 +1254,38 @@ int mcpm_get_samples(mcpm_ctx *c, double *out) {
 int mcpm_reset_samples(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
   HIPCHK(hipMemsetAsync(c->samples, 0, (size_t)c->nc * MCPM_NSAMP * sizeof(double), c->s));
+  HIPCHK(hipMemsetAsync(c->press, 0, (size_t)c->nc * sizeof(double), c->s));
+  return MCPM_OK;
+}
+
+int mcpm_measure_pressure(mcpm_ctx *c, int32_t enable) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims != 1) return MCPM_ERR_ARG;
+  c->pressure_on = enable != 0;
+  return MCPM_OK;
+}
+
+int mcpm_get_pressure(mcpm_ctx *c, double *p_m, double *p_rep, double *p_tot) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims != 1) return MCPM_ERR_ARG;
+  const size_t nc = (size_t)c->nc;
+  if (p_rep || p_tot) {
+    if (!c->sorted) return MCPM_ERR_STATE;
+    int st = MCPM_DISPATCH(c, launch_moments, c);
+    if (st) return st;
+    hipLaunchKernelGGL(k_pressure, dim3((c->nc + 255) / 256), dim3(256), 0, c->s, c->moments,
+                       c->nden, c->yc, c->press + nc, c->press + 2 * nc, c->kp, c->p.phi);
+    LAUNCHCHK();
+  }
+  if (p_m)
+    HIPCHK(hipMemcpyAsync(p_m, c->press, nc * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  if (p_rep)
+    HIPCHK(hipMemcpyAsync(p_rep, c->press + nc, nc * sizeof(double), hipMemcpyDeviceToHost,
+                          c->s));
+  if (p_tot)
+    HIPCHK(hipMemcpyAsync(p_tot, c->press + 2 * nc, nc * sizeof(double),
+                          hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
   return MCPM_OK;
 }
 

@@ -37,6 +37,7 @@ struct Kick {
   const double *F2;
   const double *xnode;
   double coef;
+  double *pm;  // measure_pressure face sums (slab), or null when not sampling
 };
 
 // Constants every kernel reads, passed by value.  n_cells is the total

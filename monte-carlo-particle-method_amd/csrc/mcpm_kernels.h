@@ -114,6 +114,8 @@ __global__ void k_moments(const R *P, const uint32_t *cnt, const uint32_t *start
 template <typename R>
 __global__ void k_sample(const R *P, const uint32_t *cnt, const uint32_t *start,
                          double *acc);
+__global__ void k_pressure(const double *mom, const double *nden, const double *yc,
+                           double *p_rep, double *p_tot, KParams kp, double phi);
 __global__ void k_fp_closures(const double *mom, const uint32_t *cnt, const double *nden,
                               const double *yc, double *clos, double *drift, KParams kp,
                               FPParams fpp, int64_t step, uint32_t *err);

@@ -19,6 +19,32 @@ namespace mcpm {
 // ---------------------------------------------------------------------------
 // kick + stream + boundary of one particle
 // ---------------------------------------------------------------------------
+// measure_pressure (Main_Functions.cpp:13-34): every face between the cell of
+// x2_old (after the drift, before streaming) and the cell of x2 (after the
+// boundary) adds Fn m |U2| to the cell below it.  Periodic slab: the faces of
+// the unwrapped flight, wrapped into [0, nc) (the reference's slab has walls).
+// Faces outside the slab are dropped instead of written out of bounds.
+__device__ __forceinline__ void pressure_faces(double *pm, const KParams &kp, double y_old,
+                                               double y_new, double u2, bool periodic) {
+  const double qa = mcpm_floor(y_old / kp.dy), qb = mcpm_floor(y_new / kp.dy);
+  if (!(qa != qb) || !(__builtin_fabs(qa) < 1e12) || !(__builtin_fabs(qb) < 1e12)) return;
+  const int64_t a = (int64_t)qa, b = (int64_t)qb;
+  const int64_t nc = kp.n_cells;
+  int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+  if (hi - lo > nc) hi = lo + nc;
+  const double f = kp.Fn * kp.mass * __builtin_fabs(u2);
+  for (int64_t k = lo; k < hi; ++k) {
+    int64_t c = k;
+    if (periodic) {
+      c %= nc;
+      if (c < 0) c += nc;
+    } else if (c < 0 || c >= nc) {
+      continue;
+    }
+    atomicAdd(&pm[c], f);
+  }
+}
+
 // Position_FP_ideal_gas (Fokker_Planck.cpp:948-969) for the slab axis, the
 // kick U2 += F2 dt of poisson_solver (Main_Functions.cpp:3574-3577), and the
 // slab-axis part of BC_thermal_wall_3D (Main_Functions.cpp:4917-5248):
@@ -53,10 +79,12 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
       drifted = true;
     }
   }
+  const double y_old = y;  // store_old_positions after DFP_extra_streaming
   y = y + v * kp.dt;
   // speed used for the time spent past a wall: the displacement this step
   const double v_eff = drifted ? (y - y_start) / kp.dt : v;
   if (kp.boundary == MCPM_BC_PERIODIC) {
+    if (kick.pm) pressure_faces(kick.pm, kp, y_old, y, v, true);
     if (y < 0.0) y += kp.length;
     else if (y >= kp.length) y -= kp.length;
     real yr = (real)y;
@@ -77,6 +105,7 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
     if (guard >= 20) atomicOr(err, ERRBIT_BC);
     r.y = (real)y;
     r.u2 = (real)v;
+    if (kick.pm) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
     return r;
   }
   // diffuse walls
@@ -116,6 +145,7 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
     ++bounce;
   }
   r.y = (real)y;
+  if (kick.pm) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
   return r;
 }
 
@@ -923,6 +953,20 @@ __global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ o
     omax[c] = o;  // collisions.cpp:714-716
   }
   omega_new[c] = 0.0;
+}
+
+// The pressure columns of PostProc (myIO.cpp:855, 890): n kB T (1 + n b Y(n))
+// and that minus the mean-field 2 b n^2 phi, from the moments' T and the cell
+// density.
+__global__ void k_pressure(const double *__restrict__ mom, const double *__restrict__ nden,
+                           const double *__restrict__ yc, double *__restrict__ p_rep,
+                           double *__restrict__ p_tot, KParams kp, double phi) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= kp.n_cells) return;
+  const double n = nden[c], T = mom[(size_t)c * MCPM_NMOM + 29];
+  const double pe = n * kp.kb * T * (1.0 + n * kp.b * yc[c]);
+  p_rep[c] = pe;
+  p_tot[c] = pe - 2.0 * kp.b * n * n * phi;
 }
 
 __global__ void k_zero_u32(uint32_t *p, int n) {

@@ -216,6 +216,8 @@ def load_library() -> ctypes.CDLL:
         "mcpm_sample": (i32, [vp]),
         "mcpm_get_samples": (i32, [vp, dpp]),
         "mcpm_reset_samples": (i32, [vp]),
+        "mcpm_measure_pressure": (i32, [vp, i32]),
+        "mcpm_get_pressure": (i32, [vp, dpp, dpp, dpp]),
         "mcpm_contact_y": (ctypes.c_double, [ctypes.POINTER(CParams), ctypes.c_double]),
         "mcpm_debug_collision_log": (i32, [vp, i32]),
         "mcpm_get_collision_log": (i32, [vp, i64, u32pp, u32pp, u32pp,
@@ -428,6 +430,18 @@ class Simulation:
         out = np.empty(self.n_cells * NSAMP)
         self._check(self._lib.mcpm_get_samples(self._h, _dp(out)), "mcpm_get_samples")
         return out.reshape(self.n_cells, NSAMP)
+
+    def measure_pressure(self, enable: bool = True):
+        """measure_pressure (Main_Functions.cpp:13-34) on every moving push."""
+        self._check(self._lib.mcpm_measure_pressure(self._h, 1 if enable else 0),
+                    "mcpm_measure_pressure")
+
+    def pressure(self) -> dict:
+        """PostProc's p_m, p_rep and ptot columns (myIO.cpp:855, 889-890)."""
+        out = {k: np.empty(self.n_cells) for k in ("p_m", "p_rep", "p_tot")}
+        self._check(self._lib.mcpm_get_pressure(self._h, _dp(out["p_m"]), _dp(out["p_rep"]),
+                                                _dp(out["p_tot"])), "mcpm_get_pressure")
+        return out
 
     def reset_samples(self):
         self._check(self._lib.mcpm_reset_samples(self._h), "mcpm_reset_samples")

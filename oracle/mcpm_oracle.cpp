@@ -101,6 +101,9 @@ struct Oracle {
   double kick_coef = 0.0;
   std::vector<int> cnt, start;
   std::vector<double> nden, yc, omax, F2, samples;
+  // measure_pressure face sums (slab)
+  std::vector<double> pm;
+  bool pressure_on = false;
   bool kick_pending = false;
   bool sorted = false;
   uint32_t err = 0;
@@ -215,6 +218,23 @@ void orc_sort(Oracle &o, bool check) {
   o.sorted = true;
 }
 
+// measure_pressure (Main_Functions.cpp:13-34): cells num in [min, max) of
+// the old and new cell indices take Fn m |U2|; on a periodic slab the cells
+// of the unwrapped flight modulo nc; indices outside the slab are skipped.
+void orc_pressure_faces(Oracle &o, double y_old, double y_new, double u2, bool periodic) {
+  double qa = floor(y_old / o.dy), qb = floor(y_new / o.dy);
+  if (qa == qb || !(fabs(qa) < 1e12) || !(fabs(qb) < 1e12)) return;
+  long long a = (long long)qa, b = (long long)qb;
+  long long lo = std::min(a, b), hi = std::max(a, b);
+  if (hi - lo > o.nc) hi = lo + o.nc;
+  for (long long num = lo; num < hi; ++num) {
+    long long c = num;
+    if (periodic) c = ((c % o.nc) + o.nc) % o.nc;
+    else if (c < 0 || c >= o.nc) continue;
+    o.pm[c] += o.p.Fn * o.p.mass * fabs(u2);
+  }
+}
+
 // Kick, stream and boundary of one particle (see header list).
 template <typename real>
 void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, uint32_t shi) {
@@ -246,10 +266,12 @@ void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, u
       drifted = true;
     }
   }
+  const double y_old = y;  // x2_old (store_old_positions, main.cpp:484)
   y = y + v * p.delta_t;  // x2 += dt U2 (Fokker_Planck.cpp:963)
   // speed used for the time spent past a wall: the displacement this step
   const double v_eff = drifted ? (y - y_start) / p.delta_t : v;
   if (p.boundary == MCPM_BC_PERIODIC) {
+    if (o.pressure_on) orc_pressure_faces(o, y_old, y, v, true);
     if (y < 0.0) y += p.length;
     else if (y >= p.length) y -= p.length;
     real yr = (real)y;
@@ -269,6 +291,7 @@ void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, u
     if (guard >= 20) o.err |= 4u;
     r[0] = rnd<real>(y);
     r[2] = rnd<real>(v);
+    if (o.pressure_on) orc_pressure_faces(o, y_old, r[0], r[2], false);
     return;
   }
   // diffuse walls (BC_thermal_wall_3D, upper/lower wall branches :5044-5127)
@@ -308,6 +331,7 @@ void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, u
     ++bounce;
   }
   r[0] = rnd<real>(y);
+  if (o.pressure_on) orc_pressure_faces(o, y_old, r[0], r[2], false);
 }
 
 // One coordinate's periodic wrap, stored through `real` (the value must stay
@@ -1771,6 +1795,7 @@ void *orc_create(const mcpm_params *p) {
   o->omax.assign(o->nc, 0.0);
   o->F2.assign(o->nc, 0.0);
   o->samples.assign((size_t)o->nc * MCPM_NSAMP, 0.0);
+  o->pm.assign(o->nc, 0.0);
   return o;
 }
 
@@ -2029,6 +2054,36 @@ int orc_sample(void *h) {
 int orc_get_samples(void *h, double *out) {
   Oracle &o = *(Oracle *)h;
   memcpy(out, o.samples.data(), o.samples.size() * sizeof(double));
+  return MCPM_OK;
+}
+
+int orc_reset_samples(void *h) {
+  Oracle &o = *(Oracle *)h;
+  std::fill(o.samples.begin(), o.samples.end(), 0.0);
+  std::fill(o.pm.begin(), o.pm.end(), 0.0);
+  return MCPM_OK;
+}
+
+int orc_measure_pressure(void *h, int32_t enable) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 1) return MCPM_ERR_ARG;
+  o.pressure_on = enable != 0;
+  return MCPM_OK;
+}
+
+// PostProc's ptot and p_rep columns (myIO.cpp:855, 890) from the moments' T.
+int orc_get_pressure(void *h, double *p_m, double *p_rep, double *p_tot) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 1) return MCPM_ERR_ARG;
+  std::vector<double> mom((size_t)o.nc * MCPM_NMOM);
+  orc_moments(o, mom.data());
+  for (int c = 0; c < o.nc; ++c) {
+    const double n = o.nden[c], T = mom[(size_t)c * MCPM_NMOM + 29];
+    const double pr = n * o.p.kb * T * (1.0 + n * o.p.b * orc_y(n, o.p));
+    if (p_m) p_m[c] = o.pm[c];
+    if (p_rep) p_rep[c] = pr;
+    if (p_tot) p_tot[c] = pr - 2.0 * o.p.b * n * n * o.p.phi;
+  }
   return MCPM_OK;
 }
 

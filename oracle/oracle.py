@@ -54,6 +54,9 @@ def load() -> ctypes.CDLL:
         "orc_moments": (i32, [vp, dpp]),
         "orc_sample": (i32, [vp]),
         "orc_get_samples": (i32, [vp, dpp]),
+        "orc_reset_samples": (i32, [vp]),
+        "orc_measure_pressure": (i32, [vp, i32]),
+        "orc_get_pressure": (i32, [vp, dpp, dpp, dpp]),
         "orc_set_omega_max": (i32, [vp, dpp]),
         "orc_init_omega_max": (i32, [vp, ctypes.c_double, i32]),
         "orc_get_cells": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), dpp, dpp, dpp]),
@@ -261,6 +264,19 @@ class OracleSim:
 
     def sample(self):
         self._check(self._lib.orc_sample(self._h), "orc_sample")
+
+    def reset_samples(self):
+        self._check(self._lib.orc_reset_samples(self._h), "orc_reset_samples")
+
+    def measure_pressure(self, enable=True):
+        self._check(self._lib.orc_measure_pressure(self._h, 1 if enable else 0),
+                    "orc_measure_pressure")
+
+    def pressure(self):
+        out = {k: np.empty(self.n_cells) for k in ("p_m", "p_rep", "p_tot")}
+        self._check(self._lib.orc_get_pressure(self._h, _dp(out["p_m"]), _dp(out["p_rep"]),
+                                               _dp(out["p_tot"])), "orc_get_pressure")
+        return out
 
     def samples(self):
         out = np.empty(self.n_cells * NSAMP)
