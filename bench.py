@@ -42,6 +42,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-particles", type=int, default=20_000_000)
+    ap.add_argument("--workload", default="interface", choices=["interface", "droplets"],
+                    help="interface: the periodic 400 sigma slab the metric is quoted on; "
+                         "droplets: the 512 x 512 periodic plane with 20 liquid discs")
     ap.add_argument("--pressure", action="store_true",
                     help="sample measure_pressure's face sums in every step (the reference "
                          "does once step > times * after)")
@@ -69,10 +72,18 @@ def main():
         del src, dst
 
     N = args.particles
-    p, arrs, meta = W.interface(N=N, n_cells=args.cells)
-    log(f"workload: interface N={N} cells={args.cells} dt={p.delta_t} FP32")
-    sim = mcpm.Simulation(p)
-    sim.upload(*arrs)
+    if args.workload == "droplets":
+        p, arrs, meta = W.droplets(N=64_000_000 if N == 20_000_000 else N)
+        N = meta["N"]
+        args.cells = p.n_cells * p.n_cells_x
+        log(f"workload: droplets N={N} cells={p.n_cells_x}x{p.n_cells} dt={p.delta_t} FP32")
+        sim = mcpm.Simulation(p)
+        sim.upload_xy(*arrs)
+    else:
+        p, arrs, meta = W.interface(N=N, n_cells=args.cells)
+        log(f"workload: interface N={N} cells={args.cells} dt={p.delta_t} FP32")
+        sim = mcpm.Simulation(p)
+        sim.upload(*arrs)
     del arrs
     sim.cell_update()
     sim.init_omega_max(meta["n_gas"])
@@ -128,13 +139,17 @@ def main():
     #  16-byte operation record, summed over the step's launches.
     sort_ms = timers["sort"]["per_step_ms"]
     coll_ms = timers["collide_rounds"]["per_step_ms"]
-    sort_bytes = 48.0 * N
-    coll_bytes = valid * (64.0 + 16.0)
+    plane = args.workload == "droplets"
+    rec = 32.0 if plane else 16.0       # FP32 record: (x, y, u1..3, pad) or (y, u1..3)
+    passes = 2.0 if args.cells > 16384 else 1.0   # two-digit sort past 16384 cells
+    sort_bytes = 3.0 * rec * N * passes
+    coll_bytes = valid * (4.0 * rec + 16.0)
+    step_alg = 44 if plane else 36      # 2 S_p + 4 with S_p = 20 (plane) or 16 (slab)
     if coll_ms >= sort_ms:
         launches = timers["collide_rounds"]["calls"] / args.steps
         dom, dom_ms, dom_bytes = "k_round", coll_ms, coll_bytes
     else:
-        launches = 2.0
+        launches = 2.0 * passes
         dom, dom_ms, dom_bytes = "k_push (both passes)", sort_ms, sort_bytes
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     traffic = measured_traffic(dom, N, args.cells)
@@ -144,15 +159,23 @@ def main():
                 "launches_per_step": launches,
                 "avg_launch_us": round(1000.0 * dom_ms / max(launches, 1.0), 2),
                 "alg_bytes_per_launch": round(dom_bytes / max(launches, 1.0)),
-                "step_alg_bytes_per_particle": 36,
-                "step_frac": round(value * 36.0 / 1e9 / HBM_PEAK_GBS, 4)}
+                "step_alg_bytes_per_particle": step_alg,
+                "step_frac": round(value * step_alg / 1e9 / HBM_PEAK_GBS, 4)}
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and not plane:
         cpu = cpu_baseline(args)
+    if plane:
+        desc = (f"droplets: periodic 128 x 128 sigma plane, {p.n_cells_x} x {p.n_cells} cells, "
+                f"{N} particles, 20 discs n=0.55 in vapour 0.02, T=0.75, ESMC + "
+                "screened-Poisson FEM, dt=0.01")
+    else:
+        desc = ("interface: periodic 400 sigma, 4096 cells, 20M particles, "
+                "n=0.55/0.02, T=0.7, ESMC + screened-Poisson, dt=0.02")
 
     out = {
-        "metric": "particle-steps/s on 1 MI355X (Enskog-Vlasov interface step)",
+        "metric": "particle-steps/s on 1 MI355X (Enskog-Vlasov "
+                  + ("plane step)" if plane else "interface step)"),
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,
@@ -164,8 +187,7 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 particles / fp64 field and moments",
         "data": "synthetic (numpy Philox initial state)",
-        "config": {"workload": "interface: periodic 400 sigma, 4096 cells, 20M particles, "
-                               "n=0.55/0.02, T=0.7, ESMC + screened-Poisson, dt=0.02",
+        "config": {"workload": desc,
                    "particles": N, "cells": args.cells, "parallelism": "1 GPU",
                    "pressure_sampling": bool(args.pressure),
                    "valid_ops_per_step": valid, "accepted_per_step": acc,

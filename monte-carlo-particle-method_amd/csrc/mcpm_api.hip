@@ -5,6 +5,7 @@
 #include <stdio.h>
 #include <string.h>
 #include <algorithm>
+#include <cmath>
 #include <utility>
 #include <vector>
 
@@ -36,6 +37,12 @@ struct mcpm_ctx {
   int64_t nchunks_cap = 0;
   int seg = 32;
   uint32_t *table = nullptr, *partial = nullptr;
+  // more cells than one histogram holds: two digits of the cell key,
+  // low = key % kdiv (nb_lo bins), high = key / kdiv (nb_hi bins)
+  bool two_digit = false;
+  int32_t kdiv = 0;
+  int nb_lo = 0, nb_hi = 0;
+  uint32_t *dcnt = nullptr, *dstart = nullptr, *keys = nullptr;
   // cells
   uint32_t *cnt = nullptr, *start = nullptr, *mcand = nullptr, *cstart = nullptr;
   double *nden = nullptr, *yc = nullptr, *omax = nullptr, *omega_new = nullptr, *F2 = nullptr;
@@ -72,6 +79,7 @@ struct mcpm_ctx {
   // the plane: nodal density, nodal field, CG work arrays, quadrature tables
   FemParams fem{};
   uint32_t *q4 = nullptr, *fem_iters = nullptr;
+  double *cg_part = nullptr, *cg_st = nullptr;  // multi-workgroup CG partials and scalars
   double *fem_buf = nullptr;  // n_node, pot, r, p, q, then the two tables
   double *n_node = nullptr, *pot = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr;
   double *tab_sp = nullptr, *tab_vl = nullptr;
@@ -338,7 +346,7 @@ int build_fem(mcpm_ctx *c) {
   f.maxit = 20000;
   f.nq = 0;
   const size_t nn = (size_t)(f.nx + 1) * (f.ny + 1);
-  int st = dalloc(&c->fem_buf, nn * 5 + 2 * QT_N * 128);
+  int st = dalloc(&c->fem_buf, nn * 5 + 2 * QT_N * 128 + 2 * CG_GRID + 4);
   if (st) return st;
   c->n_node = c->fem_buf;
   c->pot = c->n_node + nn;
@@ -347,6 +355,8 @@ int build_fem(mcpm_ctx *c) {
   c->cg_q = c->cg_p + nn;
   c->tab_sp = c->cg_q + nn;
   c->tab_vl = c->tab_sp + QT_N * 128;
+  c->cg_part = c->tab_vl + QT_N * 128;
+  c->cg_st = c->cg_part + 2 * CG_GRID;
   if ((st = dalloc(&c->q4, (size_t)c->nc * 4 + 2))) return st;
   c->fem_iters = c->q4 + (size_t)c->nc * 4;
   HIPCHK(hipMemset(c->fem_buf, 0, nn * 5 * sizeof(double)));
@@ -414,32 +424,60 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   if (nch > c->nchunks_cap) return MCPM_ERR_CAPACITY;
   const int nseg = (int)((nch + c->seg - 1) / c->seg);
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
-  const R *in = (const R *)c->P[c->cur];
-  R *out = (R *)c->P[1 - c->cur];
   const Kick kick{c->F2, c->pot, c->kick_coef, nullptr};
   // the face sums are taken once, in the scatter pass, of a moving push
   Kick kick_scatter = kick;
   if (c->pressure_on && (mode & PUSH_MOVE)) kick_scatter.pm = c->press;
-  const size_t lds = (size_t)nc * sizeof(uint32_t);
   const double *drift = (c->p.model == MCPM_MODEL_FP_DENSE && c->have_closures &&
                          (mode & PUSH_MOVE))
                             ? c->drift
                             : nullptr;
-  hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
-                     nullptr, c->n, chunk, c->kp, kick, drift, mode, slo, shi, c->table,
-                     nullptr, c->err);
-  LAUNCHCHK();
-  dim3 gs((nc + 255) / 256, nseg);
-  hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
-                     c->seg);
-  hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, c->s, c->partial, nseg, nc, c->cnt,
-                     c->start, c->nden, c->yc, c->kp);
-  hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nc,
-                     c->seg);
-  hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
-                     c->n, chunk, c->kp, kick_scatter, drift, mode, slo, shi, c->table, c->start,
-                     c->err);
-  LAUNCHCHK();
+  // one stable counting-sort pass over nb bins: histogram, scans, scatter
+  auto pass = [&](const R *in, R *out, int pmode, const Kick &ks, int nb, int kmode,
+                  uint32_t *cnt, uint32_t *start, double *nden, double *yc,
+                  uint32_t *okey) -> int {
+    const size_t lds = (size_t)nb * sizeof(uint32_t);
+    hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
+                       nullptr, c->n, chunk, c->kp, kick, drift, pmode, slo, shi, c->table,
+                       nullptr, c->err, nb, kmode, c->kdiv, nullptr);
+    LAUNCHCHK();
+    dim3 gs((nb + 255) / 256, nseg);
+    hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
+                       c->seg);
+    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, c->s, c->partial, nseg, nb, cnt,
+                       start, nden, yc, c->kp);
+    hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
+                       c->seg);
+    hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
+                       c->n, chunk, c->kp, ks, drift, pmode, slo, shi, c->table, start,
+                       c->err, nb, kmode, c->kdiv, okey);
+    LAUNCHCHK();
+    return MCPM_OK;
+  };
+  const R *in = (const R *)c->P[c->cur];
+  R *out = (R *)c->P[1 - c->cur];
+  int st;
+  if (!c->two_digit) {
+    if ((st = pass(in, out, mode, kick_scatter, nc, 0, c->cnt, c->start, c->nden, c->yc,
+                   nullptr)))
+      return st;
+  } else {
+    // low digit with the push, then the high digit of the pushed records
+    // back into the first buffer; cells from the sorted keys
+    if ((st = pass(in, out, mode, kick_scatter, c->nb_lo, 1, c->dcnt, c->dstart, nullptr,
+                   nullptr, nullptr)))
+      return st;
+    if ((st = pass(out, (R *)in, 0, kick, c->nb_hi, 2, c->dcnt, c->dstart, nullptr, nullptr,
+                   c->keys)))
+      return st;
+    hipLaunchKernelGGL(k_cell_bounds, dim3(2048), dim3(256), 0, c->s, c->keys, c->n, nc,
+                       c->start);
+    hipLaunchKernelGGL(k_cell_fill, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->start, nc,
+                       c->cnt, c->nden, c->yc, c->kp);
+    LAUNCHCHK();
+    c->sorted = true;
+    return MCPM_OK;
+  }
   c->cur = 1 - c->cur;
   c->sorted = true;
   return MCPM_OK;
@@ -678,6 +716,40 @@ int launch_nodes(mcpm_ctx *c) {
   }
 }
 
+// The FEM solve of a large plane over CG_GRID workgroups.  Iterations are
+// launched in batches; between batches the host reads the convergence flag.
+constexpr int CG_ONE_WG_NODES = 129 * 129;
+
+int fem_cg_grid(mcpm_ctx *c, const FemParams &f) {
+  const dim3 g(CG_GRID), b(256);
+  hipLaunchKernelGGL(k_cgm_zero, g, b, 0, c->s, c->pot, f);
+  hipLaunchKernelGGL(k_cgm_start, g, b, 0, c->s, c->n_node, c->pot, c->cg_r, c->cg_p, f,
+                     c->cg_part);
+  hipLaunchKernelGGL(k_cgm_start2, dim3(1), b, 0, c->s, c->cg_st, c->cg_part, CG_GRID, f,
+                     c->fem_iters);
+  LAUNCHCHK();
+  int k = 0;
+  uint32_t flag[2] = {0u, 0u};
+  while (k < f.maxit) {
+    const int batch = std::min(32, f.maxit - k);
+    for (int t = 0; t < batch; ++t, ++k) {
+      hipLaunchKernelGGL(k_cgm_aq, g, b, 0, c->s, c->cg_p, c->cg_q, f, c->cg_part,
+                         c->fem_iters);
+      hipLaunchKernelGGL(k_cgm_xr, g, b, 0, c->s, c->pot, c->cg_r, c->cg_p, c->cg_q, f,
+                         c->cg_part, c->cg_st, k, c->fem_iters);
+      hipLaunchKernelGGL(k_cgm_p, g, b, 0, c->s, c->cg_p, c->cg_r, f, c->cg_part, c->cg_st, k,
+                         c->fem_iters);
+    }
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(flag, c->fem_iters, sizeof flag, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (flag[1]) break;
+  }
+  hipLaunchKernelGGL(k_cgm_seams, g, b, 0, c->s, c->pot, f);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
 // compute_SP_2D (FEM.cpp:249-610) up to the nodal field; the kick is the push's.
 int sp_2d(mcpm_ctx *c) {
   int st = MCPM_DISPATCH(c, launch_nodes, c);
@@ -690,8 +762,13 @@ int sp_2d(mcpm_ctx *c) {
     hipLaunchKernelGGL(k_node_integral, dim3(nb), dim3(256), 0, c->s, c->nden, c->tab_sp, c->pot,
                        c->kp, f, 1);
   }
-  hipLaunchKernelGGL(k_fem_cg, dim3(1), dim3(1024), 0, c->s, c->n_node, c->pot, c->cg_r,
-                     c->cg_p, c->cg_q, f, c->fem_iters);
+  if (nn <= CG_ONE_WG_NODES) {
+    hipLaunchKernelGGL(k_fem_cg, dim3(1), dim3(1024), 0, c->s, c->n_node, c->pot, c->cg_r,
+                       c->cg_p, c->cg_q, f, c->fem_iters);
+  } else {
+    int st2 = fem_cg_grid(c, f);
+    if (st2) return st2;
+  }
   hipLaunchKernelGGL(k_fem_core, dim3((nn + 255) / 256), dim3(256), 0, c->s, c->n_node, c->pot,
                      f);
   LAUNCHCHK();
@@ -739,7 +816,7 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   const mcpm_params *p = &pv;
   if (p->dims != 1 && p->dims != 2) return MCPM_ERR_ARG;
   const int64_t nx = p->dims == 2 ? p->n_cells_x : 1;
-  if (p->n_cells < 1 || nx < 1 || nx * p->n_cells > 16384) return MCPM_ERR_ARG;
+  if (p->n_cells < 1 || nx < 1 || nx * p->n_cells > MAX_CELLS) return MCPM_ERR_ARG;
   if (p->dims == 2) {
     // the plane: specular walls or periodic axes, hard-sphere or DSMC
     // collisions, direct or screened-Poisson attraction (FEM.cpp)
@@ -817,14 +894,27 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   if ((st = dalloc((char **)&c->P[1], c->rec * (size_t)p->capacity))) return fail(st);
   // chunks of at least 1024 particles; the per-chunk histogram table is kept
   // under 512 MB (chunk_for's automatic size makes about 2048 chunks)
+  // more cells than one LDS histogram holds: sort by two balanced digits
+  int nbins = nc;
+  if (nc > SORT_BINS) {
+    c->two_digit = true;
+    int32_t d = (int32_t)std::ceil(std::sqrt((double)nc));
+    c->kdiv = d;
+    c->nb_lo = d;
+    c->nb_hi = (nc + d - 1) / d;
+    nbins = std::max(c->nb_lo, c->nb_hi);
+    if ((st = dalloc(&c->dcnt, nbins))) return fail(st);
+    if ((st = dalloc(&c->dstart, nbins + 1))) return fail(st);
+    if ((st = dalloc(&c->keys, (size_t)p->capacity))) return fail(st);
+  }
   c->nchunks_cap = (p->capacity + 1023) / 1024;
   {
-    const int64_t tab_cap = ((int64_t)512 << 20) / (4 * (int64_t)c->nc);
+    const int64_t tab_cap = ((int64_t)512 << 20) / (4 * (int64_t)nbins);
     if (c->nchunks_cap > tab_cap) c->nchunks_cap = tab_cap;
     if (c->nchunks_cap < 2100) c->nchunks_cap = 2100;
   }
-  if ((st = dalloc(&c->table, (size_t)c->nchunks_cap * nc))) return fail(st);
-  if ((st = dalloc(&c->partial, (size_t)((c->nchunks_cap + c->seg - 1) / c->seg + 1) * nc)))
+  if ((st = dalloc(&c->table, (size_t)c->nchunks_cap * nbins))) return fail(st);
+  if ((st = dalloc(&c->partial, (size_t)((c->nchunks_cap + c->seg - 1) / c->seg + 1) * nbins)))
     return fail(st);
   if ((st = dalloc(&c->cnt, nc))) return fail(st);
   if ((st = dalloc(&c->start, nc + 1))) return fail(st);
@@ -873,7 +963,8 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
 int mcpm_destroy(mcpm_ctx *c) {
   if (!c) return MCPM_OK;
   if (c->s) hipStreamSynchronize(c->s);
-  void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->cnt, c->start, c->mcand,
+  void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->dcnt, c->dstart, c->keys,
+                  c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
                   c->samples, c->press, c->counters, c->err, c->res, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,

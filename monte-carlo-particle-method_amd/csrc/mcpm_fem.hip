@@ -7,6 +7,7 @@
 //                    (vlasov_taking_the_integral_2D, FEM.cpp:157-195)
 //   k_fem_cg         Q1 stiffness + lambda^2 V mass system (compute_SP_2D,
 //                    FEM.cpp:249-458) by conjugate gradients, one workgroup
+//   k_cgm_*          the same solve over a grid of workgroups (large planes)
 //   k_fem_core       core subtraction with its 5-point correction (FEM.cpp:497-554)
 // The kick from the nodal field is part of the push (mcpm_kernels.hip).
 #include "mcpm_geom.h"
@@ -290,6 +291,152 @@ __global__ __launch_bounds__(1024) void k_fem_cg(const double *__restrict__ n_no
     }
   }
   if (threadIdx.x == 0) *iters = (uint32_t)it;
+}
+
+// The same conjugate gradients over a grid of CG_GRID workgroups for large
+// planes: three launches per iteration (q = A p with partial p.q; x and r
+// with partial r.r; the new direction), every workgroup reducing the
+// partials of the previous launch in one fixed order, so all of them see the
+// same scalars and the solve is reproducible.  Iteration k reads rr from
+// st[k & 1] and writes the next one to st[(k + 1) & 1]; flag[0] counts
+// iterations, flag[1] = 1 once converged (later launches return at once).
+namespace {
+
+__device__ double grid_partials_sum(const double *part, int G, double *sh) {
+  double v = 0.0;
+  for (int b = threadIdx.x; b < G; b += blockDim.x) v += part[b];
+  return wg_sum(v, sh);
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_cgm_zero(double *__restrict__ pot, FemParams f) {
+  const int w = f.nx + 1, nn = w * (f.ny + 1);
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x)
+    if (fem_unknown(id % w, id / w, f)) pot[id] = 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_cgm_start(const double *__restrict__ n_node,
+                                                   const double *__restrict__ pot,
+                                                   double *__restrict__ r,
+                                                   double *__restrict__ p, FemParams f,
+                                                   double *__restrict__ part) {
+  __shared__ double sh[16];
+  const int w = f.nx + 1, nn = w * (f.ny + 1);
+  const Stencil A{8.0 / 3.0 + 4.0 * f.lam2V / 9.0, -1.0 / 3.0 + f.lam2V / 9.0,
+                  -1.0 / 3.0 + f.lam2V / 36.0};
+  const Stencil M{4.0 / 9.0 * f.volume, 1.0 / 9.0 * f.volume, 1.0 / 36.0 * f.volume};
+  double rr = 0.0, bb = 0.0;
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+    const int i = id % w, j = id / w;
+    double rv = 0.0, bv = 0.0;
+    if (fem_unknown(i, j, f)) {
+      bv = apply9(n_node, i, j, f, M);
+      rv = bv - apply9(pot, i, j, f, A);
+    }
+    r[id] = rv;
+    p[id] = rv;
+    rr += rv * rv;
+    bb += bv * bv;
+  }
+  rr = wg_sum(rr, sh);
+  bb = wg_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rr;
+    part[gridDim.x + blockIdx.x] = bb;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cgm_start2(double *__restrict__ st,
+                                                    const double *__restrict__ part, int G,
+                                                    FemParams f, uint32_t *__restrict__ flag) {
+  __shared__ double sh[16];
+  const double rr = grid_partials_sum(part, G, sh);
+  const double bb = grid_partials_sum(part + G, G, sh);
+  if (threadIdx.x == 0) {
+    st[0] = rr;
+    st[2] = f.tol2 * bb;
+    flag[0] = 0u;
+    flag[1] = (rr > f.tol2 * bb && f.maxit > 0) ? 0u : 1u;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_cgm_aq(const double *__restrict__ p,
+                                                double *__restrict__ q, FemParams f,
+                                                double *__restrict__ part,
+                                                const uint32_t *__restrict__ flag) {
+  __shared__ double sh[16];
+  if (flag[1]) return;
+  const int w = f.nx + 1, nn = w * (f.ny + 1);
+  const Stencil A{8.0 / 3.0 + 4.0 * f.lam2V / 9.0, -1.0 / 3.0 + f.lam2V / 9.0,
+                  -1.0 / 3.0 + f.lam2V / 36.0};
+  double pq = 0.0;
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+    const int i = id % w, j = id / w;
+    double qv = 0.0;
+    if (fem_unknown(i, j, f)) qv = apply9(p, i, j, f, A);
+    q[id] = qv;
+    pq += p[id] * qv;
+  }
+  pq = wg_sum(pq, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = pq;
+}
+
+__global__ __launch_bounds__(256) void k_cgm_xr(double *__restrict__ pot, double *__restrict__ r,
+                                                const double *__restrict__ p,
+                                                const double *__restrict__ q, FemParams f,
+                                                double *__restrict__ part,
+                                                const double *__restrict__ st, int k,
+                                                const uint32_t *__restrict__ flag) {
+  __shared__ double sh[16];
+  if (flag[1]) return;
+  const int G = gridDim.x;
+  const double alpha = st[k & 1] / grid_partials_sum(part, G, sh);
+  const int w = f.nx + 1, nn = w * (f.ny + 1);
+  double rn = 0.0;
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+    if (fem_unknown(id % w, id / w, f)) {
+      pot[id] += alpha * p[id];
+      const double rv = r[id] - alpha * q[id];
+      r[id] = rv;
+      rn += rv * rv;
+    }
+  }
+  rn = wg_sum(rn, sh);
+  if (threadIdx.x == 0) part[G + blockIdx.x] = rn;
+}
+
+__global__ __launch_bounds__(256) void k_cgm_p(double *__restrict__ p,
+                                               const double *__restrict__ r, FemParams f,
+                                               const double *__restrict__ part,
+                                               double *__restrict__ st, int k,
+                                               uint32_t *__restrict__ flag) {
+  __shared__ double sh[16];
+  if (flag[1]) return;
+  const int G = gridDim.x;
+  const double rn = grid_partials_sum(part + G, G, sh);
+  const double beta = rn / st[k & 1];
+  const int w = f.nx + 1, nn = w * (f.ny + 1);
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x)
+    p[id] = r[id] + beta * p[id];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st[(k + 1) & 1] = rn;
+    flag[0] = (uint32_t)(k + 1);
+    if (!(rn > st[2]) || k + 1 >= f.maxit) flag[1] = 1u;
+  }
+}
+
+// Periodic seams after the solve: node nx repeats node 0, row ny row 0.
+__global__ __launch_bounds__(256) void k_cgm_seams(double *__restrict__ pot, FemParams f) {
+  const int w = f.nx + 1, nn = w * (f.ny + 1);
+  for (int id = blockIdx.x * blockDim.x + threadIdx.x; id < nn; id += gridDim.x * blockDim.x) {
+    const int i = id % w, j = id / w;
+    if ((f.per_x && i == f.nx) || (f.per_y && j == f.ny)) {
+      const int i0 = (f.per_x && i == f.nx) ? 0 : i;
+      const int j0 = (f.per_y && j == f.ny) ? 0 : j;
+      pot[id] = pot[j0 * w + i0];
+    }
+  }
 }
 
 // Core subtraction x -= n dummy + (Laplacian-like sum) dummy2

@@ -18,7 +18,7 @@ enum : int {
   CNT_LOG = 4,     // entries in the debug collision log
   NREG = 8,        // operation regions: contiguous candidate ranges, one per XCD
   CNT_LIST = 8,    // CNT_LIST + r NREG + x: losers of round r - 1 in region x (r >= 1)
-  MAX_ROUNDS = 512,
+  MAX_ROUNDS = 4096,
   MAX_BANDS = 16,
   CNT_BAND = CNT_LIST + (MAX_ROUNDS + 2) * NREG,  // [b NREG + x]: operations of band b, region x
   CNT_BSTART = CNT_BAND + MAX_BANDS * NREG,       // start of list [b NREG + x] in the band lists
@@ -58,7 +58,15 @@ template <typename R, bool SCATTER>
 __global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp, Kick kick,
                        const double *drift, int mode,
                        uint32_t step_lo, uint32_t step_hi, uint32_t *table,
-                       const uint32_t *start, uint32_t *err);
+                       const uint32_t *start, uint32_t *err, int nb, int kmode, int32_t kdiv,
+                       uint32_t *okey);
+// Most bins of one sort pass (one LDS histogram of 64 KB per chunk); more
+// cells sort by two digits of the cell key, up to MAX_CELLS.
+constexpr int SORT_BINS = 16384;
+constexpr int64_t MAX_CELLS = (int64_t)1 << 22;
+__global__ void k_cell_bounds(const uint32_t *key, int64_t n, int nc, uint32_t *start);
+__global__ void k_cell_fill(const uint32_t *start, int nc, uint32_t *cnt, double *nden,
+                            double *yc, KParams kp);
 __global__ void k_scan_a(const uint32_t *table, uint32_t *partial, int nchunks, int nc, int seg);
 __global__ void k_scan_b(uint32_t *partial, int nseg, int nc, uint32_t *cnt, uint32_t *start,
                          double *nden, double *yc, KParams kp);
@@ -154,6 +162,21 @@ __global__ void k_quadrants(const R *P, const uint32_t *cnt, const uint32_t *sta
 __global__ void k_nodes(const uint32_t *q4, double *n_node, FemParams f);
 __global__ void k_node_integral(const double *nden, const double *tab, double *pot, KParams kp,
                                 FemParams f, int boundary_only);
+// Multi-workgroup CG (large planes): CG_GRID workgroups of 256 threads;
+// part holds 2 CG_GRID partial sums, st three scalars, flag two words.
+constexpr int CG_GRID = 512;
+__global__ void k_cgm_zero(double *pot, FemParams f);
+__global__ void k_cgm_start(const double *n_node, const double *pot, double *r, double *p,
+                            FemParams f, double *part);
+__global__ void k_cgm_start2(double *st, const double *part, int G, FemParams f,
+                             uint32_t *flag);
+__global__ void k_cgm_aq(const double *p, double *q, FemParams f, double *part,
+                         const uint32_t *flag);
+__global__ void k_cgm_xr(double *pot, double *r, const double *p, const double *q, FemParams f,
+                         double *part, const double *st, int k, const uint32_t *flag);
+__global__ void k_cgm_p(double *p, const double *r, FemParams f, const double *part, double *st,
+                        int k, uint32_t *flag);
+__global__ void k_cgm_seams(double *pot, FemParams f);
 __global__ void k_fem_cg(const double *n_node, double *pot, double *r, double *p, double *q,
                          FemParams f, uint32_t *iters);
 __global__ void k_fem_core(const double *n_node, double *pot, FemParams f);

@@ -241,7 +241,16 @@ __device__ __forceinline__ bool in_domain(const Rec2<real> &r, const KParams &kp
 //                  rank taken among equal keys in the wavefront with ballots.
 // Particles of a chunk are processed in index order, so equal keys keep their
 // relative order: the sort is stable.
+// The bins are the cells (kmode 0) or, for more cells than one LDS histogram
+// holds, the low (kmode 1: key % kdiv) then the high digit (kmode 2:
+// key / kdiv) of the cell key in two stable passes (least significant digit
+// first), which together sort by the key.  okey, when given, receives each
+// particle's cell key at its destination.
 // ---------------------------------------------------------------------------
+__device__ __forceinline__ int32_t sort_bin(int32_t key, int kmode, int32_t kdiv) {
+  return kmode == 0 ? key : (kmode == 1 ? key % kdiv : key / kdiv);
+}
+
 template <typename R, bool SCATTER>
 __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__restrict__ out,
                                              int64_t n, int32_t chunk, KParams kp, Kick kick,
@@ -249,9 +258,10 @@ __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__rest
                                              uint32_t step_lo, uint32_t step_hi,
                                              uint32_t *__restrict__ table,
                                              const uint32_t *__restrict__ start,
-                                             uint32_t *err) {
+                                             uint32_t *err, int nb, int kmode, int32_t kdiv,
+                                             uint32_t *__restrict__ okey) {
   extern __shared__ uint32_t lds_cnt[];
-  const int nc = kp.n_cells;
+  const int nc = nb;
   const int lane = threadIdx.x;
   const int64_t b = blockIdx.x;
   const int64_t beg = b * (int64_t)chunk;
@@ -277,13 +287,14 @@ __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__rest
     for (int u = 0; u < U; ++u) {
       int64_t p = t0 + u * 64 + lane;
       bool valid = p < end;
-      int32_t key = 0;
+      int32_t key = 0, ckey = 0;
       R r = rr[u];
       if (valid) {
         if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
         if ((mode & PUSH_CHECK) && !in_domain(r, kp)) atomicOr(err, ERRBIT_DOMAIN);
         r = push_one(r, (uint64_t)p, kp, kick, drift, mode, step_lo, step_hi, err);
-        key = cell_key(r, kp);
+        ckey = cell_key(r, kp);
+        key = sort_bin(ckey, kmode, kdiv);
       }
       uint64_t active = __ballot(valid);
       while (active) {
@@ -295,6 +306,7 @@ __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__rest
           if (mine) {
             uint32_t dst = lds_cnt[lkey] + (uint32_t)__popcll(m & lt);
             out[dst] = r;
+            if (okey) okey[dst] = (uint32_t)ckey;
           }
           __builtin_amdgcn_wave_barrier();
           if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
@@ -347,9 +359,11 @@ __global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ partial,
     }
     cnt[c] = run;
     tot += run;
-    double nd = kp.n_ratio * kp.Fn * (double)run / kp.volume;  // Main_Functions.cpp:2601
-    nden[c] = nd;
-    yc[c] = contact_y(nd, kp.b, kp.y_model);
+    if (nden) {
+      double nd = kp.n_ratio * kp.Fn * (double)run / kp.volume;  // Main_Functions.cpp:2601
+      nden[c] = nd;
+      yc[c] = contact_y(nd, kp.b, kp.y_model);
+    }
   }
   sh[tid] = tot;
   __syncthreads();
@@ -367,6 +381,31 @@ __global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ partial,
     run += cnt[c];
   }
   if (tid == 1023) start[nc] = sh[1023];
+}
+
+// Cells of the two-digit sort: start[c] = first particle of cell c from the
+// sorted keys (every c in [0, nc] written once; keys ascend), then counts,
+// densities and contact values as k_scan_b.
+__global__ void k_cell_bounds(const uint32_t *__restrict__ key, int64_t n, int nc,
+                              uint32_t *__restrict__ start) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t prev = i > 0 ? (int64_t)key[i - 1] : -1;
+    const int64_t cur = i < n ? (int64_t)key[i] : (int64_t)nc;
+    for (int64_t c = prev + 1; c <= cur; ++c) start[c] = (uint32_t)i;
+  }
+}
+
+__global__ void k_cell_fill(const uint32_t *__restrict__ start, int nc,
+                            uint32_t *__restrict__ cnt, double *__restrict__ nden,
+                            double *__restrict__ yc, KParams kp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  const uint32_t run = start[c + 1] - start[c];
+  cnt[c] = run;
+  double nd = kp.n_ratio * kp.Fn * (double)run / kp.volume;  // Main_Functions.cpp:2601
+  nden[c] = nd;
+  yc[c] = contact_y(nd, kp.b, kp.y_model);
 }
 
 __global__ void k_scan_c(uint32_t *__restrict__ table, const uint32_t *__restrict__ partial,
@@ -1343,10 +1382,12 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
 #define MCPM_INST(R)                                                                       \
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
                                             Kick, const double *, int, uint32_t, uint32_t, \
-                                            uint32_t *, const uint32_t *, uint32_t *);     \
+                                            uint32_t *, const uint32_t *, uint32_t *, int, \
+                                            int, int32_t, uint32_t *);                     \
   template __global__ void k_push<R, true>(const R *, R *, int64_t, int32_t, KParams,      \
                                            Kick, const double *, int, uint32_t, uint32_t,  \
-                                           uint32_t *, const uint32_t *, uint32_t *);      \
+                                           uint32_t *, const uint32_t *, uint32_t *, int,  \
+                                           int, int32_t, uint32_t *);                      \
   MCPM_INST_COLL(R, COLL_ESMC)                                                             \
   MCPM_INST_COLL(R, COLL_DSMC)                                                             \
   template __global__ void k_cba_hs<R>(R *, const uint32_t *, const uint32_t *,            \

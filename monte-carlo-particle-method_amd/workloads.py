@@ -147,11 +147,29 @@ def droplets(nx=512, ny=512, Lx=128.0, Ly=128.0, N=64_000_000, n_drops=20, r_lo=
              n_liq=0.55, n_vap=0.02, T=0.75, dt=0.01, precision=FP32, seed=1,
              long_range=LR_SCREENED_POISSON, capacity=None):
     """Periodic plane with random liquid discs: n_drops discs, radii uniform in
-    [r_lo, r_hi], at n_liq in vapour n_vap, about N particles in all."""
+    [r_lo, r_hi], at n_liq in vapour n_vap, about N particles in all.  Discs
+    are placed one by one at random centres that keep a gap of at least one
+    sigma to every disc placed before (periodic distance), so no region holds
+    the liquid density twice."""
     rng = np.random.Generator(np.random.Philox(seed + 1000))
-    drops = [(rng.random() * Lx, rng.random() * Ly, r_lo + (r_hi - r_lo) * rng.random())
-             for _ in range(n_drops)]
-    # liquid fraction of the particles from the disc areas (overlaps ignored)
+    drops = []
+    tries = 0
+    while len(drops) < n_drops:
+        tries += 1
+        if tries > 100_000:
+            raise ValueError("cannot place the discs without overlap")
+        cx, cy, r = rng.random() * Lx, rng.random() * Ly, r_lo + (r_hi - r_lo) * rng.random()
+        ok = True
+        for (qx, qy, qr) in drops:
+            dx = abs(cx - qx)
+            dy = abs(cy - qy)
+            dx, dy = min(dx, Lx - dx), min(dy, Ly - dy)
+            if math.hypot(dx, dy) < r + qr + 1.0:
+                ok = False
+                break
+        if ok:
+            drops.append((cx, cy, r))
+    # liquid fraction of the particles from the disc areas
     a_liq = sum(math.pi * r * r for (_, _, r) in drops)
     mass = n_vap * Lx * Ly + (n_liq - n_vap) * a_liq
     ppc = max(1, int(round(N * (n_vap * Lx * Ly / mass) / (nx * ny))))
