@@ -56,6 +56,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <random>
 #include <vector>
 
 #include "mcpm.h"
@@ -541,6 +542,83 @@ void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
   if (st) {
     st->candidates = (int64_t)M;
     st->valid = (int64_t)order.size();
+    st->accepted = acc;
+    st->rounds = 0;
+    st->reserved0 = 0;
+  }
+}
+
+// Collision_ESMC_new3 as the reference runs it (collisions.cpp:583-720), for
+// ensemble comparisons with the canonical order: one std::mt19937 and the
+// standard library's distributions; candidate counts floor(0.5 omega_max N_c +
+// U); each candidate's cell drawn by z = uniform_int(0, total) against the
+// running quota sums, which are then decremented from that cell on (the
+// reference's own weighting, off-by-one included); i uniform in the cell,
+// theta = 2 pi u, phi = acos(2 v - 1), dir = (cos theta sin phi,
+// sin theta sin phi, cos phi); the partner cell and the rest as the canonical
+// path.  The generator is seeded from (seed, step) so runs are repeatable.
+template <typename real>
+void orc_collide_mt(Oracle &o, int64_t step, mcpm_collision_stats *st) {
+  const mcpm_params &p = o.p;
+  const int nc = o.nc;
+  std::seed_seq sq{(uint32_t)p.seed, (uint32_t)((uint64_t)p.seed >> 32), (uint32_t)step,
+                   (uint32_t)((uint64_t)step >> 32)};
+  std::mt19937 gen(sq);
+  std::uniform_real_distribution<> dis_r(0.0, 1.0), dis_u(0.0, 1.0), dis_v(0.0, 1.0);
+  std::vector<int> sum_M(nc);
+  std::vector<double> omega(nc, 0.0);
+  int M_cand = 0;
+  for (int c = 0; c < nc; ++c) {
+    int mtemp = (int)floor(0.5 * o.omax[c] * (1.0 * o.cnt[c]) + dis_r(gen));
+    sum_M[c] = M_cand + mtemp;
+    M_cand = sum_M[c];
+  }
+  int64_t valid = 0, acc = 0;
+  const double pref = 4.0 * PI * p.sigma * p.sigma;
+  for (int m = 0; m < M_cand; ++m) {
+    if (sum_M[nc - 1] <= 0) continue;
+    std::uniform_int_distribution<> d1(0, sum_M[nc - 1]);
+    const int z = d1(gen);
+    int I = -1;
+    for (int c = 0; c < nc; ++c)
+      if (z <= sum_M[c]) {
+        I = c;
+        break;
+      }
+    if (I < 0) continue;
+    for (int c = I; c < nc; ++c) sum_M[c]--;
+    if (o.cnt[I] <= 0) continue;
+    std::uniform_int_distribution<> d2(0, o.cnt[I] - 1);
+    const int64_t i = o.start[I] + d2(gen);
+    const double u = dis_u(gen), v = dis_v(gen);
+    const double theta = 2.0 * PI * u, phi = acos(2.0 * v - 1.0);
+    const double d0 = cos(theta) * sin(phi), dd1 = sin(theta) * sin(phi), d2v = cos(phi);
+    int J;
+    if (!orc_partner(o, i, d0, dd1, &J) || o.cnt[J] <= 0) continue;
+    std::uniform_int_distribution<> d3(0, o.cnt[J] - 1);
+    const int64_t j = o.start[J] + d3(gen);
+    if (i == j) continue;
+    ++valid;
+    double *ri = &o.rec[(size_t)o.S * i] + o.U - 1, *rj = &o.rec[(size_t)o.S * j] + o.U - 1;
+    const double Ymid = o.yc[orc_mid(o, i, j)];
+    const double g0 = ri[1] - rj[1], g1 = ri[2] - rj[2], g2 = ri[3] - rj[3];
+    const double gd = g0 * d0 + g1 * dd1 + g2 * d2v;
+    const double pv = pref * gd * Ymid * o.nden[J] * p.delta_t;
+    if (pv - omega[I] > 0.0) omega[I] = pv;
+    if (pv / o.omax[I] > dis_r(gen)) {
+      ri[1] = rnd<real>(ri[1] - gd * d0);
+      ri[2] = rnd<real>(ri[2] - gd * dd1);
+      ri[3] = rnd<real>(ri[3] - gd * d2v);
+      rj[1] = rnd<real>(rj[1] + gd * d0);
+      rj[2] = rnd<real>(rj[2] + gd * dd1);
+      rj[3] = rnd<real>(rj[3] + gd * d2v);
+      ++acc;
+    }
+  }
+  for (int c = 0; c < nc; ++c) o.omax[c] = std::max(omega[c], o.omax[c]);
+  if (st) {
+    st->candidates = M_cand;
+    st->valid = valid;
     st->accepted = acc;
     st->rounds = 0;
     st->reserved0 = 0;
@@ -2001,6 +2079,14 @@ int orc_collide_dsmc_hs(void *h, int64_t step, mcpm_collision_stats *st) {
 
 int orc_collide_cba_vhs(void *h, int64_t step, mcpm_collision_stats *st) {
   return orc_dsmc_any(h, step, st, MCPM_MODEL_CBA_VHS);
+}
+
+int orc_collide_esmc_mt(void *h, int64_t step, mcpm_collision_stats *st) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  if (o.p.precision == MCPM_FP32) orc_collide_mt<float>(o, step, st);
+  else orc_collide_mt<double>(o, step, st);
+  return MCPM_OK;
 }
 
 int orc_collide_cba_hs(void *h, int64_t step, mcpm_collision_stats *st) {

@@ -69,6 +69,7 @@ def load() -> ctypes.CDLL:
         "orc_collide_dsmc_hs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_collide_cba_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_collide_cba_hs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "orc_collide_esmc_mt": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_vlasov_integral": (i32, [vp]),
         "orc_density_expansion": (i32, [vp]),
         "orc_get_kde": (i32, [vp, dpp, dpp, dpp]),
@@ -246,6 +247,11 @@ class OracleSim:
 
     def collide_cba_hs(self, step):
         return self._collide("orc_collide_cba_hs", step)
+
+    def collide_esmc_mt(self, step):
+        """Collision_ESMC_new3 with std::mt19937 and the reference's own candidate
+        order and direction formula (ensemble comparisons only)."""
+        return self._collide("orc_collide_esmc_mt", step)
 
     def init_omega_max(self, n_gas, times_y=True):
         self._check(self._lib.orc_init_omega_max(self._h, float(n_gas), 1 if times_y else 0),
