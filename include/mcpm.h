@@ -329,6 +329,63 @@ int mcpm_get_pressure(mcpm_ctx *ctx, double *p_m, double *p_rep, double *p_tot);
    evaluated on the host with the context's parameters. */
 double mcpm_contact_y(const mcpm_params *p, double n);
 
+/* ---- settings file, initial state and PostProc output ---------------------- */
+/* What readSettingsFile (cpp_headers.h:624; myIO.cpp:1756-3016) derives from a
+   reference settings file, for the problems on this path: closed_box (1,
+   slab, specular walls), couette_flow (2, slab, diffuse walls at sigma and
+   L - sigma) and injection (12, plane, specular walls).  params is ready for
+   mcpm_create (seed 1, FP64, the reference's Y(n); set them afterwards). */
+typedef struct mcpm_settings {
+  mcpm_params params;
+  int64_t  n_particles;      /* gas.N after the problem's set-up (injection adds Nh) */
+  int64_t  n_drop;           /* injection: gas.Nh, particles of the liquid disc */
+  int32_t  problem;          /* box.problem */
+  int32_t  num_steps;        /* box.num_steps ("steps") */
+  int32_t  every;            /* box.every: PostProc cadence */
+  int32_t  after;            /* box.after: sampling starts after step times * after */
+  int32_t  reset_every;      /* box.reset_every */
+  int32_t  num_realizations; /* box.num_realizations */
+  int32_t  init_step;        /* box.init_step: first step with collisions */
+  int32_t  direction[3];     /* box.direction: axes that stream */
+  int32_t  num_cells[3];     /* box.N */
+  int32_t  reserved2;
+  double   box_len[3];       /* box.Len after the problem's scaling */
+  double   n;                /* gas.n */
+  double   T;                /* gas.T */
+  double   U0[3];            /* gas.U0: mean of the initial Maxwellian */
+  double   T_init;           /* temperature of the initial Maxwellian (Main_Functions.cpp:4791-4792) */
+  double   drop_dia;         /* injection: box.drop_dia (scaled by sigma) */
+  double   epsilon;          /* gas.epsilon */
+  double   lambda;           /* gas.lambda: mean free path (myIO.cpp:2837-2840) */
+  double   Kn;               /* gas.Kn */
+  char     model[32];        /* gas.model as written in the file */
+  char     output[128];      /* gas.output_name */
+} mcpm_settings;
+
+/* Parse a settings file: "keyword values..." lines, '#' in the first column
+   starts a comment, the rest of a line after its values is ignored; an
+   unknown keyword is MCPM_ERR_ARG (the reference aborts).  MCPM_ERR_ARG also
+   for problems and models this library does not run. */
+int mcpm_read_settings(const char *path, mcpm_settings *out);
+
+/* initialization (cpp_headers.h:566; Main_Functions.cpp:4222-4815) for the
+   problems above, drawn on the device with Philox (stream MCPM_TAG_INIT,
+   particle index i): positions uniform along the streaming axes (the others
+   at L/2); injection: the first N - Nh uniform in the plane, the last Nh in
+   the disc of diameter drop_dia at the centre (r = R sqrt(u), theta = 2 pi
+   u'); velocities Maxwellian about U0 at T_init.  Replaces the particles of
+   the context (n_particles of them), then bins them. */
+int mcpm_initialize(mcpm_ctx *ctx, const mcpm_settings *s);
+
+/* write_post_processing's PostProc file (cpp_headers.h:399; myIO.cpp:597-897): dir/<output>_
+   <direct|denexp|screened-Poisson|noatt>_<step>.dat, the reference's header
+   line and 118 columns per cell, one block per x column.  Columns this path
+   computes: cell geometry, t_avg (dt times the samples taken since the last
+   reset), dt, U_space, T, n, the cell_poly_velocity sums, volume, n_smooth,
+   N, ptot, Fn, F2, PIJ[3], Q[1], div U, alpha, beta, gamma_st, phi, p_m,
+   p_rep; the others (face sums of the inverted problem, xx, UU*) are 0. */
+int mcpm_write_postproc(mcpm_ctx *ctx, const mcpm_settings *s, int64_t step, const char *dir);
+
 /* ---- diagnostics --------------------------------------------------------- */
 /* Record the outcome of every valid collision candidate of the next
    mcpm_collide_esmc call (for parity checks): priority, i, j, accepted. */

@@ -9,6 +9,7 @@
 #include <utility>
 #include <vector>
 
+#include "mcpm_internal.h"
 #include "mcpm_kernels.h"
 
 using namespace mcpm;
@@ -50,6 +51,7 @@ struct mcpm_ctx {
   // measure_pressure face sums, then the two PostProc pressure columns (3 x nc)
   double *press = nullptr;
   bool pressure_on = false;
+  int64_t n_samples = 0;  // cell_poly_velocity calls since the last reset
   uint32_t *counters = nullptr, *err = nullptr;
   uint32_t *res = nullptr;  // reservations, two interleaved arrays
   // collisions
@@ -792,6 +794,18 @@ double contact_y_host(const mcpm_params *p, double n) { return contact_y(n, p->b
 
 }  // namespace
 
+const mcpm_params *ctx_params(const mcpm_ctx *c) { return &c->p; }
+int64_t ctx_sample_count(const mcpm_ctx *c) { return c->n_samples; }
+
+template <typename R>
+int launch_init(mcpm_ctx *c, const InitParams &ip) {
+  if (ip.n > 0)
+    hipLaunchKernelGGL((k_init<R>), dim3((unsigned)((ip.n + 255) / 256)), dim3(256), 0, c->s,
+                       (R *)c->P[c->cur], ip, c->kp);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
 extern "C" {
 
 const char *mcpm_status_string(int s) {
@@ -1315,7 +1329,9 @@ int mcpm_moments(mcpm_ctx *c, double *out) {
 int mcpm_sample(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
-  return MCPM_DISPATCH(c, launch_sample, c);
+  int st = MCPM_DISPATCH(c, launch_sample, c);
+  if (!st) ++c->n_samples;
+  return st;
 }
 
 int mcpm_get_nodes(mcpm_ctx *c, double *n_node, double *potential) {
@@ -1346,7 +1362,37 @@ int mcpm_reset_samples(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
   HIPCHK(hipMemsetAsync(c->samples, 0, (size_t)c->nc * MCPM_NSAMP * sizeof(double), c->s));
   HIPCHK(hipMemsetAsync(c->press, 0, (size_t)c->nc * sizeof(double), c->s));
+  c->n_samples = 0;
   return MCPM_OK;
+}
+
+int mcpm_initialize(mcpm_ctx *c, const mcpm_settings *s) {
+  if (!c || !s) return MCPM_ERR_ARG;
+  if (s->n_particles < 0 || s->n_particles > c->p.capacity) return MCPM_ERR_CAPACITY;
+  if (s->n_drop < 0 || s->n_drop > s->n_particles) return MCPM_ERR_ARG;
+  const bool disc = s->problem == 12;
+  if (disc != (c->p.dims == 2)) return MCPM_ERR_ARG;
+  InitParams ip{};
+  ip.n = s->n_particles;
+  ip.n_drop = disc ? s->n_drop : 0;
+  ip.disc = disc ? 1 : 0;
+  for (int q = 0; q < 3; ++q) {
+    ip.dirs[q] = s->direction[q];
+    ip.len[q] = s->box_len[q];
+    ip.U0[q] = s->U0[q];
+  }
+  // the context's own geometry along its axes
+  ip.len[1] = c->p.length;
+  if (c->p.dims == 2) ip.len[0] = c->p.length_x;
+  ip.drop_r = s->drop_dia / 2.0;
+  ip.sd = sqrt(c->p.kb * s->T_init / c->p.mass);
+  int st = MCPM_DISPATCH(c, launch_init, c, ip);
+  if (st) return st;
+  c->n = s->n_particles;
+  c->sorted = false;
+  c->have_closures = false;
+  c->kick_pending = false;
+  return mcpm_cell_update(c);
 }
 
 int mcpm_measure_pressure(mcpm_ctx *c, int32_t enable) {

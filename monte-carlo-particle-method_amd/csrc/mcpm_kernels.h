@@ -122,6 +122,17 @@ __global__ void k_moments(const R *P, const uint32_t *cnt, const uint32_t *start
 template <typename R>
 __global__ void k_sample(const R *P, const uint32_t *cnt, const uint32_t *start,
                          double *acc);
+// The initial state of initialization (Main_Functions.cpp:4222-4815).
+struct InitParams {
+  int64_t n, n_drop;        // particles; the last n_drop fill the disc
+  int32_t dirs[3];          // streaming axes (box.direction)
+  int32_t disc;             // injection: background + centred disc
+  double len[3];            // box.Len
+  double drop_r;            // disc radius
+  double U0[3], sd;         // Maxwellian mean and standard deviation
+};
+template <typename R>
+__global__ void k_init(R *P, InitParams ip, KParams kp);
 __global__ void k_pressure(const double *mom, const double *nden, const double *yc,
                            double *p_rep, double *p_tot, KParams kp, double phi);
 __global__ void k_fp_closures(const double *mom, const uint32_t *cnt, const double *nden,

@@ -994,6 +994,53 @@ __global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ o
   omega_new[c] = 0.0;
 }
 
+// initialization (Main_Functions.cpp:4222-4815) for particle i: Philox block
+// (i, TAG_INIT, 0) for the position, (i, TAG_INIT, 1) and (i, TAG_INIT, 2)
+// for two Box-Muller pairs of which three normals make the velocity.
+// Positions uniform along streaming axes, L/2 elsewhere (:4657-4671);
+// injection (:4640-4655): the last n_drop particles at r = R sqrt(u),
+// theta = 2 pi u' about the centre.  Velocities U0 + sd z (:4790-4800).  A
+// position that rounds up to L in the stored type is put at 0.
+template <typename R>
+__global__ void k_init(R *__restrict__ P, InitParams ip, KParams kp) {
+  using real = typename R::real_type;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ip.n) return;
+  const uint32_t lo = (uint32_t)i, hi = (uint32_t)((uint64_t)i >> 32);
+  const mcpm_u32x4 w = mcpm_philox(lo, hi, 0u, mcpm_ctr3(MCPM_TAG_INIT, 0u, 0u), kp.key0, kp.key1);
+  const mcpm_u32x4 w1 = mcpm_philox(lo, hi, 0u, mcpm_ctr3(MCPM_TAG_INIT, 0u, 1u), kp.key0, kp.key1);
+  double x, y;
+  if (ip.disc && i >= ip.n - ip.n_drop) {
+    const double r = ip.drop_r * __builtin_sqrt(mcpm_u01(w.v[0]));
+    double sn, cs;
+    mcpm_sincos2pi(mcpm_u01(w.v[1]), &sn, &cs);
+    x = 0.5 * ip.len[0] + r * cs;
+    y = 0.5 * ip.len[1] + r * sn;
+  } else if (ip.disc) {
+    x = mcpm_u01(w.v[0]) * ip.len[0];
+    y = mcpm_u01(w.v[1]) * ip.len[1];
+  } else {
+    x = ip.dirs[0] ? mcpm_u01(w.v[0]) * ip.len[0] : 0.5 * ip.len[0];
+    y = ip.dirs[1] ? mcpm_u01(w.v[1]) * ip.len[1] : 0.5 * ip.len[1];
+  }
+  double z0, z1, z2, z3;
+  mcpm_normal2(w1.v[0], w1.v[1], &z0, &z1);
+  mcpm_normal2(w1.v[2], w1.v[3], &z2, &z3);
+  R r{};
+  real yr = (real)y;
+  if (!(yr < (real)ip.len[1])) yr = (real)0;
+  r.y = yr;
+  if constexpr (R::dims == 2) {
+    real xr = (real)x;
+    if (!(xr < (real)ip.len[0])) xr = (real)0;
+    r.x = xr;
+  }
+  r.u1 = (real)(ip.U0[0] + ip.sd * z0);
+  r.u2 = (real)(ip.U0[1] + ip.sd * z1);
+  r.u3 = (real)(ip.U0[2] + ip.sd * z2);
+  P[i] = r;
+}
+
 // The pressure columns of PostProc (myIO.cpp:855, 890): n kB T (1 + n b Y(n))
 // and that minus the mean-field 2 b n^2 phi, from the moments' T and the cell
 // density.
@@ -1390,6 +1437,7 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                            int, int32_t, uint32_t *);                      \
   MCPM_INST_COLL(R, COLL_ESMC)                                                             \
   MCPM_INST_COLL(R, COLL_DSMC)                                                             \
+  template __global__ void k_init<R>(R *, InitParams, KParams);                           \
   template __global__ void k_cba_hs<R>(R *, const uint32_t *, const uint32_t *,            \
                                        const double *, const double *, KParams, uint32_t,  \
                                        uint32_t, uint32_t *, uint32_t *);                  \

@@ -163,6 +163,37 @@ class Params:
             setattr(c, f.name, getattr(self, f.name))
         return c
 
+    @classmethod
+    def from_c(cls, c: CParams) -> "Params":
+        return cls(**{f.name: getattr(c, f.name) for f in fields(cls)})
+
+
+class CSettings(ctypes.Structure):
+    """mcpm_settings: what readSettingsFile derives (myIO.cpp:1756-3016)."""
+    _fields_ = [
+        ("params", CParams), ("n_particles", ctypes.c_int64), ("n_drop", ctypes.c_int64),
+        ("problem", ctypes.c_int32), ("num_steps", ctypes.c_int32), ("every", ctypes.c_int32),
+        ("after", ctypes.c_int32), ("reset_every", ctypes.c_int32),
+        ("num_realizations", ctypes.c_int32), ("init_step", ctypes.c_int32),
+        ("direction", ctypes.c_int32 * 3), ("num_cells", ctypes.c_int32 * 3),
+        ("reserved2", ctypes.c_int32), ("box_len", ctypes.c_double * 3),
+        ("n", ctypes.c_double), ("T", ctypes.c_double), ("U0", ctypes.c_double * 3),
+        ("T_init", ctypes.c_double), ("drop_dia", ctypes.c_double), ("epsilon", ctypes.c_double),
+        ("lambda_", ctypes.c_double), ("Kn", ctypes.c_double), ("model", ctypes.c_char * 32),
+        ("output", ctypes.c_char * 128),
+    ]
+
+
+def read_settings(path: str) -> CSettings:
+    """mcpm_read_settings: a reference settings file -> CSettings (params
+    ready for Simulation after Params.from_c)."""
+    lib = load_library()
+    s = CSettings()
+    st = lib.mcpm_read_settings(os.fsencode(path), ctypes.byref(s))
+    if st != 0:
+        raise McpmError(st, "mcpm_read_settings")
+    return s
+
 
 def _u8p(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
@@ -217,6 +248,9 @@ def load_library() -> ctypes.CDLL:
         "mcpm_get_samples": (i32, [vp, dpp]),
         "mcpm_reset_samples": (i32, [vp]),
         "mcpm_measure_pressure": (i32, [vp, i32]),
+        "mcpm_read_settings": (i32, [ctypes.c_char_p, ctypes.POINTER(CSettings)]),
+        "mcpm_initialize": (i32, [vp, ctypes.POINTER(CSettings)]),
+        "mcpm_write_postproc": (i32, [vp, ctypes.POINTER(CSettings), i64, ctypes.c_char_p]),
         "mcpm_get_pressure": (i32, [vp, dpp, dpp, dpp]),
         "mcpm_contact_y": (ctypes.c_double, [ctypes.POINTER(CParams), ctypes.c_double]),
         "mcpm_debug_collision_log": (i32, [vp, i32]),
@@ -430,6 +464,16 @@ class Simulation:
         out = np.empty(self.n_cells * NSAMP)
         self._check(self._lib.mcpm_get_samples(self._h, _dp(out)), "mcpm_get_samples")
         return out.reshape(self.n_cells, NSAMP)
+
+    def initialize(self, settings: "CSettings"):
+        """initialization (Main_Functions.cpp:4222-4815) on the device."""
+        self._check(self._lib.mcpm_initialize(self._h, ctypes.byref(settings)), "mcpm_initialize")
+
+    def write_postproc(self, settings: "CSettings", step: int, directory: str):
+        """write_post_processing's PostProc file (myIO.cpp:597-897)."""
+        self._check(self._lib.mcpm_write_postproc(self._h, ctypes.byref(settings), step,
+                                                  os.fsencode(directory)),
+                    "mcpm_write_postproc")
 
     def measure_pressure(self, enable: bool = True):
         """measure_pressure (Main_Functions.cpp:13-34) on every moving push."""

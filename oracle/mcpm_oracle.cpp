@@ -1911,6 +1911,69 @@ int orc_upload_xy(void *h, int64_t n, const double *x, const double *y, const do
   return MCPM_OK;
 }
 
+// initialization (Main_Functions.cpp:4222-4815) with the library's Philox
+// blocks: (i, TAG_INIT, 0) position, (i, TAG_INIT, 1) two Box-Muller pairs;
+// slab and closed box: uniform along streaming axes, L/2 elsewhere;
+// injection: the last Nh particles in the centred disc (r = R sqrt(u),
+// theta = 2 pi u'); velocities U0 + sqrt(kB T_init / m) z.
+int orc_cell_update(void *h);
+int orc_initialize(void *h, const mcpm_settings *s) {
+  Oracle &o = *(Oracle *)h;
+  const mcpm_params &p = o.p;
+  const bool disc = s->problem == 12;
+  if (disc != (o.dims == 2) || s->n_particles < 0 || s->n_drop > s->n_particles)
+    return MCPM_ERR_ARG;
+  const int64_t n = s->n_particles;
+  const bool f32 = p.precision == MCPM_FP32;
+  auto rnd_t = [&](double v) { return f32 ? (double)(float)v : v; };
+  const uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
+  const double Lx = o.dims == 2 ? p.length_x : s->box_len[0], Ly = p.length;
+  const double sd = sqrt(p.kb * s->T_init / p.mass);
+  o.n = n;
+  o.rec.assign((size_t)o.S * n, 0.0);
+  for (int64_t i = 0; i < n; ++i) {
+    const uint32_t lo = (uint32_t)i, hi = (uint32_t)((uint64_t)i >> 32);
+    mcpm_u32x4 w = mcpm_philox(lo, hi, 0u, mcpm_ctr3(MCPM_TAG_INIT, 0u, 0u), key0, key1);
+    mcpm_u32x4 w1 = mcpm_philox(lo, hi, 0u, mcpm_ctr3(MCPM_TAG_INIT, 0u, 1u), key0, key1);
+    double x, y;
+    if (disc && i >= n - s->n_drop) {
+      double r = (s->drop_dia / 2.0) * sqrt(mcpm_u01(w.v[0]));
+      double sn, cs;
+      mcpm_sincos2pi(mcpm_u01(w.v[1]), &sn, &cs);
+      x = 0.5 * Lx + r * cs;
+      y = 0.5 * Ly + r * sn;
+    } else if (disc) {
+      x = mcpm_u01(w.v[0]) * Lx;
+      y = mcpm_u01(w.v[1]) * Ly;
+    } else {
+      x = s->direction[0] ? mcpm_u01(w.v[0]) * Lx : 0.5 * Lx;
+      y = s->direction[1] ? mcpm_u01(w.v[1]) * Ly : 0.5 * Ly;
+    }
+    double z0, z1, z2, z3;
+    mcpm_normal2(w1.v[0], w1.v[1], &z0, &z1);
+    mcpm_normal2(w1.v[2], w1.v[3], &z2, &z3);
+    double *r = &o.rec[(size_t)o.S * i];
+    double yr = rnd_t(y);
+    if (!(yr < rnd_t(Ly))) yr = 0.0;
+    if (o.dims == 2) {
+      double xr = rnd_t(x);
+      if (!(xr < rnd_t(Lx))) xr = 0.0;
+      r[0] = xr;
+      r[1] = yr;
+    } else {
+      r[0] = yr;
+    }
+    double *u = r + o.U;
+    u[0] = rnd_t(s->U0[0] + sd * z0);
+    u[1] = rnd_t(s->U0[1] + sd * z1);
+    u[2] = rnd_t(s->U0[2] + sd * z2);
+  }
+  o.sorted = false;
+  o.kick_pending = false;
+  o.have_closures = false;
+  return orc_cell_update(h);
+}
+
 int orc_download_xy(void *h, double *x, double *y, double *u1, double *u2, double *u3) {
   Oracle &o = *(Oracle *)h;
   if (o.dims != 2) return MCPM_ERR_ARG;
