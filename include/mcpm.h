@@ -329,6 +329,33 @@ int mcpm_get_pressure(mcpm_ctx *ctx, double *p_m, double *p_rep, double *p_tot);
    evaluated on the host with the context's parameters. */
 double mcpm_contact_y(const mcpm_params *p, double n);
 
+/* ---- batched replicas ------------------------------------------------------ */
+/* No reference counterpart (the reference runs one system per process): many
+   independent slabs of one length in one context, laid end to end in its
+   cells and particles, run by the same launches.  Each replica has its own
+   cells, cell volume, Fn and seed and draws exactly what it would draw as a
+   context of its own, so its particles evolve bit for bit as in an unbatched
+   run.  Models ESMC and NONE; no field or the screened-Poisson field; all
+   three slab boundaries.  The shared constants come from `common`, whose
+   n_cells and capacity are ignored. */
+typedef struct mcpm_replica {
+  int32_t  n_cells;      /* cells of this replica                               */
+  int32_t  reserved;
+  int64_t  capacity;     /* most particles it holds                             */
+  uint64_t seed;         /* its Philox key                                      */
+  double   cell_volume;  /* its cell volume                                     */
+  double   Fn;           /* its particle weight                                 */
+} mcpm_replica;
+int mcpm_create_batch(const mcpm_params *common, int32_t n_rep, const mcpm_replica *reps,
+                      mcpm_ctx **out);
+/* Particles replica after replica (counts[r] of them each); the counts stay
+   fixed afterwards.  mcpm_download returns them in the same replica order,
+   each replica's in its cell order.  mcpm_upload is refused on a batch. */
+int mcpm_upload_batch(mcpm_ctx *ctx, const int64_t *counts, const double *y,
+                      const double *u1, const double *u2, const double *u3);
+/* mcpm_init_omega_max with each replica's own mean density n_gas[r]. */
+int mcpm_init_omega_max_batch(mcpm_ctx *ctx, const double *n_gas, int32_t times_y);
+
 /* ---- settings file, initial state and PostProc output ---------------------- */
 /* What readSettingsFile (cpp_headers.h:624; myIO.cpp:1756-3016) derives from a
    reference settings file, for the problems on this path: closed_box (1,

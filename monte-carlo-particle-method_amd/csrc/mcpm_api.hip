@@ -43,7 +43,7 @@ struct mcpm_ctx {
   bool two_digit = false;
   int32_t kdiv = 0;
   int nb_lo = 0, nb_hi = 0;
-  uint32_t *dcnt = nullptr, *dstart = nullptr, *keys = nullptr;
+  uint32_t *dcnt = nullptr, *dstart = nullptr, *keys = nullptr, *keys_tmp = nullptr;
   // cells
   uint32_t *cnt = nullptr, *start = nullptr, *mcand = nullptr, *cstart = nullptr;
   double *nden = nullptr, *yc = nullptr, *omax = nullptr, *omega_new = nullptr, *F2 = nullptr;
@@ -75,6 +75,17 @@ struct mcpm_ctx {
   // direct and density-expansion long range
   double *lr_tab = nullptr, *lr_V = nullptr;
   double *kde = nullptr;  // n_smooth, dn2, dn3
+  // batched replicas (kp.nrep > 0): per-replica constants on the host and
+  // device, cell/particle/candidate offsets, per-replica field factors
+  std::vector<KParams> rkp_host;
+  std::vector<int32_t> rcell_host;
+  std::vector<int64_t> rpart_host;
+  std::vector<double> rep_ngas;
+  KParams *rkp = nullptr;
+  int32_t *cell_rep = nullptr, *rcell_off = nullptr;
+  int64_t *rpart_off = nullptr;
+  uint32_t *rprio = nullptr, *rcand_off = nullptr;
+  FieldArrays *rfa = nullptr;
   // field
   FieldArrays fa{};
   double *field_buf = nullptr;
@@ -206,12 +217,12 @@ void gauss_legendre_8dec(int n, double *x, double *w) {
 }
 
 // Factor the constant tridiagonal matrix of the 1D field solve.
-int build_field(mcpm_ctx *c) {
-  const int nc = c->nc;
-  const bool periodic = c->p.boundary == MCPM_BC_PERIODIC;
+// LU factors of one slab's face system (and, periodic, its Sherman-Morrison
+// vector), copied to the device buffer b of 9 (nf + 1) doubles; fa's face
+// pointers are set into b.
+int field_factors(int nc, double dy, bool periodic, double bst, double *b, FieldArrays *fa) {
   const int nf = periodic ? nc : nc + 1;
-  const double dy = c->kp.dy;
-  const double diag = 2.0 + pow(dy * c->p.bst, 2.0);  // Main_Functions.cpp:3375-3377
+  const double diag = 2.0 + pow(dy * bst, 2.0);  // Main_Functions.cpp:3375-3377
   std::vector<double> a(nf, diag), lo(nf, -1.0), up(nf, -1.0);  // lo[k]=A[k+1][k], up[k]=A[k][k+1]
   double vlast = 0.0, den = 1.0;
   std::vector<double> z;
@@ -240,7 +251,7 @@ int build_field(mcpm_ctx *c) {
     mU[k] = (k < nf - 1) ? -up[k] / d[k] : 0.0;
   }
   if (periodic) {
-    double gamma = -(2.0 + pow(dy * c->p.bst, 2.0));
+    double gamma = -(2.0 + pow(dy * bst, 2.0));
     double alpha = -1.0;
     std::vector<double> u(nf, 0.0), yv(nf), zz(nf);
     u[0] = gamma;
@@ -253,30 +264,44 @@ int build_field(mcpm_ctx *c) {
     den = 1.0 + zz[0] + vlast * zz[nf - 1];
   }
   const size_t nfa = (size_t)nf + 1;
-  int st = dalloc(&c->field_buf, nfa * 9 + 250);
+  fa->nj = b; b += nfa;
+  fa->rhs = b; b += nfa;
+  fa->tB = b; b += nfa;
+  fa->y = b; b += nfa;
+  fa->x = b; b += nfa;
+  fa->mL = b; b += nfa;
+  fa->invd = b; b += nfa;
+  fa->mU = b; b += nfa;
+  fa->sm_z = b; b += nfa;
+  fa->sm_vlast = vlast;
+  fa->sm_den = den;
+  HIPCHK(hipMemcpy(fa->mL, mL.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa->invd, invd.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(fa->mU, mU.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  if (periodic) HIPCHK(hipMemcpy(fa->sm_z, z.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  return MCPM_OK;
+}
+
+// The 1D field set-up: face-system factors (one slab, or every replica of a
+// batch) and the 50-point Dirichlet quadrature tables.
+int build_field(mcpm_ctx *c) {
+  const bool periodic = c->p.boundary == MCPM_BC_PERIODIC;
+  const int nrep = c->kp.nrep;
+  std::vector<size_t> off(std::max(nrep, 1) + 1, 0);
+  for (int r = 0; r < std::max(nrep, 1); ++r) {
+    const int n = nrep ? c->rkp_host[r].n_cells : c->nc;
+    off[r + 1] = off[r] + 9 * ((size_t)(periodic ? n : n + 1) + 1);
+  }
+  const size_t nbuf = off.back();
+  int st = dalloc(&c->field_buf, nbuf + 250);
   if (st) return st;
-  double *b = c->field_buf;
-  FieldArrays &fa = c->fa;
-  fa.nj = b; b += nfa;
-  fa.rhs = b; b += nfa;
-  fa.tB = b; b += nfa;
-  fa.y = b; b += nfa;
-  fa.x = b; b += nfa;
-  fa.mL = b; b += nfa;
-  fa.invd = b; b += nfa;
-  fa.mU = b; b += nfa;
-  fa.sm_z = b; b += nfa;
-  fa.gl_r = b; b += 50;
-  fa.gl_dr = b; b += 50;
-  fa.gl_cos = b; b += 50;
-  fa.gl_sin = b; b += 50;
-  fa.gl_dphi = b; b += 50;
-  fa.sm_vlast = vlast;
-  fa.sm_den = den;
-  HIPCHK(hipMemcpy(fa.mL, mL.data(), nf * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(fa.invd, invd.data(), nf * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(fa.mU, mU.data(), nf * sizeof(double), hipMemcpyHostToDevice));
-  if (periodic) HIPCHK(hipMemcpy(fa.sm_z, z.data(), nf * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<FieldArrays> rfa(std::max(nrep, 1));
+  for (int r = 0; r < std::max(nrep, 1); ++r) {
+    const int n = nrep ? c->rkp_host[r].n_cells : c->nc;
+    const double dy = nrep ? c->rkp_host[r].dy : c->kp.dy;
+    if ((st = field_factors(n, dy, periodic, c->p.bst, c->field_buf + off[r], &rfa[r]))) return st;
+  }
+  double *g = c->field_buf + nbuf;
   double xi[50], wi[50], r[50], dr[50], ph[50], dph[50], cph[50], sph[50];
   gauss_legendre_8dec(50, xi, wi);
   const double bb = 3.0 * c->p.eqdist, aa = 0.0, cc = 0.0, dd = PI;
@@ -288,11 +313,23 @@ int build_field(mcpm_ctx *c) {
     cph[i] = cos(ph[i]);
     sph[i] = sin(ph[i]);
   }
-  HIPCHK(hipMemcpy(fa.gl_r, r, sizeof r, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(fa.gl_dr, dr, sizeof dr, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(fa.gl_cos, cph, sizeof cph, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(fa.gl_sin, sph, sizeof sph, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(fa.gl_dphi, dph, sizeof dph, hipMemcpyHostToDevice));
+  for (FieldArrays &fa : rfa) {
+    fa.gl_r = g;
+    fa.gl_dr = g + 50;
+    fa.gl_cos = g + 100;
+    fa.gl_sin = g + 150;
+    fa.gl_dphi = g + 200;
+  }
+  HIPCHK(hipMemcpy(g, r, sizeof r, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g + 50, dr, sizeof dr, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g + 100, cph, sizeof cph, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g + 150, sph, sizeof sph, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(g + 200, dph, sizeof dph, hipMemcpyHostToDevice));
+  c->fa = rfa[0];
+  if (nrep) {
+    if ((st = dalloc(&c->rfa, (size_t)nrep))) return st;
+    HIPCHK(hipMemcpy(c->rfa, rfa.data(), nrep * sizeof(FieldArrays), hipMemcpyHostToDevice));
+  }
   return MCPM_OK;
 }
 
@@ -398,6 +435,7 @@ int check_err(mcpm_ctx *c) {
   if (e & ERRBIT_DOMAIN) return MCPM_ERR_DOMAIN;
   if (e & ERRBIT_NAN) return MCPM_ERR_NAN;
   if (e & ERRBIT_BC) return MCPM_ERR_BC;
+  if (e & ERRBIT_ORDER) return MCPM_ERR_STATE;
   return MCPM_ERR_CAPACITY;
 }
 
@@ -436,12 +474,12 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                             : nullptr;
   // one stable counting-sort pass over nb bins: histogram, scans, scatter
   auto pass = [&](const R *in, R *out, int pmode, const Kick &ks, int nb, int kmode,
-                  uint32_t *cnt, uint32_t *start, double *nden, double *yc,
-                  uint32_t *okey) -> int {
+                  uint32_t *cnt, uint32_t *start, double *nden, double *yc, uint32_t *okey,
+                  const uint32_t *ikey) -> int {
     const size_t lds = (size_t)nb * sizeof(uint32_t);
     hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
                        nullptr, c->n, chunk, c->kp, kick, drift, pmode, slo, shi, c->table,
-                       nullptr, c->err, nb, kmode, c->kdiv, nullptr);
+                       nullptr, c->err, nb, kmode, c->kdiv, nullptr, ikey);
     LAUNCHCHK();
     dim3 gs((nb + 255) / 256, nseg);
     hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
@@ -452,7 +490,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                        c->seg);
     hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
                        c->n, chunk, c->kp, ks, drift, pmode, slo, shi, c->table, start,
-                       c->err, nb, kmode, c->kdiv, okey);
+                       c->err, nb, kmode, c->kdiv, okey, ikey);
     LAUNCHCHK();
     return MCPM_OK;
   };
@@ -461,19 +499,21 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   int st;
   if (!c->two_digit) {
     if ((st = pass(in, out, mode, kick_scatter, nc, 0, c->cnt, c->start, c->nden, c->yc,
-                   nullptr)))
-      return st;
-  } else {
-    // low digit with the push, then the high digit of the pushed records
-    // back into the first buffer; cells from the sorted keys
-    if ((st = pass(in, out, mode, kick_scatter, c->nb_lo, 1, c->dcnt, c->dstart, nullptr,
                    nullptr, nullptr)))
       return st;
-    if ((st = pass(out, (R *)in, 0, kick, c->nb_hi, 2, c->dcnt, c->dstart, nullptr, nullptr,
-                   c->keys)))
+  } else {
+    // low digit with the push (keys carried to the destinations), then the
+    // high digit of those keys back into the first buffer; cells from the
+    // sorted keys
+    if ((st = pass(in, out, mode, kick_scatter, c->nb_lo, 1, c->dcnt, c->dstart, nullptr,
+                   nullptr, c->keys_tmp, nullptr)))
       return st;
+    if ((st = pass(out, (R *)in, 0, kick, c->nb_hi, 2, c->dcnt, c->dstart, nullptr, nullptr,
+                   c->keys, c->keys_tmp)))
+      return st;
+    hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->s, c->start, (int64_t)nc + 1, 0u);
     hipLaunchKernelGGL(k_cell_bounds, dim3(2048), dim3(256), 0, c->s, c->keys, c->n, nc,
-                       c->start);
+                       c->start, c->err);
     hipLaunchKernelGGL(k_cell_fill, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->start, nc,
                        c->cnt, c->nden, c->yc, c->kp);
     LAUNCHCHK();
@@ -539,6 +579,11 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   if (st) return st;
   mcpm_u32x4 pk = mcpm_philox(0u, 0u, slo, mcpm_ctr3(MCPM_TAG_PRIO, shi, 0u), c->kp.key0,
                               c->kp.key1);
+  if (c->kp.nrep) {
+    hipLaunchKernelGGL(k_rep_step, dim3((c->kp.nrep + 256) / 256), dim3(256), 0, c->s, c->kp,
+                       c->cstart, c->rprio, c->rcand_off, slo, shi);
+    LAUNCHCHK();
+  }
   // ESMC priorities spread over 32 bits and enter the rounds in bands; DSMC
   // operations of different cells never conflict, one band keeps every cell
   // busy from the first round
@@ -553,7 +598,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
     const int esmc = MODEL == COLL_ESMC;
     hipLaunchKernelGGL(k_cand_count, dim3(nchunks), dim3(256), 0, c->s, M, pk.v[0], esmc,
-                       band_mul, nbands, c->gen_table);
+                       band_mul, nbands, c->gen_table, c->kp);
     hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
                        nchunks, nbands * NREG, c->counters);
     hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
@@ -822,7 +867,8 @@ const char *mcpm_status_string(int s) {
   }
 }
 
-int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
+static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica *reps,
+                      mcpm_ctx **out) {
   if (!p_in || !out) return MCPM_ERR_ARG;
   *out = nullptr;
   mcpm_params pv = *p_in;
@@ -891,6 +937,47 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   k.coll_pref = 4.0 * PI * p->sigma * p->sigma;
   k.key0 = (uint32_t)p->seed;
   k.key1 = (uint32_t)(p->seed >> 32);
+  k.nrep = 0;
+  if (nrep > 0) {
+    // per-replica constants: own cells, dy, volume, Fn and key; the rest shared
+    c->rkp_host.resize(nrep);
+    c->rcell_host.assign(nrep + 1, 0);
+    c->rpart_host.assign(nrep + 1, 0);
+    std::vector<int32_t> crep(c->nc);
+    for (int r = 0; r < nrep; ++r) {
+      KParams q = k;
+      q.n_cells = reps[r].n_cells;
+      q.ny = reps[r].n_cells;
+      q.dy = p->length / (double)reps[r].n_cells;
+      q.volume = reps[r].cell_volume;
+      q.Fn = reps[r].Fn;
+      q.key0 = (uint32_t)reps[r].seed;
+      q.key1 = (uint32_t)(reps[r].seed >> 32);
+      c->rkp_host[r] = q;
+      c->rcell_host[r + 1] = c->rcell_host[r] + reps[r].n_cells;
+      for (int cc = c->rcell_host[r]; cc < c->rcell_host[r + 1]; ++cc) crep[cc] = r;
+    }
+    int st0;
+    if ((st0 = dalloc(&c->rkp, (size_t)nrep)) || (st0 = dalloc(&c->cell_rep, (size_t)c->nc)) ||
+        (st0 = dalloc(&c->rcell_off, (size_t)nrep + 1)) ||
+        (st0 = dalloc(&c->rpart_off, (size_t)nrep + 1)) || (st0 = dalloc(&c->rprio, (size_t)nrep)) ||
+        (st0 = dalloc(&c->rcand_off, (size_t)nrep + 1))) {
+      mcpm_destroy(c);
+      return st0;
+    }
+    HIPCHK(hipMemcpy(c->rkp, c->rkp_host.data(), nrep * sizeof(KParams), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->cell_rep, crep.data(), c->nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->rcell_off, c->rcell_host.data(), (nrep + 1) * sizeof(int32_t),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->rpart_off, 0, (nrep + 1) * sizeof(int64_t)));
+    k.nrep = nrep;
+    k.rkp = c->rkp;
+    k.cell_rep = c->cell_rep;
+    k.rcell_off = c->rcell_off;
+    k.rpart_off = c->rpart_off;
+    k.rprio = c->rprio;
+    k.rcand_off = c->rcand_off;
+  }
   c->fpp.model = p->model;
   c->fpp.reserved = 0;
   c->fpp.mu = p->mu;
@@ -920,6 +1007,7 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
     if ((st = dalloc(&c->dcnt, nbins))) return fail(st);
     if ((st = dalloc(&c->dstart, nbins + 1))) return fail(st);
     if ((st = dalloc(&c->keys, (size_t)p->capacity))) return fail(st);
+    if ((st = dalloc(&c->keys_tmp, (size_t)p->capacity))) return fail(st);
   }
   c->nchunks_cap = (p->capacity + 1023) / 1024;
   {
@@ -974,10 +1062,61 @@ int mcpm_create(const mcpm_params *p_in, mcpm_ctx **out) {
   return MCPM_OK;
 }
 
+int mcpm_create(const mcpm_params *p, mcpm_ctx **out) { return create_ctx(p, 0, nullptr, out); }
+
+int mcpm_create_batch(const mcpm_params *common, int32_t n_rep, const mcpm_replica *reps,
+                      mcpm_ctx **out) {
+  if (!common || !reps || !out || n_rep < 1 || n_rep > 65536) return MCPM_ERR_ARG;
+  *out = nullptr;
+  // independent slabs of one length: ESMC or free flight, no field or the
+  // screened-Poisson field
+  if (common->dims > 1) return MCPM_ERR_ARG;
+  if (common->model != MCPM_MODEL_ESMC && common->model != MCPM_MODEL_NONE) return MCPM_ERR_ARG;
+  if (common->long_range != MCPM_LR_NONE && common->long_range != MCPM_LR_SCREENED_POISSON)
+    return MCPM_ERR_ARG;
+  mcpm_params p = *common;
+  int64_t cells = 0, cap = 0;
+  for (int r = 0; r < n_rep; ++r) {
+    if (reps[r].n_cells < 1 || reps[r].capacity < 0 || !(reps[r].cell_volume > 0.0) ||
+        !(reps[r].Fn > 0.0))
+      return MCPM_ERR_ARG;
+    cells += reps[r].n_cells;
+    cap += reps[r].capacity;
+  }
+  if (cells > MAX_CELLS || cap < 1) return MCPM_ERR_ARG;
+  p.dims = 1;
+  p.n_cells = (int32_t)cells;
+  p.capacity = cap;
+  return create_ctx(&p, n_rep, reps, out);
+}
+
+int mcpm_init_omega_max_batch(mcpm_ctx *c, const double *n_gas, int32_t times_y) {
+  if (!c || !n_gas) return MCPM_ERR_ARG;
+  if (c->kp.nrep == 0) return MCPM_ERR_STATE;
+  std::vector<double> mom((size_t)c->nc * MCPM_NMOM), nd(c->nc), om(c->nc);
+  int st = mcpm_moments(c, mom.data());
+  if (st) return st;
+  st = mcpm_get_cells(c, nullptr, nullptr, nd.data(), nullptr, nullptr);
+  if (st) return st;
+  for (int r = 0; r < c->kp.nrep; ++r) {
+    const double y = times_y ? contact_y_host(&c->p, n_gas[r]) : 1.0;
+    for (int k = c->rcell_host[r]; k < c->rcell_host[r + 1]; ++k) {
+      double T = mom[(size_t)k * MCPM_NMOM + 29];
+      double crm = 10.0 * sqrt(c->p.kb * T / c->p.mass);                          // main.cpp:259
+      double o = 4.0 * PI * c->p.sigma * c->p.sigma * crm * nd[k] * c->p.delta_t;  // :260
+      om[k] = o * y;                                                                  // :268-269
+    }
+  }
+  return mcpm_set_omega_max(c, om.data());
+}
+
 int mcpm_destroy(mcpm_ctx *c) {
   if (!c) return MCPM_OK;
   if (c->s) hipStreamSynchronize(c->s);
-  void *bufs[] = {c->P[0], c->P[1], c->table, c->partial, c->dcnt, c->dstart, c->keys,
+  void *bufs[] = {c->rkp, c->cell_rep, c->rcell_off, c->rpart_off, c->rprio, c->rcand_off,
+                  c->rfa,
+                  c->P[0], c->P[1], c->table, c->partial, c->dcnt, c->dstart, c->keys,
+                  c->keys_tmp,
                   c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
                   c->samples, c->press, c->counters, c->err, c->res, c->opA,
@@ -1060,9 +1199,30 @@ extern "C" {
 int mcpm_upload(mcpm_ctx *c, int64_t n, const double *y, const double *u1, const double *u2,
                 const double *u3) {
   if (!c || c->p.dims == 2) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // a batch needs its per-replica counts
   const double *src[4] = {y, u1, u2, u3};
   return upload_any(c, n, src, 4);
 }
+
+int mcpm_upload_batch(mcpm_ctx *c, const int64_t *counts, const double *y, const double *u1,
+                      const double *u2, const double *u3) {
+  if (!c || !counts) return MCPM_ERR_ARG;
+  const int nrep = c->kp.nrep;
+  if (nrep == 0) return MCPM_ERR_STATE;
+  int64_t n = 0;
+  for (int r = 0; r < nrep; ++r) {
+    if (counts[r] < 0) return MCPM_ERR_ARG;
+    c->rpart_host[r] = n;
+    n += counts[r];
+  }
+  c->rpart_host[nrep] = n;
+  if (n > c->p.capacity) return MCPM_ERR_CAPACITY;
+  HIPCHK(hipMemcpy(c->rpart_off, c->rpart_host.data(), (nrep + 1) * sizeof(int64_t),
+                   hipMemcpyHostToDevice));
+  const double *src[4] = {y, u1, u2, u3};
+  return upload_any(c, n, src, 4);
+}
+
 
 int mcpm_upload_xy(mcpm_ctx *c, int64_t n, const double *x, const double *y, const double *u1,
                    const double *u2, const double *u3) {
@@ -1108,6 +1268,7 @@ int mcpm_collide_esmc(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
 
 int mcpm_collide_dsmc_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->n == 0) {
     if (stats) memset(stats, 0, sizeof *stats);
@@ -1118,6 +1279,7 @@ int mcpm_collide_dsmc_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats
 
 int mcpm_collide_dsmc_hs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->n == 0) {
     if (stats) memset(stats, 0, sizeof *stats);
@@ -1128,6 +1290,7 @@ int mcpm_collide_dsmc_hs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats)
 
 int mcpm_collide_cba_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->n == 0) {
     if (stats) memset(stats, 0, sizeof *stats);
@@ -1138,6 +1301,7 @@ int mcpm_collide_cba_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats)
 
 int mcpm_collide_cba_hs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->n == 0) {
     if (stats) memset(stats, 0, sizeof *stats);
@@ -1148,6 +1312,7 @@ int mcpm_collide_cba_hs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) 
 
 int mcpm_fp_closures(mcpm_ctx *c, int64_t step) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->p.dims != 1) return MCPM_ERR_ARG;
   int st = MCPM_DISPATCH(c, launch_moments, c);
@@ -1195,8 +1360,9 @@ int mcpm_poisson_solver(mcpm_ctx *c) {
     c->kick_pending = true;
     return MCPM_OK;
   }
-  hipLaunchKernelGGL(k_field, dim3(1), dim3(1024), 0, c->s, c->nden, c->fa, c->F2, c->kp,
-                     c->p.ast, c->p.bst, c->p.eqdist);
+  hipLaunchKernelGGL(k_field, dim3(c->kp.nrep ? c->kp.nrep : 1), dim3(1024), 0, c->s, c->nden,
+                     c->fa, c->F2, c->kp, c->p.ast, c->p.bst, c->p.eqdist,
+                     (const FieldArrays *)c->rfa);
   LAUNCHCHK();
   c->kick_pending = true;
   return MCPM_OK;
@@ -1204,6 +1370,7 @@ int mcpm_poisson_solver(mcpm_ctx *c) {
 
 int mcpm_vlasov_integral(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   Timer tm(c, T_FIELD);
   if (c->p.dims == 2) {
@@ -1238,6 +1405,7 @@ static int run_kde(mcpm_ctx *c) {
 
 int mcpm_density_expansion(mcpm_ctx *c) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   Timer tm(c, T_FIELD);
   int st = run_kde(c);
@@ -1253,6 +1421,7 @@ int mcpm_density_expansion(mcpm_ctx *c) {
 
 int mcpm_get_kde(mcpm_ctx *c, double *ns, double *d2, double *d3) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   int st = run_kde(c);
   if (st) return st;
@@ -1368,6 +1537,7 @@ int mcpm_reset_samples(mcpm_ctx *c) {
 
 int mcpm_initialize(mcpm_ctx *c, const mcpm_settings *s) {
   if (!c || !s) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (s->n_particles < 0 || s->n_particles > c->p.capacity) return MCPM_ERR_CAPACITY;
   if (s->n_drop < 0 || s->n_drop > s->n_particles) return MCPM_ERR_ARG;
   const bool disc = s->problem == 12;
@@ -1428,6 +1598,7 @@ int mcpm_get_pressure(mcpm_ctx *c, double *p_m, double *p_rep, double *p_tot) {
 
 int mcpm_init_omega_max(mcpm_ctx *c, double n_gas, int32_t times_y) {
   if (!c) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   std::vector<double> mom((size_t)c->nc * MCPM_NMOM), nd(c->nc), om(c->nc);
   int st = mcpm_moments(c, mom.data());
   if (st) return st;

@@ -56,7 +56,41 @@ struct KParams {
   double sigma, mass, kb, b, Fn, volume, n_ratio, dt;
   double coll_pref;  // 4 pi sigma^2, multiplied left to right as in collisions.cpp:691
   uint32_t key0, key1;  // Philox key from the seed
+  // Batched replicas (nrep > 0): independent slabs laid end to end, replica
+  // r owning global cells [rcell_off[r], rcell_off[r + 1]) and particles
+  // [rpart_off[r], rpart_off[r + 1]).  rkp[r] holds the replica's own
+  // constants (cells, dy, volume, Fn, Philox key) with nrep = 0; kernels
+  // work in the replica's local cell numbers and add rcell_off[r].
+  int32_t nrep;
+  int32_t reserved2;
+  const KParams *rkp;
+  const int32_t *cell_rep;    // replica of each global cell
+  const int32_t *rcell_off;   // nrep + 1
+  const int64_t *rpart_off;   // nrep + 1
+  const uint32_t *rprio;      // candidate priority key of the step, per replica
+  const uint32_t *rcand_off;  // first candidate of each replica in this step, nrep + 1
 };
+
+// Replica of particle p (batched contexts): the last r with rpart_off[r] <= p.
+__device__ __forceinline__ int32_t rep_of_particle(const KParams &kp, int64_t p) {
+  int32_t a = 0, b = kp.nrep;
+  while (b - a > 1) {
+    const int32_t m = (a + b) >> 1;
+    if (kp.rpart_off[m] <= p) a = m;
+    else b = m;
+  }
+  return a;
+}
+// Replica of candidate k of the step: the last r with rcand_off[r] <= k.
+__device__ __forceinline__ int32_t rep_of_cand(const KParams &kp, uint32_t k) {
+  int32_t a = 0, b = kp.nrep;
+  while (b - a > 1) {
+    const int32_t m = (a + b) >> 1;
+    if (kp.rcand_off[m] <= k) a = m;
+    else b = m;
+  }
+  return a;
+}
 
 // Parameters of the Fokker-Planck and DSMC velocity models.
 struct FPParams {

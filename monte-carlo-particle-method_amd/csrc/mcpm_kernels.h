@@ -9,7 +9,13 @@ namespace mcpm {
 enum : int { PUSH_KICK = 1, PUSH_MOVE = 2, PUSH_CHECK = 4 };
 
 // device error bits -> status codes
-enum : uint32_t { ERRBIT_DOMAIN = 1u, ERRBIT_NAN = 2u, ERRBIT_BC = 4u, ERRBIT_CAPACITY = 8u };
+enum : uint32_t {
+  ERRBIT_DOMAIN = 1u,
+  ERRBIT_NAN = 2u,
+  ERRBIT_BC = 4u,
+  ERRBIT_CAPACITY = 8u,
+  ERRBIT_ORDER = 16u  // sorted keys out of order (an internal error)
+};
 
 // layout of the u32 device counter block
 enum : int {
@@ -59,12 +65,13 @@ __global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp
                        const double *drift, int mode,
                        uint32_t step_lo, uint32_t step_hi, uint32_t *table,
                        const uint32_t *start, uint32_t *err, int nb, int kmode, int32_t kdiv,
-                       uint32_t *okey);
+                       uint32_t *okey, const uint32_t *ikey);
 // Most bins of one sort pass (one LDS histogram of 64 KB per chunk); more
 // cells sort by two digits of the cell key, up to MAX_CELLS.
 constexpr int SORT_BINS = 16384;
 constexpr int64_t MAX_CELLS = (int64_t)1 << 22;
-__global__ void k_cell_bounds(const uint32_t *key, int64_t n, int nc, uint32_t *start);
+__global__ void k_cell_bounds(const uint32_t *key, int64_t n, int nc, uint32_t *start,
+                              uint32_t *err);
 __global__ void k_cell_fill(const uint32_t *start, int nc, uint32_t *cnt, double *nden,
                             double *yc, KParams kp);
 __global__ void k_scan_a(const uint32_t *table, uint32_t *partial, int nchunks, int nc, int seg);
@@ -88,7 +95,7 @@ __device__ __forceinline__ uint32_t op_band(uint32_t pr, uint32_t band_mul, int 
 constexpr uint32_t GEN_CHUNK = 2048;
 constexpr int NLIST = MAX_BANDS * NREG;
 __global__ void k_cand_count(uint32_t M, uint32_t prio_key, int esmc, uint32_t band_mul,
-                             int nbands, uint32_t *table);
+                             int nbands, uint32_t *table, KParams kp);
 __global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters);
 __global__ void k_list_starts(uint32_t *counters, int nl);
 template <typename R, int MODEL>
@@ -112,10 +119,14 @@ template <typename R>
 __global__ void k_cba_hs(R *P, const uint32_t *cnt, const uint32_t *start, const double *mom,
                          const double *nden, KParams kp, uint32_t step_lo, uint32_t step_hi,
                          uint32_t *counters, uint32_t *err);
+// Batched replicas, per step: each replica's candidate priority key and its
+// first candidate (rcand_off[nrep] = all candidates).
+__global__ void k_rep_step(KParams kp, const uint32_t *cstart, uint32_t *rprio,
+                           uint32_t *rcand_off, uint32_t step_lo, uint32_t step_hi);
 __global__ void k_zero_u32(uint32_t *p, int n);
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v);
 __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams kp, double ast,
-                        double bst, double eqdist);
+                        double bst, double eqdist, const FieldArrays *rfa);
 template <typename R>
 __global__ void k_moments(const R *P, const uint32_t *cnt, const uint32_t *start,
                           double *out, KParams kp);

@@ -245,7 +245,9 @@ __device__ __forceinline__ bool in_domain(const Rec2<real> &r, const KParams &kp
 // holds, the low (kmode 1: key % kdiv) then the high digit (kmode 2:
 // key / kdiv) of the cell key in two stable passes (least significant digit
 // first), which together sort by the key.  okey, when given, receives each
-// particle's cell key at its destination.
+// particle's cell key at its destination; ikey, when given, supplies the keys
+// instead of the records (the second digit pass, whose input is in low-digit
+// order: a batched replica cannot be told from a particle's index there).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int32_t sort_bin(int32_t key, int kmode, int32_t kdiv) {
   return kmode == 0 ? key : (kmode == 1 ? key % kdiv : key / kdiv);
@@ -259,7 +261,8 @@ __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__rest
                                              uint32_t *__restrict__ table,
                                              const uint32_t *__restrict__ start,
                                              uint32_t *err, int nb, int kmode, int32_t kdiv,
-                                             uint32_t *__restrict__ okey) {
+                                             uint32_t *__restrict__ okey,
+                                             const uint32_t *__restrict__ ikey) {
   extern __shared__ uint32_t lds_cnt[];
   const int nc = nb;
   const int lane = threadIdx.x;
@@ -291,9 +294,26 @@ __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__rest
       R r = rr[u];
       if (valid) {
         if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
-        if ((mode & PUSH_CHECK) && !in_domain(r, kp)) atomicOr(err, ERRBIT_DOMAIN);
-        r = push_one(r, (uint64_t)p, kp, kick, drift, mode, step_lo, step_hi, err);
-        ckey = cell_key(r, kp);
+        if (ikey) {
+          ckey = (int32_t)ikey[p];
+        } else if (kp.nrep == 0) {
+          if ((mode & PUSH_CHECK) && !in_domain(r, kp)) atomicOr(err, ERRBIT_DOMAIN);
+          r = push_one(r, (uint64_t)p, kp, kick, drift, mode, step_lo, step_hi, err);
+          ckey = cell_key(r, kp);
+        } else {
+          // a batched replica: its own geometry, Philox key, particle slots
+          // and cells counted from its first
+          const int32_t q = rep_of_particle(kp, p);
+          const KParams &kq = kp.rkp[q];
+          const int32_t off = kp.rcell_off[q];
+          Kick kk = kick;
+          kk.F2 = kick.F2 + off;
+          if (kk.pm) kk.pm = kick.pm + off;
+          if ((mode & PUSH_CHECK) && !in_domain(r, kq)) atomicOr(err, ERRBIT_DOMAIN);
+          r = push_one(r, (uint64_t)(p - kp.rpart_off[q]), kq, kk, drift, mode, step_lo,
+                       step_hi, err);
+          ckey = off + cell_key(r, kq);
+        }
         key = sort_bin(ckey, kmode, kdiv);
       }
       uint64_t active = __ballot(valid);
@@ -360,9 +380,10 @@ __global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ partial,
     cnt[c] = run;
     tot += run;
     if (nden) {
-      double nd = kp.n_ratio * kp.Fn * (double)run / kp.volume;  // Main_Functions.cpp:2601
+      const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
+      double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
       nden[c] = nd;
-      yc[c] = contact_y(nd, kp.b, kp.y_model);
+      yc[c] = contact_y(nd, kq.b, kq.y_model);
     }
   }
   sh[tid] = tot;
@@ -384,15 +405,19 @@ __global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ partial,
 }
 
 // Cells of the two-digit sort: start[c] = first particle of cell c from the
-// sorted keys (every c in [0, nc] written once; keys ascend), then counts,
-// densities and contact values as k_scan_b.
+// sorted keys (every c in [0, nc] written once when the keys ascend, which the
+// sort guarantees), then counts, densities and contact values as k_scan_b.
+// start is zeroed first and a descent raises ERRBIT_ORDER, so even keys out
+// of order leave every start in [0, n] and every count within the array.
 __global__ void k_cell_bounds(const uint32_t *__restrict__ key, int64_t n, int nc,
-                              uint32_t *__restrict__ start) {
+                              uint32_t *__restrict__ start, uint32_t *__restrict__ err) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
        i += (int64_t)gridDim.x * blockDim.x) {
     const int64_t prev = i > 0 ? (int64_t)key[i - 1] : -1;
     const int64_t cur = i < n ? (int64_t)key[i] : (int64_t)nc;
-    for (int64_t c = prev + 1; c <= cur; ++c) start[c] = (uint32_t)i;
+    if (cur < prev || cur > nc) atomicOr(err, ERRBIT_ORDER);
+    const int64_t hi = cur < nc ? cur : (int64_t)nc;
+    for (int64_t c = prev + 1; c <= hi; ++c) start[c] = (uint32_t)i;
   }
 }
 
@@ -401,11 +426,12 @@ __global__ void k_cell_fill(const uint32_t *__restrict__ start, int nc,
                             double *__restrict__ yc, KParams kp) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
-  const uint32_t run = start[c + 1] - start[c];
+  const uint32_t run = start[c + 1] >= start[c] ? start[c + 1] - start[c] : 0u;
   cnt[c] = run;
-  double nd = kp.n_ratio * kp.Fn * (double)run / kp.volume;  // Main_Functions.cpp:2601
+  const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
+  double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
   nden[c] = nd;
-  yc[c] = contact_y(nd, kp.b, kp.y_model);
+  yc[c] = contact_y(nd, kq.b, kq.y_model);
 }
 
 __global__ void k_scan_c(uint32_t *__restrict__ table, const uint32_t *__restrict__ partial,
@@ -436,8 +462,12 @@ __global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restri
                         uint32_t step_lo, uint32_t step_hi) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= nc) return;
-  mcpm_u32x4 w = mcpm_philox((uint32_t)c, 0u, step_lo, mcpm_ctr3(MCPM_TAG_MCAND, step_hi, 0u),
-                             kp.key0, kp.key1);
+  // batched replicas draw with their own key and local cell numbers
+  const int32_t rq = kp.nrep ? kp.cell_rep[c] : 0;
+  const KParams &kq = kp.nrep ? kp.rkp[rq] : kp;
+  const uint32_t cl = (uint32_t)(kp.nrep ? c - kp.rcell_off[rq] : c);
+  mcpm_u32x4 w = mcpm_philox(cl, 0u, step_lo, mcpm_ctr3(MCPM_TAG_MCAND, step_hi, 0u),
+                             kq.key0, kq.key1);
   double m;
   if (model == COLL_DSMC) {
     if (cnt[c] < 2) {
@@ -513,6 +543,35 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *
 }
 
 
+// The candidate draw in replica rq of a batched context: the replica's key,
+// its local cell number for the counters, its geometry for the partner cell.
+template <typename R>
+__device__ __forceinline__ bool draw_candidate_rep(uint32_t c, uint32_t t, int32_t rq,
+                                                   const R *P, const uint32_t *cnt,
+                                                   const uint32_t *start, const KParams &kp,
+                                                   uint32_t step_lo, uint32_t step_hi,
+                                                   uint32_t *oi, uint32_t *oj) {
+  const KParams &kq = kp.rkp[rq];
+  const int32_t off = kp.rcell_off[rq];
+  mcpm_u32x4 w = mcpm_philox(t, c - (uint32_t)off, step_lo,
+                             mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kq.key0, kq.key1);
+  uint32_t Nc = cnt[c];
+  const uint32_t li = mcpm_uint_below(w.v[0], Nc);
+  uint32_t i = start[c] + li;
+  double d0, d1, d2;
+  mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
+  int32_t J;
+  if (!partner_cell(P[i], d0, d1, kq, &J)) return false;
+  J += off;
+  uint32_t NJ = cnt[J];
+  if (NJ == 0) return false;
+  uint32_t j = start[J] + mcpm_uint_below(w.v[3], NJ);
+  if (j == i) return false;
+  *oi = i;
+  *oj = j;
+  return true;
+}
+
 // DSMC pair draw (collisions.cpp:2093-2098): two distinct particles of cell c.
 // The draws of candidate k (global index) are keyed by (k, c): the commit
 // finds both in the operation record, whatever the particles' positions.
@@ -558,11 +617,18 @@ __device__ __forceinline__ uint32_t region_from(uint32_t k, uint32_t x0, uint32_
 __device__ __forceinline__ uint32_t cand_priority(uint32_t k, uint32_t key, int esmc) {
   return esmc ? mcpm_priority(k, key) : k;
 }
+// The same for candidate k of replica rq of a batched context: the replica's
+// own candidate number and key, as if it ran alone.  Priorities of different
+// replicas may coincide; their operations never share a particle.
+__device__ __forceinline__ uint32_t cand_priority_rep(const KParams &kp, int32_t rq, uint32_t k,
+                                                      int esmc) {
+  return cand_priority(k - kp.rcand_off[rq], kp.rprio[rq], esmc);
+}
 
 // Size of every (band, region) list per chunk of candidates.
 __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_key, int esmc,
                                                     uint32_t band_mul, int nbands,
-                                                    uint32_t *__restrict__ table) {
+                                                    uint32_t *__restrict__ table, KParams kp) {
   __shared__ uint32_t scount[NLIST];
   const int nl = nbands * NREG;
   if ((int)threadIdx.x < NLIST) scount[threadIdx.x] = 0;
@@ -572,8 +638,9 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
   for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
     const uint32_t k = lo + b0 + threadIdx.x;
     if (k >= M) break;
-    const uint32_t q = op_band(cand_priority(k, prio_key, esmc), band_mul, nbands) * NREG +
-                       region_from(k, x0, M);
+    const uint32_t pr = kp.nrep ? cand_priority_rep(kp, rep_of_cand(kp, k), k, esmc)
+                                : cand_priority(k, prio_key, esmc);
+    const uint32_t q = op_band(pr, band_mul, nbands) * NREG + region_from(k, x0, M);
     atomicAdd(&scount[q], 1u);
   }
   __syncthreads();
@@ -674,15 +741,19 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
     uint32_t pr = 0, q = 0;
     if (live) {
       const uint32_t t = k - cstart[c];
+      const int32_t rq = kp.nrep ? kp.cell_rep[c] : 0;
       if (cnt[c] > 0) {
         if (MODEL == COLL_DSMC) {
           draw_dsmc_pair(c, k, cnt, start, kp, step_lo, step_hi, &i, &j);
           ok = true;
-        } else {
+        } else if (kp.nrep == 0) {
           ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+        } else {
+          ok = draw_candidate_rep<R>(c, t, rq, P, cnt, start, kp, step_lo, step_hi, &i, &j);
         }
       }
-      pr = cand_priority(k, prio_key, MODEL == COLL_ESMC);
+      pr = kp.nrep ? cand_priority_rep(kp, rq, k, MODEL == COLL_ESMC)
+                   : cand_priority(k, prio_key, MODEL == COLL_ESMC);
       const uint32_t band = op_band(pr, band_mul, nbands);
       q = band * NREG + region_from(k, x0, M);
       if (ok) {
@@ -716,15 +787,20 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R 
                                           const KParams &kp, uint32_t step_lo,
                                           uint32_t step_hi) {
   using T = typename R::real_type;
-  const uint32_t c = (uint32_t)cell_key(a, kp);
-  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
-                             kp.key1);
-  mcpm_u32x4 w2 = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 1u), kp.key0,
-                              kp.key1);
+  // a batched replica: its geometry and key, cells counted from its first
+  const int32_t rq = kp.nrep ? rep_of_particle(kp, (int64_t)i) : 0;
+  const KParams &kq = kp.nrep ? kp.rkp[rq] : kp;
+  const int32_t off = kp.nrep ? kp.rcell_off[rq] : 0;
+  const uint32_t cl = (uint32_t)cell_key(a, kq);
+  const uint32_t c = cl + (uint32_t)off;
+  mcpm_u32x4 w = mcpm_philox(t, cl, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kq.key0,
+                             kq.key1);
+  mcpm_u32x4 w2 = mcpm_philox(t, cl, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 1u), kq.key0,
+                              kq.key1);
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-  int32_t cm = mid_cell(a, bb, kp);
-  int32_t J = cell_key(bb, kp);
+  int32_t cm = off + mid_cell(a, bb, kq);
+  int32_t J = off + cell_key(bb, kq);
   double Ymid = yc[cm];
   double g0 = (double)a.u1 - (double)bb.u1;
   double g1 = (double)a.u2 - (double)bb.u2;
@@ -1055,6 +1131,19 @@ __global__ void k_pressure(const double *__restrict__ mom, const double *__restr
   p_tot[c] = pe - 2.0 * kp.b * n * n * phi;
 }
 
+__global__ void k_rep_step(KParams kp, const uint32_t *__restrict__ cstart,
+                           uint32_t *__restrict__ rprio, uint32_t *__restrict__ rcand_off,
+                           uint32_t step_lo, uint32_t step_hi) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > kp.nrep) return;
+  rcand_off[r] = cstart[kp.rcell_off[r]];
+  if (r < kp.nrep) {
+    const KParams &kq = kp.rkp[r];
+    rprio[r] = mcpm_philox(0u, 0u, step_lo, mcpm_ctr3(MCPM_TAG_PRIO, step_hi, 0u), kq.key0,
+                           kq.key1).v[0];
+  }
+}
+
 __global__ void k_zero_u32(uint32_t *p, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0u;
@@ -1120,12 +1209,21 @@ __device__ void block_affine_scan(const double *__restrict__ A, const double *__
 }
 
 __global__ __launch_bounds__(1024) void k_field(const double *__restrict__ nden,
-                                                FieldArrays fa, double *__restrict__ F2,
-                                                KParams kp, double ast, double bst,
-                                                double eqdist) {
+                                                FieldArrays fa0, double *__restrict__ F2,
+                                                KParams kp0, double ast, double bst,
+                                                double eqdist,
+                                                const FieldArrays *__restrict__ rfa) {
   __shared__ double sA[1024];
   __shared__ double sB[1024];
   __shared__ double red[2][1024];
+  // batched replicas: workgroup b solves replica b's slab
+  const bool batch = kp0.nrep > 0;
+  const KParams &kp = batch ? kp0.rkp[blockIdx.x] : kp0;
+  const FieldArrays fa = batch ? rfa[blockIdx.x] : fa0;
+  if (batch) {
+    nden += kp0.rcell_off[blockIdx.x];
+    F2 += kp0.rcell_off[blockIdx.x];
+  }
   const int tid = threadIdx.x;
   const int nc = kp.n_cells;
   const bool periodic = kp.boundary == MCPM_BC_PERIODIC;
@@ -1430,11 +1528,11 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
                                             Kick, const double *, int, uint32_t, uint32_t, \
                                             uint32_t *, const uint32_t *, uint32_t *, int, \
-                                            int, int32_t, uint32_t *);                     \
+                                            int, int32_t, uint32_t *, const uint32_t *);   \
   template __global__ void k_push<R, true>(const R *, R *, int64_t, int32_t, KParams,      \
                                            Kick, const double *, int, uint32_t, uint32_t,  \
                                            uint32_t *, const uint32_t *, uint32_t *, int,  \
-                                           int, int32_t, uint32_t *);                      \
+                                           int, int32_t, uint32_t *, const uint32_t *);    \
   MCPM_INST_COLL(R, COLL_ESMC)                                                             \
   MCPM_INST_COLL(R, COLL_DSMC)                                                             \
   template __global__ void k_init<R>(R *, InitParams, KParams);                           \

@@ -168,6 +168,13 @@ class Params:
         return cls(**{f.name: getattr(c, f.name) for f in fields(cls)})
 
 
+class CReplica(ctypes.Structure):
+    """mcpm_replica: one slab of a batched context."""
+    _fields_ = [("n_cells", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("capacity", ctypes.c_int64), ("seed", ctypes.c_uint64),
+                ("cell_volume", ctypes.c_double), ("Fn", ctypes.c_double)]
+
+
 class CSettings(ctypes.Structure):
     """mcpm_settings: what readSettingsFile derives (myIO.cpp:1756-3016)."""
     _fields_ = [
@@ -249,6 +256,10 @@ def load_library() -> ctypes.CDLL:
         "mcpm_reset_samples": (i32, [vp]),
         "mcpm_measure_pressure": (i32, [vp, i32]),
         "mcpm_read_settings": (i32, [ctypes.c_char_p, ctypes.POINTER(CSettings)]),
+        "mcpm_create_batch": (i32, [ctypes.POINTER(CParams), i32, ctypes.POINTER(CReplica),
+                                    ctypes.POINTER(vp)]),
+        "mcpm_upload_batch": (i32, [vp, ctypes.POINTER(i64), dpp, dpp, dpp, dpp]),
+        "mcpm_init_omega_max_batch": (i32, [vp, dpp, i32]),
         "mcpm_initialize": (i32, [vp, ctypes.POINTER(CSettings)]),
         "mcpm_write_postproc": (i32, [vp, ctypes.POINTER(CSettings), i64, ctypes.c_char_p]),
         "mcpm_get_pressure": (i32, [vp, dpp, dpp, dpp]),
@@ -301,6 +312,44 @@ class Simulation:
         h = ctypes.c_void_p()
         self._check(self._lib.mcpm_create(ctypes.byref(self._cp), ctypes.byref(h)), "mcpm_create")
         self._h = h
+
+    @classmethod
+    def batch(cls, common: Params, replicas) -> "Simulation":
+        """mcpm_create_batch: independent slabs in one context; replicas is a
+        list of dicts with n_cells, capacity, seed, cell_volume, Fn."""
+        self = cls.__new__(cls)
+        self._lib = load_library()
+        reps = (CReplica * len(replicas))()
+        total = 0
+        for r, d in enumerate(replicas):
+            reps[r].n_cells = d["n_cells"]
+            reps[r].capacity = d["capacity"]
+            reps[r].seed = d["seed"]
+            reps[r].cell_volume = d["cell_volume"]
+            reps[r].Fn = d["Fn"]
+            total += d["n_cells"]
+        self._reps = reps
+        self.params = Params(**{f.name: getattr(common, f.name) for f in fields(Params)})
+        self.params.n_cells = total
+        self.params.capacity = sum(d["capacity"] for d in replicas)
+        self._cp = common.to_c()
+        h = ctypes.c_void_p()
+        self._check(self._lib.mcpm_create_batch(ctypes.byref(self._cp), len(replicas), reps,
+                                                ctypes.byref(h)), "mcpm_create_batch")
+        self._h = h
+        return self
+
+    def upload_batch(self, counts, y, u1, u2, u3):
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (y, u1, u2, u3)]
+        self._check(self._lib.mcpm_upload_batch(
+            self._h, counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+            *[_dp(a) for a in arrs]), "mcpm_upload_batch")
+
+    def init_omega_max_batch(self, n_gas, times_y: bool = True):
+        n_gas = np.ascontiguousarray(n_gas, dtype=np.float64)
+        self._check(self._lib.mcpm_init_omega_max_batch(self._h, _dp(n_gas), 1 if times_y else 0),
+                    "mcpm_init_omega_max_batch")
 
     # -- plumbing -----------------------------------------------------------
     def _check(self, st: int, what: str):

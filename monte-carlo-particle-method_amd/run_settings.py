@@ -12,8 +12,9 @@ time loop does (main.cpp:389-669):
     python run_settings.py injection.in --steps 200 --out PostProc
 
 The particles stay on the device; only the PostProc files come back.  The
-reference's thermostat for problem injection (main.cpp:429-437) is not part
-of this loop.
+reference's thermostat call for problem injection (main.cpp:429-437) does
+nothing for that problem (its body is commented out, Main_Functions.cpp:
+1679-1690), so the loop has no counterpart for it.
 """
 from __future__ import annotations
 

@@ -103,6 +103,35 @@ def interface(N=20_000_000, n_cells=4096, length=400.0, n_liq=0.55, n_vap=0.02, 
     return p, (y, u1, u2, u3), {"n_gas": n_gas, "T0": T, "N": N, "area": area}
 
 
+def replicas(n_rep=256, seed0=0, N_lo=50_000, N_hi=500_000, nliq_lo=0.45, nliq_hi=0.6,
+             T_lo=0.6, T_hi=0.9, cells_lo=512, cells_hi=2048, n_vap=0.02, length=400.0,
+             dt=0.02, precision=FP32, boundary=BC_PERIODIC, long_range=LR_SCREENED_POISSON):
+    """Independent interface slabs for one batched context: replica r has
+    seed seed0 + r, N log-uniform in [N_lo, N_hi], liquid density uniform in
+    [nliq_lo, nliq_hi], T uniform in [T_lo, T_hi], cells uniform in
+    {cells_lo..cells_hi}.  Returns (common params, replica dicts, per-replica
+    arrays, per-replica meta)."""
+    rng = np.random.Generator(np.random.Philox(10_000 + seed0))
+    reps, arrs, metas = [], [], []
+    common = None
+    for r in range(n_rep):
+        N = int(round(math.exp(rng.uniform(math.log(N_lo), math.log(N_hi)))))
+        n_liq = rng.uniform(nliq_lo, nliq_hi)
+        T = rng.uniform(T_lo, T_hi)
+        nc = int(rng.integers(cells_lo, cells_hi + 1))
+        p, a, meta = interface(N=N, n_cells=nc, length=length, n_liq=n_liq, n_vap=n_vap, T=T,
+                               dt=dt, precision=precision, seed=seed0 + r)
+        p.boundary = boundary
+        p.long_range = long_range
+        if common is None:
+            common = p
+        reps.append({"n_cells": nc, "capacity": N, "seed": seed0 + r,
+                     "cell_volume": p.cell_volume, "Fn": p.Fn, "params": p})
+        arrs.append(a)
+        metas.append(meta)
+    return common, reps, arrs, metas
+
+
 def plane(nx=40, ny=40, Lx=40.0, Ly=40.0, ppc=20, n_vap=0.1, n_liq=0.55, drops=((20.0, 20.0, 10.0),),
           T=0.835, dt=0.01, precision=FP64, seed=1, boundary=BC_SPECULAR, long_range=LR_DIRECT,
           model=MODEL_ESMC, lattice=False, capacity=None, phi=-4.0):
