@@ -42,9 +42,15 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-particles", type=int, default=20_000_000)
-    ap.add_argument("--workload", default="interface", choices=["interface", "droplets"],
+    ap.add_argument("--workload", default="interface",
+                    choices=["interface", "droplets", "replicas"],
                     help="interface: the periodic 400 sigma slab the metric is quoted on; "
-                         "droplets: the 512 x 512 periodic plane with 20 liquid discs")
+                         "droplets: the 512 x 512 periodic plane with 20 liquid discs; "
+                         "replicas: 256 independent interface slabs in one batched context")
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--unbatched-steps", type=int, default=2,
+                    help="replicas: also time this many steps of the same replicas run one "
+                         "context after the other")
     ap.add_argument("--pressure", action="store_true",
                     help="sample measure_pressure's face sums in every step (the reference "
                          "does once step > times * after)")
@@ -72,21 +78,58 @@ def main():
         del src, dst
 
     N = args.particles
-    if args.workload == "droplets":
+    unbatched = None
+    if args.workload == "replicas":
+        common, reps, rarrs, metas = W.replicas(n_rep=args.replicas, seed0=0)
+        counts = [a[0].size for a in rarrs]
+        N = int(sum(counts))
+        args.cells = int(sum(d["n_cells"] for d in reps))
+        log(f"workload: {len(reps)} replicas, N={N} cells={args.cells} FP32")
+        if args.unbatched_steps > 0:
+            sims = []
+            for d, a, m in zip(reps, rarrs, metas):
+                s1 = mcpm.Simulation(d["params"])
+                s1.upload(*a)
+                s1.cell_update()
+                s1.init_omega_max(m["n_gas"])
+                sims.append(s1)
+            for s1 in sims:
+                s1.step(0)
+            for s1 in sims:
+                s1.sync()
+            t0 = time.perf_counter()
+            for k in range(args.unbatched_steps):
+                for s1 in sims:
+                    s1.step(1 + k)
+            for s1 in sims:
+                s1.sync()
+            unbatched = 1000.0 * (time.perf_counter() - t0) / args.unbatched_steps
+            log(f"unbatched: {unbatched:.2f} ms per step of all replicas, one context after another")
+            del sims
+        sim = mcpm.Simulation.batch(common, reps)
+        sim.upload_batch(counts, *[np.concatenate([a[q] for a in rarrs]) for q in range(4)])
+        del rarrs
+        sim.cell_update()
+        sim.init_omega_max_batch([m["n_gas"] for m in metas])
+        p = common
+    elif args.workload == "droplets":
         p, arrs, meta = W.droplets(N=64_000_000 if N == 20_000_000 else N)
         N = meta["N"]
         args.cells = p.n_cells * p.n_cells_x
         log(f"workload: droplets N={N} cells={p.n_cells_x}x{p.n_cells} dt={p.delta_t} FP32")
         sim = mcpm.Simulation(p)
         sim.upload_xy(*arrs)
+        del arrs
+        sim.cell_update()
+        sim.init_omega_max(meta["n_gas"])
     else:
         p, arrs, meta = W.interface(N=N, n_cells=args.cells)
         log(f"workload: interface N={N} cells={args.cells} dt={p.delta_t} FP32")
         sim = mcpm.Simulation(p)
         sim.upload(*arrs)
-    del arrs
-    sim.cell_update()
-    sim.init_omega_max(meta["n_gas"])
+        del arrs
+        sim.cell_update()
+        sim.init_omega_max(meta["n_gas"])
     if args.pressure:
         sim.measure_pressure(True)
     step = 0
@@ -163,9 +206,14 @@ def main():
                 "step_frac": round(value * step_alg / 1e9 / HBM_PEAK_GBS, 4)}
 
     cpu = None
-    if not args.no_cpu_baseline and not plane:
+    if not args.no_cpu_baseline and args.workload == "interface":
         cpu = cpu_baseline(args)
-    if plane:
+    if args.workload == "replicas":
+        desc = (f"replicas: {args.replicas} independent periodic 400 sigma interface slabs in one "
+                f"batched context, {N} particles, {args.cells} cells; per replica N log-uniform "
+                "in [5e4, 5e5], n_liq in [0.45, 0.6], T in [0.6, 0.9], cells in [512, 2048], "
+                "seeds 0..255; ESMC + screened-Poisson, dt=0.02")
+    elif plane:
         desc = (f"droplets: periodic 128 x 128 sigma plane, {p.n_cells_x} x {p.n_cells} cells, "
                 f"{N} particles, 20 discs n=0.55 in vapour 0.02, T=0.75, ESMC + "
                 "screened-Poisson FEM, dt=0.01")
@@ -190,6 +238,7 @@ def main():
         "config": {"workload": desc,
                    "particles": N, "cells": args.cells, "parallelism": "1 GPU",
                    "pressure_sampling": bool(args.pressure),
+                   "unbatched_ms_per_step": unbatched,
                    "valid_ops_per_step": valid, "accepted_per_step": acc,
                    "reservation_rounds": rounds,
                    "phases_ms_per_step": {k: v["per_step_ms"] for k, v in timers.items()}},
