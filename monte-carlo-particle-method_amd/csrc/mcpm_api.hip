@@ -484,8 +484,9 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
     dim3 gs((nb + 255) / 256, nseg);
     hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
                        c->seg);
-    hipLaunchKernelGGL(k_scan_b, dim3(1), dim3(1024), 0, c->s, c->partial, nseg, nb, cnt,
-                       start, nden, yc, c->kp);
+    hipLaunchKernelGGL(k_scan_bins, dim3((nb + 255) / 256), dim3(256), 0, c->s, c->partial,
+                       nseg, nb, cnt, nden, yc, c->kp);
+    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, cnt, start, nb);
     hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
                        c->seg);
     hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,

@@ -75,8 +75,8 @@ __global__ void k_cell_bounds(const uint32_t *key, int64_t n, int nc, uint32_t *
 __global__ void k_cell_fill(const uint32_t *start, int nc, uint32_t *cnt, double *nden,
                             double *yc, KParams kp);
 __global__ void k_scan_a(const uint32_t *table, uint32_t *partial, int nchunks, int nc, int seg);
-__global__ void k_scan_b(uint32_t *partial, int nseg, int nc, uint32_t *cnt, uint32_t *start,
-                         double *nden, double *yc, KParams kp);
+__global__ void k_scan_bins(uint32_t *partial, int nseg, int nc, uint32_t *cnt, double *nden,
+                            double *yc, KParams kp);
 __global__ void k_scan_c(uint32_t *table, const uint32_t *partial, int nchunks, int nc, int seg);
 // variant: the mcpm_model of a COLL_DSMC collision (DSMC_VHS, DSMC_HS, CBA_VHS).
 __global__ void k_mcand(const uint32_t *cnt, const double *omax, const double *nden,

@@ -356,57 +356,32 @@ __global__ void k_scan_a(const uint32_t *__restrict__ table, uint32_t *__restric
   partial[(size_t)s * nc + c] = sum;
 }
 
-// One workgroup: per-cell exclusive scan over segments, cell counts, the
-// exclusive scan over cells (start), densities and contact values.
-__global__ __launch_bounds__(1024) void k_scan_b(uint32_t *__restrict__ partial, int nseg, int nc,
-                                                 uint32_t *__restrict__ cnt,
-                                                 uint32_t *__restrict__ start,
-                                                 double *__restrict__ nden,
-                                                 double *__restrict__ yc, KParams kp) {
-  __shared__ uint32_t sh[1024];
-  const int tid = threadIdx.x;
-  const int per = (nc + 1023) / 1024;
-  const int c0 = tid * per;
-  uint32_t tot = 0;
-  for (int k = 0; k < per; ++k) {
-    int c = c0 + k;
-    if (c >= nc) break;
-    uint32_t run = 0;
-    for (int s = 0; s < nseg; ++s) {
-      uint32_t v = partial[(size_t)s * nc + c];
-      partial[(size_t)s * nc + c] = run;
-      run += v;
-    }
-    cnt[c] = run;
-    tot += run;
-    if (nden) {
-      const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
-      double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
-      nden[c] = nd;
-      yc[c] = contact_y(nd, kq.b, kq.y_model);
-    }
+// Per bin (one thread each): exclusive scan over the segments of chunks, the
+// bin's count and, for cells, the density and contact value; the starts are
+// then one exclusive scan of the counts (k_excl_scan).
+__global__ void k_scan_bins(uint32_t *__restrict__ partial, int nseg, int nc,
+                            uint32_t *__restrict__ cnt, double *__restrict__ nden,
+                            double *__restrict__ yc, KParams kp) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nc) return;
+  uint32_t run = 0;
+  for (int s = 0; s < nseg; ++s) {
+    const uint32_t v = partial[(size_t)s * nc + c];
+    partial[(size_t)s * nc + c] = run;
+    run += v;
   }
-  sh[tid] = tot;
-  __syncthreads();
-  for (int off = 1; off < 1024; off <<= 1) {
-    uint32_t v = (tid >= off) ? sh[tid - off] : 0u;
-    __syncthreads();
-    sh[tid] += v;
-    __syncthreads();
+  cnt[c] = run;
+  if (nden) {
+    const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
+    double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
+    nden[c] = nd;
+    yc[c] = contact_y(nd, kq.b, kq.y_model);
   }
-  uint32_t run = (tid == 0) ? 0u : sh[tid - 1];
-  for (int k = 0; k < per; ++k) {
-    int c = c0 + k;
-    if (c >= nc) break;
-    start[c] = run;
-    run += cnt[c];
-  }
-  if (tid == 1023) start[nc] = sh[1023];
 }
 
 // Cells of the two-digit sort: start[c] = first particle of cell c from the
 // sorted keys (every c in [0, nc] written once when the keys ascend, which the
-// sort guarantees), then counts, densities and contact values as k_scan_b.
+// sort guarantees), then counts, densities and contact values as k_scan_bins.
 // start is zeroed first and a descent raises ERRBIT_ORDER, so even keys out
 // of order leave every start in [0, n] and every count within the array.
 __global__ void k_cell_bounds(const uint32_t *__restrict__ key, int64_t n, int nc,
