@@ -53,7 +53,8 @@ struct mcpm_ctx {
   bool pressure_on = false;
   int64_t n_samples = 0;  // cell_poly_velocity calls since the last reset
   uint32_t *counters = nullptr, *err = nullptr;
-  uint32_t *res = nullptr;  // reservations, two interleaved arrays
+  uint64_t *res = nullptr;  // reservations, two interleaved arrays
+  uint32_t round_base = 0;  // rounds run so far: the next round's tag is ~round_base
   // collisions
   uint4 *opA = nullptr, *opB = nullptr;  // one allocation: ping-pong op lists,
   uint4 *opBand = nullptr;               // then the priority-band lists
@@ -594,6 +595,16 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256),
                                               c->round_grid);
   grid = (grid + 7u) / 8u * 8u;
+  // reservation tags count down: round r of this call has tag ~round_base
+  // (round_base advanced once per launched round), and one tag is skipped
+  // first, the one the previous call's last round may have reserved with.
+  // Long before they run out the words are reset to all-ones.
+  if (c->round_base > 0xFFFFFFFFu - 2u * (uint32_t)MAX_ROUNDS - 8u) {
+    HIPCHK(hipMemsetAsync(c->res, 0xFF, 2 * (size_t)c->p.capacity * sizeof(uint64_t), c->s));
+    c->round_base = 0;
+  }
+  c->round_base += 1u;
+  const uint32_t tag0 = ~c->round_base;
   if (M > 0) {
     Timer tm(c, T_GEN);
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
@@ -605,7 +616,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
     hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nchunks), dim3(256), 0, c->s, P, c->cnt, c->start,
                        c->cstart, nc, M, c->kp, slo, shi, pk.v[0], band_mul, nbands,
-                       c->gen_table, c->opBand, c->res, c->counters);
+                       c->gen_table, c->opBand, c->res, tag0, c->counters);
     LAUNCHCHK();
   }
   int rounds = 0;
@@ -633,10 +644,12 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
         hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
                            P, (const uint4 *)((r & 1) ? c->opB : c->opA),
                            (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
-                           c->res + (r & 1), c->res + ((r + 1) & 1), c->counters, c->nden,
+                           c->res + (r & 1), c->res + ((r + 1) & 1), ~c->round_base,
+                           c->counters, c->nden,
                            c->yc, c->omax, c->omega_new, c->kp, slo, shi, fpp, lg,
                            c->dleft);
       }
+      c->round_base += 1u;
       LAUNCHCHK();
       rounds = r + 1;
       // examine the counts published so far (round q publishes hleft[q]);
@@ -1048,7 +1061,7 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
       hipMemset(c->err, 0, sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->cnt, 0, nc * sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->start, 0, (nc + 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMemset(c->res, 0xFF, 2 * (size_t)p->capacity * sizeof(uint32_t)) != hipSuccess)
+      hipMemset(c->res, 0xFF, 2 * (size_t)p->capacity * sizeof(uint64_t)) != hipSuccess)
     return fail(MCPM_ERR_HIP);
   if (p->dims == 2) {
     if ((st = build_fem(c))) return fail(st);

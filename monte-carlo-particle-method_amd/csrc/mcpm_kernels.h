@@ -102,12 +102,13 @@ template <typename R, int MODEL>
 __global__ void k_gen(const R *P, const uint32_t *cnt, const uint32_t *start,
                       const uint32_t *cstart, int nc, uint32_t M, KParams kp, uint32_t step_lo,
                       uint32_t step_hi, uint32_t prio_key, uint32_t band_mul, int nbands,
-                      const uint32_t *table, uint4 *bands, uint32_t *res0,
+                      const uint32_t *table, uint4 *bands, uint64_t *res0, uint32_t tag0,
                       uint32_t *counters);
 template <typename R, int MODEL>
 __global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
                         const uint4 *bands, uint4 *out,
-                        uint32_t *res_cur, uint32_t *res_next, uint32_t *counters,
+                        const uint64_t *res_cur, uint64_t *res_next, uint32_t tag,
+                        uint32_t *counters,
                         const double *nden, const double *yc, const double *omax,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,
                         FPParams fpp, OpLog log, uint32_t *host_left);

@@ -666,13 +666,21 @@ __global__ void k_list_starts(uint32_t *__restrict__ counters, int nl) {
   counters[CNT_BSTART + nl] = run;
 }
 
+// A reservation word: the round's tag above the operation's priority.  Tags
+// count down from one round to the next (and across steps), so a fresh
+// reservation is below every word left from an earlier round: the arrays
+// need no clearing between rounds.
+__device__ __forceinline__ uint64_t res_word(uint32_t tag, uint32_t pr) {
+  return ((uint64_t)tag << 32) | (uint64_t)pr;
+}
+
 // Candidate generation, one chunk of GEN_CHUNK candidate indices per
 // workgroup (cell order, so a chunk covers one or a few cells).  Each
 // candidate k of cell c (t = k - cstart[c]) is drawn, given its priority
 // and written to its (band, region) list as {i, j, priority, t}, or as
 // OP_NONE when it has no partner; band 0 reserves its particles for the
-// first round.  Reservations of particle i live at res[2 i] (even rounds)
-// and res[2 i + 1] (odd rounds).
+// first round (tag tag0).  Reservations of particle i live at res[2 i] (even
+// rounds) and res[2 i + 1] (odd rounds).
 template <typename R, int MODEL>
 __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              const uint32_t *__restrict__ cnt,
@@ -683,7 +691,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              uint32_t band_mul, int nbands,
                                              const uint32_t *__restrict__ table,
                                              uint4 *__restrict__ bands,
-                                             uint32_t *__restrict__ res0,
+                                             uint64_t *__restrict__ res0, uint32_t tag0,
                                              uint32_t *__restrict__ counters) {
   __shared__ uint32_t spos[NLIST];
   __shared__ uint32_t scell[1];
@@ -734,8 +742,10 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
       if (ok) {
         ++nvalid;
         if (band == 0) {
-          atomicMin(&res0[2 * (size_t)i], pr);
-          atomicMin(&res0[2 * (size_t)j], pr);
+          atomicMin((unsigned long long *)&res0[2 * (size_t)i],
+                    (unsigned long long)res_word(tag0, pr));
+          atomicMin((unsigned long long *)&res0[2 * (size_t)j],
+                    (unsigned long long)res_word(tag0, pr));
         }
       }
       (void)t;
@@ -870,9 +880,10 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t k, 
 // when every earlier-priority operation sharing a particle has committed:
 // the result equals processing all operations one by one in priority order.
 // Losers reserve in the other array for the next round and are compacted
-// into the next operation list.  Each operation that finds its own priority
-// in the current array clears it, so the array is back to all-ones when the
-// round ends.  The two arrays are interleaved (particle i at res[2 i] and
+// into the next operation list.  Round r's words carry tag tag (the next
+// round's tag - 1 is smaller), so words left from earlier rounds never win
+// and nothing is cleared: the round writes no reservation word except by
+// reserving.  The two arrays are interleaved (particle i at res[2 i] and
 // res[2 i + 1]): a loser's read and its next reservation share a cache line.
 // Operations enter by priority band: round r takes the losers of round r-1
 // plus band r, and reserves band r+1 for round r+1.  Every operation of a
@@ -889,8 +900,8 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
                                                const uint4 *__restrict__ in,
                                                const uint4 *__restrict__ bands,
                                                uint4 *__restrict__ out,
-                                               uint32_t *__restrict__ res_cur,
-                                               uint32_t *__restrict__ res_next,
+                                               const uint64_t *__restrict__ res_cur,
+                                               uint64_t *__restrict__ res_next, uint32_t tag,
                                                uint32_t *__restrict__ counters,
                                                const double *__restrict__ nden,
                                                const double *__restrict__ yc,
@@ -928,8 +939,9 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
   for (uint32_t idx = lb * blockDim.x + threadIdx.x; idx < countC; idx += stride) {
     const uint4 rec = inC[idx];
     if (rec.x == OP_NONE) continue;
-    atomicMin(&res_next[2 * (size_t)rec.x], rec.z);
-    atomicMin(&res_next[2 * (size_t)rec.y], rec.z);
+    const unsigned long long w = res_word(tag - 1u, rec.z);
+    atomicMin((unsigned long long *)&res_next[2 * (size_t)rec.x], w);
+    atomicMin((unsigned long long *)&res_next[2 * (size_t)rec.y], w);
   }
   // U operations per thread per tile: their loads are issued together
   constexpr int U = 4;
@@ -937,7 +949,7 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
   uint32_t acc_local = 0;
   for (uint32_t base = lb * tile; base < count; base += nbx * tile) {
     uint4 rec[U];
-    uint32_t ri[U], rj[U];
+    uint64_t ri[U], rj[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t idx = base + u * blockDim.x + threadIdx.x;
@@ -958,7 +970,8 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool valid = rec[u].x != OP_NONE;
-      win[u] = valid && ri[u] == rec[u].z && rj[u] == rec[u].z;
+      const uint64_t mine = res_word(tag, rec[u].z);
+      win[u] = valid && ri[u] == mine && rj[u] == mine;
       lose[u] = valid && !win[u];
       if (win[u]) {
         pa[u] = P[rec[u].x];
@@ -982,13 +995,10 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
           log.j[q] = j;
           log.a[q] = acc ? 1 : 0;
         }
-        res_cur[2 * (size_t)i] = 0xFFFFFFFFu;
-        res_cur[2 * (size_t)j] = 0xFFFFFFFFu;
       } else if (lose[u]) {
-        if (ri[u] == pr) res_cur[2 * (size_t)i] = 0xFFFFFFFFu;
-        if (rj[u] == pr) res_cur[2 * (size_t)j] = 0xFFFFFFFFu;
-        atomicMin(&res_next[2 * (size_t)i], pr);
-        atomicMin(&res_next[2 * (size_t)j], pr);
+        const unsigned long long w = res_word(tag - 1u, pr);
+        atomicMin((unsigned long long *)&res_next[2 * (size_t)i], w);
+        atomicMin((unsigned long long *)&res_next[2 * (size_t)j], w);
       }
     }
     // compact the tile's losers: ranks in the wavefront, one LDS atomic per
@@ -1491,10 +1501,11 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
   template __global__ void k_gen<R, M>(const R *, const uint32_t *, const uint32_t *,      \
                                        const uint32_t *, int, uint32_t, KParams, uint32_t, \
                                        uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
-                                       uint4 *, uint32_t *, uint32_t *);                   \
+                                       uint4 *, uint64_t *, uint32_t, uint32_t *);         \
   template __global__ void k_round<R, M>(int, int, uint32_t, R *, const uint4 *,           \
                                          const uint4 *, uint4 *,                           \
-                                         uint32_t *, uint32_t *, uint32_t *,               \
+                                         const uint64_t *, uint64_t *, uint32_t,           \
+                                         uint32_t *,                                       \
                                          const double *, const double *, const double *,   \
                                          double *, KParams, uint32_t, uint32_t, FPParams,  \
                                          OpLog, uint32_t *);
