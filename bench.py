@@ -171,6 +171,8 @@ def main():
     acc = float(np.mean([s["accepted"] for s in stats]))
     valid = float(np.mean([s["valid"] for s in stats]))
     rounds = float(np.mean([s["rounds"] for s in stats]))
+    revisits = float(np.mean([s["revisits"] for s in stats]))
+    nbands = sim.get_tuning(mcpm.TUNE_BANDS)
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, {value:.4g} particle-steps/s")
     log(f"phases: {json.dumps(timers)}; valid ops {valid:.0f}, accepted {acc:.0f}, rounds {rounds}")
 
@@ -204,6 +206,9 @@ def main():
                 "alg_bytes_per_launch": round(dom_bytes / max(launches, 1.0)),
                 "step_alg_bytes_per_particle": step_alg,
                 "step_frac": round(value * step_alg / 1e9 / HBM_PEAK_GBS, 4)}
+    if dom == "k_round":
+        roofline["random_access"] = random_access_roofline(
+            N, int(rec), valid, acc, revisits, nbands, coll_ms, launches)
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "interface":
@@ -246,6 +251,36 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def random_access_roofline(n, rec, valid, acc, revisits, nbands, coll_ms, launches):
+    """The reservation rounds against the device's random-access rates.
+
+    Every access of an operation goes to an unpredictable particle, so the
+    rounds are priced in accesses at the rates mcpm_probe_random_access
+    measures on arrays of the same shape, summed over the access kinds (they
+    share one memory system).  Per step, in k_round: 2 reservation atomics
+    per valid operation outside band 0 (band 0 reserves in k_gen) and per
+    revisit, 2 reservation-word reads per visit (operations + revisits), 2
+    record reads per operation (its one winning visit), 2 record writes per
+    accepted collision, and the 16-byte operation records streamed in (each
+    visit) and out (each revisit) at the HBM peak."""
+    import mcpm
+    rates = mcpm.probe_random_access(n, rec)
+    visits = valid + revisits
+    counts = {"atomic_min_word": 2.0 * valid * (1.0 - 1.0 / max(nbands, 1)) + 2.0 * revisits,
+              "gather_word": 2.0 * visits,
+              "gather_record": 2.0 * valid,
+              "scatter_record": 2.0 * acc}
+    floor_s = sum(counts[k] / rates[k] for k in counts)
+    floor_s += 16.0 * (visits + revisits) / (HBM_PEAK_GBS * 1e9)
+    floor_ms = 1000.0 * floor_s
+    return {"rates_G_per_s": {k: round(v / 1e9, 2) for k, v in rates.items()},
+            "accesses_per_step": {k: round(v) for k, v in counts.items()},
+            "visits_per_step": round(visits),
+            "floor_ms_per_step": round(floor_ms, 3),
+            "floor_us_per_launch": round(1000.0 * floor_ms / max(launches, 1.0), 2),
+            "frac": round(floor_ms / coll_ms, 4)}
 
 
 def measured_traffic(kernel, particles, cells):

@@ -131,7 +131,8 @@ typedef struct mcpm_collision_stats {
   int64_t valid;         /* candidates with a partner j != i                    */
   int64_t accepted;      /* collisions performed                                */
   int32_t rounds;        /* parallel rounds the reservation schedule took       */
-  int32_t reserved0;
+  int32_t revisits;      /* operations carried into a later round, summed over
+                            the rounds (0 where operations run one by one)    */
 } mcpm_collision_stats;
 
 /* ---- lifetime ---------------------------------------------------------- */
@@ -426,6 +427,22 @@ int mcpm_get_collision_log(mcpm_ctx *ctx, int64_t cap, uint32_t *prio, uint32_t 
    4 whole step. */
 int mcpm_timing_enable(mcpm_ctx *ctx, int32_t enable);
 int mcpm_timing_get(mcpm_ctx *ctx, int32_t which, double *avg_ms, int64_t *calls);
+
+/* ---- device random-access rates ------------------------------------------- */
+/* No counterpart in the reference: a measurement of the current device for
+   the roofline of the reservation rounds, whose accesses all go to
+   unpredictable particles.  Over n_records records of record_bytes (16 or 32)
+   and 2 n_records 8-byte words, with indices hashed uniformly, the best of
+   five launches of min(n_records, 2^24) accesses each, in accesses per
+   second: record gathers, word gathers, 64-bit atomic minimums (result
+   unused) and record stores.  Allocates and frees its own arrays. */
+typedef struct mcpm_access_rates {
+  double gather_record;
+  double gather_word;
+  double atomic_min_word;
+  double scatter_record;
+} mcpm_access_rates;
+int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes, mcpm_access_rates *out);
 
 /* ---- launch-shape knobs ---------------------------------------------------- */
 /* No counterpart in the reference (its loops are serial): the tile and

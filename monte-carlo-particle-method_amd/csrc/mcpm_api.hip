@@ -63,7 +63,7 @@ struct mcpm_ctx {
   // round termination: round r + 1 writes the losers left after round r into
   // pinned host memory, which the host polls without touching the stream
   uint32_t *hleft = nullptr, *dleft = nullptr;
-  int round_grid = 2048;
+  int round_grid = 1024;
   int sort_chunk = 0;
   OpLog log{};                           // one allocation: debug log
   int64_t op_cap = 0;
@@ -621,15 +621,6 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   }
   int rounds = 0;
   const uint32_t seg = (M + NREG - 1) / NREG;  // loser-list segment of one region
-  auto left_after = [&](int rr, uint32_t *left) -> int {
-    uint32_t v[NREG];
-    HIPCHK(hipMemcpyAsync(v, c->counters + CNT_LIST + rr * NREG, sizeof v, hipMemcpyDeviceToHost,
-                          c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    *left = 0;
-    for (int x = 0; x < NREG; ++x) *left += v[x];
-    return MCPM_OK;
-  };
   {
     OpLog lg = c->debug_log ? c->log : OpLog{};
     // hleft[q] = losers after round q - 1, written by round q; unset = ~0
@@ -683,12 +674,16 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   hipLaunchKernelGGL(k_omega_finish, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->omax,
                      c->omega_new, c->cnt, nc, (int)MODEL, variant);
   LAUNCHCHK();
-  uint32_t hc[CNT_LIST + 2];
-  HIPCHK(hipMemcpyAsync(hc, c->counters, sizeof hc, hipMemcpyDeviceToHost, c->s));
+  // the counters up to the loser counts of the last round
+  std::vector<uint32_t> hc(CNT_LIST + (size_t)(rounds + 1) * NREG);
+  HIPCHK(hipMemcpyAsync(hc.data(), c->counters, hc.size() * sizeof(uint32_t),
+                        hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   uint32_t left = 0;
-  if ((st = left_after(rounds, &left))) return st;
+  for (int x = 0; x < NREG; ++x) left += hc[CNT_LIST + (size_t)rounds * NREG + x];
   if (left != 0) return MCPM_ERR_CAPACITY;  // more rounds than MAX_ROUNDS
+  int64_t revisits = 0;
+  for (size_t q = CNT_LIST + NREG; q < hc.size(); ++q) revisits += hc[q];
   c->last_ops = hc[CNT_OPS];
   c->last_log = hc[CNT_LOG];
   if (stats) {
@@ -698,7 +693,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     memcpy(&acc, &hc[CNT_ACC], sizeof acc);
     stats->accepted = (int64_t)acc;
     stats->rounds = rounds;
-    stats->reserved0 = 0;
+    stats->revisits = (int32_t)std::min<int64_t>(revisits, INT32_MAX);
   }
   return MCPM_OK;
 }
@@ -740,7 +735,7 @@ int collide_cba_hs_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     memcpy(&acc, &hc[CNT_ACC], sizeof acc);
     stats->accepted = (int64_t)acc;
     stats->rounds = 0;
-    stats->reserved0 = 0;
+    stats->revisits = 0;
   }
   return MCPM_OK;
 }

@@ -211,7 +211,7 @@ extern "C" int mcpm_read_settings(const char *path, mcpm_settings *s) {
   int64_t Nh = 0;
   double drop_dia = P.get("drop_dia", 0.0);
   double nc = P.get("nc", 0.0), nh = P.get("nh", 0.0);
-  const double Tc = P.get("Tc", 0.0), Th = P.get("Th", 0.0);
+  const double Tc = P.get("Tc", 0.0);  // Th is read but used by none of these problems
   if (problem == 12) {
     // injection (myIO.cpp:2729-2761)
     sigma = 3.405e-10;

@@ -71,14 +71,19 @@ class CParams(ctypes.Structure):
     ]
 
 
+class CAccessRates(ctypes.Structure):
+    _fields_ = [("gather_record", ctypes.c_double), ("gather_word", ctypes.c_double),
+                ("atomic_min_word", ctypes.c_double), ("scatter_record", ctypes.c_double)]
+
+
 class CStats(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_int64), ("valid", ctypes.c_int64),
                 ("accepted", ctypes.c_int64), ("rounds", ctypes.c_int32),
-                ("reserved0", ctypes.c_int32)]
+                ("revisits", ctypes.c_int32)]
 
     def as_dict(self):
         return {"candidates": self.candidates, "valid": self.valid,
-                "accepted": self.accepted, "rounds": self.rounds}
+                "accepted": self.accepted, "rounds": self.rounds, "revisits": self.revisits}
 
 
 @dataclass
@@ -282,6 +287,7 @@ def load_library() -> ctypes.CDLL:
         "mcpm_get_kde": (i32, [vp, dpp, dpp, dpp]),
         "mcpm_set_tuning": (i32, [vp, i32, i64]),
         "mcpm_get_tuning": (i32, [vp, i32, ctypes.POINTER(i64)]),
+        "mcpm_probe_random_access": (i32, [i64, i32, ctypes.POINTER(CAccessRates)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -296,6 +302,18 @@ def contact_y(p: Params, n: float) -> float:
     lib = load_library()
     cp = p.to_c()
     return lib.mcpm_contact_y(ctypes.byref(cp), float(n))
+
+
+def probe_random_access(n_records: int, record_bytes: int) -> dict:
+    """Random-access rates of the current device (accesses per second) over
+    n_records records of record_bytes and 2 n_records 8-byte words: record
+    gathers, word gathers, 64-bit atomic minimums, record stores."""
+    lib = load_library()
+    r = CAccessRates()
+    st = lib.mcpm_probe_random_access(int(n_records), int(record_bytes), ctypes.byref(r))
+    if st != 0:
+        raise McpmError(st, "probe_random_access")
+    return {f: getattr(r, f) for f, _ in CAccessRates._fields_}
 
 
 TIMER_COLLIDE, TIMER_SORT, TIMER_GEN, TIMER_FIELD, TIMER_STEP = 0, 1, 2, 3, 4

@@ -544,7 +544,7 @@ void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
     st->valid = (int64_t)order.size();
     st->accepted = acc;
     st->rounds = 0;
-    st->reserved0 = 0;
+    st->revisits = 0;
   }
 }
 
@@ -621,7 +621,7 @@ void orc_collide_mt(Oracle &o, int64_t step, mcpm_collision_stats *st) {
     st->valid = valid;
     st->accepted = acc;
     st->rounds = 0;
-    st->reserved0 = 0;
+    st->revisits = 0;
   }
 }
 
@@ -1315,7 +1315,7 @@ void orc_dsmc(Oracle &o, int64_t step, mcpm_collision_stats *st, int variant) {
     st->valid = valid;
     st->accepted = acc;
     st->rounds = 0;
-    st->reserved0 = 0;
+    st->revisits = 0;
   }
 }
 
@@ -1383,7 +1383,7 @@ void orc_cba_hs(Oracle &o, int64_t step, mcpm_collision_stats *st) {
     st->valid = M;
     st->accepted = acc;
     st->rounds = 0;
-    st->reserved0 = 0;
+    st->revisits = 0;
   }
 }
 

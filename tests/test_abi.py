@@ -58,3 +58,11 @@ def test_params_struct_layout_matches_header(tmp_path):
     for f in fields:
         assert getattr(mcpm.CParams, f).offset == int(got[f]), f
     assert ctypes.sizeof(mcpm.CParams) == int(got["sizeof"])
+
+
+def test_probe_rejects_bad_arguments_before_touching_the_device():
+    lib = mcpm.load_library()
+    r = mcpm.CAccessRates()
+    assert lib.mcpm_probe_random_access(1 << 20, 24, ctypes.byref(r)) == 1
+    assert lib.mcpm_probe_random_access(100, 16, ctypes.byref(r)) == 1
+    assert lib.mcpm_probe_random_access(1 << 20, 16, None) == 1
