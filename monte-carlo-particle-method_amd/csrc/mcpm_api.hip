@@ -478,8 +478,8 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                   uint32_t *cnt, uint32_t *start, double *nden, double *yc, uint32_t *okey,
                   const uint32_t *ikey) -> int {
     const size_t lds = (size_t)nb * sizeof(uint32_t);
-    hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64), lds, c->s, in,
-                       nullptr, c->n, chunk, c->kp, kick, drift, pmode, slo, shi, c->table,
+    hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
+                       c->s, in, nullptr, c->n, chunk, c->kp, kick, drift, pmode, slo, shi, c->table,
                        nullptr, c->err, nb, kmode, c->kdiv, nullptr, ikey);
     LAUNCHCHK();
     dim3 gs((nb + 255) / 256, nseg);
@@ -490,7 +490,8 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
     hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, cnt, start, nb);
     hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
                        c->seg);
-    hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64), lds, c->s, in, out,
+    hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
+                       c->s, in, out,
                        c->n, chunk, c->kp, ks, drift, pmode, slo, shi, c->table, start,
                        c->err, nb, kmode, c->kdiv, okey, ikey);
     LAUNCHCHK();

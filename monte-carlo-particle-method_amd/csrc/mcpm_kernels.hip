@@ -51,16 +51,18 @@ __device__ __forceinline__ void pressure_faces(double *pm, const KParams &kp, do
 // a particle past a wall is put on the wall, re-emitted with a half-range
 // Maxwellian at the wall temperature, and flies the time it spent past the
 // wall with the new velocity.  Draws: Philox(slot, bounce, step, WALL).
+// With have_f2 the caller has read kick.F2 at the particle's cell into f2.
 template <typename real>
 __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const KParams &kp,
                                               const Kick &kick,
                                               const double *__restrict__ drift, int mode,
                                               uint32_t step_lo, uint32_t step_hi,
-                                              uint32_t *err) {
+                                              uint32_t *err, double f2 = 0.0,
+                                              bool have_f2 = false) {
   double y = (double)r.y;
   if (mode & PUSH_KICK) {
-    int32_t c = cell_of(y, kp.dy, kp.n_cells);
-    r.u2 = (real)((double)r.u2 + kick.F2[c] * kp.dt);
+    if (!have_f2) f2 = kick.F2[cell_of(y, kp.dy, kp.n_cells)];
+    r.u2 = (real)((double)r.u2 + f2 * kp.dt);
   }
   if (!(mode & PUSH_MOVE)) return r;
   double v = (double)r.u2;
@@ -179,8 +181,9 @@ __device__ __forceinline__ Rec2<real> push_one(Rec2<real> r, uint64_t slot, cons
                                                const Kick &kick,
                                                const double *__restrict__ drift, int mode,
                                                uint32_t step_lo, uint32_t step_hi,
-                                               uint32_t *err) {
-  (void)slot; (void)drift; (void)step_lo; (void)step_hi;
+                                               uint32_t *err, double f2 = 0.0,
+                                               bool have_f2 = false) {
+  (void)slot; (void)drift; (void)step_lo; (void)step_hi; (void)f2; (void)have_f2;
   if (mode & PUSH_KICK) {
     const int32_t cx = cell_of((double)r.x, kp.dx, kp.nx);
     const int32_t cy = cell_of((double)r.y, kp.dy, kp.ny);
@@ -233,12 +236,14 @@ __device__ __forceinline__ bool in_domain(const Rec2<real> &r, const KParams &kp
 }
 
 // ---------------------------------------------------------------------------
-// Stable counting sort, pass 1 and pass 3.  One wavefront per chunk of
-// consecutive particles; the chunk's cell histogram lives in LDS.
+// Stable counting sort, pass 1 and pass 3.  One workgroup of PUSH_WAVES
+// wavefronts per chunk of consecutive particles; the chunk's cell histogram
+// lives in LDS, shared by the wavefronts.
 //   SCATTER=false: count particles per new cell -> table[chunk][cell]
 //   SCATTER=true : running[cell] = start[cell] + table[chunk][cell] (exclusive
 //                  over chunks) and each particle lands at running + rank, the
-//                  rank taken among equal keys in the wavefront with ballots.
+//                  rank taken among equal keys in the wavefront with ballots,
+//                  the wavefronts of a tile in turn.
 // Particles of a chunk are processed in index order, so equal keys keep their
 // relative order: the sort is stable.
 // The bins are the cells (kmode 0) or, for more cells than one LDS histogram
@@ -254,42 +259,65 @@ __device__ __forceinline__ int32_t sort_bin(int32_t key, int kmode, int32_t kdiv
 }
 
 template <typename R, bool SCATTER>
-__global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__restrict__ out,
-                                             int64_t n, int32_t chunk, KParams kp, Kick kick,
-                                             const double *__restrict__ drift, int mode,
-                                             uint32_t step_lo, uint32_t step_hi,
-                                             uint32_t *__restrict__ table,
-                                             const uint32_t *__restrict__ start,
-                                             uint32_t *err, int nb, int kmode, int32_t kdiv,
-                                             uint32_t *__restrict__ okey,
-                                             const uint32_t *__restrict__ ikey) {
+__global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
+    const R *__restrict__ in, R *__restrict__ out, int64_t n, int32_t chunk, KParams kp,
+    Kick kick, const double *__restrict__ drift, int mode, uint32_t step_lo, uint32_t step_hi,
+    uint32_t *__restrict__ table, const uint32_t *__restrict__ start, uint32_t *err, int nb,
+    int kmode, int32_t kdiv, uint32_t *__restrict__ okey, const uint32_t *__restrict__ ikey) {
   extern __shared__ uint32_t lds_cnt[];
+  constexpr int TILE = 64 * PUSH_WAVES;  // particles of one tile, wavefront w the w-th 64
   const int nc = nb;
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int64_t b = blockIdx.x;
   const int64_t beg = b * (int64_t)chunk;
   int64_t end = beg + chunk;
   if (end > n) end = n;
   uint32_t *trow = table + (size_t)b * (size_t)nc;
   if (SCATTER) {
-    for (int c = lane; c < nc; c += 64) lds_cnt[c] = start[c] + trow[c];
+    for (int c = threadIdx.x; c < nc; c += TILE) lds_cnt[c] = start[c] + trow[c];
   } else {
-    for (int c = lane; c < nc; c += 64) lds_cnt[c] = 0u;
+    for (int c = threadIdx.x; c < nc; c += TILE) lds_cnt[c] = 0u;
   }
   __syncthreads();
   const uint64_t lt = lanemask_lt();
-  constexpr int U = 4;  // tiles loaded ahead
-  for (int64_t t0 = beg; t0 < end; t0 += 64 * U) {
+  // U tiles per batch; the next batch's records are loaded while this
+  // batch is processed.  Loads complete in issue order, so the slab kick's
+  // field values, the one other read of the common path, are read first.
+  constexpr int U = 4;
+  const bool pre = R::dims == 1 && (mode & PUSH_KICK) && kp.nrep == 0 && !ikey;
+  const int64_t mine0 = (int64_t)wave * 64 + lane;  // this lane's offset inside a tile
+  // (reads past the chunk's end repeat its last record, never used)
+  R nx[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int64_t p = beg + u * TILE + mine0;
+    nx[u] = in[p < end ? p : end - 1];
+  }
+  for (int64_t t0 = beg; t0 < end; t0 += (int64_t)TILE * U) {
     R rr[U];
+    double f2v[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      int64_t p = t0 + u * 64 + lane;
-      if (p < end) rr[u] = in[p];
+      rr[u] = nx[u];
+      f2v[u] = 0.0;
+    }
+    if (pre) {  // uniform: one block of U reads, one wait
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int32_t c =
+            t0 + u * TILE + mine0 < end ? cell_of((double)rr[u].y, kp.dy, kp.n_cells) : 0;
+        f2v[u] = kick.F2[c];
+      }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      int64_t p = t0 + u * 64 + lane;
-      bool valid = p < end;
+      const int64_t p = t0 + (int64_t)TILE * U + u * TILE + mine0;
+      nx[u] = in[p < end ? p : end - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t p = t0 + u * TILE + mine0;
+      const bool valid = p < end;
       int32_t key = 0, ckey = 0;
       R r = rr[u];
       if (valid) {
@@ -298,7 +326,8 @@ __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__rest
           ckey = (int32_t)ikey[p];
         } else if (kp.nrep == 0) {
           if ((mode & PUSH_CHECK) && !in_domain(r, kp)) atomicOr(err, ERRBIT_DOMAIN);
-          r = push_one(r, (uint64_t)p, kp, kick, drift, mode, step_lo, step_hi, err);
+          r = push_one(r, (uint64_t)p, kp, kick, drift, mode, step_lo, step_hi, err, f2v[u],
+                       pre);
           ckey = cell_key(r, kp);
         } else {
           // a batched replica: its own geometry, Philox key, particle slots
@@ -316,31 +345,46 @@ __global__ __launch_bounds__(64) void k_push(const R *__restrict__ in, R *__rest
         }
         key = sort_bin(ckey, kmode, kdiv);
       }
-      uint64_t active = __ballot(valid);
-      while (active) {
-        int leader = __ffsll((unsigned long long)active) - 1;
-        int lkey = __shfl(key, leader);
-        bool mine = valid && key == lkey;
-        uint64_t m = __ballot(mine);
-        if (SCATTER) {
-          if (mine) {
-            uint32_t dst = lds_cnt[lkey] + (uint32_t)__popcll(m & lt);
-            out[dst] = r;
-            if (okey) okey[dst] = (uint32_t)ckey;
+      if (SCATTER) {
+        // the tile's wavefronts take turns, in particle order: equal keys
+        // land in index order (the sort is stable)
+        for (int w = 0; w < PUSH_WAVES; ++w) {
+          if (wave == w) {
+            uint64_t active = __ballot(valid);
+            while (active) {
+              const int leader = __ffsll((unsigned long long)active) - 1;
+              const int lkey = __shfl(key, leader);
+              const bool mine = valid && key == lkey;
+              const uint64_t m = __ballot(mine);
+              if (mine) {
+                const uint32_t dst = lds_cnt[lkey] + (uint32_t)__popcll(m & lt);
+                out[dst] = r;
+                if (okey) okey[dst] = (uint32_t)ckey;
+              }
+              __builtin_amdgcn_wave_barrier();
+              if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
+              __builtin_amdgcn_wave_barrier();
+              active &= ~m;
+            }
           }
-          __builtin_amdgcn_wave_barrier();
-          if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
-          __builtin_amdgcn_wave_barrier();
-        } else {
-          if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
+          __syncthreads();
         }
-        active &= ~m;
+      } else {
+        // counts only: the order of the additions does not matter
+        uint64_t active = __ballot(valid);
+        while (active) {
+          const int leader = __ffsll((unsigned long long)active) - 1;
+          const int lkey = __shfl(key, leader);
+          const uint64_t m = __ballot(valid && key == lkey);
+          if (lane == leader) atomicAdd(&lds_cnt[lkey], (uint32_t)__popcll(m));
+          active &= ~m;
+        }
       }
     }
   }
   if (!SCATTER) {
     __syncthreads();
-    for (int c = lane; c < nc; c += 64) trow[c] = lds_cnt[c];
+    for (int c = threadIdx.x; c < nc; c += TILE) trow[c] = lds_cnt[c];
   }
 }
 
