@@ -69,8 +69,9 @@ __global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp
 // Most bins of one sort pass (one LDS histogram of 64 KB per chunk); more
 // cells sort by two digits of the cell key, up to MAX_CELLS.
 constexpr int SORT_BINS = 16384;
-// Wavefronts of one sort workgroup (they share its chunk's histogram).
-constexpr int PUSH_WAVES = 4;
+// Wavefronts of one sort workgroup (they would share its chunk's histogram;
+// one measured fastest: the push is bound by its double-precision arithmetic).
+constexpr int PUSH_WAVES = 1;
 constexpr int64_t MAX_CELLS = (int64_t)1 << 22;
 __global__ void k_cell_bounds(const uint32_t *key, int64_t n, int nc, uint32_t *start,
                               uint32_t *err);

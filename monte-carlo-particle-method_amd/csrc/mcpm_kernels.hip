@@ -367,7 +367,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
               active &= ~m;
             }
           }
-          __syncthreads();
+          if constexpr (PUSH_WAVES > 1) __syncthreads();
         }
       } else {
         // counts only: the order of the additions does not matter
