@@ -458,6 +458,73 @@ int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes, mcpm_acces
 int mcpm_set_tuning(mcpm_ctx *ctx, int32_t knob, int64_t value);
 int mcpm_get_tuning(mcpm_ctx *ctx, int32_t knob, int64_t *value);
 
+/* ---- learned field surrogate (spectral neural operator) ------------------ */
+/* No counterpart in the reference: a network that stands in for the periodic
+   plane's screened-Poisson solve (compute_SP_2D, FEM.cpp:249-610).  It maps
+   the nodal density of an N0 x N1 periodic plane (unique nodes, p = j N0 + i)
+   to the Q1 FEM solution of (K + lambda^2 M) phi = M n before the core
+   subtraction (FEM.cpp:497-554).  Width 64, four Fourier layers with `modes`
+   retained modes per axis (8..32, multiple of 8; ky in [0,m) and [N1-m,N1),
+   kx in [0,m)), tanh GELU, BF16 activations on MFMA with FP32 accumulation.
+   The flat FP32 parameter vector has mcpm_fno_param_count entries:
+   lift w (64), b (64); per layer W[o][i] (64 x 64), b (64), spectral
+   weights R[a][kx][i][o] complex (re, im) (2m x m x 64 x 64 x 2); projection
+   w (64), b (1); zero padding to a multiple of 4.
+   Normalisation: u = (n - in_shift) in_scale, phi = y / out_scale + out_shift.
+   Loss over a snapshot (e = y - t in normalised units, periodic forward
+   differences): sum(|grad e|^2 + w0 e^2) / sum(|grad t|^2 + w0 t^2).
+   N0 must be a multiple of 64, N1 a multiple of 4 and at least 2 modes.
+   The object owns its stream and device memory; calls are synchronous
+   except where they run on a context's stream (add_snapshot, field). */
+typedef struct mcpm_fno mcpm_fno;
+typedef struct mcpm_fno_field_stats {
+  double residual;        /* |M n - A phi_net| / |M n| of the network's field */
+  int32_t cg_iterations;  /* conjugate-gradient iterations that followed */
+  int32_t used_network;   /* 1 when the network's field was kept as it is */
+} mcpm_fno_field_stats;
+int mcpm_fno_param_count(int32_t modes, int64_t *count);
+int mcpm_fno_create(int32_t n0, int32_t n1, int32_t modes, const float *params, mcpm_fno **out);
+int mcpm_fno_destroy(mcpm_fno *fno);
+int mcpm_fno_set_params(mcpm_fno *fno, const float *params);
+int mcpm_fno_get_params(mcpm_fno *fno, float *params);
+int mcpm_fno_set_norm(mcpm_fno *fno, double in_shift, double in_scale, double out_shift,
+                      double out_scale);
+/* Inference on host arrays of N0 N1 normalised densities / outputs. */
+int mcpm_fno_forward(mcpm_fno *fno, const float *u, float *y);
+/* Loss and full parameter gradient of one sample (host arrays; grad and y
+   may be null). */
+int mcpm_fno_loss_grad(mcpm_fno *fno, const float *u, const float *t, float w0, float *loss,
+                       float *grad, float *y);
+/* Snapshot store on the device (normalised density and target).  Every
+   snapshot uses the same w0.  mcpm_fno_add_snapshot solves the context's
+   field exactly (conjugate gradients from zero) and stores density and
+   solution; the context's field is then the exact one, as after
+   mcpm_poisson_solver. */
+int mcpm_fno_reserve_snapshots(mcpm_fno *fno, int32_t capacity);
+int mcpm_fno_add_snapshot_host(mcpm_fno *fno, const float *u, const float *t, float w0);
+int mcpm_fno_add_snapshot(mcpm_fno *fno, mcpm_ctx *ctx, float w0);
+int mcpm_fno_snapshot_count(mcpm_fno *fno, int32_t *count);
+/* `iters` Adam steps (beta 0.9, 0.999, eps 1e-8) on the device, one snapshot
+   per step drawn from the first n_snap (all when <= 0) by a splitmix64
+   stream of `seed`; losses (may be null) gets each step's loss. */
+int mcpm_fno_train(mcpm_fno *fno, int32_t iters, float lr, uint64_t seed, int32_t n_snap,
+                   float *losses);
+/* The context's field from the network: nodal density, network, the field
+   written as the solution; then, with exact = 1, conjugate gradients from it
+   to the context's tolerance; with accept_residual >= 0, the same unless
+   the network's residual is at most accept_residual; with accept_residual
+   < 0, the network's field as it is.  Then the core subtraction and the
+   kick of the next push, as mcpm_poisson_solver. */
+int mcpm_fno_field(mcpm_fno *fno, mcpm_ctx *ctx, double accept_residual, int32_t exact,
+                   mcpm_fno_field_stats *stats);
+/* Attach (fno) or detach (null) the surrogate for mcpm_step and
+   mcpm_poisson_solver on a periodic plane: the step's field is exact (from
+   the network's field) when step % every == 0 (every > 0), else the
+   network's field when its residual is at most accept_residual (always
+   when accept_residual < 0), else exact.  Detach before destroying fno. */
+int mcpm_set_field_surrogate(mcpm_ctx *ctx, mcpm_fno *fno, double accept_residual, int64_t every);
+int mcpm_get_field_surrogate_stats(mcpm_ctx *ctx, mcpm_fno_field_stats *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,6 +11,7 @@
 
 #include "mcpm_internal.h"
 #include "mcpm_kernels.h"
+#include "mcpm_plane.h"
 
 using namespace mcpm;
 
@@ -98,6 +99,12 @@ struct mcpm_ctx {
   double *n_node = nullptr, *pot = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr;
   double *tab_sp = nullptr, *tab_vl = nullptr;
   double kick_coef = 0.0;
+  // field surrogate of a periodic screened-Poisson plane (mcpm_set_field_surrogate)
+  mcpm_fno *fno = nullptr;
+  double fno_accept = 1e-3;
+  int64_t fno_every = 50;
+  int64_t cur_step = 0;
+  mcpm_fno_field_stats fno_last{};
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[T_NCLASS];
@@ -777,14 +784,30 @@ int launch_nodes(mcpm_ctx *c) {
 // launched in batches; between batches the host reads the convergence flag.
 constexpr int CG_ONE_WG_NODES = 129 * 129;
 
-int fem_cg_grid(mcpm_ctx *c, const FemParams &f) {
+int fem_cg_grid(mcpm_ctx *c, const FemParams &f, int warm = 0, int mode = 0,
+                double accept = 0.0, double *res0 = nullptr, int *iters = nullptr) {
   const dim3 g(CG_GRID), b(256);
-  hipLaunchKernelGGL(k_cgm_zero, g, b, 0, c->s, c->pot, f);
+  if (!warm) hipLaunchKernelGGL(k_cgm_zero, g, b, 0, c->s, c->pot, f);
   hipLaunchKernelGGL(k_cgm_start, g, b, 0, c->s, c->n_node, c->pot, c->cg_r, c->cg_p, f,
                      c->cg_part);
   hipLaunchKernelGGL(k_cgm_start2, dim3(1), b, 0, c->s, c->cg_st, c->cg_part, CG_GRID, f,
                      c->fem_iters);
   LAUNCHCHK();
+  if (res0) *res0 = -1.0;
+  if (iters) *iters = 0;
+  if (warm) {
+    // starting residual |M n - A pot| / |M n| (one host read)
+    double sc[4];
+    HIPCHK(hipMemcpyAsync(sc, c->cg_st, sizeof sc, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    const double r = sc[3] > 0.0 ? std::sqrt(sc[0] / sc[3]) : (sc[0] > 0.0 ? INFINITY : 0.0);
+    if (res0) *res0 = r;
+    if (mode == 2 || (mode == 1 && r <= accept)) {
+      hipLaunchKernelGGL(k_cgm_seams, g, b, 0, c->s, c->pot, f);
+      LAUNCHCHK();
+      return MCPM_OK;
+    }
+  }
   int k = 0;
   uint32_t flag[2] = {0u, 0u};
   while (k < f.maxit) {
@@ -802,6 +825,7 @@ int fem_cg_grid(mcpm_ctx *c, const FemParams &f) {
     HIPCHK(hipStreamSynchronize(c->s));
     if (flag[1]) break;
   }
+  if (iters) *iters = (int)flag[0];
   hipLaunchKernelGGL(k_cgm_seams, g, b, 0, c->s, c->pot, f);
   LAUNCHCHK();
   return MCPM_OK;
@@ -851,6 +875,57 @@ double contact_y_host(const mcpm_params *p, double n) { return contact_y(n, p->b
 
 const mcpm_params *ctx_params(const mcpm_ctx *c) { return &c->p; }
 int64_t ctx_sample_count(const mcpm_ctx *c) { return c->n_samples; }
+
+int ctx_plane_nodes(mcpm_ctx *c, PlaneField *pf) {
+  if (!c || !pf) return MCPM_ERR_ARG;
+  if (c->p.dims != 2 || c->p.long_range != MCPM_LR_SCREENED_POISSON) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  int st = MCPM_DISPATCH(c, launch_nodes, c);
+  if (st) return st;
+  pf->n_node = c->n_node;
+  pf->pot = c->pot;
+  pf->fem = c->fem;
+  pf->s = c->s;
+  return MCPM_OK;
+}
+
+int ctx_plane_cg(mcpm_ctx *c, int warm, int mode, double accept, double *res0, int *iters) {
+  if (!c || c->p.dims != 2 || !c->fem.per_x || !c->fem.per_y) return MCPM_ERR_ARG;
+  return fem_cg_grid(c, c->fem, warm, mode, accept, res0, iters);
+}
+
+int ctx_plane_finish(mcpm_ctx *c) {
+  const int nn = (c->fem.nx + 1) * (c->fem.ny + 1);
+  hipLaunchKernelGGL(k_fem_core, dim3((nn + 255) / 256), dim3(256), 0, c->s, c->n_node, c->pot,
+                     c->fem);
+  LAUNCHCHK();
+  c->kick_coef = c->p.ast;
+  c->kick_pending = true;
+  return MCPM_OK;
+}
+
+int mcpm_set_field_surrogate(mcpm_ctx *c, mcpm_fno *f, double accept_residual, int64_t every) {
+  if (!c) return MCPM_ERR_ARG;
+  if (f && (c->p.dims != 2 || c->p.long_range != MCPM_LR_SCREENED_POISSON || !c->fem.per_x ||
+            !c->fem.per_y || every < 0))
+    return MCPM_ERR_ARG;
+  c->fno = f;
+  c->fno_accept = accept_residual;
+  c->fno_every = every;
+  c->fno_last = mcpm_fno_field_stats{};
+  return MCPM_OK;
+}
+
+int ctx_set_surrogate_stats(mcpm_ctx *c, const mcpm_fno_field_stats *st) {
+  c->fno_last = *st;
+  return MCPM_OK;
+}
+
+int mcpm_get_field_surrogate_stats(mcpm_ctx *c, mcpm_fno_field_stats *out) {
+  if (!c || !out) return MCPM_ERR_ARG;
+  *out = c->fno_last;
+  return MCPM_OK;
+}
 
 template <typename R>
 int launch_init(mcpm_ctx *c, const InitParams &ip) {
@@ -1364,6 +1439,12 @@ int mcpm_poisson_solver(mcpm_ctx *c) {
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->p.long_range != MCPM_LR_SCREENED_POISSON) return MCPM_ERR_ARG;
   Timer tm(c, T_FIELD);
+  if (c->p.dims == 2 && c->fno) {
+    int st = fno_field_step(c->fno, c, c->cur_step, c->fno_accept, c->fno_every);
+    if (st) return st;
+    c->kick_pending = true;
+    return MCPM_OK;
+  }
   if (c->p.dims == 2) {
     int st = sp_2d(c);
     if (st) return st;
@@ -1485,6 +1566,7 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   }
   if (st) return st;
   // long-range attraction (Main_Functions.cpp:1612-1668)
+  c->cur_step = step;
   if (c->p.long_range == MCPM_LR_SCREENED_POISSON) st = mcpm_poisson_solver(c);
   else if (c->p.long_range == MCPM_LR_DIRECT) st = mcpm_vlasov_integral(c);
   else if (c->p.long_range == MCPM_LR_DENSITY_EXPANSION) st = mcpm_density_expansion(c);

@@ -356,6 +356,7 @@ __global__ __launch_bounds__(256) void k_cgm_start2(double *__restrict__ st,
   if (threadIdx.x == 0) {
     st[0] = rr;
     st[2] = f.tol2 * bb;
+    st[3] = bb;
     flag[0] = 0u;
     flag[1] = (rr > f.tol2 * bb && f.maxit > 0) ? 0u : 1u;
   }

@@ -76,6 +76,15 @@ class CAccessRates(ctypes.Structure):
                 ("atomic_min_word", ctypes.c_double), ("scatter_record", ctypes.c_double)]
 
 
+class CFieldStats(ctypes.Structure):
+    _fields_ = [("residual", ctypes.c_double), ("cg_iterations", ctypes.c_int32),
+                ("used_network", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {"residual": self.residual, "cg_iterations": self.cg_iterations,
+                "used_network": bool(self.used_network)}
+
+
 class CStats(ctypes.Structure):
     _fields_ = [("candidates", ctypes.c_int64), ("valid", ctypes.c_int64),
                 ("accepted", ctypes.c_int64), ("rounds", ctypes.c_int32),
@@ -236,6 +245,7 @@ def load_library() -> ctypes.CDLL:
     lib = ctypes.CDLL(LIB_PATH)
     vp, i32, i64, dpp = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_double)
     u32pp = ctypes.POINTER(ctypes.c_uint32)
+    fpp = ctypes.POINTER(ctypes.c_float)
     sig = {
         "mcpm_create": (i32, [ctypes.POINTER(CParams), ctypes.POINTER(vp)]),
         "mcpm_destroy": (i32, [vp]),
@@ -288,6 +298,23 @@ def load_library() -> ctypes.CDLL:
         "mcpm_set_tuning": (i32, [vp, i32, i64]),
         "mcpm_get_tuning": (i32, [vp, i32, ctypes.POINTER(i64)]),
         "mcpm_probe_random_access": (i32, [i64, i32, ctypes.POINTER(CAccessRates)]),
+        "mcpm_fno_param_count": (i32, [i32, ctypes.POINTER(i64)]),
+        "mcpm_fno_create": (i32, [i32, i32, i32, fpp, ctypes.POINTER(vp)]),
+        "mcpm_fno_destroy": (i32, [vp]),
+        "mcpm_fno_set_params": (i32, [vp, fpp]),
+        "mcpm_fno_get_params": (i32, [vp, fpp]),
+        "mcpm_fno_set_norm": (i32, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                    ctypes.c_double]),
+        "mcpm_fno_forward": (i32, [vp, fpp, fpp]),
+        "mcpm_fno_loss_grad": (i32, [vp, fpp, fpp, ctypes.c_float, fpp, fpp, fpp]),
+        "mcpm_fno_reserve_snapshots": (i32, [vp, i32]),
+        "mcpm_fno_add_snapshot_host": (i32, [vp, fpp, fpp, ctypes.c_float]),
+        "mcpm_fno_add_snapshot": (i32, [vp, vp, ctypes.c_float]),
+        "mcpm_fno_snapshot_count": (i32, [vp, ctypes.POINTER(i32)]),
+        "mcpm_fno_train": (i32, [vp, i32, ctypes.c_float, ctypes.c_uint64, i32, fpp]),
+        "mcpm_fno_field": (i32, [vp, vp, ctypes.c_double, i32, ctypes.POINTER(CFieldStats)]),
+        "mcpm_set_field_surrogate": (i32, [vp, vp, ctypes.c_double, i64]),
+        "mcpm_get_field_surrogate_stats": (i32, [vp, ctypes.POINTER(CFieldStats)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -589,3 +616,130 @@ class Simulation:
         self._check(self._lib.mcpm_timing_get(self._h, which, ctypes.byref(ms), ctypes.byref(calls)),
                     "timing_get")
         return ms.value, calls.value
+
+    def set_field_surrogate(self, fno: "FieldSurrogate | None", accept_residual: float = 1e-3,
+                            every: int = 50):
+        """Attach the field surrogate to mcpm_step / poisson_solver (None detaches)."""
+        h = fno._h if fno is not None else None
+        self._check(self._lib.mcpm_set_field_surrogate(self._h, h, float(accept_residual),
+                                                       int(every)), "set_field_surrogate")
+        self._fno = fno
+
+    def field_surrogate_stats(self) -> dict:
+        st = CFieldStats()
+        self._check(self._lib.mcpm_get_field_surrogate_stats(self._h, ctypes.byref(st)),
+                    "get_field_surrogate_stats")
+        return st.as_dict()
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+class FieldSurrogate:
+    """Spectral neural operator standing in for the periodic plane's
+    screened-Poisson solve (compute_SP_2D, FEM.cpp:249-610); no counterpart
+    in the reference.  Width 64, four Fourier layers, `modes` retained modes
+    per axis; trained on the GPU by the library's own forward, backward and
+    Adam kernels on snapshots that the exact solver writes."""
+
+    C = 64
+
+    def __init__(self, n0: int, n1: int, modes: int, params: np.ndarray):
+        self._lib = load_library()
+        self.n0, self.n1, self.modes = n0, n1, modes
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        if params.size != self.param_count(modes):
+            raise ValueError(f"parameter vector of {params.size}, expected {self.param_count(modes)}")
+        h = ctypes.c_void_p()
+        st = self._lib.mcpm_fno_create(n0, n1, modes, _fp(params), ctypes.byref(h))
+        if st:
+            raise McpmError(st, "mcpm_fno_create")
+        self._h = h
+
+    @staticmethod
+    def param_count(modes: int) -> int:
+        n = ctypes.c_int64()
+        st = load_library().mcpm_fno_param_count(modes, ctypes.byref(n))
+        if st:
+            raise McpmError(st, "mcpm_fno_param_count")
+        return n.value
+
+    def _check(self, st: int, what: str):
+        if st:
+            raise McpmError(st, what)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.mcpm_fno_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def P(self) -> int:
+        return self.n0 * self.n1
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        self._check(self._lib.mcpm_fno_set_params(self._h, _fp(params)), "fno_set_params")
+
+    def params(self) -> np.ndarray:
+        out = np.empty(self.param_count(self.modes), np.float32)
+        self._check(self._lib.mcpm_fno_get_params(self._h, _fp(out)), "fno_get_params")
+        return out
+
+    def set_norm(self, in_shift, in_scale, out_shift, out_scale):
+        self.norm = (in_shift, in_scale, out_shift, out_scale)
+        self._check(self._lib.mcpm_fno_set_norm(self._h, float(in_shift), float(in_scale),
+                                                float(out_shift), float(out_scale)), "fno_set_norm")
+
+    def forward(self, u) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        y = np.empty(self.P, np.float32)
+        self._check(self._lib.mcpm_fno_forward(self._h, _fp(u), _fp(y)), "fno_forward")
+        return y.reshape(self.n1, self.n0)
+
+    def loss_grad(self, u, t, w0: float):
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        loss = ctypes.c_float()
+        g = np.empty(self.param_count(self.modes), np.float32)
+        y = np.empty(self.P, np.float32)
+        self._check(self._lib.mcpm_fno_loss_grad(self._h, _fp(u), _fp(t), w0, ctypes.byref(loss),
+                                                 _fp(g), _fp(y)), "fno_loss_grad")
+        return loss.value, g, y.reshape(self.n1, self.n0)
+
+    def reserve_snapshots(self, cap: int):
+        self._check(self._lib.mcpm_fno_reserve_snapshots(self._h, cap), "fno_reserve_snapshots")
+
+    def add_snapshot_host(self, u, t, w0: float):
+        u = np.ascontiguousarray(u, dtype=np.float32).reshape(-1)
+        t = np.ascontiguousarray(t, dtype=np.float32).reshape(-1)
+        self._check(self._lib.mcpm_fno_add_snapshot_host(self._h, _fp(u), _fp(t), w0),
+                    "fno_add_snapshot_host")
+
+    def add_snapshot(self, sim: "Simulation", w0: float):
+        self._check(self._lib.mcpm_fno_add_snapshot(self._h, sim._h, w0), "fno_add_snapshot")
+
+    def snapshot_count(self) -> int:
+        n = ctypes.c_int32()
+        self._check(self._lib.mcpm_fno_snapshot_count(self._h, ctypes.byref(n)), "fno_snapshot_count")
+        return n.value
+
+    def train(self, iters: int, lr: float, seed: int = 0, n_snap: int = 0) -> np.ndarray:
+        out = np.empty(max(iters, 1), np.float32)
+        self._check(self._lib.mcpm_fno_train(self._h, iters, lr, seed, n_snap, _fp(out)),
+                    "fno_train")
+        return out[:iters]
+
+    def field(self, sim: "Simulation", accept_residual: float = -1.0, exact: bool = False) -> dict:
+        st = CFieldStats()
+        self._check(self._lib.mcpm_fno_field(self._h, sim._h, float(accept_residual),
+                                             1 if exact else 0, ctypes.byref(st)), "fno_field")
+        return st.as_dict()
