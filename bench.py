@@ -54,6 +54,13 @@ def main():
     ap.add_argument("--pressure", action="store_true",
                     help="sample measure_pressure's face sums in every step (the reference "
                          "does once step > times * after)")
+    ap.add_argument("--no-surrogate", action="store_true",
+                    help="skip the learned-field phase (512 x 512 droplet plane)")
+    ap.add_argument("--sur-particles", type=int, default=64_000_000)
+    ap.add_argument("--sur-snapshots", type=int, default=64,
+                    help="exact-solver snapshots the network trains on (one per step)")
+    ap.add_argument("--sur-iters", type=int, default=2000, help="Adam steps of the training")
+    ap.add_argument("--sur-steps", type=int, default=20, help="timed steps with and without the network")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (a known-size kernel for "
                          "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
@@ -213,6 +220,11 @@ def main():
     cpu = None
     if not args.no_cpu_baseline and args.workload == "interface":
         cpu = cpu_baseline(args)
+    surrogate = None
+    if not args.no_surrogate and args.workload in ("interface", "droplets"):
+        sim.close()
+        del sim
+        surrogate = surrogate_phase(args)
     if args.workload == "replicas":
         desc = (f"replicas: {args.replicas} independent periodic 400 sigma interface slabs in one "
                 f"batched context, {N} particles, {args.cells} cells; per replica N log-uniform "
@@ -249,8 +261,156 @@ def main():
                    "phases_ms_per_step": {k: v["per_step_ms"] for k, v in timers.items()}},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "surrogate": surrogate,
     }
     print(json.dumps(out), flush=True)
+
+
+def surrogate_phase(args):
+    """The learned field on the 512 x 512 droplet plane (64M particles, 20
+    discs at 0.55 in vapour 0.02, T = 0.75, periodic, screened-Poisson FEM).
+
+    The network (width 64, four Fourier layers, 16 modes per axis, BF16 on
+    MFMA) trains on the GPU on snapshots that the exact solver writes along
+    the run (one per step), then is measured on later steps it never saw:
+    force error of the Q1-interpolated field against the exact solve of the
+    same density, the FEM residual of its field, the field time (deposit +
+    network + residual check + core term, against deposit + conjugate
+    gradients + core term), and the step time with the network standing in
+    for the solve (exact solve every 50 steps) against the exact step."""
+    import numpy as np
+
+    import mcpm
+    import workloads as W
+
+    p, arrs, meta = W.droplets(N=args.sur_particles)
+    N = meta["N"]
+    n0, n1, m = p.n_cells_x, p.n_cells, 16
+    log(f"surrogate: droplets N={N} {n0}x{n1}")
+    sim = mcpm.Simulation(p)
+    sim.upload_xy(*arrs)
+    del arrs
+    sim.cell_update()
+    sim.init_omega_max(meta["n_gas"])
+    step = 0
+    for _ in range(3):
+        sim.step(step)
+        step += 1
+
+    def timed_steps(k):
+        nonlocal step
+        sim.timing_enable(True)
+        sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(k):
+            sim.step(step)
+            step += 1
+        sim.sync()
+        ms = 1000.0 * (time.perf_counter() - t0) / k
+        fms, calls = sim.timing(mcpm.TIMER_FIELD)
+        sim.timing_enable(False)
+        return ms, fms
+
+    sim.poisson_solver()
+    nd, pot = sim.nodes()
+    net = mcpm.FieldSurrogate(n0, n1, m, mcpm.FieldSurrogate.init_params(m, seed=0))
+    net.set_norm(nd.mean(), 1.0 / nd.std(), pot.mean(), 1.0 / pot.std())
+    S = args.sur_snapshots
+    net.reserve_snapshots(S)
+    t0 = time.perf_counter()
+    for _ in range(S):
+        sim.step(step)
+        step += 1
+        net.add_snapshot(sim, 0.05)
+    t_snap = time.perf_counter() - t0
+
+    def held_out(k):
+        nonlocal step
+        errs, res = [], []
+        for _ in range(k):
+            sim.step(step)
+            step += 1
+            st = net.field(sim, accept_residual=-1.0)
+            _, pn = sim.nodes()
+            sim.poisson_solver()
+            _, pe = sim.nodes()
+            fe, fn = mcpm.plane_force(pe, p.dx), mcpm.plane_force(pn, p.dx)
+            errs.append(float(np.linalg.norm(fn - fe) / np.linalg.norm(fe)))
+            res.append(st["residual"])
+        return float(np.mean(errs)), float(np.mean(res))
+
+    err0, res0 = held_out(2)
+    T = args.sur_iters
+    t0 = time.perf_counter()
+    l1 = net.train(T // 2, 1e-3, seed=1)
+    l2 = net.train(T - T // 2, 3e-4, seed=2)
+    t_train = time.perf_counter() - t0
+    err1, res1 = held_out(4)
+    log(f"surrogate: loss {l1[:10].mean():.4f} -> {l2[-20:].mean():.5f}; force error "
+        f"{err0:.4f} -> {err1:.5f}; residual {res0:.3f} -> {res1:.4f}")
+
+    # the field alone, network against exact, same deposit and core term
+    for _ in range(2):
+        net.field(sim, accept_residual=-1.0)
+    sim.sync()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        net.field(sim, accept_residual=-1.0)
+    sim.sync()
+    f_net = 100.0 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        sim.poisson_solver()
+    sim.sync()
+    f_ex = 100.0 * (time.perf_counter() - t0)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        st_ws = net.field(sim, exact=True)
+    sim.sync()
+    f_warm = 200.0 * (time.perf_counter() - t0)
+
+    # the step with and without the network, in alternating blocks (the
+    # droplet system's collision work grows along the run)
+    blocks, k = 4, max(1, args.sur_steps // 8)
+    ex, sur = [], []
+    for _ in range(blocks):
+        sim.set_field_surrogate(None)
+        ex.append(timed_steps(k))
+        sim.set_field_surrogate(net, accept_residual=-1.0, every=50)
+        sur.append(timed_steps(k))
+    step_exact = float(np.mean([a for a, _ in ex]))
+    field_exact = float(np.mean([b for _, b in ex]))
+    step_sur = float(np.mean([a for a, _ in sur]))
+    field_sur = float(np.mean([b for _, b in sur]))
+    # with the residual gate at 1e-3 the exact solve runs whenever the
+    # network's FEM residual is larger
+    sim.set_field_surrogate(net, accept_residual=1e-3, every=50)
+    sim.step(step)
+    step += 1
+    gated = sim.field_surrogate_stats()
+    sim.set_field_surrogate(None)
+    sim.close()
+    P = n0 * n1
+    flops = 4 * (2.0 * 2 * m * n0 * 64 * n1 + 2.0 * 64 * (64 + 2 * m) * P)  # MFMA work per forward
+    return {
+        "workload": f"droplets: periodic 128 x 128 sigma plane, {n0} x {n1} cells, {N} particles, "
+                    "20 discs n=0.55 in vapour 0.02, T=0.75, ESMC + screened-Poisson FEM, dt=0.01",
+        "network": {"layers": 4, "width": 64, "modes": m, "dtype": "bf16 MFMA, fp32 accumulate",
+                    "params": int(mcpm.FieldSurrogate.param_count(m))},
+        "training": {"snapshots": S, "snapshot_s": round(t_snap, 2), "iters": T,
+                     "ms_per_iter": round(1000.0 * t_train / max(T, 1), 3),
+                     "loss_first": float(l1[:10].mean()), "loss_last": float(l2[-20:].mean())},
+        "force_rel_err": {"untrained": err0, "trained": err1, "held_out_steps": 4},
+        "fem_residual": {"untrained": res0, "trained": res1},
+        "field_ms": {"exact_cg": f_ex, "network": f_net, "exact_cg_from_network": f_warm,
+                     "speedup": f_ex / f_net, "cg_iterations_from_network": st_ws["cg_iterations"]},
+        "step_ms": {"blocks": f"{blocks} x {k} steps each, alternating",
+                    "exact": step_exact, "network_every_50": step_sur,
+                    "speedup": step_exact / step_sur,
+                    "field_ms_per_step_exact": field_exact, "field_ms_per_step_network": field_sur},
+        "residual_gate_1e-3": gated,
+        "mfma_gflop_per_forward": round(flops / 1e9, 2),
+    }
 
 
 def random_access_roofline(n, rec, valid, acc, revisits, nbands, coll_ms, launches):

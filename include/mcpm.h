@@ -473,7 +473,7 @@ int mcpm_get_tuning(mcpm_ctx *ctx, int32_t knob, int64_t *value);
    Normalisation: u = (n - in_shift) in_scale, phi = y / out_scale + out_shift.
    Loss over a snapshot (e = y - t in normalised units, periodic forward
    differences): sum(|grad e|^2 + w0 e^2) / sum(|grad t|^2 + w0 t^2).
-   N0 must be a multiple of 64, N1 a multiple of 4 and at least 2 modes.
+   N0 must be a multiple of 64, N1 a multiple of 16 and at least 2 modes.
    The object owns its stream and device memory; calls are synchronous
    except where they run on a context's stream (add_snapshot, field). */
 typedef struct mcpm_fno mcpm_fno;

@@ -79,55 +79,77 @@ __device__ __forceinline__ f32x4 mfma(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// GELU in the tanh form, written as z sigma(2u), u = sqrt(2/pi)(z + 0.044715 z^3)
+// (0.5 (1 + tanh u) = sigma(2u)), with the native exponential and division.
 constexpr float GELU_K = 0.7978845608028654f;  // sqrt(2/pi)
-__device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + tanhf(GELU_K * (z + 0.044715f * z * z * z)));
+__device__ __forceinline__ float gelu_sig(float z) {
+  const float u = GELU_K * (z + 0.044715f * z * z * z);
+  return __fdividef(1.0f, 1.0f + __expf(-2.0f * u));
 }
+__device__ __forceinline__ float gelu(float z) { return z * gelu_sig(z); }
 __device__ __forceinline__ float gelu_d(float z) {
-  const float t = tanhf(GELU_K * (z + 0.044715f * z * z * z));
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * GELU_K * (1.0f + 3.0f * 0.044715f * z * z);
+  const float sg = gelu_sig(z);
+  return sg + 2.0f * z * sg * (1.0f - sg) * GELU_K * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
 // x-DFT of one row per workgroup: A[y] (2m x 64) = Tf (2m x N0) . act[y] (N0 x 64).
-// Wave w owns channels 16w..16w+15 and all 2m/16 row tiles.
+// Wave w takes a quarter of the row (all 64 channels, all 2m/16 row tiles);
+// the four partial sums meet in LDS.
 template <typename T>
 __global__ __launch_bounds__(256) void k_dftx(const T *__restrict__ act,
                                               const uint16_t *__restrict__ tf,
                                               float *__restrict__ A, int n0, int m2) {
+  __shared__ float red[4][64 * 64];
   const int y = blockIdx.x, w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int r16 = l & 15, q = l >> 4;
   const int nrt = m2 >> 4;
-  f32x4 acc[4];
+  f32x4 acc[4][4];  // [row tile][channel tile]
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const T *row = act + (size_t)y * n0 * C + 16 * w + r16;
-  for (int x0 = 0; x0 < n0; x0 += 16) {
-    bf16x4 b;
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) b[j] = bits(row[(size_t)(x0 + 4 * q + j) * C]);
+    for (int c = 0; c < 4; ++c) acc[t][c] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int xq = n0 / 4, xb = w * xq;  // n0 is a multiple of 64
+  const T *row = act + (size_t)y * n0 * C + r16;
+  for (int x0 = xb; x0 < xb + xq; x0 += 16) {
+    bf16x4 b[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) b[c][j] = bits(row[(size_t)(x0 + 4 * q + j) * C + 16 * c]);
 #pragma unroll
     for (int t = 0; t < 4; ++t)
-      if (t < nrt) acc[t] = mfma(load4(tf + (size_t)(16 * t + r16) * n0 + x0 + 4 * q), b, acc[t]);
+      if (t < nrt) {
+        const bf16x4 a = load4(tf + (size_t)(16 * t + r16) * n0 + x0 + 4 * q);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[t][c] = mfma(a, b[c], acc[t][c]);
+      }
   }
+  // D[row = k' = 16 t + 4 q + j][col = channel 16 c + r16]
 #pragma unroll
   for (int t = 0; t < 4; ++t)
     if (t < nrt)
 #pragma unroll
-      for (int j = 0; j < 4; ++j)
-        A[((size_t)y * m2 + 16 * t + 4 * q + j) * C + 16 * w + r16] = acc[t][j];
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) red[w][(16 * t + 4 * q + j) * 64 + 16 * c + r16] = acc[t][c][j];
+  __syncthreads();
+  for (int e = threadIdx.x; e < m2 * 64; e += 256)
+    A[(size_t)y * m2 * C + e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
 }
 
-// y-DFT of the retained kx: X[a][kx][c] = s(kx) sum_y A[y][kx][c] e^{-i phi(a, y)}.
-// Grid (m, 2m/4); wave -> a, lane -> channel.
+// y-DFT of the retained kx, partial over a group of DFTY_ROWS rows:
+// Xp[g][a][kx][c] = sum_{y in group g} A[y][kx][c] e^{-i phi(a, y)}.
+// Grid (m, 2m/4, groups); wave -> a, lane -> channel.
+constexpr int DFTY_ROWS = 32;
 __global__ __launch_bounds__(256) void k_dfty(const float *__restrict__ A,
                                               const float2 *__restrict__ twy,
-                                              float2 *__restrict__ X, int n1, int m, float s0,
-                                              float s1) {
+                                              float2 *__restrict__ Xp, int n1, int m) {
   const int kx = blockIdx.x, a = blockIdx.y * 4 + (threadIdx.x >> 6), c = threadIdx.x & 63;
-  const int m2 = 2 * m;
+  const int m2 = 2 * m, g = blockIdx.z;
+  const int y0 = g * DFTY_ROWS, y1 = min(n1, y0 + DFTY_ROWS);
   float xr0 = 0.f, xi0 = 0.f, xr1 = 0.f, xi1 = 0.f;
   const float2 *tw = twy + (size_t)a * n1;
-  for (int y = 0; y < n1; y += 2) {
+  for (int y = y0; y < y1; y += 2) {  // n1 and DFTY_ROWS are even
     const float2 t0 = tw[y], t1 = tw[y + 1];
     const float ar0 = A[((size_t)y * m2 + kx) * C + c], ai0 = A[((size_t)y * m2 + m + kx) * C + c];
     const float ar1 = A[((size_t)(y + 1) * m2 + kx) * C + c],
@@ -137,8 +159,25 @@ __global__ __launch_bounds__(256) void k_dfty(const float *__restrict__ A,
     xr1 += ar1 * t1.x + ai1 * t1.y;
     xi1 += ai1 * t1.x - ar1 * t1.y;
   }
-  const float s = kx ? s1 : s0;
-  X[((size_t)a * m + kx) * C + c] = make_float2((xr0 + xr1) * s, (xi0 + xi1) * s);
+  Xp[(((size_t)g * m2 + a) * m + kx) * C + c] = make_float2(xr0 + xr1, xi0 + xi1);
+}
+
+// X[e] = s(kx) sum_g Xp[g][e] over the 2m m 64 mode entries.
+__global__ __launch_bounds__(256) void k_dfty_sum(const float2 *__restrict__ Xp, int groups,
+                                                  float2 *__restrict__ X, int m, float s0,
+                                                  float s1) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ne = 2 * m * m * C;
+  if (e >= ne) return;
+  float xr = 0.f, xi = 0.f;
+  for (int g = 0; g < groups; ++g) {
+    const float2 v = Xp[(size_t)g * ne + e];
+    xr += v.x;
+    xi += v.y;
+  }
+  const int kx = (e / C) % m;
+  const float sc = kx ? s1 : s0;
+  X[e] = make_float2(xr * sc, xi * sc);
 }
 
 // Spectral mixing Y[mode][o] = sum_i X[mode][i] R[mode][i][o]; wave -> mode, lane -> o.
@@ -193,25 +232,56 @@ __global__ __launch_bounds__(256) void k_mix_dr(const float2 *__restrict__ X,
   }
 }
 
-// y-inverse: Cp[y][o][kx | m + kx] = bf16(s(kx) sum_a Y[a][kx][o] e^{+i phi(a, y)}).
-// Wave -> (y, kx), lane -> o.
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ uint32_t pack2(float a, float b) {
+  return (uint32_t)f2bf(a) | ((uint32_t)f2bf(b) << 16);
+}
+
+// y-inverse: Cp[y][o][kx | m + kx] = bf16(s(kx) sum_a Y[a][kx][o] e^{+i phi(a, y)}),
+// on FP32 MFMA: D[o][y] = sum_a Y[a][kx][o] (cos, sin)(a, y) for a 16 x 16
+// tile (wave w: channels 16w.., workgroup: rows 16 blockIdx.x..) and a chunk
+// of 8 kx (blockIdx.y), so that a lane stores 8 consecutive k' of one (y, o).
 __global__ __launch_bounds__(256) void k_idfty(const float2 *__restrict__ Y,
                                                const float2 *__restrict__ twy,
                                                uint16_t *__restrict__ Cp, int n1, int m, float s0,
                                                float s1) {
-  const int q = blockIdx.x * 4 + (threadIdx.x >> 6), o = threadIdx.x & 63;
-  const int y = q / m, kx = q - y * m, m2 = 2 * m;
-  float cr = 0.f, ci = 0.f;
-  for (int a = 0; a < m2; ++a) {
-    const float2 t = twy[(size_t)a * n1 + y];
-    const float2 v = Y[((size_t)a * m + kx) * C + o];
-    cr += t.x * v.x - t.y * v.y;
-    ci += t.x * v.y + t.y * v.x;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r16 = l & 15, q = l >> 4;
+  const int y0 = blockIdx.x * 16, kc = blockIdx.y * 8, m2 = 2 * m;
+  f32x4 ar[8], ai[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) ar[k] = ai[k] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int a0 = 0; a0 < m2; a0 += 4) {
+    const int a = a0 + q;
+    const float2 tw = twy[(size_t)a * n1 + y0 + r16];
+    const float2 *ya = Y + ((size_t)a * m + kc) * C + 16 * w + r16;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const float2 v = ya[(size_t)k * C];
+      ar[k] = mfma4(v.x, tw.x, ar[k]);
+      ar[k] = mfma4(-v.y, tw.y, ar[k]);
+      ai[k] = mfma4(v.y, tw.x, ai[k]);
+      ai[k] = mfma4(v.x, tw.y, ai[k]);
+    }
   }
-  const float s = kx ? s1 : s0;
-  uint16_t *dst = Cp + ((size_t)y * C + o) * m2;
-  dst[kx] = f2bf(cr * s);
-  dst[m + kx] = f2bf(ci * s);
+  float sc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) sc[k] = (kc + k) ? s1 : s0;
+  const int y = y0 + r16;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int o = 16 * w + 4 * q + j;
+    uint16_t *dst = Cp + ((size_t)y * C + o) * m2;
+    *(uint4 *)(dst + kc) =
+        make_uint4(pack2(ar[0][j] * sc[0], ar[1][j] * sc[1]), pack2(ar[2][j] * sc[2], ar[3][j] * sc[3]),
+                   pack2(ar[4][j] * sc[4], ar[5][j] * sc[5]), pack2(ar[6][j] * sc[6], ar[7][j] * sc[7]));
+    *(uint4 *)(dst + m + kc) =
+        make_uint4(pack2(ai[0][j] * sc[0], ai[1][j] * sc[1]), pack2(ai[2][j] * sc[2], ai[3][j] * sc[3]),
+                   pack2(ai[4][j] * sc[4], ai[5][j] * sc[5]), pack2(ai[6][j] * sc[6], ai[7][j] * sc[7]));
+  }
 }
 
 enum : int { CMB_FWD = 0, CMB_LAST = 1, CMB_BWD = 2 };
@@ -220,61 +290,95 @@ enum : int { CMB_FWD = 0, CMB_LAST = 1, CMB_BWD = 2 };
 // Wt[n][k] (n = output channel).  FWD: z = out + b, Z = z (if given),
 // H = bf16(gelu(z)).  LAST: z = out + b, Z = z (if given), yout = wq . z + bq.
 // BWD: dh = out; outZ = dh gelu'(Zprev) (or dh when Zprev is null).
+// The MFMA computes the transpose, D[channel][node], so that a lane holds
+// four consecutive channels of one node: 8- and 16-byte stores.
 template <int MODE, typename TIn>
 __global__ __launch_bounds__(256) void k_combine(
     const TIn *__restrict__ in, const uint16_t *__restrict__ Wt, const uint16_t *__restrict__ tinv,
     const uint16_t *__restrict__ Cp, const float *__restrict__ bias, const float *__restrict__ wq,
     const float *__restrict__ Zprev, float *__restrict__ Z, uint16_t *__restrict__ H,
-    float *__restrict__ yout, int n0, int m2) {
+    float *__restrict__ yout, int n0, int m2, int tiles) {
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int r16 = l & 15, q = l >> 4;
-  const size_t p0 = (size_t)blockIdx.x * 64;
-  const int y = (int)(p0 / n0), x0 = (int)(p0 - (size_t)y * n0);
-  f32x4 acc[4];
+  // `tiles` consecutive 64-node tiles of one row; the weights and the row's
+  // spectral coefficients (A operands) stay in registers across them
+  const size_t pbase = (size_t)blockIdx.x * 64 * tiles;
+  const int y = (int)(pbase / n0), xbase = (int)(pbase - (size_t)y * n0);
+  bf16x4 wa[4][4], ca[4][4];
 #pragma unroll
-  for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-  const TIn *arow = in + (p0 + 16 * w + r16) * C + 4 * q;
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
-  for (int ks = 0; ks < 4; ++ks) {
-    const bf16x4 a = load4(arow + 16 * ks);
+    for (int ks = 0; ks < 4; ++ks) {
+      wa[t][ks] = load4(Wt + (16 * t + r16) * C + 16 * ks + 4 * q);
+      if (16 * ks < m2)
+        ca[t][ks] = load4(Cp + ((size_t)y * C + 16 * t + r16) * m2 + 16 * ks + 4 * q);
+      else
+        ca[t][ks] = bf16x4{0, 0, 0, 0};
+    }
+  float bb[4][4], wqv[4][4];
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
-      acc[t] = mfma(a, load4(Wt + (16 * t + r16) * C + 16 * ks + 4 * q), acc[t]);
-  }
-  const uint16_t *trow = tinv + (size_t)(x0 + 16 * w + r16) * m2 + 4 * q;
-  const uint16_t *crow = Cp + ((size_t)y * C + r16) * m2 + 4 * q;
-  for (int ks = 0; ks < m2; ks += 16) {
-    const bf16x4 a = load4(trow + ks);
-#pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = mfma(a, load4(crow + (size_t)16 * t * m2 + ks), acc[t]);
-  }
-  float pj[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int t = 0; t < 4; ++t) {
-    const int o = 16 * t + r16;
+  for (int t = 0; t < 4; ++t)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const size_t p = p0 + 16 * w + 4 * q + j;
-      const float v = acc[t][j];
-      if (MODE == CMB_BWD) {
-        Z[p * C + o] = Zprev ? v * gelu_d(Zprev[p * C + o]) : v;
-      } else {
-        const float z = v + bias[o];
-        if (Z) Z[p * C + o] = z;
-        if (MODE == CMB_FWD) H[p * C + o] = f2bf(gelu(z));
-        else pj[j] += wq[o] * z;
+      bb[t][j] = MODE == CMB_BWD ? 0.f : bias[16 * t + 4 * q + j];
+      wqv[t][j] = MODE == CMB_LAST ? wq[16 * t + 4 * q + j] : 0.f;
+    }
+  for (int tt = 0; tt < tiles; ++tt) {
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const size_t node = pbase + (size_t)tt * 64 + 16 * w + r16;
+    const TIn *brow = in + node * C + 4 * q;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      const bf16x4 b = load4(brow + 16 * ks);
+#pragma unroll
+      for (int t = 0; t < 4; ++t) acc[t] = mfma(wa[t][ks], b, acc[t]);
+    }
+    const uint16_t *trow = tinv + (size_t)(xbase + tt * 64 + 16 * w + r16) * m2 + 4 * q;
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      if (16 * ks < m2) {
+        const bf16x4 b = load4(trow + 16 * ks);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) acc[t] = mfma(ca[t][ks], b, acc[t]);
       }
     }
-  }
-  if (MODE == CMB_LAST) {
+    // lane: node `node`, channels 16 t + 4 q + j
+    float pj = 0.f;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      float v = pj[j];
-      v += __shfl_xor(v, 1);
-      v += __shfl_xor(v, 2);
-      v += __shfl_xor(v, 4);
-      v += __shfl_xor(v, 8);
-      if (r16 == 0) yout[p0 + 16 * w + 4 * q + j] = v + wq[C];
+    for (int t = 0; t < 4; ++t) {
+      const size_t e = node * C + 16 * t + 4 * q;
+      if (MODE == CMB_BWD) {
+        float4 r = make_float4(acc[t][0], acc[t][1], acc[t][2], acc[t][3]);
+        if (Zprev) {
+          const float4 zp = *(const float4 *)(Zprev + e);
+          r.x *= gelu_d(zp.x);
+          r.y *= gelu_d(zp.y);
+          r.z *= gelu_d(zp.z);
+          r.w *= gelu_d(zp.w);
+        }
+        *(float4 *)(Z + e) = r;
+      } else {
+        const float4 z = make_float4(acc[t][0] + bb[t][0], acc[t][1] + bb[t][1],
+                                     acc[t][2] + bb[t][2], acc[t][3] + bb[t][3]);
+        if (Z) *(float4 *)(Z + e) = z;
+        if (MODE == CMB_FWD) {
+          ushort4 hv;
+          hv.x = f2bf(gelu(z.x));
+          hv.y = f2bf(gelu(z.y));
+          hv.z = f2bf(gelu(z.z));
+          hv.w = f2bf(gelu(z.w));
+          *(ushort4 *)(H + e) = hv;
+        } else {
+          pj += wqv[t][0] * z.x + wqv[t][1] * z.y + wqv[t][2] * z.z + wqv[t][3] * z.w;
+        }
+      }
+    }
+    if (MODE == CMB_LAST) {
+      pj += __shfl_xor(pj, 16);
+      pj += __shfl_xor(pj, 32);
+      if (q == 0) yout[node] = pj + wq[C];
     }
   }
 }
@@ -318,15 +422,18 @@ __global__ __launch_bounds__(256) void k_wgrad(const float *__restrict__ dZ,
   if (q == 0) dst[C * C + 16 * w + r16] = bs;
 }
 
-// Column sums of one chunk: part[chunk] = {sum_p M[p][c] v[p] (64), sum_p M[p][c] (64), sum_p v[p]}.
+// Column sums of 256 nodes per workgroup:
+// part[blk] = {sum_p M[p][c] v[p] (64), sum_p M[p][c] (64), sum_p v[p]}.
+constexpr int COLSUM_NODES = 256;
 __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ M,
                                                 const float *__restrict__ v,
                                                 float *__restrict__ part, int64_t P) {
   __shared__ float sh[4][2 * C + 1];
   const int w = threadIdx.x >> 6, c = threadIdx.x & 63;
-  const int64_t pb = (int64_t)blockIdx.x * CHUNK, pe = min(P, pb + (int64_t)CHUNK);
+  const int64_t pb = (int64_t)blockIdx.x * COLSUM_NODES + w * (COLSUM_NODES / 4);
+  const int64_t pe = min(P, pb + (int64_t)(COLSUM_NODES / 4));
   float smv = 0.f, sm = 0.f, sv = 0.f;
-  for (int64_t p = pb + w; p < pe; p += 4) {
+  for (int64_t p = pb; p < pe; ++p) {
     const float mv = M[(size_t)p * C + c], vv = v[p];
     smv += mv * vv;
     sm += mv;
@@ -344,24 +451,40 @@ __global__ __launch_bounds__(256) void k_colsum(const float *__restrict__ M,
   }
 }
 
-// dst[k] (+)= sum_chunk part[chunk * stride + off + k], k < len.
-__global__ __launch_bounds__(256) void k_reduce(const float *__restrict__ part, int nchunk,
-                                                int stride, int off, int len,
-                                                float *__restrict__ dst, int accumulate) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= len) return;
+// dst[k] = sum_chunk part[chunk * stride + off + k], k < len: 64 entries per
+// workgroup, the 16 waves split the chunks.
+__global__ __launch_bounds__(1024) void k_reduce(const float *__restrict__ part, int nchunk,
+                                                 int stride, int off, int len,
+                                                 float *__restrict__ dst) {
+  __shared__ float sh[16][64];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int k = blockIdx.x * 64 + l;
   float s = 0.f;
-  for (int b = 0; b < nchunk; ++b) s += part[(size_t)b * stride + off + k];
-  dst[k] = accumulate ? dst[k] + s : s;
+  if (k < len)
+    for (int b = w; b < nchunk; b += 16) s += part[(size_t)b * stride + off + k];
+  sh[w][l] = s;
+  __syncthreads();
+  if (w == 0 && k < len) {
+    float t = 0.f;
+    for (int i = 0; i < 16; ++i) t += sh[i][l];
+    dst[k] = t;
+  }
 }
 
+// h0[p][c] = bf16(wl[c] u[p] + bl[c]); a thread writes 8 channels.
 __global__ __launch_bounds__(256) void k_lift(const float *__restrict__ u,
                                               const float *__restrict__ wl,
                                               uint16_t *__restrict__ H, int64_t P) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (id >= P * C) return;
-  const int c = (int)(id & (C - 1));
-  H[id] = f2bf(wl[c] * u[id >> 6] + wl[C + c]);
+  if (id >= P * (C / 8)) return;
+  const int c0 = (int)(id & (C / 8 - 1)) * 8;
+  const float uv = u[id >> 3];
+  uint32_t wv[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    wv[k] = (uint32_t)f2bf(wl[c0 + 2 * k] * uv + wl[C + c0 + 2 * k]) |
+            ((uint32_t)f2bf(wl[c0 + 2 * k + 1] * uv + wl[C + c0 + 2 * k + 1]) << 16);
+  *(uint4 *)(H + id * 8) = make_uint4(wv[0], wv[1], wv[2], wv[3]);
 }
 
 // dZ[p][c] = dy[p] wq[c]
@@ -411,18 +534,29 @@ __global__ __launch_bounds__(256) void k_loss(const float *__restrict__ yv,
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
 
-// Denominator sum (|grad t|^2 + w0 t^2) of a snapshot, into D[0].
-__global__ __launch_bounds__(1024) void k_tden(const float *__restrict__ t, double *__restrict__ D,
-                                               int n0, int n1, float w0) {
-  __shared__ double sh[16];
+// Partial denominator sums (|grad t|^2 + w0 t^2) of a snapshot, 256 nodes per workgroup.
+__global__ __launch_bounds__(256) void k_tden(const float *__restrict__ t,
+                                              double *__restrict__ part, int n0, int n1,
+                                              float w0) {
+  __shared__ double sh[4];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double acc = 0.0;
-  const int64_t P = (int64_t)n0 * n1;
-  for (int64_t p = threadIdx.x; p < P; p += blockDim.x) {
+  if (p < (int64_t)n0 * n1) {
     const int j = (int)(p / n0), i = (int)(p - (int64_t)j * n0);
     const int ir = i + 1 == n0 ? 0 : i + 1, ju = j + 1 == n1 ? 0 : j + 1;
     const double v = t[p], gx = t[(int64_t)j * n0 + ir] - v, gy = t[(int64_t)ju * n0 + i] - v;
-    acc += gx * gx + gy * gy + (double)w0 * v * v;
+    acc = gx * gx + gy * gy + (double)w0 * v * v;
   }
+  const double s = block_sum(acc, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// D[0] = sum of the partials.
+__global__ __launch_bounds__(1024) void k_sum_final(const double *__restrict__ part, int nb,
+                                                    double *__restrict__ D) {
+  __shared__ double sh[16];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) acc += part[b];
   const double s = block_sum(acc, sh);
   if (threadIdx.x == 0) D[0] = s;
 }
@@ -548,7 +682,7 @@ struct mcpm_fno {
   float *dZ[2] = {};
   float *A = nullptr;
   float2 *X[LAYERS] = {};
-  float2 *Y = nullptr, *dX = nullptr;
+  float2 *Y = nullptr, *dX = nullptr, *Xp = nullptr;
   uint16_t *Cp = nullptr;
   float *part = nullptr;
   double *lpart = nullptr, *dscr = nullptr;
@@ -570,6 +704,34 @@ int falloc(mcpm_fno *f, void **p, size_t bytes) {
 }
 
 int nchunks(const mcpm_fno *f) { return (int)((f->P + CHUNK - 1) / CHUNK); }
+int ncolsum(const mcpm_fno *f) { return (int)((f->P + COLSUM_NODES - 1) / COLSUM_NODES); }
+int ndfty(const mcpm_fno *f) { return (f->n1 + DFTY_ROWS - 1) / DFTY_ROWS; }
+
+// X = s(kx) DFT_y(A) through the row-group partials.
+void dfty(mcpm_fno *f, hipStream_t s, float2 *X, float s0, float s1) {
+  const int m = f->m, m2 = 2 * m, G = ndfty(f);
+  hipLaunchKernelGGL(k_dfty, dim3(m, m2 / 4, G), dim3(256), 0, s, f->A, f->twy, f->Xp, f->n1, m);
+  hipLaunchKernelGGL(k_dfty_sum, dim3((m2 * m * C + 255) / 256), dim3(256), 0, s, f->Xp, G, X, m,
+                     s0, s1);
+}
+
+void idfty(mcpm_fno *f, hipStream_t s, const float2 *Y, float s0, float s1) {
+  hipLaunchKernelGGL(k_idfty, dim3(f->n1 / 16, f->m / 8), dim3(256), 0, s, Y, f->twy, f->Cp, f->n1,
+                     f->m, s0, s1);
+}
+
+// 64-node tiles per combine workgroup: up to 4, all in one row.
+int combine_tiles(const mcpm_fno *f) {
+  const int r = f->n0 / 64;
+  return r % 4 == 0 ? 4 : (r % 2 == 0 ? 2 : 1);
+}
+
+// D[0] = sum over the snapshot t of (|grad t|^2 + w0 t^2).
+void tden(mcpm_fno *f, hipStream_t s, const float *t, double *D, float w0) {
+  const unsigned nb = (unsigned)((f->P + 255) / 256);
+  hipLaunchKernelGGL(k_tden, dim3(nb), dim3(256), 0, s, t, f->lpart, f->n0, f->n1, w0);
+  hipLaunchKernelGGL(k_sum_final, dim3(1), dim3(1024), 0, s, f->lpart, (int)nb, D);
+}
 
 void settle(mcpm_fno *f) {
   if (f->pending) {
@@ -586,29 +748,28 @@ int forward(mcpm_fno *f, hipStream_t s, bool train, const float *u) {
   const float sinv0 = (float)(1.0 / ((double)n0 * n1)), sinv1 = (float)(2.0 / ((double)n0 * n1));
   const int64_t P = f->P;
   auto hbuf = [&](int l) { return train ? f->H[l] : f->H[l & 1]; };
-  hipLaunchKernelGGL(k_lift, dim3((unsigned)((P * C + 255) / 256)), dim3(256), 0, s, u,
+  const int ct = combine_tiles(f);
+  hipLaunchKernelGGL(k_lift, dim3((unsigned)((P * (C / 8) + 255) / 256)), dim3(256), 0, s, u,
                      f->params + f->L.wl, hbuf(0), P);
   for (int l = 0; l < LAYERS; ++l) {
     const uint16_t *h = hbuf(l);
     float2 *Xl = train ? f->X[l] : f->X[0];
     hipLaunchKernelGGL((k_dftx<uint16_t>), dim3(n1), dim3(256), 0, s, h, f->tf, f->A, n0, m2);
-    hipLaunchKernelGGL(k_dfty, dim3(m, m2 / 4), dim3(256), 0, s, f->A, f->twy, Xl, n1, m, 1.0f,
-                       1.0f);
+    dfty(f, s, Xl, 1.0f, 1.0f);
     hipLaunchKernelGGL(k_mix, dim3(m2 * m / 4), dim3(256), 0, s, Xl,
                        (const float2 *)(f->params + f->L.R(l)), f->Y);
-    hipLaunchKernelGGL(k_idfty, dim3(n1 * m / 4), dim3(256), 0, s, f->Y, f->twy, f->Cp, n1, m,
-                       sinv0, sinv1);
+    idfty(f, s, f->Y, sinv0, sinv1);
     float *Zl = train ? f->Z[l] : nullptr;
     if (l < LAYERS - 1) {
-      hipLaunchKernelGGL((k_combine<CMB_FWD, uint16_t>), dim3((unsigned)(P / 64)), dim3(256), 0,
+      hipLaunchKernelGGL((k_combine<CMB_FWD, uint16_t>), dim3((unsigned)(P / (64 * ct))), dim3(256), 0,
                          s, h, f->Wf + l * C * C, f->tinv, f->Cp, f->params + f->L.b(l),
                          (const float *)nullptr, (const float *)nullptr, Zl, hbuf(l + 1),
-                         (float *)nullptr, n0, m2);
+                         (float *)nullptr, n0, m2, ct);
     } else {
-      hipLaunchKernelGGL((k_combine<CMB_LAST, uint16_t>), dim3((unsigned)(P / 64)), dim3(256), 0,
+      hipLaunchKernelGGL((k_combine<CMB_LAST, uint16_t>), dim3((unsigned)(P / (64 * ct))), dim3(256), 0,
                          s, h, f->Wf + l * C * C, f->tinv, f->Cp, f->params + f->L.b(l),
                          f->params + f->L.wq, (const float *)nullptr, Zl, (uint16_t *)nullptr,
-                         f->y, n0, m2);
+                         f->y, n0, m2, ct);
     }
   }
   FLAUNCH();
@@ -623,39 +784,39 @@ int backward(mcpm_fno *f, hipStream_t s, const float *u) {
   const int nb = nchunks(f);
   const float sadj0 = (float)(1.0 / ((double)n0 * n1)), sadj1 = (float)(2.0 / ((double)n0 * n1));
   const unsigned gPC = (unsigned)((P * C + 255) / 256);
+  const int ct = combine_tiles(f);
   // projection: wq, bq
-  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, s, f->Z[LAYERS - 1], f->dy, f->part, P);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, f->part, nb, 2 * C + 1, 0, C,
-                     f->grad + f->L.wq, 0);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, f->part, nb, 2 * C + 1, 2 * C, 1,
-                     f->grad + f->L.wq + C, 0);
+  const int ncs = ncolsum(f);
+  hipLaunchKernelGGL(k_colsum, dim3(ncs), dim3(256), 0, s, f->Z[LAYERS - 1], f->dy, f->part, P);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, f->part, ncs, 2 * C + 1, 0, C,
+                     f->grad + f->L.wq);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(1024), 0, s, f->part, ncs, 2 * C + 1, 2 * C, 1,
+                     f->grad + f->L.wq + C);
   hipLaunchKernelGGL(k_proj_bwd, dim3(gPC), dim3(256), 0, s, f->dy, f->params + f->L.wq, f->dZ[0],
                      P);
   int cur = 0;
   for (int l = LAYERS - 1; l >= 0; --l) {
     float *dz = f->dZ[cur], *dzn = f->dZ[cur ^ 1];
     hipLaunchKernelGGL(k_wgrad, dim3(nb), dim3(256), 0, s, dz, f->H[l], f->part, P);
-    hipLaunchKernelGGL(k_reduce, dim3((C * C + C + 255) / 256), dim3(256), 0, s, f->part, nb,
-                       C * C + C, 0, C * C + C, f->grad + f->L.W(l), 0);
+    hipLaunchKernelGGL(k_reduce, dim3((C * C + C + 63) / 64), dim3(1024), 0, s, f->part, nb,
+                       C * C + C, 0, C * C + C, f->grad + f->L.W(l));
     hipLaunchKernelGGL((k_dftx<float>), dim3(n1), dim3(256), 0, s, dz, f->tf, f->A, n0, m2);
-    hipLaunchKernelGGL(k_dfty, dim3(m, m2 / 4), dim3(256), 0, s, f->A, f->twy, f->Y, n1, m, sadj0,
-                       sadj1);
+    dfty(f, s, f->Y, sadj0, sadj1);
     hipLaunchKernelGGL(k_mix_dr, dim3(m2 * m / 4), dim3(256), 0, s, f->X[l], f->Y,
                        (float2 *)(f->grad + f->L.R(l)), 0);
     hipLaunchKernelGGL(k_mix_dx, dim3(m2 * m / 4), dim3(256), 0, s, f->Y,
                        (const float2 *)(f->params + f->L.R(l)), f->dX);
-    hipLaunchKernelGGL(k_idfty, dim3(n1 * m / 4), dim3(256), 0, s, f->dX, f->twy, f->Cp, n1, m,
-                       1.0f, 1.0f);
-    hipLaunchKernelGGL((k_combine<CMB_BWD, float>), dim3((unsigned)(P / 64)), dim3(256), 0, s, dz,
+    idfty(f, s, f->dX, 1.0f, 1.0f);
+    hipLaunchKernelGGL((k_combine<CMB_BWD, float>), dim3((unsigned)(P / (64 * ct))), dim3(256), 0, s, dz,
                        f->Wb + l * C * C, f->tinv, f->Cp, (const float *)nullptr,
                        (const float *)nullptr, l > 0 ? f->Z[l - 1] : (const float *)nullptr, dzn,
-                       (uint16_t *)nullptr, (float *)nullptr, n0, m2);
+                       (uint16_t *)nullptr, (float *)nullptr, n0, m2, ct);
     cur ^= 1;
   }
   // lift: wl, bl from dH0 = dZ[cur]
-  hipLaunchKernelGGL(k_colsum, dim3(nb), dim3(256), 0, s, f->dZ[cur], u, f->part, P);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, f->part, nb, 2 * C + 1, 0, 2 * C,
-                     f->grad + f->L.wl, 0);
+  hipLaunchKernelGGL(k_colsum, dim3(ncs), dim3(256), 0, s, f->dZ[cur], u, f->part, P);
+  hipLaunchKernelGGL(k_reduce, dim3(2), dim3(1024), 0, s, f->part, ncs, 2 * C + 1, 0, 2 * C,
+                     f->grad + f->L.wl);
   FLAUNCH();
   return MCPM_OK;
 }
@@ -709,7 +870,7 @@ int mcpm_fno_create(int32_t n0, int32_t n1, int32_t modes, const float *params, 
   if (!out || !params) return MCPM_ERR_ARG;
   *out = nullptr;
   if (modes < 8 || modes % 8 || modes > 32) return MCPM_ERR_ARG;
-  if (n0 < 64 || n0 % 64 || n1 < 2 * modes || n1 % 4 || 2 * modes > n0) return MCPM_ERR_ARG;
+  if (n0 < 64 || n0 % 64 || n1 < 2 * modes || n1 % 16 || 2 * modes > n0) return MCPM_ERR_ARG;
   mcpm_fno *f = new mcpm_fno;
   f->n0 = n0;
   f->n1 = n1;
@@ -749,8 +910,10 @@ int mcpm_fno_create(int32_t n0, int32_t n1, int32_t modes, const float *params, 
   FA(f->A, (int64_t)n1 * m2 * C * 4);
   FA(f->Y, (int64_t)m2 * modes * C * 8);
   FA(f->dX, (int64_t)m2 * modes * C * 8);
+  FA(f->Xp, (int64_t)((n1 + DFTY_ROWS - 1) / DFTY_ROWS) * m2 * modes * C * 8);
   FA(f->Cp, (int64_t)n1 * C * m2 * 2);
-  FA(f->part, (int64_t)((P + CHUNK - 1) / CHUNK) * (C * C + C) * 4);
+  FA(f->part, std::max((int64_t)((P + CHUNK - 1) / CHUNK) * (C * C + C),
+                        (int64_t)((P + COLSUM_NODES - 1) / COLSUM_NODES) * (2 * C + 1)) * 4);
   FA(f->lpart, ((P + 255) / 256) * 8);
   FA(f->dscr, 64);
 #undef FA
@@ -834,7 +997,7 @@ int mcpm_fno_loss_grad(mcpm_fno *f, const float *u, const float *t, float w0, fl
   settle(f);
   FHIP(hipMemcpyAsync(f->u, u, f->P * 4, hipMemcpyHostToDevice, f->s));
   FHIP(hipMemcpyAsync(f->t, t, f->P * 4, hipMemcpyHostToDevice, f->s));
-  hipLaunchKernelGGL(k_tden, dim3(1), dim3(1024), 0, f->s, f->t, f->dscr, f->n0, f->n1, w0);
+  tden(f, f->s, f->t, f->dscr, w0);
   int st = forward(f, f->s, true, f->u);
   if (!st) st = loss(f, f->s, f->t, f->dscr, w0, (float *)(f->dscr + 1));
   if (!st) st = backward(f, f->s, f->u);
@@ -868,8 +1031,7 @@ int mcpm_fno_add_snapshot_host(mcpm_fno *f, const float *u, const float *t, floa
   const size_t off = (size_t)f->snap_n * f->P;
   FHIP(hipMemcpyAsync(f->su + off, u, f->P * 4, hipMemcpyHostToDevice, f->s));
   FHIP(hipMemcpyAsync(f->st + off, t, f->P * 4, hipMemcpyHostToDevice, f->s));
-  hipLaunchKernelGGL(k_tden, dim3(1), dim3(1024), 0, f->s, f->st + off, f->sD + f->snap_n, f->n0,
-                     f->n1, w0);
+  tden(f, f->s, f->st + off, f->sD + f->snap_n, w0);
   FLAUNCH();
   FHIP(hipStreamSynchronize(f->s));
   ++f->snap_n;
@@ -978,8 +1140,7 @@ int mcpm_fno_add_snapshot(mcpm_fno *f, mcpm_ctx *c, float w0) {
                      f->in_shift, f->in_scale);
   hipLaunchKernelGGL(k_from_nodes, dim3(g), dim3(256), 0, s, pf.pot, f->st + off, f->n0, f->n1,
                      f->out_shift, f->out_scale);
-  hipLaunchKernelGGL(k_tden, dim3(1), dim3(1024), 0, s, f->st + off, f->sD + f->snap_n, f->n0,
-                     f->n1, w0);
+  tden(f, s, f->st + off, f->sD + f->snap_n, w0);
   FLAUNCH();
   if ((st = ctx_plane_finish(c))) return st;
   f->pending = s;

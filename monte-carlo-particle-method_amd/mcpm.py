@@ -632,6 +632,14 @@ class Simulation:
         return st.as_dict()
 
 
+def plane_force(pot, h):
+    """Gradient of the Q1 interpolant of a nodal field at the cell centres
+    (FEM.cpp:570-583 evaluated at xi = eta = 1/2): shape (2, N1, N0)."""
+    gx = 0.5 * ((pot[:-1, 1:] - pot[:-1, :-1]) + (pot[1:, 1:] - pot[1:, :-1])) / h
+    gy = 0.5 * ((pot[1:, :-1] - pot[:-1, :-1]) + (pot[1:, 1:] - pot[:-1, 1:])) / h
+    return np.stack([gx, gy])
+
+
 def _fp(a):
     assert a.dtype == np.float32 and a.flags.c_contiguous
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
@@ -657,6 +665,24 @@ class FieldSurrogate:
         if st:
             raise McpmError(st, "mcpm_fno_create")
         self._h = h
+
+    @staticmethod
+    def init_params(modes: int, seed: int = 0) -> np.ndarray:
+        """Initial FP32 parameters (numpy PCG64 stream of `seed`): lift weight
+        and bias U(-1, 1); per layer W and b U(-1/8, 1/8) (1/sqrt(64)) and the
+        spectral weights U(-1, 1)/64^2 per component; projection weight
+        U(-1/8, 1/8), bias 0.  Layout as in include/mcpm.h."""
+        C = FieldSurrogate.C
+        rng = np.random.default_rng(seed)
+        rsz = 2 * modes * modes * C * C * 2
+        out = [rng.uniform(-1, 1, C), rng.uniform(-1, 1, C)]
+        a = 1.0 / np.sqrt(C)
+        for _ in range(4):
+            out += [rng.uniform(-a, a, C * C), rng.uniform(-a, a, C), rng.uniform(-1, 1, rsz) / (C * C)]
+        out += [rng.uniform(-a, a, C), np.zeros(1)]
+        v = np.concatenate(out).astype(np.float32)
+        n = FieldSurrogate.param_count(modes)
+        return np.concatenate([v, np.zeros(n - v.size, np.float32)])
 
     @staticmethod
     def param_count(modes: int) -> int:
