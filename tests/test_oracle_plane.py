@@ -216,3 +216,42 @@ def test_plane_periodic_free_flight_closed_form():
     assert np.array_equal(c0[idx], c[jdx])
     assert np.abs(x[jdx] - xe[idx]).max() < 1e-12
     assert np.abs(y[jdx] - ye[idx]).max() < 1e-12
+
+
+@pytest.mark.parametrize("h,unstable", [(0.25, True), (0.5, True), (1.0, False)])
+def test_core_stencil_makes_the_grid_mode_attractive_on_fine_meshes(h, unstable):
+    """Sign of the screened-Poisson field's response to a checkerboard of
+    nodal density on a periodic plane (compute_SP_2D with the core
+    subtraction of FEM.cpp:497-554).  The Laplacian part of the core term,
+    dummy2 / dx^2 (FEM.cpp:502), grows as 1/h^2; for h below about 0.67
+    sigma it outweighs the point term at the grid's Nyquist mode, so the
+    potential rises where the density does and, with ast < 0, the force
+    pulls particles into the denser nodes: the grid-scale mode is unstable.
+    On a sigma-sized mesh the response is repulsive."""
+    n = 16
+    L = n * h
+    p, _, _ = W.plane(nx=n, ny=n, Lx=L, Ly=L, ppc=1, boundary=mcpm.BC_PERIODIC,
+                      long_range=mcpm.LR_SCREENED_POISSON, model=mcpm.MODEL_NONE)
+    xs, ys, heavy = [], [], []
+    for j in range(n):
+        for i in range(n):
+            c = 6 if (i + j) % 2 == 0 else 4   # particles per quadrant around node (i, j)
+            for dx in (-0.25 * h, 0.25 * h):
+                for dy in (-0.25 * h, 0.25 * h):
+                    xs += [((i * h + dx) % L)] * c
+                    ys += [((j * h + dy) % L)] * c
+    x, y = np.array(xs), np.array(ys)
+    p.Fn = 0.3 * L * L / x.size
+    p.capacity = x.size
+    z = np.zeros_like(x)
+    o = orc.OracleSim(p)
+    o.upload_xy(x, y, z, z, z)
+    o.cell_update()
+    o.poisson_solver()
+    nd, pot = o.nodes()
+    nd, pot = nd[:-1, :-1], pot[:-1, :-1]
+    even = (np.add.outer(np.arange(n), np.arange(n)) % 2) == 0
+    dn = nd[even].mean() - nd[~even].mean()
+    dphi = pot[even].mean() - pot[~even].mean()
+    assert dn > 0
+    assert (dphi > 0) == unstable, (h, dphi / dn)
