@@ -43,8 +43,19 @@ enum mcpm_precision { MCPM_FP64 = 0, MCPM_FP32 = 1 };
    DIFFUSE: BC_thermal_wall_3D (Main_Functions.cpp:4917-5248), walls at
    wall_lo / wall_hi.  SPECULAR: BC_specular_reflection
    (Main_Functions.cpp:6399-6525).  PERIODIC: a periodic box, which the
-   reference's 1D screened-Poisson path does not offer. */
-enum mcpm_boundary { MCPM_BC_DIFFUSE = 0, MCPM_BC_PERIODIC = 1, MCPM_BC_SPECULAR = 2 };
+   reference's 1D screened-Poisson path does not offer.  REINJECT (slab
+   only): BC_evaporation of the vacuum problem (Main_Functions.cpp:5444-5617):
+   a particle past wall_lo / wall_hi (the reference's 1e-15 and L - 1e-15)
+   is put back at L/2 + U(-1,1) L/4 with a Maxwellian velocity at T_wall_lo
+   (gas.T), and counted (mcpm_get_escaped).  The inverted problem's
+   BC_inverted (Main_Functions.cpp:5251-5441) is DIFFUSE with walls at
+   1e-15 and L - 1e-15 and T_wall_lo = gas.Tc, T_wall_hi = gas.Th. */
+enum mcpm_boundary {
+  MCPM_BC_DIFFUSE = 0,
+  MCPM_BC_PERIODIC = 1,
+  MCPM_BC_SPECULAR = 2,
+  MCPM_BC_REINJECT = 3
+};
 /* Contact value Y(n): the reference's active polynomial
    (Main_Functions.cpp:1906-1915) or Carnahan-Starling (the commented
    alternative at Main_Functions.cpp:1912). */
@@ -320,6 +331,21 @@ int mcpm_reset_samples(mcpm_ctx *ctx);
    unwrapped flight are taken.  Slab only.  p_m accumulates on the device
    until mcpm_reset_samples. */
 int mcpm_measure_pressure(mcpm_ctx *ctx, int32_t enable);
+
+/* Particles re-inserted by the REINJECT boundary since the context was
+   created (the reference's box.out of BC_evaporation, which it counts once
+   step > box.after; here every step). */
+int mcpm_get_escaped(mcpm_ctx *ctx, int64_t *count);
+
+/* thermostat (Main_Functions.cpp:1674-1869), the inverted problem's branch
+   (:1760-1830): after the cell moments are taken from the current particles
+   (cell_update), every particle of a cell whose centre lies below zone_lo
+   (box.thermc) has its peculiar velocity scaled by sqrt(T_lo / T_cell)
+   (gas.Tc), and every particle of a cell whose centre lies above
+   length - zone_hi (box.thermh) by sqrt(T_hi / T_cell) (gas.Th); cells with
+   T_cell = 0 are left alone.  Each cell keeps its mean velocity.  Slab
+   only. */
+int mcpm_thermostat(mcpm_ctx *ctx, double zone_lo, double T_lo, double zone_hi, double T_hi);
 /* The pressure columns of PostProc (myIO.cpp:743-744, 855, 889-890): p_m,
    p_rep = n kB T (1 + n b Y(n)) and ptot = p_rep - 2 b n^2 phi, the last two
    from the current particles (cell T of the moments, cell n).  Arrays of

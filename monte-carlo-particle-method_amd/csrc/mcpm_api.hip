@@ -54,6 +54,7 @@ struct mcpm_ctx {
   bool pressure_on = false;
   int64_t n_samples = 0;  // cell_poly_velocity calls since the last reset
   uint32_t *counters = nullptr, *err = nullptr;
+  unsigned long long *esc = nullptr;  // re-inserted particles (REINJECT boundary)
   uint64_t *res = nullptr;  // reservations, two interleaved arrays
   uint32_t round_base = 0;  // rounds run so far: the next round's tag is ~round_base
   // collisions
@@ -476,6 +477,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   // the face sums are taken once, in the scatter pass, of a moving push
   Kick kick_scatter = kick;
   if (c->pressure_on && (mode & PUSH_MOVE)) kick_scatter.pm = c->press;
+  if (mode & PUSH_MOVE) kick_scatter.esc = c->esc;
   const double *drift = (c->p.model == MCPM_MODEL_FP_DENSE && c->have_closures &&
                          (mode & PUSH_MOVE))
                             ? c->drift
@@ -979,7 +981,10 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
   }
   if (p->capacity < 1 || p->capacity > 2000000000LL) return MCPM_ERR_ARG;
   if (p->precision != MCPM_FP32 && p->precision != MCPM_FP64) return MCPM_ERR_ARG;
-  if (p->boundary < 0 || p->boundary > 2) return MCPM_ERR_ARG;
+  if (p->boundary < 0 || p->boundary > MCPM_BC_REINJECT) return MCPM_ERR_ARG;
+  if (p->boundary == MCPM_BC_REINJECT && (p->dims != 1 || !(p->T_wall_lo > 0.0) ||
+                                          !(p->wall_hi > p->wall_lo)))
+    return MCPM_ERR_ARG;
   if (!(p->length > 0.0) || !(p->delta_t > 0.0) || !(p->cell_volume > 0.0)) return MCPM_ERR_ARG;
   if (p->long_range == MCPM_LR_SCREENED_POISSON && !(p->bst > 0.0)) return MCPM_ERR_ARG;
   if (p->long_range < MCPM_LR_NONE || p->long_range > MCPM_LR_SCREENED_POISSON)
@@ -1119,6 +1124,7 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
   if ((st = dalloc(&c->clos, (size_t)nc * MCPM_NCLOS))) return fail(st);
   if ((st = dalloc(&c->drift, (size_t)nc * DRIFT_N))) return fail(st);
   if ((st = dalloc(&c->err, 1))) return fail(st);
+  if ((st = dalloc(&c->esc, 1))) return fail(st);
   if (hipHostMalloc((void **)&c->hleft, (size_t)(MAX_ROUNDS + 2) * sizeof(uint32_t),
                     hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->dleft, c->hleft, 0) != hipSuccess)
@@ -1130,6 +1136,7 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
       hipMemset(c->samples, 0, (size_t)nc * MCPM_NSAMP * sizeof(double)) != hipSuccess ||
       hipMemset(c->press, 0, (size_t)nc * 3 * sizeof(double)) != hipSuccess ||
       hipMemset(c->err, 0, sizeof(uint32_t)) != hipSuccess ||
+      hipMemset(c->esc, 0, sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->cnt, 0, nc * sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->start, 0, (nc + 1) * sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->res, 0xFF, 2 * (size_t)p->capacity * sizeof(uint64_t)) != hipSuccess)
@@ -1155,7 +1162,7 @@ int mcpm_create_batch(const mcpm_params *common, int32_t n_rep, const mcpm_repli
   *out = nullptr;
   // independent slabs of one length: ESMC or free flight, no field or the
   // screened-Poisson field
-  if (common->dims > 1) return MCPM_ERR_ARG;
+  if (common->dims > 1 || common->boundary == MCPM_BC_REINJECT) return MCPM_ERR_ARG;
   if (common->model != MCPM_MODEL_ESMC && common->model != MCPM_MODEL_NONE) return MCPM_ERR_ARG;
   if (common->long_range != MCPM_LR_NONE && common->long_range != MCPM_LR_SCREENED_POISSON)
     return MCPM_ERR_ARG;
@@ -1204,7 +1211,7 @@ int mcpm_destroy(mcpm_ctx *c) {
                   c->keys_tmp,
                   c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
-                  c->samples, c->press, c->counters, c->err, c->res, c->opA,
+                  c->samples, c->press, c->counters, c->err, c->esc, c->res, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
                   c->fem_buf, c->q4};
   for (void *b : bufs)
@@ -1277,6 +1284,26 @@ int download_any(mcpm_ctx *c, double *const *dst, int ndst) {
   else unpack<double>(host, n, stride, dst, ndst);
   return MCPM_OK;
 }
+}  // namespace
+
+namespace {
+
+template <typename R>
+int launch_thermostat(mcpm_ctx *c, double zone_lo, double T_lo, double zone_hi, double T_hi) {
+  if constexpr (sizeof(R) == 4 * sizeof(typename R::real_type)) {
+    int st = launch_moments<R>(c);
+    if (st) return st;
+    if (c->n > 0)
+      hipLaunchKernelGGL((k_thermostat<R>), dim3((unsigned)((c->n + 255) / 256)), dim3(256), 0,
+                         c->s, (R *)c->P[c->cur], c->n, c->moments, c->kp, zone_lo, T_lo, zone_hi,
+                         T_hi);
+    LAUNCHCHK();
+    return MCPM_OK;
+  } else {
+    return MCPM_ERR_ARG;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1662,6 +1689,23 @@ int mcpm_measure_pressure(mcpm_ctx *c, int32_t enable) {
   if (c->p.dims != 1) return MCPM_ERR_ARG;
   c->pressure_on = enable != 0;
   return MCPM_OK;
+}
+
+int mcpm_get_escaped(mcpm_ctx *c, int64_t *count) {
+  if (!c || !count) return MCPM_ERR_ARG;
+  unsigned long long v = 0;
+  HIPCHK(hipMemcpyAsync(&v, c->esc, sizeof v, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  *count = (int64_t)v;
+  return MCPM_OK;
+}
+
+int mcpm_thermostat(mcpm_ctx *c, double zone_lo, double T_lo, double zone_hi, double T_hi) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims != 1 || c->kp.nrep) return MCPM_ERR_ARG;
+  if (!(T_lo >= 0.0) || !(T_hi >= 0.0)) return MCPM_ERR_ARG;
+  if (!c->sorted) return MCPM_ERR_STATE;
+  return MCPM_DISPATCH(c, launch_thermostat, c, zone_lo, T_lo, zone_hi, T_hi);
 }
 
 int mcpm_get_pressure(mcpm_ctx *c, double *p_m, double *p_rep, double *p_tot) {

@@ -38,6 +38,7 @@ struct Kick {
   const double *xnode;
   double coef;
   double *pm;  // measure_pressure face sums (slab), or null when not sampling
+  unsigned long long *esc;  // re-inserted particles of the evaporation boundary, or null
 };
 
 // Constants every kernel reads, passed by value.  n_cells is the total

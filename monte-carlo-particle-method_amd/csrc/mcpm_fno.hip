@@ -137,47 +137,59 @@ __global__ __launch_bounds__(256) void k_dftx(const T *__restrict__ act,
     A[(size_t)y * m2 * C + e] = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
 }
 
-// y-DFT of the retained kx, partial over a group of DFTY_ROWS rows:
-// Xp[g][a][kx][c] = sum_{y in group g} A[y][kx][c] e^{-i phi(a, y)}.
-// Grid (m, 2m/4, groups); wave -> a, lane -> channel.
-constexpr int DFTY_ROWS = 32;
-__global__ __launch_bounds__(256) void k_dfty(const float *__restrict__ A,
-                                              const float2 *__restrict__ twy,
-                                              float2 *__restrict__ Xp, int n1, int m) {
-  const int kx = blockIdx.x, a = blockIdx.y * 4 + (threadIdx.x >> 6), c = threadIdx.x & 63;
-  const int m2 = 2 * m, g = blockIdx.z;
-  const int y0 = g * DFTY_ROWS, y1 = min(n1, y0 + DFTY_ROWS);
-  float xr0 = 0.f, xi0 = 0.f, xr1 = 0.f, xi1 = 0.f;
-  const float2 *tw = twy + (size_t)a * n1;
-  for (int y = y0; y < y1; y += 2) {  // n1 and DFTY_ROWS are even
-    const float2 t0 = tw[y], t1 = tw[y + 1];
-    const float ar0 = A[((size_t)y * m2 + kx) * C + c], ai0 = A[((size_t)y * m2 + m + kx) * C + c];
-    const float ar1 = A[((size_t)(y + 1) * m2 + kx) * C + c],
-                ai1 = A[((size_t)(y + 1) * m2 + m + kx) * C + c];
-    xr0 += ar0 * t0.x + ai0 * t0.y;
-    xi0 += ai0 * t0.x - ar0 * t0.y;
-    xr1 += ar1 * t1.x + ai1 * t1.y;
-    xi1 += ai1 * t1.x - ar1 * t1.y;
-  }
-  Xp[(((size_t)g * m2 + a) * m + kx) * C + c] = make_float2(xr0 + xr1, xi0 + xi1);
+__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// X[e] = s(kx) sum_g Xp[g][e] over the 2m m 64 mode entries.
-__global__ __launch_bounds__(256) void k_dfty_sum(const float2 *__restrict__ Xp, int groups,
-                                                  float2 *__restrict__ X, int m, float s0,
-                                                  float s1) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ne = 2 * m * m * C;
-  if (e >= ne) return;
-  float xr = 0.f, xi = 0.f;
-  for (int g = 0; g < groups; ++g) {
-    const float2 v = Xp[(size_t)g * ne + e];
-    xr += v.x;
-    xi += v.y;
+// y-DFT of the retained kx on FP32 MFMA:
+// X[a][kx][c] = s(kx) sum_y A[y][kx][c] e^{-i phi(a, y)} (A complex from the
+// packed re | im rows).  Workgroup: one kx (blockIdx.x), a 16-row tile of a
+// (blockIdx.y), a 16-channel tile (blockIdx.z); its four waves split the rows
+// y and meet in LDS.  D[a][c] = sum_y tw(a, y) A[y][c].
+__global__ __launch_bounds__(256) void k_dfty(const float *__restrict__ A,
+                                              const float2 *__restrict__ twy,
+                                              float2 *__restrict__ X, int n1, int m, float s0,
+                                              float s1) {
+  __shared__ float red[4][64][8];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int r16 = l & 15, q = l >> 4;
+  const int kx = blockIdx.x, a0 = blockIdx.y * 16, c0 = blockIdx.z * 16, m2 = 2 * m;
+  f32x4 xr = {0.f, 0.f, 0.f, 0.f}, xi = {0.f, 0.f, 0.f, 0.f};
+  const int yq = n1 / 4, yb = w * yq;  // n1 is a multiple of 16
+  const float2 *tw = twy + (size_t)(a0 + r16) * n1 + q;
+  const float *ar = A + (size_t)kx * C + c0 + r16 + (size_t)q * m2 * C;
+  const float *ai = ar + (size_t)m * C;
+  for (int y0 = yb; y0 < yb + yq; y0 += 8) {  // yq is a multiple of 4; two steps in flight
+    const float2 t = tw[y0];
+    const float br = ar[(size_t)y0 * m2 * C], bi = ai[(size_t)y0 * m2 * C];
+    const bool two = y0 + 4 < yb + yq;
+    const float2 t2 = two ? tw[y0 + 4] : make_float2(0.f, 0.f);
+    const float br2 = two ? ar[(size_t)(y0 + 4) * m2 * C] : 0.f;
+    const float bi2 = two ? ai[(size_t)(y0 + 4) * m2 * C] : 0.f;
+    xr = mfma4(t.x, br, xr);
+    xr = mfma4(t.y, bi, xr);
+    xi = mfma4(t.x, bi, xi);
+    xi = mfma4(-t.y, br, xi);
+    xr = mfma4(t2.x, br2, xr);
+    xr = mfma4(t2.y, bi2, xr);
+    xi = mfma4(t2.x, bi2, xi);
+    xi = mfma4(-t2.y, br2, xi);
   }
-  const int kx = (e / C) % m;
-  const float sc = kx ? s1 : s0;
-  X[e] = make_float2(xr * sc, xi * sc);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    red[w][l][j] = xr[j];
+    red[w][l][4 + j] = xi[j];
+  }
+  __syncthreads();
+  if (w == 0) {
+    const float sc = kx ? s1 : s0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const float vr = (red[0][l][j] + red[1][l][j]) + (red[2][l][j] + red[3][l][j]);
+      const float vi = (red[0][l][4 + j] + red[1][l][4 + j]) + (red[2][l][4 + j] + red[3][l][4 + j]);
+      X[((size_t)(a0 + 4 * q + j) * m + kx) * C + c0 + r16] = make_float2(vr * sc, vi * sc);
+    }
+  }
 }
 
 // Spectral mixing Y[mode][o] = sum_i X[mode][i] R[mode][i][o]; wave -> mode, lane -> o.
@@ -230,10 +242,6 @@ __global__ __launch_bounds__(256) void k_mix_dr(const float2 *__restrict__ X,
     }
     gm[(size_t)i * C] = g;
   }
-}
-
-__device__ __forceinline__ f32x4 mfma4(float a, float b, f32x4 c) {
-  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
 __device__ __forceinline__ uint32_t pack2(float a, float b) {
@@ -292,12 +300,13 @@ enum : int { CMB_FWD = 0, CMB_LAST = 1, CMB_BWD = 2 };
 // BWD: dh = out; outZ = dh gelu'(Zprev) (or dh when Zprev is null).
 // The MFMA computes the transpose, D[channel][node], so that a lane holds
 // four consecutive channels of one node: 8- and 16-byte stores.
-template <int MODE, typename TIn>
+template <int MODE, typename TIn, int TILES>
 __global__ __launch_bounds__(256) void k_combine(
     const TIn *__restrict__ in, const uint16_t *__restrict__ Wt, const uint16_t *__restrict__ tinv,
     const uint16_t *__restrict__ Cp, const float *__restrict__ bias, const float *__restrict__ wq,
     const float *__restrict__ Zprev, float *__restrict__ Z, uint16_t *__restrict__ H,
-    float *__restrict__ yout, int n0, int m2, int tiles) {
+    float *__restrict__ yout, int n0, int m2) {
+  constexpr int tiles = TILES;
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int r16 = l & 15, q = l >> 4;
   // `tiles` consecutive 64-node tiles of one row; the weights and the row's
@@ -323,6 +332,7 @@ __global__ __launch_bounds__(256) void k_combine(
       bb[t][j] = MODE == CMB_BWD ? 0.f : bias[16 * t + 4 * q + j];
       wqv[t][j] = MODE == CMB_LAST ? wq[16 * t + 4 * q + j] : 0.f;
     }
+#pragma unroll
   for (int tt = 0; tt < tiles; ++tt) {
     f32x4 acc[4];
 #pragma unroll
@@ -682,7 +692,7 @@ struct mcpm_fno {
   float *dZ[2] = {};
   float *A = nullptr;
   float2 *X[LAYERS] = {};
-  float2 *Y = nullptr, *dX = nullptr, *Xp = nullptr;
+  float2 *Y = nullptr, *dX = nullptr;
   uint16_t *Cp = nullptr;
   float *part = nullptr;
   double *lpart = nullptr, *dscr = nullptr;
@@ -705,14 +715,12 @@ int falloc(mcpm_fno *f, void **p, size_t bytes) {
 
 int nchunks(const mcpm_fno *f) { return (int)((f->P + CHUNK - 1) / CHUNK); }
 int ncolsum(const mcpm_fno *f) { return (int)((f->P + COLSUM_NODES - 1) / COLSUM_NODES); }
-int ndfty(const mcpm_fno *f) { return (f->n1 + DFTY_ROWS - 1) / DFTY_ROWS; }
 
-// X = s(kx) DFT_y(A) through the row-group partials.
+// X = s(kx) DFT_y(A).
 void dfty(mcpm_fno *f, hipStream_t s, float2 *X, float s0, float s1) {
-  const int m = f->m, m2 = 2 * m, G = ndfty(f);
-  hipLaunchKernelGGL(k_dfty, dim3(m, m2 / 4, G), dim3(256), 0, s, f->A, f->twy, f->Xp, f->n1, m);
-  hipLaunchKernelGGL(k_dfty_sum, dim3((m2 * m * C + 255) / 256), dim3(256), 0, s, f->Xp, G, X, m,
-                     s0, s1);
+  const int m = f->m;
+  hipLaunchKernelGGL(k_dfty, dim3(m, 2 * m / 16, C / 16), dim3(256), 0, s, f->A, f->twy, X, f->n1,
+                     m, s0, s1);
 }
 
 void idfty(mcpm_fno *f, hipStream_t s, const float2 *Y, float s0, float s1) {
@@ -724,6 +732,23 @@ void idfty(mcpm_fno *f, hipStream_t s, const float2 *Y, float s0, float s1) {
 int combine_tiles(const mcpm_fno *f) {
   const int r = f->n0 / 64;
   return r % 4 == 0 ? 4 : (r % 2 == 0 ? 2 : 1);
+}
+
+template <int MODE, typename TIn>
+void combine(const mcpm_fno *f, hipStream_t s, const TIn *in, const uint16_t *Wt, const float *bias,
+             const float *wq, const float *Zprev, float *Z, uint16_t *H, float *yout) {
+  const int ct = combine_tiles(f);
+  const dim3 g((unsigned)(f->P / (64 * ct))), b(256);
+  const int m2 = 2 * f->m;
+  if (ct == 4)
+    hipLaunchKernelGGL((k_combine<MODE, TIn, 4>), g, b, 0, s, in, Wt, f->tinv, f->Cp, bias, wq,
+                       Zprev, Z, H, yout, f->n0, m2);
+  else if (ct == 2)
+    hipLaunchKernelGGL((k_combine<MODE, TIn, 2>), g, b, 0, s, in, Wt, f->tinv, f->Cp, bias, wq,
+                       Zprev, Z, H, yout, f->n0, m2);
+  else
+    hipLaunchKernelGGL((k_combine<MODE, TIn, 1>), g, b, 0, s, in, Wt, f->tinv, f->Cp, bias, wq,
+                       Zprev, Z, H, yout, f->n0, m2);
 }
 
 // D[0] = sum over the snapshot t of (|grad t|^2 + w0 t^2).
@@ -748,7 +773,6 @@ int forward(mcpm_fno *f, hipStream_t s, bool train, const float *u) {
   const float sinv0 = (float)(1.0 / ((double)n0 * n1)), sinv1 = (float)(2.0 / ((double)n0 * n1));
   const int64_t P = f->P;
   auto hbuf = [&](int l) { return train ? f->H[l] : f->H[l & 1]; };
-  const int ct = combine_tiles(f);
   hipLaunchKernelGGL(k_lift, dim3((unsigned)((P * (C / 8) + 255) / 256)), dim3(256), 0, s, u,
                      f->params + f->L.wl, hbuf(0), P);
   for (int l = 0; l < LAYERS; ++l) {
@@ -761,15 +785,11 @@ int forward(mcpm_fno *f, hipStream_t s, bool train, const float *u) {
     idfty(f, s, f->Y, sinv0, sinv1);
     float *Zl = train ? f->Z[l] : nullptr;
     if (l < LAYERS - 1) {
-      hipLaunchKernelGGL((k_combine<CMB_FWD, uint16_t>), dim3((unsigned)(P / (64 * ct))), dim3(256), 0,
-                         s, h, f->Wf + l * C * C, f->tinv, f->Cp, f->params + f->L.b(l),
-                         (const float *)nullptr, (const float *)nullptr, Zl, hbuf(l + 1),
-                         (float *)nullptr, n0, m2, ct);
+      combine<CMB_FWD, uint16_t>(f, s, h, f->Wf + l * C * C, f->params + f->L.b(l), nullptr,
+                                 nullptr, Zl, hbuf(l + 1), nullptr);
     } else {
-      hipLaunchKernelGGL((k_combine<CMB_LAST, uint16_t>), dim3((unsigned)(P / (64 * ct))), dim3(256), 0,
-                         s, h, f->Wf + l * C * C, f->tinv, f->Cp, f->params + f->L.b(l),
-                         f->params + f->L.wq, (const float *)nullptr, Zl, (uint16_t *)nullptr,
-                         f->y, n0, m2, ct);
+      combine<CMB_LAST, uint16_t>(f, s, h, f->Wf + l * C * C, f->params + f->L.b(l),
+                                  f->params + f->L.wq, nullptr, Zl, nullptr, f->y);
     }
   }
   FLAUNCH();
@@ -784,7 +804,6 @@ int backward(mcpm_fno *f, hipStream_t s, const float *u) {
   const int nb = nchunks(f);
   const float sadj0 = (float)(1.0 / ((double)n0 * n1)), sadj1 = (float)(2.0 / ((double)n0 * n1));
   const unsigned gPC = (unsigned)((P * C + 255) / 256);
-  const int ct = combine_tiles(f);
   // projection: wq, bq
   const int ncs = ncolsum(f);
   hipLaunchKernelGGL(k_colsum, dim3(ncs), dim3(256), 0, s, f->Z[LAYERS - 1], f->dy, f->part, P);
@@ -807,10 +826,8 @@ int backward(mcpm_fno *f, hipStream_t s, const float *u) {
     hipLaunchKernelGGL(k_mix_dx, dim3(m2 * m / 4), dim3(256), 0, s, f->Y,
                        (const float2 *)(f->params + f->L.R(l)), f->dX);
     idfty(f, s, f->dX, 1.0f, 1.0f);
-    hipLaunchKernelGGL((k_combine<CMB_BWD, float>), dim3((unsigned)(P / (64 * ct))), dim3(256), 0, s, dz,
-                       f->Wb + l * C * C, f->tinv, f->Cp, (const float *)nullptr,
-                       (const float *)nullptr, l > 0 ? f->Z[l - 1] : (const float *)nullptr, dzn,
-                       (uint16_t *)nullptr, (float *)nullptr, n0, m2, ct);
+    combine<CMB_BWD, float>(f, s, dz, f->Wb + l * C * C, nullptr, nullptr,
+                            l > 0 ? f->Z[l - 1] : nullptr, dzn, nullptr, nullptr);
     cur ^= 1;
   }
   // lift: wl, bl from dH0 = dZ[cur]
@@ -910,7 +927,6 @@ int mcpm_fno_create(int32_t n0, int32_t n1, int32_t modes, const float *params, 
   FA(f->A, (int64_t)n1 * m2 * C * 4);
   FA(f->Y, (int64_t)m2 * modes * C * 8);
   FA(f->dX, (int64_t)m2 * modes * C * 8);
-  FA(f->Xp, (int64_t)((n1 + DFTY_ROWS - 1) / DFTY_ROWS) * m2 * modes * C * 8);
   FA(f->Cp, (int64_t)n1 * C * m2 * 2);
   FA(f->part, std::max((int64_t)((P + CHUNK - 1) / CHUNK) * (C * C + C),
                         (int64_t)((P + COLSUM_NODES - 1) / COLSUM_NODES) * (2 * C + 1)) * 4);

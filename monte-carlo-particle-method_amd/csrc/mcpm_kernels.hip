@@ -110,6 +110,31 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
     if (kick.pm) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
     return r;
   }
+  if (kp.boundary == MCPM_BC_REINJECT) {
+    // BC_evaporation (Main_Functions.cpp:5444-5617), slab axis: a particle
+    // past a wall leaves the system and is put back at L/2 + U(-1,1) L/4 with
+    // a Maxwellian velocity at T_lo (gas.T).  Draws: Philox(slot, 2k | 2k+1,
+    // step, REINJECT).  The re-inserted position is inside, so k ends at 1.
+    uint32_t k = 0;
+    while ((y < kp.wall_lo || y > kp.wall_hi) && k < 4) {
+      const uint32_t ctr3 = mcpm_ctr3(MCPM_TAG_REINJECT, step_hi, (uint32_t)(slot >> 32));
+      mcpm_u32x4 a = mcpm_philox((uint32_t)slot, 2u * k, step_lo, ctr3, kp.key0, kp.key1);
+      mcpm_u32x4 b = mcpm_philox((uint32_t)slot, 2u * k + 1u, step_lo, ctr3, kp.key0, kp.key1);
+      y = 0.5 * kp.length + (2.0 * mcpm_u01(a.v[0]) - 1.0) * (0.25 * kp.length);
+      double z0, z1, z2, z3;
+      mcpm_normal2(a.v[1], a.v[2], &z0, &z1);
+      mcpm_normal2(b.v[0], b.v[1], &z2, &z3);
+      const double sd = __builtin_sqrt(kp.kb * kp.T_lo / kp.mass);
+      r.u1 = (real)(sd * z0);
+      r.u2 = (real)(sd * z1);
+      r.u3 = (real)(sd * z2);
+      if (kick.esc) atomicAdd(kick.esc, 1ull);
+      ++k;
+    }
+    r.y = (real)y;
+    if (kick.pm && k == 0) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
+    return r;
+  }
   // diffuse walls
   double dt_left = kp.dt;
   double v_prev = v_eff;
@@ -1578,5 +1603,38 @@ MCPM_INST(Rec<float>)
 MCPM_INST(Rec<double>)
 MCPM_INST(Rec2<float>)
 MCPM_INST(Rec2<double>)
+
+// thermostat, the inverted problem's branch (Main_Functions.cpp:1760-1830):
+// U = lambda (U - U_cell) + U_cell with lambda = sqrt(T_target / T_cell) for
+// the particles of cells whose centre lies below zone_lo (T_lo) or above
+// length - zone_hi (T_hi); cells with T_cell = 0 keep their velocities.
+// mom: the per-cell moments of these positions (U at 0..2, T at 29).
+template <typename R>
+__global__ __launch_bounds__(256) void k_thermostat(R *__restrict__ P, int64_t n,
+                                                    const double *__restrict__ mom, KParams kp,
+                                                    double zone_lo, double T_lo, double zone_hi,
+                                                    double T_hi) {
+  using real = typename R::real_type;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  R r = P[i];
+  const int32_t c = cell_of((double)r.y, kp.dy, kp.n_cells);
+  const double centre = ((double)c + 0.5) * kp.dy;
+  double Tt;
+  if (centre < zone_lo) Tt = T_lo;
+  else if (centre > kp.length - zone_hi) Tt = T_hi;
+  else return;
+  const double *m = mom + (size_t)c * MCPM_NMOM;
+  if (!(m[29] > 0.0)) return;
+  const double lam = __builtin_sqrt(Tt / m[29]);
+  r.u1 = (real)(lam * ((double)r.u1 - m[0]) + m[0]);
+  r.u2 = (real)(lam * ((double)r.u2 - m[1]) + m[1]);
+  r.u3 = (real)(lam * ((double)r.u3 - m[2]) + m[2]);
+  P[i] = r;
+}
+template __global__ void k_thermostat<Rec<float>>(Rec<float> *, int64_t, const double *, KParams,
+                                                  double, double, double, double);
+template __global__ void k_thermostat<Rec<double>>(Rec<double> *, int64_t, const double *,
+                                                   KParams, double, double, double, double);
 
 }  // namespace mcpm

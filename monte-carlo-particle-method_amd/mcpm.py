@@ -30,7 +30,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmcpm.so")
 
 FP64, FP32 = 0, 1
-BC_DIFFUSE, BC_PERIODIC, BC_SPECULAR = 0, 1, 2
+BC_DIFFUSE, BC_PERIODIC, BC_SPECULAR, BC_REINJECT = 0, 1, 2, 3
 Y_REFERENCE, Y_CARNAHAN_STARLING = 0, 1
 LR_NONE, LR_DIRECT, LR_DENSITY_EXPANSION, LR_SCREENED_POISSON = 0, 1, 2, 3
 MODEL_ESMC, MODEL_FP_DENSE, MODEL_FP_IDEAL, MODEL_DSMC_VHS, MODEL_NONE = 0, 1, 2, 3, 4
@@ -270,6 +270,9 @@ def load_library() -> ctypes.CDLL:
         "mcpm_get_samples": (i32, [vp, dpp]),
         "mcpm_reset_samples": (i32, [vp]),
         "mcpm_measure_pressure": (i32, [vp, i32]),
+        "mcpm_get_escaped": (i32, [vp, ctypes.POINTER(i64)]),
+        "mcpm_thermostat": (i32, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double]),
         "mcpm_read_settings": (i32, [ctypes.c_char_p, ctypes.POINTER(CSettings)]),
         "mcpm_create_batch": (i32, [ctypes.POINTER(CParams), i32, ctypes.POINTER(CReplica),
                                     ctypes.POINTER(vp)]),
@@ -568,6 +571,17 @@ class Simulation:
         self._check(self._lib.mcpm_write_postproc(self._h, ctypes.byref(settings), step,
                                                   os.fsencode(directory)),
                     "mcpm_write_postproc")
+
+    def escaped(self) -> int:
+        """Particles the evaporation boundary (BC_REINJECT) has re-inserted."""
+        v = ctypes.c_int64()
+        self._check(self._lib.mcpm_get_escaped(self._h, ctypes.byref(v)), "mcpm_get_escaped")
+        return v.value
+
+    def thermostat(self, zone_lo: float, T_lo: float, zone_hi: float, T_hi: float):
+        """thermostat of the inverted problem (Main_Functions.cpp:1760-1830)."""
+        self._check(self._lib.mcpm_thermostat(self._h, zone_lo, T_lo, zone_hi, T_hi),
+                    "mcpm_thermostat")
 
     def measure_pressure(self, enable: bool = True):
         """measure_pressure (Main_Functions.cpp:13-34) on every moving push."""

@@ -105,6 +105,7 @@ struct Oracle {
   // measure_pressure face sums (slab)
   std::vector<double> pm;
   bool pressure_on = false;
+  int64_t escaped = 0;  // particles re-inserted by the REINJECT boundary
   bool kick_pending = false;
   bool sorted = false;
   uint32_t err = 0;
@@ -293,6 +294,32 @@ void orc_push_one(Oracle &o, int64_t slot, bool kick, bool move, uint32_t slo, u
     r[0] = rnd<real>(y);
     r[2] = rnd<real>(v);
     if (o.pressure_on) orc_pressure_faces(o, y_old, r[0], r[2], false);
+    return;
+  }
+  if (p.boundary == MCPM_BC_REINJECT) {
+    // BC_evaporation (Main_Functions.cpp:5444-5617): a particle past y > L - eps
+    // (:5519-5546) or y < eps (:5547-5574) is put back at
+    // L/2 + uniform(-1, 1) L/4 with three normal velocity components at
+    // gas.T (here T_wall_lo), and box.out counts it.
+    uint32_t k = 0;
+    while ((y < p.wall_lo || y > p.wall_hi) && k < 4) {
+      uint32_t key0 = (uint32_t)p.seed, key1 = (uint32_t)(p.seed >> 32);
+      const uint32_t c3 = mcpm_ctr3(MCPM_TAG_REINJECT, shi, (uint32_t)((uint64_t)slot >> 32));
+      mcpm_u32x4 a = mcpm_philox((uint32_t)slot, 2u * k, slo, c3, key0, key1);
+      mcpm_u32x4 b = mcpm_philox((uint32_t)slot, 2u * k + 1u, slo, c3, key0, key1);
+      y = 0.5 * p.length + (2.0 * mcpm_u01(a.v[0]) - 1.0) * (0.25 * p.length);
+      double z0, z1, z2, z3;
+      mcpm_normal2(a.v[1], a.v[2], &z0, &z1);
+      mcpm_normal2(b.v[0], b.v[1], &z2, &z3);
+      const double sd = sqrt(p.kb * p.T_wall_lo / p.mass);
+      r[1] = rnd<real>(sd * z0);
+      r[2] = rnd<real>(sd * z1);
+      r[3] = rnd<real>(sd * z2);
+      ++o.escaped;
+      ++k;
+    }
+    r[0] = rnd<real>(y);
+    if (o.pressure_on && k == 0) orc_pressure_faces(o, y_old, r[0], r[2], false);
     return;
   }
   // diffuse walls (BC_thermal_wall_3D, upper/lower wall branches :5044-5127)
@@ -2192,6 +2219,40 @@ int orc_step(void *h, int64_t step, mcpm_collision_stats *st) {
 
 int orc_moments(void *h, double *out) {
   orc_moments(*(Oracle *)h, out);
+  return MCPM_OK;
+}
+
+int orc_get_escaped(void *h, int64_t *count) {
+  *count = ((Oracle *)h)->escaped;
+  return MCPM_OK;
+}
+
+// thermostat, inverted problem (Main_Functions.cpp:1760-1830): cells by cell
+// centre, lambda = sqrt(T_target / cells[num].T), U = lambda (U - U_space) +
+// U_space for every particle of the cell (cells with T = 0 are skipped).
+int orc_thermostat(void *h, double zone_lo, double T_lo, double zone_hi, double T_hi) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 1) return MCPM_ERR_ARG;
+  std::vector<double> mom((size_t)o.nc * MCPM_NMOM);
+  orc_moments(o, mom.data());
+  const bool f32 = o.p.precision == MCPM_FP32;
+  for (int c = 0; c < o.nc; ++c) {
+    const double centre = ((double)c + 0.5) * o.dy;
+    double Tt;
+    if (centre < zone_lo) Tt = T_lo;
+    else if (centre > o.p.length - zone_hi) Tt = T_hi;
+    else continue;
+    const double *m = &mom[(size_t)c * MCPM_NMOM];
+    if (!(m[29] > 0.0)) continue;
+    const double lam = sqrt(Tt / m[29]);
+    for (int u = 0; u < o.cnt[c]; ++u) {
+      double *r = &o.rec[4 * (size_t)(o.start[c] + u)];
+      for (int d = 0; d < 3; ++d) {
+        const double v = lam * (r[1 + d] - m[d]) + m[d];
+        r[1 + d] = f32 ? (double)(float)v : v;
+      }
+    }
+  }
   return MCPM_OK;
 }
 

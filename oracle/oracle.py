@@ -56,6 +56,8 @@ def load() -> ctypes.CDLL:
         "orc_get_samples": (i32, [vp, dpp]),
         "orc_reset_samples": (i32, [vp]),
         "orc_measure_pressure": (i32, [vp, i32]),
+        "orc_get_escaped": (i32, [vp, ctypes.POINTER(i64)]),
+        "orc_thermostat": (i32, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
         "orc_get_pressure": (i32, [vp, dpp, dpp, dpp]),
         "orc_set_omega_max": (i32, [vp, dpp]),
         "orc_init_omega_max": (i32, [vp, ctypes.c_double, i32]),
@@ -278,6 +280,14 @@ class OracleSim:
 
     def reset_samples(self):
         self._check(self._lib.orc_reset_samples(self._h), "orc_reset_samples")
+
+    def escaped(self) -> int:
+        v = ctypes.c_int64()
+        self._check(self._lib.orc_get_escaped(self._h, ctypes.byref(v)), "orc_get_escaped")
+        return v.value
+
+    def thermostat(self, zone_lo, T_lo, zone_hi, T_hi):
+        self._check(self._lib.orc_thermostat(self._h, zone_lo, T_lo, zone_hi, T_hi), "orc_thermostat")
 
     def measure_pressure(self, enable=True):
         self._check(self._lib.orc_measure_pressure(self._h, 1 if enable else 0),
