@@ -269,24 +269,3 @@ def fem_transfer(n0, n1, h, lam):
 def fem_solve_fft(n, h, lam):
     n1, n0 = n.shape
     return np.real(np.fft.ifft2(np.fft.fft2(n) * fem_transfer(n0, n1, h, lam)))
-
-
-def force_error_floor(n, h, lam, m):
-    """Smallest relative error of the forward-difference gradient that any
-    map of the form  phi_hat = F(k) n_hat  with F free on the retained box
-    |kx|, |ky| < m and one constant outside it can reach: the floor of a
-    spectral layer whose only path to the other modes is pointwise."""
-    n1, n0 = n.shape
-    T = fem_transfer(n0, n1, h, lam)
-    nh = np.fft.fft2(n)
-    tx = 2 * np.pi * np.fft.fftfreq(n0)
-    ty = 2 * np.pi * np.fft.fftfreq(n1)
-    G = np.abs(np.exp(1j * tx)[None, :] - 1) ** 2 + np.abs(np.exp(1j * ty)[:, None] - 1) ** 2
-    kx = np.abs(np.fft.fftfreq(n0) * n0)[None, :]
-    ky = np.abs(np.fft.fftfreq(n1) * n1)[:, None]
-    out = ~((kx < m) & (ky < m))
-    w = G * np.abs(nh) ** 2
-    if not out.any():
-        return 0.0
-    a = (w[out] * T[out]).sum() / w[out].sum()
-    return float(np.sqrt((w[out] * (T[out] - a) ** 2).sum() / (w * T ** 2).sum()))
