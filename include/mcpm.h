@@ -337,6 +337,11 @@ int mcpm_measure_pressure(mcpm_ctx *ctx, int32_t enable);
    step > box.after; here every step). */
 int mcpm_get_escaped(mcpm_ctx *ctx, int64_t *count);
 
+/* Energy check, FP64 on the device whatever the particle precision: out[0]
+   = (1/2) m Fn sum |U|^2 (kinetic energy of the molecules the particles
+   stand for), out[1..3] = m Fn sum U (momentum).  Not on batched replicas. */
+int mcpm_energy_momentum(mcpm_ctx *ctx, double *out);
+
 /* thermostat (Main_Functions.cpp:1674-1869), the inverted problem's branch
    (:1760-1830): after the cell moments are taken from the current particles
    (cell_update), every particle of a cell whose centre lies below zone_lo

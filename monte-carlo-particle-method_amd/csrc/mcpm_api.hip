@@ -55,6 +55,7 @@ struct mcpm_ctx {
   int64_t n_samples = 0;  // cell_poly_velocity calls since the last reset
   uint32_t *counters = nullptr, *err = nullptr;
   unsigned long long *esc = nullptr;  // re-inserted particles (REINJECT boundary)
+  double *energy_part = nullptr;      // per-workgroup sums of the energy check
   uint64_t *res = nullptr;  // reservations, two interleaved arrays
   uint32_t round_base = 0;  // rounds run so far: the next round's tag is ~round_base
   // collisions
@@ -1125,6 +1126,7 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
   if ((st = dalloc(&c->drift, (size_t)nc * DRIFT_N))) return fail(st);
   if ((st = dalloc(&c->err, 1))) return fail(st);
   if ((st = dalloc(&c->esc, 1))) return fail(st);
+  if ((st = dalloc(&c->energy_part, 4 * 1024))) return fail(st);
   if (hipHostMalloc((void **)&c->hleft, (size_t)(MAX_ROUNDS + 2) * sizeof(uint32_t),
                     hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->dleft, c->hleft, 0) != hipSuccess)
@@ -1211,7 +1213,7 @@ int mcpm_destroy(mcpm_ctx *c) {
                   c->keys_tmp,
                   c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
-                  c->samples, c->press, c->counters, c->err, c->esc, c->res, c->opA,
+                  c->samples, c->press, c->counters, c->err, c->esc, c->energy_part, c->res, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
                   c->fem_buf, c->q4};
   for (void *b : bufs)
@@ -1302,6 +1304,28 @@ int launch_thermostat(mcpm_ctx *c, double zone_lo, double T_lo, double zone_hi, 
   } else {
     return MCPM_ERR_ARG;
   }
+}
+
+constexpr int ENERGY_GRID = 1024;
+
+template <typename R>
+int launch_energy(mcpm_ctx *c, double *out) {
+  std::vector<double> part(4 * (size_t)ENERGY_GRID, 0.0);
+  if (c->n > 0) {
+    hipLaunchKernelGGL((k_energy<R>), dim3(ENERGY_GRID), dim3(256), 0, c->s,
+                       (const R *)c->P[c->cur], c->n, c->energy_part);
+    LAUNCHCHK();
+    HIPCHK(hipMemcpyAsync(part.data(), c->energy_part, part.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = 0; b < ENERGY_GRID; ++b)
+    for (int k = 0; k < 4; ++k) s[k] += part[4 * (size_t)b + k];
+  const double mf = c->p.mass * c->p.Fn;
+  out[0] = 0.5 * mf * s[0];
+  for (int k = 1; k < 4; ++k) out[k] = mf * s[k];
+  return MCPM_OK;
 }
 
 }  // namespace
@@ -1698,6 +1722,12 @@ int mcpm_get_escaped(mcpm_ctx *c, int64_t *count) {
   HIPCHK(hipStreamSynchronize(c->s));
   *count = (int64_t)v;
   return MCPM_OK;
+}
+
+int mcpm_energy_momentum(mcpm_ctx *c, double *out) {
+  if (!c || !out) return MCPM_ERR_ARG;
+  if (c->kp.nrep) return MCPM_ERR_ARG;
+  return MCPM_DISPATCH(c, launch_energy, c, out);
 }
 
 int mcpm_thermostat(mcpm_ctx *c, double zone_lo, double T_lo, double zone_hi, double T_hi) {

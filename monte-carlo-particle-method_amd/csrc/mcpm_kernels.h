@@ -132,6 +132,8 @@ __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v);
 __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams kp, double ast,
                         double bst, double eqdist, const FieldArrays *rfa);
 template <typename R>
+__global__ void k_energy(const R *P, int64_t n, double *part);
+template <typename R>
 __global__ void k_thermostat(R *P, int64_t n, const double *mom, KParams kp, double zone_lo,
                              double T_lo, double zone_hi, double T_hi);
 template <typename R>

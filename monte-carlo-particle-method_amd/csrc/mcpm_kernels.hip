@@ -1632,6 +1632,39 @@ __global__ __launch_bounds__(256) void k_thermostat(R *__restrict__ P, int64_t n
   r.u3 = (real)(lam * ((double)r.u3 - m[2]) + m[2]);
   P[i] = r;
 }
+// FP64 sums over the particles for the energy and momentum check: per
+// workgroup {sum |U|^2, sum U1, sum U2, sum U3} into part[4 blockIdx.x].
+template <typename R>
+__global__ __launch_bounds__(256) void k_energy(const R *__restrict__ P, int64_t n,
+                                                double *__restrict__ part) {
+  __shared__ double sh[4][4];
+  double e = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const R r = P[i];
+    const double a = (double)r.u1, b = (double)r.u2, c = (double)r.u3;
+    e += a * a + b * b + c * c;
+    m1 += a;
+    m2 += b;
+    m3 += c;
+  }
+  double v[4] = {e, m1, m2, m3};
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    for (int o = 32; o > 0; o >>= 1) v[k] += __shfl_xor(v[k], o);
+    if (l == 0) sh[w][k] = v[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    const int k = threadIdx.x;
+    part[4 * (size_t)blockIdx.x + k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
+  }
+}
+template __global__ void k_energy<Rec<float>>(const Rec<float> *, int64_t, double *);
+template __global__ void k_energy<Rec<double>>(const Rec<double> *, int64_t, double *);
+template __global__ void k_energy<Rec2<float>>(const Rec2<float> *, int64_t, double *);
+template __global__ void k_energy<Rec2<double>>(const Rec2<double> *, int64_t, double *);
 template __global__ void k_thermostat<Rec<float>>(Rec<float> *, int64_t, const double *, KParams,
                                                   double, double, double, double);
 template __global__ void k_thermostat<Rec<double>>(Rec<double> *, int64_t, const double *,

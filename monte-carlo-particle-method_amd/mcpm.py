@@ -271,6 +271,7 @@ def load_library() -> ctypes.CDLL:
         "mcpm_reset_samples": (i32, [vp]),
         "mcpm_measure_pressure": (i32, [vp, i32]),
         "mcpm_get_escaped": (i32, [vp, ctypes.POINTER(i64)]),
+        "mcpm_energy_momentum": (i32, [vp, dpp]),
         "mcpm_thermostat": (i32, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double]),
         "mcpm_read_settings": (i32, [ctypes.c_char_p, ctypes.POINTER(CSettings)]),
@@ -577,6 +578,12 @@ class Simulation:
         v = ctypes.c_int64()
         self._check(self._lib.mcpm_get_escaped(self._h, ctypes.byref(v)), "mcpm_get_escaped")
         return v.value
+
+    def energy_momentum(self) -> np.ndarray:
+        """FP64 kinetic energy (1/2) m Fn sum |U|^2 and momentum m Fn sum U."""
+        out = np.empty(4)
+        self._check(self._lib.mcpm_energy_momentum(self._h, _dp(out)), "mcpm_energy_momentum")
+        return out
 
     def thermostat(self, zone_lo: float, T_lo: float, zone_hi: float, T_hi: float):
         """thermostat of the inverted problem (Main_Functions.cpp:1760-1830)."""
