@@ -96,6 +96,7 @@ struct mcpm_ctx {
   // the plane: nodal density, nodal field, CG work arrays, quadrature tables
   FemParams fem{};
   uint32_t *q4 = nullptr, *fem_iters = nullptr;
+  bool keys_current = false;  // two-digit sort keys (with quarters) match the positions
   double *cg_part = nullptr, *cg_st = nullptr;  // multi-workgroup CG partials and scalars
   double *fem_buf = nullptr;  // n_node, pot, r, p, q, then the two tables
   double *n_node = nullptr, *pot = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr;
@@ -461,6 +462,7 @@ int chunk_for(int64_t n, int fixed) {
 
 template <typename R>
 int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
+  c->keys_current = false;
   if (c->n == 0) {
     HIPCHK(hipMemsetAsync(c->cnt, 0, c->nc * sizeof(uint32_t), c->s));
     HIPCHK(hipMemsetAsync(c->start, 0, (c->nc + 1) * sizeof(uint32_t), c->s));
@@ -531,6 +533,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                        c->cnt, c->nden, c->yc, c->kp);
     LAUNCHCHK();
     c->sorted = true;
+    c->keys_current = true;
     return MCPM_OK;
   }
   c->cur = 1 - c->cur;
@@ -771,8 +774,12 @@ int launch_sample(mcpm_ctx *c) {
 template <typename R>
 int launch_nodes(mcpm_ctx *c) {
   if constexpr (sizeof(R) == 8 * sizeof(typename R::real_type)) {
-    hipLaunchKernelGGL((k_quadrants<R>), dim3(c->nc), dim3(256), 0, c->s,
-                       (const R *)c->P[c->cur], c->cnt, c->start, c->q4, c->kp);
+    if (c->two_digit && c->keys_current)  // the sort keys carry each particle's quarter
+      hipLaunchKernelGGL(k_quadrants_keys, dim3((c->nc + 3) / 4), dim3(256), 0, c->s, c->keys,
+                         c->cnt, c->start, c->q4, c->nc);
+    else
+      hipLaunchKernelGGL((k_quadrants<R>), dim3(c->nc), dim3(256), 0, c->s,
+                         (const R *)c->P[c->cur], c->cnt, c->start, c->q4, c->kp);
     const int nn = (c->fem.nx + 1) * (c->fem.ny + 1);
     hipLaunchKernelGGL(k_nodes, dim3((nn + 255) / 256), dim3(256), 0, c->s, c->q4, c->n_node,
                        c->fem);
@@ -1432,6 +1439,7 @@ int mcpm_collide_cba_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats)
     if (stats) memset(stats, 0, sizeof *stats);
     return MCPM_OK;
   }
+  c->keys_current = false;  // the collisions displace particles
   return MCPM_DISPATCH(c, collide_dsmc_t, c, step, stats, (int)MCPM_MODEL_CBA_VHS);
 }
 
@@ -1443,6 +1451,7 @@ int mcpm_collide_cba_hs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) 
     if (stats) memset(stats, 0, sizeof *stats);
     return MCPM_OK;
   }
+  c->keys_current = false;  // the collisions displace particles
   return MCPM_DISPATCH(c, collide_cba_hs_t, c, step, stats);
 }
 

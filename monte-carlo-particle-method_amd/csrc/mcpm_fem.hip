@@ -128,6 +128,32 @@ __global__ __launch_bounds__(256) void k_quadrants(const R *__restrict__ P,
   if (threadIdx.x < 4) q4[4 * c + threadIdx.x] = sq[threadIdx.x];
 }
 
+// The same counts from the sort keys of a two-digit sort, whose top bits
+// carry each particle's quarter (key_quarter_bits): one wavefront per cell
+// reads 4 bytes per particle instead of the record.
+__global__ __launch_bounds__(256) void k_quadrants_keys(const uint32_t *__restrict__ keys,
+                                                        const uint32_t *__restrict__ cnt,
+                                                        const uint32_t *__restrict__ start,
+                                                        uint32_t *__restrict__ q4, int nc) {
+  const int c = blockIdx.x * 4 + (threadIdx.x >> 6), l = threadIdx.x & 63;
+  if (c >= nc) return;
+  uint32_t n[4] = {0, 0, 0, 0};
+  const uint32_t N = cnt[c], s = start[c];
+  for (uint32_t k = l; k < N; k += 64) {
+    const uint32_t q = keys[s + k] >> KEY_QUARTER_SHIFT;
+    n[0] += q == 0u;
+    n[1] += q == 1u;
+    n[2] += q == 2u;
+    n[3] += q == 3u;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    uint32_t v = n[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (l == q) q4[4 * c + q] = v;
+  }
+}
+
 // Node density: each cell around node (i,j) contributes its quarter that
 // touches the node; Fn / V per particle, times 2 per wall the node lies on
 // (Main_Functions.cpp:2344-2363).  A periodic axis wraps instead.

@@ -21,6 +21,27 @@ __host__ __device__ __forceinline__ int32_t cell_key(const Rec2<real> &r, const 
   return cy * kp.nx + cx;  // num = k N0 + j (Main_Functions.cpp:2263)
 }
 
+// Sort keys carried by the two-digit sort: the cell in the low bits; on the
+// plane the top two bits hold the particle's quarter of its cell, q =
+// (x >= centre_x) + 2 (y >= centre_y), the nodal deposit's index
+// (Main_Functions.cpp:2268-2342).  The sort bins on the cell bits only.
+constexpr uint32_t KEY_QUARTER_SHIFT = 30;
+constexpr uint32_t KEY_CELL_MASK = (1u << KEY_QUARTER_SHIFT) - 1u;
+template <typename real>
+__host__ __device__ __forceinline__ uint32_t key_quarter_bits(const Rec<real> &, int32_t,
+                                                              const KParams &) {
+  return 0u;
+}
+template <typename real>
+__host__ __device__ __forceinline__ uint32_t key_quarter_bits(const Rec2<real> &r, int32_t ckey,
+                                                              const KParams &kp) {
+  const int32_t cx = ckey % kp.nx, cy = ckey / kp.nx;
+  const double ccx = 0.5 * ((double)cx * kp.dx + (double)(cx + 1) * kp.dx);
+  const double ccy = 0.5 * ((double)cy * kp.dy + (double)(cy + 1) * kp.dy);
+  const uint32_t q = ((double)r.x < ccx ? 0u : 1u) + ((double)r.y < ccy ? 0u : 2u);
+  return q << KEY_QUARTER_SHIFT;
+}
+
 __host__ __device__ __forceinline__ double wrap_coord(double v, double L) {
   if (v < 0.0) v += L;
   else if (v >= L) v -= L;

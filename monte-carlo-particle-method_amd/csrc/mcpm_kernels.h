@@ -191,6 +191,8 @@ template <typename R>
 __global__ void k_quadrants(const R *P, const uint32_t *cnt, const uint32_t *start, uint32_t *q4,
                             KParams kp);
 __global__ void k_nodes(const uint32_t *q4, double *n_node, FemParams f);
+__global__ void k_quadrants_keys(const uint32_t *keys, const uint32_t *cnt, const uint32_t *start,
+                                 uint32_t *q4, int nc);
 __global__ void k_node_integral(const double *nden, const double *tab, double *pot, KParams kp,
                                 FemParams f, int boundary_only);
 // Multi-workgroup CG (large planes): CG_GRID workgroups of 256 threads;

@@ -344,16 +344,19 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
       const int64_t p = t0 + u * TILE + mine0;
       const bool valid = p < end;
       int32_t key = 0, ckey = 0;
+      uint32_t okv = 0;  // the key carried to the destination (plane: with its quarter)
       R r = rr[u];
       if (valid) {
         if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
         if (ikey) {
-          ckey = (int32_t)ikey[p];
+          okv = ikey[p];
+          ckey = (int32_t)(okv & KEY_CELL_MASK);
         } else if (kp.nrep == 0) {
           if ((mode & PUSH_CHECK) && !in_domain(r, kp)) atomicOr(err, ERRBIT_DOMAIN);
           r = push_one(r, (uint64_t)p, kp, kick, drift, mode, step_lo, step_hi, err, f2v[u],
                        pre);
           ckey = cell_key(r, kp);
+          okv = (uint32_t)ckey | key_quarter_bits(r, ckey, kp);
         } else {
           // a batched replica: its own geometry, Philox key, particle slots
           // and cells counted from its first
@@ -367,6 +370,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
           r = push_one(r, (uint64_t)(p - kp.rpart_off[q]), kq, kk, drift, mode, step_lo,
                        step_hi, err);
           ckey = off + cell_key(r, kq);
+          okv = (uint32_t)ckey;
         }
         key = sort_bin(ckey, kmode, kdiv);
       }
@@ -384,7 +388,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
               if (mine) {
                 const uint32_t dst = lds_cnt[lkey] + (uint32_t)__popcll(m & lt);
                 out[dst] = r;
-                if (okey) okey[dst] = (uint32_t)ckey;
+                if (okey) okey[dst] = okv;
               }
               __builtin_amdgcn_wave_barrier();
               if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
@@ -457,8 +461,8 @@ __global__ void k_cell_bounds(const uint32_t *__restrict__ key, int64_t n, int n
                               uint32_t *__restrict__ start, uint32_t *__restrict__ err) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t prev = i > 0 ? (int64_t)key[i - 1] : -1;
-    const int64_t cur = i < n ? (int64_t)key[i] : (int64_t)nc;
+    const int64_t prev = i > 0 ? (int64_t)(key[i - 1] & KEY_CELL_MASK) : -1;
+    const int64_t cur = i < n ? (int64_t)(key[i] & KEY_CELL_MASK) : (int64_t)nc;
     if (cur < prev || cur > nc) atomicOr(err, ERRBIT_ORDER);
     const int64_t hi = cur < nc ? cur : (int64_t)nc;
     for (int64_t c = prev + 1; c <= hi; ++c) start[c] = (uint32_t)i;

@@ -65,6 +65,8 @@ def test_plane_160x160_sort_deposit_and_esmc(boundary):
         _cells_equal(g, o)
         for a, b in zip(g.download_xy(), o.download_xy()):
             np.testing.assert_array_equal(a, b)
+        # the deposit of the moved particles (from the sort keys' quarters)
+        np.testing.assert_array_equal(g.nodes()[0], o.nodes()[0])
 
 
 def test_plane_160x160_screened_poisson_walls_grid_cg():
