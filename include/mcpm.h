@@ -510,7 +510,8 @@ int mcpm_get_tuning(mcpm_ctx *ctx, int32_t knob, int64_t *value);
 typedef struct mcpm_fno mcpm_fno;
 typedef struct mcpm_fno_field_stats {
   double residual;        /* |M n - A phi_net| / |M n| of the network's field */
-  int32_t cg_iterations;  /* conjugate-gradient iterations that followed */
+  int32_t cg_iterations;  /* conjugate-gradient iterations that followed (0 when the
+                             exact solve was the FFT of a power-of-two plane) */
   int32_t used_network;   /* 1 when the network's field was kept as it is */
 } mcpm_fno_field_stats;
 int mcpm_fno_param_count(int32_t modes, int64_t *count);

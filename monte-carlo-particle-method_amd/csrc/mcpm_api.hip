@@ -101,6 +101,12 @@ struct mcpm_ctx {
   double *fem_buf = nullptr;  // n_node, pot, r, p, q, then the two tables
   double *n_node = nullptr, *pot = nullptr, *cg_r = nullptr, *cg_p = nullptr, *cg_q = nullptr;
   double *tab_sp = nullptr, *tab_vl = nullptr;
+  // periodic power-of-two planes: the solve by FFT (row and column twiddles,
+  // cosines of the two axes, one complex work array)
+  bool fft_ok = false;
+  int fft_lgx = 0, fft_lgy = 0;
+  double2 *fft_buf = nullptr, *fft_twx = nullptr, *fft_twy = nullptr;
+  double *fft_cosx = nullptr, *fft_cosy = nullptr;
   double kick_coef = 0.0;
   // field surrogate of a periodic screened-Poisson plane (mcpm_set_field_surrogate)
   mcpm_fno *fno = nullptr;
@@ -398,6 +404,33 @@ int build_fem(mcpm_ctx *c) {
   const size_t nn = (size_t)(f.nx + 1) * (f.ny + 1);
   int st = dalloc(&c->fem_buf, nn * 5 + 2 * QT_N * 128 + 2 * CG_GRID + 4);
   if (st) return st;
+  auto pow2 = [](int v) { return v >= 4 && v <= 4096 && (v & (v - 1)) == 0; };
+  if (f.per_x && f.per_y && pow2(f.nx) && pow2(f.ny) && p.long_range == MCPM_LR_SCREENED_POISSON) {
+    const int nx = f.nx, ny = f.ny;
+    if ((st = dalloc(&c->fft_buf, (size_t)nx * ny))) return st;
+    if ((st = dalloc(&c->fft_twx, (size_t)nx / 2))) return st;
+    if ((st = dalloc(&c->fft_twy, (size_t)ny / 2))) return st;
+    if ((st = dalloc(&c->fft_cosx, (size_t)nx))) return st;
+    if ((st = dalloc(&c->fft_cosy, (size_t)ny))) return st;
+    // angles reduced exactly in integers, then FP64 cos / sin
+    auto tables = [&](int n, double2 *tw, double *cs) -> int {
+      std::vector<double2> t(n / 2);
+      std::vector<double> co(n);
+      for (int k = 0; k < n; ++k) {
+        const double a = 2.0 * PI * (double)k / (double)n;
+        co[k] = std::cos(a);
+        if (k < n / 2) t[k] = make_double2(std::cos(a), -std::sin(a));
+      }
+      HIPCHK(hipMemcpy(tw, t.data(), t.size() * sizeof(double2), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(cs, co.data(), co.size() * sizeof(double), hipMemcpyHostToDevice));
+      return MCPM_OK;
+    };
+    if ((st = tables(nx, c->fft_twx, c->fft_cosx))) return st;
+    if ((st = tables(ny, c->fft_twy, c->fft_cosy))) return st;
+    c->fft_lgx = __builtin_ctz((unsigned)nx);
+    c->fft_lgy = __builtin_ctz((unsigned)ny);
+    c->fft_ok = true;
+  }
   c->n_node = c->fem_buf;
   c->pot = c->n_node + nn;
   c->cg_r = c->pot + nn;
@@ -841,6 +874,22 @@ int fem_cg_grid(mcpm_ctx *c, const FemParams &f, int warm = 0, int mode = 0,
   return MCPM_OK;
 }
 
+// The periodic power-of-two plane's Q1 solve by FFT (k_fft_rows / k_fft_cols):
+// the exact solution of the system the conjugate gradients iterate on.
+int fem_fft(mcpm_ctx *c) {
+  const FemParams &f = c->fem;
+  const int cols = std::max(1, std::min(4, 4096 / f.ny));
+  hipLaunchKernelGGL((k_fft_rows<false>), dim3(f.ny), dim3(256), (size_t)f.nx * sizeof(double2),
+                     c->s, c->n_node, c->fft_buf, c->pot, c->fft_twx, c->fft_lgx, f);
+  hipLaunchKernelGGL(k_fft_cols, dim3(f.nx / cols), dim3(256),
+                     (size_t)cols * f.ny * sizeof(double2), c->s, c->fft_buf, c->fft_twy,
+                     c->fft_lgy, c->fft_cosx, c->fft_cosy, cols, f);
+  hipLaunchKernelGGL((k_fft_rows<true>), dim3(f.ny), dim3(256), (size_t)f.nx * sizeof(double2),
+                     c->s, c->n_node, c->fft_buf, c->pot, c->fft_twx, c->fft_lgx, f);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
 // compute_SP_2D (FEM.cpp:249-610) up to the nodal field; the kick is the push's.
 int sp_2d(mcpm_ctx *c) {
   int st = MCPM_DISPATCH(c, launch_nodes, c);
@@ -853,7 +902,10 @@ int sp_2d(mcpm_ctx *c) {
     hipLaunchKernelGGL(k_node_integral, dim3(nb), dim3(256), 0, c->s, c->nden, c->tab_sp, c->pot,
                        c->kp, f, 1);
   }
-  if (nn <= CG_ONE_WG_NODES) {
+  if (c->fft_ok) {
+    int st2 = fem_fft(c);
+    if (st2) return st2;
+  } else if (nn <= CG_ONE_WG_NODES) {
     hipLaunchKernelGGL(k_fem_cg, dim3(1), dim3(1024), 0, c->s, c->n_node, c->pot, c->cg_r,
                        c->cg_p, c->cg_q, f, c->fem_iters);
   } else {
@@ -899,9 +951,23 @@ int ctx_plane_nodes(mcpm_ctx *c, PlaneField *pf) {
   return MCPM_OK;
 }
 
-int ctx_plane_cg(mcpm_ctx *c, int warm, int mode, double accept, double *res0, int *iters) {
+int ctx_plane_cg(mcpm_ctx *c, int warm, int mode, double accept, double *res0, int *iters,
+                 int *solved) {
   if (!c || c->p.dims != 2 || !c->fem.per_x || !c->fem.per_y) return MCPM_ERR_ARG;
-  return fem_cg_grid(c, c->fem, warm, mode, accept, res0, iters);
+  if (iters) *iters = 0;
+  if (res0) *res0 = -1.0;
+  if (solved) *solved = 0;
+  double r = -1.0;
+  if (warm) {
+    // the starting residual; mode 2, or mode 1 within `accept`, keeps the guess
+    int st = fem_cg_grid(c, c->fem, 1, 2, 0.0, &r, nullptr);
+    if (st) return st;
+    if (res0) *res0 = r;
+    if (mode == 2 || (mode == 1 && r <= accept)) return MCPM_OK;
+  }
+  if (solved) *solved = 1;
+  if (c->fft_ok) return fem_fft(c);  // the exact solve of a periodic power-of-two plane
+  return fem_cg_grid(c, c->fem, warm, 0, 0.0, nullptr, iters);
 }
 
 int ctx_plane_finish(mcpm_ctx *c) {
@@ -1222,7 +1288,8 @@ int mcpm_destroy(mcpm_ctx *c) {
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
                   c->samples, c->press, c->counters, c->err, c->esc, c->energy_part, c->res, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
-                  c->fem_buf, c->q4};
+                  c->fem_buf, c->q4, c->fft_buf, c->fft_twx, c->fft_twy,
+                  c->fft_cosx, c->fft_cosy};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < T_NCLASS; ++k)

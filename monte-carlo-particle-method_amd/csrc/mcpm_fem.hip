@@ -466,6 +466,121 @@ __global__ __launch_bounds__(256) void k_cgm_seams(double *__restrict__ pot, Fem
   }
 }
 
+// ---------------------------------------------------------------------------
+// Periodic planes of power-of-two size: the Q1 system (K + lambda^2 V M) x =
+// V M n is diagonal in the plane's DFT, with
+//   x_hat = V Mu / (K + lam2V Mu) n_hat,  Mu = (2 + cx)(2 + cy) / 9,
+//   K = 8/3 - (2/3)(cx + cy) - (4/3) cx cy,  c = cos(2 pi k / N),
+// the symbols of the 9-point stencils the conjugate-gradient kernels apply.
+// Three launches: FFT of every row (from the nodal density), FFT of every
+// column, multiply, inverse FFT of the column (one kernel), inverse FFT of
+// every row (into the nodal field, seams included).  Radix-2, in LDS, FP64.
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// In-place radix-2 FFT of x[0..n) in LDS (input already in bit-reversed
+// order); tw[k] = exp(-2 pi i k / n), k < n/2; inverse uses the conjugates.
+// Threads t0, t0 + nt, ... of the workgroup share the butterflies.
+__device__ void lds_fft(double2 *x, int n, const double2 *__restrict__ tw, bool inverse, int t0,
+                        int nt) {
+  for (int len = 2; len <= n; len <<= 1) {
+    const int half = len >> 1, step = n / len;
+    for (int j = t0; j < n / 2; j += nt) {
+      const int pos = j % half, i0 = (j / half) * len + pos, i1 = i0 + half;
+      double2 w = tw[pos * step];
+      if (inverse) w.y = -w.y;
+      const double2 a = x[i0], b = cmul(w, x[i1]);
+      x[i0] = make_double2(a.x + b.x, a.y + b.y);
+      x[i1] = make_double2(a.x - b.x, a.y - b.y);
+    }
+    __syncthreads();
+  }
+}
+
+__device__ __forceinline__ int bitrev(int i, int lg) {
+  return (int)(__builtin_bitreverse32((uint32_t)i) >> (32 - lg));
+}
+
+}  // namespace
+
+// Row pass.  Forward: rows of the nodal density (seams skipped) -> buf.
+// Inverse: buf rows -> real part / (N0 N1) into the nodal field, seams copied.
+template <bool INVERSE>
+__global__ __launch_bounds__(256) void k_fft_rows(const double *__restrict__ n_node,
+                                                  double2 *__restrict__ buf,
+                                                  double *__restrict__ pot,
+                                                  const double2 *__restrict__ twx, int lgx,
+                                                  FemParams f) {
+  extern __shared__ double2 fx[];
+  const int j = blockIdx.x, n0 = f.nx, w = f.nx + 1;
+  for (int i = threadIdx.x; i < n0; i += blockDim.x) {
+    const double2 v = INVERSE ? buf[(size_t)j * n0 + i]
+                              : make_double2(n_node[(size_t)j * w + i], 0.0);
+    fx[bitrev(i, lgx)] = v;
+  }
+  __syncthreads();
+  lds_fft(fx, n0, twx, INVERSE, threadIdx.x, blockDim.x);
+  if (!INVERSE) {
+    for (int i = threadIdx.x; i < n0; i += blockDim.x) buf[(size_t)j * n0 + i] = fx[i];
+  } else {
+    const double inv = 1.0 / ((double)f.nx * (double)f.ny);
+    for (int i = threadIdx.x; i <= n0; i += blockDim.x) {
+      const double v = fx[i == n0 ? 0 : i].x * inv;
+      pot[(size_t)j * w + i] = v;
+      if (j == 0) pot[(size_t)f.ny * w + i] = v;  // seam row ny repeats row 0
+    }
+  }
+}
+
+// Column pass: FFT of `cols` columns per workgroup, multiply by the symbol,
+// inverse FFT, back into buf.  cosx/cosy: cos(2 pi k / N) per axis.
+__global__ __launch_bounds__(256) void k_fft_cols(double2 *__restrict__ buf,
+                                                  const double2 *__restrict__ twy, int lgy,
+                                                  const double *__restrict__ cosx,
+                                                  const double *__restrict__ cosy, int cols,
+                                                  FemParams f) {
+  extern __shared__ double2 fy[];
+  const int n0 = f.nx, n1 = f.ny;
+  const int c0 = blockIdx.x * cols;
+  const int per = blockDim.x / cols;  // threads per column
+  const int col = threadIdx.x / per, t0 = threadIdx.x % per;
+  double2 *x = fy + (size_t)col * n1;
+  const int i = c0 + col;
+  for (int jj = t0; jj < n1; jj += per) x[bitrev(jj, lgy)] = buf[(size_t)jj * n0 + i];
+  __syncthreads();
+  lds_fft(x, n1, twy, false, t0, per);
+  const double cx = cosx[i];
+  for (int jj = t0; jj < n1; jj += per) {
+    const double cy = cosy[jj];
+    const double mu = (2.0 + cx) * (2.0 + cy) / 9.0;
+    const double K = 8.0 / 3.0 - 2.0 / 3.0 * (cx + cy) - 4.0 / 3.0 * cx * cy;
+    const double s = f.volume * mu / (K + f.lam2V * mu);
+    x[jj] = make_double2(x[jj].x * s, x[jj].y * s);
+  }
+  __syncthreads();
+  // inverse: reorder to bit-reversed positions first
+  for (int jj = t0; jj < n1; jj += per) {
+    const int r = bitrev(jj, lgy);
+    if (r > jj) {
+      const double2 a = x[jj];
+      x[jj] = x[r];
+      x[r] = a;
+    }
+  }
+  __syncthreads();
+  lds_fft(x, n1, twy, true, t0, per);
+  for (int jj = t0; jj < n1; jj += per) buf[(size_t)jj * n0 + i] = x[jj];
+}
+
+template __global__ void k_fft_rows<false>(const double *, double2 *, double *, const double2 *,
+                                           int, FemParams);
+template __global__ void k_fft_rows<true>(const double *, double2 *, double *, const double2 *,
+                                          int, FemParams);
+
 // Core subtraction x -= n dummy + (Laplacian-like sum) dummy2
 // (FEM.cpp:497-554).  On walled planes the neighbour set is the
 // reference's: node (i,j) is corner k of its owner cell (ii,jj), and the

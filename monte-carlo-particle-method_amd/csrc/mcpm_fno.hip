@@ -1146,7 +1146,8 @@ int mcpm_fno_add_snapshot(mcpm_fno *f, mcpm_ctx *c, float w0) {
   if ((st = fno_check_plane(f, pf.fem))) return st;
   int iters = 0;
   double res0 = 0.0;
-  if ((st = ctx_plane_cg(c, 0, 0, 0.0, &res0, &iters))) return st;
+  int solved = 0;
+  if ((st = ctx_plane_cg(c, 0, 0, 0.0, &res0, &iters, &solved))) return st;
   settle(f);
   f->w0 = w0;
   const hipStream_t s = pf.s;
@@ -1174,12 +1175,13 @@ int mcpm_fno_field(mcpm_fno *f, mcpm_ctx *c, double accept_residual, int32_t exa
   // exact: solve from the network's field; accept_residual >= 0: keep the
   // network's field when its residual is at most that, else solve; < 0: keep it
   const int mode = exact ? 0 : (accept_residual >= 0.0 ? 1 : 2);
-  if ((st = ctx_plane_cg(c, 1, mode, accept_residual, &res0, &iters))) return st;
+  int solved = 0;
+  if ((st = ctx_plane_cg(c, 1, mode, accept_residual, &res0, &iters, &solved))) return st;
   if ((st = ctx_plane_finish(c))) return st;
   if (stats) {
     stats->residual = res0;
     stats->cg_iterations = iters;
-    stats->used_network = (iters == 0 && mode != 0) ? 1 : 0;
+    stats->used_network = solved ? 0 : 1;
   }
   return MCPM_OK;
 }

@@ -191,6 +191,11 @@ template <typename R>
 __global__ void k_quadrants(const R *P, const uint32_t *cnt, const uint32_t *start, uint32_t *q4,
                             KParams kp);
 __global__ void k_nodes(const uint32_t *q4, double *n_node, FemParams f);
+template <bool INVERSE>
+__global__ void k_fft_rows(const double *n_node, double2 *buf, double *pot, const double2 *twx,
+                           int lgx, FemParams f);
+__global__ void k_fft_cols(double2 *buf, const double2 *twy, int lgy, const double *cosx,
+                           const double *cosy, int cols, FemParams f);
 __global__ void k_quadrants_keys(const uint32_t *keys, const uint32_t *cnt, const uint32_t *start,
                                  uint32_t *q4, int nc);
 __global__ void k_node_integral(const double *nden, const double *tab, double *pot, KParams kp,

@@ -16,12 +16,14 @@ struct PlaneField {
 // part of compute_SP_2D, Main_Functions.cpp:2268-2364) and describe its field.
 int ctx_plane_nodes(mcpm_ctx *c, PlaneField *pf);
 
-// Conjugate gradients on the Q1 system (K + lambda^2 M) pot = M n.
-// warm = 0 starts from zero, warm = 1 from the current pot.  mode 0 solves
-// to the context's tolerance; mode 1 stops at once when the starting
-// residual |M n - A pot| / |M n| is at most accept; mode 2 only measures it.
-// res0 gets the starting residual when warm (else -1), iters the iterations.
-int ctx_plane_cg(mcpm_ctx *c, int warm, int mode, double accept, double *res0, int *iters);
+// The Q1 system (K + lambda^2 M) pot = M n.  warm = 1: measure the starting
+// residual |M n - A pot| / |M n| of the current pot into res0 (else -1);
+// mode 2 stops there, mode 1 stops there when it is at most accept.  Else
+// solve: by FFT on periodic power-of-two planes, otherwise by conjugate
+// gradients to the context's tolerance (from the current pot when warm);
+// iters gets the CG iterations, solved 1 when a solve ran.
+int ctx_plane_cg(mcpm_ctx *c, int warm, int mode, double accept, double *res0, int *iters,
+                 int *solved);
 
 // Core subtraction (FEM.cpp:497-554) and the kick of the next push.
 int ctx_plane_finish(mcpm_ctx *c);

