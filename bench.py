@@ -43,10 +43,16 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--cpu-particles", type=int, default=20_000_000)
     ap.add_argument("--workload", default="interface",
-                    choices=["interface", "droplets", "replicas"],
+                    choices=["interface", "droplets", "replicas", "dense"],
                     help="interface: the periodic 400 sigma slab the metric is quoted on; "
                          "droplets: the 512 x 512 periodic plane with 20 liquid discs; "
-                         "replicas: 256 independent interface slabs in one batched context")
+                         "replicas: 256 independent interface slabs in one batched context; "
+                         "dense: the 20 sigma dense slab between diffuse walls at 1.0 and 1.5, "
+                         "4000 cells x 2500 particles (10M)")
+    ap.add_argument("--model", default="esmc", choices=["esmc", "fp"],
+                    help="dense workload: Enskog collisions (ESMC) or the dense-gas "
+                         "Fokker-Planck model (FP_dense)")
+    ap.add_argument("--density", type=float, default=0.3, help="dense workload: n sigma^3")
     ap.add_argument("--replicas", type=int, default=256)
     ap.add_argument("--unbatched-steps", type=int, default=2,
                     help="replicas: also time this many steps of the same replicas run one "
@@ -129,6 +135,19 @@ def main():
         del arrs
         sim.cell_update()
         sim.init_omega_max(meta["n_gas"])
+    elif args.workload == "dense":
+        p, arrs, meta = W.dense_slab(n_density=args.density)
+        if args.model == "fp":
+            p.model = mcpm.MODEL_FP_DENSE
+        N = meta["N"]
+        args.cells = p.n_cells
+        log(f"workload: dense slab N={N} cells={p.n_cells} n={args.density} model={args.model} "
+            f"dt={p.delta_t:.5f} FP32")
+        sim = mcpm.Simulation(p)
+        sim.upload(*arrs)
+        del arrs
+        sim.cell_update()
+        sim.init_omega_max(meta["n_gas"])
     else:
         p, arrs, meta = W.interface(N=N, n_cells=args.cells)
         log(f"workload: interface N={N} cells={args.cells} dt={p.delta_t} FP32")
@@ -171,10 +190,12 @@ def main():
     timers = {}
     for name, k in (("collide_rounds", mcpm.TIMER_COLLIDE), ("sort", mcpm.TIMER_SORT),
                     ("candidates", mcpm.TIMER_GEN), ("field", mcpm.TIMER_FIELD),
-                    ("step", mcpm.TIMER_STEP)):
+                    ("step", mcpm.TIMER_STEP), ("fp_velocity", mcpm.TIMER_FP_VELOCITY),
+                    ("fp_closures", mcpm.TIMER_FP_CLOSURES)):
         ms, calls = sim.timing(k)
         timers[name] = {"avg_ms": ms, "calls": calls, "per_step_ms": ms * calls / args.steps}
     sim.timing_enable(False)
+    timers = {k: v for k, v in timers.items() if v["calls"] > 0}
     acc = float(np.mean([s["accepted"] for s in stats]))
     valid = float(np.mean([s["valid"] for s in stats]))
     rounds = float(np.mean([s["rounds"] for s in stats]))
@@ -190,14 +211,19 @@ def main():
     #  the two particle records are read and written once (4 x 16 B) plus the
     #  16-byte operation record, summed over the step's launches.
     sort_ms = timers["sort"]["per_step_ms"]
-    coll_ms = timers["collide_rounds"]["per_step_ms"]
+    coll_ms = timers.get("collide_rounds", {"per_step_ms": 0.0})["per_step_ms"]
+    fpv_ms = timers.get("fp_velocity", {"per_step_ms": 0.0})["per_step_ms"]
     plane = args.workload == "droplets"
     rec = 32.0 if plane else 16.0       # FP32 record: (x, y, u1..3, pad) or (y, u1..3)
     passes = 2.0 if args.cells > 16384 else 1.0   # two-digit sort past 16384 cells
     sort_bytes = 3.0 * rec * N * passes
     coll_bytes = valid * (4.0 * rec + 16.0)
     step_alg = 44 if plane else 36      # 2 S_p + 4 with S_p = 20 (plane) or 16 (slab)
-    if coll_ms >= sort_ms:
+    if fpv_ms >= max(coll_ms, sort_ms):
+        # k_fp_velocity: one compulsory read and write of each 16-byte record
+        launches = 1.0
+        dom, dom_ms, dom_bytes = "k_fp_velocity", fpv_ms, 2.0 * rec * N
+    elif coll_ms >= sort_ms:
         launches = timers["collide_rounds"]["calls"] / args.steps
         dom, dom_ms, dom_bytes = "k_round", coll_ms, coll_bytes
     else:
@@ -230,6 +256,11 @@ def main():
                 f"batched context, {N} particles, {args.cells} cells; per replica N log-uniform "
                 "in [5e4, 5e5], n_liq in [0.45, 0.6], T in [0.6, 0.9], cells in [512, 2048], "
                 "seeds 0..255; ESMC + screened-Poisson, dt=0.02")
+    elif args.workload == "dense":
+        desc = (f"dense slab: 20 sigma between diffuse walls at T=1.0 and 1.5, {p.n_cells} cells, "
+                f"{N} particles, n={args.density}, T0=1.25, "
+                + ("Enskog ESMC" if args.model == "esmc" else "dense-gas Fokker-Planck (FP_dense)")
+                + ", dt=0.05 mean collision time, FP32 particles, FP64 moments")
     elif plane:
         desc = (f"droplets: periodic 128 x 128 sigma plane, {p.n_cells_x} x {p.n_cells} cells, "
                 f"{N} particles, 20 discs n=0.55 in vapour 0.02, T=0.75, ESMC + "
@@ -240,7 +271,8 @@ def main():
 
     out = {
         "metric": "particle-steps/s on 1 MI355X (Enskog-Vlasov "
-                  + ("plane step)" if plane else "interface step)"),
+                  + ("plane step)" if plane else ("dense slab step)" if args.workload == "dense"
+                                                  else "interface step)")),
         "value": value,
         "unit": "particle-steps/s",
         "n_gpus": 1,

@@ -455,7 +455,8 @@ int mcpm_get_collision_log(mcpm_ctx *ctx, int64_t cap, uint32_t *prio, uint32_t 
 /* Average device duration (ms) of each kernel class over the calls since the
    last reset, measured with HIP events on the context's stream.  `which`:
    0 collide rounds, 1 sort (push+scatter), 2 candidate generation, 3 field,
-   4 whole step. */
+   4 whole step, 5 Fokker-Planck velocity update, 6 Fokker-Planck moments and
+   closures. */
 int mcpm_timing_enable(mcpm_ctx *ctx, int32_t enable);
 int mcpm_timing_get(mcpm_ctx *ctx, int32_t which, double *avg_ms, int64_t *calls);
 

@@ -19,7 +19,16 @@ namespace {
 
 const double PI = 3.14159265358979323846;
 
-enum { T_COLLIDE = 0, T_SORT = 1, T_GEN = 2, T_FIELD = 3, T_STEP = 4, T_NCLASS = 5 };
+enum {
+  T_COLLIDE = 0,
+  T_SORT = 1,
+  T_GEN = 2,
+  T_FIELD = 3,
+  T_STEP = 4,
+  T_FPVEL = 5,   // Fokker-Planck velocity update (k_fp_velocity)
+  T_FPCLOS = 6,  // Fokker-Planck moments and closures
+  T_NCLASS = 7
+};
 
 }  // namespace
 
@@ -1527,6 +1536,7 @@ int mcpm_fp_closures(mcpm_ctx *c, int64_t step) {
   if (c->kp.nrep) return MCPM_ERR_STATE;  // not on batched replicas
   if (!c->sorted) return MCPM_ERR_STATE;
   if (c->p.dims != 1) return MCPM_ERR_ARG;
+  Timer tm(c, T_FPCLOS);
   int st = MCPM_DISPATCH(c, launch_moments, c);
   if (st) return st;
   hipLaunchKernelGGL(k_fp_closures, dim3((c->nc + 63) / 64), dim3(64), 0, c->s, c->moments,
@@ -1540,12 +1550,13 @@ int mcpm_velocity_fp(mcpm_ctx *c, int64_t step) {
   if (!c) return MCPM_ERR_ARG;
   if (!c->have_closures || c->p.dims != 1) return MCPM_ERR_STATE;
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  Timer tm(c, T_FPVEL);
   if (c->p.precision == MCPM_FP32)
-    hipLaunchKernelGGL((k_fp_velocity<float>), dim3(c->nc), dim3(256), 0, c->s,
+    hipLaunchKernelGGL((k_fp_velocity<float, FP_CACHE_K>), dim3(c->nc), dim3(256), 0, c->s,
                        (Rec<float> *)c->P[c->cur], c->cnt, c->start, c->moments, c->clos, c->yc,
                        c->kp, c->fpp, slo, shi, c->err);
   else
-    hipLaunchKernelGGL((k_fp_velocity<double>), dim3(c->nc), dim3(256), 0, c->s,
+    hipLaunchKernelGGL((k_fp_velocity<double, FP_CACHE_K>), dim3(c->nc), dim3(256), 0, c->s,
                        (Rec<double> *)c->P[c->cur], c->cnt, c->start, c->moments, c->clos, c->yc,
                        c->kp, c->fpp, slo, shi, c->err);
   LAUNCHCHK();

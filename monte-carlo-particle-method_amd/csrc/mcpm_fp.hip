@@ -241,7 +241,11 @@ __device__ __forceinline__ void fp_block_sum(double (&v)[K], double *sh) {
 }
 
 // random_generator + Velocity_FP_parallel for one cell per workgroup.
-template <typename real>
+// Cells of at most 256 KMAX particles: a thread keeps the noise of its
+// particles (k = tid + 256 j) in registers, so the noise is drawn once
+// instead of once per pass.  Larger cells draw it again in each of the four
+// passes.  The same operations in the same order either way.
+template <typename real, int KMAX>
 __global__ __launch_bounds__(256) void k_fp_velocity(Rec<real> *__restrict__ P,
                                                      const uint32_t *__restrict__ cnt,
                                                      const uint32_t *__restrict__ start,
@@ -269,21 +273,56 @@ __global__ __launch_bounds__(256) void k_fp_velocity(Rec<real> *__restrict__ P,
   }
   const double *cl = clos + (size_t)c * MCPM_NCLOS;
   const double T = m[29];
+  constexpr int KC = KMAX > 0 ? KMAX : 1;
+  const bool cached = KMAX > 0 && N <= 256u * (uint32_t)KC && blockDim.x == 256;
+  double zc[KC][3];
+  // noise of particle k (the j-th of this thread when cached)
+  auto noise = [&](uint32_t k, int j, double *z) {
+    if (cached) {
+      z[0] = zc[0][0];
+      z[1] = zc[0][1];
+      z[2] = zc[0][2];
+#pragma unroll
+      for (int q = 1; q < KC; ++q)
+        if (q == j) {
+          z[0] = zc[q][0];
+          z[1] = zc[q][1];
+          z[2] = zc[q][2];
+        }
+    } else {
+      fp_noise(s + k, kp, slo, shi, z);
+    }
+  };
   // noise: per-cell mean and rms (Fokker_Planck.cpp:104-138)
   double acc[3] = {0.0, 0.0, 0.0};
-  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-    double z[3];
-    fp_noise(s + k, kp, slo, shi, z);
-    acc[0] += z[0]; acc[1] += z[1]; acc[2] += z[2];
+  {
+    int j = 0;
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x, ++j) {
+      double z[3];
+      fp_noise(s + k, kp, slo, shi, z);
+      if (cached) {
+#pragma unroll
+        for (int q = 0; q < KC; ++q)
+          if (q == j) {
+            zc[q][0] = z[0];
+            zc[q][1] = z[1];
+            zc[q][2] = z[2];
+          }
+      }
+      acc[0] += z[0]; acc[1] += z[1]; acc[2] += z[2];
+    }
   }
   fp_block_sum<3>(acc, sh);
   const double mean0 = acc[0] / (1.0 * N), mean1 = acc[1] / (1.0 * N), mean2 = acc[2] / (1.0 * N);
   double sq[3] = {0.0, 0.0, 0.0};
-  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-    double z[3];
-    fp_noise(s + k, kp, slo, shi, z);
-    double a = z[0] - mean0, b = z[1] - mean1, d = z[2] - mean2;
-    sq[0] += a * a; sq[1] += b * b; sq[2] += d * d;
+  {
+    int j = 0;
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x, ++j) {
+      double z[3];
+      noise(k, j, z);
+      double a = z[0] - mean0, b = z[1] - mean1, d = z[2] - mean2;
+      sq[0] += a * a; sq[1] += b * b; sq[2] += d * d;
+    }
   }
   fp_block_sum<3>(sq, sh);
   const double rms0 = __builtin_sqrt(sq[0] / (1.0 * N));
@@ -301,12 +340,13 @@ __global__ __launch_bounds__(256) void k_fp_velocity(Rec<real> *__restrict__ P,
   const double e0 = 3.0 * kp.kb * T / kp.mass;
   // Velocity_FP_linear (Fokker_Planck.cpp:1240-1264): the same step without drift
   const bool linear = fpp.model == MCPM_MODEL_FP_LINEAR;
-  auto update = [&](const Rec<real> &r, double &n0, double &n1, double &n2, uint32_t k) {
+  auto update = [&](const Rec<real> &r, double &n0, double &n1, double &n2, uint32_t k,
+                    int j) {
     double a = (double)r.u1 - U0, b = (double)r.u2 - U1, d = (double)r.u3 - U2;
     double e = a * a + b * b + d * d;
     if (linear) {
       double z[3];
-      fp_noise(s + k, kp, slo, shi, z);
+      noise(k, j, z);
       n0 = A1 * a + A3 * ((z[0] - mean0) / rms0);
       n1 = A1 * b + A3 * ((z[1] - mean1) / rms1);
       n2 = A1 * d + A3 * ((z[2] - mean2) / rms2);
@@ -322,40 +362,50 @@ __global__ __launch_bounds__(256) void k_fp_velocity(Rec<real> *__restrict__ P,
     d2 += g[2] * (e - e0);
     d2 += L * (d * e - Q2);
     double z[3];
-    fp_noise(s + k, kp, slo, shi, z);
+    noise(k, j, z);
     double x0 = (z[0] - mean0) / rms0, x1 = (z[1] - mean1) / rms1, x2 = (z[2] - mean2) / rms2;
     n0 = A1 * a + A2 * d0 + A3 * x0;
     n1 = A1 * b + A2 * d1 + A3 * x1;
     n2 = A1 * d + A2 * d2 + A3 * x2;
     return e;
   };
+  auto record = [&](uint32_t k, int j) {
+    (void)j;
+    return P[s + k];
+  };
   double fr[2] = {0.0, 0.0};
-  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-    Rec<real> r = P[s + k];
-    double n0, n1, n2;
-    double e = update(r, n0, n1, n2, k);
-    if (n1 != n1) atomicOr(err, ERRBIT_NAN);  // Fokker_Planck.cpp:1137-1149
-    fr[0] += e;
-    fr[1] += n0 * n0 + n1 * n1 + n2 * n2;
+  {
+    int j = 0;
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x, ++j) {
+      Rec<real> r = record(k, j);
+      double n0, n1, n2;
+      double e = update(r, n0, n1, n2, k, j);
+      if (n1 != n1) atomicOr(err, ERRBIT_NAN);  // Fokker_Planck.cpp:1137-1149
+      fr[0] += e;
+      fr[1] += n0 * n0 + n1 * n1 + n2 * n2;
+    }
   }
   fp_block_sum<2>(fr, sh);
   const double sc = __builtin_sqrt(__builtin_fabs(fr[0] / fr[1]));
-  for (uint32_t k = threadIdx.x; k < N; k += blockDim.x) {
-    Rec<real> r = P[s + k];
-    double n0, n1, n2;
-    update(r, n0, n1, n2, k);
-    r.u1 = (real)(n0 * sc + U0);
-    r.u2 = (real)(n1 * sc + U1);
-    r.u3 = (real)(n2 * sc + U2);
-    P[s + k] = r;
+  {
+    int j = 0;
+    for (uint32_t k = threadIdx.x; k < N; k += blockDim.x, ++j) {
+      Rec<real> r = record(k, j);
+      double n0, n1, n2;
+      update(r, n0, n1, n2, k, j);
+      r.u1 = (real)(n0 * sc + U0);
+      r.u2 = (real)(n1 * sc + U1);
+      r.u3 = (real)(n2 * sc + U2);
+      P[s + k] = r;
+    }
   }
 }
 
-template __global__ void k_fp_velocity<float>(Rec<float> *, const uint32_t *, const uint32_t *,
-                                              const double *, const double *, const double *,
-                                              KParams, FPParams, uint32_t, uint32_t, uint32_t *);
-template __global__ void k_fp_velocity<double>(Rec<double> *, const uint32_t *, const uint32_t *,
-                                               const double *, const double *, const double *,
-                                               KParams, FPParams, uint32_t, uint32_t, uint32_t *);
+#define MCPM_INST_FPV(real, K)                                                                   \
+  template __global__ void k_fp_velocity<real, K>(Rec<real> *, const uint32_t *, const uint32_t *, \
+                                                  const double *, const double *, const double *,  \
+                                                  KParams, FPParams, uint32_t, uint32_t, uint32_t *);
+MCPM_INST_FPV(float, FP_CACHE_K)
+MCPM_INST_FPV(double, FP_CACHE_K)
 
 }  // namespace mcpm

@@ -158,7 +158,10 @@ __global__ void k_pressure(const double *mom, const double *nden, const double *
 __global__ void k_fp_closures(const double *mom, const uint32_t *cnt, const double *nden,
                               const double *yc, double *clos, double *drift, KParams kp,
                               FPParams fpp, int64_t step, uint32_t *err);
-template <typename real>
+// particles per thread whose noise the FP velocity kernel keeps in registers
+// (cells up to 256 x this)
+constexpr int FP_CACHE_K = 12;
+template <typename real, int KMAX>
 __global__ void k_fp_velocity(Rec<real> *P, const uint32_t *cnt, const uint32_t *start,
                               const double *mom, const double *clos, const double *yc,
                               KParams kp, FPParams fpp, uint32_t slo, uint32_t shi,
