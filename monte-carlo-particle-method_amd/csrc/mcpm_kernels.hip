@@ -379,35 +379,57 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
         // land in index order (the sort is stable)
         for (int w = 0; w < PUSH_WAVES; ++w) {
           if (wave == w) {
+            // rank among the lanes below with the same key, and how many
+            // lanes share it: one round of scalar ballots per distinct key,
+            // no memory access inside the loop
+            uint32_t rk = 0, tot = 0;
             uint64_t active = __ballot(valid);
             while (active) {
               const int leader = __ffsll((unsigned long long)active) - 1;
-              const int lkey = __shfl(key, leader);
+              const int lkey = __builtin_amdgcn_readlane(key, leader);
               const bool mine = valid && key == lkey;
               const uint64_t m = __ballot(mine);
               if (mine) {
-                const uint32_t dst = lds_cnt[lkey] + (uint32_t)__popcll(m & lt);
-                out[dst] = r;
-                if (okey) okey[dst] = okv;
+                rk = (uint32_t)__popcll(m & lt);
+                tot = (uint32_t)__popcll(m);
               }
-              __builtin_amdgcn_wave_barrier();
-              if (lane == leader) lds_cnt[lkey] += (uint32_t)__popcll(m);
-              __builtin_amdgcn_wave_barrier();
               active &= ~m;
             }
+            // one counter read per lane (lanes of one key read the same
+            // word), one store per lane; the last lane of each key advances
+            // its counter (distinct keys: no two lanes write one word)
+            uint32_t base = 0;
+            if (valid) base = lds_cnt[key];
+            __builtin_amdgcn_wave_barrier();
+            if (valid) {
+              const uint32_t dst = base + rk;
+              out[dst] = r;
+              if (okey) okey[dst] = okv;
+              if (rk + 1 == tot) lds_cnt[key] = base + tot;
+            }
+            __builtin_amdgcn_wave_barrier();
           }
           if constexpr (PUSH_WAVES > 1) __syncthreads();
         }
       } else {
-        // counts only: the order of the additions does not matter
+        // counts only: the order of the additions does not matter.  The
+        // first two distinct keys are counted with one ballot each (on
+        // coarse cells they hold most of the wavefront); every lane left
+        // adds itself, one LDS atomic per lane instead of one ballot round
+        // per distinct key (on fine cells a wavefront's particles spread
+        // over tens of cells).
         uint64_t active = __ballot(valid);
-        while (active) {
-          const int leader = __ffsll((unsigned long long)active) - 1;
-          const int lkey = __shfl(key, leader);
-          const uint64_t m = __ballot(valid && key == lkey);
-          if (lane == leader) atomicAdd(&lds_cnt[lkey], (uint32_t)__popcll(m));
-          active &= ~m;
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+          if (active) {
+            const int leader = __ffsll((unsigned long long)active) - 1;
+            const int lkey = __builtin_amdgcn_readlane(key, leader);
+            const uint64_t m = __ballot(valid && key == lkey);
+            if (lane == leader) atomicAdd(&lds_cnt[lkey], (uint32_t)__popcll(m));
+            active &= ~m;
+          }
         }
+        if ((active >> lane) & 1ull) atomicAdd(&lds_cnt[key], 1u);
       }
     }
   }
