@@ -123,6 +123,53 @@ double rnd(double v) {
   return (double)(real)v;
 }
 
+// Sums of per-particle terms in one fixed order, the order of a 256-thread
+// workgroup: term k goes to lane k mod 256, each lane adds its terms in the
+// order they arrive; each group of 64 lanes then folds pairwise (lane l adds
+// lane l + off for off = 32, 16, ..., 1) and the four group totals are
+// added in turn to 0.  The reference sums serially (e.g. cell_update,
+// Main_Functions.cpp:2483-2593); this is the same sum in another
+// association, fixed so that sums over a cell come out bit for bit the same
+// on every run.
+template <int K>
+struct WgSum {
+  static const int NT = 256;
+  std::vector<double> lane;  // NT x K; lanes at or past `used` hold 0
+  int used = 0;
+  WgSum() : lane((size_t)NT * K) {}
+  void add(int l, const double (&v)[K]) {
+    double *a = &lane[(size_t)l * K];
+    for (; used <= l; ++used)
+      for (int q = 0; q < K; ++q) lane[(size_t)used * K + q] = 0.0;
+    for (int q = 0; q < K; ++q) a[q] += v[q];
+  }
+  // lane l exists (holds its 0 start value) without a term
+  void touch(int l) {
+    for (; used <= l; ++used)
+      for (int q = 0; q < K; ++q) lane[(size_t)used * K + q] = 0.0;
+  }
+  void total(double (&out)[K]) const {
+    if (used == 0) {
+      for (int q = 0; q < K; ++q) out[q] = 0.0;
+      return;
+    }
+    for (int q = 0; q < K; ++q) {
+      double t = 0.0;
+      for (int w = 0; w < NT / 64; ++w) {
+        double x[64];
+        for (int l = 0; l < 64; ++l) {
+          const int g = 64 * w + l;
+          x[l] = g < used ? lane[(size_t)g * K + q] : 0.0;
+        }
+        for (int off = 32; off > 0; off >>= 1)
+          for (int l = 0; l < off; ++l) x[l] = x[l] + x[l + off];
+        t += x[0];
+      }
+      out[q] = t;
+    }
+  }
+};
+
 // Cell of particle k: floor(y/dy) on the slab, k N0 + j on the plane
 // (Main_Functions.cpp:2233-2263).
 int orc_key(const Oracle &o, int64_t k) {
@@ -692,24 +739,85 @@ void gl50(double *x, double *w) {
   }
 }
 
-// poisson_solver (Main_Functions.cpp:3285-3581) without the kick.
+// Forward or backward substitution x_k = A_k x_{k-1} + B_k (x_{-1} = 0) in
+// the association of a 1024-thread blocked scan: thread t composes the maps
+// of elements [tE, tE + E) (E = ceil(n / 1024)) into one affine map, an
+// inclusive Hillis-Steele scan over the threads composes the prefixes, and
+// each thread then replays its elements from its predecessor's value.  With
+// reverse the element order is n-1, ..., 0.  Mathematically this is the
+// serial recurrence of Thompson_3d (Math_Func.cpp:28-52); the blocked
+// association is what makes the result bit for bit reproducible against the
+// HIP path.
+void affine_scan(const std::vector<double> &A, const std::vector<double> &B,
+                 std::vector<double> &out, int n, bool reverse) {
+  const int nt = 1024;
+  const int E = (n + nt - 1) / nt;
+  std::vector<double> sA(nt), sB(nt);
+  for (int t = 0; t < nt; ++t) {
+    const int s0 = t * E, s1 = std::min(n, s0 + E);
+    double a = 1.0, b = 0.0;
+    for (int s = s0; s < s1; ++s) {
+      const int k = reverse ? (n - 1 - s) : s;
+      a = A[k] * a;
+      b = A[k] * b + B[k];
+    }
+    sA[t] = a;
+    sB[t] = b;
+  }
+  for (int off = 1; off < nt; off <<= 1) {
+    std::vector<double> nA = sA, nB = sB;
+    for (int t = off; t < nt; ++t) {
+      nA[t] = sA[t] * sA[t - off];
+      nB[t] = sA[t] * sB[t - off] + sB[t];
+    }
+    sA.swap(nA);
+    sB.swap(nB);
+  }
+  out.assign(n, 0.0);
+  for (int t = 0; t < nt; ++t) {
+    const int s0 = t * E, s1 = std::min(n, s0 + E);
+    double v = t == 0 ? 0.0 : sB[t - 1];
+    for (int s = s0; s < s1; ++s) {
+      const int k = reverse ? (n - 1 - s) : s;
+      v = A[k] * v + B[k];
+      out[k] = v;
+    }
+  }
+}
+
+// poisson_solver (Main_Functions.cpp:3285-3581) without the kick.  The face
+// system (2 + (dy bst)^2) x_f - x_{f-1} - x_{f+1} = dy^2 n_f (:3375-3377)
+// with Dirichlet rows x_0 = V_lo, x_nc = V_hi (:3448-3454), or, periodic,
+// the cyclic system by Sherman-Morrison.  Thompson_3d's elimination is
+// written as its LU factors (d_k = a_k - u_{k-1} l_{k-1} / d_{k-1}) and the
+// two substitutions run through affine_scan.  The Dirichlet values sum the
+// 50 x 50 Gauss-Legendre points (:3322-3442) as 1024 partial sums (point q
+// to partial q mod 1024) folded pairwise (partial t adds t + off for
+// off = 512, ..., 1).  exp is the shared deterministic mcpm_exp.
 void orc_field(Oracle &o) {
   const mcpm_params &p = o.p;
   const int nc = o.nc;
   const double dy = o.dy;
   const bool periodic = p.boundary == MCPM_BC_PERIODIC;
   const int nf = periodic ? nc : nc + 1;
-  std::vector<double> nj(nf), a(nf), b(nf, -1.0), c(nf, -1.0), rhs(nf), x;
+  std::vector<double> nj(nf), rhs(nf), x;
   for (int i = 0; i < nf; ++i) {
     if (periodic) nj[i] = (o.nden[i] + o.nden[(i + nc - 1) % nc]) / 2.0;
     else if (i == 0) nj[i] = o.nden[0];
     else if (i == nc) nj[i] = o.nden[nc - 1];
     else nj[i] = (o.nden[i] + o.nden[i - 1]) / 2.0;
-    a[i] = 2.0 + pow(dy * p.bst, 2.0);
-    rhs[i] = nj[i] * pow(dy, 2.0);
+    rhs[i] = nj[i] * (dy * dy);
   }
+  // matrix: diagonal a, lo[k] = A[k+1][k], up[k] = A[k][k+1]
+  const double diag = 2.0 + pow(dy * p.bst, 2.0);
+  std::vector<double> a(nf, diag), lo(nf, -1.0), up(nf, -1.0);
+  double vlast = 0.0;
   if (!periodic) {
-    double xi[50], wi[50], rr[50], dr[50], ph[50], dph[50];
+    a[0] = 1.0;
+    up[0] = 0.0;
+    a[nf - 1] = 1.0;
+    lo[nf - 2] = 0.0;
+    double xi[50], wi[50], rr[50], dr[50], ph[50], dph[50], cph[50], sph[50];
     gl50(xi, wi);
     double bb = 3.0 * p.eqdist, aa = 0.0, cc = 0.0, dd = PI;
     for (int i = 0; i < 50; ++i) {
@@ -717,45 +825,69 @@ void orc_field(Oracle &o) {
       dr[i] = wi[i] * (bb - aa) / 2.0;
       ph[i] = (dd - cc) / 2.0 * xi[i] + (cc + dd) / 2.0;
       dph[i] = wi[i] * (dd - cc) / 2.0;
+      cph[i] = cos(ph[i]);
+      sph[i] = sin(ph[i]);
     }
-    double V[2] = {0.0, 0.0};
-    for (int side = 0; side < 2; ++side) {
-      double f = side == 0 ? 0.0 : (double)nc;
-      for (int i = 0; i < 50; ++i) {
-        double radial = exp(-p.bst * rr[i]) / 2.0 * rr[i] * dr[i];
-        for (int ii = 0; ii < 50; ++ii) {
-          double y = f * dy + rr[i] * cos(ph[ii]);
-          double q = floor(y / dy);
-          double n;
-          if (q < 0.0) n = o.nden[0];
-          else if (q > (double)(nc - 1)) n = o.nden[nc - 1];
-          else n = o.nden[(int)q];
-          V[side] += radial * n * sin(ph[ii]) * dph[ii];
-        }
+    const int NT = 1024;
+    std::vector<double> part[2] = {std::vector<double>(NT, 0.0), std::vector<double>(NT, 0.0)};
+    for (int q = 0; q < 2500; ++q) {
+      const int i = q / 50, ii = q % 50;
+      const double radial = mcpm_exp(-p.bst * rr[i]) / 2.0 * rr[i] * dr[i];
+      for (int side = 0; side < 2; ++side) {
+        double y = (side == 0 ? 0.0 : (double)nc * dy) + rr[i] * cph[ii];
+        double qq = mcpm_floor(y / dy);
+        double n;
+        if (qq < 0.0) n = o.nden[0];
+        else if (qq > (double)(nc - 1)) n = o.nden[nc - 1];
+        else n = o.nden[(int)qq];
+        part[side][q % NT] += radial * n * sph[ii] * dph[ii];
       }
     }
-    c[0] = 0.0;
-    a[0] = 1.0;
-    rhs[0] = V[0];
-    a[nf - 1] = 1.0;
-    rhs[nf - 1] = V[1];
-    b[nf - 2] = 0.0;
-    thomas(a, b, c, x, rhs);
+    for (int off = NT / 2; off > 0; off >>= 1)
+      for (int t = 0; t < off; ++t) {
+        part[0][t] += part[0][t + off];
+        part[1][t] += part[1][t + off];
+      }
+    rhs[0] = part[0][0];
+    rhs[nf - 1] = part[1][0];
   } else {
-    // cyclic tridiagonal: Sherman-Morrison around Thompson_3d
+    // cyclic corners A[0][n-1] = beta = -1, A[n-1][0] = alpha = -1
     double gamma = -a[0], alpha = -1.0, beta = -1.0;
-    std::vector<double> ap = a, u(nf, 0.0), z, y;
-    ap[0] = a[0] - gamma;
-    ap[nf - 1] = a[nf - 1] - alpha * beta / gamma;
-    thomas(ap, b, c, y, rhs);
-    u[0] = gamma;
-    u[nf - 1] = alpha;
-    thomas(ap, b, c, z, u);
-    double fact = (y[0] + beta * y[nf - 1] / gamma) / (1.0 + z[0] + beta * z[nf - 1] / gamma);
-    x.resize(nf);
-    for (int i = 0; i < nf; ++i) x[i] = y[i] - fact * z[i];
+    a[0] -= gamma;
+    a[nf - 1] -= alpha * beta / gamma;
+    vlast = beta / gamma;
   }
-  double core = 1.0 - (p.sigma * p.bst + 1.0) * exp(-p.bst * p.sigma);  // :3539-3543
+  std::vector<double> mL(nf, 0.0), d(nf), invd(nf), mU(nf, 0.0);
+  d[0] = a[0];
+  for (int k = 1; k < nf; ++k) {
+    double l = lo[k - 1] / d[k - 1];
+    d[k] = a[k] - up[k - 1] * l;
+    mL[k] = -l;
+  }
+  for (int k = 0; k < nf; ++k) {
+    invd[k] = 1.0 / d[k];
+    mU[k] = (k < nf - 1) ? -up[k] / d[k] : 0.0;
+  }
+  std::vector<double> yv, tB(nf);
+  affine_scan(mL, rhs, yv, nf, false);
+  for (int k = 0; k < nf; ++k) tB[k] = yv[k] * invd[k];
+  affine_scan(mU, tB, x, nf, true);
+  if (periodic) {
+    // Sherman-Morrison vector z = A'^-1 u, u = (gamma, 0, ..., 0, alpha),
+    // by the serial substitutions
+    double gamma = -(2.0 + pow(dy * p.bst, 2.0));
+    std::vector<double> u(nf, 0.0), w(nf), z(nf);
+    u[0] = gamma;
+    u[nf - 1] = -1.0;
+    w[0] = u[0];
+    for (int k = 1; k < nf; ++k) w[k] = u[k] + mL[k] * w[k - 1];
+    z[nf - 1] = w[nf - 1] * invd[nf - 1];
+    for (int k = nf - 2; k >= 0; --k) z[k] = w[k] * invd[k] + mU[k] * z[k + 1];
+    const double den = 1.0 + z[0] + vlast * z[nf - 1];
+    const double corr = (x[0] + vlast * x[nf - 1]) / den;
+    for (int k = 0; k < nf; ++k) x[k] = x[k] - corr * z[k];
+  }
+  double core = 1.0 - (p.sigma * p.bst + 1.0) * mcpm_exp(-p.bst * p.sigma);  // :3539-3543
   core = core / p.bst / p.bst;
   for (int i = 0; i < nf; ++i) x[i] = x[i] - nj[i] * core;
   for (int cc = 0; cc < nc; ++cc) {
@@ -795,17 +927,25 @@ void orc_vlasov(Oracle &o) {
     ph[i] = (d - c) / 2.0 * xi[i] + (c + d) / 2.0;
     dph[i] = wi[i] * (d - c) / 2.0;
   }
+  double rad[50];
+  for (int i = 0; i < 50; ++i) {
+    double q = p.sigma / rr[i];
+    double q2 = q * q;
+    rad[i] = -(-(q2 * q2 * q2)) * rr[i] * rr[i] * (2.0 * PI) * dr[i];
+  }
+  // each face's 2500 points summed in WgSum's order (point 50 i + ii)
   std::vector<double> V(nf, 0.0);
   for (int fc = 0; fc < nf; ++fc) {
-    for (int i = 0; i < 50; ++i) {
-      double q = p.sigma / rr[i];
-      double q2 = q * q;
-      double radial = -(-(q2 * q2 * q2)) * rr[i] * rr[i] * (2.0 * PI) * dr[i];
-      for (int ii = 0; ii < 50; ++ii) {
-        double y = fc * o.dy + rr[i] * cos(ph[ii]);
-        V[fc] += -radial * orc_n_at(o, y) * sin(ph[ii]) * dph[ii];
-      }
+    WgSum<1> ws;
+    for (int q = 0; q < 2500; ++q) {
+      const int i = q / 50, ii = q % 50;
+      double y = (double)fc * o.dy + rr[i] * cos(ph[ii]);
+      double v[1] = {-rad[i] * orc_n_at(o, y) * sin(ph[ii]) * dph[ii]};
+      ws.add(q % 256, v);
     }
+    double t[1];
+    ws.total(t);
+    V[fc] = t[0];
   }
   for (int num = 0; num < o.nc; ++num) {
     int f1 = periodic ? (num + 1) % o.nc : num + 1;
@@ -823,14 +963,18 @@ void orc_kde(Oracle &o, std::vector<double> &ns, std::vector<double> &d2, std::v
   const int nc = o.nc;
   const double r_cut = 2.0 * o.dy, h = r_cut / 4.0;
   const double sqrt_pi = sqrt(2.0 * PI);
-  const int search = (int)floor(r_cut / o.dy);
+  const int search = (int)mcpm_floor(r_cut / o.dy);
   const double mb = o.p.Fn;
-  const double s2h2 = pow(1.0 / h / h, 2.0);
+  const double ih2 = 1.0 / h / h;
+  const double s2h2 = ih2 * ih2;
   ns.assign(nc, 0.0);
   d2.assign(nc, 0.0);
   d3.assign(nc, 0.0);
+  // each cell's sums in WgSum's order, lane i mod 256 for the i-th particle
+  // of each neighbour cell in turn
   for (int k = 0; k < nc; ++k) {
     const double r1 = (k + 0.5) * o.dy;
+    WgSum<3> ws;
     for (int kk = k - search; kk <= k + search; ++kk) {
       int kkk;
       double shift;
@@ -845,15 +989,24 @@ void orc_kde(Oracle &o, std::vector<double> &ns, std::vector<double> &d2, std::v
         double r2 = o.rec[4 * (size_t)(o.start[kkk] + i)] + shift;
         double r1r2 = r2 - r1;
         double rr = fabs(r2 - r1);
+        double v[3] = {0.0, 0.0, 0.0};
         if (rr < r_cut) {
           double ex = mcpm_exp(-rr * rr / h / h / 2.0);
-          ns[k] += mb / sqrt_pi / h * ex;
+          v[0] = mb / sqrt_pi / h * ex;
           double W = 1.0 / sqrt_pi / h * ex;
-          d2[k] += mb * W * r1r2 * (-1.0 / h / h);
-          d3[k] += mb * W * s2h2 * (3.0 * r1r2 - 1.0 / h / h * r1r2 * r1r2 * r1r2);
+          v[1] = mb * W * r1r2 * (-1.0 / h / h);
+          v[2] = mb * W * s2h2 * (3.0 * r1r2 - 1.0 / h / h * r1r2 * r1r2 * r1r2);
+          ws.add(i % 256, v);
+        } else {
+          ws.touch(i % 256);
         }
       }
     }
+    double t[3];
+    ws.total(t);
+    ns[k] = t[0];
+    d2[k] = t[1];
+    d3[k] = t[2];
   }
 }
 
@@ -871,45 +1024,48 @@ void orc_lr2(Oracle &o) {
 }
 
 void orc_moments(const Oracle &o, double *out) {
-  // two-pass moments of cell_update (Main_Functions.cpp:2483-2593)
+  // two-pass moments of cell_update (Main_Functions.cpp:2483-2593), each
+  // sum in the workgroup order of WgSum
   for (int c = 0; c < o.nc; ++c) {
     double *m = out + (size_t)c * MCPM_NMOM;
     for (int k = 0; k < MCPM_NMOM; ++k) m[k] = 0.0;
     int N = o.cnt[c];
     double U[3] = {0, 0, 0};
-    for (int k = 0; k < N; ++k) {
-      const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
-      U[0] += r[1];
-      U[1] += r[2];
-      U[2] += r[3];
+    {
+      WgSum<3> ws;
+      for (int k = 0; k < N; ++k) {
+        const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
+        double v[3] = {r[1], r[2], r[3]};
+        ws.add(k % 256, v);
+      }
+      ws.total(U);
     }
     if (N > 0)
       for (int q = 0; q < 3; ++q) U[q] /= (double)N;
-    double a[26] = {0};
-    double mp[3] = {0, 0, 0};
-    for (int k = 0; k < N; ++k) {
-      const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
-      double p1 = r[1] - U[0], p2 = r[2] - U[1], p3 = r[3] - U[2];
-      mp[0] += p1;
-      mp[1] += p2;
-      mp[2] += p3;
-      double e = p1 * p1 + p2 * p2 + p3 * p3;
-      a[0] += p1 * p1; a[1] += p1 * p2; a[2] += p1 * p3;
-      a[3] += p2 * p2; a[4] += p2 * p3; a[5] += p3 * p3;
-      a[6] += p1 * p1 * p1; a[7] += p1 * p1 * p2; a[8] += p1 * p1 * p3;
-      a[9] += p1 * p2 * p2; a[10] += p1 * p2 * p3; a[11] += p1 * p3 * p3;
-      a[12] += p2 * p2 * p2; a[13] += p2 * p2 * p3; a[14] += p2 * p3 * p3;
-      a[15] += p3 * p3 * p3;
-      a[16] += p1 * p1 * e; a[17] += p1 * p2 * e; a[18] += p1 * p3 * e;
-      a[19] += p2 * p2 * e; a[20] += p2 * p3 * e; a[21] += p3 * p3 * e;
-      a[22] += p1 * e * e; a[23] += p2 * e * e; a[24] += p3 * e * e;
-      a[25] += e;
+    // a[0..25] as below, a[26..28] the peculiar-velocity sums
+    double a[29];
+    {
+      WgSum<29> ws;
+      for (int k = 0; k < N; ++k) {
+        const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
+        double p1 = r[1] - U[0], p2 = r[2] - U[1], p3 = r[3] - U[2];
+        double e = p1 * p1 + p2 * p2 + p3 * p3;
+        double v[29] = {p1 * p1, p1 * p2, p1 * p3, p2 * p2, p2 * p3, p3 * p3,
+                        p1 * p1 * p1, p1 * p1 * p2, p1 * p1 * p3, p1 * p2 * p2,
+                        p1 * p2 * p3, p1 * p3 * p3, p2 * p2 * p2, p2 * p2 * p3,
+                        p2 * p3 * p3, p3 * p3 * p3,
+                        p1 * p1 * e, p1 * p2 * e, p1 * p3 * e,
+                        p2 * p2 * e, p2 * p3 * e, p3 * p3 * e,
+                        p1 * e * e, p2 * e * e, p3 * e * e, e, p1, p2, p3};
+        ws.add(k % 256, v);
+      }
+      ws.total(a);
     }
     m[0] = U[0]; m[1] = U[1]; m[2] = U[2];
     for (int q = 0; q < 26; ++q) m[3 + q] = N > 0 ? a[q] / (double)N : 0.0;
     m[29] = o.p.mass * m[28] / (3.0 * o.p.kb);  // T = m DM2 / (3 kB)
     m[30] = (double)N;
-    for (int q = 0; q < 3; ++q) m[31 + q] = N > 0 ? mp[q] / (double)N : 0.0;
+    for (int q = 0; q < 3; ++q) m[31 + q] = N > 0 ? a[26 + q] / (double)N : 0.0;
   }
 }
 
@@ -1138,19 +1294,34 @@ void orc_noise(const Oracle &o, int64_t step, std::vector<double> &xi) {
     xi[3 * k + 1] = z1;
     xi[3 * k + 2] = z2;
   }
+  // per-cell sums in WgSum's order
   for (int c = 0; c < o.nc; ++c) {
     int N = o.cnt[c];
     if (N < 2) continue;
-    double s[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
-    for (int k = 0; k < N; ++k)
-      for (int q = 0; q < 3; ++q) s[q] += xi[3 * (size_t)(o.start[c] + k) + q];
-    for (int q = 0; q < 3; ++q) s[q] = s[q] / (1.0 * N);
-    for (int k = 0; k < N; ++k)
-      for (int q = 0; q < 3; ++q) {
-        double v = xi[3 * (size_t)(o.start[c] + k) + q] - s[q];
-        xi[3 * (size_t)(o.start[c] + k) + q] = v;
-        s2[q] += v * v;
+    double s[3], s2[3];
+    {
+      WgSum<3> ws;
+      for (int k = 0; k < N; ++k) {
+        const double *x = &xi[3 * (size_t)(o.start[c] + k)];
+        double v[3] = {x[0], x[1], x[2]};
+        ws.add(k % 256, v);
       }
+      ws.total(s);
+    }
+    for (int q = 0; q < 3; ++q) s[q] = s[q] / (1.0 * N);
+    {
+      WgSum<3> ws;
+      for (int k = 0; k < N; ++k) {
+        double *x = &xi[3 * (size_t)(o.start[c] + k)];
+        double v[3];
+        for (int q = 0; q < 3; ++q) {
+          x[q] = x[q] - s[q];
+          v[q] = x[q] * x[q];
+        }
+        ws.add(k % 256, v);
+      }
+      ws.total(s2);
+    }
     for (int q = 0; q < 3; ++q) s2[q] = sqrt(s2[q] / (1.0 * N));
     for (int k = 0; k < N; ++k)
       for (int q = 0; q < 3; ++q) xi[3 * (size_t)(o.start[c] + k) + q] /= s2[q];
@@ -1193,12 +1364,12 @@ void orc_fp_velocity(Oracle &o, int64_t step) {
     const double *cc = cl + 1, *g = cl + 7;
     const double Q[3] = {m[9] + m[12] + m[14], m[10] + m[15] + m[17], m[11] + m[16] + m[18]};
     const double e0 = 3.0 * p.kb * T / p.mass;
-    double frac1 = 0.0, frac2 = 0.0;
+    // the rescale's two sums in WgSum's order
+    WgSum<2> ws;
     std::vector<double> mp(3 * (size_t)N);
     for (int k = 0; k < N; ++k) {
       const double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
       double a = r[1] - U[0], b = r[2] - U[1], d = r[3] - U[2];
-      frac1 += a * a + b * b + d * d;
       double e = a * a + b * b + d * d;
       double d0 = cc[0] * a + cc[1] * b + cc[2] * d;
       d0 += g[0] * (e - e0);
@@ -1223,9 +1394,12 @@ void orc_fp_velocity(Oracle &o, int64_t step) {
       mp[3 * k] = n0;
       mp[3 * k + 1] = n1;
       mp[3 * k + 2] = n2;
-      frac2 += n0 * n0 + n1 * n1 + n2 * n2;
+      double fr[2] = {e, n0 * n0 + n1 * n1 + n2 * n2};
+      ws.add(k % 256, fr);
     }
-    double s = sqrt(fabs(frac1 / frac2));
+    double fr[2];
+    ws.total(fr);
+    double s = sqrt(fabs(fr[0] / fr[1]));
     for (int k = 0; k < N; ++k) {
       double *r = &o.rec[4 * (size_t)(o.start[c] + k)];
       r[1] = rnd<real>(mp[3 * k] * s + U[0]);
@@ -1414,28 +1588,32 @@ void orc_cba_hs(Oracle &o, int64_t step, mcpm_collision_stats *st) {
   }
 }
 
+// cell_poly_velocity (Main_Functions.cpp:1918-1968): per cell the count and
+// the raw velocity sums of orders 1-3 and the fourth-order products with
+// |U|^2.  pow(U, k) is written as k-fold products, the third-order terms
+// U1^m U2^n U3^l (m = 0..3, n = 0..3-m, l = 3-m-n, :1940-1948) as the
+// products below; each sum over the cell in WgSum's order.
 void orc_sample(Oracle &o) {
-  // cell_poly_velocity (Main_Functions.cpp:1918-1968)
   for (int c = 0; c < o.nc; ++c) {
     double *s = &o.samples[(size_t)c * MCPM_NSAMP];
-    for (int k = 0; k < o.cnt[c]; ++k) {
+    const int N = o.cnt[c];
+    WgSum<25> ws;
+    for (int k = 0; k < N; ++k) {
       const double *r = &o.rec[(size_t)o.S * (o.start[c] + k)] + o.U - 1;
-      double U1 = r[1], U2 = r[2], U3 = r[3];
-      s[0] += 1.0;
-      s[1] += U1; s[2] += U2; s[3] += U3;
-      s[4] += U1 * U1; s[5] += U1 * U2; s[6] += U1 * U3;
-      s[7] += U2 * U2; s[8] += U2 * U3; s[9] += U3 * U3;
-      int id = 0;
-      for (int mm = 0; mm < 4; ++mm)
-        for (int nn = 0; nn < 4 - mm; ++nn) {
-          int ll = 3 - mm - nn;
-          s[10 + id] += pow(U1, mm) * pow(U2, nn) * pow(U3, ll);
-          ++id;
-        }
-      double e = U1 * U1 + U2 * U2 + U3 * U3;
-      s[20] += U1 * U1 * e; s[21] += U1 * U2 * e; s[22] += U1 * U3 * e;
-      s[23] += U2 * U2 * e; s[24] += U2 * U3 * e; s[25] += U3 * U3 * e;
+      const double u = r[1], v = r[2], w = r[3];
+      const double e = u * u + v * v + w * w;
+      double t[25] = {u, v, w,
+                      u * u, u * v, u * w, v * v, v * w, w * w,
+                      w * w * w, v * w * w, v * v * w, v * v * v,
+                      u * w * w, u * v * w, u * v * v,
+                      u * u * w, u * u * v, u * u * u,
+                      u * u * e, u * v * e, u * w * e, v * v * e, v * w * e, w * w * e};
+      ws.add(k % 256, t);
     }
+    double a[25];
+    ws.total(a);
+    s[0] += (double)N;
+    for (int q = 0; q < 25; ++q) s[1 + q] += a[q];
   }
 }
 
@@ -2024,6 +2202,20 @@ int orc_get_nodes(void *h, double *n_node, double *pot) {
     if (o.pot.size() != nn) o.pot.assign(nn, 0.0);
     memcpy(pot, o.pot.data(), nn * sizeof(double));
   }
+  return MCPM_OK;
+}
+
+/* The plane's nodal field set from outside (a test hands the oracle the
+ * field another solver produced, to compare the kick alone); the next push
+ * kicks with coef (ast for the screened Poisson field, phi for the direct
+ * integral). */
+int orc_set_nodes(void *h, const double *pot, double coef) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 2 || !pot) return MCPM_ERR_ARG;
+  const size_t nn = (size_t)(o.nx + 1) * (o.ny + 1);
+  o.pot.assign(pot, pot + nn);
+  o.kick_coef = coef;
+  o.kick_pending = true;
   return MCPM_OK;
 }
 

@@ -42,6 +42,7 @@ def load() -> ctypes.CDLL:
         "orc_upload_xy": (i32, [vp, i64, dpp, dpp, dpp, dpp, dpp]),
         "orc_download_xy": (i32, [vp, dpp, dpp, dpp, dpp, dpp]),
         "orc_get_nodes": (i32, [vp, dpp, dpp]),
+        "orc_set_nodes": (i32, [vp, dpp, ctypes.c_double]),
         "orc_gl_full": (None, [i32, dpp, dpp]),
         "orc_fem_force": (i32, [vp, ctypes.c_double, ctypes.c_double, dpp, ctypes.c_double,
                                 dpp, dpp]),
@@ -165,6 +166,11 @@ class OracleSim:
         self._check(self._lib.orc_get_nodes(self._h, _dp(nd), _dp(pot)), "orc_get_nodes")
         shape = (self.params.n_cells + 1, self.params.n_cells_x + 1)
         return nd.reshape(shape), pot.reshape(shape)
+
+    def set_nodes(self, pot, coef):
+        """Use nodal field pot (kick coefficient coef) for the next push."""
+        pot = np.ascontiguousarray(pot, dtype=np.float64).ravel()
+        self._check(self._lib.orc_set_nodes(self._h, _dp(pot), float(coef)), "orc_set_nodes")
 
     def fem_force(self, x, y, pot, coef):
         """Probe: force interpolation of the plane at (x, y) from nodal field pot."""

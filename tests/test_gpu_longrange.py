@@ -1,9 +1,9 @@
 """GPU parity of the direct (long_range 1) and density-expansion
 (long_range 2) mean-field forces and of the kernel density estimate.
 
-Sums over quadrature points and particles are reduced in a different order
-on the GPU than in the oracle: compared within 1e-12 relative to the field's
-scale."""
+Sums over quadrature points and particles are reduced in one fixed
+workgroup order on both sides, so forces, the density estimate and
+multi-step states are compared bit for bit."""
 import numpy as np
 import pytest
 
@@ -38,7 +38,8 @@ def test_direct_vlasov_matches_oracle(boundary):
     g, o = _pair(p, arrs)
     g.vlasov_integral()
     o.vlasov_integral()
-    assert _close(g.cells()["F2"], o.cells()["F2"])
+    assert np.abs(o.cells()["F2"]).max() > 0.0
+    np.testing.assert_array_equal(g.cells()["F2"], o.cells()["F2"])
 
 
 @pytest.mark.parametrize("boundary", ["walls", "periodic"])
@@ -52,10 +53,10 @@ def test_kde_and_density_expansion_match_oracle(boundary, precision):
     p.phi = -1.0
     g, o = _pair(p, arrs)
     for a, b in zip(g.kde(), o.kde()):
-        assert _close(a, b)
+        np.testing.assert_array_equal(a, b)
     g.density_expansion()
     o.density_expansion()
-    assert _close(g.cells()["F2"], o.cells()["F2"])
+    np.testing.assert_array_equal(g.cells()["F2"], o.cells()["F2"])
 
 
 @pytest.mark.parametrize("lr", [mcpm.LR_DIRECT, mcpm.LR_DENSITY_EXPANSION])
@@ -68,11 +69,12 @@ def test_full_steps_with_direct_and_expansion_forces(lr):
     g, o = _pair(p, arrs)
     o.init_omega_max(meta["n_gas"])
     g.set_omega_max(o.cells()["omega_max"])
-    for s in range(4):
+    for s in range(5):
         sg, so = g.step(s), o.step(s)
-        assert sg["candidates"] == so["candidates"]
-    cg, co = g.cells(), o.cells()
-    assert np.abs(cg["count"] - co["count"]).sum() <= 4
-    mg, mo = g.moments(), o.moments()
-    sel = co["count"] > 50
-    assert np.abs(mg[sel, 29] - mo[sel, 29]).max() <= 1e-9
+        for k in ("candidates", "valid", "accepted"):
+            assert sg[k] == so[k], (s, k)
+        cg, co = g.cells(), o.cells()
+        np.testing.assert_array_equal(cg["count"], co["count"])
+        np.testing.assert_array_equal(cg["F2"], co["F2"])
+        for k, (a, b) in enumerate(zip(g.download(), o.download())):
+            np.testing.assert_array_equal(a, b, err_msg=f"component {k} step {s}")

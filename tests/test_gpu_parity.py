@@ -1,13 +1,12 @@
 """GPU parity: libmcpm.so on the MI355X against the CPU oracle.
 
-Bit-exact where the work is integer or follows the same sequence of IEEE
-operations: cell counts and the stable particle order, the collision
-candidates (priority, i, j) and their accept decisions, post-collision
-velocities, wall re-emission.  Within 1e-12 relative (FP64) where only the
-summation order differs: the field solve (block scan vs serial Thomas) and
-per-cell moments.  Multi-step runs start from the same state; after a field
-solve the kicked velocities can differ in the last bit, so multi-step
-comparisons use 1e-12 (FP64) and 1e-5 (FP32) relative tolerances.
+Bit-exact throughout: cell counts and the stable particle order, the
+collision candidates (priority, i, j) and their accept decisions,
+post-collision velocities, wall re-emission, the field solve and the
+per-cell moments.  The oracle's sums and its tridiagonal solve follow the
+HIP path's fixed association (a 256-lane workgroup fold for per-cell sums,
+a 1024-thread blocked affine scan for the substitutions), so multi-step runs
+with a field stay bit-identical as well.
 """
 import numpy as np
 import pytest
@@ -143,55 +142,40 @@ def test_field_solve_matches_oracle(boundary):
     g.poisson_solver()
     o.poisson_solver()
     fg, fo = g.cells()["F2"], o.cells()["F2"]
-    assert np.abs(fg - fo).max() <= 1e-12 * np.abs(fo).max()
+    assert np.abs(fo).max() > 0.0
+    np.testing.assert_array_equal(fg, fo)
 
 
 @pytest.mark.parametrize("case", ["heat_fp64", "heat_fp32", "interface_fp32", "interface_fp64"])
 def test_full_steps_match_oracle(case):
+    prec = mcpm.FP64 if case.endswith("fp64") else mcpm.FP32
     if case.startswith("heat"):
-        prec = mcpm.FP64 if case.endswith("fp64") else mcpm.FP32
         p, arrs, meta = W.heat_conduction(precision=prec, seed=8)
     else:
-        prec = mcpm.FP64 if case.endswith("fp64") else mcpm.FP32
         p, arrs, meta = W.interface(N=50000, n_cells=256, length=100.0, precision=prec, seed=8)
     g, o = _pair(p, arrs)
     o.init_omega_max(meta["n_gas"])
     g.set_omega_max(o.cells()["omega_max"])
-    steps = 5
-    for s in range(steps):
+    for s in range(6):
         sg = g.step(s)
         so = o.step(s)
-        assert sg["candidates"] == so["candidates"]
-        assert sg["valid"] == so["valid"]
-        if case.startswith("heat"):
-            # no field solve: every operation is the same IEEE sequence
-            assert sg["accepted"] == so["accepted"]
-    if case.startswith("heat"):
+        for k in ("candidates", "valid", "accepted"):
+            assert sg[k] == so[k], (s, k)
         _assert_cells_equal(g, o)
         _assert_particles_equal(g, o)
-    else:
-        tol = 1e-12 if prec == mcpm.FP64 else 1e-5
-        cg, co = g.cells(), o.cells()
-        assert np.abs(cg["count"] - co["count"]).sum() <= 4
-        mg, mo = g.moments(), o.moments()
-        # per-cell temperature and mean velocity
-        sel = co["count"] > 50
-        assert np.abs(mg[sel, 29] - mo[sel, 29]).max() <= max(tol, 1e-4 if prec == mcpm.FP32 else tol) * 10
+        np.testing.assert_array_equal(g.cells()["F2"], o.cells()["F2"])
+    np.testing.assert_array_equal(g.moments(), o.moments())
 
 
 def test_moments_and_samples_match_oracle():
     p, arrs, _ = W.heat_conduction(seed=9)
     g, o = _pair(p, arrs)
-    mg, mo = g.moments(), o.moments()
-    scale = np.maximum(np.abs(mo), 1.0)
-    assert np.all(np.abs(mg - mo) <= 1e-12 * scale)
+    np.testing.assert_array_equal(g.moments(), o.moments())
     g.sample()
     o.sample()
     g.sample()
     o.sample()
-    sg, so = g.samples(), o.samples()
-    scale = np.maximum(np.abs(so), 1.0)
-    assert np.all(np.abs(sg - so) <= 1e-12 * scale)
+    np.testing.assert_array_equal(g.samples(), o.samples())
 
 
 def test_domain_and_nan_errors_are_reported():
