@@ -342,6 +342,21 @@ int mcpm_get_escaped(mcpm_ctx *ctx, int64_t *count);
    stand for), out[1..3] = m Fn sum U (momentum).  Not on batched replicas. */
 int mcpm_energy_momentum(mcpm_ctx *ctx, double *out);
 
+/* The energy check without allocation.  No counterpart in the reference:
+   for callers that cannot allocate device memory (code under stream
+   capture, tight serving loops, harnesses that forbid hipMalloc).  The
+   same sums, in the same order, as mcpm_energy_momentum, over caller-owned
+   device arrays u1, u2, u3 of n FP64 velocities; out: device pointer to 4
+   doubles, out[0] = (1/2) mass_fn sum |U|^2, out[1..3] = mass_fn sum U
+   (mass_fn = m Fn).  Asynchronous on `stream` (a hipStream_t; null for the
+   default stream).  Calls no hipMalloc/hipFree: the per-workgroup partials
+   sit in a fixed 32 KiB device array of the library (one per GPU, kept
+   until the process ends), which holds them for any n.  Calls that overlap
+   on one GPU share that array and race.  MCPM_ERR_ARG for n < 0 or a null
+   pointer. */
+int mcpm_energy_momentum_noalloc(const double *u1, const double *u2, const double *u3,
+                                 int64_t n, double mass_fn, double *out, void *stream);
+
 /* thermostat (Main_Functions.cpp:1674-1869), the inverted problem's branch
    (:1760-1830): after the cell moments are taken from the current particles
    (cell_update), every particle of a cell whose centre lies below zone_lo

@@ -1389,7 +1389,6 @@ int launch_thermostat(mcpm_ctx *c, double zone_lo, double T_lo, double zone_hi, 
   }
 }
 
-constexpr int ENERGY_GRID = 1024;
 
 template <typename R>
 int launch_energy(mcpm_ctx *c, double *out) {

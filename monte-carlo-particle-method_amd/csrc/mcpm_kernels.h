@@ -133,6 +133,11 @@ __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams 
                         double bst, double eqdist, const FieldArrays *rfa);
 template <typename R>
 __global__ void k_energy(const R *P, int64_t n, double *part);
+__global__ void k_energy_soa(const double *u1, const double *u2, const double *u3, int64_t n,
+                             double *part);
+__global__ void k_energy_finish(const double *part, int nblk, double mf, double *out);
+// workgroups of the energy check (its per-workgroup partials: 4 doubles each)
+constexpr int ENERGY_GRID = 1024;
 template <typename R>
 __global__ void k_thermostat(R *P, int64_t n, const double *mom, KParams kp, double zone_lo,
                              double T_lo, double zone_hi, double T_hi);

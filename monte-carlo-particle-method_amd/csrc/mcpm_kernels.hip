@@ -1659,16 +1659,16 @@ __global__ __launch_bounds__(256) void k_thermostat(R *__restrict__ P, int64_t n
   P[i] = r;
 }
 // FP64 sums over the particles for the energy and momentum check: per
-// workgroup {sum |U|^2, sum U1, sum U2, sum U3} into part[4 blockIdx.x].
-template <typename R>
-__global__ __launch_bounds__(256) void k_energy(const R *__restrict__ P, int64_t n,
-                                                double *__restrict__ part) {
+// workgroup {sum |U|^2, sum U1, sum U2, sum U3} into part[4 blockIdx.x],
+// particle i (grid-stride) read by load(i, &u1, &u2, &u3).
+template <typename Load>
+__device__ __forceinline__ void energy_partial(Load load, int64_t n, double *__restrict__ part) {
   __shared__ double sh[4][4];
   double e = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const R r = P[i];
-    const double a = (double)r.u1, b = (double)r.u2, c = (double)r.u3;
+    double a, b, c;
+    load(i, &a, &b, &c);
     e += a * a + b * b + c * c;
     m1 += a;
     m2 += b;
@@ -1686,6 +1686,44 @@ __global__ __launch_bounds__(256) void k_energy(const R *__restrict__ P, int64_t
     const int k = threadIdx.x;
     part[4 * (size_t)blockIdx.x + k] = (sh[0][k] + sh[1][k]) + (sh[2][k] + sh[3][k]);
   }
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_energy(const R *__restrict__ P, int64_t n,
+                                                double *__restrict__ part) {
+  energy_partial(
+      [&](int64_t i, double *a, double *b, double *c) {
+        const R r = P[i];
+        *a = (double)r.u1;
+        *b = (double)r.u2;
+        *c = (double)r.u3;
+      },
+      n, part);
+}
+
+// The same sums over three FP64 velocity arrays (the reference's U1, U2, U3).
+__global__ __launch_bounds__(256) void k_energy_soa(const double *__restrict__ u1,
+                                                    const double *__restrict__ u2,
+                                                    const double *__restrict__ u3, int64_t n,
+                                                    double *__restrict__ part) {
+  energy_partial(
+      [&](int64_t i, double *a, double *b, double *c) {
+        *a = u1[i];
+        *b = u2[i];
+        *c = u3[i];
+      },
+      n, part);
+}
+
+// out[0] = (1/2) mf sum_b part[4b], out[k] = mf sum_b part[4b + k]: the
+// partials added in workgroup order, one thread per sum.
+__global__ void k_energy_finish(const double *__restrict__ part, int nblk, double mf,
+                                double *__restrict__ out) {
+  const int k = threadIdx.x;
+  if (blockIdx.x != 0 || k >= 4) return;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[4 * (size_t)b + k];
+  out[k] = k == 0 ? 0.5 * mf * s : mf * s;
 }
 template __global__ void k_energy<Rec<float>>(const Rec<float> *, int64_t, double *);
 template __global__ void k_energy<Rec<double>>(const Rec<double> *, int64_t, double *);

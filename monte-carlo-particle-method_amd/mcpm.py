@@ -272,6 +272,7 @@ def load_library() -> ctypes.CDLL:
         "mcpm_measure_pressure": (i32, [vp, i32]),
         "mcpm_get_escaped": (i32, [vp, ctypes.POINTER(i64)]),
         "mcpm_energy_momentum": (i32, [vp, dpp]),
+        "mcpm_energy_momentum_noalloc": (i32, [vp, vp, vp, i64, ctypes.c_double, vp, vp]),
         "mcpm_thermostat": (i32, [vp, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_double]),
         "mcpm_read_settings": (i32, [ctypes.c_char_p, ctypes.POINTER(CSettings)]),
@@ -333,6 +334,17 @@ def contact_y(p: Params, n: float) -> float:
     lib = load_library()
     cp = p.to_c()
     return lib.mcpm_contact_y(ctypes.byref(cp), float(n))
+
+
+def energy_momentum_noalloc(u1_ptr: int, u2_ptr: int, u3_ptr: int, n: int, mass_fn: float,
+                            out_ptr: int, stream: int = 0) -> None:
+    """mcpm_energy_momentum_noalloc on device addresses (u1, u2, u3: n FP64
+    velocities; out: 4 doubles); asynchronous on `stream` (0: default)."""
+    lib = load_library()
+    st = lib.mcpm_energy_momentum_noalloc(u1_ptr, u2_ptr, u3_ptr, int(n), float(mass_fn), out_ptr,
+                                          stream or None)
+    if st:
+        raise McpmError(st, "mcpm_energy_momentum_noalloc")
 
 
 def probe_random_access(n_records: int, record_bytes: int) -> dict:
