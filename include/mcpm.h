@@ -74,7 +74,8 @@ enum mcpm_long_range {
    with or without the dense-gas Y(n) and closures, DSMC_VHS ->
    Collision_DSMC_VHS, NONE -> free flight, DSMC_HS -> Collision_DSMC_HS,
    CBA_VHS -> Collision_CBA_VHS, CBA_HS -> Collision_CBA_HS, FP_LINEAR ->
-   Velocity_FP_linear. */
+   Velocity_FP_linear, ESMC_NEW4 -> Collision_ESMC_new4 (slab only; the
+   reference's call of it is commented out at Main_Functions.cpp:1560). */
 enum mcpm_model {
   MCPM_MODEL_ESMC = 0,
   MCPM_MODEL_FP_DENSE = 1,
@@ -84,7 +85,8 @@ enum mcpm_model {
   MCPM_MODEL_DSMC_HS = 5,
   MCPM_MODEL_CBA_VHS = 6,
   MCPM_MODEL_CBA_HS = 7,
-  MCPM_MODEL_FP_LINEAR = 8
+  MCPM_MODEL_FP_LINEAR = 8,
+  MCPM_MODEL_ESMC_NEW4 = 9
 };
 
 /* ---- parameters -------------------------------------------------------- */
@@ -182,6 +184,16 @@ int mcpm_cell_update(mcpm_ctx *ctx);
    SMC collisions with the Y(n) contact factor.  `step` addresses the random
    stream. */
 int mcpm_collide_esmc(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
+
+/* Collision_ESMC_new4 (cpp_headers.h:364; collisions.cpp:722-837): the same
+   Enskog collisions in new4's own order, cells from the last to the first
+   and each cell's candidates in turn, candidates only in cells of two or
+   more particles, the partner cell floor((y + sigma dir_y) / dy) clamped into
+   the slab (wrapped on a periodic slab), and acceptance only of approaching
+   pairs (g.dir > 0).  The strict cell order chains the cells together: the
+   reservation rounds grow with the number of cells, up to the schedule's
+   limit (MCPM_ERR_CAPACITY beyond it).  Slab only. */
+int mcpm_collide_esmc_new4(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
 
 /* poisson_solver (cpp_headers.h:450; Main_Functions.cpp:3285-3581): the 1D
    screened-Poisson mean field and the kick U2 += F2 dt.  This call only
@@ -356,6 +368,17 @@ int mcpm_energy_momentum(mcpm_ctx *ctx, double *out);
    pointer. */
 int mcpm_energy_momentum_noalloc(const double *u1, const double *u2, const double *u3,
                                  int64_t n, double mass_fn, double *out, void *stream);
+
+/* Particle identities.  The reference keeps every particle at its own index
+   of the caller's arrays and rebuilds cells[c].indices_inside from those
+   indices (Main_Functions.cpp:2367-2376); here particles move through the
+   stable sort.  With tracking on, every sort also carries each particle's
+   upload slot (its index in the arrays of the last mcpm_upload /
+   mcpm_upload_xy / mcpm_initialize, or in the order when tracking was
+   switched on), and mcpm_download_ids returns, for each position of
+   mcpm_download, that slot: a caller can keep its own indexing. */
+int mcpm_track_ids(mcpm_ctx *ctx, int32_t enable);
+int mcpm_download_ids(mcpm_ctx *ctx, int32_t *ids);
 
 /* thermostat (Main_Functions.cpp:1674-1869), the inverted problem's branch
    (:1760-1830): after the cell moments are taken from the current particles

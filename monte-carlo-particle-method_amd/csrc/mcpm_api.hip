@@ -44,6 +44,10 @@ struct mcpm_ctx {
   size_t rec = 0;
   void *P[2] = {nullptr, nullptr};
   int cur = 0;
+  // particle ids carried through every sort when tracking is on: ids[cur][k]
+  // is the upload slot of the particle at position k
+  uint32_t *ids[2] = {nullptr, nullptr};
+  bool track_ids = false;
   // sort
   int64_t nchunks_cap = 0;
   int seg = 32;
@@ -530,11 +534,11 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   // one stable counting-sort pass over nb bins: histogram, scans, scatter
   auto pass = [&](const R *in, R *out, int pmode, const Kick &ks, int nb, int kmode,
                   uint32_t *cnt, uint32_t *start, double *nden, double *yc, uint32_t *okey,
-                  const uint32_t *ikey) -> int {
+                  const uint32_t *ikey, const uint32_t *iid, uint32_t *oid) -> int {
     const size_t lds = (size_t)nb * sizeof(uint32_t);
     hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
                        c->s, in, nullptr, c->n, chunk, c->kp, kick, drift, pmode, slo, shi, c->table,
-                       nullptr, c->err, nb, kmode, c->kdiv, nullptr, ikey);
+                       nullptr, c->err, nb, kmode, c->kdiv, nullptr, ikey, nullptr, nullptr);
     LAUNCHCHK();
     dim3 gs((nb + 255) / 256, nseg);
     hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
@@ -547,26 +551,28 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
     hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
                        c->s, in, out,
                        c->n, chunk, c->kp, ks, drift, pmode, slo, shi, c->table, start,
-                       c->err, nb, kmode, c->kdiv, okey, ikey);
+                       c->err, nb, kmode, c->kdiv, okey, ikey, iid, oid);
     LAUNCHCHK();
     return MCPM_OK;
   };
   const R *in = (const R *)c->P[c->cur];
   R *out = (R *)c->P[1 - c->cur];
+  uint32_t *id_in = c->track_ids ? c->ids[c->cur] : nullptr;
+  uint32_t *id_out = c->track_ids ? c->ids[1 - c->cur] : nullptr;
   int st;
   if (!c->two_digit) {
     if ((st = pass(in, out, mode, kick_scatter, nc, 0, c->cnt, c->start, c->nden, c->yc,
-                   nullptr, nullptr)))
+                   nullptr, nullptr, id_in, id_out)))
       return st;
   } else {
     // low digit with the push (keys carried to the destinations), then the
     // high digit of those keys back into the first buffer; cells from the
     // sorted keys
     if ((st = pass(in, out, mode, kick_scatter, c->nb_lo, 1, c->dcnt, c->dstart, nullptr,
-                   nullptr, c->keys_tmp, nullptr)))
+                   nullptr, c->keys_tmp, nullptr, id_in, id_out)))
       return st;
     if ((st = pass(out, (R *)in, 0, kick, c->nb_hi, 2, c->dcnt, c->dstart, nullptr, nullptr,
-                   c->keys, c->keys_tmp)))
+                   c->keys, c->keys_tmp, id_out, id_in)))
       return st;
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->s, c->start, (int64_t)nc + 1, 0u);
     hipLaunchKernelGGL(k_cell_bounds, dim3(2048), dim3(256), 0, c->s, c->keys, c->n, nc,
@@ -645,7 +651,8 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   // ESMC priorities spread over 32 bits and enter the rounds in bands; DSMC
   // operations of different cells never conflict, one band keeps every cell
   // busy from the first round
-  const int nbands = MODEL == COLL_ESMC ? std::max(1, std::min<int>(c->nbands, MAX_BANDS)) : 1;
+  // (new4's priorities are ranks in its cell order: its bands are rank ranges)
+  const int nbands = MODEL != COLL_DSMC ? std::max(1, std::min<int>(c->nbands, MAX_BANDS)) : 1;
   const uint32_t band_mul = MODEL == COLL_ESMC ? (uint32_t)nbands : 0u;
   // a multiple of 8 blocks so that every XCD gets the same share
   unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256),
@@ -664,9 +671,8 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   if (M > 0) {
     Timer tm(c, T_GEN);
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
-    const int esmc = MODEL == COLL_ESMC;
-    hipLaunchKernelGGL(k_cand_count, dim3(nchunks), dim3(256), 0, c->s, M, pk.v[0], esmc,
-                       band_mul, nbands, c->gen_table, c->kp);
+    hipLaunchKernelGGL(k_cand_count, dim3(nchunks), dim3(256), 0, c->s, M, pk.v[0], (int)MODEL,
+                       band_mul, nbands, c->gen_table, c->kp, c->cstart, nc);
     hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
                        nchunks, nbands * NREG, c->counters);
     hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
@@ -757,6 +763,10 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
 template <typename R>
 int collide_esmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   return collide_t<R, COLL_ESMC>(c, step, stats, MCPM_MODEL_ESMC);
+}
+template <typename R>
+int collide_esmc4_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  return collide_t<R, COLL_ESMC4>(c, step, stats, MCPM_MODEL_ESMC_NEW4);
 }
 template <typename R>
 int collide_dsmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int variant) {
@@ -1017,6 +1027,8 @@ int launch_init(mcpm_ctx *c, const InitParams &ip) {
   if (ip.n > 0)
     hipLaunchKernelGGL((k_init<R>), dim3((unsigned)((ip.n + 255) / 256)), dim3(256), 0, c->s,
                        (R *)c->P[c->cur], ip, c->kp);
+  if (c->track_ids && ip.n > 0)
+    hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], ip.n);
   LAUNCHCHK();
   return MCPM_OK;
 }
@@ -1072,7 +1084,8 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
   if (p->long_range == MCPM_LR_SCREENED_POISSON && !(p->bst > 0.0)) return MCPM_ERR_ARG;
   if (p->long_range < MCPM_LR_NONE || p->long_range > MCPM_LR_SCREENED_POISSON)
     return MCPM_ERR_ARG;
-  if (p->model < MCPM_MODEL_ESMC || p->model > MCPM_MODEL_FP_LINEAR) return MCPM_ERR_ARG;
+  if (p->model < MCPM_MODEL_ESMC || p->model > MCPM_MODEL_ESMC_NEW4) return MCPM_ERR_ARG;
+  if (p->model == MCPM_MODEL_ESMC_NEW4 && p->dims == 2) return MCPM_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return MCPM_ERR_HIP;
   mcpm_ctx *c = new mcpm_ctx();
@@ -1298,7 +1311,7 @@ int mcpm_destroy(mcpm_ctx *c) {
                   c->samples, c->press, c->counters, c->err, c->esc, c->energy_part, c->res, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
                   c->fem_buf, c->q4, c->fft_buf, c->fft_twx, c->fft_twy,
-                  c->fft_cosx, c->fft_cosy};
+                  c->fft_cosx, c->fft_cosy, c->ids[0]};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < T_NCLASS; ++k)
@@ -1355,6 +1368,10 @@ int upload_any(mcpm_ctx *c, int64_t n, const double *const *src, int nsrc) {
   c->n = n;
   c->sorted = false;
   c->kick_pending = false;
+  if (c->track_ids && n > 0) {
+    hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], n);
+    LAUNCHCHK();
+  }
   return MCPM_OK;
 }
 
@@ -1482,6 +1499,17 @@ int mcpm_collide_esmc(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
     return MCPM_OK;
   }
   return MCPM_DISPATCH(c, collide_esmc_t, c, step, stats);
+}
+
+int mcpm_collide_esmc_new4(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims == 2 || c->kp.nrep) return MCPM_ERR_ARG;  // the reference's 1D operator
+  if (!c->sorted) return MCPM_ERR_STATE;
+  if (c->n == 0) {
+    if (stats) memset(stats, 0, sizeof *stats);
+    return MCPM_OK;
+  }
+  return MCPM_DISPATCH(c, collide_esmc4_t, c, step, stats);
 }
 
 int mcpm_collide_dsmc_vhs(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
@@ -1689,6 +1717,7 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   // velocity_update dispatch (Main_Functions.cpp:1545-1600)
   switch (c->p.model) {
     case MCPM_MODEL_ESMC: st = mcpm_collide_esmc(c, step, stats); break;
+    case MCPM_MODEL_ESMC_NEW4: st = mcpm_collide_esmc_new4(c, step, stats); break;
     case MCPM_MODEL_FP_DENSE:
     case MCPM_MODEL_FP_IDEAL:
     case MCPM_MODEL_FP_LINEAR:
@@ -1814,6 +1843,34 @@ int mcpm_energy_momentum(mcpm_ctx *c, double *out) {
   if (!c || !out) return MCPM_ERR_ARG;
   if (c->kp.nrep) return MCPM_ERR_ARG;
   return MCPM_DISPATCH(c, launch_energy, c, out);
+}
+
+int mcpm_track_ids(mcpm_ctx *c, int32_t enable) {
+  if (!c) return MCPM_ERR_ARG;
+  if (!enable) {
+    c->track_ids = false;
+    return MCPM_OK;
+  }
+  if (!c->ids[0]) {
+    int st = dalloc(&c->ids[0], 2 * (size_t)c->p.capacity);
+    if (st) return st;
+    c->ids[1] = c->ids[0] + c->p.capacity;
+  }
+  c->track_ids = true;
+  if (c->n > 0) {
+    hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], c->n);
+    LAUNCHCHK();
+  }
+  return MCPM_OK;
+}
+
+int mcpm_download_ids(mcpm_ctx *c, int32_t *ids) {
+  if (!c || (!ids && c->n > 0)) return MCPM_ERR_ARG;
+  if (!c->track_ids) return MCPM_ERR_STATE;
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (c->n > 0)
+    HIPCHK(hipMemcpy(ids, c->ids[c->cur], (size_t)c->n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+  return MCPM_OK;
 }
 
 int mcpm_thermostat(mcpm_ctx *c, double zone_lo, double T_lo, double zone_hi, double T_hi) {

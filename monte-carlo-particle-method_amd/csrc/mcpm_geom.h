@@ -85,6 +85,17 @@ __host__ __device__ __forceinline__ bool partner_cell(const Rec2<real> &ri, doub
   return true;
 }
 
+// Collision_ESMC_new4's partner cell on the slab (collisions.cpp:781-792):
+// floor((y_i + sigma dir_y) / dy) clamped into the slab's cells.
+template <typename real>
+__host__ __device__ __forceinline__ int32_t partner_cell_clamped(const Rec<real> &ri, double d1,
+                                                                 const KParams &kp) {
+  double q = mcpm_floor(((double)ri.y + kp.sigma * d1) / kp.dy);
+  if (q < 0.0) return 0;
+  if (q > (double)(kp.n_cells - 1)) return kp.n_cells - 1;
+  return (int32_t)q;
+}
+
 // ---- midpoint cell of a pair (collisions.cpp:682-686) ----------------------
 __host__ __device__ __forceinline__ double axis_mid(double a, double b, double L, bool periodic) {
   if (!periodic) return (a + b) / 2.0;

@@ -56,8 +56,10 @@ struct FieldArrays {
   double sm_vlast, sm_den;
 };
 
-// collision models sharing the reservation rounds
-enum : int { COLL_ESMC = 0, COLL_DSMC = 1 };
+// collision models sharing the reservation rounds: ESMC in the canonical
+// order of Collision_ESMC_new3, DSMC in cell order, ESMC in the order of
+// Collision_ESMC_new4 (cells descending, each cell's candidates in turn)
+enum : int { COLL_ESMC = 0, COLL_DSMC = 1, COLL_ESMC4 = 2 };
 
 // R is Rec<float|double> (1D slab) or Rec2<float|double> (2D plane).
 template <typename R, bool SCATTER>
@@ -65,7 +67,8 @@ __global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp
                        const double *drift, int mode,
                        uint32_t step_lo, uint32_t step_hi, uint32_t *table,
                        const uint32_t *start, uint32_t *err, int nb, int kmode, int32_t kdiv,
-                       uint32_t *okey, const uint32_t *ikey);
+                       uint32_t *okey, const uint32_t *ikey, const uint32_t *iid,
+                       uint32_t *oid);
 // Most bins of one sort pass (one LDS histogram of 64 KB per chunk); more
 // cells sort by two digits of the cell key, up to MAX_CELLS.
 constexpr int SORT_BINS = 16384;
@@ -92,13 +95,26 @@ __device__ __forceinline__ uint32_t op_band(uint32_t pr, uint32_t band_mul, int 
   uint32_t b = (uint32_t)(((uint64_t)pr * band_mul) >> 32);
   return b < (uint32_t)nbands ? b : (uint32_t)nbands - 1u;
 }
+// Collision_ESMC_new4 processes the cells in descending order and each
+// cell's candidates in turn (collisions.cpp:758-835): the rank of candidate t
+// of cell c in that order is its priority, and its band the rank's share of
+// the M candidates.
+__device__ __forceinline__ uint32_t rank_new4(uint32_t c, uint32_t t, const uint32_t *cstart,
+                                              uint32_t M) {
+  return (M - cstart[c + 1]) + t;
+}
+__device__ __forceinline__ uint32_t band_new4(uint32_t rank, uint32_t M, int nbands) {
+  uint32_t b = (uint32_t)(((uint64_t)rank * (uint64_t)nbands) / (uint64_t)(M ? M : 1u));
+  return b < (uint32_t)nbands ? b : (uint32_t)nbands - 1u;
+}
 // Candidates are generated in chunks of GEN_CHUNK consecutive candidate
 // indices (cell order).  A candidate's list (priority band, region) depends
 // only on its index, so the lists are sized before anything is drawn.
 constexpr uint32_t GEN_CHUNK = 2048;
 constexpr int NLIST = MAX_BANDS * NREG;
-__global__ void k_cand_count(uint32_t M, uint32_t prio_key, int esmc, uint32_t band_mul,
-                             int nbands, uint32_t *table, KParams kp);
+__global__ void k_cand_count(uint32_t M, uint32_t prio_key, int model, uint32_t band_mul,
+                             int nbands, uint32_t *table, KParams kp, const uint32_t *cstart,
+                             int nc);
 __global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters);
 __global__ void k_list_starts(uint32_t *counters, int nl);
 template <typename R, int MODEL>
@@ -129,6 +145,7 @@ __global__ void k_rep_step(KParams kp, const uint32_t *cstart, uint32_t *rprio,
                            uint32_t *rcand_off, uint32_t step_lo, uint32_t step_hi);
 __global__ void k_zero_u32(uint32_t *p, int n);
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v);
+__global__ void k_iota_u32(uint32_t *p, int64_t n);
 __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams kp, double ast,
                         double bst, double eqdist, const FieldArrays *rfa);
 template <typename R>

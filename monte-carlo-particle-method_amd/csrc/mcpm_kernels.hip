@@ -288,7 +288,8 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
     const R *__restrict__ in, R *__restrict__ out, int64_t n, int32_t chunk, KParams kp,
     Kick kick, const double *__restrict__ drift, int mode, uint32_t step_lo, uint32_t step_hi,
     uint32_t *__restrict__ table, const uint32_t *__restrict__ start, uint32_t *err, int nb,
-    int kmode, int32_t kdiv, uint32_t *__restrict__ okey, const uint32_t *__restrict__ ikey) {
+    int kmode, int32_t kdiv, uint32_t *__restrict__ okey, const uint32_t *__restrict__ ikey,
+    const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid) {
   extern __shared__ uint32_t lds_cnt[];
   constexpr int TILE = 64 * PUSH_WAVES;  // particles of one tile, wavefront w the w-th 64
   const int nc = nb;
@@ -405,6 +406,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
               const uint32_t dst = base + rk;
               out[dst] = r;
               if (okey) okey[dst] = okv;
+              if (oid) oid[dst] = iid[p];
               if (rk + 1 == tot) lds_cnt[key] = base + tot;
             }
             __builtin_amdgcn_wave_barrier();
@@ -549,6 +551,12 @@ __global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restri
     if (variant == MCPM_MODEL_CBA_VHS) Mc = Mc * yc[c];
     m = mcpm_floor(Mc + mcpm_u01(w.v[0]));
   } else {
+    // Collision_ESMC_new4 draws only in cells with two or more particles
+    // (collisions.cpp:763)
+    if (model == COLL_ESMC4 && cnt[c] < 2) {
+      mcand[c] = 0u;
+      return;
+    }
     m = mcpm_floor(0.5 * omax[c] * (1.0 * (double)cnt[c]) + mcpm_u01(w.v[0]));
   }
   mcand[c] = m > 0.0 ? (uint32_t)m : 0u;
@@ -589,11 +597,15 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__
 // at y_i + sigma dir_y (collisions.cpp:648-671; 1D rule of new4,
 // collisions.cpp:784-792), j in J, j != i (collisions.cpp:675-679).
 
+// clamp (Collision_ESMC_new4, collisions.cpp:784-792): a partner point
+// beyond a wall takes the wall's cell instead of leaving the candidate
+// without a partner.
 template <typename R>
 __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *P,
                                                const uint32_t *cnt, const uint32_t *start,
                                                const KParams &kp, uint32_t step_lo,
-                                               uint32_t step_hi, uint32_t *oi, uint32_t *oj) {
+                                               uint32_t step_hi, uint32_t *oi, uint32_t *oj,
+                                               bool clamp = false) {
   mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
                              kp.key1);
   uint32_t Nc = cnt[c];
@@ -602,7 +614,12 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
   int32_t J;
-  if (!partner_cell(P[i], d0, d1, kp, &J)) return false;
+  if (clamp) {
+    if constexpr (R::dims == 1) J = partner_cell_clamped(P[i], d1, kp);
+    else return false;
+  } else if (!partner_cell(P[i], d0, d1, kp, &J)) {
+    return false;
+  }
   uint32_t NJ = cnt[J];
   if (NJ == 0) return false;
   uint32_t j = start[J] + mcpm_uint_below(w.v[3], NJ);
@@ -696,21 +713,45 @@ __device__ __forceinline__ uint32_t cand_priority_rep(const KParams &kp, int32_t
 }
 
 // Size of every (band, region) list per chunk of candidates.
-__global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_key, int esmc,
+// First cell of the candidates of chunk lo: the last c with cstart[c] <= lo.
+__device__ __forceinline__ uint32_t first_cell_of(uint32_t lo, const uint32_t *cstart, int nc) {
+  int a = 0, b = nc;  // cstart[a] <= lo < cstart[b] (cstart[nc] = M > lo)
+  while (b - a > 1) {
+    const int mid = (a + b) >> 1;
+    if (cstart[mid] <= lo) a = mid;
+    else b = mid;
+  }
+  return (uint32_t)a;
+}
+
+__global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_key, int model,
                                                     uint32_t band_mul, int nbands,
-                                                    uint32_t *__restrict__ table, KParams kp) {
+                                                    uint32_t *__restrict__ table, KParams kp,
+                                                    const uint32_t *__restrict__ cstart,
+                                                    int nc) {
   __shared__ uint32_t scount[NLIST];
+  __shared__ uint32_t scell;
   const int nl = nbands * NREG;
   if ((int)threadIdx.x < NLIST) scount[threadIdx.x] = 0;
-  __syncthreads();
   const uint32_t lo = blockIdx.x * GEN_CHUNK;
+  if (model == COLL_ESMC4 && threadIdx.x == 0) scell = first_cell_of(lo, cstart, nc);
+  __syncthreads();
+  const int esmc = model != COLL_DSMC;
   const uint32_t x0 = op_region(lo, M);
+  uint32_t c = model == COLL_ESMC4 ? scell : 0u;
   for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
     const uint32_t k = lo + b0 + threadIdx.x;
     if (k >= M) break;
-    const uint32_t pr = kp.nrep ? cand_priority_rep(kp, rep_of_cand(kp, k), k, esmc)
-                                : cand_priority(k, prio_key, esmc);
-    const uint32_t q = op_band(pr, band_mul, nbands) * NREG + region_from(k, x0, M);
+    uint32_t band;
+    if (model == COLL_ESMC4) {
+      while (cstart[c + 1] <= k) ++c;
+      band = band_new4(rank_new4(c, k - cstart[c], cstart, M), M, nbands);
+    } else {
+      const uint32_t pr = kp.nrep ? cand_priority_rep(kp, rep_of_cand(kp, k), k, esmc)
+                                  : cand_priority(k, prio_key, esmc);
+      band = op_band(pr, band_mul, nbands);
+    }
+    const uint32_t q = band * NREG + region_from(k, x0, M);
     atomicAdd(&scount[q], 1u);
   }
   __syncthreads();
@@ -795,16 +836,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
   if ((int)threadIdx.x < nl)  // list start + this chunk's offset inside the list
     spos[threadIdx.x] = counters[CNT_BSTART + threadIdx.x] +
                         table[(size_t)blockIdx.x * NLIST + threadIdx.x];
-  if (threadIdx.x == 0) {
-    // first cell of the chunk: the last c with cstart[c] <= lo
-    int a = 0, b = nc;  // cstart[a] <= lo < cstart[b] (cstart[nc] = M > lo)
-    while (b - a > 1) {
-      const int mid = (a + b) >> 1;
-      if (cstart[mid] <= lo) a = mid;
-      else b = mid;
-    }
-    scell[0] = (uint32_t)a;
-  }
+  if (threadIdx.x == 0) scell[0] = first_cell_of(lo, cstart, nc);
   __syncthreads();
   uint32_t c = scell[0];
   const uint32_t x0 = op_region(lo, M);
@@ -825,14 +857,21 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
           draw_dsmc_pair(c, k, cnt, start, kp, step_lo, step_hi, &i, &j);
           ok = true;
         } else if (kp.nrep == 0) {
-          ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+          ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j,
+                                 MODEL == COLL_ESMC4 && kp.boundary != MCPM_BC_PERIODIC);
         } else {
           ok = draw_candidate_rep<R>(c, t, rq, P, cnt, start, kp, step_lo, step_hi, &i, &j);
         }
       }
-      pr = kp.nrep ? cand_priority_rep(kp, rq, k, MODEL == COLL_ESMC)
-                   : cand_priority(k, prio_key, MODEL == COLL_ESMC);
-      const uint32_t band = op_band(pr, band_mul, nbands);
+      uint32_t band;
+      if (MODEL == COLL_ESMC4) {
+        pr = rank_new4(c, t, cstart, M);
+        band = band_new4(pr, M, nbands);
+      } else {
+        pr = kp.nrep ? cand_priority_rep(kp, rq, k, MODEL == COLL_ESMC)
+                     : cand_priority(k, prio_key, MODEL == COLL_ESMC);
+        band = op_band(pr, band_mul, nbands);
+      }
       q = band * NREG + region_from(k, x0, M);
       if (ok) {
         ++nvalid;
@@ -865,7 +904,7 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R 
                                           const double *nden, const double *yc,
                                           const double *omax, double *omega_new,
                                           const KParams &kp, uint32_t step_lo,
-                                          uint32_t step_hi) {
+                                          uint32_t step_hi, bool new4 = false) {
   using T = typename R::real_type;
   // a batched replica: its geometry and key, cells counted from its first
   const int32_t rq = kp.nrep ? rep_of_particle(kp, (int64_t)i) : 0;
@@ -888,7 +927,8 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R 
   double gd = g0 * d0 + g1 * d1 + g2 * d2;
   double pv = kp.coll_pref * gd * Ymid * nden[J] * kp.dt;
   atomic_max_pos_double(&omega_new[c], pv);
-  if (pv / omax[c] > mcpm_u01(w2.v[0])) {
+  // new4 also requires an approaching pair, g.dir > 0 (collisions.cpp:819)
+  if (pv / omax[c] > mcpm_u01(w2.v[0]) && (!new4 || gd > 0.0)) {
     a.u1 = (T)((double)a.u1 - gd * d0);
     a.u2 = (T)((double)a.u2 - gd * d1);
     a.u3 = (T)((double)a.u3 - gd * d2);
@@ -1081,7 +1121,7 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
                        ? commit_dsmc<R>(i, j, pr, rec[u].w, pa[u], pb[u], P, omax, omega_new,
                                         kp, fpp, step_lo, step_hi)
                        : commit_op<R>(i, j, rec[u].w, pa[u], pb[u], P, nden, yc, omax,
-                                      omega_new, kp, step_lo, step_hi);
+                                      omega_new, kp, step_lo, step_hi, MODEL == COLL_ESMC4);
         acc_local += acc ? 1u : 0u;
         if (log.p) {
           uint32_t q = atomicAdd(&counters[CNT_LOG], 1u);
@@ -1227,6 +1267,10 @@ __global__ void k_rep_step(KParams kp, const uint32_t *__restrict__ cstart,
 __global__ void k_zero_u32(uint32_t *p, int n) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0u;
+}
+__global__ void k_iota_u32(uint32_t *p, int64_t n) {
+  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; k < n; k += (int64_t)gridDim.x * blockDim.x) p[k] = (uint32_t)k;
 }
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v) {
   int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1609,13 +1653,16 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
                                             Kick, const double *, int, uint32_t, uint32_t, \
                                             uint32_t *, const uint32_t *, uint32_t *, int, \
-                                            int, int32_t, uint32_t *, const uint32_t *);   \
+                                            int, int32_t, uint32_t *, const uint32_t *,    \
+                                            const uint32_t *, uint32_t *);                 \
   template __global__ void k_push<R, true>(const R *, R *, int64_t, int32_t, KParams,      \
                                            Kick, const double *, int, uint32_t, uint32_t,  \
                                            uint32_t *, const uint32_t *, uint32_t *, int,  \
-                                           int, int32_t, uint32_t *, const uint32_t *);    \
+                                           int, int32_t, uint32_t *, const uint32_t *,     \
+                                           const uint32_t *, uint32_t *);                  \
   MCPM_INST_COLL(R, COLL_ESMC)                                                             \
   MCPM_INST_COLL(R, COLL_DSMC)                                                             \
+  MCPM_INST_COLL(R, COLL_ESMC4)                                                            \
   template __global__ void k_init<R>(R *, InitParams, KParams);                           \
   template __global__ void k_cba_hs<R>(R *, const uint32_t *, const uint32_t *,            \
                                        const double *, const double *, KParams, uint32_t,  \

@@ -35,6 +35,7 @@ Y_REFERENCE, Y_CARNAHAN_STARLING = 0, 1
 LR_NONE, LR_DIRECT, LR_DENSITY_EXPANSION, LR_SCREENED_POISSON = 0, 1, 2, 3
 MODEL_ESMC, MODEL_FP_DENSE, MODEL_FP_IDEAL, MODEL_DSMC_VHS, MODEL_NONE = 0, 1, 2, 3, 4
 MODEL_DSMC_HS, MODEL_CBA_VHS, MODEL_CBA_HS, MODEL_FP_LINEAR = 5, 6, 7, 8
+MODEL_ESMC_NEW4 = 9
 NMOM, NSAMP, NCLOS = 34, 26, 28
 
 STATUS = {
@@ -258,6 +259,9 @@ def load_library() -> ctypes.CDLL:
         "mcpm_particle_count": (i64, [vp]),
         "mcpm_cell_update": (i32, [vp]),
         "mcpm_collide_esmc": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_collide_esmc_new4": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_track_ids": (i32, [vp, i32]),
+        "mcpm_download_ids": (i32, [vp, ctypes.POINTER(i32)]),
         "mcpm_poisson_solver": (i32, [vp]),
         "mcpm_kick": (i32, [vp]),
         "mcpm_stream_bc": (i32, [vp, i64, i32]),
@@ -490,6 +494,21 @@ class Simulation:
     def collide_esmc(self, step: int) -> dict:
         s = CStats()
         self._check(self._lib.mcpm_collide_esmc(self._h, step, ctypes.byref(s)), "mcpm_collide_esmc")
+        return s.as_dict()
+
+    def track_ids(self, enable: bool = True):
+        self._check(self._lib.mcpm_track_ids(self._h, 1 if enable else 0), "mcpm_track_ids")
+
+    def ids(self) -> np.ndarray:
+        """Upload slot of the particle at each position of download()."""
+        out = np.empty(int(self._lib.mcpm_particle_count(self._h)), dtype=np.int32)
+        self._check(self._lib.mcpm_download_ids(self._h, _i32p(out)), "mcpm_download_ids")
+        return out
+
+    def collide_esmc_new4(self, step: int) -> dict:
+        s = CStats()
+        self._check(self._lib.mcpm_collide_esmc_new4(self._h, step, ctypes.byref(s)),
+                    "mcpm_collide_esmc_new4")
         return s.as_dict()
 
     def poisson_solver(self):

@@ -532,8 +532,14 @@ struct Cand {
 };
 
 // Collision_ESMC_new3 (collisions.cpp:583-720), candidates in priority order.
+// With new4, Collision_ESMC_new4 (collisions.cpp:722-837) instead: cells
+// from the last to the first, each cell's candidates in turn (:758), only
+// cells of two or more particles draw (:763-765), the partner cell
+// floor((y_i + sigma dir_y) / dy) clamped into the slab (:781-792; wrapped on
+// a periodic slab), acceptance only with g.dir > 0 (:819).  Its log carries
+// each candidate's rank in that order as the priority.
 template <typename real>
-void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
+void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st, bool new4 = false) {
   const mcpm_params &p = o.p;
   const int nc = o.nc;
   uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
@@ -545,6 +551,7 @@ void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
     mcpm_u32x4 w = mcpm_philox((uint32_t)c, 0u, slo, mcpm_ctr3(MCPM_TAG_MCAND, shi, 0u), key0, key1);
     double m = mcpm_floor(0.5 * o.omax[c] * (1.0 * (double)o.cnt[c]) + mcpm_u01(w.v[0]));
     mc[c] = m > 0.0 ? (uint32_t)m : 0u;
+    if (new4 && o.cnt[c] < 2) mc[c] = 0u;
     cs[c] = (uint32_t)M;
     M += mc[c];
   }
@@ -562,11 +569,16 @@ void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
       double d0, d1, d2;
       mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
       int J;  // cell at r_i + sigma dir (collisions.cpp:658-671)
-      if (!orc_partner(o, i, d0, d1, &J)) continue;
+      if (new4 && p.boundary != MCPM_BC_PERIODIC) {
+        double q = mcpm_floor((o.rec[(size_t)o.S * i] + p.sigma * d1) / o.dy);
+        J = q < 0.0 ? 0 : (q > (double)(nc - 1) ? nc - 1 : (int)q);
+      } else if (!orc_partner(o, i, d0, d1, &J)) {
+        continue;
+      }
       if (o.cnt[J] == 0) continue;
       uint32_t j = (uint32_t)o.start[J] + mcpm_uint_below(w.v[3], (uint32_t)o.cnt[J]);
       if (i == j) continue;  // collisions.cpp:677-678
-      uint32_t pr = mcpm_priority(cs[c] + t, pkey);
+      uint32_t pr = new4 ? ((uint32_t)M - cs[c + 1]) + t : mcpm_priority(cs[c] + t, pkey);
       order.push_back(((uint64_t)pr << 32) | (uint64_t)cand.size());
       cand.push_back(Cand{(uint32_t)c, t, i, j});
     }
@@ -595,6 +607,7 @@ void orc_collide(Oracle &o, int64_t step, mcpm_collision_stats *st) {
     double pv = pref * gd * Ymid * o.nden[J] * p.delta_t;  // collisions.cpp:691
     if (pv - omega[cd.c] > 0.0) omega[cd.c] = pv;          // :692-693
     bool a = pv / o.omax[cd.c] > mcpm_u01(w2.v[0]);        // :694
+    if (new4 && !(gd > 0.0)) a = false;                      // :819
     if (a) {
       ri[1] = rnd<real>(ri[1] - gd * d0);
       ri[2] = rnd<real>(ri[2] - gd * d1);
@@ -2258,6 +2271,15 @@ int orc_collide_esmc(void *h, int64_t step, mcpm_collision_stats *st) {
   return MCPM_OK;
 }
 
+int orc_collide_esmc_new4(void *h, int64_t step, mcpm_collision_stats *st) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.sorted) return MCPM_ERR_STATE;
+  if (o.dims == 2) return MCPM_ERR_ARG;
+  if (o.p.precision == MCPM_FP32) orc_collide<float>(o, step, st, true);
+  else orc_collide<double>(o, step, st, true);
+  return MCPM_OK;
+}
+
 int orc_poisson_solver(void *h) {
   Oracle &o = *(Oracle *)h;
   if (!o.sorted) return MCPM_ERR_STATE;
@@ -2387,6 +2409,7 @@ int orc_step(void *h, int64_t step, mcpm_collision_stats *st) {
   if (st) memset(st, 0, sizeof *st);
   switch (o.p.model) {
     case MCPM_MODEL_ESMC: r = orc_collide_esmc(h, step, st); break;
+    case MCPM_MODEL_ESMC_NEW4: r = orc_collide_esmc_new4(h, step, st); break;
     case MCPM_MODEL_FP_DENSE:
     case MCPM_MODEL_FP_IDEAL:
     case MCPM_MODEL_FP_LINEAR:

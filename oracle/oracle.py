@@ -73,6 +73,7 @@ def load() -> ctypes.CDLL:
         "orc_collide_cba_vhs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_collide_cba_hs": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_collide_esmc_mt": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "orc_collide_esmc_new4": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "orc_initialize": (i32, [vp, ctypes.POINTER(CSettings)]),
         "orc_vlasov_integral": (i32, [vp]),
         "orc_density_expansion": (i32, [vp]),
@@ -260,6 +261,9 @@ class OracleSim:
     def initialize(self, settings):
         self._check(self._lib.orc_initialize(self._h, ctypes.byref(settings)), "orc_initialize")
         self._n = settings.n_particles
+
+    def collide_esmc_new4(self, step):
+        return self._collide("orc_collide_esmc_new4", step)
 
     def collide_esmc_mt(self, step):
         """Collision_ESMC_new3 with std::mt19937 and the reference's own candidate
