@@ -521,10 +521,20 @@ int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes, mcpm_acces
    MCPM_TUNE_BANDS       priority bands of the ESMC reservation rounds, 1..16
    MCPM_TUNE_ROUND_GRID  most workgroups of one reservation round (multiple of 8)
    MCPM_TUNE_SORT_CHUNK  particles per sort chunk (multiple of 256, 1024..32768;
-                         0 = about 2048 chunks) */
+                         0 = about 2048 chunks)
+   MCPM_TUNE_SORT_WINDOW 1 (default): on slabs whose particles moved a few
+                         cells, sort by per-chunk windowed histograms (1024
+                         particles per chunk, a few cells each), redone by the
+                         full histogram when a particle left its window; 0: the
+                         full histogram always
+   MCPM_TUNE_SORT_WINDOW_RUNS, MCPM_TUNE_SORT_WINDOW_REDONE (read only):
+                         windowed sorts run and redone since creation */
 #define MCPM_TUNE_BANDS      0
 #define MCPM_TUNE_ROUND_GRID 1
 #define MCPM_TUNE_SORT_CHUNK 2
+#define MCPM_TUNE_SORT_WINDOW 3
+#define MCPM_TUNE_SORT_WINDOW_RUNS 4
+#define MCPM_TUNE_SORT_WINDOW_REDONE 5
 int mcpm_set_tuning(mcpm_ctx *ctx, int32_t knob, int64_t value);
 int mcpm_get_tuning(mcpm_ctx *ctx, int32_t knob, int64_t *value);
 

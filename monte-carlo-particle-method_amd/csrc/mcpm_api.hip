@@ -44,6 +44,21 @@ struct mcpm_ctx {
   size_t rec = 0;
   void *P[2] = {nullptr, nullptr};
   int cur = 0;
+  // windowed sort (particles a few cells from where they were): chunk bounds
+  // and replicas (host-built when the particle counts change), windows, table
+  // (chunks from WSPAN-cell blocks of the previous sort's cells, built on the
+  // device per sort; the blocks are fixed by the mesh)
+  uint32_t *wbeg = nullptr, *rchunk = nullptr, *wpieces = nullptr, *wpbase = nullptr;
+  int32_t *wrep = nullptr, *wlo = nullptr, *wlen = nullptr;
+  int32_t *wbcell = nullptr, *wbrep = nullptr, *wrblk = nullptr;
+  uint32_t *wtab = nullptr, *wflag = nullptr;  // wflag[0] overflow, wflag[1] chunk count
+  int64_t wcap = 0;        // chunks the arrays hold
+  int nwblk = 0;           // cell blocks
+  bool order_sorted = false;  // the particles are in the cell order of their positions
+  int w_fallbacks = 0;     // windowed sorts redone by the full histogram in a row
+  bool w_enabled = true;
+  int wdelta = WDELTA;     // slack cells of the windows
+  int64_t w_runs = 0, w_redone = 0;
   // particle ids carried through every sort when tracking is on: ids[cur][k]
   // is the upload slot of the particle at position k
   uint32_t *ids[2] = {nullptr, nullptr};
@@ -506,6 +521,68 @@ int chunk_for(int64_t n, int fixed) {
   return (int)ch;
 }
 
+// Whether the windowed sort applies: a slab (or batched slabs) whose
+// particles are still in the cell order of their positions, moved by a push
+// that adds nothing per step (no pressure face sums, no re-insertion), dense
+// enough that a chunk spans a few cells.
+bool window_ok(const mcpm_ctx *c, int mode) {
+  if (!c->w_enabled || c->p.dims == 2 || !c->wbeg || !c->order_sorted || c->w_fallbacks >= 3)
+    return false;
+  if (c->p.boundary == MCPM_BC_REINJECT) return false;
+  if (c->pressure_on && (mode & PUSH_MOVE)) return false;
+  if (c->p.model == MCPM_MODEL_CBA_VHS || c->p.model == MCPM_MODEL_CBA_HS) return false;
+  return c->n >= 16 * (int64_t)c->nc;
+}
+
+// 1: a particle left its chunk's window (sort again by the full histogram).
+template <typename R>
+int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const double *drift) {
+  const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  const R *in = (const R *)c->P[c->cur];
+  R *out = (R *)c->P[1 - c->cur];
+  const uint32_t *iid = c->track_ids ? c->ids[c->cur] : nullptr;
+  uint32_t *oid = c->track_ids ? c->ids[1 - c->cur] : nullptr;
+  const int nblk = c->nwblk;
+  const int nrep1 = std::max(c->kp.nrep, 1) + 1;
+  uint32_t *nchp = c->wflag + 1;
+  HIPCHK(hipMemsetAsync(c->wflag, 0, sizeof(uint32_t), c->s));
+  // the chunks: pieces of the cell blocks of the previous sort (start[])
+  hipLaunchKernelGGL(k_wblocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, c->s,
+                     c->wbcell, nblk, c->start, c->wpieces);
+  hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->wpieces, c->wpbase, nblk);
+  hipLaunchKernelGGL(k_wfill, dim3((unsigned)(nblk + (nrep1 + 255) / 256)), dim3(256), 0, c->s,
+                     c->wbcell, c->wbrep, nblk, c->start, c->wpbase, c->wrblk, nrep1,
+                     (uint32_t)c->n, c->wbeg, c->wrep, nchp, c->rchunk);
+  // the most chunks there can be; waves past the count stop at once
+  const unsigned gw = (unsigned)((c->wcap + 3) / 4);
+  hipLaunchKernelGGL((k_wpush<R, false>), dim3(gw), dim3(256), 0, c->s, in, nullptr, c->kp, kick,
+                     drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
+                     nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta);
+  hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 3) / 4)), dim3(256), 0, c->s, c->wlo,
+                     c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc);
+  hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->cnt, c->start, c->nc);
+  hipLaunchKernelGGL((k_wpush<R, true>), dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
+                     drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
+                     c->start, c->err, c->wflag, iid, oid, c->wdelta);
+  LAUNCHCHK();
+  uint32_t flag = 0;
+  HIPCHK(hipMemcpyAsync(&flag, c->wflag, sizeof flag, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  ++c->w_runs;
+  if (flag) {
+    // wider windows for the next sorts, up to the table's width
+    ++c->w_redone;
+    if (c->wdelta < WDELTA_MAX) c->wdelta = std::min(2 * c->wdelta, (int)WDELTA_MAX);
+    else ++c->w_fallbacks;
+    return 1;
+  }
+  c->w_fallbacks = 0;
+  c->cur = 1 - c->cur;
+  c->sorted = true;
+  c->order_sorted = true;
+  return MCPM_OK;
+}
+
 template <typename R>
 int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   c->keys_current = false;
@@ -531,6 +608,12 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                          (mode & PUSH_MOVE))
                             ? c->drift
                             : nullptr;
+  if constexpr (R::dims == 1) {
+    if (window_ok(c, mode)) {
+      const int w = run_wsort_t<R>(c, mode, step, kick, drift);
+      if (w <= 0) return w;  // done, or an error; 1: the full histogram below
+    }
+  }
   // one stable counting-sort pass over nb bins: histogram, scans, scatter
   auto pass = [&](const R *in, R *out, int pmode, const Kick &ks, int nb, int kmode,
                   uint32_t *cnt, uint32_t *start, double *nden, double *yc, uint32_t *okey,
@@ -581,11 +664,13 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                        c->cnt, c->nden, c->yc, c->kp);
     LAUNCHCHK();
     c->sorted = true;
+    c->order_sorted = true;
     c->keys_current = true;
     return MCPM_OK;
   }
   c->cur = 1 - c->cur;
   c->sorted = true;
+  c->order_sorted = true;
   return MCPM_OK;
 }
 
@@ -1029,6 +1114,7 @@ int launch_init(mcpm_ctx *c, const InitParams &ip) {
                        (R *)c->P[c->cur], ip, c->kp);
   if (c->track_ids && ip.n > 0)
     hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], ip.n);
+  c->order_sorted = false;
   LAUNCHCHK();
   return MCPM_OK;
 }
@@ -1202,6 +1288,38 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
     if (c->nchunks_cap < 2100) c->nchunks_cap = 2100;
   }
   if ((st = dalloc(&c->table, (size_t)c->nchunks_cap * nbins))) return fail(st);
+  // windowed sort (slabs): blocks of WSPAN cells of one replica, and at most
+  // one partial chunk per block beyond capacity / WCH
+  if (p->dims != 2) {
+    std::vector<int32_t> bcell, brep, rblk;
+    for (int r = 0; r < std::max(nrep, 1); ++r) {
+      const int32_t a = nrep ? c->rcell_host[r] : 0, z = nrep ? c->rcell_host[r + 1] : nc;
+      rblk.push_back((int32_t)brep.size());
+      for (int32_t k = a; k < z; k += WSPAN) {
+        bcell.push_back(k);
+        brep.push_back(r);
+      }
+    }
+    rblk.push_back((int32_t)brep.size());
+    bcell.push_back(nc);
+    c->nwblk = (int)brep.size();
+    c->wcap = (p->capacity + WCH - 1) / WCH + c->nwblk + 1;
+    if ((st = dalloc(&c->wbeg, (size_t)c->wcap + 1))) return fail(st);
+    if ((st = dalloc(&c->wrep, (size_t)c->wcap))) return fail(st);
+    if ((st = dalloc(&c->wlo, (size_t)c->wcap))) return fail(st);
+    if ((st = dalloc(&c->wlen, (size_t)c->wcap))) return fail(st);
+    if ((st = dalloc(&c->wtab, (size_t)c->wcap * WMAX))) return fail(st);
+    if ((st = dalloc(&c->rchunk, rblk.size()))) return fail(st);
+    if ((st = dalloc(&c->wpieces, (size_t)c->nwblk + 1))) return fail(st);
+    if ((st = dalloc(&c->wpbase, (size_t)c->nwblk + 1))) return fail(st);
+    if ((st = dalloc(&c->wbcell, bcell.size()))) return fail(st);
+    if ((st = dalloc(&c->wbrep, brep.size() + 1))) return fail(st);
+    if ((st = dalloc(&c->wrblk, rblk.size()))) return fail(st);
+    if ((st = dalloc(&c->wflag, 2))) return fail(st);
+    HIPCHK(hipMemcpy(c->wbcell, bcell.data(), bcell.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->wbrep, brep.data(), brep.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->wrblk, rblk.data(), rblk.size() * 4, hipMemcpyHostToDevice));
+  }
   if ((st = dalloc(&c->partial, (size_t)((c->nchunks_cap + c->seg - 1) / c->seg + 1) * nbins)))
     return fail(st);
   if ((st = dalloc(&c->cnt, nc))) return fail(st);
@@ -1311,7 +1429,9 @@ int mcpm_destroy(mcpm_ctx *c) {
                   c->samples, c->press, c->counters, c->err, c->esc, c->energy_part, c->res, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
                   c->fem_buf, c->q4, c->fft_buf, c->fft_twx, c->fft_twy,
-                  c->fft_cosx, c->fft_cosy, c->ids[0]};
+                  c->fft_cosx, c->fft_cosy, c->ids[0], c->wbeg, c->wrep, c->wlo,
+                  c->wlen, c->rchunk, c->wtab, c->wflag, c->wpieces, c->wpbase, c->wbcell,
+                  c->wbrep, c->wrblk};
   for (void *b : bufs)
     if (b) hipFree(b);
   for (int k = 0; k < T_NCLASS; ++k)
@@ -1368,6 +1488,7 @@ int upload_any(mcpm_ctx *c, int64_t n, const double *const *src, int nsrc) {
   c->n = n;
   c->sorted = false;
   c->kick_pending = false;
+  c->order_sorted = false;
   if (c->track_ids && n > 0) {
     hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], n);
     LAUNCHCHK();
@@ -1984,6 +2105,11 @@ int mcpm_set_tuning(mcpm_ctx *c, int32_t knob, int64_t v) {
       c->sort_chunk = (int)v;
       return MCPM_OK;
     }
+    case MCPM_TUNE_SORT_WINDOW:
+      if (v != 0 && v != 1) return MCPM_ERR_ARG;
+      c->w_enabled = v == 1;
+      c->w_fallbacks = 0;
+      return MCPM_OK;
     default: return MCPM_ERR_ARG;
   }
 }
@@ -1994,6 +2120,9 @@ int mcpm_get_tuning(mcpm_ctx *c, int32_t knob, int64_t *v) {
     case MCPM_TUNE_BANDS: *v = c->nbands; return MCPM_OK;
     case MCPM_TUNE_ROUND_GRID: *v = c->round_grid; return MCPM_OK;
     case MCPM_TUNE_SORT_CHUNK: *v = c->sort_chunk; return MCPM_OK;
+    case MCPM_TUNE_SORT_WINDOW: *v = c->w_enabled ? 1 : 0; return MCPM_OK;
+    case MCPM_TUNE_SORT_WINDOW_RUNS: *v = c->w_runs; return MCPM_OK;
+    case MCPM_TUNE_SORT_WINDOW_REDONE: *v = c->w_redone; return MCPM_OK;
     default: return MCPM_ERR_ARG;
   }
 }

@@ -69,6 +69,32 @@ __global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp
                        const uint32_t *start, uint32_t *err, int nb, int kmode, int32_t kdiv,
                        uint32_t *okey, const uint32_t *ikey, const uint32_t *iid,
                        uint32_t *oid);
+// The windowed sort of particles that move a few cells per step: WCH
+// particles per chunk, histograms over at most WMAX cells, WDELTA cells of
+// slack either side of the chunk's old cells.
+// A chunk covers at most WSPAN old cells; its window adds wdelta cells each
+// side, WDELTA at first and up to WDELTA_MAX after particles left it.
+constexpr int WCH = 1024;
+constexpr int WMAX = 128;
+constexpr int WDELTA = 4;
+constexpr int WSPAN = 32;
+constexpr int WDELTA_MAX = (WMAX - WSPAN) / 2;
+static_assert(WSPAN + 2 * WDELTA_MAX <= WMAX, "window of a chunk");
+template <typename R, bool SCATTER>
+__global__ void k_wpush(const R *in, R *out, KParams kp, Kick kick, const double *drift, int mode,
+                        uint32_t step_lo, uint32_t step_hi, const uint32_t *wbeg,
+                        const int32_t *wrep, const uint32_t *nchp, int32_t *wlo, int32_t *wlen,
+                        uint32_t *wtab, const uint32_t *start, uint32_t *err, uint32_t *wflag,
+                        const uint32_t *iid, uint32_t *oid, int wdelta);
+__global__ void k_wblocks(const int32_t *bcell, int nblk, const uint32_t *start,
+                          uint32_t *pieces);
+__global__ void k_wfill(const int32_t *bcell, const int32_t *brep, int nblk,
+                        const uint32_t *start, const uint32_t *pbase, const int32_t *rblk,
+                        int nrep1, uint32_t n, uint32_t *wbeg, int32_t *wrep, uint32_t *nchp,
+                        uint32_t *rchunk);
+__global__ void k_wscan(const int32_t *wlo, const int32_t *wlen, uint32_t *wtab,
+                        const uint32_t *nchp, const uint32_t *rchunk, KParams kp, uint32_t *cnt,
+                        double *nden, double *yc);
 // Most bins of one sort pass (one LDS histogram of 64 KB per chunk); more
 // cells sort by two digits of the cell key, up to MAX_CELLS.
 constexpr int SORT_BINS = 16384;

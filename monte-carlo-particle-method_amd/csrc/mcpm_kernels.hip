@@ -404,9 +404,13 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
             __builtin_amdgcn_wave_barrier();
             if (valid) {
               const uint32_t dst = base + rk;
-              out[dst] = r;
-              if (okey) okey[dst] = okv;
-              if (oid) oid[dst] = iid[p];
+              if (dst < (uint32_t)n) {
+                out[dst] = r;
+                if (okey) okey[dst] = okv;
+                if (oid) oid[dst] = iid[p];
+              } else {
+                atomicOr(err, ERRBIT_ORDER);  // the two passes disagreed on a key
+              }
               if (rk + 1 == tot) lds_cnt[key] = base + tot;
             }
             __builtin_amdgcn_wave_barrier();
@@ -438,6 +442,301 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
   if (!SCATTER) {
     __syncthreads();
     for (int c = threadIdx.x; c < nc; c += TILE) trow[c] = lds_cnt[c];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The same stable counting sort when particles move only a few cells per
+// step (the slab and the batched slabs): chunk b of consecutive particles
+// (WCH of them, never two replicas in one chunk) was sorted by cell in the
+// previous step, so its particles' new cells lie within WDELTA cells of the
+// old cells of its first and last particle.  Its histogram is kept only over
+// that window of at most WMAX cells, lo = old cell of the first particle -
+// WDELTA (in the replica's own, unwrapped cell numbers: a periodic slab's
+// cells -1, -2, ... are its last ones), so the table is chunks x WMAX instead
+// of chunks x cells and a chunk can be small: one wavefront per chunk of
+// 1024 particles instead of one per several thousand.  A particle outside
+// its chunk's window (a long flight, a wide chunk, input out of cell order)
+// sets *wflag and is neither counted nor stored; the caller then sorts again
+// by the full histogram, from the same input.
+//   SCATTER=false: window and histogram of each chunk -> wlo, wlen, wtab
+//   SCATTER=true : wtab holds each window cell's offset among the chunks
+//                  (k_wscan), start[] the cells' starts; each particle lands
+//                  at start + offset + its stable rank in the chunk
+// ---------------------------------------------------------------------------
+// Window offset of a particle of new local cell key in a window [lo, lo + len)
+// of a slab of nc cells, -1 when outside.
+__device__ __forceinline__ int32_t window_offset(int32_t key, int32_t lo, int32_t len, int32_t nc,
+                                                 bool periodic) {
+  int32_t d = key - lo;
+  if (periodic) {
+    if (d < 0) d += nc;
+    else if (d >= len) d -= nc;
+  }
+  return (d >= 0 && d < len) ? d : -1;
+}
+
+template <typename R, bool SCATTER>
+__global__ __launch_bounds__(256) void k_wpush(
+    const R *__restrict__ in, R *__restrict__ out, KParams kp, Kick kick,
+    const double *__restrict__ drift, int mode, uint32_t step_lo, uint32_t step_hi,
+    const uint32_t *__restrict__ wbeg, const int32_t *__restrict__ wrep,
+    const uint32_t *__restrict__ nchp, int32_t *__restrict__ wlo, int32_t *__restrict__ wlen,
+    uint32_t *__restrict__ wtab, const uint32_t *__restrict__ start, uint32_t *__restrict__ err,
+    uint32_t *__restrict__ wflag, const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid,
+    int wdelta) {
+  __shared__ uint32_t lcnt[4][WMAX];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nch = (int)*nchp;
+  const int b = blockIdx.x * 4 + wave;  // one chunk per wavefront, no workgroup barrier
+  if (b >= nch) return;
+  uint32_t *cnt = lcnt[wave];
+  const uint32_t beg = wbeg[b], end = wbeg[b + 1], ntot = wbeg[nch];
+  // the chunk's replica: its constants, force, cell offset and slot base
+  const int32_t q = kp.nrep ? wrep[b] : 0;
+  const KParams &kq = kp.nrep ? kp.rkp[q] : kp;
+  const int32_t off = kp.nrep ? kp.rcell_off[q] : 0;
+  const int64_t slot0 = kp.nrep ? kp.rpart_off[q] : 0;
+  Kick kk = kick;
+  if (kp.nrep) kk.F2 = kick.F2 + off;
+  const int32_t nc = kq.n_cells;
+  const bool periodic = kq.boundary == MCPM_BC_PERIODIC;
+  int32_t lo, len;
+  if (SCATTER) {
+    lo = wlo[b];
+    len = wlen[b];
+  } else {
+    const int32_t kf = beg < end ? cell_key(in[beg], kq) : 0;
+    const int32_t kl = beg < end ? cell_key(in[end - 1], kq) : 0;
+    lo = kf - wdelta;
+    len = kl - kf + 2 * wdelta + 1;
+    if (len > WMAX || len < 1 || (periodic && len > nc)) {
+      if (lane == 0) atomicOr(wflag, 1u);
+      len = 0;
+    }
+    if (lane == 0) {
+      wlo[b] = lo;
+      wlen[b] = len;
+    }
+  }
+  if (len == 0) return;
+  // every particle counted once: otherwise a chunk was missed (input out of
+  // cell order) and the offsets are not a permutation
+  if (SCATTER && b == 0 && lane == 0 && start[kp.n_cells] != ntot) atomicOr(wflag, 1u);
+  for (int w = lane; w < len; w += 64) {
+    if (SCATTER) {
+      int32_t c = lo + w;
+      c = ((c % nc) + nc) % nc;  // cells beyond a wall's end are empty windows
+      cnt[w] = start[off + c] + wtab[(size_t)b * WMAX + w];
+    } else {
+      cnt[w] = 0u;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
+  const uint64_t lt = lanemask_lt();
+  constexpr int U = 4;  // tiles of 64 in flight
+  R nx[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const uint32_t p = beg + u * 64 + lane;
+    nx[u] = in[p < end ? p : end - 1];
+  }
+  for (uint32_t t0 = beg; t0 < end; t0 += 64 * U) {
+    R rr[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) rr[u] = nx[u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t p = t0 + 64 * U + u * 64 + lane;
+      nx[u] = in[p < end ? p : end - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t p = t0 + u * 64 + lane;
+      bool valid = p < end;
+      int32_t d = 0;
+      R r = rr[u];
+      if (valid) {
+        if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
+        if ((mode & PUSH_CHECK) && !in_domain(r, kq)) atomicOr(err, ERRBIT_DOMAIN);
+        r = push_one(r, (uint64_t)((int64_t)p - slot0), kq, kk, drift, mode, step_lo, step_hi,
+                     err);
+        d = window_offset(cell_key(r, kq), lo, len, nc, periodic);
+        if (d < 0) {
+          atomicOr(wflag, 1u);
+          valid = false;
+        }
+      }
+      if (SCATTER) {
+        // stable rank among the lanes with the same window cell, as k_push
+        uint32_t rk = 0, tot = 0;
+        uint64_t active = __ballot(valid);
+        while (active) {
+          const int leader = __ffsll((unsigned long long)active) - 1;
+          const int ld = __builtin_amdgcn_readlane(d, leader);
+          const bool mine = valid && d == ld;
+          const uint64_t m = __ballot(mine);
+          if (mine) {
+            rk = (uint32_t)__popcll(m & lt);
+            tot = (uint32_t)__popcll(m);
+          }
+          active &= ~m;
+        }
+        uint32_t base = 0;
+        if (valid) base = cnt[d];
+        __builtin_amdgcn_wave_barrier();
+        if (valid) {
+          const uint32_t dst = base + rk;
+          if (dst < ntot) {
+            out[dst] = r;
+            if (oid) oid[dst] = iid[p];
+          } else {
+            atomicOr(wflag, 1u);  // counts out of step with the particles
+          }
+          if (rk + 1 == tot) cnt[d] = base + tot;
+        }
+        __builtin_amdgcn_wave_barrier();
+      } else {
+        uint64_t active = __ballot(valid);
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+          if (active) {
+            const int leader = __ffsll((unsigned long long)active) - 1;
+            const int ld = __builtin_amdgcn_readlane(d, leader);
+            const uint64_t m = __ballot(valid && d == ld);
+            if (lane == leader) atomicAdd(&cnt[ld], (uint32_t)__popcll(m));
+            active &= ~m;
+          }
+        }
+        if ((active >> lane) & 1ull) atomicAdd(&cnt[d], 1u);
+      }
+    }
+  }
+  if (!SCATTER) {
+    __builtin_amdgcn_wave_barrier();
+    for (int w = lane; w < len; w += 64) wtab[(size_t)b * WMAX + w] = cnt[w];
+  }
+}
+
+// The chunks of a windowed sort, from the cell bounds of the previous sort:
+// block j of WSPAN consecutive cells of one replica (cells [bcell[j],
+// bcell[j + 1])) holds the particles [start[bcell[j]], start[bcell[j + 1]]),
+// cut into pieces of at most WCH.  A chunk then spans at most WSPAN old cells
+// however sparse the gas.  k_wblocks counts each block's pieces, k_excl_scan
+// numbers them, k_wfill writes the pieces' bounds and replicas, the total
+// (*nch, wbeg[*nch] = n) and the first piece of each replica.
+__global__ void k_wblocks(const int32_t *__restrict__ bcell, int nblk,
+                          const uint32_t *__restrict__ start, uint32_t *__restrict__ pieces) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nblk) return;
+  const uint32_t a = start[bcell[j]], z = start[bcell[j + 1]];
+  pieces[j] = (z - a + WCH - 1) / WCH;
+}
+// One workgroup per block (blockIdx.x < nblk), its threads over the pieces;
+// the workgroups past nblk write the replicas' first pieces.
+__global__ void k_wfill(const int32_t *__restrict__ bcell, const int32_t *__restrict__ brep,
+                        int nblk, const uint32_t *__restrict__ start,
+                        const uint32_t *__restrict__ pbase, const int32_t *__restrict__ rblk,
+                        int nrep1, uint32_t n, uint32_t *__restrict__ wbeg,
+                        int32_t *__restrict__ wrep, uint32_t *__restrict__ nchp,
+                        uint32_t *__restrict__ rchunk) {
+  const int j = blockIdx.x;
+  if (j < nblk) {
+    const uint32_t a = start[bcell[j]], z = start[bcell[j + 1]];
+    const uint32_t k0 = pbase[j], np = pbase[j + 1] - k0;
+    for (uint32_t t = threadIdx.x; t < np; t += blockDim.x) {
+      wbeg[k0 + t] = a + t * WCH;
+      wrep[k0 + t] = brep[j];
+    }
+    (void)z;
+    if (j == 0 && threadIdx.x == 0) {
+      *nchp = pbase[nblk];
+      wbeg[pbase[nblk]] = n;
+    }
+  } else {
+    const int r = (j - nblk) * blockDim.x + threadIdx.x;
+    if (r < nrep1) rchunk[r] = pbase[rblk[r]];
+  }
+}
+
+// First i in [a, z) with pred(i) (pred false then true along i), or z: the
+// wavefront tests 64 evenly spread indices per round, so a range of N takes
+// about log64(N) rounds of one load per lane.
+template <typename Pred>
+__device__ __forceinline__ int wave_lower_bound(int a, int z, Pred pred) {
+  const int lane = (int)lane_id();
+  while (z > a) {
+    const int step = (z - a + 63) / 64;
+    const int i = a + lane * step;
+    const bool t = i >= z || pred(i);
+    const uint64_t m = __ballot(t);
+    if (m == 0) {
+      a = a + 63 * step + 1;
+      continue;
+    }
+    const int f = __ffsll((unsigned long long)m) - 1;
+    if (f == 0) return a;
+    const int hi = a + f * step;
+    a = a + (f - 1) * step + 1;
+    z = hi < z ? hi : z;
+  }
+  return z;
+}
+
+// Per cell (one wavefront each): the chunks whose windows hold it, in
+// chunk order (a contiguous run for each of the cell's unwrapped numbers
+// k - nc, k, k + nc of a periodic slab, in that order), their counts
+// replaced by the running offset; the cell's count, density and contact
+// value.  wlo and wlo + wlen ascend along a replica's chunks when the input
+// was in cell order; out of order the counts are only wrong, never out of
+// bounds, and the chunks' own checks have raised *wflag.
+__global__ __launch_bounds__(256) void k_wscan(const int32_t *__restrict__ wlo,
+                                               const int32_t *__restrict__ wlen,
+                                               uint32_t *__restrict__ wtab,
+                                               const uint32_t *__restrict__ nchp,
+                                               const uint32_t *__restrict__ rchunk, KParams kp,
+                                               uint32_t *__restrict__ cnt,
+                                               double *__restrict__ nden,
+                                               double *__restrict__ yc) {
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (k >= kp.n_cells) return;
+  const int32_t q = kp.nrep ? kp.cell_rep[k] : 0;
+  const KParams &kq = kp.nrep ? kp.rkp[q] : kp;
+  const int32_t off = kp.nrep ? kp.rcell_off[q] : 0;
+  const int32_t nc = kq.n_cells;
+  const int32_t kl = k - off;
+  const int b0 = kp.nrep ? (int)rchunk[q] : 0, b1 = kp.nrep ? (int)rchunk[q + 1] : (int)*nchp;
+  const bool periodic = kq.boundary == MCPM_BC_PERIODIC;
+  uint32_t run = 0;
+  for (int im = periodic ? -1 : 0; im <= (periodic ? 1 : 0); ++im) {
+    const int32_t u = kl + im * nc;
+    // first chunk whose window ends past u, first whose window starts past u
+    const int blo = wave_lower_bound(b0, b1, [&](int m) { return wlo[m] + wlen[m] > u; });
+    const int bhi = wave_lower_bound(blo, b1, [&](int m) { return wlo[m] > u; });
+    for (int bb = blo; bb < bhi; bb += 64) {
+      const int b = bb + lane;
+      uint32_t v = 0;
+      int32_t w = -1;
+      if (b < bhi) {
+        w = u - wlo[b];
+        if (w >= 0 && w < wlen[b]) v = wtab[(size_t)b * WMAX + w];
+        else w = -1;
+      }
+      uint32_t x = v;  // inclusive scan over the wavefront
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+      }
+      if (w >= 0) wtab[(size_t)b * WMAX + w] = run + x - v;
+      run += __shfl(x, 63);
+    }
+  }
+  if (lane == 0) {
+    cnt[k] = run;
+    const double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
+    nden[k] = nd;
+    yc[k] = contact_y(nd, kq.b, kq.y_model);
   }
 }
 
@@ -832,10 +1131,13 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
   __shared__ uint32_t spos[NLIST];
   __shared__ uint32_t scell[1];
   const int nl = nbands * NREG;
-  const uint32_t lo = blockIdx.x * GEN_CHUNK;
+  // consecutive chunks (one cell's candidates, drawing from its particles) on
+  // one XCD, whose L2 then serves their reads of the cell's records
+  const uint32_t bx = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t lo = bx * GEN_CHUNK;
   if ((int)threadIdx.x < nl)  // list start + this chunk's offset inside the list
     spos[threadIdx.x] = counters[CNT_BSTART + threadIdx.x] +
-                        table[(size_t)blockIdx.x * NLIST + threadIdx.x];
+                        table[(size_t)bx * NLIST + threadIdx.x];
   if (threadIdx.x == 0) scell[0] = first_cell_of(lo, cstart, nc);
   __syncthreads();
   uint32_t c = scell[0];
@@ -1649,6 +1951,20 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                          double *, KParams, uint32_t, uint32_t, FPParams,  \
                                          OpLog, uint32_t *);
 
+#define MCPM_INST_W(R)                                                                     \
+  template __global__ void k_wpush<R, false>(const R *, R *, KParams, Kick, const double *, \
+                                             int, uint32_t, uint32_t, const uint32_t *,     \
+                                             const int32_t *, const uint32_t *, int32_t *,  \
+                                             int32_t *, uint32_t *, const uint32_t *,       \
+                                             uint32_t *, uint32_t *, const uint32_t *,      \
+                                             uint32_t *, int);                              \
+  template __global__ void k_wpush<R, true>(const R *, R *, KParams, Kick, const double *,  \
+                                            int, uint32_t, uint32_t, const uint32_t *,      \
+                                            const int32_t *, const uint32_t *, int32_t *,   \
+                                            int32_t *, uint32_t *, const uint32_t *,        \
+                                            uint32_t *, uint32_t *, const uint32_t *,       \
+                                            uint32_t *, int);
+
 #define MCPM_INST(R)                                                                       \
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
                                             Kick, const double *, int, uint32_t, uint32_t, \
@@ -1674,6 +1990,8 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
 
 MCPM_INST(Rec<float>)
 MCPM_INST(Rec<double>)
+MCPM_INST_W(Rec<float>)
+MCPM_INST_W(Rec<double>)
 MCPM_INST(Rec2<float>)
 MCPM_INST(Rec2<double>)
 
