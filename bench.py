@@ -456,21 +456,25 @@ def random_access_roofline(n, rec, valid, acc, revisits, nbands, coll_ms, launch
     Every access of an operation goes to an unpredictable particle, so the
     rounds are priced in accesses at the rates mcpm_probe_random_access
     measures on arrays of the same shape, summed over the access kinds (they
-    share one memory system).  Per step, in k_round: 2 reservation atomics
-    per valid operation outside band 0 (band 0 reserves in k_gen) and per
-    revisit, 2 reservation-word reads per visit (operations + revisits), 2
-    record reads per operation (its one winning visit), 2 record writes per
-    accepted collision, and the 16-byte operation records streamed in (each
-    visit) and out (each revisit) at the HBM peak."""
+    share one memory system).  Per step, in k_check + k_round: 2 priority
+    stores per valid operation outside band 0 (band 0 is stored by k_gen) and
+    per revisit, 2 priority-word reads (k_check) and 2 mark-byte reads
+    (k_round) per visit (operations + revisits), 2 record reads per operation
+    (its one winning visit), 2 record writes per accepted collision, and the
+    16-byte operation records streamed in (twice per visit, once per band
+    admission) and out (each revisit) at the HBM peak.  The atomics and
+    word re-reads of particles that several operations share are left out
+    (a few per cent of the operations)."""
     import mcpm
     rates = mcpm.probe_random_access(n, rec)
     visits = valid + revisits
-    counts = {"atomic_min_word": 2.0 * valid * (1.0 - 1.0 / max(nbands, 1)) + 2.0 * revisits,
+    counts = {"scatter_word": 2.0 * valid * (1.0 - 1.0 / max(nbands, 1)) + 2.0 * revisits,
               "gather_word": 2.0 * visits,
+              "gather_byte": 2.0 * visits,
               "gather_record": 2.0 * valid,
               "scatter_record": 2.0 * acc}
     floor_s = sum(counts[k] / rates[k] for k in counts)
-    floor_s += 16.0 * (visits + revisits) / (HBM_PEAK_GBS * 1e9)
+    floor_s += 16.0 * (2.0 * visits + valid + revisits) / (HBM_PEAK_GBS * 1e9)
     floor_ms = 1000.0 * floor_s
     return {"rates_G_per_s": {k: round(v / 1e9, 2) for k, v in rates.items()},
             "accesses_per_step": {k: round(v) for k, v in counts.items()},

@@ -501,16 +501,19 @@ int mcpm_timing_get(mcpm_ctx *ctx, int32_t which, double *avg_ms, int64_t *calls
 /* ---- device random-access rates ------------------------------------------- */
 /* No counterpart in the reference: a measurement of the current device for
    the roofline of the reservation rounds, whose accesses all go to
-   unpredictable particles.  Over n_records records of record_bytes (16 or 32)
-   and 2 n_records 8-byte words, with indices hashed uniformly, the best of
-   five launches of min(n_records, 2^24) accesses each, in accesses per
-   second: record gathers, word gathers, 64-bit atomic minimums (result
-   unused) and record stores.  Allocates and frees its own arrays. */
+   unpredictable particles.  Over n_records records of record_bytes (16 or 32),
+   n_records 4-byte words and n_records bytes, with indices hashed uniformly,
+   the best of five launches of min(n_records, 2^24) accesses each, in
+   accesses per second: record gathers, word gathers, 32-bit atomic minimums
+   (result unused), record stores, word stores and byte gathers.  Allocates
+   and frees its own arrays. */
 typedef struct mcpm_access_rates {
   double gather_record;
   double gather_word;
   double atomic_min_word;
   double scatter_record;
+  double scatter_word;
+  double gather_byte;
 } mcpm_access_rates;
 int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes, mcpm_access_rates *out);
 

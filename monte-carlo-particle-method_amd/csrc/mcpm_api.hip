@@ -44,6 +44,11 @@ struct mcpm_ctx {
   size_t rec = 0;
   void *P[2] = {nullptr, nullptr};
   int cur = 0;
+  // the slab's sorted positions y (record precision), written by the sorts
+  // beside the records; valid while no other kernel has moved or reordered
+  // a particle since
+  void *ys = nullptr;
+  bool ys_valid = false;
   // windowed sort (particles a few cells from where they were): chunk bounds
   // and replicas (host-built when the particle counts change), windows, table
   // (chunks from WSPAN-cell blocks of the previous sort's cells, built on the
@@ -84,8 +89,11 @@ struct mcpm_ctx {
   uint32_t *counters = nullptr, *err = nullptr;
   unsigned long long *esc = nullptr;  // re-inserted particles (REINJECT boundary)
   double *energy_part = nullptr;      // per-workgroup sums of the energy check
-  uint64_t *res = nullptr;  // reservations, two interleaved arrays
-  uint32_t round_base = 0;  // rounds run so far: the next round's tag is ~round_base
+  // reservations: the priority tables of even and odd rounds (capacity words
+  // each), then the shared-particle marks (capacity bytes), one allocation
+  uint32_t *rtab = nullptr;
+  uint8_t *cflag = nullptr;
+  uint32_t round_base = 0;  // rounds run so far: round marks are its low byte
   // collisions
   uint4 *opA = nullptr, *opB = nullptr;  // one allocation: ping-pong op lists,
   uint4 *opBand = nullptr;               // then the priority-band lists
@@ -555,15 +563,16 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
                      (uint32_t)c->n, c->wbeg, c->wrep, nchp, c->rchunk);
   // the most chunks there can be; waves past the count stop at once
   const unsigned gw = (unsigned)((c->wcap + 3) / 4);
+  using real = typename R::real_type;
   hipLaunchKernelGGL((k_wpush<R, false>), dim3(gw), dim3(256), 0, c->s, in, nullptr, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
-                     nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta);
+                     nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta, (real *)nullptr);
   hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 3) / 4)), dim3(256), 0, c->s, c->wlo,
                      c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc);
   hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->cnt, c->start, c->nc);
   hipLaunchKernelGGL((k_wpush<R, true>), dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
-                     c->start, c->err, c->wflag, iid, oid, c->wdelta);
+                     c->start, c->err, c->wflag, iid, oid, c->wdelta, (real *)c->ys);
   LAUNCHCHK();
   uint32_t flag = 0;
   HIPCHK(hipMemcpyAsync(&flag, c->wflag, sizeof flag, hipMemcpyDeviceToHost, c->s));
@@ -580,12 +589,15 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
   c->cur = 1 - c->cur;
   c->sorted = true;
   c->order_sorted = true;
+  c->ys_valid = c->ys != nullptr;
   return MCPM_OK;
 }
 
 template <typename R>
 int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
+  using real = typename R::real_type;
   c->keys_current = false;
+  c->ys_valid = false;
   if (c->n == 0) {
     HIPCHK(hipMemsetAsync(c->cnt, 0, c->nc * sizeof(uint32_t), c->s));
     HIPCHK(hipMemsetAsync(c->start, 0, (c->nc + 1) * sizeof(uint32_t), c->s));
@@ -617,11 +629,12 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   // one stable counting-sort pass over nb bins: histogram, scans, scatter
   auto pass = [&](const R *in, R *out, int pmode, const Kick &ks, int nb, int kmode,
                   uint32_t *cnt, uint32_t *start, double *nden, double *yc, uint32_t *okey,
-                  const uint32_t *ikey, const uint32_t *iid, uint32_t *oid) -> int {
+                  const uint32_t *ikey, const uint32_t *iid, uint32_t *oid, real *oys) -> int {
     const size_t lds = (size_t)nb * sizeof(uint32_t);
     hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
                        c->s, in, nullptr, c->n, chunk, c->kp, kick, drift, pmode, slo, shi, c->table,
-                       nullptr, c->err, nb, kmode, c->kdiv, nullptr, ikey, nullptr, nullptr);
+                       nullptr, c->err, nb, kmode, c->kdiv, nullptr, ikey, nullptr, nullptr,
+                       (real *)nullptr);
     LAUNCHCHK();
     dim3 gs((nb + 255) / 256, nseg);
     hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
@@ -634,7 +647,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
     hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
                        c->s, in, out,
                        c->n, chunk, c->kp, ks, drift, pmode, slo, shi, c->table, start,
-                       c->err, nb, kmode, c->kdiv, okey, ikey, iid, oid);
+                       c->err, nb, kmode, c->kdiv, okey, ikey, iid, oid, oys);
     LAUNCHCHK();
     return MCPM_OK;
   };
@@ -645,17 +658,17 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   int st;
   if (!c->two_digit) {
     if ((st = pass(in, out, mode, kick_scatter, nc, 0, c->cnt, c->start, c->nden, c->yc,
-                   nullptr, nullptr, id_in, id_out)))
+                   nullptr, nullptr, id_in, id_out, (real *)c->ys)))
       return st;
   } else {
     // low digit with the push (keys carried to the destinations), then the
     // high digit of those keys back into the first buffer; cells from the
     // sorted keys
     if ((st = pass(in, out, mode, kick_scatter, c->nb_lo, 1, c->dcnt, c->dstart, nullptr,
-                   nullptr, c->keys_tmp, nullptr, id_in, id_out)))
+                   nullptr, c->keys_tmp, nullptr, id_in, id_out, nullptr)))
       return st;
     if ((st = pass(out, (R *)in, 0, kick, c->nb_hi, 2, c->dcnt, c->dstart, nullptr, nullptr,
-                   c->keys, c->keys_tmp, id_out, id_in)))
+                   c->keys, c->keys_tmp, id_out, id_in, nullptr)))
       return st;
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->s, c->start, (int64_t)nc + 1, 0u);
     hipLaunchKernelGGL(k_cell_bounds, dim3(2048), dim3(256), 0, c->s, c->keys, c->n, nc,
@@ -671,6 +684,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   c->cur = 1 - c->cur;
   c->sorted = true;
   c->order_sorted = true;
+  c->ys_valid = c->ys != nullptr;
   return MCPM_OK;
 }
 
@@ -743,16 +757,10 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   unsigned grid = (unsigned)std::min<int64_t>(std::max<int64_t>(8, ((int64_t)M + 255) / 256),
                                               c->round_grid);
   grid = (grid + 7u) / 8u * 8u;
-  // reservation tags count down: round r of this call has tag ~round_base
-  // (round_base advanced once per launched round), and one tag is skipped
-  // first, the one the previous call's last round may have reserved with.
-  // Long before they run out the words are reset to all-ones.
-  if (c->round_base > 0xFFFFFFFFu - 2u * (uint32_t)MAX_ROUNDS - 8u) {
-    HIPCHK(hipMemsetAsync(c->res, 0xFF, 2 * (size_t)c->p.capacity * sizeof(uint64_t), c->s));
-    c->round_base = 0;
-  }
-  c->round_base += 1u;
-  const uint32_t tag0 = ~c->round_base;
+  // round r's priority table is rtab + (r & 1) capacity; its shared-particle
+  // mark is the low byte of round_base, which advances once per round (and
+  // across calls), so consecutive rounds never share a mark
+  uint32_t *const tabs[2] = {c->rtab, c->rtab + (size_t)c->p.capacity};
   if (M > 0) {
     Timer tm(c, T_GEN);
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
@@ -761,9 +769,12 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
                        nchunks, nbands * NREG, c->counters);
     hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
-    hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nchunks), dim3(256), 0, c->s, P, c->cnt, c->start,
+    const typename R::real_type *ys =
+        c->ys_valid ? (const typename R::real_type *)c->ys : nullptr;
+    hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nchunks), dim3(256), 0, c->s, P, ys, c->cnt,
+                       c->start,
                        c->cstart, nc, M, c->kp, slo, shi, pk.v[0], band_mul, nbands,
-                       c->gen_table, c->opBand, c->res, tag0, c->counters);
+                       c->gen_table, c->opBand, tabs[0], c->counters);
     LAUNCHCHK();
   }
   int rounds = 0;
@@ -779,13 +790,16 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     for (int r = 0; r < MAX_ROUNDS && !done; ++r) {
       {
         Timer tm(c, T_COLLIDE);
+        const uint4 *in = (const uint4 *)((r & 1) ? c->opB : c->opA);
+        const uint32_t mark = c->round_base & 0xFFu;
+        hipLaunchKernelGGL((k_check<MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
+                           (const uint4 *)c->opBand, tabs[r & 1], c->cflag, mark,
+                           (const uint32_t *)c->counters, c->dleft);
         hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
-                           P, (const uint4 *)((r & 1) ? c->opB : c->opA),
-                           (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
-                           c->res + (r & 1), c->res + ((r + 1) & 1), ~c->round_base,
-                           c->counters, c->nden,
-                           c->yc, c->omax, c->omega_new, c->kp, slo, shi, fpp, lg,
-                           c->dleft);
+                           P, in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
+                           (const uint32_t *)tabs[r & 1], tabs[(r + 1) & 1],
+                           (const uint8_t *)c->cflag, mark, c->counters, c->nden,
+                           c->yc, c->omax, c->omega_new, c->kp, slo, shi, fpp, lg);
       }
       c->round_base += 1u;
       LAUNCHCHK();
@@ -855,6 +869,7 @@ int collide_esmc4_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
 }
 template <typename R>
 int collide_dsmc_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int variant) {
+  if (variant == MCPM_MODEL_CBA_VHS) c->ys_valid = false;  // CBA displaces the pairs
   return collide_t<R, COLL_DSMC>(c, step, stats, variant);
 }
 
@@ -865,6 +880,7 @@ int launch_moments(mcpm_ctx *c);
 template <typename R>
 int collide_cba_hs_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  c->ys_valid = false;  // CBA displaces the pairs
   int st = launch_moments<R>(c);
   if (st) return st;
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->s, c->counters, (int)CNT_LIST);
@@ -1115,6 +1131,7 @@ int launch_init(mcpm_ctx *c, const InitParams &ip) {
   if (c->track_ids && ip.n > 0)
     hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], ip.n);
   c->order_sorted = false;
+  c->ys_valid = false;
   LAUNCHCHK();
   return MCPM_OK;
 }
@@ -1265,6 +1282,9 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
   };
   if ((st = dalloc((char **)&c->P[0], c->rec * (size_t)p->capacity))) return fail(st);
   if ((st = dalloc((char **)&c->P[1], c->rec * (size_t)p->capacity))) return fail(st);
+  if (p->dims != 2 &&
+      (st = dalloc((char **)&c->ys, (c->rec / 4) * (size_t)p->capacity)))
+    return fail(st);
   // chunks of at least 1024 particles; the per-chunk histogram table is kept
   // under 512 MB (chunk_for's automatic size makes about 2048 chunks)
   // more cells than one LDS histogram holds: sort by two balanced digits
@@ -1344,7 +1364,9 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
                     hipHostMallocMapped) != hipSuccess ||
       hipHostGetDevicePointer((void **)&c->dleft, c->hleft, 0) != hipSuccess)
     return fail(MCPM_ERR_HIP);
-  if ((st = dalloc(&c->res, 2 * (size_t)p->capacity))) return fail(st);
+  if ((st = dalloc(&c->rtab, 2 * (size_t)p->capacity + ((size_t)p->capacity + 3) / 4)))
+    return fail(st);
+  c->cflag = (uint8_t *)(c->rtab + 2 * (size_t)p->capacity);
   if (hipMemset(c->omax, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->omega_new, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->F2, 0, nc * sizeof(double)) != hipSuccess ||
@@ -1354,7 +1376,7 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
       hipMemset(c->esc, 0, sizeof(unsigned long long)) != hipSuccess ||
       hipMemset(c->cnt, 0, nc * sizeof(uint32_t)) != hipSuccess ||
       hipMemset(c->start, 0, (nc + 1) * sizeof(uint32_t)) != hipSuccess ||
-      hipMemset(c->res, 0xFF, 2 * (size_t)p->capacity * sizeof(uint64_t)) != hipSuccess)
+      hipMemset(c->cflag, 0, (size_t)p->capacity) != hipSuccess)
     return fail(MCPM_ERR_HIP);
   if (p->dims == 2) {
     if ((st = build_fem(c))) return fail(st);
@@ -1422,11 +1444,11 @@ int mcpm_destroy(mcpm_ctx *c) {
   if (c->s) hipStreamSynchronize(c->s);
   void *bufs[] = {c->rkp, c->cell_rep, c->rcell_off, c->rpart_off, c->rprio, c->rcand_off,
                   c->rfa,
-                  c->P[0], c->P[1], c->table, c->partial, c->dcnt, c->dstart, c->keys,
+                  c->P[0], c->P[1], c->ys, c->table, c->partial, c->dcnt, c->dstart, c->keys,
                   c->keys_tmp,
                   c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
-                  c->samples, c->press, c->counters, c->err, c->esc, c->energy_part, c->res, c->opA,
+                  c->samples, c->press, c->counters, c->err, c->esc, c->energy_part, c->rtab, c->opA,
                   c->log.p, c->field_buf, c->clos, c->drift, c->lr_tab, c->kde, c->gen_table,
                   c->fem_buf, c->q4, c->fft_buf, c->fft_twx, c->fft_twy,
                   c->fft_cosx, c->fft_cosy, c->ids[0], c->wbeg, c->wrep, c->wlo,
@@ -1489,6 +1511,7 @@ int upload_any(mcpm_ctx *c, int64_t n, const double *const *src, int nsrc) {
   c->sorted = false;
   c->kick_pending = false;
   c->order_sorted = false;
+  c->ys_valid = false;
   if (c->track_ids && n > 0) {
     hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], n);
     LAUNCHCHK();

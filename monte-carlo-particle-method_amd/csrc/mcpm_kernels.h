@@ -68,7 +68,7 @@ __global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp
                        uint32_t step_lo, uint32_t step_hi, uint32_t *table,
                        const uint32_t *start, uint32_t *err, int nb, int kmode, int32_t kdiv,
                        uint32_t *okey, const uint32_t *ikey, const uint32_t *iid,
-                       uint32_t *oid);
+                       uint32_t *oid, typename R::real_type *oys);
 // The windowed sort of particles that move a few cells per step: WCH
 // particles per chunk, histograms over at most WMAX cells, WDELTA cells of
 // slack either side of the chunk's old cells.
@@ -85,7 +85,8 @@ __global__ void k_wpush(const R *in, R *out, KParams kp, Kick kick, const double
                         uint32_t step_lo, uint32_t step_hi, const uint32_t *wbeg,
                         const int32_t *wrep, const uint32_t *nchp, int32_t *wlo, int32_t *wlen,
                         uint32_t *wtab, const uint32_t *start, uint32_t *err, uint32_t *wflag,
-                        const uint32_t *iid, uint32_t *oid, int wdelta);
+                        const uint32_t *iid, uint32_t *oid, int wdelta,
+                        typename R::real_type *oys);
 __global__ void k_wblocks(const int32_t *bcell, int nblk, const uint32_t *start,
                           uint32_t *pieces);
 __global__ void k_wfill(const int32_t *bcell, const int32_t *brep, int nblk,
@@ -144,19 +145,27 @@ __global__ void k_cand_count(uint32_t M, uint32_t prio_key, int model, uint32_t 
 __global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters);
 __global__ void k_list_starts(uint32_t *counters, int nl);
 template <typename R, int MODEL>
-__global__ void k_gen(const R *P, const uint32_t *cnt, const uint32_t *start,
+__global__ void k_gen(const R *P, const typename R::real_type *ys, const uint32_t *cnt,
+                      const uint32_t *start,
                       const uint32_t *cstart, int nc, uint32_t M, KParams kp, uint32_t step_lo,
                       uint32_t step_hi, uint32_t prio_key, uint32_t band_mul, int nbands,
-                      const uint32_t *table, uint4 *bands, uint64_t *res0, uint32_t tag0,
+                      const uint32_t *table, uint4 *bands, uint32_t *tab0,
                       uint32_t *counters);
+// A reservation round: k_check resolves the particles that several of the
+// round's operations share, k_round commits the winners and admits the next
+// round's operations.
+template <int MODEL>
+__global__ void k_check(int r, int nbands, uint32_t seg, const uint4 *in, const uint4 *bands,
+                        uint32_t *tab, uint8_t *cflag, uint32_t tag8, const uint32_t *counters,
+                        uint32_t *host_left);
 template <typename R, int MODEL>
 __global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
-                        const uint4 *bands, uint4 *out,
-                        const uint64_t *res_cur, uint64_t *res_next, uint32_t tag,
+                        const uint4 *bands, uint4 *out, const uint32_t *tab_cur,
+                        uint32_t *tab_next, const uint8_t *cflag, uint32_t tag8,
                         uint32_t *counters,
                         const double *nden, const double *yc, const double *omax,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,
-                        FPParams fpp, OpLog log, uint32_t *host_left);
+                        FPParams fpp, OpLog log);
 __global__ void k_omega_finish(double *omax, double *omega_new, const uint32_t *cnt, int nc,
                                int model, int variant);
 // Most candidates one cell may run in one Collision_CBA_HS call.

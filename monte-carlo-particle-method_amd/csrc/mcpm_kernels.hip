@@ -289,7 +289,8 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
     Kick kick, const double *__restrict__ drift, int mode, uint32_t step_lo, uint32_t step_hi,
     uint32_t *__restrict__ table, const uint32_t *__restrict__ start, uint32_t *err, int nb,
     int kmode, int32_t kdiv, uint32_t *__restrict__ okey, const uint32_t *__restrict__ ikey,
-    const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid) {
+    const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid,
+    typename R::real_type *__restrict__ oys) {
   extern __shared__ uint32_t lds_cnt[];
   constexpr int TILE = 64 * PUSH_WAVES;  // particles of one tile, wavefront w the w-th 64
   const int nc = nb;
@@ -408,6 +409,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
                 out[dst] = r;
                 if (okey) okey[dst] = okv;
                 if (oid) oid[dst] = iid[p];
+                if (oys) oys[dst] = r.y;
               } else {
                 atomicOr(err, ERRBIT_ORDER);  // the two passes disagreed on a key
               }
@@ -484,7 +486,7 @@ __global__ __launch_bounds__(256) void k_wpush(
     const uint32_t *__restrict__ nchp, int32_t *__restrict__ wlo, int32_t *__restrict__ wlen,
     uint32_t *__restrict__ wtab, const uint32_t *__restrict__ start, uint32_t *__restrict__ err,
     uint32_t *__restrict__ wflag, const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid,
-    int wdelta) {
+    int wdelta, typename R::real_type *__restrict__ oys) {
   __shared__ uint32_t lcnt[4][WMAX];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nch = (int)*nchp;
@@ -590,6 +592,7 @@ __global__ __launch_bounds__(256) void k_wpush(
           if (dst < ntot) {
             out[dst] = r;
             if (oid) oid[dst] = iid[p];
+            if (oys) oys[dst] = r.y;
           } else {
             atomicOr(wflag, 1u);  // counts out of step with the particles
           }
@@ -899,8 +902,25 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__
 // clamp (Collision_ESMC_new4, collisions.cpp:784-792): a partner point
 // beyond a wall takes the wall's cell instead of leaving the candidate
 // without a partner.
+// Particle i's record as the partner-cell rule reads it: on the slab only y,
+// from the sorted positions ys when the sort wrote them.
+template <typename R>
+__device__ __forceinline__ R position_of(const R *P, const typename R::real_type *ys,
+                                         uint32_t i) {
+  if constexpr (R::dims == 1) {
+    if (ys) {
+      R r;
+      r.y = ys[i];
+      r.u1 = r.u2 = r.u3 = 0;
+      return r;
+    }
+  }
+  return P[i];
+}
+
 template <typename R>
 __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *P,
+                                               const typename R::real_type *ys,
                                                const uint32_t *cnt, const uint32_t *start,
                                                const KParams &kp, uint32_t step_lo,
                                                uint32_t step_hi, uint32_t *oi, uint32_t *oj,
@@ -913,10 +933,11 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
   int32_t J;
+  const R ri = position_of(P, ys, i);
   if (clamp) {
-    if constexpr (R::dims == 1) J = partner_cell_clamped(P[i], d1, kp);
+    if constexpr (R::dims == 1) J = partner_cell_clamped(ri, d1, kp);
     else return false;
-  } else if (!partner_cell(P[i], d0, d1, kp, &J)) {
+  } else if (!partner_cell(ri, d0, d1, kp, &J)) {
     return false;
   }
   uint32_t NJ = cnt[J];
@@ -933,7 +954,9 @@ __device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *
 // its local cell number for the counters, its geometry for the partner cell.
 template <typename R>
 __device__ __forceinline__ bool draw_candidate_rep(uint32_t c, uint32_t t, int32_t rq,
-                                                   const R *P, const uint32_t *cnt,
+                                                   const R *P,
+                                                   const typename R::real_type *ys,
+                                                   const uint32_t *cnt,
                                                    const uint32_t *start, const KParams &kp,
                                                    uint32_t step_lo, uint32_t step_hi,
                                                    uint32_t *oi, uint32_t *oj) {
@@ -947,7 +970,7 @@ __device__ __forceinline__ bool draw_candidate_rep(uint32_t c, uint32_t t, int32
   double d0, d1, d2;
   mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
   int32_t J;
-  if (!partner_cell(P[i], d0, d1, kq, &J)) return false;
+  if (!partner_cell(position_of(P, ys, i), d0, d1, kq, &J)) return false;
   J += off;
   uint32_t NJ = cnt[J];
   if (NJ == 0) return false;
@@ -1101,23 +1124,15 @@ __global__ void k_list_starts(uint32_t *__restrict__ counters, int nl) {
   counters[CNT_BSTART + nl] = run;
 }
 
-// A reservation word: the round's tag above the operation's priority.  Tags
-// count down from one round to the next (and across steps), so a fresh
-// reservation is below every word left from an earlier round: the arrays
-// need no clearing between rounds.
-__device__ __forceinline__ uint64_t res_word(uint32_t tag, uint32_t pr) {
-  return ((uint64_t)tag << 32) | (uint64_t)pr;
-}
-
 // Candidate generation, one chunk of GEN_CHUNK candidate indices per
 // workgroup (cell order, so a chunk covers one or a few cells).  Each
 // candidate k of cell c (t = k - cstart[c]) is drawn, given its priority
 // and written to its (band, region) list as {i, j, priority, t}, or as
-// OP_NONE when it has no partner; band 0 reserves its particles for the
-// first round (tag tag0).  Reservations of particle i live at res[2 i] (even
-// rounds) and res[2 i + 1] (odd rounds).
+// OP_NONE when it has no partner; band 0 enters the first round: its
+// priority is stored at both of its particles in tab0.
 template <typename R, int MODEL>
 __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
+                                             const typename R::real_type *__restrict__ ys,
                                              const uint32_t *__restrict__ cnt,
                                              const uint32_t *__restrict__ start,
                                              const uint32_t *__restrict__ cstart, int nc,
@@ -1126,7 +1141,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              uint32_t band_mul, int nbands,
                                              const uint32_t *__restrict__ table,
                                              uint4 *__restrict__ bands,
-                                             uint64_t *__restrict__ res0, uint32_t tag0,
+                                             uint32_t *__restrict__ tab0,
                                              uint32_t *__restrict__ counters) {
   __shared__ uint32_t spos[NLIST];
   __shared__ uint32_t scell[1];
@@ -1159,10 +1174,10 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
           draw_dsmc_pair(c, k, cnt, start, kp, step_lo, step_hi, &i, &j);
           ok = true;
         } else if (kp.nrep == 0) {
-          ok = draw_candidate<R>(c, t, P, cnt, start, kp, step_lo, step_hi, &i, &j,
+          ok = draw_candidate<R>(c, t, P, ys, cnt, start, kp, step_lo, step_hi, &i, &j,
                                  MODEL == COLL_ESMC4 && kp.boundary != MCPM_BC_PERIODIC);
         } else {
-          ok = draw_candidate_rep<R>(c, t, rq, P, cnt, start, kp, step_lo, step_hi, &i, &j);
+          ok = draw_candidate_rep<R>(c, t, rq, P, ys, cnt, start, kp, step_lo, step_hi, &i, &j);
         }
       }
       uint32_t band;
@@ -1178,10 +1193,8 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
       if (ok) {
         ++nvalid;
         if (band == 0) {
-          atomicMin((unsigned long long *)&res0[2 * (size_t)i],
-                    (unsigned long long)res_word(tag0, pr));
-          atomicMin((unsigned long long *)&res0[2 * (size_t)j],
-                    (unsigned long long)res_word(tag0, pr));
+          tab0[i] = pr;
+          tab0[j] = pr;
         }
       }
       (void)t;
@@ -1312,48 +1325,114 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t k, 
   return true;
 }
 
-// One round of deterministic reservations.  An operation commits when it
-// holds the smallest priority on both of its particles, which is exactly
-// when every earlier-priority operation sharing a particle has committed:
-// the result equals processing all operations one by one in priority order.
-// Losers reserve in the other array for the next round and are compacted
-// into the next operation list.  Round r's words carry tag tag (the next
-// round's tag - 1 is smaller), so words left from earlier rounds never win
-// and nothing is cleared: the round writes no reservation word except by
-// reserving.  The two arrays are interleaved (particle i at res[2 i] and
-// res[2 i + 1]): a loser's read and its next reservation share a cache line.
+// Deterministic reservations without atomics on the common path.  An
+// operation commits when it holds the smallest priority on both of its
+// particles among the operations of its round, which is exactly when every
+// earlier-priority operation sharing a particle has committed: the result
+// equals processing all operations one by one in priority order.
 // Operations enter by priority band: round r takes the losers of round r-1
-// plus band r, and reserves band r+1 for round r+1.  Every operation of a
-// lower priority than an admitted one is admitted too, so the smallest
-// reservation among admitted operations is the smallest among all unresolved
-// ones; late operations no longer lose round after round to early ones.
+// plus band r.  Every operation of a lower priority than an admitted one is
+// admitted too, so the smallest priority among a round's operations is the
+// smallest among all unresolved ones; late operations no longer lose round
+// after round to early ones.
+// A round is two launches over its operations:
+//  * admission (the round before, or candidate generation for round 0):
+//    each operation stores its priority at both particles of tab (plain
+//    stores; one of the values stored at a particle survives);
+//  * k_check: an operation that does not find its own priority at a
+//    particle lowers the word with atomicMin and marks the particle in
+//    cflag with the round's tag.  After it tab holds, at every particle,
+//    the smallest priority of the round's operations on it, and a particle
+//    left unmarked has exactly one operation on it;
+//  * k_round: an operation wins when each of its particles is unmarked or
+//    holds its priority.  Winners commit; losers store their priority into
+//    the next round's table and are compacted into the next list, and band
+//    r+1 is admitted there.
+// Every word a round reads was stored in that round, so the tables are never
+// cleared; a stale mark equal to the round's tag only sends an operation to
+// read its word, which then holds its own priority.  Atomics are left to
+// the particles that several operations share.
 // Region x of the operations is processed by the workgroups with
 // blockIdx % 8 == x, which the hardware places on XCD x: each XCD keeps
 // working on the same run of cells, whose particles and reservations stay in
 // its L2.  Losers of region x are kept in segment x of the output list.
-template <typename R, int MODEL>
-__global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
-                                               R *__restrict__ P,
+template <int MODEL>
+__global__ __launch_bounds__(256) void k_check(int r, int nbands, uint32_t seg,
                                                const uint4 *__restrict__ in,
                                                const uint4 *__restrict__ bands,
-                                               uint4 *__restrict__ out,
-                                               const uint64_t *__restrict__ res_cur,
-                                               uint64_t *__restrict__ res_next, uint32_t tag,
-                                               uint32_t *__restrict__ counters,
-                                               const double *__restrict__ nden,
-                                               const double *__restrict__ yc,
-                                               const double *__restrict__ omax,
-                                               double *__restrict__ omega_new, KParams kp,
-                                               uint32_t step_lo, uint32_t step_hi,
-                                               FPParams fpp, OpLog log,
+                                               uint32_t *__restrict__ tab,
+                                               uint8_t *__restrict__ cflag, uint32_t tag8,
+                                               const uint32_t *__restrict__ counters,
                                                uint32_t *__restrict__ host_left) {
-  __shared__ uint32_t s_lose, s_base, s_acc;
   // publish the losers left after round r - 1 (final now) to the host
   if (blockIdx.x == 0 && threadIdx.x == 0 && r > 0) {
     uint32_t left = 0;
     for (int q = 0; q < NREG; ++q) left += counters[CNT_LIST + r * NREG + q];
     __hip_atomic_store(&host_left[r], left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
+  const uint32_t x = blockIdx.x % NREG;
+  const uint32_t lb = blockIdx.x / NREG, nbx = gridDim.x / NREG;
+  const int qB = r * NREG + (int)x;
+  const bool hasB = r < nbands;
+  const uint32_t countB = hasB ? counters[CNT_BAND + qB] : 0u;
+  const uint32_t sB = hasB ? counters[CNT_BSTART + qB] : 0u;
+  const uint32_t countA = (r == 0) ? 0u : counters[CNT_LIST + r * NREG + x];
+  const uint32_t count = countA + countB;
+  const uint4 *inA = in + (size_t)x * seg;
+  const uint4 *inB = bands + sB;
+  const uint8_t mark = (uint8_t)tag8;
+  constexpr int U = 4;
+  const uint32_t tile = blockDim.x * U;
+  for (uint32_t base = lb * tile; base < count; base += nbx * tile) {
+    uint4 rec[U];
+    uint32_t ti[U], tj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t idx = base + u * blockDim.x + threadIdx.x;
+      rec[u] = make_uint4(OP_NONE, 0u, 0u, 0u);
+      if (idx < countA) rec[u] = inA[idx];
+      else if (idx < count) rec[u] = inB[idx - countA];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      ti[u] = tj[u] = 0u;
+      if (rec[u].x != OP_NONE) {
+        ti[u] = tab[rec[u].x];
+        tj[u] = tab[rec[u].y];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (rec[u].x == OP_NONE) continue;
+      if (ti[u] != rec[u].z) {
+        atomicMin(&tab[rec[u].x], rec[u].z);
+        cflag[rec[u].x] = mark;
+      }
+      if (tj[u] != rec[u].z) {
+        atomicMin(&tab[rec[u].y], rec[u].z);
+        cflag[rec[u].y] = mark;
+      }
+    }
+  }
+}
+
+template <typename R, int MODEL>
+__global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
+                                               R *__restrict__ P,
+                                               const uint4 *__restrict__ in,
+                                               const uint4 *__restrict__ bands,
+                                               uint4 *__restrict__ out,
+                                               const uint32_t *__restrict__ tab_cur,
+                                               uint32_t *__restrict__ tab_next,
+                                               const uint8_t *__restrict__ cflag, uint32_t tag8,
+                                               uint32_t *__restrict__ counters,
+                                               const double *__restrict__ nden,
+                                               const double *__restrict__ yc,
+                                               const double *__restrict__ omax,
+                                               double *__restrict__ omega_new, KParams kp,
+                                               uint32_t step_lo, uint32_t step_hi,
+                                               FPParams fpp, OpLog log) {
+  __shared__ uint32_t s_lose, s_base, s_acc;
   const uint32_t x = blockIdx.x % NREG;
   const uint32_t lb = blockIdx.x / NREG, nbx = gridDim.x / NREG;
   // band r and band r+1 of region x in the band-major lists
@@ -1371,14 +1450,14 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
   uint4 *outx = out + (size_t)x * seg;
   uint32_t *next_count = &counters[CNT_LIST + (r + 1) * NREG + x];
   const uint32_t stride = nbx * blockDim.x;
-  // admit band r+1 first: its reservations (no return value) overlap with
-  // the work below; the lists also hold partnerless candidates as OP_NONE
+  const uint8_t mark = (uint8_t)tag8;
+  // admit band r+1 first: its stores overlap with the work below; the lists
+  // also hold partnerless candidates as OP_NONE
   for (uint32_t idx = lb * blockDim.x + threadIdx.x; idx < countC; idx += stride) {
     const uint4 rec = inC[idx];
     if (rec.x == OP_NONE) continue;
-    const unsigned long long w = res_word(tag - 1u, rec.z);
-    atomicMin((unsigned long long *)&res_next[2 * (size_t)rec.x], w);
-    atomicMin((unsigned long long *)&res_next[2 * (size_t)rec.y], w);
+    tab_next[rec.x] = rec.z;
+    tab_next[rec.y] = rec.z;
   }
   // U operations per thread per tile: their loads are issued together
   constexpr int U = 4;
@@ -1386,7 +1465,6 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
   uint32_t acc_local = 0;
   for (uint32_t base = lb * tile; base < count; base += nbx * tile) {
     uint4 rec[U];
-    uint64_t ri[U], rj[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const uint32_t idx = base + u * blockDim.x + threadIdx.x;
@@ -1394,12 +1472,13 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
       if (idx < countA) rec[u] = inA[idx];
       else if (idx < count) rec[u] = inB[idx - countA];
     }
+    bool fi[U], fj[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      ri[u] = rj[u] = 0u;
+      fi[u] = fj[u] = false;
       if (rec[u].x != OP_NONE) {
-        ri[u] = res_cur[2 * (size_t)rec[u].x];
-        rj[u] = res_cur[2 * (size_t)rec[u].y];
+        fi[u] = cflag[rec[u].x] == mark;
+        fj[u] = cflag[rec[u].y] == mark;
       }
     }
     R pa[U], pb[U];
@@ -1407,8 +1486,10 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const bool valid = rec[u].x != OP_NONE;
-      const uint64_t mine = res_word(tag, rec[u].z);
-      win[u] = valid && ri[u] == mine && rj[u] == mine;
+      // a marked particle is shared: its word holds the round's smallest priority
+      const bool oki = !fi[u] || (valid && tab_cur[rec[u].x] == rec[u].z);
+      const bool okj = !fj[u] || (valid && tab_cur[rec[u].y] == rec[u].z);
+      win[u] = valid && oki && okj;
       lose[u] = valid && !win[u];
       if (win[u]) {
         pa[u] = P[rec[u].x];
@@ -1433,9 +1514,8 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
           log.a[q] = acc ? 1 : 0;
         }
       } else if (lose[u]) {
-        const unsigned long long w = res_word(tag - 1u, pr);
-        atomicMin((unsigned long long *)&res_next[2 * (size_t)i], w);
-        atomicMin((unsigned long long *)&res_next[2 * (size_t)j], w);
+        tab_next[i] = pr;
+        tab_next[j] = pr;
       }
     }
     // compact the tile's losers: ranks in the wavefront, one LDS atomic per
@@ -1939,17 +2019,28 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
 }
 
 #define MCPM_INST_COLL(R, M)                                                               \
-  template __global__ void k_gen<R, M>(const R *, const uint32_t *, const uint32_t *,      \
+  template __global__ void k_gen<R, M>(const R *, const R::real_type *, const uint32_t *,  \
+                                       const uint32_t *,                                   \
                                        const uint32_t *, int, uint32_t, KParams, uint32_t, \
                                        uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
-                                       uint4 *, uint64_t *, uint32_t, uint32_t *);         \
+                                       uint4 *, uint32_t *, uint32_t *);                   \
   template __global__ void k_round<R, M>(int, int, uint32_t, R *, const uint4 *,           \
-                                         const uint4 *, uint4 *,                           \
-                                         const uint64_t *, uint64_t *, uint32_t,           \
+                                         const uint4 *, uint4 *, const uint32_t *,         \
+                                         uint32_t *, const uint8_t *, uint32_t,            \
                                          uint32_t *,                                       \
                                          const double *, const double *, const double *,   \
                                          double *, KParams, uint32_t, uint32_t, FPParams,  \
-                                         OpLog, uint32_t *);
+                                         OpLog);
+
+template __global__ void k_check<COLL_ESMC>(int, int, uint32_t, const uint4 *, const uint4 *,
+                                             uint32_t *, uint8_t *, uint32_t, const uint32_t *,
+                                             uint32_t *);
+template __global__ void k_check<COLL_DSMC>(int, int, uint32_t, const uint4 *, const uint4 *,
+                                             uint32_t *, uint8_t *, uint32_t, const uint32_t *,
+                                             uint32_t *);
+template __global__ void k_check<COLL_ESMC4>(int, int, uint32_t, const uint4 *, const uint4 *,
+                                              uint32_t *, uint8_t *, uint32_t, const uint32_t *,
+                                              uint32_t *);
 
 #define MCPM_INST_W(R)                                                                     \
   template __global__ void k_wpush<R, false>(const R *, R *, KParams, Kick, const double *, \
@@ -1957,25 +2048,25 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                              const int32_t *, const uint32_t *, int32_t *,  \
                                              int32_t *, uint32_t *, const uint32_t *,       \
                                              uint32_t *, uint32_t *, const uint32_t *,      \
-                                             uint32_t *, int);                              \
+                                             uint32_t *, int, R::real_type *);              \
   template __global__ void k_wpush<R, true>(const R *, R *, KParams, Kick, const double *,  \
                                             int, uint32_t, uint32_t, const uint32_t *,      \
                                             const int32_t *, const uint32_t *, int32_t *,   \
                                             int32_t *, uint32_t *, const uint32_t *,        \
                                             uint32_t *, uint32_t *, const uint32_t *,       \
-                                            uint32_t *, int);
+                                            uint32_t *, int, R::real_type *);
 
 #define MCPM_INST(R)                                                                       \
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
                                             Kick, const double *, int, uint32_t, uint32_t, \
                                             uint32_t *, const uint32_t *, uint32_t *, int, \
                                             int, int32_t, uint32_t *, const uint32_t *,    \
-                                            const uint32_t *, uint32_t *);                 \
+                                            const uint32_t *, uint32_t *, R::real_type *); \
   template __global__ void k_push<R, true>(const R *, R *, int64_t, int32_t, KParams,      \
                                            Kick, const double *, int, uint32_t, uint32_t,  \
                                            uint32_t *, const uint32_t *, uint32_t *, int,  \
                                            int, int32_t, uint32_t *, const uint32_t *,     \
-                                           const uint32_t *, uint32_t *);                  \
+                                           const uint32_t *, uint32_t *, R::real_type *);  \
   MCPM_INST_COLL(R, COLL_ESMC)                                                             \
   MCPM_INST_COLL(R, COLL_DSMC)                                                             \
   MCPM_INST_COLL(R, COLL_ESMC4)                                                            \

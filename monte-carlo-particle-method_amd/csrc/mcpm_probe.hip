@@ -1,6 +1,6 @@
 // Random-access rates of the device, measured on arrays shaped like the
-// collision schedule's: particle records and two interleaved 8-byte
-// reservation words per particle.  The reservation rounds are bound by
+// collision schedule's: particle records, one 4-byte priority word and one
+// mark byte per particle.  The reservation rounds are bound by
 // these rates (every access of an operation goes to an unpredictable
 // particle), not by streaming bandwidth, so the rates are the ceiling
 // their timings are set against.
@@ -43,21 +43,38 @@ __global__ void k_probe_gather_rec(const uint4 *__restrict__ a, uint32_t n, int 
   if (acc.x == 0x5a5a5a5au && acc.y == 0xa5a5a5a5u) sink[0] = acc;
 }
 
-// Gathers of one 8-byte word of 2 n.
-__global__ void k_probe_gather_word(const uint64_t *__restrict__ a, uint32_t n2, uint32_t nops,
-                                    uint64_t *__restrict__ sink) {
-  uint64_t acc = 0;
+// Gathers of one 4-byte word of n.
+__global__ void k_probe_gather_word(const uint32_t *__restrict__ a, uint32_t n, uint32_t nops,
+                                    uint32_t *__restrict__ sink) {
+  uint32_t acc = 0;
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nops;
        t += gridDim.x * blockDim.x)
-    acc ^= a[probe_index(t, n2)];
-  if (acc == 0x5a5a5a5a5a5a5a5aull) sink[0] = acc;
+    acc ^= a[probe_index(t, n)];
+  if (acc == 0x5a5a5a5au) sink[0] = acc;
 }
 
-// 64-bit atomic minimum into one word of 2 n, result unused.
-__global__ void k_probe_atomic_word(uint64_t *__restrict__ a, uint32_t n2, uint32_t nops) {
+// 32-bit atomic minimum into one word of n, result unused.
+__global__ void k_probe_atomic_word(uint32_t *__restrict__ a, uint32_t n, uint32_t nops) {
   for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nops;
        t += gridDim.x * blockDim.x)
-    atomicMin((unsigned long long *)&a[probe_index(t, n2)], (unsigned long long)t);
+    atomicMin(&a[probe_index(t, n)], t);
+}
+
+// Stores of one 4-byte word of n.
+__global__ void k_probe_scatter_word(uint32_t *__restrict__ a, uint32_t n, uint32_t nops) {
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nops;
+       t += gridDim.x * blockDim.x)
+    a[probe_index(t, n)] = t;
+}
+
+// Gathers of one byte of n.
+__global__ void k_probe_gather_byte(const uint8_t *__restrict__ a, uint32_t n, uint32_t nops,
+                                    uint32_t *__restrict__ sink) {
+  uint32_t acc = 0;
+  for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nops;
+       t += gridDim.x * blockDim.x)
+    acc += a[probe_index(t, n)];
+  if (acc == 0x5a5a5a5au) sink[0] = acc;
 }
 
 // Stores of whole records.
@@ -82,13 +99,16 @@ extern "C" int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes,
   const uint32_t n = (uint32_t)n_records, w = (uint32_t)record_bytes / 16u;
   const uint32_t nops = n < (1u << 24) ? n : (1u << 24);
   uint4 *rec = nullptr, *sink = nullptr;
-  uint64_t *words = nullptr;
+  uint32_t *words = nullptr;
+  uint8_t *bytes = nullptr;
   hipStream_t s = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int st = MCPM_OK;
-  double best[4] = {1e30, 1e30, 1e30, 1e30};
+  constexpr int NKIND = 6;
+  double best[NKIND] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
   if (hipMalloc((void **)&rec, (size_t)n * w * sizeof(uint4)) != hipSuccess ||
-      hipMalloc((void **)&words, (size_t)n * 2 * sizeof(uint64_t)) != hipSuccess ||
+      hipMalloc((void **)&words, (size_t)n * sizeof(uint32_t)) != hipSuccess ||
+      hipMalloc((void **)&bytes, (size_t)n) != hipSuccess ||
       hipMalloc((void **)&sink, 2 * sizeof(uint4)) != hipSuccess ||
       hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess) {
@@ -96,24 +116,30 @@ extern "C" int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes,
     goto done;
   }
   if (hipMemsetAsync(rec, 0x11, (size_t)n * w * sizeof(uint4), s) != hipSuccess ||
-      hipMemsetAsync(words, 0xFF, (size_t)n * 2 * sizeof(uint64_t), s) != hipSuccess) {
+      hipMemsetAsync(words, 0xFF, (size_t)n * sizeof(uint32_t), s) != hipSuccess ||
+      hipMemsetAsync(bytes, 0, (size_t)n, s) != hipSuccess) {
     st = MCPM_ERR_HIP;
     goto done;
   }
   // best of five launches of each kind, 8192 workgroups of 256
   for (int rep = 0; rep < 5 && st == MCPM_OK; ++rep) {
-    for (int kind = 0; kind < 4; ++kind) {
+    for (int kind = 0; kind < NKIND; ++kind) {
       hipEventRecord(e0, s);
       if (kind == 0)
         hipLaunchKernelGGL(k_probe_gather_rec, dim3(8192), dim3(256), 0, s, rec, n, (int)w, nops,
                            sink);
       else if (kind == 1)
-        hipLaunchKernelGGL(k_probe_gather_word, dim3(8192), dim3(256), 0, s, words, 2u * n, nops,
-                           (uint64_t *)sink);
+        hipLaunchKernelGGL(k_probe_gather_word, dim3(8192), dim3(256), 0, s, words, n, nops,
+                           (uint32_t *)sink);
       else if (kind == 2)
-        hipLaunchKernelGGL(k_probe_atomic_word, dim3(8192), dim3(256), 0, s, words, 2u * n, nops);
-      else
+        hipLaunchKernelGGL(k_probe_atomic_word, dim3(8192), dim3(256), 0, s, words, n, nops);
+      else if (kind == 3)
         hipLaunchKernelGGL(k_probe_scatter_rec, dim3(8192), dim3(256), 0, s, rec, n, (int)w, nops);
+      else if (kind == 4)
+        hipLaunchKernelGGL(k_probe_scatter_word, dim3(8192), dim3(256), 0, s, words, n, nops);
+      else
+        hipLaunchKernelGGL(k_probe_gather_byte, dim3(8192), dim3(256), 0, s, bytes, n, nops,
+                           (uint32_t *)sink);
       hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) {
         st = MCPM_ERR_HIP;
@@ -129,6 +155,8 @@ extern "C" int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes,
     out->gather_word = nops / (best[1] * 1e-3);
     out->atomic_min_word = nops / (best[2] * 1e-3);
     out->scatter_record = nops / (best[3] * 1e-3);
+    out->scatter_word = nops / (best[4] * 1e-3);
+    out->gather_byte = nops / (best[5] * 1e-3);
   }
 done:
   if (s) hipStreamSynchronize(s);
@@ -137,6 +165,7 @@ done:
   if (s) hipStreamDestroy(s);
   if (rec) hipFree(rec);
   if (words) hipFree(words);
+  if (bytes) hipFree(bytes);
   if (sink) hipFree(sink);
   return st;
 }

@@ -74,7 +74,8 @@ class CParams(ctypes.Structure):
 
 class CAccessRates(ctypes.Structure):
     _fields_ = [("gather_record", ctypes.c_double), ("gather_word", ctypes.c_double),
-                ("atomic_min_word", ctypes.c_double), ("scatter_record", ctypes.c_double)]
+                ("atomic_min_word", ctypes.c_double), ("scatter_record", ctypes.c_double),
+                ("scatter_word", ctypes.c_double), ("gather_byte", ctypes.c_double)]
 
 
 class CFieldStats(ctypes.Structure):
@@ -353,8 +354,9 @@ def energy_momentum_noalloc(u1_ptr: int, u2_ptr: int, u3_ptr: int, n: int, mass_
 
 def probe_random_access(n_records: int, record_bytes: int) -> dict:
     """Random-access rates of the current device (accesses per second) over
-    n_records records of record_bytes and 2 n_records 8-byte words: record
-    gathers, word gathers, 64-bit atomic minimums, record stores."""
+    n_records records of record_bytes, n_records 4-byte words and n_records
+    bytes: record gathers, word gathers, 32-bit atomic minimums, record
+    stores, word stores, byte gathers."""
     lib = load_library()
     r = CAccessRates()
     st = lib.mcpm_probe_random_access(int(n_records), int(record_bytes), ctypes.byref(r))
