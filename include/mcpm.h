@@ -380,6 +380,41 @@ int mcpm_energy_momentum_noalloc(const double *u1, const double *u2, const doubl
 int mcpm_track_ids(mcpm_ctx *ctx, int32_t enable);
 int mcpm_download_ids(mcpm_ctx *ctx, int32_t *ids);
 
+/* Particle weights that differ from cell to cell (the inverted and
+   evaporation problems' heavy liquid zones, myIO.cpp:1493-1525).
+   mcpm_set_cell_ratio: cells[num].n_ratio for every cell (n_cells integers
+   >= 1), which then enters each cell's density n = n_ratio Fn N / V
+   (Main_Functions.cpp:2588); null restores p.n_ratio everywhere.  Every
+   particle takes its cell's weight (main.cpp:239-242), now if the particles
+   are sorted, else at the next sort (and again after an upload or
+   initialisation); the sorts carry the weights with the particles.
+   mcpm_set_particle_ratio / mcpm_get_particle_ratio: the weights of the
+   particles in mcpm_download order.  Slab contexts without replicas. */
+int mcpm_set_cell_ratio(mcpm_ctx *ctx, const int32_t *ratio);
+int mcpm_set_particle_ratio(mcpm_ctx *ctx, const int32_t *w);
+int mcpm_get_particle_ratio(mcpm_ctx *ctx, int32_t *w);
+
+/* inverted_addremove_particles (Main_Functions.cpp:1206-1450) with gas
+   n_ratio = n_ratio on the sorted particles (after mcpm_cell_update or a
+   step): a particle heavier than its cell splits into n_ratio particles (its
+   n_ratio - 1 copies at its position, velocity + h N(0,1) per component,
+   h = 1.06 sig (n_rem + n_ratio)^(-1/5), sig and the mean velocity formed as
+   at :1302-1325), a lighter one is deleted with probability 1 - 1/n_ratio or
+   takes the cell's weight; then every particle has its cell's weight.  N
+   changes; the particles stay sorted, each cell's kept particles first and
+   its new ones after them.  Philox draws (p, copy, step).  MCPM_ERR_CAPACITY
+   (nothing changed) when the particles would exceed the capacity;
+   MCPM_ERR_STATE without cell weights or unsorted particles; MCPM_ERR_ARG
+   with identity tracking on. */
+typedef struct mcpm_addremove_stats {
+  int64_t before;   /* particles before */
+  int64_t split;    /* particles that split */
+  int64_t created;  /* new particles: split x (n_ratio - 1) */
+  int64_t deleted;  /* particles removed */
+  int64_t after;    /* particles after */
+} mcpm_addremove_stats;
+int mcpm_addremove(mcpm_ctx *ctx, int32_t n_ratio, int64_t step, mcpm_addremove_stats *stats);
+
 /* thermostat (Main_Functions.cpp:1674-1869), the inverted problem's branch
    (:1760-1830): after the cell moments are taken from the current particles
    (cell_update), every particle of a cell whose centre lies below zone_lo

@@ -71,7 +71,8 @@ enum {
   MCPM_TAG_DSMC    = 7,  /* plain DSMC / CBA_VHS draws                      */
   MCPM_TAG_NUDGE   = 8,  /* cell-face tie-break nudge                       */
   MCPM_TAG_CBA     = 9,  /* CBA_HS per-candidate draws                      */
-  MCPM_TAG_REINJECT = 10 /* evaporation boundary: re-insertion draws        */
+  MCPM_TAG_REINJECT = 10, /* evaporation boundary: re-insertion draws       */
+  MCPM_TAG_ADDREM  = 11   /* particle add/remove: deletion and copy draws   */
 };
 
 MCPM_HD uint32_t mcpm_ctr3(uint32_t tag, uint32_t step_hi, uint32_t sub) {

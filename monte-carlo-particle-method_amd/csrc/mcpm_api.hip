@@ -49,6 +49,16 @@ struct mcpm_ctx {
   // a particle since
   void *ys = nullptr;
   bool ys_valid = false;
+  // per-cell weights (device, nc; null: every cell has p.n_ratio), the
+  // particles' weights (ping-pong like P, carried by the sorts while
+  // track_w), and the work arrays of mcpm_addremove
+  int32_t *cratio = nullptr;
+  int32_t *pw[2] = {nullptr, nullptr};
+  bool track_w = false;
+  bool w_pending = false;  // weights from the cells at the next sort
+  uint8_t *ar_fl = nullptr;
+  uint32_t *ar_cst = nullptr, *ar_tot = nullptr, *ar_ostart = nullptr;
+  double *ar_h = nullptr;
   // windowed sort (particles a few cells from where they were): chunk bounds
   // and replicas (host-built when the particle counts change), windows, table
   // (chunks from WSPAN-cell blocks of the previous sort's cells, built on the
@@ -533,6 +543,18 @@ int chunk_for(int64_t n, int fixed) {
 // particles are still in the cell order of their positions, moved by a push
 // that adds nothing per step (no pressure face sums, no re-insertion), dense
 // enough that a chunk spans a few cells.
+// After a sort: particles whose weights are still to be set (new particles
+// with per-cell weights in force) take their cells' (main.cpp:239-242).
+int weights_after_sort(mcpm_ctx *c) {
+  if (!c->kp.cell_ratio || !c->w_pending) return MCPM_OK;
+  hipLaunchKernelGGL(k_w_from_cells, dim3((unsigned)c->nc), dim3(256), 0, c->s, c->start, c->nc,
+                     (const int32_t *)c->cratio, c->pw[c->cur]);
+  LAUNCHCHK();
+  c->w_pending = false;
+  c->track_w = true;
+  return MCPM_OK;
+}
+
 bool window_ok(const mcpm_ctx *c, int mode) {
   if (!c->w_enabled || c->p.dims == 2 || !c->wbeg || !c->order_sorted || c->w_fallbacks >= 3)
     return false;
@@ -566,13 +588,16 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
   using real = typename R::real_type;
   hipLaunchKernelGGL((k_wpush<R, false>), dim3(gw), dim3(256), 0, c->s, in, nullptr, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
-                     nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta, (real *)nullptr);
+                     nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta, (real *)nullptr,
+                     nullptr, nullptr);
   hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 3) / 4)), dim3(256), 0, c->s, c->wlo,
                      c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc);
   hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->cnt, c->start, c->nc);
   hipLaunchKernelGGL((k_wpush<R, true>), dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
-                     c->start, c->err, c->wflag, iid, oid, c->wdelta, (real *)c->ys);
+                     c->start, c->err, c->wflag, iid, oid, c->wdelta, (real *)c->ys,
+                     c->track_w ? c->pw[c->cur] : nullptr,
+                     c->track_w ? c->pw[1 - c->cur] : nullptr);
   LAUNCHCHK();
   uint32_t flag = 0;
   HIPCHK(hipMemcpyAsync(&flag, c->wflag, sizeof flag, hipMemcpyDeviceToHost, c->s));
@@ -590,7 +615,7 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
   c->sorted = true;
   c->order_sorted = true;
   c->ys_valid = c->ys != nullptr;
-  return MCPM_OK;
+  return weights_after_sort(c);
 }
 
 template <typename R>
@@ -629,12 +654,13 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   // one stable counting-sort pass over nb bins: histogram, scans, scatter
   auto pass = [&](const R *in, R *out, int pmode, const Kick &ks, int nb, int kmode,
                   uint32_t *cnt, uint32_t *start, double *nden, double *yc, uint32_t *okey,
-                  const uint32_t *ikey, const uint32_t *iid, uint32_t *oid, real *oys) -> int {
+                  const uint32_t *ikey, const uint32_t *iid, uint32_t *oid, real *oys,
+                  const int32_t *iw, int32_t *ow) -> int {
     const size_t lds = (size_t)nb * sizeof(uint32_t);
     hipLaunchKernelGGL((k_push<R, false>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
                        c->s, in, nullptr, c->n, chunk, c->kp, kick, drift, pmode, slo, shi, c->table,
                        nullptr, c->err, nb, kmode, c->kdiv, nullptr, ikey, nullptr, nullptr,
-                       (real *)nullptr);
+                       (real *)nullptr, nullptr, nullptr);
     LAUNCHCHK();
     dim3 gs((nb + 255) / 256, nseg);
     hipLaunchKernelGGL(k_scan_a, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
@@ -647,7 +673,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
     hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
                        c->s, in, out,
                        c->n, chunk, c->kp, ks, drift, pmode, slo, shi, c->table, start,
-                       c->err, nb, kmode, c->kdiv, okey, ikey, iid, oid, oys);
+                       c->err, nb, kmode, c->kdiv, okey, ikey, iid, oid, oys, iw, ow);
     LAUNCHCHK();
     return MCPM_OK;
   };
@@ -655,20 +681,22 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   R *out = (R *)c->P[1 - c->cur];
   uint32_t *id_in = c->track_ids ? c->ids[c->cur] : nullptr;
   uint32_t *id_out = c->track_ids ? c->ids[1 - c->cur] : nullptr;
+  int32_t *w_in = c->track_w ? c->pw[c->cur] : nullptr;
+  int32_t *w_out = c->track_w ? c->pw[1 - c->cur] : nullptr;
   int st;
   if (!c->two_digit) {
     if ((st = pass(in, out, mode, kick_scatter, nc, 0, c->cnt, c->start, c->nden, c->yc,
-                   nullptr, nullptr, id_in, id_out, (real *)c->ys)))
+                   nullptr, nullptr, id_in, id_out, (real *)c->ys, w_in, w_out)))
       return st;
   } else {
     // low digit with the push (keys carried to the destinations), then the
     // high digit of those keys back into the first buffer; cells from the
     // sorted keys
     if ((st = pass(in, out, mode, kick_scatter, c->nb_lo, 1, c->dcnt, c->dstart, nullptr,
-                   nullptr, c->keys_tmp, nullptr, id_in, id_out, nullptr)))
+                   nullptr, c->keys_tmp, nullptr, id_in, id_out, nullptr, w_in, w_out)))
       return st;
     if ((st = pass(out, (R *)in, 0, kick, c->nb_hi, 2, c->dcnt, c->dstart, nullptr, nullptr,
-                   c->keys, c->keys_tmp, id_out, id_in, nullptr)))
+                   c->keys, c->keys_tmp, id_out, id_in, nullptr, w_out, w_in)))
       return st;
     hipLaunchKernelGGL(k_fill_u32, dim3(64), dim3(256), 0, c->s, c->start, (int64_t)nc + 1, 0u);
     hipLaunchKernelGGL(k_cell_bounds, dim3(2048), dim3(256), 0, c->s, c->keys, c->n, nc,
@@ -679,13 +707,13 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
     c->sorted = true;
     c->order_sorted = true;
     c->keys_current = true;
-    return MCPM_OK;
+    return weights_after_sort(c);
   }
   c->cur = 1 - c->cur;
   c->sorted = true;
   c->order_sorted = true;
   c->ys_valid = c->ys != nullptr;
-  return MCPM_OK;
+  return weights_after_sort(c);
 }
 
 int run_sort(mcpm_ctx *c, int mode, int64_t step) {
@@ -1132,8 +1160,72 @@ int launch_init(mcpm_ctx *c, const InitParams &ip) {
     hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], ip.n);
   c->order_sorted = false;
   c->ys_valid = false;
+  if (c->kp.cell_ratio) {
+    c->track_w = false;
+    c->w_pending = true;
+  }
   LAUNCHCHK();
   return MCPM_OK;
+}
+
+// inverted_addremove_particles on the sorted slab (mcpm_addremove.hip).
+template <typename R>
+int addremove_t(mcpm_ctx *c, int32_t nr, int64_t step, mcpm_addremove_stats *stats) {
+  if constexpr (R::dims != 1) {
+    return MCPM_ERR_ARG;
+  } else {
+    using real = typename R::real_type;
+    const int nc = c->nc;
+    const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+    const R *P = (const R *)c->P[c->cur];
+    R *out = (R *)c->P[1 - c->cur];
+    uint32_t *tot = c->ar_tot, *ostart = c->ar_ostart;
+    if (nc > 0)
+      hipLaunchKernelGGL((k_ar_classify<R>), dim3((unsigned)nc), dim3(256), 0, c->s, P, c->cnt,
+                         c->start, (const int32_t *)c->pw[c->cur], (const int32_t *)c->cratio,
+                         nr, c->kp, slo, shi, c->ar_fl, c->ar_cst, c->ar_h, tot);
+    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, tot, ostart, nc);
+    LAUNCHCHK();
+    uint32_t total = 0;
+    HIPCHK(hipMemcpyAsync(&total, ostart + nc, sizeof total, hipMemcpyDeviceToHost, c->s));
+    std::vector<uint32_t> cst;
+    if (stats) {
+      cst.resize(4 * (size_t)nc);
+      HIPCHK(hipMemcpyAsync(cst.data(), c->ar_cst, cst.size() * sizeof(uint32_t),
+                            hipMemcpyDeviceToHost, c->s));
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    if ((int64_t)total > c->p.capacity) return MCPM_ERR_CAPACITY;  // nothing changed yet
+    if (nc > 0)
+      hipLaunchKernelGGL((k_ar_scatter<R>), dim3((unsigned)nc), dim3(256), 0, c->s, P, out,
+                         c->cnt, c->start, (const uint32_t *)ostart, (const uint8_t *)c->ar_fl,
+                         (const uint32_t *)c->ar_cst, (const double *)c->ar_h,
+                         (const int32_t *)c->cratio, nr, c->kp, slo, shi, c->pw[1 - c->cur],
+                         (real *)c->ys);
+    HIPCHK(hipMemcpyAsync(c->start, ostart, ((size_t)nc + 1) * sizeof(uint32_t),
+                          hipMemcpyDeviceToDevice, c->s));
+    hipLaunchKernelGGL(k_cell_fill, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->s,
+                       c->start, nc, c->cnt, c->nden, c->yc, c->kp);
+    LAUNCHCHK();
+    if (stats) {
+      int64_t split = 0, del = 0;
+      for (int q = 0; q < nc; ++q) {
+        split += cst[4 * (size_t)q];
+        del += cst[4 * (size_t)q + 2];
+      }
+      stats->before = c->n;
+      stats->split = split;
+      stats->created = split * (int64_t)(nr - 1);
+      stats->deleted = del;
+      stats->after = (int64_t)total;
+    }
+    c->n = (int64_t)total;
+    c->cur = 1 - c->cur;
+    c->ys_valid = c->ys != nullptr;
+    c->keys_current = false;
+    c->have_closures = false;
+    return MCPM_OK;
+  }
 }
 
 extern "C" {
@@ -1227,6 +1319,7 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
   k.key0 = (uint32_t)p->seed;
   k.key1 = (uint32_t)(p->seed >> 32);
   k.nrep = 0;
+  k.cell_ratio = nullptr;
   if (nrep > 0) {
     // per-replica constants: own cells, dy, volume, Fn and key; the rest shared
     c->rkp_host.resize(nrep);
@@ -1444,7 +1537,8 @@ int mcpm_destroy(mcpm_ctx *c) {
   if (c->s) hipStreamSynchronize(c->s);
   void *bufs[] = {c->rkp, c->cell_rep, c->rcell_off, c->rpart_off, c->rprio, c->rcand_off,
                   c->rfa,
-                  c->P[0], c->P[1], c->ys, c->table, c->partial, c->dcnt, c->dstart, c->keys,
+                  c->P[0], c->P[1], c->ys, c->cratio, c->pw[0], c->ar_fl, c->ar_cst,
+                  c->ar_h, c->table, c->partial, c->dcnt, c->dstart, c->keys,
                   c->keys_tmp,
                   c->cnt, c->start, c->mcand,
                   c->cstart, c->nden, c->yc, c->omax, c->omega_new, c->F2, c->moments,
@@ -1512,6 +1606,10 @@ int upload_any(mcpm_ctx *c, int64_t n, const double *const *src, int nsrc) {
   c->kick_pending = false;
   c->order_sorted = false;
   c->ys_valid = false;
+  if (c->kp.cell_ratio) {
+    c->track_w = false;
+    c->w_pending = true;
+  }
   if (c->track_ids && n > 0) {
     hipLaunchKernelGGL(k_iota_u32, dim3(1024), dim3(256), 0, c->s, c->ids[c->cur], n);
     LAUNCHCHK();
@@ -2015,6 +2113,74 @@ int mcpm_download_ids(mcpm_ctx *c, int32_t *ids) {
   if (c->n > 0)
     HIPCHK(hipMemcpy(ids, c->ids[c->cur], (size_t)c->n * sizeof(uint32_t), hipMemcpyDeviceToHost));
   return MCPM_OK;
+}
+
+int mcpm_set_cell_ratio(mcpm_ctx *c, const int32_t *ratio) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims != 1 || c->kp.nrep) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (!ratio) {  // every cell back to p.n_ratio
+    c->kp.cell_ratio = nullptr;
+    c->track_w = false;
+    c->w_pending = false;
+  } else {
+    const int nc = c->nc;
+    for (int q = 0; q < nc; ++q)
+      if (ratio[q] < 1) return MCPM_ERR_ARG;
+    int st;
+    if (!c->cratio) {
+      const size_t cap = (size_t)c->p.capacity;
+      if ((st = dalloc(&c->cratio, (size_t)nc))) return st;
+      if ((st = dalloc(&c->pw[0], 2 * cap))) return st;
+      c->pw[1] = c->pw[0] + cap;
+      if ((st = dalloc(&c->ar_fl, cap))) return st;
+      if ((st = dalloc(&c->ar_cst, 4 * (size_t)nc + 2 * ((size_t)nc + 1)))) return st;
+      c->ar_tot = c->ar_cst + 4 * (size_t)nc;
+      c->ar_ostart = c->ar_tot + ((size_t)nc + 1);
+      if ((st = dalloc(&c->ar_h, (size_t)nc))) return st;
+    }
+    HIPCHK(hipMemcpy(c->cratio, ratio, (size_t)nc * sizeof(int32_t), hipMemcpyHostToDevice));
+    c->kp.cell_ratio = c->cratio;
+    c->w_pending = true;
+    c->track_w = false;
+    if (c->sorted) {
+      // every particle takes its cell's weight, the densities the new weights
+      if ((st = weights_after_sort(c))) return st;
+      hipLaunchKernelGGL(k_cell_fill, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, c->s,
+                         c->start, nc, c->cnt, c->nden, c->yc, c->kp);
+      LAUNCHCHK();
+    }
+  }
+  return MCPM_OK;
+}
+
+int mcpm_set_particle_ratio(mcpm_ctx *c, const int32_t *w) {
+  if (!c || (!w && c->n > 0)) return MCPM_ERR_ARG;
+  if (!c->kp.cell_ratio) return MCPM_ERR_STATE;
+  for (int64_t q = 0; q < c->n; ++q)
+    if (w[q] < 1) return MCPM_ERR_ARG;
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (c->n > 0)
+    HIPCHK(hipMemcpy(c->pw[c->cur], w, (size_t)c->n * sizeof(int32_t), hipMemcpyHostToDevice));
+  c->track_w = true;
+  c->w_pending = false;
+  return MCPM_OK;
+}
+
+int mcpm_get_particle_ratio(mcpm_ctx *c, int32_t *w) {
+  if (!c || (!w && c->n > 0)) return MCPM_ERR_ARG;
+  if (!c->track_w) return MCPM_ERR_STATE;
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (c->n > 0)
+    HIPCHK(hipMemcpy(w, c->pw[c->cur], (size_t)c->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+  return MCPM_OK;
+}
+
+int mcpm_addremove(mcpm_ctx *c, int32_t n_ratio, int64_t step, mcpm_addremove_stats *stats) {
+  if (!c || n_ratio < 1) return MCPM_ERR_ARG;
+  if (c->p.dims != 1 || c->kp.nrep || c->track_ids) return MCPM_ERR_ARG;
+  if (!c->kp.cell_ratio || !c->track_w || !c->sorted) return MCPM_ERR_STATE;
+  return MCPM_DISPATCH(c, addremove_t, c, n_ratio, step, stats);
 }
 
 int mcpm_thermostat(mcpm_ctx *c, double zone_lo, double T_lo, double zone_hi, double T_hi) {

@@ -70,7 +70,15 @@ struct KParams {
   const int64_t *rpart_off;   // nrep + 1
   const uint32_t *rprio;      // candidate priority key of the step, per replica
   const uint32_t *rcand_off;  // first candidate of each replica in this step, nrep + 1
+  // per-cell particle weights (cells[num].n_ratio) when they differ from
+  // cell to cell, else null and every cell has n_ratio
+  const int32_t *cell_ratio;
 };
+
+// The weight of cell c's particles in its density n = n_ratio Fn N / V.
+__host__ __device__ __forceinline__ double cell_weight(const KParams &kp, int32_t c) {
+  return kp.cell_ratio ? (double)kp.cell_ratio[c] : kp.n_ratio;
+}
 
 // Replica of particle p (batched contexts): the last r with rpart_off[r] <= p.
 __device__ __forceinline__ int32_t rep_of_particle(const KParams &kp, int64_t p) {
@@ -145,6 +153,31 @@ __device__ __forceinline__ void atomic_max_pos_double(double *addr, double v) {
   if (v > 0.0 &&
       v > __hip_atomic_load(addr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
     atomicMax((unsigned long long *)addr, (unsigned long long)mcpm_dbits(v));
+}
+
+// Sum of v over the workgroup in one fixed order: each wavefront folds its
+// lanes pairwise (lane l adds lane l + off, off = 32, 16, ..., 1), then the
+// wavefront totals are added in turn to 0; sh holds K x (blockDim / 64)
+// doubles.  Every thread must call it.
+template <int K>
+__device__ __forceinline__ void block_sum(double (&v)[K], double *sh /* [K][blockDim/64] */) {
+  const int lane = lane_id();
+  const int wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double x = v[k];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
+    if (lane == 0) sh[k * nw + wid] = x;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    double x = 0.0;
+    for (int w = 0; w < nw; ++w) x += sh[k * nw + w];
+    v[k] = x;
+  }
+  __syncthreads();
 }
 
 }  // namespace mcpm

@@ -68,7 +68,8 @@ __global__ void k_push(const R *in, R *out, int64_t n, int32_t chunk, KParams kp
                        uint32_t step_lo, uint32_t step_hi, uint32_t *table,
                        const uint32_t *start, uint32_t *err, int nb, int kmode, int32_t kdiv,
                        uint32_t *okey, const uint32_t *ikey, const uint32_t *iid,
-                       uint32_t *oid, typename R::real_type *oys);
+                       uint32_t *oid, typename R::real_type *oys, const int32_t *iw,
+                       int32_t *ow);
 // The windowed sort of particles that move a few cells per step: WCH
 // particles per chunk, histograms over at most WMAX cells, WDELTA cells of
 // slack either side of the chunk's old cells.
@@ -86,7 +87,7 @@ __global__ void k_wpush(const R *in, R *out, KParams kp, Kick kick, const double
                         const int32_t *wrep, const uint32_t *nchp, int32_t *wlo, int32_t *wlen,
                         uint32_t *wtab, const uint32_t *start, uint32_t *err, uint32_t *wflag,
                         const uint32_t *iid, uint32_t *oid, int wdelta,
-                        typename R::real_type *oys);
+                        typename R::real_type *oys, const int32_t *iw, int32_t *ow);
 __global__ void k_wblocks(const int32_t *bcell, int nblk, const uint32_t *start,
                           uint32_t *pieces);
 __global__ void k_wfill(const int32_t *bcell, const int32_t *brep, int nblk,
@@ -229,6 +230,21 @@ template <typename real>
 __global__ void k_kde(const Rec<real> *P, const uint32_t *cnt, const uint32_t *start, double *ns,
                       double *d2, double *d3, KParams kp);
 __global__ void k_lr2_force(const double *d2, double *F2, KParams kp, double pref);
+
+// ---- particle add/remove (mcpm_addremove.hip) --------------------------------
+enum : uint8_t { AR_KEEP = 0, AR_SPLIT = 1, AR_DELETE = 2 };
+template <typename R>
+__global__ void k_ar_classify(const R *P, const uint32_t *cnt, const uint32_t *start,
+                              const int32_t *pw, const int32_t *cr, int32_t nr, KParams kp,
+                              uint32_t slo, uint32_t shi, uint8_t *fl, uint32_t *cst,
+                              double *ch, uint32_t *tot);
+template <typename R>
+__global__ void k_ar_scatter(const R *P, R *out, const uint32_t *cnt, const uint32_t *start,
+                             const uint32_t *ostart, const uint8_t *fl, const uint32_t *cst,
+                             const double *ch, const int32_t *cr, int32_t nr, KParams kp,
+                             uint32_t slo, uint32_t shi, int32_t *ow,
+                             typename R::real_type *oys);
+__global__ void k_w_from_cells(const uint32_t *start, int nc, const int32_t *cr, int32_t *w);
 
 // ---- the plane's mean field (mcpm_fem.hip) ----------------------------------
 // Constants of the nodal deposit and the Q1 screened-Poisson solve.

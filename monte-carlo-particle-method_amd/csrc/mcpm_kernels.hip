@@ -290,7 +290,8 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
     uint32_t *__restrict__ table, const uint32_t *__restrict__ start, uint32_t *err, int nb,
     int kmode, int32_t kdiv, uint32_t *__restrict__ okey, const uint32_t *__restrict__ ikey,
     const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid,
-    typename R::real_type *__restrict__ oys) {
+    typename R::real_type *__restrict__ oys, const int32_t *__restrict__ iw,
+    int32_t *__restrict__ ow) {
   extern __shared__ uint32_t lds_cnt[];
   constexpr int TILE = 64 * PUSH_WAVES;  // particles of one tile, wavefront w the w-th 64
   const int nc = nb;
@@ -410,6 +411,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
                 if (okey) okey[dst] = okv;
                 if (oid) oid[dst] = iid[p];
                 if (oys) oys[dst] = r.y;
+                if (ow) ow[dst] = iw[p];
               } else {
                 atomicOr(err, ERRBIT_ORDER);  // the two passes disagreed on a key
               }
@@ -486,7 +488,8 @@ __global__ __launch_bounds__(256) void k_wpush(
     const uint32_t *__restrict__ nchp, int32_t *__restrict__ wlo, int32_t *__restrict__ wlen,
     uint32_t *__restrict__ wtab, const uint32_t *__restrict__ start, uint32_t *__restrict__ err,
     uint32_t *__restrict__ wflag, const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid,
-    int wdelta, typename R::real_type *__restrict__ oys) {
+    int wdelta, typename R::real_type *__restrict__ oys, const int32_t *__restrict__ iw,
+    int32_t *__restrict__ ow) {
   __shared__ uint32_t lcnt[4][WMAX];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nch = (int)*nchp;
@@ -593,6 +596,7 @@ __global__ __launch_bounds__(256) void k_wpush(
             out[dst] = r;
             if (oid) oid[dst] = iid[p];
             if (oys) oys[dst] = r.y;
+            if (ow) ow[dst] = iw[p];
           } else {
             atomicOr(wflag, 1u);  // counts out of step with the particles
           }
@@ -737,7 +741,7 @@ __global__ __launch_bounds__(256) void k_wscan(const int32_t *__restrict__ wlo,
   }
   if (lane == 0) {
     cnt[k] = run;
-    const double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
+    const double nd = cell_weight(kq, k) * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
     nden[k] = nd;
     yc[k] = contact_y(nd, kq.b, kq.y_model);
   }
@@ -772,7 +776,7 @@ __global__ void k_scan_bins(uint32_t *__restrict__ partial, int nseg, int nc,
   cnt[c] = run;
   if (nden) {
     const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
-    double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
+    double nd = cell_weight(kq, c) * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
     nden[c] = nd;
     yc[c] = contact_y(nd, kq.b, kq.y_model);
   }
@@ -803,7 +807,7 @@ __global__ void k_cell_fill(const uint32_t *__restrict__ start, int nc,
   const uint32_t run = start[c + 1] >= start[c] ? start[c + 1] - start[c] : 0u;
   cnt[c] = run;
   const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
-  double nd = kq.n_ratio * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
+  double nd = cell_weight(kq, c) * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
   nden[c] = nd;
   yc[c] = contact_y(nd, kq.b, kq.y_model);
 }
@@ -1813,32 +1817,12 @@ __global__ __launch_bounds__(1024) void k_field(const double *__restrict__ nden,
   }
 }
 
+
 // ---------------------------------------------------------------------------
 // Per-cell moments (cell_update, Main_Functions.cpp:2461-2593) and the
 // time-average sums of cell_poly_velocity (Main_Functions.cpp:1918-1968).
 // One workgroup per cell; FP64 accumulation; fixed-order tree reductions.
 // ---------------------------------------------------------------------------
-template <int K>
-__device__ __forceinline__ void block_sum(double (&v)[K], double *sh /* [K][blockDim/64] */) {
-  const int lane = lane_id();
-  const int wid = threadIdx.x >> 6;
-  const int nw = blockDim.x >> 6;
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double x = v[k];
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off);
-    if (lane == 0) sh[k * nw + wid] = x;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) {
-    double x = 0.0;
-    for (int w = 0; w < nw; ++w) x += sh[k * nw + w];
-    v[k] = x;
-  }
-  __syncthreads();
-}
-
 template <typename R>
 __global__ __launch_bounds__(256) void k_moments(const R *__restrict__ P,
                                                  const uint32_t *__restrict__ cnt,
@@ -2048,25 +2032,29 @@ template __global__ void k_check<COLL_ESMC4>(int, int, uint32_t, const uint4 *, 
                                              const int32_t *, const uint32_t *, int32_t *,  \
                                              int32_t *, uint32_t *, const uint32_t *,       \
                                              uint32_t *, uint32_t *, const uint32_t *,      \
-                                             uint32_t *, int, R::real_type *);              \
+                                             uint32_t *, int, R::real_type *,               \
+                                             const int32_t *, int32_t *);                   \
   template __global__ void k_wpush<R, true>(const R *, R *, KParams, Kick, const double *,  \
                                             int, uint32_t, uint32_t, const uint32_t *,      \
                                             const int32_t *, const uint32_t *, int32_t *,   \
                                             int32_t *, uint32_t *, const uint32_t *,        \
                                             uint32_t *, uint32_t *, const uint32_t *,       \
-                                            uint32_t *, int, R::real_type *);
+                                            uint32_t *, int, R::real_type *,                \
+                                            const int32_t *, int32_t *);
 
 #define MCPM_INST(R)                                                                       \
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \
                                             Kick, const double *, int, uint32_t, uint32_t, \
                                             uint32_t *, const uint32_t *, uint32_t *, int, \
                                             int, int32_t, uint32_t *, const uint32_t *,    \
-                                            const uint32_t *, uint32_t *, R::real_type *); \
+                                            const uint32_t *, uint32_t *, R::real_type *,  \
+                                            const int32_t *, int32_t *);                   \
   template __global__ void k_push<R, true>(const R *, R *, int64_t, int32_t, KParams,      \
                                            Kick, const double *, int, uint32_t, uint32_t,  \
                                            uint32_t *, const uint32_t *, uint32_t *, int,  \
                                            int, int32_t, uint32_t *, const uint32_t *,     \
-                                           const uint32_t *, uint32_t *, R::real_type *);  \
+                                           const uint32_t *, uint32_t *, R::real_type *,   \
+                                           const int32_t *, int32_t *);                    \
   MCPM_INST_COLL(R, COLL_ESMC)                                                             \
   MCPM_INST_COLL(R, COLL_DSMC)                                                             \
   MCPM_INST_COLL(R, COLL_ESMC4)                                                            \

@@ -78,6 +78,15 @@ class CAccessRates(ctypes.Structure):
                 ("scatter_word", ctypes.c_double), ("gather_byte", ctypes.c_double)]
 
 
+class CAddRemoveStats(ctypes.Structure):
+    _fields_ = [("before", ctypes.c_int64), ("split", ctypes.c_int64),
+                ("created", ctypes.c_int64), ("deleted", ctypes.c_int64),
+                ("after", ctypes.c_int64)]
+
+    def as_dict(self):
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 class CFieldStats(ctypes.Structure):
     _fields_ = [("residual", ctypes.c_double), ("cg_iterations", ctypes.c_int32),
                 ("used_network", ctypes.c_int32)]
@@ -263,6 +272,10 @@ def load_library() -> ctypes.CDLL:
         "mcpm_collide_esmc_new4": (i32, [vp, i64, ctypes.POINTER(CStats)]),
         "mcpm_track_ids": (i32, [vp, i32]),
         "mcpm_download_ids": (i32, [vp, ctypes.POINTER(i32)]),
+        "mcpm_set_cell_ratio": (i32, [vp, ctypes.POINTER(i32)]),
+        "mcpm_set_particle_ratio": (i32, [vp, ctypes.POINTER(i32)]),
+        "mcpm_get_particle_ratio": (i32, [vp, ctypes.POINTER(i32)]),
+        "mcpm_addremove": (i32, [vp, i32, i64, ctypes.POINTER(CAddRemoveStats)]),
         "mcpm_poisson_solver": (i32, [vp]),
         "mcpm_kick": (i32, [vp]),
         "mcpm_stream_bc": (i32, [vp, i64, i32]),
@@ -507,6 +520,33 @@ class Simulation:
         out = np.empty(int(self._lib.mcpm_particle_count(self._h)), dtype=np.int32)
         self._check(self._lib.mcpm_download_ids(self._h, _i32p(out)), "mcpm_download_ids")
         return out
+
+    def set_cell_ratio(self, ratio):
+        """Per-cell particle weights cells[num].n_ratio (None: p.n_ratio
+        everywhere); every particle takes its cell's weight."""
+        if ratio is None:
+            self._check(self._lib.mcpm_set_cell_ratio(self._h, None), "mcpm_set_cell_ratio")
+            return
+        r = np.ascontiguousarray(ratio, dtype=np.int32)
+        self._check(self._lib.mcpm_set_cell_ratio(self._h, _i32p(r)), "mcpm_set_cell_ratio")
+
+    def set_particle_ratio(self, w):
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        self._check(self._lib.mcpm_set_particle_ratio(self._h, _i32p(w)),
+                    "mcpm_set_particle_ratio")
+
+    def particle_ratio(self) -> np.ndarray:
+        out = np.empty(int(self._lib.mcpm_particle_count(self._h)), dtype=np.int32)
+        self._check(self._lib.mcpm_get_particle_ratio(self._h, _i32p(out)),
+                    "mcpm_get_particle_ratio")
+        return out
+
+    def addremove(self, n_ratio: int, step: int) -> dict:
+        """inverted_addremove_particles with gas n_ratio = n_ratio."""
+        s = CAddRemoveStats()
+        self._check(self._lib.mcpm_addremove(self._h, int(n_ratio), int(step), ctypes.byref(s)),
+                    "mcpm_addremove")
+        return s.as_dict()
 
     def collide_esmc_new4(self, step: int) -> dict:
         s = CStats()

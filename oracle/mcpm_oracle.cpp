@@ -116,6 +116,10 @@ struct Oracle {
   // Fokker-Planck closures of the last orc_closures call
   std::vector<double> mom, clos;
   bool have_closures = false;
+  // per-cell weights cells[num].n_ratio (empty: p.n_ratio everywhere) and
+  // the particles' weights, carried by the sort while track_w
+  std::vector<int32_t> cr, pw;
+  bool track_w = false, w_pending = false;
 };
 
 template <typename real>
@@ -232,6 +236,16 @@ int orc_mid(const Oracle &o, int64_t i, int64_t j) {
                   o.nc);
 }
 
+// n = n_ratio Fn weight / volume (Main_Functions.cpp:2588, 2601) with the
+// cell's own n_ratio where the cells have weights, and Y(n).
+void orc_densities(Oracle &o) {
+  for (int c = 0; c < o.nc; ++c) {
+    const double w = o.cr.empty() ? o.p.n_ratio : (double)o.cr[c];
+    o.nden[c] = w * o.p.Fn * (double)o.cnt[c] / o.p.cell_volume;
+    o.yc[c] = orc_y(o.nden[c], o.p);
+  }
+}
+
 // Stable counting sort of the particles by cell (cell_update binning and
 // index lists, Main_Functions.cpp:2231-2376).
 void orc_sort(Oracle &o, bool check) {
@@ -254,15 +268,22 @@ void orc_sort(Oracle &o, bool check) {
   o.start[nc] = run;
   std::vector<int> fill(o.start.begin(), o.start.end() - 1);
   std::vector<double> out(o.rec.size());
+  std::vector<int32_t> wout(o.track_w ? (size_t)o.n : 0);
   for (int64_t k = 0; k < o.n; ++k) {
     int d = fill[key[k]]++;
     memcpy(&out[(size_t)o.S * d], &o.rec[(size_t)o.S * k], o.S * sizeof(double));
+    if (o.track_w) wout[d] = o.pw[k];
   }
   o.rec.swap(out);
-  for (int c = 0; c < nc; ++c) {
-    // n = n_ratio Fn weight / volume (Main_Functions.cpp:2601)
-    o.nden[c] = o.p.n_ratio * o.p.Fn * (double)o.cnt[c] / o.p.cell_volume;
-    o.yc[c] = orc_y(o.nden[c], o.p);
+  if (o.track_w) o.pw.swap(wout);
+  orc_densities(o);
+  if (!o.cr.empty() && o.w_pending) {
+    // the particles take their cells' weights (main.cpp:239-242)
+    o.pw.assign((size_t)o.n, 0);
+    for (int c = 0; c < nc; ++c)
+      for (int u = 0; u < o.cnt[c]; ++u) o.pw[(size_t)o.start[c] + u] = o.cr[c];
+    o.track_w = true;
+    o.w_pending = false;
   }
   o.sorted = true;
 }
@@ -2110,6 +2131,10 @@ int orc_upload(void *h, int64_t n, const double *y, const double *u1, const doub
   }
   o.sorted = false;
   o.kick_pending = false;
+  if (!o.cr.empty()) {
+    o.track_w = false;
+    o.w_pending = true;
+  }
   return MCPM_OK;
 }
 
@@ -2189,6 +2214,10 @@ int orc_initialize(void *h, const mcpm_settings *s) {
   o.sorted = false;
   o.kick_pending = false;
   o.have_closures = false;
+  if (!o.cr.empty()) {
+    o.track_w = false;
+    o.w_pending = true;
+  }
   return orc_cell_update(h);
 }
 
@@ -2445,6 +2474,201 @@ int orc_get_escaped(void *h, int64_t *count) {
 // thermostat, inverted problem (Main_Functions.cpp:1760-1830): cells by cell
 // centre, lambda = sqrt(T_target / cells[num].T), U = lambda (U - U_space) +
 // U_space for every particle of the cell (cells with T = 0 are skipped).
+/* Per-cell weights (cells[num].n_ratio); null: p.n_ratio everywhere. */
+int orc_set_cell_ratio(void *h, const int32_t *ratio) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 1) return MCPM_ERR_ARG;
+  if (!ratio) {
+    o.cr.clear();
+    o.track_w = o.w_pending = false;
+    if (o.sorted) orc_densities(o);
+    return MCPM_OK;
+  }
+  for (int c = 0; c < o.nc; ++c)
+    if (ratio[c] < 1) return MCPM_ERR_ARG;
+  o.cr.assign(ratio, ratio + o.nc);
+  o.track_w = false;
+  o.w_pending = true;
+  if (o.sorted) {
+    o.pw.assign((size_t)o.n, 0);
+    for (int c = 0; c < o.nc; ++c)
+      for (int u = 0; u < o.cnt[c]; ++u) o.pw[(size_t)o.start[c] + u] = o.cr[c];
+    o.track_w = true;
+    o.w_pending = false;
+    orc_densities(o);
+  }
+  return MCPM_OK;
+}
+
+int orc_set_particle_ratio(void *h, const int32_t *w) {
+  Oracle &o = *(Oracle *)h;
+  if (o.cr.empty()) return MCPM_ERR_STATE;
+  for (int64_t k = 0; k < o.n; ++k)
+    if (w[k] < 1) return MCPM_ERR_ARG;
+  o.pw.assign(w, w + o.n);
+  o.track_w = true;
+  o.w_pending = false;
+  return MCPM_OK;
+}
+
+int orc_get_particle_ratio(void *h, int32_t *w) {
+  Oracle &o = *(Oracle *)h;
+  if (!o.track_w) return MCPM_ERR_STATE;
+  for (int64_t k = 0; k < o.n; ++k) w[k] = o.pw[k];
+  return MCPM_OK;
+}
+
+/* inverted_addremove_particles (Main_Functions.cpp:1206-1450) on the sorted
+ * slab, cell by cell.  Per particle: heavier than its cell -> it splits
+ * (:1254-1259); lighter -> deleted when the draw (p, 0, TAG_ADDREM, 0) is
+ * below 1 - 1/n_ratio (:1262-1268), else it takes the cell's weight
+ * (:1270-1273).  Per cell with splits: the reference's mean velocity, the
+ * last kept particle's velocity times its weight over the cell weight
+ * (:1302-1313), sig^2 = sum of weight |U - mean|^2 over the kept / weight
+ * (:1314-1325, summed in WgSum's order), h = 1.06 sig (n_rem +
+ * n_ratio)^(-1/5) (:1343) by the deterministic exp and log; copy j of a
+ * split particle at its position with U + h (z0, z1, z2) from the block
+ * (p, j, TAG_ADDREM, 1) (:1344-1352).  Output per cell: the kept particles
+ * in order, then the copies by parent and j, all with the cell's weight
+ * (:1435-1441). */
+typedef struct orc_addremove_stats {
+  int64_t before, split, created, deleted, after;
+} orc_addremove_stats;
+
+}  // extern "C"
+
+template <typename real>
+int orc_addremove_t(Oracle &o, int32_t nr, int64_t step, orc_addremove_stats *st) {
+  const int nc = o.nc;
+  const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
+  const uint32_t key0 = (uint32_t)o.p.seed, key1 = (uint32_t)(o.p.seed >> 32);
+  const double pdel = 1.0 - 1.0 / ((double)nr * 1.0);
+  std::vector<uint8_t> fl((size_t)o.n, 0);
+  std::vector<int> ns(nc, 0), nk(nc, 0), nd(nc, 0);
+  std::vector<double> hc(nc, 0.0);
+  int64_t total = 0;
+  for (int c = 0; c < nc; ++c) {
+    const int N = o.cnt[c], s0 = o.start[c];
+    const int32_t wc = o.cr[c];
+    uint64_t W = 0;
+    int last = -1;
+    for (int u = 0; u < N; ++u) {
+      const uint32_t p = (uint32_t)(s0 + u);
+      int32_t w = o.pw[p];
+      uint8_t f = 0;
+      if (w > wc) {
+        f = 1;
+        ++ns[c];
+      } else if (w < wc) {
+        mcpm_u32x4 r = mcpm_philox(p, 0u, slo, mcpm_ctr3(MCPM_TAG_ADDREM, shi, 0u), key0, key1);
+        if (mcpm_u01(r.v[0]) < pdel) {
+          f = 2;
+          ++nd[c];
+        } else {
+          w = wc;
+          ++nk[c];
+        }
+      } else {
+        ++nk[c];
+      }
+      fl[p] = f;
+      if (f != 2) {
+        W += (uint64_t)(uint32_t)w;
+        last = u;
+      }
+    }
+    if (ns[c] > 0) {
+      const uint32_t pl = (uint32_t)(s0 + last);
+      const double *rl = &o.rec[4 * (size_t)pl];
+      const double wl = (double)std::max(o.pw[pl], wc);
+      const double Wd = 1.0 * (double)W;
+      const double m1 = (rl[1] * wl) / Wd, m2 = (rl[2] * wl) / Wd, m3 = (rl[3] * wl) / Wd;
+      WgSum<1> ws;
+      for (int u = 0; u < N; ++u) {
+        const uint32_t p = (uint32_t)(s0 + u);
+        if (fl[p] == 2) {
+          ws.touch(u % 256);
+          continue;
+        }
+        const double *r = &o.rec[4 * (size_t)p];
+        const double a = r[1] - m1, b = r[2] - m2, d = r[3] - m3;
+        const double e = a * a + b * b + d * d;
+        const double v[1] = {e * (double)std::max(o.pw[p], wc)};
+        ws.add(u % 256, v);
+      }
+      double S[1];
+      ws.total(S);
+      const double sig = sqrt(S[0] / Wd);
+      hc[c] = 1.06 * sig * mcpm_exp((-1.0 / 5.0) * mcpm_log(1.0 * (double)(nk[c] + nr)));
+    }
+    total += ns[c] + nk[c] + (int64_t)ns[c] * (nr - 1);
+  }
+  if (total > o.p.capacity) return MCPM_ERR_CAPACITY;
+  std::vector<double> out(4 * (size_t)total);
+  std::vector<int32_t> wout((size_t)total);
+  int64_t d = 0;
+  int64_t split = 0, del = 0;
+  for (int c = 0; c < nc; ++c) {
+    const int N = o.cnt[c], s0 = o.start[c];
+    const int32_t wc = o.cr[c];
+    const int64_t c0 = d;
+    for (int u = 0; u < N; ++u) {
+      const size_t p = (size_t)(s0 + u);
+      if (fl[p] == 2) continue;
+      memcpy(&out[4 * (size_t)d], &o.rec[4 * p], 4 * sizeof(double));
+      wout[(size_t)d] = wc;
+      ++d;
+    }
+    for (int u = 0; u < N; ++u) {
+      const uint32_t p = (uint32_t)(s0 + u);
+      if (fl[p] != 1) continue;
+      const double *r = &o.rec[4 * (size_t)p];
+      for (int32_t j = 0; j < nr - 1; ++j) {
+        mcpm_u32x4 q =
+            mcpm_philox(p, (uint32_t)j, slo, mcpm_ctr3(MCPM_TAG_ADDREM, shi, 1u), key0, key1);
+        double z0, z1, z2, z3;
+        mcpm_normal2(q.v[0], q.v[1], &z0, &z1);
+        mcpm_normal2(q.v[2], q.v[3], &z2, &z3);
+        double *n = &out[4 * (size_t)d];
+        n[0] = r[0];
+        n[1] = rnd<real>(r[1] + hc[c] * z0);
+        n[2] = rnd<real>(r[2] + hc[c] * z1);
+        n[3] = rnd<real>(r[3] + hc[c] * z2);
+        wout[(size_t)d] = wc;
+        ++d;
+      }
+    }
+    o.start[c] = (int)c0;
+    o.cnt[c] = (int)(d - c0);
+    split += ns[c];
+    del += nd[c];
+  }
+  o.start[nc] = (int)d;
+  if (st) {
+    st->before = o.n;
+    st->split = split;
+    st->created = split * (int64_t)(nr - 1);
+    st->deleted = del;
+    st->after = total;
+  }
+  o.rec.swap(out);
+  o.pw.swap(wout);
+  o.n = total;
+  orc_densities(o);
+  o.have_closures = false;
+  return MCPM_OK;
+}
+
+extern "C" {
+
+int orc_addremove(void *h, int32_t nr, int64_t step, orc_addremove_stats *st) {
+  Oracle &o = *(Oracle *)h;
+  if (o.dims != 1 || nr < 1) return MCPM_ERR_ARG;
+  if (o.cr.empty() || !o.track_w || !o.sorted) return MCPM_ERR_STATE;
+  if (o.p.precision == MCPM_FP32) return orc_addremove_t<float>(o, nr, step, st);
+  return orc_addremove_t<double>(o, nr, step, st);
+}
+
 int orc_thermostat(void *h, double zone_lo, double T_lo, double zone_hi, double T_hi) {
   Oracle &o = *(Oracle *)h;
   if (o.dims != 1) return MCPM_ERR_ARG;
