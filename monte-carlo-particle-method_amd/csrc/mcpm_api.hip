@@ -52,6 +52,8 @@ struct mcpm_ctx {
   // per-cell weights (device, nc; null: every cell has p.n_ratio), the
   // particles' weights (ping-pong like P, carried by the sorts while
   // track_w), and the work arrays of mcpm_addremove
+  uint32_t *scan_tmp = nullptr;  // tile sums and offsets of the long scans
+  int scan_tiles = 0;             // the most tiles they hold
   int32_t *cratio = nullptr;
   int32_t *pw[2] = {nullptr, nullptr};
   bool track_w = false;
@@ -543,6 +545,24 @@ int chunk_for(int64_t n, int fixed) {
 // particles are still in the cell order of their positions, moved by a push
 // that adds nothing per step (no pressure face sums, no re-insertion), dense
 // enough that a chunk spans a few cells.
+// Exclusive scan of n counts into out[0..n] (out[n] the total): one
+// workgroup for short arrays, tiles over many workgroups for long ones.
+int excl_scan(mcpm_ctx *c, const uint32_t *in, uint32_t *out, int n) {
+  if (n <= 16384) {
+    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, in, out, n);
+  } else {
+    const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nt > c->scan_tiles) return MCPM_ERR_CAPACITY;
+    uint32_t *tsum = c->scan_tmp, *toff = c->scan_tmp + nt + 1;
+    hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)nt), dim3(1024), 0, c->s, in, n, tsum);
+    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, tsum, toff, nt);
+    hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nt), dim3(1024), 0, c->s, in, out, n,
+                       (const uint32_t *)toff, nt);
+  }
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
 // After a sort: particles whose weights are still to be set (new particles
 // with per-cell weights in force) take their cells' (main.cpp:239-242).
 int weights_after_sort(mcpm_ctx *c) {
@@ -579,7 +599,7 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
   // the chunks: pieces of the cell blocks of the previous sort (start[])
   hipLaunchKernelGGL(k_wblocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, c->s,
                      c->wbcell, nblk, c->start, c->wpieces);
-  hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->wpieces, c->wpbase, nblk);
+  if (int st = excl_scan(c, c->wpieces, c->wpbase, nblk)) return st;
   hipLaunchKernelGGL(k_wfill, dim3((unsigned)(nblk + (nrep1 + 255) / 256)), dim3(256), 0, c->s,
                      c->wbcell, c->wbrep, nblk, c->start, c->wpbase, c->wrblk, nrep1,
                      (uint32_t)c->n, c->wbeg, c->wrep, nchp, c->rchunk);
@@ -592,7 +612,7 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
                      nullptr, nullptr);
   hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 3) / 4)), dim3(256), 0, c->s, c->wlo,
                      c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc);
-  hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->cnt, c->start, c->nc);
+  if (int st = excl_scan(c, c->cnt, c->start, c->nc)) return st;
   hipLaunchKernelGGL((k_wpush<R, true>), dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
                      c->start, c->err, c->wflag, iid, oid, c->wdelta, (real *)c->ys,
@@ -667,7 +687,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
                        c->seg);
     hipLaunchKernelGGL(k_scan_bins, dim3((nb + 255) / 256), dim3(256), 0, c->s, c->partial,
                        nseg, nb, cnt, nden, yc, c->kp);
-    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, cnt, start, nb);
+    if (int st = excl_scan(c, cnt, start, nb)) return st;
     hipLaunchKernelGGL(k_scan_c, gs, dim3(256), 0, c->s, c->table, c->partial, (int)nch, nb,
                        c->seg);
     hipLaunchKernelGGL((k_push<R, true>), dim3((unsigned)nch), dim3(64 * PUSH_WAVES), lds,
@@ -759,7 +779,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   fpp.model = variant;
   hipLaunchKernelGGL(k_mcand, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->cnt, c->omax,
                      c->nden, c->yc, c->mcand, nc, c->kp, (int)MODEL, variant, slo, shi);
-  hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, c->mcand, c->cstart, nc);
+  if (int st = excl_scan(c, c->mcand, c->cstart, nc)) return st;
   hipLaunchKernelGGL(k_zero_u32, dim3((CNT_TOTAL + 255) / 256), dim3(256), 0, c->s,
                      c->counters, (int)CNT_TOTAL);
   LAUNCHCHK();
@@ -1184,7 +1204,7 @@ int addremove_t(mcpm_ctx *c, int32_t nr, int64_t step, mcpm_addremove_stats *sta
       hipLaunchKernelGGL((k_ar_classify<R>), dim3((unsigned)nc), dim3(256), 0, c->s, P, c->cnt,
                          c->start, (const int32_t *)c->pw[c->cur], (const int32_t *)c->cratio,
                          nr, c->kp, slo, shi, c->ar_fl, c->ar_cst, c->ar_h, tot);
-    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, tot, ostart, nc);
+    if (int st = excl_scan(c, tot, ostart, nc)) return st;
     LAUNCHCHK();
     uint32_t total = 0;
     HIPCHK(hipMemcpyAsync(&total, ostart + nc, sizeof total, hipMemcpyDeviceToHost, c->s));
@@ -1460,6 +1480,8 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
   if ((st = dalloc(&c->rtab, 2 * (size_t)p->capacity + ((size_t)p->capacity + 3) / 4)))
     return fail(st);
   c->cflag = (uint8_t *)(c->rtab + 2 * (size_t)p->capacity);
+  c->scan_tiles = (int)(((int64_t)c->nc + SCAN_TILE - 1) / SCAN_TILE) + 1;
+  if ((st = dalloc(&c->scan_tmp, 2 * ((size_t)c->scan_tiles + 1)))) return fail(st);
   if (hipMemset(c->omax, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->omega_new, 0, nc * sizeof(double)) != hipSuccess ||
       hipMemset(c->F2, 0, nc * sizeof(double)) != hipSuccess ||
@@ -1537,7 +1559,7 @@ int mcpm_destroy(mcpm_ctx *c) {
   if (c->s) hipStreamSynchronize(c->s);
   void *bufs[] = {c->rkp, c->cell_rep, c->rcell_off, c->rpart_off, c->rprio, c->rcand_off,
                   c->rfa,
-                  c->P[0], c->P[1], c->ys, c->cratio, c->pw[0], c->ar_fl, c->ar_cst,
+                  c->P[0], c->P[1], c->ys, c->scan_tmp, c->cratio, c->pw[0], c->ar_fl, c->ar_cst,
                   c->ar_h, c->table, c->partial, c->dcnt, c->dstart, c->keys,
                   c->keys_tmp,
                   c->cnt, c->start, c->mcand,

@@ -117,6 +117,11 @@ __global__ void k_mcand(const uint32_t *cnt, const double *omax, const double *n
                         const double *yc, uint32_t *mcand, int nc, KParams kp, int model,
                         int variant, uint32_t step_lo, uint32_t step_hi);
 __global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc);
+// the same scan over several workgroups for long arrays (SCAN_TILE each)
+constexpr int SCAN_TILE = 4096;
+__global__ void k_tile_sums(const uint32_t *in, int n, uint32_t *tsum);
+__global__ void k_tile_scan(const uint32_t *in, uint32_t *out, int n, const uint32_t *toff,
+                            int nt);
 // Priority band of an operation: the top bits of the priority scaled to
 // nbands (band_mul = nbands for priorities spread over 32 bits, 0 for one band).
 __device__ __forceinline__ uint32_t op_band(uint32_t pr, uint32_t band_mul, int nbands) {

@@ -899,6 +899,57 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__
   if (tid == 1023) out[nc] = sh[1023];
 }
 
+// Exclusive scan of long arrays over many workgroups: SCAN_TILE elements per
+// workgroup, 4 consecutive ones per thread.  k_tile_sums sums each tile,
+// k_excl_scan scans the tile sums into tile offsets, k_tile_scan scans each
+// tile from its offset.  Integer sums: the same result as one serial scan.
+__global__ __launch_bounds__(1024) void k_tile_sums(const uint32_t *__restrict__ in, int n,
+                                                    uint32_t *__restrict__ tsum) {
+  __shared__ uint32_t sw[16];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + 4 * (int64_t)threadIdx.x;
+  uint32_t v = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q)
+    if (base + q < n) v += in[base + q];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+  if (lane_id() == 0) sw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < 16; ++w) t += sw[w];
+    tsum[blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_tile_scan(const uint32_t *__restrict__ in,
+                                                    uint32_t *__restrict__ out, int n,
+                                                    const uint32_t *__restrict__ toff, int nt) {
+  __shared__ uint32_t sw[16];
+  const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + 4 * (int64_t)threadIdx.x;
+  uint32_t e[4], v = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    e[q] = base + q < n ? in[base + q] : 0u;
+    v += e[q];
+  }
+  // exclusive prefix of the thread sums over the workgroup
+  uint32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o);
+    if ((int)lane_id() >= o) x += y;
+  }
+  if (lane_id() == 63) sw[threadIdx.x >> 6] = x;
+  __syncthreads();
+  uint32_t run = toff[blockIdx.x] + x - v;
+  for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) run += sw[w];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    if (base + q < n) out[base + q] = run;
+    run += e[q];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = toff[nt];
+}
+
 // The candidate draw: i in cell c, isotropic direction, the partner cell J
 // at y_i + sigma dir_y (collisions.cpp:648-671; 1D rule of new4,
 // collisions.cpp:784-792), j in J, j != i (collisions.cpp:675-679).
