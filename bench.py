@@ -207,7 +207,7 @@ def main():
     # Dominant kernel and its roofline.  Algorithmic bytes:
     #  sort (push pass 1 + scatter pass): read 16 B + read 16 B + write 16 B per
     #  particle (pass 1 reads the record, pass 2 reads it again and writes it);
-    #  collision rounds (k_round, one launch per round): per valid operation
+    #  collision rounds (one k_check and one k_round launch per round): per valid operation
     #  the two particle records read once (2 x 16 B) and its 16-byte operation
     #  record, per accepted collision the two records written back (2 x 16 B),
     #  summed over the step's launches.
@@ -227,8 +227,9 @@ def main():
         launches = 1.0
         dom, dom_ms, dom_bytes = "k_fp_velocity", fpv_ms, 2.0 * rec * N
     elif coll_ms >= sort_ms:
+        # one reservation round = one k_check and one k_round launch
         launches = timers["collide_rounds"]["calls"] / args.steps
-        dom, dom_ms, dom_bytes = "k_round", coll_ms, coll_bytes
+        dom, dom_ms, dom_bytes = "k_check + k_round", coll_ms, coll_bytes
     else:
         launches = 2.0 * passes
         dom, dom_ms, dom_bytes = "k_push (both passes)", sort_ms, sort_bytes
@@ -242,7 +243,7 @@ def main():
                 "alg_bytes_per_launch": round(dom_bytes / max(launches, 1.0)),
                 "step_alg_bytes_per_particle": step_alg,
                 "step_frac": round(value * step_alg / 1e9 / HBM_PEAK_GBS, 4)}
-    if dom == "k_round":
+    if dom == "k_check + k_round":
         roofline["random_access"] = random_access_roofline(
             N, int(rec), valid, acc, revisits, nbands, coll_ms, launches)
 

@@ -5,11 +5,9 @@ Velocity_FP_linear on the slab.
 DSMC_HS and CBA_VHS run on the collision rounds: pairs, decisions,
 velocities, displaced positions and crm are compared bit for bit with the
 oracle.  CBA_HS runs one thread per cell: counts, velocities and positions bit
-for bit.  Its Vr_max starts from the cell temperature of the moments, whose
-FP64 sums are reduced in another order than the oracle's; a last-bit
-difference there would only matter at an exact tie of floor(M) or of an
-acceptance test.  FP_linear: within 1e-10 (FP64) / 1e-5 (FP32) relative, as
-the other Fokker-Planck models (same moment sums)."""
+for bit (its Vr_max starts from the cell temperature of the moments, whose
+FP64 sums both sides take in one fixed workgroup order).  FP_linear: bit for
+bit, as the other Fokker-Planck models."""
 import numpy as np
 import pytest
 
