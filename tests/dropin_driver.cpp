@@ -7,8 +7,11 @@
 //
 // <in>: int64 N, int32 n_cells, int32 problem, then doubles length, sigma,
 // m, kb, b, Fn, cell volume, dt, T_wall_1, T_wall_2, U_wall_1, U_wall_2,
-// then N doubles each of x2, U1, U2, U3 and n_cells doubles of omega_max.
-// <operator>: 3 or 4 (Collision_ESMC_new3 / _new4).
+// mu, T0, visp, then N doubles each of x2, U1, U2, U3 and n_cells doubles
+// of omega_max.
+// <operator>: 3 or 4 (ESMC, Collision_ESMC_new3 / _new4), 5 (FP_dense:
+// random_generator, Velocity_FP_parallel and DFP_extra_streaming,
+// main.cpp:415-470).
 // <out>: the cells' counts (int32), omega_max and n (doubles), then the
 // particles (x2, U1, U2, U3, doubles) listed cell by cell through
 // cells[c].indices_inside, and index[] (int32) in the caller's order.
@@ -33,6 +36,8 @@ int main(int argc, char **argv) {
     fprintf(stderr, "usage: dropin_driver in out steps operator\n");
     return 2;
   }
+  const int steps = atoi(argv[3]);
+  const int op = atoi(argv[4]);
   FILE *f = fopen(argv[1], "rb");
   if (!f) return 2;
   int64_t N;
@@ -40,8 +45,8 @@ int main(int argc, char **argv) {
   rd(f, &N, 1);
   rd(f, &nc, 1);
   rd(f, &problem, 1);
-  double v[12];
-  rd(f, v, 12);
+  double v[15];
+  rd(f, v, 15);
   std::vector<double> x1(N, 0.0), x2(N), x3(N, 0.0), U1(N), U2(N), U3(N);
   rd(f, x2.data(), N);
   rd(f, U1.data(), N);
@@ -50,13 +55,14 @@ int main(int argc, char **argv) {
   std::vector<double> om(nc);
   rd(f, om.data(), nc);
   fclose(f);
-  const int steps = atoi(argv[3]);
-  const int op = atoi(argv[4]);
 
   // the structs as readSettingsFile / make_cells leave them (myIO.cpp)
   GAS gas{};
   BOX box{};
-  gas.model = "ESMC";
+  gas.model = op == 5 ? "FP_dense" : "ESMC";
+  gas.mu = v[12];
+  gas.T0 = v[13];
+  gas.visp = v[14];
   gas.N = N;
   gas.sigma = v[1];
   gas.m = v[2];
@@ -95,13 +101,24 @@ int main(int argc, char **argv) {
       box.step = step;
       cell_update(x1.data(), x2.data(), U1.data(), U2.data(), U3.data(), &gas, cells.data(), &box,
                   index.data(), &T, rho.data(), color.data());
-      // velocity_update, ESMC branch (Main_Functions.cpp:1555-1561)
-      if (op == 4)
+      if (op == 5) {
+        // random_generator, then velocity_update's FP_dense branch
+        // (Main_Functions.cpp:1566-1570) and the extra streaming of FP_dense
+        random_generator(xi.data(), xi.data(), xi.data(), xi.data(), xi.data(), xi.data(),
+                         cells.data(), &box, &gas, index.data());
+        Velocity_FP_parallel(U1.data(), U2.data(), U3.data(), &gas, cells.data(), &box,
+                             index.data(), xi.data(), xi.data(), xi.data(), xi.data(), xi.data(),
+                             xi.data());
+        DFP_extra_streaming(U1.data(), U2.data(), U3.data(), x1.data(), x2.data(), &gas, &box,
+                            cells.data(), index.data(), gas.delta_t);
+      } else if (op == 4) {
+        // velocity_update, ESMC branch (Main_Functions.cpp:1555-1561)
         Collision_ESMC_new4(U1.data(), U2.data(), U3.data(), x1.data(), x2.data(), &gas, &box,
                             cells.data(), index.data());
-      else
+      } else {
         Collision_ESMC_new3(U1.data(), U2.data(), U3.data(), x1.data(), x2.data(), &gas, &box,
                             cells.data(), index.data());
+      }
       apply_BC(U1.data(), U2.data(), U3.data(), x1.data(), x2.data(), x3.data(), x1o.data(),
                x2o.data(), x3o.data(), &gas, &box, cells.data(), index.data(), flag.data(),
                xi.data(), xi.data(), xi.data(), xi.data(), xi.data(), xi.data(), xi.data(),

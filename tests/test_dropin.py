@@ -102,7 +102,8 @@ def test_driver_reports_a_missing_device_as_an_error(tmp_path):
     with open(fin, "wb") as f:
         np.array([n], np.int64).tofile(f)
         np.array([nc, 2], np.int32).tofile(f)
-        np.array([2.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 0.01, 1.0, 1.0, 0.0, 0.0]).tofile(f)
+        np.array([2.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 0.01, 1.0, 1.0, 0.0, 0.0, 0.1, 1.0,
+                  0.5]).tofile(f)
         np.linspace(0.1, 1.9, n).tofile(f)
         np.zeros(3 * n).tofile(f)
         np.ones(nc).tofile(f)

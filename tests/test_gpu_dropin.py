@@ -2,7 +2,8 @@
 
 tests/_bin/dropin_driver builds GAS / BOX / CELLS and the particle arrays as
 the reference's own code does and runs cell_update, Collision_ESMC_new3 (or
-_new4), apply_BC, store_old_positions, stream_particles, apply_BC for a few
+_new4, or for FP_dense random_generator, Velocity_FP_parallel and
+DFP_extra_streaming), apply_BC, store_old_positions, stream_particles, apply_BC for a few
 steps of the heat-conduction slab (200 cells, 100 particles per cell, FP64,
 diffuse walls): exactly what the reference's main.cpp calls.  The state it
 leaves (per-cell counts, omega_max and n, the particles listed through
@@ -30,7 +31,8 @@ def _write_input(path, p, arrs, omega):
         np.array([y.size], np.int64).tofile(f)
         np.array([p.n_cells, 2], np.int32).tofile(f)  # problem 2: couette_flow (diffuse walls)
         np.array([p.length, p.sigma, p.mass, p.kb, p.b, p.Fn, p.cell_volume, p.delta_t,
-                  p.T_wall_lo, p.T_wall_hi, p.U_wall_lo, p.U_wall_hi], np.float64).tofile(f)
+                  p.T_wall_lo, p.T_wall_hi, p.U_wall_lo, p.U_wall_hi, p.mu, p.T0, p.visp],
+                 np.float64).tofile(f)
         for a in arrs:
             np.ascontiguousarray(a, np.float64).tofile(f)
         np.ascontiguousarray(omega, np.float64).tofile(f)
@@ -49,11 +51,11 @@ def _read_output(path, n, nc):
     return cnt, om, nd, parts, index
 
 
-@pytest.mark.parametrize("op", [3, 4])
+@pytest.mark.parametrize("op", [3, 4, 5])
 def test_reference_loop_through_dropin_symbols_matches_oracle(op, tmp_path):
     steps = 3
     p, arrs, meta = W.heat_conduction(precision=mcpm.FP64, seed=401, walls_at_sigma=True)
-    p.model = mcpm.MODEL_ESMC if op == 3 else mcpm.MODEL_ESMC_NEW4
+    p.model = {3: mcpm.MODEL_ESMC, 4: mcpm.MODEL_ESMC_NEW4, 5: mcpm.MODEL_FP_DENSE}[op]
     o = orc.OracleSim(p)
     o.upload(*arrs)
     o.cell_update()
@@ -61,7 +63,7 @@ def test_reference_loop_through_dropin_symbols_matches_oracle(op, tmp_path):
     omega = o.cells()["omega_max"].copy()
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
     _write_input(fin, p, arrs, omega)
-    print(f"dropin driver op=new{op} N={arrs[0].size} steps={steps}", flush=True)
+    print(f"dropin driver op={op} N={arrs[0].size} steps={steps}", flush=True)
     env = dict(os.environ, MCPM_SEED=str(p.seed))  # the drop-in's Philox key
     r = subprocess.run([DRIVER, str(fin), str(fout), str(steps), str(op)], capture_output=True,
                        text=True, timeout=120, env=env)
