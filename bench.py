@@ -178,6 +178,22 @@ def main():
     elapsed = t1 - t0
     value = N * args.steps / elapsed
     ms_step = 1000.0 * elapsed / args.steps
+    # the reference's cell_update takes the cell moments every step and its
+    # post-processing samples them (cell_poly_velocity); the step above
+    # leaves out what the collision model does not need: the same steps with
+    # both added, for a second figure
+    sampled_ms = None
+    if args.workload in ("interface", "dense"):
+        sim.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            sim.step(step)
+            sim.compute_moments()
+            sim.sample()
+            step += 1
+        sim.sync()
+        sampled_ms = 1000.0 * (time.perf_counter() - t0) / args.steps
+        log(f"with moments and sampling each step: {sampled_ms:.3f} ms/step")
     # as many steps again with HIP-event timers on the library's stream, for
     # the per-kernel figures (events around every launch cost a little time)
     sim.timing_enable(True)
@@ -291,6 +307,7 @@ def main():
         "config": {"workload": desc,
                    "particles": N, "cells": args.cells, "parallelism": "1 GPU",
                    "pressure_sampling": bool(args.pressure),
+                   "ms_per_step_with_moments_and_sampling": sampled_ms,
                    "unbatched_ms_per_step": unbatched,
                    "valid_ops_per_step": valid, "accepted_per_step": acc,
                    "reservation_rounds": rounds,

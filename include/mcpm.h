@@ -275,7 +275,8 @@ int mcpm_get_cells(mcpm_ctx *ctx, int32_t *count, int32_t *start, double *n,
    two-pass: out[c*MCPM_NMOM + k] with k = 0..2 U_space, 3..8 PIJ (11,12,
    13,22,23,33), 9..18 M3 (111,112,113,122,123,133,222,223,233,333),
    19..24 M4, 25..27 M5, 28 DM2, 29 T, 30 N_c, 31..33 Mp (mean peculiar
-   velocity, zero up to round-off). */
+   velocity, zero up to round-off).  out = NULL takes the moments on the
+   device only (as cell_update does every step), without a copy or a wait. */
 #define MCPM_NMOM 34
 int mcpm_moments(mcpm_ctx *ctx, double *out);
 

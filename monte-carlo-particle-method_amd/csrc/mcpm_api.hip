@@ -110,6 +110,8 @@ struct mcpm_ctx {
   uint4 *opA = nullptr, *opB = nullptr;  // one allocation: ping-pong op lists,
   uint4 *opBand = nullptr;               // then the priority-band lists
   uint32_t *gen_table = nullptr;         // per-chunk list sizes of the candidate draw
+  uint32_t *gen_cells = nullptr;         // each chunk's first and last cell (in gen_table)
+  int32_t gen_reach = -1;                // cells a partner can lie from its candidate's (-1: any)
   int nbands = 12;
   // round termination: round r + 1 writes the losers left after round r into
   // pinned host memory, which the host polls without touching the stream
@@ -757,7 +759,9 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   c->opBand = blk + 2 * cap;
   if (c->gen_table) HIPCHK(hipFree(c->gen_table));
   c->gen_table = nullptr;
-  if ((st = dalloc(&c->gen_table, ((size_t)cap / GEN_CHUNK + 2) * NLIST))) return st;
+  const size_t nch_cap = (size_t)cap / GEN_CHUNK + 2;
+  if ((st = dalloc(&c->gen_table, nch_cap * (NLIST + 2)))) return st;
+  c->gen_cells = c->gen_table + nch_cap * NLIST;
   uint32_t *lg = nullptr;
   st = dalloc(&lg, (size_t)cap * 3 + ((size_t)cap + 3) / 4);
   if (st) return st;
@@ -813,7 +817,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     Timer tm(c, T_GEN);
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
     hipLaunchKernelGGL(k_cand_count, dim3(nchunks), dim3(256), 0, c->s, M, pk.v[0], (int)MODEL,
-                       band_mul, nbands, c->gen_table, c->kp, c->cstart, nc);
+                       band_mul, nbands, c->gen_table, c->kp, c->cstart, nc, c->gen_cells);
     hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
                        nchunks, nbands * NREG, c->counters);
     hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
@@ -822,7 +826,8 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nchunks), dim3(256), 0, c->s, P, ys, c->cnt,
                        c->start,
                        c->cstart, nc, M, c->kp, slo, shi, pk.v[0], band_mul, nbands,
-                       c->gen_table, c->opBand, tabs[0], c->counters);
+                       c->gen_table, (const uint32_t *)c->gen_cells, c->gen_reach, c->opBand,
+                       tabs[0], c->counters);
     LAUNCHCHK();
   }
   int rounds = 0;
@@ -1379,6 +1384,14 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
     k.rpart_off = c->rpart_off;
     k.rprio = c->rprio;
     k.rcand_off = c->rcand_off;
+  }
+  // a partner point lies within sigma of its candidate's particle: on the
+  // slab at most ceil(sigma / dy) + 1 cells from the candidate's cell (the
+  // draw keeps those cells' counts at hand; the plane looks them up)
+  if (p->dims != 2) {
+    double r = std::ceil(p->sigma / k.dy) + 1.0;
+    for (const KParams &q : c->rkp_host) r = std::max(r, std::ceil(p->sigma / q.dy) + 1.0);
+    c->gen_reach = r < (double)GEN_WIN ? (int32_t)r : -1;
   }
   c->fpp.model = p->model;
   c->fpp.reserved = 0;
@@ -2007,10 +2020,10 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
 }
 
 int mcpm_moments(mcpm_ctx *c, double *out) {
-  if (!c || !out) return MCPM_ERR_ARG;
+  if (!c) return MCPM_ERR_ARG;
   if (!c->sorted) return MCPM_ERR_STATE;
   int st = MCPM_DISPATCH(c, launch_moments, c);
-  if (st) return st;
+  if (st || !out) return st;
   HIPCHK(hipMemcpyAsync(out, c->moments, (size_t)c->nc * MCPM_NMOM * sizeof(double),
                         hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));

@@ -147,7 +147,11 @@ constexpr uint32_t GEN_CHUNK = 2048;
 constexpr int NLIST = MAX_BANDS * NREG;
 __global__ void k_cand_count(uint32_t M, uint32_t prio_key, int model, uint32_t band_mul,
                              int nbands, uint32_t *table, KParams kp, const uint32_t *cstart,
-                             int nc);
+                             int nc, uint32_t *gcells);
+// cells of one candidate chunk whose candidate starts the draw keeps in LDS,
+// and partner cells whose counts and starts it keeps there
+constexpr int GEN_CELLS = 512;
+constexpr int GEN_WIN = 1024;
 __global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters);
 __global__ void k_list_starts(uint32_t *counters, int nl);
 template <typename R, int MODEL>
@@ -155,8 +159,8 @@ __global__ void k_gen(const R *P, const typename R::real_type *ys, const uint32_
                       const uint32_t *start,
                       const uint32_t *cstart, int nc, uint32_t M, KParams kp, uint32_t step_lo,
                       uint32_t step_hi, uint32_t prio_key, uint32_t band_mul, int nbands,
-                      const uint32_t *table, uint4 *bands, uint32_t *tab0,
-                      uint32_t *counters);
+                      const uint32_t *table, const uint32_t *gcells, int32_t reach,
+                      uint4 *bands, uint32_t *tab0, uint32_t *counters);
 // A reservation round: k_check resolves the particles that several of the
 // round's operations share, k_round commits the winners and admits the next
 // round's operations.

@@ -973,69 +973,6 @@ __device__ __forceinline__ R position_of(const R *P, const typename R::real_type
   return P[i];
 }
 
-template <typename R>
-__device__ __forceinline__ bool draw_candidate(uint32_t c, uint32_t t, const R *P,
-                                               const typename R::real_type *ys,
-                                               const uint32_t *cnt, const uint32_t *start,
-                                               const KParams &kp, uint32_t step_lo,
-                                               uint32_t step_hi, uint32_t *oi, uint32_t *oj,
-                                               bool clamp = false) {
-  mcpm_u32x4 w = mcpm_philox(t, c, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kp.key0,
-                             kp.key1);
-  uint32_t Nc = cnt[c];
-  const uint32_t li = mcpm_uint_below(w.v[0], Nc);
-  uint32_t i = start[c] + li;
-  double d0, d1, d2;
-  mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-  int32_t J;
-  const R ri = position_of(P, ys, i);
-  if (clamp) {
-    if constexpr (R::dims == 1) J = partner_cell_clamped(ri, d1, kp);
-    else return false;
-  } else if (!partner_cell(ri, d0, d1, kp, &J)) {
-    return false;
-  }
-  uint32_t NJ = cnt[J];
-  if (NJ == 0) return false;
-  uint32_t j = start[J] + mcpm_uint_below(w.v[3], NJ);
-  if (j == i) return false;
-  *oi = i;
-  *oj = j;
-  return true;
-}
-
-
-// The candidate draw in replica rq of a batched context: the replica's key,
-// its local cell number for the counters, its geometry for the partner cell.
-template <typename R>
-__device__ __forceinline__ bool draw_candidate_rep(uint32_t c, uint32_t t, int32_t rq,
-                                                   const R *P,
-                                                   const typename R::real_type *ys,
-                                                   const uint32_t *cnt,
-                                                   const uint32_t *start, const KParams &kp,
-                                                   uint32_t step_lo, uint32_t step_hi,
-                                                   uint32_t *oi, uint32_t *oj) {
-  const KParams &kq = kp.rkp[rq];
-  const int32_t off = kp.rcell_off[rq];
-  mcpm_u32x4 w = mcpm_philox(t, c - (uint32_t)off, step_lo,
-                             mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kq.key0, kq.key1);
-  uint32_t Nc = cnt[c];
-  const uint32_t li = mcpm_uint_below(w.v[0], Nc);
-  uint32_t i = start[c] + li;
-  double d0, d1, d2;
-  mcpm_iso_dir(w.v[1], w.v[2], &d0, &d1, &d2);
-  int32_t J;
-  if (!partner_cell(position_of(P, ys, i), d0, d1, kq, &J)) return false;
-  J += off;
-  uint32_t NJ = cnt[J];
-  if (NJ == 0) return false;
-  uint32_t j = start[J] + mcpm_uint_below(w.v[3], NJ);
-  if (j == i) return false;
-  *oi = i;
-  *oj = j;
-  return true;
-}
-
 // DSMC pair draw (collisions.cpp:2093-2098): two distinct particles of cell c.
 // The draws of candidate k (global index) are keyed by (k, c): the commit
 // finds both in the operation record, whatever the particles' positions.
@@ -1105,13 +1042,19 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
                                                     uint32_t band_mul, int nbands,
                                                     uint32_t *__restrict__ table, KParams kp,
                                                     const uint32_t *__restrict__ cstart,
-                                                    int nc) {
+                                                    int nc, uint32_t *__restrict__ gcells) {
   __shared__ uint32_t scount[NLIST];
   __shared__ uint32_t scell;
   const int nl = nbands * NREG;
   if ((int)threadIdx.x < NLIST) scount[threadIdx.x] = 0;
   const uint32_t lo = blockIdx.x * GEN_CHUNK;
-  if (model == COLL_ESMC4 && threadIdx.x == 0) scell = first_cell_of(lo, cstart, nc);
+  // the chunk's first and last cell, for the draw
+  if (threadIdx.x == 0) {
+    scell = first_cell_of(lo, cstart, nc);
+    const uint32_t hi = lo + GEN_CHUNK < M ? lo + GEN_CHUNK : M;
+    gcells[2 * blockIdx.x] = scell;
+    gcells[2 * blockIdx.x + 1] = first_cell_of(hi - 1u, cstart, nc);
+  }
   __syncthreads();
   const int esmc = model != COLL_DSMC;
   const uint32_t x0 = op_region(lo, M);
@@ -1195,56 +1138,139 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              uint32_t step_hi, uint32_t prio_key,
                                              uint32_t band_mul, int nbands,
                                              const uint32_t *__restrict__ table,
+                                             const uint32_t *__restrict__ gcells, int32_t reach,
                                              uint4 *__restrict__ bands,
                                              uint32_t *__restrict__ tab0,
                                              uint32_t *__restrict__ counters) {
   __shared__ uint32_t spos[NLIST];
-  __shared__ uint32_t scell[1];
+  __shared__ uint32_t s_cs[GEN_CELLS + 1];
+  __shared__ uint32_t s_wn[GEN_WIN], s_ws[GEN_WIN];
   const int nl = nbands * NREG;
   // consecutive chunks (one cell's candidates, drawing from its particles) on
   // one XCD, whose L2 then serves their reads of the cell's records
   const uint32_t bx = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t lo = bx * GEN_CHUNK;
+  // the chunk's cells c0..c1 (their candidate starts in LDS when they fit)
+  // and the cells its partners can lie in, [wlo, wlo + wn) (their counts
+  // and starts in LDS when they fit); lookups outside fall back to memory
+  const uint32_t c0 = gcells[2 * bx], c1 = gcells[2 * bx + 1];
+  const int ncs = (int)(c1 - c0) + 2;
+  const bool lcs = ncs <= GEN_CELLS + 1;
+  int32_t wlo = (int32_t)c0 - reach, whi = (int32_t)c1 + reach + 1;
+  if (wlo < 0) wlo = 0;
+  if (whi > nc) whi = nc;
+  const int32_t wn = whi - wlo;
+  const bool lw = reach >= 0 && wn <= GEN_WIN;
   if ((int)threadIdx.x < nl)  // list start + this chunk's offset inside the list
     spos[threadIdx.x] = counters[CNT_BSTART + threadIdx.x] +
                         table[(size_t)bx * NLIST + threadIdx.x];
-  if (threadIdx.x == 0) scell[0] = first_cell_of(lo, cstart, nc);
+  if (lcs)
+    for (int q = threadIdx.x; q < ncs; q += blockDim.x) s_cs[q] = cstart[c0 + q];
+  if (lw)
+    for (int q = threadIdx.x; q < wn; q += blockDim.x) {
+      s_wn[q] = cnt[wlo + q];
+      s_ws[q] = start[wlo + q];
+    }
   __syncthreads();
-  uint32_t c = scell[0];
+  auto cs = [&](uint32_t cc) { return lcs ? s_cs[cc - c0] : cstart[cc]; };
+  auto ccnt = [&](int32_t J) {
+    const uint32_t o = (uint32_t)(J - wlo);
+    return (lw && o < (uint32_t)wn) ? s_wn[o] : cnt[J];
+  };
+  auto cstr = [&](int32_t J) {
+    const uint32_t o = (uint32_t)(J - wlo);
+    return (lw && o < (uint32_t)wn) ? s_ws[o] : start[J];
+  };
+  const bool clamp = MODEL == COLL_ESMC4 && kp.boundary != MCPM_BC_PERIODIC;
   const uint32_t x0 = op_region(lo, M);
+  uint32_t c = c0;  // this thread's cell, walking forward with its candidates
   uint32_t nvalid = 0;
-  for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
-    const uint32_t k = lo + b0 + threadIdx.x;
-    const bool live = k < M;
-    if (live)
-      while (cstart[c + 1] <= k) ++c;  // empty cells have cstart[c] == cstart[c + 1]
-    uint32_t i = 0, j = 0;
-    bool ok = false;
-    uint32_t pr = 0, q = 0;
-    if (live) {
-      const uint32_t t = k - cstart[c];
-      const int32_t rq = kp.nrep ? kp.cell_rep[c] : 0;
-      if (cnt[c] > 0) {
-        if (MODEL == COLL_DSMC) {
-          draw_dsmc_pair(c, k, cnt, start, kp, step_lo, step_hi, &i, &j);
+  // U candidates per thread at a time: their position loads are issued together
+  constexpr int U = 4;
+  for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x * U) {
+    uint32_t kk[U], cc[U], tt[U], ii[U];
+    bool live[U], drawn[U];
+    double d0[U], d1[U];
+    uint32_t w3[U];
+    R ri[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t k = lo + b0 + u * blockDim.x + threadIdx.x;
+      kk[u] = k;
+      live[u] = k < M;
+      drawn[u] = false;
+      ii[u] = 0;
+      tt[u] = 0;
+      cc[u] = c;
+      if (!live[u]) continue;
+      while (cs(c + 1) <= k) ++c;  // empty cells have cstart[c] == cstart[c + 1]
+      cc[u] = c;
+      tt[u] = k - cs(c);
+      const uint32_t Nc = ccnt((int32_t)c);
+      if (MODEL == COLL_DSMC || Nc == 0) continue;
+      // the candidate's draws: i in c, the direction, j's uniform
+      // (collisions.cpp:648-679); a replica's own key and local cell
+      const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
+      const uint32_t cl = kp.nrep ? c - (uint32_t)kp.rcell_off[kp.cell_rep[c]] : c;
+      const mcpm_u32x4 w =
+          mcpm_philox(tt[u], cl, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kq.key0, kq.key1);
+      ii[u] = cstr((int32_t)c) + mcpm_uint_below(w.v[0], Nc);
+      double e2;
+      mcpm_iso_dir(w.v[1], w.v[2], &d0[u], &d1[u], &e2);
+      w3[u] = w.v[3];
+      drawn[u] = true;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (drawn[u]) ri[u] = position_of(P, ys, ii[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (!live[u]) continue;
+      const uint32_t k = kk[u], cu = cc[u], t = tt[u];
+      uint32_t i = ii[u], j = 0;
+      bool ok = false;
+      if (MODEL == COLL_DSMC) {
+        if (ccnt((int32_t)cu) > 0) {
+          draw_dsmc_pair(cu, k, cnt, start, kp, step_lo, step_hi, &i, &j);
           ok = true;
-        } else if (kp.nrep == 0) {
-          ok = draw_candidate<R>(c, t, P, ys, cnt, start, kp, step_lo, step_hi, &i, &j,
-                                 MODEL == COLL_ESMC4 && kp.boundary != MCPM_BC_PERIODIC);
+        }
+      } else if (drawn[u]) {
+        int32_t J;
+        bool inside;
+        if (kp.nrep == 0) {
+          if (clamp) {
+            if constexpr (R::dims == 1) {
+              J = partner_cell_clamped(ri[u], d1[u], kp);
+              inside = true;
+            } else {
+              inside = false;
+            }
+          } else {
+            inside = partner_cell(ri[u], d0[u], d1[u], kp, &J);
+          }
         } else {
-          ok = draw_candidate_rep<R>(c, t, rq, P, ys, cnt, start, kp, step_lo, step_hi, &i, &j);
+          const int32_t rq = kp.cell_rep[cu];
+          inside = partner_cell(ri[u], d0[u], d1[u], kp.rkp[rq], &J);
+          J += kp.rcell_off[rq];
+        }
+        if (inside) {
+          const uint32_t NJ = ccnt(J);
+          if (NJ > 0) {
+            j = cstr(J) + mcpm_uint_below(w3[u], NJ);
+            ok = j != i;
+          }
         }
       }
-      uint32_t band;
+      uint32_t pr, band;
       if (MODEL == COLL_ESMC4) {
-        pr = rank_new4(c, t, cstart, M);
+        pr = (M - cs(cu + 1)) + t;  // rank_new4
         band = band_new4(pr, M, nbands);
       } else {
-        pr = kp.nrep ? cand_priority_rep(kp, rq, k, MODEL == COLL_ESMC)
+        pr = kp.nrep ? cand_priority_rep(kp, kp.cell_rep[cu], k, MODEL == COLL_ESMC)
                      : cand_priority(k, prio_key, MODEL == COLL_ESMC);
         band = op_band(pr, band_mul, nbands);
       }
-      q = band * NREG + region_from(k, x0, M);
+      const uint32_t q = band * NREG + region_from(k, x0, M);
       if (ok) {
         ++nvalid;
         if (band == 0) {
@@ -1252,12 +1278,9 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
           tab0[j] = pr;
         }
       }
-      (void)t;
-    }
-    if (live) {
       const uint32_t pos = atomicAdd(&spos[q], 1u);
       // the fourth word: the candidate's index in its cell (ESMC) or its cell (DSMC)
-      const uint32_t w4 = MODEL == COLL_DSMC ? c : k - cstart[c];
+      const uint32_t w4 = MODEL == COLL_DSMC ? cu : t;
       bands[pos] = ok ? make_uint4(i, j, pr, w4) : make_uint4(OP_NONE, 0u, 0u, 0u);
     }
   }
@@ -2058,6 +2081,7 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                        const uint32_t *,                                   \
                                        const uint32_t *, int, uint32_t, KParams, uint32_t, \
                                        uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
+                                       const uint32_t *, int32_t,                          \
                                        uint4 *, uint32_t *, uint32_t *);                   \
   template __global__ void k_round<R, M>(int, int, uint32_t, R *, const uint4 *,           \
                                          const uint4 *, uint4 *, const uint32_t *,         \

@@ -630,6 +630,10 @@ class Simulation:
         self._check(self._lib.mcpm_moments(self._h, _dp(out)), "mcpm_moments")
         return out.reshape(self.n_cells, NMOM)
 
+    def compute_moments(self):
+        """The cell moments on the device only (cell_update's every step)."""
+        self._check(self._lib.mcpm_moments(self._h, None), "mcpm_moments")
+
     def sample(self):
         self._check(self._lib.mcpm_sample(self._h), "mcpm_sample")
 
