@@ -577,6 +577,20 @@ int weights_after_sort(mcpm_ctx *c) {
   return MCPM_OK;
 }
 
+// The sorted y array serves only the candidate draw: a sort under a model
+// that draws no candidates leaves it unwritten (and not valid).
+template <typename real>
+real *ys_out(const mcpm_ctx *c) {
+  switch (c->p.model) {
+    case MCPM_MODEL_FP_DENSE:
+    case MCPM_MODEL_FP_IDEAL:
+    case MCPM_MODEL_FP_LINEAR:
+    case MCPM_MODEL_NONE:
+    case MCPM_MODEL_CBA_HS: return nullptr;
+    default: return (real *)c->ys;
+  }
+}
+
 bool window_ok(const mcpm_ctx *c, int mode) {
   if (!c->w_enabled || c->p.dims == 2 || !c->wbeg || !c->order_sorted || c->w_fallbacks >= 3)
     return false;
@@ -617,7 +631,7 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
   if (int st = excl_scan(c, c->cnt, c->start, c->nc)) return st;
   hipLaunchKernelGGL((k_wpush<R, true>), dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
-                     c->start, c->err, c->wflag, iid, oid, c->wdelta, (real *)c->ys,
+                     c->start, c->err, c->wflag, iid, oid, c->wdelta, ys_out<real>(c),
                      c->track_w ? c->pw[c->cur] : nullptr,
                      c->track_w ? c->pw[1 - c->cur] : nullptr);
   LAUNCHCHK();
@@ -636,7 +650,7 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
   c->cur = 1 - c->cur;
   c->sorted = true;
   c->order_sorted = true;
-  c->ys_valid = c->ys != nullptr;
+  c->ys_valid = ys_out<real>(c) != nullptr;
   return weights_after_sort(c);
 }
 
@@ -708,7 +722,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   int st;
   if (!c->two_digit) {
     if ((st = pass(in, out, mode, kick_scatter, nc, 0, c->cnt, c->start, c->nden, c->yc,
-                   nullptr, nullptr, id_in, id_out, (real *)c->ys, w_in, w_out)))
+                   nullptr, nullptr, id_in, id_out, ys_out<real>(c), w_in, w_out)))
       return st;
   } else {
     // low digit with the push (keys carried to the destinations), then the
@@ -734,7 +748,7 @@ int run_sort_t(mcpm_ctx *c, int mode, int64_t step) {
   c->cur = 1 - c->cur;
   c->sorted = true;
   c->order_sorted = true;
-  c->ys_valid = c->ys != nullptr;
+  c->ys_valid = ys_out<real>(c) != nullptr;
   return weights_after_sort(c);
 }
 
@@ -1843,7 +1857,7 @@ int mcpm_fp_closures(mcpm_ctx *c, int64_t step) {
   Timer tm(c, T_FPCLOS);
   int st = MCPM_DISPATCH(c, launch_moments, c);
   if (st) return st;
-  hipLaunchKernelGGL(k_fp_closures, dim3((c->nc + 63) / 64), dim3(64), 0, c->s, c->moments,
+  hipLaunchKernelGGL(k_fp_closures, dim3(c->nc), dim3(64), 0, c->s, c->moments,
                      c->cnt, c->nden, c->yc, c->clos, c->drift, c->kp, c->fpp, step, c->err);
   LAUNCHCHK();
   c->have_closures = true;

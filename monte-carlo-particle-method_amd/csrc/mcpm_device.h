@@ -31,6 +31,22 @@ struct alignas(8 * sizeof(real)) Rec2 {
   real x, y, u1, u2, u3, p0, p1, p2;
 };
 
+// A record read once by a streaming pass, with the non-temporal hint: the L2
+// keeps such lines only briefly and leaves its space to the lines a scatter
+// is still filling.
+template <typename R>
+__device__ __forceinline__ R load_stream(const R *p) {
+  typedef uint32_t w4 __attribute__((ext_vector_type(4)));
+  static_assert(sizeof(R) % 16 == 0, "records are whole 16-byte words");
+  w4 v[sizeof(R) / 16];
+  const w4 *s = reinterpret_cast<const w4 *>(p);
+#pragma unroll
+  for (int i = 0; i < (int)(sizeof(R) / 16); ++i) v[i] = __builtin_nontemporal_load(s + i);
+  R r;
+  __builtin_memcpy(&r, v, sizeof(R));
+  return r;
+}
+
 // What the push's kick reads: the per-cell slab force F2 (1D) or the nodal
 // potential and its coefficient (2D, compute_SP_2D / compute_Vlasov_2D).
 struct Kick {
@@ -108,6 +124,15 @@ struct FPParams {
   double mu, T0, visp, crref;
 };
 
+// Hardware places workgroup b on XCD b % 8.  Renumber blocks so that each
+// XCD runs a contiguous range of logical blocks: neighbouring cells (and the
+// operations drawn in them) then share one XCD's L2.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t per = nb / 8u;
+  if (per == 0u || b >= per * 8u) return b;
+  return (b % 8u) * per + b / 8u;
+}
+
 // Per-cell constants of the FP_dense position drift, see k_fp_closures.
 constexpr int DRIFT_N = 12;
 
@@ -143,6 +168,26 @@ __host__ __device__ __forceinline__ int32_t cell_of(double y, double dy, int32_t
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() {
   return (1ull << lane_id()) - 1ull;
+}
+
+// The valid lanes of the wavefront whose key equals this lane's (empty for
+// an invalid lane), from one ballot per key bit: every key lies in
+// [0, 2^kbits).  Its cost does not depend on how many distinct keys the
+// wavefront holds.
+__device__ __forceinline__ uint64_t match_key(uint32_t key, bool valid, int kbits) {
+  uint64_t m = __ballot(valid);
+  for (int b = 0; b < kbits; ++b) {
+    const bool bit = (key >> b) & 1u;
+    const uint64_t on = __ballot(bit);
+    m &= bit ? on : ~on;
+  }
+  return valid ? m : 0ull;
+}
+// Bits that hold every bin index of [0, nb).
+__host__ __device__ __forceinline__ int key_bits(int nb) {
+  int k = 0;
+  while (k < 31 && (1 << k) < nb) ++k;
+  return k;
 }
 
 __device__ __forceinline__ void atomic_max_pos_double(double *addr, double v) {

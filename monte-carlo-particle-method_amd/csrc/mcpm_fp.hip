@@ -1,7 +1,8 @@
 // mcpm_fp.hip — Fokker-Planck model kernels for gfx950.
 //
-//   k_fp_closures  one thread per cell: Lambda, gamma_st, alpha/beta (12x12),
-//                  c/gamma (9x9) by Gauss-Jordan with scaled pivoting, phi,
+//   k_fp_closures  one wavefront per cell: Lambda, gamma_st, alpha/beta
+//                  (12x12), c/gamma (9x9) by Gauss-Jordan with scaled
+//                  pivoting (the elimination spread over the lanes), phi,
 //                  and the per-cell constants of the position drift
 //   k_fp_velocity  one workgroup per cell: Philox Gaussian noise normalised
 //                  per cell, exact OU step with the closure drift, per-cell
@@ -13,14 +14,19 @@
 
 namespace mcpm {
 
-// Gauss_Jordan (Math_Func.cpp:165-232) on a private n x n row-major matrix.
+// Gauss_Jordan (Math_Func.cpp:165-232) on an n x n row-major matrix in LDS,
+// by the one wavefront of the workgroup.  The elimination's element updates
+// are independent of each other and run on the lanes; the pivot choice and
+// the back substitution run in the reference's sequence in every lane (the
+// pivot) or in lane 0 (the substitution), so each value comes out of the
+// same operations as in a serial elimination.  x is left in LDS.
 template <int n>
-__device__ void gauss_jordan(double *A, double *rhs, double *x) {
+__device__ void wave_gauss_jordan(double *A, double *rhs, double *x, double *zb, double *fb) {
+  const int lane = (int)lane_id();
   for (int i = 0; i < n - 1; ++i) {
-    int best = 0;
-    double bz = -1.0;
-    for (int j = i; j < n; ++j) {
-      const double *row = A + j * n;
+    // each row's scaled pivot candidate |A[j][i]| / max_k |A[j][k]|
+    if (lane >= i && lane < n) {
+      const double *row = A + lane * n;
       int l = 0;
       double b = __builtin_fabs(row[0]);
       for (int k = 0; k < n; ++k)
@@ -28,61 +34,91 @@ __device__ void gauss_jordan(double *A, double *rhs, double *x) {
           b = __builtin_fabs(row[k]);
           l = k;
         }
-      double z = __builtin_fabs(row[i]) / __builtin_fabs(row[l]);
+      zb[lane] = __builtin_fabs(row[i]) / __builtin_fabs(row[l]);
+    }
+    __syncthreads();
+    int best = 0;
+    double bz = -1.0;
+    for (int j = i; j < n; ++j) {
+      const double z = zb[j];
       if (j == i || z > bz) {
         bz = z;
         best = j - i;
       }
     }
-    int k = i + best;
+    const int k = i + best;
     if (k != i) {
-      for (int j = 0; j < n; ++j) {
-        double t = A[i * n + j];
-        A[i * n + j] = A[k * n + j];
-        A[k * n + j] = t;
+      if (lane < n) {
+        const double t = A[i * n + lane];
+        A[i * n + lane] = A[k * n + lane];
+        A[k * n + lane] = t;
       }
-      double t = rhs[i];
-      rhs[i] = rhs[k];
-      rhs[k] = t;
+      if (lane == 0) {
+        const double t = rhs[i];
+        rhs[i] = rhs[k];
+        rhs[k] = t;
+      }
     }
-    for (int j = i + 1; j < n; ++j) {
-      double F = (1.0 / A[i * n + i]) * A[j * n + i];
-      for (int kk = 0; kk < n; ++kk) A[j * n + kk] = A[j * n + kk] - F * A[i * n + kk];
-      rhs[j] = rhs[j] - F * rhs[i];
+    __syncthreads();
+    if (lane > i && lane < n) fb[lane] = (1.0 / A[i * n + i]) * A[lane * n + i];
+    __syncthreads();
+    for (int e = lane; e < (n - 1 - i) * n; e += 64) {
+      const int j = i + 1 + e / n, kk = e % n;
+      A[j * n + kk] = A[j * n + kk] - fb[j] * A[i * n + kk];
     }
+    if (lane > i && lane < n) rhs[lane] = rhs[lane] - fb[lane] * rhs[i];
+    __syncthreads();
   }
-  for (int i = n - 1; i >= 0; --i) {
-    double a = 0.0;
-    for (int j = i + 1; j < n; ++j) a = a + A[i * n + j] * x[j];
-    x[i] = (rhs[i] - a) / A[i * n + i];
-  }
+  if (lane == 0)
+    for (int i = n - 1; i >= 0; --i) {
+      double a = 0.0;
+      for (int j = i + 1; j < n; ++j) a = a + A[i * n + j] * x[j];
+      x[i] = (rhs[i] - a) / A[i * n + i];
+    }
+  __syncthreads();
 }
 
 // Closures of one cell.  Layout of clos: 0 Lambda, 1-6 c_ij, 7-9 gamma,
 // 10-18 alpha, 19-21 beta, 22 gamma_st, 23-25 phi, 26 div U, 27 tau.
 // drift (DRIFT_N per cell): U0 U1 U2 T alpha3 alpha4 alpha5 beta1 gamma_st
 // Q1 phi1 valid.
-__global__ void k_fp_closures(const double *__restrict__ mom, const uint32_t *__restrict__ cnt,
-                              const double *__restrict__ nden, const double *__restrict__ yc,
-                              double *__restrict__ clos, double *__restrict__ drift, KParams kp,
-                              FPParams fpp, int64_t step, uint32_t *err) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ __launch_bounds__(64) void k_fp_closures(const double *__restrict__ mom,
+                                                   const uint32_t *__restrict__ cnt,
+                                                   const double *__restrict__ nden,
+                                                   const double *__restrict__ yc,
+                                                   double *__restrict__ clos,
+                                                   double *__restrict__ drift, KParams kp,
+                                                   FPParams fpp, int64_t step, uint32_t *err) {
+  // one wavefront per cell: every lane forms the cell's scalars, the lanes
+  // share the two eliminations, lane 0 writes the cell's results
+  __shared__ double sA[144], sR[12], sX[12], sZ[12], sF[12];
+  const int c = blockIdx.x;
   const int nc = kp.n_cells;
   if (c >= nc) return;
+  const bool w0 = lane_id() == 0;
   const double *m = mom + (size_t)c * MCPM_NMOM;
   double *cl = clos + (size_t)c * MCPM_NCLOS;
   double *dr = drift + (size_t)c * DRIFT_N;
-  for (int q = 0; q < MCPM_NCLOS; ++q) cl[q] = 0.0;
-  for (int q = 0; q < DRIFT_N; ++q) dr[q] = 0.0;
+  // the cell's results, kept by lane 0 and written at the end
+  double out[MCPM_NCLOS];
+  for (int q = 0; q < MCPM_NCLOS; ++q) out[q] = 0.0;
+  auto finish = [&](bool with_drift, const double *d) {
+    if (!w0) return;
+    for (int q = 0; q < MCPM_NCLOS; ++q) cl[q] = out[q];
+    for (int q = 0; q < DRIFT_N; ++q) dr[q] = with_drift ? d[q] : 0.0;
+  };
   const double *P = m + 3, *M3 = m + 9, *M4 = m + 19, *M5 = m + 25;
   const double DM2 = m[28], T = m[29], N = m[30];
   const double n = nden[c];
-  if (!(N > 1e-14)) return;
+  if (!(N > 1e-14)) {
+    finish(false, nullptr);
+    return;
+  }
   const double Q[3] = {M3[0] + M3[3] + M3[5], M3[1] + M3[6] + M3[8], M3[2] + M3[7] + M3[9]};
   const double DM4 = M4[0] + M4[3] + M4[5];
   const double Yn = yc[c];
   const double mass = kp.mass, kb = kp.kb;
-  cl[22] = -0.001 * (n * kp.b * Yn / (kb * T / mass));
+  out[22] = -0.001 * (n * kp.b * Yn / (kb * T / mass));
   double mu = fpp.mu * det_pow(T / fpp.T0, fpp.visp);
   double tau = 2.0 * mu / (n * kb * T);
   double D3 = DM2 / 3.0;
@@ -91,15 +127,20 @@ __global__ void k_fp_closures(const double *__restrict__ mom, const uint32_t *__
   L = L + P[2] * (P[1] * P[4] - P[2] * (P[3] - D3));
   double D2 = DM2 * DM2;
   L = -L / (tau * (D2 * D2));
-  cl[0] = L;
-  cl[27] = tau;
+  out[0] = L;
+  out[27] = tau;
   const bool dense = fpp.model == MCPM_MODEL_FP_DENSE;
   const bool fp = dense || fpp.model == MCPM_MODEL_FP_IDEAL;
-  if (!fp || cnt[c] < 2) return;
+  if (!fp || cnt[c] < 2) {
+    finish(false, nullptr);
+    return;
+  }
   if (dense && step > 0) {
-    double A[144], rhs[12], x[12];
-    for (int q = 0; q < 144; ++q) A[q] = 0.0;
-    for (int q = 0; q < 12; ++q) x[q] = 0.0;
+    double *A = sA;
+    double rhs[12];
+    for (int q = lane_id(); q < 144; q += 64) A[q] = 0.0;
+    if ((int)lane_id() < 12) sX[lane_id()] = 0.0;
+    __syncthreads();
     double trace = P[0] + P[3] + P[5];
     double dev[6] = {P[0], P[1], P[2], P[3], P[4], P[5]};
     dev[0] -= (1.0 / 3.0) * trace;
@@ -108,7 +149,7 @@ __global__ void k_fp_closures(const double *__restrict__ mom, const uint32_t *__
     double a = Q[0], b = Q[1], cc = Q[2];
     double d = DM4 - DM2 * DM2;
     // block rows of alpha_calculation (Fokker_Planck.cpp:277-370)
-    for (int blk = 0; blk < 3; ++blk) {
+    for (int blk = 0; blk < 3 && w0; ++blk) {
       const double qv = blk == 0 ? a : (blk == 1 ? b : cc);
       const int r0 = 3 * blk;
       A[(r0 + 0) * 12 + r0 + 0] = P[0]; A[(r0 + 0) * 12 + r0 + 1] = P[1]; A[(r0 + 0) * 12 + r0 + 2] = P[2];
@@ -135,10 +176,10 @@ __global__ void k_fp_closures(const double *__restrict__ mom, const uint32_t *__
     else if (c == 0) divU = (Uy(1 % nc) - Uy(0)) / kp.dy;
     else if (c == nc - 1) divU = (Uy(c) - Uy(c - 1)) / kp.dy;
     else divU = (Uy(c + 1) - Uy(c - 1)) / kp.dy / 2.0;
-    cl[26] = divU;
+    out[26] = divU;
     double cmu = 3.0 * 1.002 / 5.0 * (nbY * nbY) / (1 + 2.0 * nbY / 5.0);
     double ck = 1.002 * (nbY * nbY) / (1 + 3.0 * nbY / 5.0) / 5.0;
-    double gst = cl[22];
+    double gst = out[22];
     double bulk = w * divU / (mass * n);
     rhs[0] = (mu_crr - 1.0) * dev[0] + (p_crr - 1.0) * trace / 3.0 - bulk + cmu * dev[0] - gst * M4[0];
     rhs[1] = (mu_crr - 1.0) * dev[1] + cmu * dev[1] - gst * M4[1];
@@ -152,15 +193,22 @@ __global__ void k_fp_closures(const double *__restrict__ mom, const uint32_t *__
     rhs[9] = (k_crr - 1.0) * a + ck * a - gst * M5[0] + gst * (DM2 * Q[0]);
     rhs[10] = (k_crr - 1.0) * b + ck * b - gst * M5[1] + gst * (DM2 * Q[1]);
     rhs[11] = (k_crr - 1.0) * cc + ck * cc - gst * M5[2] + gst * (DM2 * Q[2]);
-    gauss_jordan<12>(A, rhs, x);
-    for (int q = 0; q < 9; ++q) cl[10 + q] = x[q];
-    for (int q = 0; q < 3; ++q) cl[19 + q] = x[9 + q];
-    if (x[0] != x[0]) atomicOr(err, ERRBIT_NAN);
+    if (w0)
+      for (int q = 0; q < 12; ++q) sR[q] = rhs[q];
+    __syncthreads();
+    wave_gauss_jordan<12>(A, sR, sX, sZ, sF);
+    for (int q = 0; q < 9; ++q) out[10 + q] = sX[q];
+    for (int q = 0; q < 3; ++q) out[19 + q] = sX[9 + q];
+    if (w0 && sX[0] != sX[0]) atomicOr(err, ERRBIT_NAN);
+    __syncthreads();  // sX is read before the next elimination writes it
   }
   {
-    double A[81], rhs[9], x[9];
-    for (int q = 0; q < 81; ++q) A[q] = 0.0;
-    for (int q = 0; q < 9; ++q) x[q] = 0.0;
+    double *A = sA;
+    double rhs[9];
+    for (int q = lane_id(); q < 81; q += 64) A[q] = 0.0;
+    if ((int)lane_id() < 9) sX[lane_id()] = 0.0;
+    __syncthreads();
+    if (w0) {
 #define AB(i, j) A[(i) * 9 + (j)]
     AB(0, 0) = 2.0 * P[0]; AB(0, 1) = 2.0 * P[1]; AB(0, 2) = 2.0 * P[2]; AB(0, 6) = 2.0 * Q[0];
     AB(1, 0) = P[1]; AB(1, 1) = P[0] + P[3]; AB(1, 2) = P[4]; AB(1, 3) = P[1]; AB(1, 4) = P[2];
@@ -187,6 +235,7 @@ __global__ void k_fp_closures(const double *__restrict__ mom, const uint32_t *__
     AB(7, 7) += DM4 - DM2 * DM2;
     AB(8, 8) += DM4 - DM2 * DM2;
 #undef AB
+    }
     const double Pr = 2.0 / 3.0;
     const double Yc = dense ? Yn : 1.0;
     for (int q = 0; q < 6; ++q) rhs[q] = -2.0 * L * M4[q];
@@ -194,19 +243,24 @@ __global__ void k_fp_closures(const double *__restrict__ mom, const uint32_t *__
     rhs[6] += L * (-3.0 * M5[0] + DM2 * Q[0] + 2.0 * (P[0] * Q[0] + P[1] * Q[1] + P[2] * Q[2]));
     rhs[7] += L * (-3.0 * M5[1] + DM2 * Q[1] + 2.0 * (P[1] * Q[0] + P[3] * Q[1] + P[4] * Q[2]));
     rhs[8] += L * (-3.0 * M5[2] + DM2 * Q[2] + 2.0 * (P[2] * Q[0] + P[4] * Q[1] + P[5] * Q[2]));
-    gauss_jordan<9>(A, rhs, x);
-    for (int q = 0; q < 6; ++q) cl[1 + q] = x[q];
-    for (int q = 0; q < 3; ++q) cl[7 + q] = x[6 + q];
+    if (w0)
+      for (int q = 0; q < 9; ++q) sR[q] = rhs[q];
+    __syncthreads();
+    wave_gauss_jordan<9>(A, sR, sX, sZ, sF);
+    for (int q = 0; q < 6; ++q) out[1 + q] = sX[q];
+    for (int q = 0; q < 3; ++q) out[7 + q] = sX[6 + q];
   }
-  const double *al = cl + 10, *mp = m + 31;
-  cl[23] = al[0] * mp[0] + al[1] * mp[1] + al[2] * mp[2];
-  cl[24] = al[3] * mp[0] + al[4] * mp[1] + al[5] * mp[2];
-  cl[25] = al[6] * mp[0] + al[7] * mp[1] + al[8] * mp[2];
+  const double *al = out + 10, *mp = m + 31;
+  out[23] = al[0] * mp[0] + al[1] * mp[1] + al[2] * mp[2];
+  out[24] = al[3] * mp[0] + al[4] * mp[1] + al[5] * mp[2];
+  out[25] = al[6] * mp[0] + al[7] * mp[1] + al[8] * mp[2];
+  double d[DRIFT_N];
   if (dense) {
-    dr[0] = m[0]; dr[1] = m[1]; dr[2] = m[2]; dr[3] = T;
-    dr[4] = cl[13]; dr[5] = cl[14]; dr[6] = cl[15]; dr[7] = cl[20];
-    dr[8] = cl[22]; dr[9] = M3[1] + M3[6] + M3[8]; dr[10] = cl[24]; dr[11] = 1.0;
+    d[0] = m[0]; d[1] = m[1]; d[2] = m[2]; d[3] = T;
+    d[4] = out[13]; d[5] = out[14]; d[6] = out[15]; d[7] = out[20];
+    d[8] = out[22]; d[9] = M3[1] + M3[6] + M3[8]; d[10] = out[24]; d[11] = 1.0;
   }
+  finish(dense, d);
 }
 
 __device__ __forceinline__ void fp_noise(uint64_t slot, const KParams &kp, uint32_t slo,

@@ -308,6 +308,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
   }
   __syncthreads();
   const uint64_t lt = lanemask_lt();
+  const int kbits = key_bits(nb);
   // U tiles per batch; the next batch's records are loaded while this
   // batch is processed.  Loads complete in issue order, so the slab kick's
   // field values, the one other read of the common path, are read first.
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const int64_t p = beg + u * TILE + mine0;
-    nx[u] = in[p < end ? p : end - 1];
+    nx[u] = load_stream(in + (p < end ? p : end - 1));
   }
   for (int64_t t0 = beg; t0 < end; t0 += (int64_t)TILE * U) {
     R rr[U];
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int64_t p = t0 + (int64_t)TILE * U + u * TILE + mine0;
-      nx[u] = in[p < end ? p : end - 1];
+      nx[u] = load_stream(in + (p < end ? p : end - 1));
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -383,21 +384,9 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
         for (int w = 0; w < PUSH_WAVES; ++w) {
           if (wave == w) {
             // rank among the lanes below with the same key, and how many
-            // lanes share it: one round of scalar ballots per distinct key,
-            // no memory access inside the loop
-            uint32_t rk = 0, tot = 0;
-            uint64_t active = __ballot(valid);
-            while (active) {
-              const int leader = __ffsll((unsigned long long)active) - 1;
-              const int lkey = __builtin_amdgcn_readlane(key, leader);
-              const bool mine = valid && key == lkey;
-              const uint64_t m = __ballot(mine);
-              if (mine) {
-                rk = (uint32_t)__popcll(m & lt);
-                tot = (uint32_t)__popcll(m);
-              }
-              active &= ~m;
-            }
+            // lanes share it: one ballot per key bit, no memory access
+            const uint64_t eq = match_key((uint32_t)key, valid, kbits);
+            const uint32_t rk = (uint32_t)__popcll(eq & lt), tot = (uint32_t)__popcll(eq);
             // one counter read per lane (lanes of one key read the same
             // word), one store per lane; the last lane of each key advances
             // its counter (distinct keys: no two lanes write one word)
@@ -461,8 +450,9 @@ __global__ __launch_bounds__(64 * PUSH_WAVES) void k_push(
 // of chunks x cells and a chunk can be small: one wavefront per chunk of
 // 1024 particles instead of one per several thousand.  A particle outside
 // its chunk's window (a long flight, a wide chunk, input out of cell order)
-// sets *wflag and is neither counted nor stored; the caller then sorts again
-// by the full histogram, from the same input.
+// sets *wflag and ends its wavefront's count pass; the scatter pass then
+// stores nothing and the caller sorts again by the full histogram, from the
+// same input.
 //   SCATTER=false: window and histogram of each chunk -> wlo, wlen, wtab
 //   SCATTER=true : wtab holds each window cell's offset among the chunks
 //                  (k_wscan), start[] the cells' starts; each particle lands
@@ -495,6 +485,8 @@ __global__ __launch_bounds__(256) void k_wpush(
   const int nch = (int)*nchp;
   const int b = blockIdx.x * 4 + wave;  // one chunk per wavefront, no workgroup barrier
   if (b >= nch) return;
+  // a particle left its window in the count pass: this sort is redone anyway
+  if (SCATTER && *(volatile const uint32_t *)wflag != 0u) return;
   uint32_t *cnt = lcnt[wave];
   const uint32_t beg = wbeg[b], end = wbeg[b + 1], ntot = wbeg[nch];
   // the chunk's replica: its constants, force, cell offset and slot base
@@ -567,26 +559,18 @@ __global__ __launch_bounds__(256) void k_wpush(
         r = push_one(r, (uint64_t)((int64_t)p - slot0), kq, kk, drift, mode, step_lo, step_hi,
                      err);
         d = window_offset(cell_key(r, kq), lo, len, nc, periodic);
-        if (d < 0) {
-          atomicOr(wflag, 1u);
-          valid = false;
-        }
+        if (d < 0) valid = false;
+      }
+      // a particle outside the window: the sort is redone by the full
+      // histogram, so the count pass stops here (one flag store per wavefront)
+      if (!SCATTER && __ballot(p < end && d < 0)) {
+        if (lane == 0) atomicOr(wflag, 1u);
+        return;
       }
       if (SCATTER) {
         // stable rank among the lanes with the same window cell, as k_push
-        uint32_t rk = 0, tot = 0;
-        uint64_t active = __ballot(valid);
-        while (active) {
-          const int leader = __ffsll((unsigned long long)active) - 1;
-          const int ld = __builtin_amdgcn_readlane(d, leader);
-          const bool mine = valid && d == ld;
-          const uint64_t m = __ballot(mine);
-          if (mine) {
-            rk = (uint32_t)__popcll(m & lt);
-            tot = (uint32_t)__popcll(m);
-          }
-          active &= ~m;
-        }
+        const uint64_t eq = match_key((uint32_t)d, valid, key_bits(WMAX));
+        const uint32_t rk = (uint32_t)__popcll(eq & lt), tot = (uint32_t)__popcll(eq);
         uint32_t base = 0;
         if (valid) base = cnt[d];
         __builtin_amdgcn_wave_barrier();
@@ -988,15 +972,6 @@ __device__ __forceinline__ void draw_dsmc_pair(uint32_t c, uint32_t k, const uin
   if (lj >= li) lj += 1u;
   *oi = start[c] + li;
   *oj = start[c] + lj;
-}
-
-// Hardware places workgroup b on XCD b % 8.  Renumber blocks so that each
-// XCD runs a contiguous range of logical blocks: neighbouring cells (and the
-// operations drawn in them) then share one XCD's L2.
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-  const uint32_t per = nb / 8u;
-  if (per == 0u || b >= per * 8u) return b;
-  return (b % 8u) * per + b / 8u;
 }
 
 // First candidate of region x: the smallest k with op_region(k, M) >= x.
