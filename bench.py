@@ -67,6 +67,8 @@ def main():
                     help="exact-solver snapshots the network trains on (one per step)")
     ap.add_argument("--sur-iters", type=int, default=2000, help="Adam steps of the training")
     ap.add_argument("--sur-steps", type=int, default=20, help="timed steps with and without the network")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time the steps one call at a time only (no HIP-graph replays)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 1 GiB (a known-size kernel for "
                          "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
@@ -158,26 +160,67 @@ def main():
         sim.init_omega_max(meta["n_gas"])
     if args.pressure:
         sim.measure_pressure(True)
+    # ESMC interface and replica steps run as HIP-graph replays (no host wait
+    # inside a step, bit for bit the same steps as one mcpm_step call each);
+    # the first warmup step is a plain call (it sizes the capture)
+    use_graph = (not args.no_graph and args.workload in ("interface", "replicas")
+                 and not args.pressure)
     step = 0
     t0 = time.time()
-    for _ in range(args.warmup):
-        st = sim.step(step)
+    for w in range(args.warmup):
+        if use_graph and w > 0:
+            if w == 1:
+                sim.graph_capture()
+            st = sim.graph_steps(step, 1)
+        else:
+            st = sim.step(step)
         step += 1
+    if use_graph and args.warmup < 2:
+        sim.graph_capture()
     sim.sync()
     log(f"warmup {args.warmup} steps in {time.time() - t0:.2f}s, last {st}")
 
     # the measured steps, no instrumentation inside
     sim.sync()
-    stats = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        stats.append(sim.step(step))
-        step += 1
+    if use_graph:
+        gs = sim.graph_steps(step, args.steps)
+        stats = [{k: v / args.steps for k, v in gs.items()}]
+        step += args.steps
+    else:
+        stats = []
+        for _ in range(args.steps):
+            stats.append(sim.step(step))
+            step += 1
     sim.sync()
     t1 = time.perf_counter()
     elapsed = t1 - t0
     value = N * args.steps / elapsed
     ms_step = 1000.0 * elapsed / args.steps
+    log(f"timed {args.steps} steps ({'graph replays' if use_graph else 'one call each'}): "
+        f"{ms_step:.3f} ms/step")
+    # one call per step against graph replays, in alternating blocks (the
+    # step cost drifts up as omega_max grows, so blocks of each side alternate)
+    mode_cmp = None
+    if use_graph:
+        blk = max(2, args.steps // 4)
+        t_e = t_g = 0.0
+        for _ in range(2):
+            sim.sync()
+            t0 = time.perf_counter()
+            for _ in range(blk):
+                sim.step(step)
+                step += 1
+            sim.sync()
+            t_e += time.perf_counter() - t0
+            t0 = time.perf_counter()
+            sim.graph_steps(step, blk)
+            step += blk
+            sim.sync()
+            t_g += time.perf_counter() - t0
+        mode_cmp = {"steps_each": 2 * blk, "one_call_per_step_ms": 1000.0 * t_e / (2 * blk),
+                    "graph_ms": 1000.0 * t_g / (2 * blk)}
+        log(f"alternating blocks: {mode_cmp}")
     # the reference's cell_update takes the cell moments every step and its
     # post-processing samples them (cell_poly_velocity); the step above
     # leaves out what the collision model does not need: the same steps with
@@ -197,8 +240,9 @@ def main():
     # as many steps again with HIP-event timers on the library's stream, for
     # the per-kernel figures (events around every launch cost a little time)
     sim.timing_enable(True)
+    tstats = []
     for _ in range(args.steps):
-        sim.step(step)
+        tstats.append(sim.step(step))
         step += 1
     sim.sync()
     # HIP events on the library's stream: the collision rounds per k_round
@@ -212,10 +256,14 @@ def main():
         timers[name] = {"avg_ms": ms, "calls": calls, "per_step_ms": ms * calls / args.steps}
     sim.timing_enable(False)
     timers = {k: v for k, v in timers.items() if v["calls"] > 0}
+    # operation counts of the timed steps; round and revisit counts (and the
+    # roofline's per-launch figures) of the same steps as the timers
     acc = float(np.mean([s["accepted"] for s in stats]))
     valid = float(np.mean([s["valid"] for s in stats]))
-    rounds = float(np.mean([s["rounds"] for s in stats]))
-    revisits = float(np.mean([s["revisits"] for s in stats]))
+    rounds = float(np.mean([s["rounds"] for s in tstats]))
+    revisits = float(np.mean([s["revisits"] for s in tstats]))
+    t_valid = float(np.mean([s["valid"] for s in tstats]))
+    t_acc = float(np.mean([s["accepted"] for s in tstats]))
     nbands = sim.get_tuning(mcpm.TUNE_BANDS)
     log(f"timed {args.steps} steps: {ms_step:.3f} ms/step, {value:.4g} particle-steps/s")
     log(f"phases: {json.dumps(timers)}; valid ops {valid:.0f}, accepted {acc:.0f}, rounds {rounds}")
@@ -236,7 +284,7 @@ def main():
     sort_bytes = 3.0 * rec * N * passes
     # collision rounds: each valid operation reads its 16-byte record once and
     # the two particle records once; an accepted collision writes both back
-    coll_bytes = valid * (2.0 * rec + 16.0) + acc * 2.0 * rec
+    coll_bytes = t_valid * (2.0 * rec + 16.0) + t_acc * 2.0 * rec
     step_alg = 44 if plane else 36      # 2 S_p + 4 with S_p = 20 (plane) or 16 (slab)
     if fpv_ms >= max(coll_ms, sort_ms):
         # k_fp_velocity: one compulsory read and write of each 16-byte record
@@ -261,7 +309,7 @@ def main():
                 "step_frac": round(value * step_alg / 1e9 / HBM_PEAK_GBS, 4)}
     if dom == "k_check + k_round":
         roofline["random_access"] = random_access_roofline(
-            N, int(rec), valid, acc, revisits, nbands, coll_ms, launches)
+            N, int(rec), t_valid, t_acc, revisits, nbands, coll_ms, launches)
 
     cpu = None
     if not args.no_cpu_baseline and args.workload == "interface":
@@ -307,6 +355,9 @@ def main():
         "config": {"workload": desc,
                    "particles": N, "cells": args.cells, "parallelism": "1 GPU",
                    "pressure_sampling": bool(args.pressure),
+                   "step_mode": ("HIP-graph replays (mcpm_graph_steps)" if use_graph
+                                 else "one mcpm_step call per step"),
+                   "graph_vs_one_call_per_step": mode_cmp,
                    "ms_per_step_with_moments_and_sampling": sampled_ms,
                    "unbatched_ms_per_step": unbatched,
                    "valid_ops_per_step": valid, "accepted_per_step": acc,

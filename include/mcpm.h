@@ -258,6 +258,33 @@ int mcpm_stream_bc(mcpm_ctx *ctx, int64_t step, int32_t with_kick);
    boundary and re-sort. */
 int mcpm_step(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
 
+/* ---- host-free steps (no reference counterpart: main.cpp:389-669 calls the
+   step's functions one by one from the host) ------------------------------
+   mcpm_graph_capture records one mcpm_step of the context as HIP graphs
+   (one per parity of the particle buffers): every launch reads the step
+   number, the candidate count and the round tags from device memory, the
+   reservation rounds are `rounds` grid rounds (0: the band count + 6)
+   followed by one workgroup that finishes what is left, and nothing waits
+   for the host inside a step.  Covers ESMC (Collision_ESMC_new3) on a slab
+   or a batch of slabs, without a mean field or with the screened-Poisson
+   one, sorted by the windowed sort, without pressure sampling, particle
+   ids, per-cell weights or the debug log (MCPM_ERR_ARG for other models,
+   MCPM_ERR_STATE when the particles are not binned or the windowed sort
+   does not apply).  `op_capacity` bounds the candidates of one step (0:
+   twice the last collision call's, at least N).
+   mcpm_graph_steps runs steps step0 .. step0 + nsteps - 1 by replaying the
+   graphs; the results are bit for bit those of as many mcpm_step calls.
+   It looks at the device state every 64 steps.  A step whose candidates
+   exceed the capacity is recaptured with twice the capacity and run again;
+   a step whose particle left its sort window finishes its sort on the host
+   (full histogram) and the run continues, recaptured; stats are sums over
+   the steps (rounds: summed; revisits: operations the one-workgroup tail
+   handled).  Allocates only when it captures. */
+int mcpm_graph_capture(mcpm_ctx *ctx, int64_t op_capacity, int32_t rounds);
+int mcpm_graph_steps(mcpm_ctx *ctx, int64_t step0, int64_t nsteps,
+                     mcpm_collision_stats *stats);
+int mcpm_graph_release(mcpm_ctx *ctx);
+
 /* ---- cell state ---------------------------------------------------------- */
 /* omega_max initialisation of main.cpp:250-269: crm = 10 sqrt(kb T_c / m),
    omega_max = 4 pi sigma^2 crm n_c dt, times Y(n_gas) for ESMC.  T_c is

@@ -164,6 +164,13 @@ struct mcpm_ctx {
   int64_t fno_every = 50;
   int64_t cur_step = 0;
   mcpm_fno_field_stats fno_last{};
+  // graph-replayed steps: the run state on the device, the captured step for
+  // each parity of cur, and what the capture assumed
+  StepDev *sd = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  int64_t g_cap = 0, g_n = -1;
+  int64_t last_m = 0;  // candidates of the last collision call
+  int g_rounds = 0, g_wdelta = 0;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[T_NCLASS];
@@ -549,17 +556,18 @@ int chunk_for(int64_t n, int fixed) {
 // enough that a chunk spans a few cells.
 // Exclusive scan of n counts into out[0..n] (out[n] the total): one
 // workgroup for short arrays, tiles over many workgroups for long ones.
-int excl_scan(mcpm_ctx *c, const uint32_t *in, uint32_t *out, int n) {
+int excl_scan(mcpm_ctx *c, const uint32_t *in, uint32_t *out, int n,
+              const StepDev *sd = nullptr) {
   if (n <= 16384) {
-    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, in, out, n);
+    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, in, out, n, sd);
   } else {
     const int nt = (n + SCAN_TILE - 1) / SCAN_TILE;
     if (nt > c->scan_tiles) return MCPM_ERR_CAPACITY;
     uint32_t *tsum = c->scan_tmp, *toff = c->scan_tmp + nt + 1;
-    hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)nt), dim3(1024), 0, c->s, in, n, tsum);
-    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, tsum, toff, nt);
+    hipLaunchKernelGGL(k_tile_sums, dim3((unsigned)nt), dim3(1024), 0, c->s, in, n, tsum, sd);
+    hipLaunchKernelGGL(k_excl_scan, dim3(1), dim3(1024), 0, c->s, tsum, toff, nt, sd);
     hipLaunchKernelGGL(k_tile_scan, dim3((unsigned)nt), dim3(1024), 0, c->s, in, out, n,
-                       (const uint32_t *)toff, nt);
+                       (const uint32_t *)toff, nt, sd);
   }
   LAUNCHCHK();
   return MCPM_OK;
@@ -600,9 +608,11 @@ bool window_ok(const mcpm_ctx *c, int mode) {
   return c->n >= 16 * (int64_t)c->nc;
 }
 
-// 1: a particle left its chunk's window (sort again by the full histogram).
+// The launches of a windowed sort (with a captured step's state when sd is
+// given: its step number, and the flag cleared by a kernel, not a memset).
 template <typename R>
-int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const double *drift) {
+int wsort_launch(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const double *drift,
+                 const StepDev *sd) {
   const uint32_t slo = (uint32_t)step, shi = (uint32_t)((uint64_t)step >> 32);
   const R *in = (const R *)c->P[c->cur];
   R *out = (R *)c->P[1 - c->cur];
@@ -611,30 +621,39 @@ int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const dou
   const int nblk = c->nwblk;
   const int nrep1 = std::max(c->kp.nrep, 1) + 1;
   uint32_t *nchp = c->wflag + 1;
-  HIPCHK(hipMemsetAsync(c->wflag, 0, sizeof(uint32_t), c->s));
+  if (sd) hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, c->s, c->wflag, (int64_t)1, 0u);
+  else HIPCHK(hipMemsetAsync(c->wflag, 0, sizeof(uint32_t), c->s));
   // the chunks: pieces of the cell blocks of the previous sort (start[])
   hipLaunchKernelGGL(k_wblocks, dim3((unsigned)((nblk + 255) / 256)), dim3(256), 0, c->s,
-                     c->wbcell, nblk, c->start, c->wpieces);
-  if (int st = excl_scan(c, c->wpieces, c->wpbase, nblk)) return st;
+                     c->wbcell, nblk, c->start, c->wpieces, sd);
+  if (int st = excl_scan(c, c->wpieces, c->wpbase, nblk, sd)) return st;
   hipLaunchKernelGGL(k_wfill, dim3((unsigned)(nblk + (nrep1 + 255) / 256)), dim3(256), 0, c->s,
                      c->wbcell, c->wbrep, nblk, c->start, c->wpbase, c->wrblk, nrep1,
-                     (uint32_t)c->n, c->wbeg, c->wrep, nchp, c->rchunk);
+                     (uint32_t)c->n, c->wbeg, c->wrep, nchp, c->rchunk, sd);
   // the most chunks there can be; waves past the count stop at once
   const unsigned gw = (unsigned)((c->wcap + 3) / 4);
   using real = typename R::real_type;
   hipLaunchKernelGGL((k_wpush<R, false>), dim3(gw), dim3(256), 0, c->s, in, nullptr, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
                      nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta, (real *)nullptr,
-                     nullptr, nullptr);
+                     nullptr, nullptr, sd);
   hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 3) / 4)), dim3(256), 0, c->s, c->wlo,
-                     c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc);
-  if (int st = excl_scan(c, c->cnt, c->start, c->nc)) return st;
+                     c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc, sd);
+  if (int st = excl_scan(c, c->cnt, c->start, c->nc, sd)) return st;
   hipLaunchKernelGGL((k_wpush<R, true>), dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
                      c->start, c->err, c->wflag, iid, oid, c->wdelta, ys_out<real>(c),
                      c->track_w ? c->pw[c->cur] : nullptr,
-                     c->track_w ? c->pw[1 - c->cur] : nullptr);
+                     c->track_w ? c->pw[1 - c->cur] : nullptr, sd);
   LAUNCHCHK();
+  return MCPM_OK;
+}
+
+// 1: a particle left its chunk's window (sort again by the full histogram).
+template <typename R>
+int run_wsort_t(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const double *drift) {
+  using real = typename R::real_type;
+  if (int st = wsort_launch<R>(c, mode, step, kick, drift, nullptr)) return st;
   uint32_t flag = 0;
   HIPCHK(hipMemcpyAsync(&flag, c->wflag, sizeof flag, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
@@ -760,6 +779,12 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   if (m <= c->op_cap) return MCPM_OK;
   int64_t cap = m + m / 4 + 1024;
   HIPCHK(hipStreamSynchronize(c->s));
+  // captured steps hold the old lists' addresses
+  for (auto &g : c->gexec)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
   if (c->opA) HIPCHK(hipFree(c->opA));
   if (c->log.p) HIPCHK(hipFree(c->log.p));
   c->opA = c->opB = c->opBand = nullptr;
@@ -796,7 +821,8 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   FPParams fpp = c->fpp;
   fpp.model = variant;
   hipLaunchKernelGGL(k_mcand, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->cnt, c->omax,
-                     c->nden, c->yc, c->mcand, nc, c->kp, (int)MODEL, variant, slo, shi);
+                     c->nden, c->yc, c->mcand, nc, c->kp, (int)MODEL, variant, slo, shi,
+                     (const StepDev *)nullptr);
   if (int st = excl_scan(c, c->mcand, c->cstart, nc)) return st;
   hipLaunchKernelGGL(k_zero_u32, dim3((CNT_TOTAL + 255) / 256), dim3(256), 0, c->s,
                      c->counters, (int)CNT_TOTAL);
@@ -804,13 +830,14 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   uint32_t M = 0;
   HIPCHK(hipMemcpyAsync(&M, c->cstart + nc, sizeof M, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
+  c->last_m = M;
   int st = ensure_ops(c, (int64_t)M);
   if (st) return st;
   mcpm_u32x4 pk = mcpm_philox(0u, 0u, slo, mcpm_ctr3(MCPM_TAG_PRIO, shi, 0u), c->kp.key0,
                               c->kp.key1);
   if (c->kp.nrep) {
     hipLaunchKernelGGL(k_rep_step, dim3((c->kp.nrep + 256) / 256), dim3(256), 0, c->s, c->kp,
-                       c->cstart, c->rprio, c->rcand_off, slo, shi);
+                       c->cstart, c->rprio, c->rcand_off, slo, shi, (const StepDev *)nullptr);
     LAUNCHCHK();
   }
   // ESMC priorities spread over 32 bits and enter the rounds in bands; DSMC
@@ -831,9 +858,11 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     Timer tm(c, T_GEN);
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
     hipLaunchKernelGGL(k_cand_count, dim3(nchunks), dim3(256), 0, c->s, M, pk.v[0], (int)MODEL,
-                       band_mul, nbands, c->gen_table, c->kp, c->cstart, nc, c->gen_cells);
+                       band_mul, nbands, c->gen_table, c->kp, c->cstart, nc, c->gen_cells,
+                       (const StepDev *)nullptr);
     hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
-                       nchunks, nbands * NREG, c->counters);
+                       nchunks, nbands * NREG, c->counters, (const uint32_t *)nullptr,
+                       (const StepDev *)nullptr);
     hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
     const typename R::real_type *ys =
         c->ys_valid ? (const typename R::real_type *)c->ys : nullptr;
@@ -841,7 +870,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
                        c->start,
                        c->cstart, nc, M, c->kp, slo, shi, pk.v[0], band_mul, nbands,
                        c->gen_table, (const uint32_t *)c->gen_cells, c->gen_reach, c->opBand,
-                       tabs[0], c->counters);
+                       tabs[0], c->counters, (const StepDev *)nullptr);
     LAUNCHCHK();
   }
   int rounds = 0;
@@ -861,12 +890,13 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
         const uint32_t mark = c->round_base & 0xFFu;
         hipLaunchKernelGGL((k_check<MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
                            (const uint4 *)c->opBand, tabs[r & 1], c->cflag, mark,
-                           (const uint32_t *)c->counters, c->dleft);
+                           (const uint32_t *)c->counters, c->dleft, (const StepDev *)nullptr);
         hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
                            P, in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
                            (const uint32_t *)tabs[r & 1], tabs[(r + 1) & 1],
                            (const uint8_t *)c->cflag, mark, c->counters, c->nden,
-                           c->yc, c->omax, c->omega_new, c->kp, slo, shi, fpp, lg);
+                           c->yc, c->omax, c->omega_new, c->kp, slo, shi, fpp, lg,
+                           (const StepDev *)nullptr);
       }
       c->round_base += 1u;
       LAUNCHCHK();
@@ -900,7 +930,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     }
   }
   hipLaunchKernelGGL(k_omega_finish, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->omax,
-                     c->omega_new, c->cnt, nc, (int)MODEL, variant);
+                     c->omega_new, c->cnt, nc, (int)MODEL, variant, (const StepDev *)nullptr);
   LAUNCHCHK();
   // the counters up to the loser counts of the last round
   std::vector<uint32_t> hc(CNT_LIST + (size_t)(rounds + 1) * NREG);
@@ -1119,6 +1149,157 @@ int vlasov_2d(mcpm_ctx *c) {
 }
 
 double contact_y_host(const mcpm_params *p, double n) { return contact_y(n, p->b, p->y_model); }
+
+// ---- graph-replayed ESMC steps ---------------------------------------------
+// What a captured step covers: Collision_ESMC_new3 on a slab (one or batched
+// replicas), no mean field or the screened-Poisson one, the windowed sort,
+// nothing that needs the host between launches (pressure face sums,
+// re-insertion, particle ids or weights, the debug log).
+bool graph_ok(const mcpm_ctx *c) {
+  if (c->p.dims != 1 || c->p.model != MCPM_MODEL_ESMC) return false;
+  if (c->p.long_range != MCPM_LR_NONE && c->p.long_range != MCPM_LR_SCREENED_POISSON) return false;
+  if (c->pressure_on || c->track_ids || c->track_w || c->kp.cell_ratio || c->debug_log) return false;
+  if (c->n <= 0 || !c->sorted || !c->order_sorted) return false;
+  return window_ok(c, PUSH_MOVE);
+}
+
+void graph_release(mcpm_ctx *c) {
+  for (auto &g : c->gexec)
+    if (g) {
+      hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  c->g_n = -1;
+}
+
+// One ESMC step (mcpm_step's order: collisions, field, kick + stream +
+// boundary + sort) as launches whose step-dependent values all come from
+// c->sd: the candidate count from cstart[nc] (grids sized for c->g_cap
+// candidates), the step and its priority key, the round tags; a fixed
+// g_rounds grid rounds, then the tail kernel.  The particles are read from
+// P[cur] and the sort writes P[1 - cur].
+template <typename R>
+int graph_step_launch(mcpm_ctx *c) {
+  const int nc = c->nc;
+  StepDev *sd = c->sd;
+  const StepDev *csd = sd;
+  R *P = (R *)c->P[c->cur];
+  FPParams fpp = c->fpp;
+  fpp.model = MCPM_MODEL_ESMC;
+  const uint32_t cap = (uint32_t)c->g_cap;
+  hipLaunchKernelGGL(k_mcand, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->cnt, c->omax,
+                     c->nden, c->yc, c->mcand, nc, c->kp, (int)COLL_ESMC, (int)MCPM_MODEL_ESMC,
+                     0u, 0u, csd);
+  if (int st = excl_scan(c, c->mcand, c->cstart, nc, csd)) return st;
+  hipLaunchKernelGGL(k_zero_u32, dim3((CNT_TOTAL + 255) / 256), dim3(256), 0, c->s, c->counters,
+                     (int)CNT_TOTAL);
+  hipLaunchKernelGGL(k_step_mid, dim3(1), dim3(64), 0, c->s, sd, (const uint32_t *)c->cstart, nc,
+                     cap, c->kp.key0, c->kp.key1);
+  if (c->kp.nrep)
+    hipLaunchKernelGGL(k_rep_step, dim3((c->kp.nrep + 256) / 256), dim3(256), 0, c->s, c->kp,
+                       c->cstart, c->rprio, c->rcand_off, 0u, 0u, csd);
+  const int nbands = std::max(1, std::min<int>(c->nbands, MAX_BANDS));
+  const uint32_t band_mul = (uint32_t)nbands;
+  const unsigned nch_cap = (unsigned)((cap + GEN_CHUNK - 1) / GEN_CHUNK);
+  uint32_t *const tabs[2] = {c->rtab, c->rtab + (size_t)c->p.capacity};
+  hipLaunchKernelGGL(k_cand_count, dim3(nch_cap), dim3(256), 0, c->s, 0u, 0u, (int)COLL_ESMC,
+                     band_mul, nbands, c->gen_table, c->kp, c->cstart, nc, c->gen_cells, csd);
+  hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table, 0u,
+                     nbands * NREG, c->counters, (const uint32_t *)c->cstart + nc, csd);
+  hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
+  using real = typename R::real_type;
+  hipLaunchKernelGGL((k_gen<R, COLL_ESMC>), dim3(nch_cap), dim3(256), 0, c->s, P,
+                     (const real *)c->ys, c->cnt, c->start, c->cstart, nc, 0u, c->kp, 0u, 0u, 0u,
+                     band_mul, nbands, c->gen_table, (const uint32_t *)c->gen_cells, c->gen_reach,
+                     c->opBand, tabs[0], c->counters, csd);
+  LAUNCHCHK();
+  const uint32_t seg = (cap + NREG - 1) / NREG;
+  const unsigned grid = (unsigned)((std::max(8, c->round_grid) + 7) / 8 * 8);
+  const OpLog nolog{};
+  for (int r = 0; r < c->g_rounds; ++r) {
+    const uint4 *in = (const uint4 *)((r & 1) ? c->opB : c->opA);
+    hipLaunchKernelGGL((k_check<COLL_ESMC>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
+                       (const uint4 *)c->opBand, tabs[r & 1], c->cflag, 0u,
+                       (const uint32_t *)c->counters, (uint32_t *)nullptr, csd);
+    hipLaunchKernelGGL((k_round<R, COLL_ESMC>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, P,
+                       in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
+                       (const uint32_t *)tabs[r & 1], tabs[(r + 1) & 1],
+                       (const uint8_t *)c->cflag, 0u, c->counters, c->nden, c->yc, c->omax,
+                       c->omega_new, c->kp, 0u, 0u, fpp, nolog, csd);
+  }
+  hipLaunchKernelGGL((k_round_tail<R, COLL_ESMC>), dim3(1), dim3(1024), 0, c->s, c->g_rounds,
+                     nbands, seg, P, c->opA, c->opB, tabs[0], tabs[1], c->cflag, c->counters,
+                     (const double *)c->nden, (const double *)c->yc, (const double *)c->omax,
+                     c->omega_new, c->kp, fpp, sd);
+  hipLaunchKernelGGL(k_omega_finish, dim3((nc + 255) / 256), dim3(256), 0, c->s, c->omax,
+                     c->omega_new, c->cnt, nc, (int)COLL_ESMC, (int)MCPM_MODEL_ESMC, csd);
+  const bool field = c->p.long_range == MCPM_LR_SCREENED_POISSON;
+  if (field)
+    hipLaunchKernelGGL(k_field, dim3(c->kp.nrep ? c->kp.nrep : 1), dim3(1024), 0, c->s, c->nden,
+                       c->fa, c->F2, c->kp, c->p.ast, c->p.bst, c->p.eqdist,
+                       (const FieldArrays *)c->rfa, csd);
+  LAUNCHCHK();
+  const Kick kick{c->F2, c->pot, c->kick_coef, nullptr};
+  if (int st = wsort_launch<R>(c, PUSH_MOVE | (field ? PUSH_KICK : 0), 0, kick, nullptr, csd))
+    return st;
+  hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, c->s, sd, (const uint32_t *)c->counters,
+                     (const uint32_t *)c->cstart, nc, (const uint32_t *)c->wflag);
+  LAUNCHCHK();
+  return MCPM_OK;
+}
+
+// Capture the step for both parities of cur (the sort swaps the particle
+// buffers every step, so replays alternate between the two graphs).
+template <typename R>
+int graph_capture_t(mcpm_ctx *c, int64_t cap, int rounds) {
+  graph_release(c);
+  cap = (cap + NREG - 1) / NREG * NREG;
+  if (cap > (int64_t)UINT32_MAX - GEN_CHUNK) return MCPM_ERR_ARG;
+  if (int st = ensure_ops(c, cap)) return st;
+  if (!c->sd) {
+    if (int st = dalloc(&c->sd, 1)) return st;
+  }
+  c->g_cap = cap;
+  c->g_rounds = rounds;
+  c->g_wdelta = c->wdelta;
+  const int cur0 = c->cur;
+  for (int par = 0; par < 2; ++par) {
+    c->cur = par;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+      c->cur = cur0;
+      return MCPM_ERR_HIP;
+    }
+    const int st = graph_step_launch<R>(c);
+    e = hipStreamEndCapture(c->s, &g);
+    if (st == MCPM_OK && e == hipSuccess) e = hipGraphInstantiate(&c->gexec[par], g, nullptr, nullptr, 0);
+    if (g) hipGraphDestroy(g);
+    if (st || e != hipSuccess) {
+      c->cur = cur0;
+      graph_release(c);
+      return st ? st : MCPM_ERR_HIP;
+    }
+  }
+  c->cur = cur0;
+  c->g_n = c->n;
+  return MCPM_OK;
+}
+
+int graph_capture(mcpm_ctx *c, int64_t cap, int rounds) {
+  return c->p.precision == MCPM_FP32 ? graph_capture_t<Rec<float>>(c, cap, rounds)
+                                     : graph_capture_t<Rec<double>>(c, cap, rounds);
+}
+
+template <typename R>
+int ys_fill_t(mcpm_ctx *c) {
+  using real = typename R::real_type;
+  hipLaunchKernelGGL(k_ys_from_records<R>, dim3(1024), dim3(256), 0, c->s,
+                     (const R *)c->P[c->cur], c->n, (real *)c->ys);
+  LAUNCHCHK();
+  c->ys_valid = true;
+  return MCPM_OK;
+}
 
 }  // namespace
 
@@ -1599,6 +1780,8 @@ int mcpm_destroy(mcpm_ctx *c) {
                   c->wbrep, c->wrblk};
   for (void *b : bufs)
     if (b) hipFree(b);
+  graph_release(c);
+  if (c->sd) hipFree(c->sd);
   for (int k = 0; k < T_NCLASS; ++k)
     for (auto &pr : c->ev[k]) {
       hipEventDestroy(pr.first);
@@ -1909,7 +2092,7 @@ int mcpm_poisson_solver(mcpm_ctx *c) {
   }
   hipLaunchKernelGGL(k_field, dim3(c->kp.nrep ? c->kp.nrep : 1), dim3(1024), 0, c->s, c->nden,
                      c->fa, c->F2, c->kp, c->p.ast, c->p.bst, c->p.eqdist,
-                     (const FieldArrays *)c->rfa);
+                     (const FieldArrays *)c->rfa, (const StepDev *)nullptr);
   LAUNCHCHK();
   c->kick_pending = true;
   return MCPM_OK;
@@ -2031,6 +2214,122 @@ int mcpm_step(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats) {
   st = mcpm_stream_bc(c, step, 1);
   c->have_closures = false;
   return st;
+}
+
+int mcpm_graph_capture(mcpm_ctx *c, int64_t op_capacity, int32_t rounds) {
+  if (!c) return MCPM_ERR_ARG;
+  if (c->p.dims != 1 || c->p.model != MCPM_MODEL_ESMC ||
+      (c->p.long_range != MCPM_LR_NONE && c->p.long_range != MCPM_LR_SCREENED_POISSON))
+    return MCPM_ERR_ARG;
+  if (!graph_ok(c)) return MCPM_ERR_STATE;
+  const int nbands = std::max(1, std::min<int>(c->nbands, MAX_BANDS));
+  if (rounds <= 0) rounds = nbands + 6;
+  if (rounds <= nbands || rounds >= MAX_ROUNDS) return MCPM_ERR_ARG;
+  if (op_capacity <= 0) op_capacity = std::max<int64_t>(2 * c->last_m, c->n) + 4096;
+  return graph_capture(c, op_capacity, rounds);
+}
+
+int mcpm_graph_release(mcpm_ctx *c) {
+  if (!c) return MCPM_ERR_ARG;
+  graph_release(c);
+  c->g_cap = 0;
+  return MCPM_OK;
+}
+
+int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_stats *stats) {
+  if (!c || step0 < 0 || nsteps < 0) return MCPM_ERR_ARG;
+  if (stats) memset(stats, 0, sizeof *stats);
+  if (c->g_cap <= 0) return MCPM_ERR_STATE;  // nothing captured
+  if (!graph_ok(c)) return MCPM_ERR_STATE;
+  // lists reallocated, other particles or wider sort windows since the capture
+  if (!c->gexec[0] || !c->gexec[1] || c->n != c->g_n || c->wdelta != c->g_wdelta)
+    if (int st = graph_capture(c, c->g_cap, c->g_rounds)) return st;
+  if (!c->ys_valid) {
+    const int st = c->p.precision == MCPM_FP32 ? ys_fill_t<Rec<float>>(c) : ys_fill_t<Rec<double>>(c);
+    if (st) return st;
+  }
+  int64_t done = 0, rsum = 0, tail = 0;
+  mcpm_collision_stats tot{};
+  StepDev h{};
+  h.lo = (uint32_t)step0;
+  h.hi = (uint32_t)((uint64_t)step0 >> 32);
+  h.mark = c->round_base;
+  constexpr int64_t BATCH = 64;  // replays between looks at the run state
+  bool eager = false;
+  while (done < nsteps) {
+    if (eager) {
+      // the windowed sort no longer applies: the remaining steps one by one
+      mcpm_collision_stats s1{};
+      if (int st = mcpm_step(c, step0 + done, &s1)) return st;
+      tot.candidates += s1.candidates;
+      tot.valid += s1.valid;
+      tot.accepted += s1.accepted;
+      rsum += s1.rounds;
+      ++done;
+      continue;
+    }
+    const StepDev up = h;
+    HIPCHK(hipMemcpyAsync(c->sd, &up, sizeof up, hipMemcpyHostToDevice, c->s));
+    const int cur0 = c->cur;
+    const int64_t todo = std::min(BATCH, nsteps - done);
+    for (int64_t k = 0; k < todo; ++k) HIPCHK(hipGraphLaunch(c->gexec[(cur0 + k) & 1], c->s));
+    HIPCHK(hipMemcpyAsync(&h, c->sd, sizeof h, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->cur = (cur0 + (int)(h.steps & 1u)) & 1;
+    done += h.steps;
+    tot.candidates += (int64_t)h.cand;
+    tot.valid += (int64_t)h.valid;
+    tot.accepted += (int64_t)h.acc;
+    rsum += (int64_t)h.rsum;
+    tail += (int64_t)h.tail_ops;
+    c->round_base = h.mark;
+    int64_t next = (int64_t)(((uint64_t)h.hi << 32) | h.lo);
+    if (h.stop & STOP_ROUNDS) return MCPM_ERR_CAPACITY;
+    if (h.stop & STOP_WINDOW) {
+      // step `next` ran its collisions and field; a particle left its sort
+      // window, so its push and sort run here (full histogram), and the
+      // wider windows need a new capture
+      // (the failed windowed sort left the cell bounds unusable: the full
+      // histogram sorts without them; the windows widen as after an eager
+      // step's failed windowed sort)
+      c->kick_pending = c->p.long_range == MCPM_LR_SCREENED_POISSON;
+      const bool we = c->w_enabled;
+      c->w_enabled = false;
+      const int sst = mcpm_stream_bc(c, next, 1);
+      c->w_enabled = we;
+      if (sst) return sst;
+      ++c->w_runs;
+      ++c->w_redone;
+      if (c->wdelta < WDELTA_MAX) c->wdelta = std::min(2 * c->wdelta, (int)WDELTA_MAX);
+      else ++c->w_fallbacks;
+      ++done;
+      ++next;
+      if (graph_ok(c)) {
+        if (int st = graph_capture(c, c->g_cap, c->g_rounds)) return st;
+      } else {
+        eager = true;
+      }
+    } else if (h.stop & STOP_CAPACITY) {
+      // nothing of step `next` ran: longer operation lists
+      if (int st = graph_capture(c, 2 * c->g_cap, c->g_rounds)) return st;
+    }
+    h = StepDev{};
+    h.lo = (uint32_t)next;
+    h.hi = (uint32_t)((uint64_t)next >> 32);
+    h.mark = c->round_base;
+  }
+  c->sorted = true;
+  c->order_sorted = true;
+  c->ys_valid = true;
+  c->kick_pending = false;
+  c->keys_current = false;
+  c->have_closures = false;
+  if (stats) {
+    *stats = tot;
+    stats->rounds = (int32_t)std::min<int64_t>(rsum, INT32_MAX);
+    stats->revisits = (int32_t)std::min<int64_t>(tail, INT32_MAX);
+  }
+  return check_err(c);
 }
 
 int mcpm_moments(mcpm_ctx *c, double *out) {

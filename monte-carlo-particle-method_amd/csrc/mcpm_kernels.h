@@ -31,6 +31,27 @@ enum : int {
   CNT_TOTAL = CNT_BSTART + MAX_BANDS * NREG + 1
 };
 
+// State of a run of graph-replayed steps, in device memory.  The kernels of
+// a captured step read the step number, its priority key and its round tags
+// from here instead of from their arguments, and return at once while stop
+// is set: a step that cannot finish on the device (its candidates exceed
+// the operation lists, a particle left its sort window, rounds ran out)
+// stops every later kernel of the run, and the host completes or reports it.
+struct StepDev {
+  uint32_t lo, hi;   // step number
+  uint32_t pk;       // the step's candidate priority key (k_step_mid)
+  uint32_t mark;     // tag of the step's round 0 (round r: mark + r, low byte)
+  uint32_t stop;     // STOP_* bits
+  uint32_t rounds;   // rounds the step used (k_round_tail)
+  uint32_t steps;    // steps completed in the run
+  uint32_t pad;
+  unsigned long long cand, valid, acc, rsum, tail_ops;  // totals over the completed steps
+};
+enum : uint32_t { STOP_WINDOW = 1u, STOP_CAPACITY = 2u, STOP_ROUNDS = 4u };
+__device__ __forceinline__ bool sd_stopped(const StepDev *sd) {
+  return sd != nullptr && *(volatile const uint32_t *)&sd->stop != 0u;
+}
+
 // Region of candidate k of M: the candidates are in cell order, so a region
 // is a run of neighbouring cells.
 __host__ __device__ __forceinline__ uint32_t op_region(uint32_t k, uint32_t M) {
@@ -87,16 +108,17 @@ __global__ void k_wpush(const R *in, R *out, KParams kp, Kick kick, const double
                         const int32_t *wrep, const uint32_t *nchp, int32_t *wlo, int32_t *wlen,
                         uint32_t *wtab, const uint32_t *start, uint32_t *err, uint32_t *wflag,
                         const uint32_t *iid, uint32_t *oid, int wdelta,
-                        typename R::real_type *oys, const int32_t *iw, int32_t *ow);
+                        typename R::real_type *oys, const int32_t *iw, int32_t *ow,
+                        const StepDev *sd);
 __global__ void k_wblocks(const int32_t *bcell, int nblk, const uint32_t *start,
-                          uint32_t *pieces);
+                          uint32_t *pieces, const StepDev *sd);
 __global__ void k_wfill(const int32_t *bcell, const int32_t *brep, int nblk,
                         const uint32_t *start, const uint32_t *pbase, const int32_t *rblk,
                         int nrep1, uint32_t n, uint32_t *wbeg, int32_t *wrep, uint32_t *nchp,
-                        uint32_t *rchunk);
+                        uint32_t *rchunk, const StepDev *sd);
 __global__ void k_wscan(const int32_t *wlo, const int32_t *wlen, uint32_t *wtab,
                         const uint32_t *nchp, const uint32_t *rchunk, KParams kp, uint32_t *cnt,
-                        double *nden, double *yc);
+                        double *nden, double *yc, const StepDev *sd);
 // Most bins of one sort pass (one LDS histogram of 64 KB per chunk); more
 // cells sort by two digits of the cell key, up to MAX_CELLS.
 constexpr int SORT_BINS = 16384;
@@ -115,13 +137,13 @@ __global__ void k_scan_c(uint32_t *table, const uint32_t *partial, int nchunks, 
 // variant: the mcpm_model of a COLL_DSMC collision (DSMC_VHS, DSMC_HS, CBA_VHS).
 __global__ void k_mcand(const uint32_t *cnt, const double *omax, const double *nden,
                         const double *yc, uint32_t *mcand, int nc, KParams kp, int model,
-                        int variant, uint32_t step_lo, uint32_t step_hi);
-__global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc);
+                        int variant, uint32_t step_lo, uint32_t step_hi, const StepDev *sd);
+__global__ void k_excl_scan(const uint32_t *in, uint32_t *out, int nc, const StepDev *sd);
 // the same scan over several workgroups for long arrays (SCAN_TILE each)
 constexpr int SCAN_TILE = 4096;
-__global__ void k_tile_sums(const uint32_t *in, int n, uint32_t *tsum);
+__global__ void k_tile_sums(const uint32_t *in, int n, uint32_t *tsum, const StepDev *sd);
 __global__ void k_tile_scan(const uint32_t *in, uint32_t *out, int n, const uint32_t *toff,
-                            int nt);
+                            int nt, const StepDev *sd);
 // Priority band of an operation: the top bits of the priority scaled to
 // nbands (band_mul = nbands for priorities spread over 32 bits, 0 for one band).
 __device__ __forceinline__ uint32_t op_band(uint32_t pr, uint32_t band_mul, int nbands) {
@@ -147,12 +169,13 @@ constexpr uint32_t GEN_CHUNK = 2048;
 constexpr int NLIST = MAX_BANDS * NREG;
 __global__ void k_cand_count(uint32_t M, uint32_t prio_key, int model, uint32_t band_mul,
                              int nbands, uint32_t *table, KParams kp, const uint32_t *cstart,
-                             int nc, uint32_t *gcells);
+                             int nc, uint32_t *gcells, const StepDev *sd);
 // cells of one candidate chunk whose candidate starts the draw keeps in LDS,
 // and partner cells whose counts and starts it keeps there
 constexpr int GEN_CELLS = 512;
 constexpr int GEN_WIN = 1024;
-__global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters);
+__global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters,
+                            const uint32_t *Mp, const StepDev *sd);
 __global__ void k_list_starts(uint32_t *counters, int nl);
 template <typename R, int MODEL>
 __global__ void k_gen(const R *P, const typename R::real_type *ys, const uint32_t *cnt,
@@ -160,14 +183,14 @@ __global__ void k_gen(const R *P, const typename R::real_type *ys, const uint32_
                       const uint32_t *cstart, int nc, uint32_t M, KParams kp, uint32_t step_lo,
                       uint32_t step_hi, uint32_t prio_key, uint32_t band_mul, int nbands,
                       const uint32_t *table, const uint32_t *gcells, int32_t reach,
-                      uint4 *bands, uint32_t *tab0, uint32_t *counters);
+                      uint4 *bands, uint32_t *tab0, uint32_t *counters, const StepDev *sd);
 // A reservation round: k_check resolves the particles that several of the
 // round's operations share, k_round commits the winners and admits the next
 // round's operations.
 template <int MODEL>
 __global__ void k_check(int r, int nbands, uint32_t seg, const uint4 *in, const uint4 *bands,
                         uint32_t *tab, uint8_t *cflag, uint32_t tag8, const uint32_t *counters,
-                        uint32_t *host_left);
+                        uint32_t *host_left, const StepDev *sd);
 template <typename R, int MODEL>
 __global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
                         const uint4 *bands, uint4 *out, const uint32_t *tab_cur,
@@ -175,9 +198,26 @@ __global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
                         uint32_t *counters,
                         const double *nden, const double *yc, const double *omax,
                         double *omega_new, KParams kp, uint32_t step_lo, uint32_t step_hi,
-                        FPParams fpp, OpLog log);
+                        FPParams fpp, OpLog log, const StepDev *sd);
+// The rounds left after a captured step's grid rounds, in one workgroup:
+// round r0 (>= nbands: every band admitted) onwards until no operation is
+// left; the step's round count goes to sd->rounds.
+template <typename R, int MODEL>
+__global__ void k_round_tail(int r0, int nbands, uint32_t seg, R *P, uint4 *opA, uint4 *opB,
+                             uint32_t *tab0, uint32_t *tab1, uint8_t *cflag, uint32_t *counters,
+                             const double *nden, const double *yc, const double *omax,
+                             double *omega_new, KParams kp, FPParams fpp, StepDev *sd);
 __global__ void k_omega_finish(double *omax, double *omega_new, const uint32_t *cnt, int nc,
-                               int model, int variant);
+                               int model, int variant, const StepDev *sd);
+// Graph-run bookkeeping: after the candidate scan (capacity check, priority
+// key), and at the end of a step (totals, window check, next step).
+__global__ void k_step_mid(StepDev *sd, const uint32_t *cstart, int nc, uint32_t cap,
+                           uint32_t key0, uint32_t key1);
+__global__ void k_step_end(StepDev *sd, const uint32_t *counters, const uint32_t *cstart, int nc,
+                           const uint32_t *wflag);
+// y of every record into the sorted-y array (the draw's positions)
+template <typename R>
+__global__ void k_ys_from_records(const R *P, int64_t n, typename R::real_type *ys);
 // Most candidates one cell may run in one Collision_CBA_HS call.
 constexpr uint32_t CBA_MAX_CAND = 1u << 26;
 template <typename R>
@@ -187,12 +227,13 @@ __global__ void k_cba_hs(R *P, const uint32_t *cnt, const uint32_t *start, const
 // Batched replicas, per step: each replica's candidate priority key and its
 // first candidate (rcand_off[nrep] = all candidates).
 __global__ void k_rep_step(KParams kp, const uint32_t *cstart, uint32_t *rprio,
-                           uint32_t *rcand_off, uint32_t step_lo, uint32_t step_hi);
+                           uint32_t *rcand_off, uint32_t step_lo, uint32_t step_hi,
+                           const StepDev *sd);
 __global__ void k_zero_u32(uint32_t *p, int n);
 __global__ void k_fill_u32(uint32_t *p, int64_t n, uint32_t v);
 __global__ void k_iota_u32(uint32_t *p, int64_t n);
 __global__ void k_field(const double *nden, FieldArrays fa, double *F2, KParams kp, double ast,
-                        double bst, double eqdist, const FieldArrays *rfa);
+                        double bst, double eqdist, const FieldArrays *rfa, const StepDev *sd);
 template <typename R>
 __global__ void k_energy(const R *P, int64_t n, double *part);
 __global__ void k_energy_soa(const double *u1, const double *u2, const double *u3, int64_t n,

@@ -479,8 +479,13 @@ __global__ __launch_bounds__(256) void k_wpush(
     uint32_t *__restrict__ wtab, const uint32_t *__restrict__ start, uint32_t *__restrict__ err,
     uint32_t *__restrict__ wflag, const uint32_t *__restrict__ iid, uint32_t *__restrict__ oid,
     int wdelta, typename R::real_type *__restrict__ oys, const int32_t *__restrict__ iw,
-    int32_t *__restrict__ ow) {
+    int32_t *__restrict__ ow, const StepDev *__restrict__ sd) {
   __shared__ uint32_t lcnt[4][WMAX];
+  if (sd_stopped(sd)) return;
+  if (sd) {
+    step_lo = sd->lo;
+    step_hi = sd->hi;
+  }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nch = (int)*nchp;
   const int b = blockIdx.x * 4 + wave;  // one chunk per wavefront, no workgroup barrier
@@ -617,9 +622,10 @@ __global__ __launch_bounds__(256) void k_wpush(
 // numbers them, k_wfill writes the pieces' bounds and replicas, the total
 // (*nch, wbeg[*nch] = n) and the first piece of each replica.
 __global__ void k_wblocks(const int32_t *__restrict__ bcell, int nblk,
-                          const uint32_t *__restrict__ start, uint32_t *__restrict__ pieces) {
+                          const uint32_t *__restrict__ start, uint32_t *__restrict__ pieces,
+                          const StepDev *__restrict__ sd) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= nblk) return;
+  if (j >= nblk || sd_stopped(sd)) return;
   const uint32_t a = start[bcell[j]], z = start[bcell[j + 1]];
   pieces[j] = (z - a + WCH - 1) / WCH;
 }
@@ -630,7 +636,8 @@ __global__ void k_wfill(const int32_t *__restrict__ bcell, const int32_t *__rest
                         const uint32_t *__restrict__ pbase, const int32_t *__restrict__ rblk,
                         int nrep1, uint32_t n, uint32_t *__restrict__ wbeg,
                         int32_t *__restrict__ wrep, uint32_t *__restrict__ nchp,
-                        uint32_t *__restrict__ rchunk) {
+                        uint32_t *__restrict__ rchunk, const StepDev *__restrict__ sd) {
+  if (sd_stopped(sd)) return;
   const int j = blockIdx.x;
   if (j < nblk) {
     const uint32_t a = start[bcell[j]], z = start[bcell[j + 1]];
@@ -688,7 +695,9 @@ __global__ __launch_bounds__(256) void k_wscan(const int32_t *__restrict__ wlo,
                                                const uint32_t *__restrict__ rchunk, KParams kp,
                                                uint32_t *__restrict__ cnt,
                                                double *__restrict__ nden,
-                                               double *__restrict__ yc) {
+                                               double *__restrict__ yc,
+                                               const StepDev *__restrict__ sd) {
+  if (sd_stopped(sd)) return;
   const int lane = threadIdx.x & 63;
   const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   if (k >= kp.n_cells) return;
@@ -821,9 +830,13 @@ __global__ void k_scan_c(uint32_t *__restrict__ table, const uint32_t *__restric
 __global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restrict__ omax,
                         const double *__restrict__ nden, const double *__restrict__ yc,
                         uint32_t *__restrict__ mcand, int nc, KParams kp, int model, int variant,
-                        uint32_t step_lo, uint32_t step_hi) {
+                        uint32_t step_lo, uint32_t step_hi, const StepDev *__restrict__ sd) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+  if (c >= nc || sd_stopped(sd)) return;
+  if (sd) {
+    step_lo = sd->lo;
+    step_hi = sd->hi;
+  }
   // batched replicas draw with their own key and local cell numbers
   const int32_t rq = kp.nrep ? kp.cell_rep[c] : 0;
   const KParams &kq = kp.nrep ? kp.rkp[rq] : kp;
@@ -854,8 +867,10 @@ __global__ void k_mcand(const uint32_t *__restrict__ cnt, const double *__restri
 
 // Exclusive scan of a per-cell u32 array (one workgroup); total in out[nc].
 __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__ in,
-                                                    uint32_t *__restrict__ out, int nc) {
+                                                    uint32_t *__restrict__ out, int nc,
+                                                    const StepDev *__restrict__ sd) {
   __shared__ uint32_t sh[1024];
+  if (sd_stopped(sd)) return;
   const int tid = threadIdx.x;
   const int per = (nc + 1023) / 1024;
   const int c0 = tid * per;
@@ -888,8 +903,10 @@ __global__ __launch_bounds__(1024) void k_excl_scan(const uint32_t *__restrict__
 // k_excl_scan scans the tile sums into tile offsets, k_tile_scan scans each
 // tile from its offset.  Integer sums: the same result as one serial scan.
 __global__ __launch_bounds__(1024) void k_tile_sums(const uint32_t *__restrict__ in, int n,
-                                                    uint32_t *__restrict__ tsum) {
+                                                    uint32_t *__restrict__ tsum,
+                                                    const StepDev *__restrict__ sd) {
   __shared__ uint32_t sw[16];
+  if (sd_stopped(sd)) return;
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + 4 * (int64_t)threadIdx.x;
   uint32_t v = 0;
 #pragma unroll
@@ -907,8 +924,10 @@ __global__ __launch_bounds__(1024) void k_tile_sums(const uint32_t *__restrict__
 
 __global__ __launch_bounds__(1024) void k_tile_scan(const uint32_t *__restrict__ in,
                                                     uint32_t *__restrict__ out, int n,
-                                                    const uint32_t *__restrict__ toff, int nt) {
+                                                    const uint32_t *__restrict__ toff, int nt,
+                                                    const StepDev *__restrict__ sd) {
   __shared__ uint32_t sw[16];
+  if (sd_stopped(sd)) return;
   const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + 4 * (int64_t)threadIdx.x;
   uint32_t e[4], v = 0;
 #pragma unroll
@@ -1017,9 +1036,17 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
                                                     uint32_t band_mul, int nbands,
                                                     uint32_t *__restrict__ table, KParams kp,
                                                     const uint32_t *__restrict__ cstart,
-                                                    int nc, uint32_t *__restrict__ gcells) {
+                                                    int nc, uint32_t *__restrict__ gcells,
+                                                    const StepDev *__restrict__ sd) {
   __shared__ uint32_t scount[NLIST];
   __shared__ uint32_t scell;
+  if (sd) {
+    // a captured step: the candidates and key of this step, chunks past them idle
+    if (sd_stopped(sd)) return;
+    M = cstart[nc];
+    prio_key = sd->pk;
+    if (blockIdx.x * GEN_CHUNK >= M) return;
+  }
   const int nl = nbands * NREG;
   if ((int)threadIdx.x < NLIST) scount[threadIdx.x] = 0;
   const uint32_t lo = blockIdx.x * GEN_CHUNK;
@@ -1057,11 +1084,14 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
 // list's total goes to the band counters.
 __global__ __launch_bounds__(1024) void k_cand_scan(uint32_t *__restrict__ table,
                                                     uint32_t nchunks, int nl,
-                                                    uint32_t *__restrict__ counters) {
+                                                    uint32_t *__restrict__ counters,
+                                                    const uint32_t *__restrict__ Mp,
+                                                    const StepDev *__restrict__ sd) {
   __shared__ uint32_t sw[16];
   __shared__ uint32_t carry;
   const int q = blockIdx.x;
-  if (q >= nl) return;
+  if (q >= nl || sd_stopped(sd)) return;
+  if (Mp) nchunks = (*Mp + GEN_CHUNK - 1) / GEN_CHUNK;
   if (threadIdx.x == 0) carry = 0;
   __syncthreads();
   for (uint32_t b0 = 0; b0 < nchunks; b0 += blockDim.x) {
@@ -1116,14 +1146,27 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              const uint32_t *__restrict__ gcells, int32_t reach,
                                              uint4 *__restrict__ bands,
                                              uint32_t *__restrict__ tab0,
-                                             uint32_t *__restrict__ counters) {
+                                             uint32_t *__restrict__ counters,
+                                             const StepDev *__restrict__ sd) {
   __shared__ uint32_t spos[NLIST];
   __shared__ uint32_t s_cs[GEN_CELLS + 1];
   __shared__ uint32_t s_wn[GEN_WIN], s_ws[GEN_WIN];
   const int nl = nbands * NREG;
   // consecutive chunks (one cell's candidates, drawing from its particles) on
   // one XCD, whose L2 then serves their reads of the cell's records
-  const uint32_t bx = xcd_block(blockIdx.x, gridDim.x);
+  uint32_t nchunks = gridDim.x;
+  if (sd) {
+    // a captured step: its candidates, step and key; the grid is sized for
+    // the most candidates and the chunks past this step's idle
+    if (sd_stopped(sd)) return;
+    M = cstart[nc];
+    step_lo = sd->lo;
+    step_hi = sd->hi;
+    prio_key = sd->pk;
+    nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
+    if (blockIdx.x >= nchunks) return;
+  }
+  const uint32_t bx = xcd_block(blockIdx.x, nchunks);
   const uint32_t lo = bx * GEN_CHUNK;
   // the chunk's cells c0..c1 (their candidate starts in LDS when they fit)
   // and the cells its partners can lie in, [wlo, wlo + wn) (their counts
@@ -1416,9 +1459,12 @@ __global__ __launch_bounds__(256) void k_check(int r, int nbands, uint32_t seg,
                                                uint32_t *__restrict__ tab,
                                                uint8_t *__restrict__ cflag, uint32_t tag8,
                                                const uint32_t *__restrict__ counters,
-                                               uint32_t *__restrict__ host_left) {
+                                               uint32_t *__restrict__ host_left,
+                                               const StepDev *__restrict__ sd) {
+  if (sd_stopped(sd)) return;
+  if (sd) tag8 = sd->mark + (uint32_t)r;
   // publish the losers left after round r - 1 (final now) to the host
-  if (blockIdx.x == 0 && threadIdx.x == 0 && r > 0) {
+  if (host_left && blockIdx.x == 0 && threadIdx.x == 0 && r > 0) {
     uint32_t left = 0;
     for (int q = 0; q < NREG; ++q) left += counters[CNT_LIST + r * NREG + q];
     __hip_atomic_store(&host_left[r], left, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
@@ -1484,8 +1530,15 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
                                                const double *__restrict__ omax,
                                                double *__restrict__ omega_new, KParams kp,
                                                uint32_t step_lo, uint32_t step_hi,
-                                               FPParams fpp, OpLog log) {
+                                               FPParams fpp, OpLog log,
+                                               const StepDev *__restrict__ sd) {
   __shared__ uint32_t s_lose, s_base, s_acc;
+  if (sd_stopped(sd)) return;
+  if (sd) {
+    tag8 = sd->mark + (uint32_t)r;
+    step_lo = sd->lo;
+    step_hi = sd->hi;
+  }
   const uint32_t x = blockIdx.x % NREG;
   const uint32_t lb = blockIdx.x / NREG, nbx = gridDim.x / NREG;
   // band r and band r+1 of region x in the band-major lists
@@ -1605,11 +1658,172 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
     atomicAdd((unsigned long long *)&counters[CNT_ACC], (unsigned long long)s_acc);
 }
 
+// The rounds of a captured step after its grid rounds, in one workgroup
+// (by then few operations are left, and a fixed count of grid rounds per
+// step leaves no host in the loop).  Round r reads the losers of round r - 1
+// (region x at in + x seg, their count in the counters); each round is the
+// same two phases as k_check and k_round, separated by workgroup barriers,
+// so it commits exactly the operations a grid round would.  Every round
+// commits at least the operation of smallest priority, so the loop ends; it
+// stops the run (STOP_ROUNDS) at the round limit.
+template <typename R, int MODEL>
+__global__ __launch_bounds__(1024) void k_round_tail(int r0, int nbands, uint32_t seg,
+                                                     R *__restrict__ P,
+                                                     uint4 *__restrict__ opA,
+                                                     uint4 *__restrict__ opB,
+                                                     uint32_t *__restrict__ tab0,
+                                                     uint32_t *__restrict__ tab1,
+                                                     uint8_t *__restrict__ cflag,
+                                                     uint32_t *__restrict__ counters,
+                                                     const double *__restrict__ nden,
+                                                     const double *__restrict__ yc,
+                                                     const double *__restrict__ omax,
+                                                     double *__restrict__ omega_new, KParams kp,
+                                                     FPParams fpp, StepDev *__restrict__ sd) {
+  __shared__ uint32_t s_cnt[NREG], s_out[NREG];
+  __shared__ uint32_t s_acc;
+  if (sd_stopped(sd)) return;
+  const uint32_t step_lo = sd->lo, step_hi = sd->hi;
+  const uint32_t tid = threadIdx.x;
+  uint32_t acc_local = 0, visits = 0;
+  int r = r0;
+  for (;;) {
+    if (tid < NREG) {
+      s_cnt[tid] = counters[CNT_LIST + r * NREG + (int)tid];
+      s_out[tid] = 0u;
+    }
+    __syncthreads();
+    uint32_t total = 0;
+    for (int x = 0; x < NREG; ++x) total += s_cnt[x];
+    if (total == 0u) break;
+    if (r >= MAX_ROUNDS) {
+      if (tid == 0) atomicOr(&sd->stop, STOP_ROUNDS);
+      return;
+    }
+    visits += total;
+    const uint4 *in = (r & 1) ? opB : opA;
+    uint4 *out = (r & 1) ? opA : opB;
+    uint32_t *tab = (r & 1) ? tab1 : tab0;
+    uint32_t *tabn = (r & 1) ? tab0 : tab1;
+    const uint8_t mark = (uint8_t)(sd->mark + (uint32_t)r);
+    // phase 1 (k_check): the smallest priority at every shared particle
+    for (int x = 0; x < NREG; ++x) {
+      for (uint32_t idx = tid; idx < s_cnt[x]; idx += blockDim.x) {
+        const uint4 rec = in[(size_t)x * seg + idx];
+        if (rec.x == OP_NONE) continue;
+        if (tab[rec.x] != rec.z) {
+          atomicMin(&tab[rec.x], rec.z);
+          cflag[rec.x] = mark;
+        }
+        if (tab[rec.y] != rec.z) {
+          atomicMin(&tab[rec.y], rec.z);
+          cflag[rec.y] = mark;
+        }
+      }
+    }
+    __syncthreads();
+    // phase 2 (k_round): winners commit, losers enter round r + 1
+    for (int x = 0; x < NREG; ++x) {
+      for (uint32_t idx = tid; idx < s_cnt[x]; idx += blockDim.x) {
+        const uint4 rec = in[(size_t)x * seg + idx];
+        if (rec.x == OP_NONE) continue;
+        const bool oki = cflag[rec.x] != mark || tab[rec.x] == rec.z;
+        const bool okj = cflag[rec.y] != mark || tab[rec.y] == rec.z;
+        if (oki && okj) {
+          const R a = P[rec.x], bb = P[rec.y];
+          const bool acc =
+              MODEL == COLL_DSMC
+                  ? commit_dsmc<R>(rec.x, rec.y, rec.z, rec.w, a, bb, P, omax, omega_new, kp,
+                                   fpp, step_lo, step_hi)
+                  : commit_op<R>(rec.x, rec.y, rec.w, a, bb, P, nden, yc, omax, omega_new, kp,
+                                 step_lo, step_hi, MODEL == COLL_ESMC4);
+          acc_local += acc ? 1u : 0u;
+        } else {
+          tabn[rec.x] = rec.z;
+          tabn[rec.y] = rec.z;
+          out[(size_t)x * seg + atomicAdd(&s_out[x], 1u)] = rec;
+        }
+      }
+    }
+    __syncthreads();
+    if (tid < NREG) counters[CNT_LIST + (r + 1) * NREG + (int)tid] = s_out[tid];
+    ++r;
+  }
+  for (int off = 32; off > 0; off >>= 1) acc_local += __shfl_down(acc_local, off);
+  if (tid == 0) s_acc = 0;
+  __syncthreads();
+  if (lane_id() == 0 && acc_local) atomicAdd(&s_acc, acc_local);
+  __syncthreads();
+  if (tid == 0) {
+    if (s_acc) atomicAdd((unsigned long long *)&counters[CNT_ACC], (unsigned long long)s_acc);
+    // rounds of the step: the grid rounds up to the first one that found
+    // nothing left after every band was admitted, or all of them and the tail's
+    int used = r;
+    if (r == r0) {
+      for (int q = nbands; q < r0; ++q) {
+        uint32_t left = 0;
+        for (int x = 0; x < NREG; ++x) left += counters[CNT_LIST + q * NREG + x];
+        if (left == 0u) {
+          used = q;
+          break;
+        }
+      }
+    }
+    sd->rounds = (uint32_t)used;
+    sd->tail_ops += visits;
+  }
+}
+
+// After the candidate scan of a captured step: stop before anything is
+// drawn when the candidates exceed the operation lists; the step's priority
+// key (the same Philox block the host derives for an eager step).
+__global__ void k_step_mid(StepDev *__restrict__ sd, const uint32_t *__restrict__ cstart, int nc,
+                           uint32_t cap, uint32_t key0, uint32_t key1) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || sd->stop) return;
+  if (cstart[nc] > cap) {
+    sd->stop |= STOP_CAPACITY;
+    return;
+  }
+  sd->pk = mcpm_philox(0u, 0u, sd->lo, mcpm_ctr3(MCPM_TAG_PRIO, sd->hi, 0u), key0, key1).v[0];
+}
+
+// End of a captured step: the step's counts into the totals; a windowed sort
+// that a particle left stops the run with the step's collisions and field
+// done and its push and sort to be redone by the host; otherwise the next
+// step.
+__global__ void k_step_end(StepDev *__restrict__ sd, const uint32_t *__restrict__ counters,
+                           const uint32_t *__restrict__ cstart, int nc,
+                           const uint32_t *__restrict__ wflag) {
+  if (threadIdx.x != 0 || blockIdx.x != 0 || sd->stop) return;
+  const unsigned long long acc = *(const unsigned long long *)&counters[CNT_ACC];
+  sd->cand += cstart[nc];
+  sd->valid += counters[CNT_OPS];
+  sd->acc += acc;
+  sd->rsum += sd->rounds;
+  sd->mark += sd->rounds + 1u;
+  if (wflag[0] != 0u) {
+    sd->stop |= STOP_WINDOW;
+    return;
+  }
+  const uint64_t st = (((uint64_t)sd->hi << 32) | sd->lo) + 1u;
+  sd->lo = (uint32_t)st;
+  sd->hi = (uint32_t)(st >> 32);
+  sd->steps += 1u;
+}
+
+template <typename R>
+__global__ void k_ys_from_records(const R *__restrict__ P, int64_t n,
+                                  typename R::real_type *__restrict__ ys) {
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n;
+       k += (int64_t)gridDim.x * blockDim.x)
+    ys[k] = P[k].y;
+}
+
 __global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ omega_new,
                                const uint32_t *__restrict__ cnt, int nc, int model,
-                               int variant) {
+                               int variant, const StepDev *__restrict__ sd) {
   int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nc) return;
+  if (c >= nc || sd_stopped(sd)) return;
   double o = omega_new[c];
   if (model == COLL_DSMC) {
     // crm <- max(1.1 crm_seen, 0.9 crm) (collisions.cpp:2136), in cells with
@@ -1688,9 +1902,13 @@ __global__ void k_pressure(const double *__restrict__ mom, const double *__restr
 
 __global__ void k_rep_step(KParams kp, const uint32_t *__restrict__ cstart,
                            uint32_t *__restrict__ rprio, uint32_t *__restrict__ rcand_off,
-                           uint32_t step_lo, uint32_t step_hi) {
+                           uint32_t step_lo, uint32_t step_hi, const StepDev *__restrict__ sd) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r > kp.nrep) return;
+  if (r > kp.nrep || sd_stopped(sd)) return;
+  if (sd) {
+    step_lo = sd->lo;
+    step_hi = sd->hi;
+  }
   rcand_off[r] = cstart[kp.rcell_off[r]];
   if (r < kp.nrep) {
     const KParams &kq = kp.rkp[r];
@@ -1771,10 +1989,12 @@ __global__ __launch_bounds__(1024) void k_field(const double *__restrict__ nden,
                                                 FieldArrays fa0, double *__restrict__ F2,
                                                 KParams kp0, double ast, double bst,
                                                 double eqdist,
-                                                const FieldArrays *__restrict__ rfa) {
+                                                const FieldArrays *__restrict__ rfa,
+                                                const StepDev *__restrict__ sd) {
   __shared__ double sA[1024];
   __shared__ double sB[1024];
   __shared__ double red[2][1024];
+  if (sd_stopped(sd)) return;
   // batched replicas: workgroup b solves replica b's slab
   const bool batch = kp0.nrep > 0;
   const KParams &kp = batch ? kp0.rkp[blockIdx.x] : kp0;
@@ -2057,24 +2277,24 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                        const uint32_t *, int, uint32_t, KParams, uint32_t, \
                                        uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
                                        const uint32_t *, int32_t,                          \
-                                       uint4 *, uint32_t *, uint32_t *);                   \
+                                       uint4 *, uint32_t *, uint32_t *, const StepDev *);  \
   template __global__ void k_round<R, M>(int, int, uint32_t, R *, const uint4 *,           \
                                          const uint4 *, uint4 *, const uint32_t *,         \
                                          uint32_t *, const uint8_t *, uint32_t,            \
                                          uint32_t *,                                       \
                                          const double *, const double *, const double *,   \
                                          double *, KParams, uint32_t, uint32_t, FPParams,  \
-                                         OpLog);
+                                         OpLog, const StepDev *);
 
 template __global__ void k_check<COLL_ESMC>(int, int, uint32_t, const uint4 *, const uint4 *,
                                              uint32_t *, uint8_t *, uint32_t, const uint32_t *,
-                                             uint32_t *);
+                                             uint32_t *, const StepDev *);
 template __global__ void k_check<COLL_DSMC>(int, int, uint32_t, const uint4 *, const uint4 *,
                                              uint32_t *, uint8_t *, uint32_t, const uint32_t *,
-                                             uint32_t *);
+                                             uint32_t *, const StepDev *);
 template __global__ void k_check<COLL_ESMC4>(int, int, uint32_t, const uint4 *, const uint4 *,
                                               uint32_t *, uint8_t *, uint32_t, const uint32_t *,
-                                              uint32_t *);
+                                              uint32_t *, const StepDev *);
 
 #define MCPM_INST_W(R)                                                                     \
   template __global__ void k_wpush<R, false>(const R *, R *, KParams, Kick, const double *, \
@@ -2083,14 +2303,22 @@ template __global__ void k_check<COLL_ESMC4>(int, int, uint32_t, const uint4 *, 
                                              int32_t *, uint32_t *, const uint32_t *,       \
                                              uint32_t *, uint32_t *, const uint32_t *,      \
                                              uint32_t *, int, R::real_type *,               \
-                                             const int32_t *, int32_t *);                   \
+                                             const int32_t *, int32_t *, const StepDev *);  \
   template __global__ void k_wpush<R, true>(const R *, R *, KParams, Kick, const double *,  \
                                             int, uint32_t, uint32_t, const uint32_t *,      \
                                             const int32_t *, const uint32_t *, int32_t *,   \
                                             int32_t *, uint32_t *, const uint32_t *,        \
                                             uint32_t *, uint32_t *, const uint32_t *,       \
                                             uint32_t *, int, R::real_type *,                \
-                                            const int32_t *, int32_t *);
+                                            const int32_t *, int32_t *, const StepDev *);   \
+  template __global__ void k_round_tail<R, COLL_ESMC>(int, int, uint32_t, R *, uint4 *,     \
+                                                      uint4 *,                              \
+                                                      uint32_t *, uint32_t *, uint8_t *,    \
+                                                      uint32_t *, const double *,           \
+                                                      const double *, const double *,       \
+                                                      double *, KParams, FPParams,          \
+                                                      StepDev *);                           \
+  template __global__ void k_ys_from_records<R>(const R *, int64_t, R::real_type *);
 
 #define MCPM_INST(R)                                                                       \
   template __global__ void k_push<R, false>(const R *, R *, int64_t, int32_t, KParams,     \

@@ -280,6 +280,9 @@ def load_library() -> ctypes.CDLL:
         "mcpm_kick": (i32, [vp]),
         "mcpm_stream_bc": (i32, [vp, i64, i32]),
         "mcpm_step": (i32, [vp, i64, ctypes.POINTER(CStats)]),
+        "mcpm_graph_capture": (i32, [vp, i64, i32]),
+        "mcpm_graph_steps": (i32, [vp, i64, i64, ctypes.POINTER(CStats)]),
+        "mcpm_graph_release": (i32, [vp]),
         "mcpm_init_omega_max": (i32, [vp, ctypes.c_double, i32]),
         "mcpm_set_omega_max": (i32, [vp, dpp]),
         "mcpm_get_cells": (i32, [vp, ctypes.POINTER(i32), ctypes.POINTER(i32), dpp, dpp, dpp]),
@@ -579,6 +582,22 @@ class Simulation:
         s = CStats()
         self._check(self._lib.mcpm_step(self._h, step, ctypes.byref(s)), "mcpm_step")
         return s.as_dict()
+
+    def graph_capture(self, op_capacity: int = 0, rounds: int = 0):
+        """Record one step as HIP graphs (mcpm_graph_capture)."""
+        self._check(self._lib.mcpm_graph_capture(self._h, op_capacity, rounds),
+                    "mcpm_graph_capture")
+
+    def graph_steps(self, step0: int, nsteps: int) -> dict:
+        """Steps step0 .. step0 + nsteps - 1 by graph replays; sums of the
+        per-step statistics (mcpm_graph_steps)."""
+        s = CStats()
+        self._check(self._lib.mcpm_graph_steps(self._h, step0, nsteps, ctypes.byref(s)),
+                    "mcpm_graph_steps")
+        return s.as_dict()
+
+    def graph_release(self):
+        self._check(self._lib.mcpm_graph_release(self._h), "mcpm_graph_release")
 
     def sync(self):
         self._check(self._lib.mcpm_sync(self._h), "mcpm_sync")

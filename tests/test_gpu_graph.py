@@ -1,0 +1,118 @@
+"""Graph-replayed steps (mcpm_graph_capture / mcpm_graph_steps) against the
+same steps called one by one (mcpm_step): positions, velocities, cell counts,
+densities, omega_max and the field bit for bit, and the same collision
+counts.  Covered: the interface slab in FP32 and FP64 with eager steps
+between replays, batched replicas, a capacity too small for a step (the run
+recaptures and repeats the step), and particles that leave their sort
+windows (the host finishes that step's sort and the run goes on)."""
+import numpy as np
+import pytest
+
+import mcpm
+import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _pair(p, arrs, n_gas):
+    sims = []
+    for _ in range(2):
+        s = mcpm.Simulation(p)
+        s.upload(*arrs)
+        s.cell_update()
+        s.init_omega_max(n_gas)
+        sims.append(s)
+    return sims
+
+
+def _same_state(a, b):
+    for k, (x, y) in enumerate(zip(a.download(), b.download())):
+        np.testing.assert_array_equal(x, y, err_msg=f"particle array {k}")
+    ca, cb = a.cells(), b.cells()
+    for k in ("count", "start", "n", "omega_max", "F2"):
+        np.testing.assert_array_equal(ca[k], cb[k], err_msg=k)
+
+
+def _sum(stats, keys=("candidates", "valid", "accepted")):
+    return {k: sum(s[k] for s in stats) for k in keys}
+
+
+@pytest.mark.parametrize("precision", [mcpm.FP32, mcpm.FP64])
+def test_graph_steps_equal_eager_steps(precision):
+    p, arrs, meta = W.interface(N=300_000, n_cells=512, length=100.0, precision=precision,
+                                seed=71)
+    _say(f"graph interface N=300000 nc=512 prec={precision}")
+    e, g = _pair(p, arrs, meta["n_gas"])
+    eager = [e.step(s) for s in range(9)]
+    g.graph_capture()
+    parts = [g.graph_steps(0, 3)]
+    parts.append(g.step(3))          # an eager step between replays
+    parts.append(g.graph_steps(4, 2))
+    parts.append(g.graph_steps(6, 3))
+    assert _sum(parts) == _sum(eager)
+    _same_state(e, g)
+
+
+def test_graph_steps_of_batched_replicas():
+    common, reps, arrs, metas = W.replicas(n_rep=6, N_lo=20_000, N_hi=60_000, cells_lo=128,
+                                           cells_hi=300)
+    counts = [a[0].size for a in arrs]
+    _say(f"graph replicas R=6 N={sum(counts)}")
+    sims = []
+    for _ in range(2):
+        b = mcpm.Simulation.batch(common, reps)
+        b.upload_batch(counts, *[np.concatenate([a[q] for a in arrs]) for q in range(4)])
+        b.cell_update()
+        b.init_omega_max_batch([m["n_gas"] for m in metas])
+        sims.append(b)
+    e, g = sims
+    eager = [e.step(s) for s in range(5)]
+    g.graph_capture()
+    got = g.graph_steps(0, 5)
+    assert _sum([got]) == _sum(eager)
+    _same_state(e, g)
+
+
+def test_graph_capacity_too_small_recaptures():
+    p, arrs, meta = W.interface(N=100_000, n_cells=256, length=100.0, seed=72)
+    _say("graph capacity N=100000 nc=256")
+    e, g = _pair(p, arrs, meta["n_gas"])
+    eager = [e.step(s) for s in range(4)]
+    g.graph_capture(op_capacity=4096)
+    got = g.graph_steps(0, 4)
+    assert _sum([got]) == _sum(eager)
+    _same_state(e, g)
+
+
+def test_graph_particles_leaving_their_windows():
+    p, arrs, meta = W.interface(N=100_000, n_cells=256, length=100.0, seed=73)
+    y, u1, u2, u3 = (a.copy() for a in arrs)
+    # a few particles cross many cells per step (20 cells at dt = 0.02)
+    u2[::5000] = 400.0
+    _say("graph window overflow N=100000 nc=256")
+    e, g = _pair(p, (y, u1, u2, u3), meta["n_gas"])
+    eager = [e.step(s) for s in range(6)]
+    g.graph_capture()
+    got = g.graph_steps(0, 6)
+    assert _sum([got]) == _sum(eager)
+    _same_state(e, g)
+
+
+def test_graph_capture_rejects_what_it_does_not_run():
+    p, arrs, meta = W.heat_conduction(n_cells=50, ppc=40, seed=74)
+    p.model = mcpm.MODEL_DSMC_VHS
+    s = mcpm.Simulation(p)
+    s.upload(*arrs)
+    s.cell_update()
+    with pytest.raises(mcpm.McpmError):
+        s.graph_capture()
+    q, a2, m2 = W.interface(N=20_000, n_cells=64, length=20.0, seed=75)
+    t = mcpm.Simulation(q)
+    t.upload(*a2)
+    t.cell_update()
+    with pytest.raises(mcpm.McpmError):
+        t.graph_steps(0, 1)  # nothing captured
