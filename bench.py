@@ -417,6 +417,9 @@ def surrogate_phase(args):
 
     sim.poisson_solver()
     nd, pot = sim.nodes()
+    # the best any m-mode spectral operator can do on this density (exact on
+    # its modes, the best pointwise gain above them), before the run heats up
+    floor_start = mcpm.spectral_force_floor(nd, p.dx, p.bst, p.sigma, m)
     net = mcpm.FieldSurrogate(n0, n1, m, mcpm.FieldSurrogate.init_params(m, seed=0))
     net.set_norm(nd.mean(), 1.0 / nd.std(), pot.mean(), 1.0 / pot.std())
     S = args.sur_snapshots
@@ -429,27 +432,36 @@ def surrogate_phase(args):
     t_snap = time.perf_counter() - t0
 
     def held_out(k):
+        # force error of the network's field against the exact solve of the
+        # same density: of the whole field (the exact core term included on
+        # both sides), and of the part the network stands in for (the FEM
+        # solution, core term left out); the m-mode floor at the same state
         nonlocal step
-        errs, res = [], []
+        errs, errs_fem, res, floors = [], [], [], []
         for _ in range(k):
             sim.step(step)
             step += 1
             st = net.field(sim, accept_residual=-1.0)
-            _, pn = sim.nodes()
+            nd_k, pn = sim.nodes()
             sim.poisson_solver()
             _, pe = sim.nodes()
             fe, fn = mcpm.plane_force(pe, p.dx), mcpm.plane_force(pn, p.dx)
+            fc = mcpm.plane_force(mcpm._periodic_nodes(
+                mcpm.plane_core(nd_k[:-1, :-1], p.dx, p.bst, p.sigma)), p.dx)
             errs.append(float(np.linalg.norm(fn - fe) / np.linalg.norm(fe)))
+            errs_fem.append(float(np.linalg.norm(fn - fe) / np.linalg.norm(fe + fc)))
             res.append(st["residual"])
-        return float(np.mean(errs)), float(np.mean(res))
+            floors.append(mcpm.spectral_force_floor(nd_k, p.dx, p.bst, p.sigma, m)["floor"])
+        return (float(np.mean(errs)), float(np.mean(res)), float(np.mean(errs_fem)),
+                float(np.mean(floors)))
 
-    err0, res0 = held_out(2)
+    err0, res0, errf0, _ = held_out(2)
     T = args.sur_iters
     t0 = time.perf_counter()
     l1 = net.train(T // 2, 1e-3, seed=1)
     l2 = net.train(T - T // 2, 3e-4, seed=2)
     t_train = time.perf_counter() - t0
-    err1, res1 = held_out(4)
+    err1, res1, errf1, floor1 = held_out(4)
     log(f"surrogate: loss {l1[:10].mean():.4f} -> {l2[-20:].mean():.5f}; force error "
         f"{err0:.4f} -> {err1:.5f}; residual {res0:.3f} -> {res1:.4f}")
 
@@ -504,7 +516,10 @@ def surrogate_phase(args):
         "training": {"snapshots": S, "snapshot_s": round(t_snap, 2), "iters": T,
                      "ms_per_iter": round(1000.0 * t_train / max(T, 1), 3),
                      "loss_first": float(l1[:10].mean()), "loss_last": float(l2[-20:].mean())},
-        "force_rel_err": {"untrained": err0, "trained": err1, "held_out_steps": 4},
+        "force_rel_err": {"untrained": err0, "trained": err1, "held_out_steps": 4,
+                          "fem_part_untrained": errf0, "fem_part_trained": errf1,
+                          "floor_of_any_16_mode_operator": {"first_steps": floor_start["floor"],
+                                                            "held_out_steps": floor1}},
         "fem_residual": {"untrained": res0, "trained": res1},
         "field_ms": {"exact": f_ex, "network": f_net, "exact_after_network": f_warm,
                      "speedup": f_ex / f_net,

@@ -759,6 +759,69 @@ def plane_force(pot, h):
     return np.stack([gx, gy])
 
 
+def _periodic_nodes(a):
+    """(N1, N0) periodic node values -> (N1 + 1, N0 + 1) with the seam rows."""
+    out = np.empty((a.shape[0] + 1, a.shape[1] + 1))
+    out[:-1, :-1] = a
+    out[-1, :-1] = a[0]
+    out[:-1, -1] = a[:, 0]
+    out[-1, -1] = a[0, 0]
+    return out
+
+
+def plane_core(n, h, lam, sigma):
+    """The core subtraction of compute_SP_2D on a periodic plane
+    (FEM.cpp:497-554): core0 n + core2 (sum of the 4 neighbours - 4 n), the
+    term the field loses, for (N1, N0) node densities."""
+    ls = lam * sigma
+    core0 = (1.0 - (ls + 1.0) * math.exp(-ls)) / lam ** 2
+    core2 = ((-ls ** 3 / 6 - ls ** 2 / 2 - ls + math.exp(ls) - 1.0) * math.exp(-ls) / lam ** 4) / h ** 2
+    nb = np.roll(n, 1, 0) + np.roll(n, -1, 0) + np.roll(n, 1, 1) + np.roll(n, -1, 1)
+    return core0 * n + core2 * (nb - 4.0 * n)
+
+
+def spectral_force_floor(n_node, h, lam, sigma, modes):
+    """The smallest force error a spectral operator with `modes` retained
+    modes per axis can reach on this density: exact on the retained modes
+    (|kx|, |ky| < modes), the best pointwise gain on the others, the exact
+    core term on both.  The field is the periodic Q1 solution of
+    (K + lambda^2 M) phi = M n (FEM.cpp:73-110 element matrices, square cells
+    of side h) minus the core term; the force the gradient of its bilinear
+    interpolant at the cell centres (plane_force).  n_node: the
+    (N1 + 1, N0 + 1) node densities of a periodic plane as Simulation.nodes
+    returns them (the last row and column repeat the first).  Returns the
+    relative force error of that best operator, of the retained modes alone,
+    and the best gain."""
+    n = np.asarray(n_node, dtype=np.float64)
+    if n.ndim != 2 or min(n.shape) < 3:
+        raise ValueError("node densities must be an (N1 + 1, N0 + 1) array")
+    n = n[:-1, :-1]
+    n1, n0 = n.shape
+    cx = np.cos(2 * np.pi * np.fft.fftfreq(n0))[None, :]
+    cy = np.cos(2 * np.pi * np.fft.fftfreq(n1))[:, None]
+    K = 8.0 / 3.0 - (2.0 / 3.0) * (cx + cy) - (4.0 / 3.0) * cx * cy
+    M = h * h * (2 + cx) * (2 + cy) / 9.0
+    T = M / (K + lam * lam * M)
+    nh = np.fft.fft2(n)
+    core = plane_core(n, h, lam, sigma)
+
+    def force(phi):
+        return plane_force(_periodic_nodes(phi), h)
+
+    exact = force(np.real(np.fft.ifft2(T * nh)) - core)
+    den = np.linalg.norm(exact)
+    kx = np.abs(np.fft.fftfreq(n0) * n0)[None, :]
+    ky = np.abs(np.fft.fftfreq(n1) * n1)[:, None]
+    keep = (kx < modes) & (ky < modes)
+    low = force(np.real(np.fft.ifft2(np.where(keep, T * nh, 0.0))) - core)
+    high = force(np.real(np.fft.ifft2(np.where(keep, 0.0, nh))))
+    rest = exact - low
+    hh = float(np.sum(high * high))
+    gain = float(np.sum(high * rest) / hh) if hh > 0 else 0.0
+    return {"floor": float(np.linalg.norm(rest - gain * high) / den),
+            "modes_only": float(np.linalg.norm(rest) / den), "gain": gain}
+
+
 def _fp(a):
     assert a.dtype == np.float32 and a.flags.c_contiguous
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
