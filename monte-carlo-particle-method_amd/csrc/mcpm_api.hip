@@ -2585,6 +2585,8 @@ int mcpm_set_omega_max(mcpm_ctx *c, const double *om) {
   if (!c || !om) return MCPM_ERR_ARG;
   HIPCHK(hipStreamSynchronize(c->s));
   HIPCHK(hipMemcpy(c->omax, om, c->nc * sizeof(double), hipMemcpyHostToDevice));
+  // the running maximum of the next collision call starts afresh
+  HIPCHK(hipMemset(c->omega_new, 0, c->nc * sizeof(double)));
   return MCPM_OK;
 }
 

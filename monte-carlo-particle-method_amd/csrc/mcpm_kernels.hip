@@ -1337,9 +1337,12 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R 
   double g2 = (double)a.u3 - (double)bb.u3;
   double gd = g0 * d0 + g1 * d1 + g2 * d2;
   double pv = kp.coll_pref * gd * Ymid * nden[J] * kp.dt;
-  atomic_max_pos_double(&omega_new[c], pv);
+  // the step's running maximum starts at omega_max (k_omega_finish), so only
+  // a p above omega_max can raise it
+  const double om = omax[c];
+  if (pv > om) atomic_max_pos_double(&omega_new[c], pv);
   // new4 also requires an approaching pair, g.dir > 0 (collisions.cpp:819)
-  if (pv / omax[c] > mcpm_u01(w2.v[0]) && (!new4 || gd > 0.0)) {
+  if (pv / om > mcpm_u01(w2.v[0]) && (!new4 || gd > 0.0)) {
     a.u1 = (T)((double)a.u1 - gd * d0);
     a.u2 = (T)((double)a.u2 - gd * d1);
     a.u3 = (T)((double)a.u3 - gd * d2);
@@ -1833,10 +1836,13 @@ __global__ void k_omega_finish(double *__restrict__ omax, double *__restrict__ o
       double a = 1.1 * o, b = 0.9 * omax[c];
       omax[c] = a > b ? a : b;
     }
-  } else if (o > omax[c]) {
-    omax[c] = o;  // collisions.cpp:714-716
+    omega_new[c] = 0.0;
+  } else {
+    // collisions.cpp:714-716; the next step's maximum starts from the new
+    // omega_max (the same max, and commits below it skip the atomic)
+    if (o > omax[c]) omax[c] = o;
+    omega_new[c] = omax[c];
   }
-  omega_new[c] = 0.0;
 }
 
 // initialization (Main_Functions.cpp:4222-4815) for particle i: Philox block
