@@ -163,8 +163,9 @@ def main():
     # ESMC interface and replica steps run as HIP-graph replays (no host wait
     # inside a step, bit for bit the same steps as one mcpm_step call each);
     # the first warmup step is a plain call (it sizes the capture)
-    use_graph = (not args.no_graph and args.workload in ("interface", "replicas")
-                 and not args.pressure)
+    use_graph = (not args.no_graph and not args.pressure
+                 and (args.workload in ("interface", "replicas")
+                      or (args.workload == "dense" and args.model == "esmc")))
     step = 0
     t0 = time.time()
     for w in range(args.warmup):
