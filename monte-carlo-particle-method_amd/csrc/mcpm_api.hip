@@ -633,14 +633,24 @@ int wsort_launch(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const do
   // the most chunks there can be; waves past the count stop at once
   const unsigned gw = (unsigned)((c->wcap + 3) / 4);
   using real = typename R::real_type;
-  hipLaunchKernelGGL((k_wpush<R, false>), dim3(gw), dim3(256), 0, c->s, in, nullptr, c->kp, kick,
+  // the lean push: every slab periodic, no position drift, no face sums or
+  // re-insertion counts to keep
+  bool lean = drift == nullptr && kick.pm == nullptr && kick.esc == nullptr;
+  if (c->kp.nrep) {
+    for (const KParams &q : c->rkp_host) lean = lean && q.boundary == MCPM_BC_PERIODIC;
+  } else {
+    lean = lean && c->kp.boundary == MCPM_BC_PERIODIC;
+  }
+  auto count_k = lean ? k_wpush<R, false, true> : k_wpush<R, false, false>;
+  auto scatter_k = lean ? k_wpush<R, true, true> : k_wpush<R, true, false>;
+  hipLaunchKernelGGL(count_k, dim3(gw), dim3(256), 0, c->s, in, nullptr, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
                      nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta, (real *)nullptr,
                      nullptr, nullptr, sd);
   hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 3) / 4)), dim3(256), 0, c->s, c->wlo,
                      c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc, sd);
   if (int st = excl_scan(c, c->cnt, c->start, c->nc, sd)) return st;
-  hipLaunchKernelGGL((k_wpush<R, true>), dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
+  hipLaunchKernelGGL(scatter_k, dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
                      c->start, c->err, c->wflag, iid, oid, c->wdelta, ys_out<real>(c),
                      c->track_w ? c->pw[c->cur] : nullptr,

@@ -176,6 +176,30 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
   return r;
 }
 
+// The same push on a periodic slab with no position drift and no face sums:
+// the kick, the free flight and the wrap of push_one's periodic branch,
+// operation for operation, without the wall, re-injection and drift paths
+// (whose registers would otherwise bound the sort kernels' occupancy).
+template <typename real>
+__device__ __forceinline__ Rec<real> push_periodic(Rec<real> r, const KParams &kp,
+                                                   const Kick &kick, int mode) {
+  double y = (double)r.y;
+  if (mode & PUSH_KICK) {
+    const double f2 = kick.F2[cell_of(y, kp.dy, kp.n_cells)];
+    r.u2 = (real)((double)r.u2 + f2 * kp.dt);
+  }
+  if (!(mode & PUSH_MOVE)) return r;
+  const double v = (double)r.u2;
+  y = y + v * kp.dt;
+  if (y < 0.0) y += kp.length;
+  else if (y >= kp.length) y -= kp.length;
+  real yr = (real)y;
+  if (!(yr < (real)kp.length)) yr = (real)0;
+  if (!(yr >= (real)0)) yr = (real)0;
+  r.y = yr;
+  return r;
+}
+
 // Periodic wrap of one coordinate into [0, L) (stored type rounding included).
 template <typename real>
 __device__ __forceinline__ real wrap_store(double v, double L) {
@@ -470,7 +494,7 @@ __device__ __forceinline__ int32_t window_offset(int32_t key, int32_t lo, int32_
   return (d >= 0 && d < len) ? d : -1;
 }
 
-template <typename R, bool SCATTER>
+template <typename R, bool SCATTER, bool LEAN>
 __global__ __launch_bounds__(256) void k_wpush(
     const R *__restrict__ in, R *__restrict__ out, KParams kp, Kick kick,
     const double *__restrict__ drift, int mode, uint32_t step_lo, uint32_t step_hi,
@@ -561,8 +585,12 @@ __global__ __launch_bounds__(256) void k_wpush(
       if (valid) {
         if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
         if ((mode & PUSH_CHECK) && !in_domain(r, kq)) atomicOr(err, ERRBIT_DOMAIN);
-        r = push_one(r, (uint64_t)((int64_t)p - slot0), kq, kk, drift, mode, step_lo, step_hi,
-                     err);
+        if constexpr (LEAN) {
+          r = push_periodic(r, kq, kk, mode);
+        } else {
+          r = push_one(r, (uint64_t)((int64_t)p - slot0), kq, kk, drift, mode, step_lo, step_hi,
+                       err);
+        }
         d = window_offset(cell_key(r, kq), lo, len, nc, periodic);
         if (d < 0) valid = false;
       }
@@ -2302,21 +2330,19 @@ template __global__ void k_check<COLL_ESMC4>(int, int, uint32_t, const uint4 *, 
                                               uint32_t *, uint8_t *, uint32_t, const uint32_t *,
                                               uint32_t *, const StepDev *);
 
-#define MCPM_INST_W(R)                                                                     \
-  template __global__ void k_wpush<R, false>(const R *, R *, KParams, Kick, const double *, \
-                                             int, uint32_t, uint32_t, const uint32_t *,     \
-                                             const int32_t *, const uint32_t *, int32_t *,  \
-                                             int32_t *, uint32_t *, const uint32_t *,       \
-                                             uint32_t *, uint32_t *, const uint32_t *,      \
-                                             uint32_t *, int, R::real_type *,               \
-                                             const int32_t *, int32_t *, const StepDev *);  \
-  template __global__ void k_wpush<R, true>(const R *, R *, KParams, Kick, const double *,  \
+#define MCPM_INST_WP(R, S, L)                                                              \
+  template __global__ void k_wpush<R, S, L>(const R *, R *, KParams, Kick, const double *,  \
                                             int, uint32_t, uint32_t, const uint32_t *,      \
                                             const int32_t *, const uint32_t *, int32_t *,   \
                                             int32_t *, uint32_t *, const uint32_t *,        \
                                             uint32_t *, uint32_t *, const uint32_t *,       \
                                             uint32_t *, int, R::real_type *,                \
-                                            const int32_t *, int32_t *, const StepDev *);   \
+                                            const int32_t *, int32_t *, const StepDev *);
+#define MCPM_INST_W(R)                                                                     \
+  MCPM_INST_WP(R, false, false)                                                            \
+  MCPM_INST_WP(R, true, false)                                                             \
+  MCPM_INST_WP(R, false, true)                                                             \
+  MCPM_INST_WP(R, true, true)                                                              \
   template __global__ void k_round_tail<R, COLL_ESMC>(int, int, uint32_t, R *, uint4 *,     \
                                                       uint4 *,                              \
                                                       uint32_t *, uint32_t *, uint8_t *,    \
