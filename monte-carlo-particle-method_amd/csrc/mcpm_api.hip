@@ -901,7 +901,10 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
         hipLaunchKernelGGL((k_check<MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
                            (const uint4 *)c->opBand, tabs[r & 1], c->cflag, mark,
                            (const uint32_t *)c->counters, c->dleft, (const StepDev *)nullptr);
-        hipLaunchKernelGGL((k_round<R, MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
+        // no replicas and no log: the kernel without their code (fewer registers)
+        auto round_k = (c->kp.nrep == 0 && !lg.p) ? k_round<R, MODEL, true>
+                                                  : k_round<R, MODEL, false>;
+        hipLaunchKernelGGL(round_k, dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
                            P, in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
                            (const uint32_t *)tabs[r & 1], tabs[(r + 1) & 1],
                            (const uint8_t *)c->cflag, mark, c->counters, c->nden,
@@ -1231,7 +1234,8 @@ int graph_step_launch(mcpm_ctx *c) {
     hipLaunchKernelGGL((k_check<COLL_ESMC>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
                        (const uint4 *)c->opBand, tabs[r & 1], c->cflag, 0u,
                        (const uint32_t *)c->counters, (uint32_t *)nullptr, csd);
-    hipLaunchKernelGGL((k_round<R, COLL_ESMC>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, P,
+    auto round_k = c->kp.nrep == 0 ? k_round<R, COLL_ESMC, true> : k_round<R, COLL_ESMC, false>;
+    hipLaunchKernelGGL(round_k, dim3(grid), dim3(256), 0, c->s, r, nbands, seg, P,
                        in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
                        (const uint32_t *)tabs[r & 1], tabs[(r + 1) & 1],
                        (const uint8_t *)c->cflag, 0u, c->counters, c->nden, c->yc, c->omax,

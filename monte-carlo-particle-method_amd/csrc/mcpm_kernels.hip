@@ -1338,7 +1338,9 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
 // p = 4 pi sigma^2 (g.dir) Y n_J dt, omega_I = max(omega_I, p),
 // accept iff p / omega_max_I > U, hard-sphere exchange along dir.
 // The cell I of the candidate is the cell of particle i (i was drawn in it).
-template <typename R>
+// REP = false: the context holds no batched replicas (kp.nrep == 0), which
+// keeps the replica lookup out of the code.
+template <typename R, bool REP = true>
 __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R a, R bb, R *P,
                                           const double *nden, const double *yc,
                                           const double *omax, double *omega_new,
@@ -1346,9 +1348,10 @@ __device__ __forceinline__ bool commit_op(uint32_t i, uint32_t j, uint32_t t, R 
                                           uint32_t step_hi, bool new4 = false) {
   using T = typename R::real_type;
   // a batched replica: its geometry and key, cells counted from its first
-  const int32_t rq = kp.nrep ? rep_of_particle(kp, (int64_t)i) : 0;
-  const KParams &kq = kp.nrep ? kp.rkp[rq] : kp;
-  const int32_t off = kp.nrep ? kp.rcell_off[rq] : 0;
+  const bool rep = REP && kp.nrep;
+  const int32_t rq = rep ? rep_of_particle(kp, (int64_t)i) : 0;
+  const KParams &kq = rep ? kp.rkp[rq] : kp;
+  const int32_t off = rep ? kp.rcell_off[rq] : 0;
   const uint32_t cl = (uint32_t)cell_key(a, kq);
   const uint32_t c = cl + (uint32_t)off;
   mcpm_u32x4 w = mcpm_philox(t, cl, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kq.key0,
@@ -1546,7 +1549,7 @@ __global__ __launch_bounds__(256) void k_check(int r, int nbands, uint32_t seg,
   }
 }
 
-template <typename R, int MODEL>
+template <typename R, int MODEL, bool PLAIN>
 __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
                                                R *__restrict__ P,
                                                const uint4 *__restrict__ in,
@@ -1640,10 +1643,10 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
         bool acc = MODEL == COLL_DSMC
                        ? commit_dsmc<R>(i, j, pr, rec[u].w, pa[u], pb[u], P, omax, omega_new,
                                         kp, fpp, step_lo, step_hi)
-                       : commit_op<R>(i, j, rec[u].w, pa[u], pb[u], P, nden, yc, omax,
+                       : commit_op<R, !PLAIN>(i, j, rec[u].w, pa[u], pb[u], P, nden, yc, omax,
                                       omega_new, kp, step_lo, step_hi, MODEL == COLL_ESMC4);
         acc_local += acc ? 1u : 0u;
-        if (log.p) {
+        if (!PLAIN && log.p) {
           uint32_t q = atomicAdd(&counters[CNT_LOG], 1u);
           log.p[q] = pr;
           log.i[q] = i;
@@ -2312,7 +2315,14 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                        uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
                                        const uint32_t *, int32_t,                          \
                                        uint4 *, uint32_t *, uint32_t *, const StepDev *);  \
-  template __global__ void k_round<R, M>(int, int, uint32_t, R *, const uint4 *,           \
+  template __global__ void k_round<R, M, false>(int, int, uint32_t, R *, const uint4 *,           \
+                                         const uint4 *, uint4 *, const uint32_t *,         \
+                                         uint32_t *, const uint8_t *, uint32_t,            \
+                                         uint32_t *,                                       \
+                                         const double *, const double *, const double *,   \
+                                         double *, KParams, uint32_t, uint32_t, FPParams,  \
+                                         OpLog, const StepDev *); \
+  template __global__ void k_round<R, M, true>(int, int, uint32_t, R *, const uint4 *,           \
                                          const uint4 *, uint4 *, const uint32_t *,         \
                                          uint32_t *, const uint8_t *, uint32_t,            \
                                          uint32_t *,                                       \
