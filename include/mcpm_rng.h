@@ -48,8 +48,11 @@ MCPM_HD mcpm_u32x4 mcpm_philox(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c
 #endif
   for (int r = 0; r < 10; ++r) {
     if (r > 0) { k0 += W0; k1 += W1; }
-    uint32_t lo0 = M0 * c0, hi0 = mcpm_mulhi32(M0, c0);
-    uint32_t lo1 = M1 * c2, hi1 = mcpm_mulhi32(M1, c2);
+    /* one 32 x 32 -> 64-bit product per multiplier (a single
+       multiply-add instruction on the GPU instead of a low and a high one) */
+    const uint64_t p0 = (uint64_t)M0 * (uint64_t)c0, p1 = (uint64_t)M1 * (uint64_t)c2;
+    uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     uint32_t n0 = hi1 ^ c1 ^ k0;
     uint32_t n2 = hi0 ^ c3 ^ k1;
     c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
