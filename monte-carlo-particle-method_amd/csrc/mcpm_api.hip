@@ -641,8 +641,8 @@ int wsort_launch(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const do
   } else {
     lean = lean && c->kp.boundary == MCPM_BC_PERIODIC;
   }
-  auto count_k = lean ? k_wpush<R, false, true> : k_wpush<R, false, false>;
-  auto scatter_k = lean ? k_wpush<R, true, true> : k_wpush<R, true, false>;
+  auto count_k = lean ? k_wpush<R, false, 1> : k_wpush<R, false, 0>;
+  auto scatter_k = lean ? k_wpush<R, true, 1> : k_wpush<R, true, 0>;
   hipLaunchKernelGGL(count_k, dim3(gw), dim3(256), 0, c->s, in, nullptr, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
                      nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta, (real *)nullptr,

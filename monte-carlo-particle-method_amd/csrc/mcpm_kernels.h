@@ -102,8 +102,9 @@ constexpr int WDELTA = 4;
 constexpr int WSPAN = 32;
 constexpr int WDELTA_MAX = (WMAX - WSPAN) / 2;
 static_assert(WSPAN + 2 * WDELTA_MAX <= WMAX, "window of a chunk");
-// LEAN: a periodic slab without position drift or face sums (push_periodic).
-template <typename R, bool SCATTER, bool LEAN>
+// PK: 0 the push of any slab, 1 of a periodic slab without position drift
+// and face sums.
+template <typename R, bool SCATTER, int PK>
 __global__ void k_wpush(const R *in, R *out, KParams kp, Kick kick, const double *drift, int mode,
                         uint32_t step_lo, uint32_t step_hi, const uint32_t *wbeg,
                         const int32_t *wrep, const uint32_t *nchp, int32_t *wlo, int32_t *wlen,

@@ -52,7 +52,11 @@ __device__ __forceinline__ void pressure_faces(double *pm, const KParams &kp, do
 // Maxwellian at the wall temperature, and flies the time it spent past the
 // wall with the new velocity.  Draws: Philox(slot, bounce, step, WALL).
 // With have_f2 the caller has read kick.F2 at the particle's cell into f2.
-template <typename real>
+// BC >= 0 fixes the boundary at compile time (kp.boundary must be it), and
+// DRIFT = false / FACES = false drop the position drift and the face sums:
+// the same operations for the cases kept, with the other paths (and the
+// registers they hold) compiled out.
+template <typename real, int BC = -1, bool DRIFT = true, bool FACES = true>
 __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const KParams &kp,
                                               const Kick &kick,
                                               const double *__restrict__ drift, int mode,
@@ -68,7 +72,7 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
   double v = (double)r.u2;
   const double y_start = y;
   bool drifted = false;
-  if (drift) {
+  if (DRIFT && drift) {
     // DFP_extra_streaming, slab axis (Fokker_Planck.cpp:932-937)
     const double *d = drift + (size_t)cell_of(y, kp.dy, kp.n_cells) * DRIFT_N;
     if (d[11] != 0.0) {
@@ -85,8 +89,9 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
   y = y + v * kp.dt;
   // speed used for the time spent past a wall: the displacement this step
   const double v_eff = drifted ? (y - y_start) / kp.dt : v;
-  if (kp.boundary == MCPM_BC_PERIODIC) {
-    if (kick.pm) pressure_faces(kick.pm, kp, y_old, y, v, true);
+  const int bc = BC >= 0 ? BC : kp.boundary;
+  if (bc == MCPM_BC_PERIODIC) {
+    if (FACES && kick.pm) pressure_faces(kick.pm, kp, y_old, y, v, true);
     if (y < 0.0) y += kp.length;
     else if (y >= kp.length) y -= kp.length;
     real yr = (real)y;
@@ -95,7 +100,7 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
     r.y = yr;
     return r;
   }
-  if (kp.boundary == MCPM_BC_SPECULAR) {
+  if (bc == MCPM_BC_SPECULAR) {
     // BC_specular_reflection (Main_Functions.cpp:6399-6525), slab axis.
     int guard = 0;
     while ((y < kp.wall_lo || y > kp.wall_hi) && guard < 20) {
@@ -107,10 +112,10 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
     if (guard >= 20) atomicOr(err, ERRBIT_BC);
     r.y = (real)y;
     r.u2 = (real)v;
-    if (kick.pm) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
+    if (FACES && kick.pm) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
     return r;
   }
-  if (kp.boundary == MCPM_BC_REINJECT) {
+  if (bc == MCPM_BC_REINJECT) {
     // BC_evaporation (Main_Functions.cpp:5444-5617), slab axis: a particle
     // past a wall leaves the system and is put back at L/2 + U(-1,1) L/4 with
     // a Maxwellian velocity at T_lo (gas.T).  Draws: Philox(slot, 2k | 2k+1,
@@ -128,11 +133,12 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
       r.u1 = (real)(sd * z0);
       r.u2 = (real)(sd * z1);
       r.u3 = (real)(sd * z2);
-      if (kick.esc) atomicAdd(kick.esc, 1ull);
+      if (FACES && kick.esc) atomicAdd(kick.esc, 1ull);
       ++k;
     }
     r.y = (real)y;
-    if (kick.pm && k == 0) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
+    if (FACES && kick.pm && k == 0)
+      pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
     return r;
   }
   // diffuse walls
@@ -172,31 +178,7 @@ __device__ __forceinline__ Rec<real> push_one(Rec<real> r, uint64_t slot, const 
     ++bounce;
   }
   r.y = (real)y;
-  if (kick.pm) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
-  return r;
-}
-
-// The same push on a periodic slab with no position drift and no face sums:
-// the kick, the free flight and the wrap of push_one's periodic branch,
-// operation for operation, without the wall, re-injection and drift paths
-// (whose registers would otherwise bound the sort kernels' occupancy).
-template <typename real>
-__device__ __forceinline__ Rec<real> push_periodic(Rec<real> r, const KParams &kp,
-                                                   const Kick &kick, int mode) {
-  double y = (double)r.y;
-  if (mode & PUSH_KICK) {
-    const double f2 = kick.F2[cell_of(y, kp.dy, kp.n_cells)];
-    r.u2 = (real)((double)r.u2 + f2 * kp.dt);
-  }
-  if (!(mode & PUSH_MOVE)) return r;
-  const double v = (double)r.u2;
-  y = y + v * kp.dt;
-  if (y < 0.0) y += kp.length;
-  else if (y >= kp.length) y -= kp.length;
-  real yr = (real)y;
-  if (!(yr < (real)kp.length)) yr = (real)0;
-  if (!(yr >= (real)0)) yr = (real)0;
-  r.y = yr;
+  if (FACES && kick.pm) pressure_faces(kick.pm, kp, y_old, (double)r.y, (double)r.u2, false);
   return r;
 }
 
@@ -494,7 +476,7 @@ __device__ __forceinline__ int32_t window_offset(int32_t key, int32_t lo, int32_
   return (d >= 0 && d < len) ? d : -1;
 }
 
-template <typename R, bool SCATTER, bool LEAN>
+template <typename R, bool SCATTER, int PK>
 __global__ __launch_bounds__(256) void k_wpush(
     const R *__restrict__ in, R *__restrict__ out, KParams kp, Kick kick,
     const double *__restrict__ drift, int mode, uint32_t step_lo, uint32_t step_hi,
@@ -585,12 +567,14 @@ __global__ __launch_bounds__(256) void k_wpush(
       if (valid) {
         if (!(r.u1 == r.u1) || !(r.u2 == r.u2) || !(r.u3 == r.u3)) atomicOr(err, ERRBIT_NAN);
         if ((mode & PUSH_CHECK) && !in_domain(r, kq)) atomicOr(err, ERRBIT_DOMAIN);
-        if constexpr (LEAN) {
-          r = push_periodic(r, kq, kk, mode);
-        } else {
-          r = push_one(r, (uint64_t)((int64_t)p - slot0), kq, kk, drift, mode, step_lo, step_hi,
-                       err);
-        }
+        // PK: the push of any slab (0), or of a periodic slab without drift
+        // and face sums (1: far fewer registers, more waves per SIMD)
+        const uint64_t slot = (uint64_t)((int64_t)p - slot0);
+        if constexpr (PK == 1)
+          r = push_one<typename R::real_type, MCPM_BC_PERIODIC, false, false>(
+              r, slot, kq, kk, drift, mode, step_lo, step_hi, err);
+        else
+          r = push_one(r, slot, kq, kk, drift, mode, step_lo, step_hi, err);
         d = window_offset(cell_key(r, kq), lo, len, nc, periodic);
         if (d < 0) valid = false;
       }
@@ -2349,10 +2333,10 @@ template __global__ void k_check<COLL_ESMC4>(int, int, uint32_t, const uint4 *, 
                                             uint32_t *, int, R::real_type *,                \
                                             const int32_t *, int32_t *, const StepDev *);
 #define MCPM_INST_W(R)                                                                     \
-  MCPM_INST_WP(R, false, false)                                                            \
-  MCPM_INST_WP(R, true, false)                                                             \
-  MCPM_INST_WP(R, false, true)                                                             \
-  MCPM_INST_WP(R, true, true)                                                              \
+  MCPM_INST_WP(R, false, 0)                                                                \
+  MCPM_INST_WP(R, true, 0)                                                                 \
+  MCPM_INST_WP(R, false, 1)                                                                \
+  MCPM_INST_WP(R, true, 1)                                                                 \
   template __global__ void k_round_tail<R, COLL_ESMC>(int, int, uint32_t, R *, uint4 *,     \
                                                       uint4 *,                              \
                                                       uint32_t *, uint32_t *, uint8_t *,    \
