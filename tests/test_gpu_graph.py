@@ -116,3 +116,30 @@ def test_graph_capture_rejects_what_it_does_not_run():
     t.cell_update()
     with pytest.raises(mcpm.McpmError):
         t.graph_steps(0, 1)  # nothing captured
+
+
+def test_graph_steps_at_the_interface_size():
+    """20M particles, 4096 cells (the benchmark's interface): three steps one
+    call at a time and three graph replays from the same state give the same
+    cells (counts, starts, densities, omega_max, field) and the same FP64
+    energy and momentum sums, bit for bit."""
+    p, arrs, meta = W.interface()
+    _say("graph interface N=20000000 nc=4096")
+    sims = []
+    for _ in range(2):
+        s = mcpm.Simulation(p)
+        s.upload(*arrs)
+        s.cell_update()
+        s.init_omega_max(meta["n_gas"])
+        s.step(0)
+        sims.append(s)
+    del arrs
+    e, g = sims
+    eager = [e.step(s) for s in (1, 2, 3)]
+    g.graph_capture()
+    got = g.graph_steps(1, 3)
+    assert _sum([got]) == _sum(eager)
+    ce, cg = e.cells(), g.cells()
+    for k in ("count", "start", "n", "omega_max", "F2"):
+        np.testing.assert_array_equal(ce[k], cg[k], err_msg=k)
+    np.testing.assert_array_equal(e.energy_momentum(), g.energy_momentum())
