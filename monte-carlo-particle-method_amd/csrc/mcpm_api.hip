@@ -138,6 +138,7 @@ struct mcpm_ctx {
   KParams *rkp = nullptr;
   int32_t *cell_rep = nullptr, *rcell_off = nullptr;
   int64_t *rpart_off = nullptr;
+  int32_t *rpart_bucket = nullptr;  // replica of every 2^REP_BUCKET_SHIFT-th particle
   uint32_t *rprio = nullptr, *rcand_off = nullptr;
   FieldArrays *rfa = nullptr;
   // field
@@ -1577,7 +1578,8 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
     if ((st0 = dalloc(&c->rkp, (size_t)nrep)) || (st0 = dalloc(&c->cell_rep, (size_t)c->nc)) ||
         (st0 = dalloc(&c->rcell_off, (size_t)nrep + 1)) ||
         (st0 = dalloc(&c->rpart_off, (size_t)nrep + 1)) || (st0 = dalloc(&c->rprio, (size_t)nrep)) ||
-        (st0 = dalloc(&c->rcand_off, (size_t)nrep + 1))) {
+        (st0 = dalloc(&c->rcand_off, (size_t)nrep + 1)) ||
+        (st0 = dalloc(&c->rpart_bucket, (size_t)(p->capacity >> REP_BUCKET_SHIFT) + 2))) {
       mcpm_destroy(c);
       return st0;
     }
@@ -1593,6 +1595,7 @@ static int create_ctx(const mcpm_params *p_in, int32_t nrep, const mcpm_replica 
     k.rpart_off = c->rpart_off;
     k.rprio = c->rprio;
     k.rcand_off = c->rcand_off;
+    k.rpart_bucket = nullptr;  // set by the first upload
   }
   // a partner point lies within sigma of its candidate's particle: on the
   // slab at most ceil(sigma / dy) + 1 cells from the candidate's cell (the
@@ -1780,6 +1783,7 @@ int mcpm_destroy(mcpm_ctx *c) {
   if (!c) return MCPM_OK;
   if (c->s) hipStreamSynchronize(c->s);
   void *bufs[] = {c->rkp, c->cell_rep, c->rcell_off, c->rpart_off, c->rprio, c->rcand_off,
+                  c->rpart_bucket,
                   c->rfa,
                   c->P[0], c->P[1], c->ys, c->scan_tmp, c->cratio, c->pw[0], c->ar_fl, c->ar_cst,
                   c->ar_h, c->table, c->partial, c->dcnt, c->dstart, c->keys,
@@ -1942,6 +1946,18 @@ int mcpm_upload_batch(mcpm_ctx *c, const int64_t *counts, const double *y, const
   if (n > c->p.capacity) return MCPM_ERR_CAPACITY;
   HIPCHK(hipMemcpy(c->rpart_off, c->rpart_host.data(), (nrep + 1) * sizeof(int64_t),
                    hipMemcpyHostToDevice));
+  // the replica of each bucket's first particle (the last r with
+  // rpart_off[r] <= b << REP_BUCKET_SHIFT)
+  std::vector<int32_t> bucket((size_t)(c->p.capacity >> REP_BUCKET_SHIFT) + 2, 0);
+  int32_t r = 0;
+  for (size_t b = 0; b < bucket.size(); ++b) {
+    const int64_t p0 = (int64_t)b << REP_BUCKET_SHIFT;
+    while (r + 1 < nrep && c->rpart_host[r + 1] <= p0) ++r;
+    bucket[b] = r;
+  }
+  HIPCHK(hipMemcpy(c->rpart_bucket, bucket.data(), bucket.size() * sizeof(int32_t),
+                   hipMemcpyHostToDevice));
+  c->kp.rpart_bucket = c->rpart_bucket;
   const double *src[4] = {y, u1, u2, u3};
   return upload_any(c, n, src, 4);
 }

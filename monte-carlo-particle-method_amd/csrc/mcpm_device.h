@@ -86,6 +86,9 @@ struct KParams {
   const int64_t *rpart_off;   // nrep + 1
   const uint32_t *rprio;      // candidate priority key of the step, per replica
   const uint32_t *rcand_off;  // first candidate of each replica in this step, nrep + 1
+  // replica of particle b << REP_BUCKET_SHIFT for every b (the search of
+  // rep_of_particle starts there)
+  const int32_t *rpart_bucket;
   // per-cell particle weights (cells[num].n_ratio) when they differ from
   // cell to cell, else null and every cell has n_ratio
   const int32_t *cell_ratio;
@@ -97,7 +100,16 @@ __host__ __device__ __forceinline__ double cell_weight(const KParams &kp, int32_
 }
 
 // Replica of particle p (batched contexts): the last r with rpart_off[r] <= p.
+constexpr int REP_BUCKET_SHIFT = 14;
 __device__ __forceinline__ int32_t rep_of_particle(const KParams &kp, int64_t p) {
+  if (kp.rpart_bucket) {
+    // from the replica of the bucket's first particle, forward past the
+    // replicas that end at or before p (one or two for replicas of more
+    // than 2^14 particles)
+    int32_t r = kp.rpart_bucket[p >> REP_BUCKET_SHIFT];
+    while (kp.rpart_off[r + 1] <= p) ++r;
+    return r;
+  }
   int32_t a = 0, b = kp.nrep;
   while (b - a > 1) {
     const int32_t m = (a + b) >> 1;
