@@ -1698,21 +1698,29 @@ __global__ __launch_bounds__(1024) void k_round_tail(int r0, int nbands, uint32_
                                                      const double *__restrict__ omax,
                                                      double *__restrict__ omega_new, KParams kp,
                                                      FPParams fpp, StepDev *__restrict__ sd) {
-  __shared__ uint32_t s_cnt[NREG], s_out[NREG];
+  __shared__ uint32_t s_cnt[NREG], s_out[NREG], s_pre[NREG + 1];
   __shared__ uint32_t s_acc;
   if (sd_stopped(sd)) return;
   const uint32_t step_lo = sd->lo, step_hi = sd->hi;
   const uint32_t tid = threadIdx.x;
   uint32_t acc_local = 0, visits = 0;
+  // the losers of the last grid round; the tail's own rounds keep their
+  // counts in LDS
+  if (tid < NREG) s_cnt[tid] = counters[CNT_LIST + r0 * NREG + (int)tid];
+  __syncthreads();
   int r = r0;
   for (;;) {
-    if (tid < NREG) {
-      s_cnt[tid] = counters[CNT_LIST + r * NREG + (int)tid];
-      s_out[tid] = 0u;
+    if (tid == 0) {
+      uint32_t run = 0;
+      for (int x = 0; x < NREG; ++x) {
+        s_pre[x] = run;
+        run += s_cnt[x];
+      }
+      s_pre[NREG] = run;
     }
+    if (tid < NREG) s_out[tid] = 0u;
     __syncthreads();
-    uint32_t total = 0;
-    for (int x = 0; x < NREG; ++x) total += s_cnt[x];
+    const uint32_t total = s_pre[NREG];
     if (total == 0u) break;
     if (r >= MAX_ROUNDS) {
       if (tid == 0) atomicOr(&sd->stop, STOP_ROUNDS);
@@ -1724,47 +1732,56 @@ __global__ __launch_bounds__(1024) void k_round_tail(int r0, int nbands, uint32_
     uint32_t *tab = (r & 1) ? tab1 : tab0;
     uint32_t *tabn = (r & 1) ? tab0 : tab1;
     const uint8_t mark = (uint8_t)(sd->mark + (uint32_t)r);
+    // the round's operations as one index over the regions' segments, so a
+    // thread's loads do not wait region after region
+    auto region_of = [&](uint32_t g) {
+      int x = 0;
+      while (g >= s_pre[x + 1]) ++x;
+      return x;
+    };
     // phase 1 (k_check): the smallest priority at every shared particle
-    for (int x = 0; x < NREG; ++x) {
-      for (uint32_t idx = tid; idx < s_cnt[x]; idx += blockDim.x) {
-        const uint4 rec = in[(size_t)x * seg + idx];
-        if (rec.x == OP_NONE) continue;
-        if (tab[rec.x] != rec.z) {
-          atomicMin(&tab[rec.x], rec.z);
-          cflag[rec.x] = mark;
-        }
-        if (tab[rec.y] != rec.z) {
-          atomicMin(&tab[rec.y], rec.z);
-          cflag[rec.y] = mark;
-        }
+    for (uint32_t g = tid; g < total; g += blockDim.x) {
+      const int x = region_of(g);
+      const uint4 rec = in[(size_t)x * seg + (g - s_pre[x])];
+      if (rec.x == OP_NONE) continue;
+      if (tab[rec.x] != rec.z) {
+        atomicMin(&tab[rec.x], rec.z);
+        cflag[rec.x] = mark;
+      }
+      if (tab[rec.y] != rec.z) {
+        atomicMin(&tab[rec.y], rec.z);
+        cflag[rec.y] = mark;
       }
     }
     __syncthreads();
     // phase 2 (k_round): winners commit, losers enter round r + 1
-    for (int x = 0; x < NREG; ++x) {
-      for (uint32_t idx = tid; idx < s_cnt[x]; idx += blockDim.x) {
-        const uint4 rec = in[(size_t)x * seg + idx];
-        if (rec.x == OP_NONE) continue;
-        const bool oki = cflag[rec.x] != mark || tab[rec.x] == rec.z;
-        const bool okj = cflag[rec.y] != mark || tab[rec.y] == rec.z;
-        if (oki && okj) {
-          const R a = P[rec.x], bb = P[rec.y];
-          const bool acc =
-              MODEL == COLL_DSMC
-                  ? commit_dsmc<R>(rec.x, rec.y, rec.z, rec.w, a, bb, P, omax, omega_new, kp,
-                                   fpp, step_lo, step_hi)
-                  : commit_op<R>(rec.x, rec.y, rec.w, a, bb, P, nden, yc, omax, omega_new, kp,
-                                 step_lo, step_hi, MODEL == COLL_ESMC4);
-          acc_local += acc ? 1u : 0u;
-        } else {
-          tabn[rec.x] = rec.z;
-          tabn[rec.y] = rec.z;
-          out[(size_t)x * seg + atomicAdd(&s_out[x], 1u)] = rec;
-        }
+    for (uint32_t g = tid; g < total; g += blockDim.x) {
+      const int x = region_of(g);
+      const uint4 rec = in[(size_t)x * seg + (g - s_pre[x])];
+      if (rec.x == OP_NONE) continue;
+      const bool oki = cflag[rec.x] != mark || tab[rec.x] == rec.z;
+      const bool okj = cflag[rec.y] != mark || tab[rec.y] == rec.z;
+      if (oki && okj) {
+        const R a = P[rec.x], bb = P[rec.y];
+        const bool acc =
+            MODEL == COLL_DSMC
+                ? commit_dsmc<R>(rec.x, rec.y, rec.z, rec.w, a, bb, P, omax, omega_new, kp,
+                                 fpp, step_lo, step_hi)
+                : commit_op<R>(rec.x, rec.y, rec.w, a, bb, P, nden, yc, omax, omega_new, kp,
+                               step_lo, step_hi, MODEL == COLL_ESMC4);
+        acc_local += acc ? 1u : 0u;
+      } else {
+        tabn[rec.x] = rec.z;
+        tabn[rec.y] = rec.z;
+        out[(size_t)x * seg + atomicAdd(&s_out[x], 1u)] = rec;
       }
     }
     __syncthreads();
-    if (tid < NREG) counters[CNT_LIST + (r + 1) * NREG + (int)tid] = s_out[tid];
+    if (tid < NREG) {
+      counters[CNT_LIST + (r + 1) * NREG + (int)tid] = s_out[tid];
+      s_cnt[tid] = s_out[tid];
+    }
+    __syncthreads();
     ++r;
   }
   for (int off = 32; off > 0; off >>= 1) acc_local += __shfl_down(acc_local, off);
