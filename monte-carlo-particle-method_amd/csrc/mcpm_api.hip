@@ -2348,12 +2348,15 @@ int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_
     h.hi = (uint32_t)((uint64_t)next >> 32);
     h.mark = c->round_base;
   }
-  c->sorted = true;
-  c->order_sorted = true;
-  c->ys_valid = true;
-  c->kick_pending = false;
-  c->keys_current = false;
-  c->have_closures = false;
+  if (done > 0) {
+    // as after as many mcpm_step calls
+    c->sorted = true;
+    c->order_sorted = true;
+    c->ys_valid = true;
+    c->kick_pending = false;
+    c->keys_current = false;
+    c->have_closures = false;
+  }
   if (stats) {
     *stats = tot;
     stats->rounds = (int32_t)std::min<int64_t>(rsum, INT32_MAX);

@@ -53,6 +53,7 @@ def test_graph_steps_equal_eager_steps(precision):
     parts.append(g.step(3))          # an eager step between replays
     parts.append(g.graph_steps(4, 2))
     parts.append(g.graph_steps(6, 3))
+    assert g.graph_steps(9, 0)["candidates"] == 0  # no step, no change
     assert _sum(parts) == _sum(eager)
     _same_state(e, g)
 
