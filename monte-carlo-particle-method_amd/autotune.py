@@ -13,6 +13,7 @@ more than `margin`.  Knobs are tuned one at a time in the order of SPACE.
 
     python autotune.py --workload interface --steps 4
     python autotune.py --workload replicas --replicas 256
+    python autotune.py --workload dense --model fp
 """
 from __future__ import annotations
 
@@ -88,6 +89,14 @@ def build(args):
         sim.upload(*arrs)
         sim.cell_update()
         sim.init_omega_max(meta["n_gas"])
+    elif args.workload == "dense":
+        p, arrs, meta = W.dense_slab(n_density=args.density)
+        if args.model == "fp":
+            p.model = mcpm.MODEL_FP_DENSE
+        sim = mcpm.Simulation(p)
+        sim.upload(*arrs)
+        sim.cell_update()
+        sim.init_omega_max(meta["n_gas"])
     elif args.workload == "replicas":
         common, reps, rarrs, metas = W.replicas(n_rep=args.replicas, seed0=0)
         counts = [a[0].size for a in rarrs]
@@ -108,7 +117,10 @@ def build(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="interface",
-                    choices=["interface", "replicas", "droplets"])
+                    choices=["interface", "replicas", "dense", "droplets"])
+    ap.add_argument("--model", default="esmc", choices=["esmc", "fp"],
+                    help="dense workload: ESMC or FP_dense")
+    ap.add_argument("--density", type=float, default=0.3, help="dense workload: n sigma^3")
     ap.add_argument("--particles", type=int, default=20_000_000)
     ap.add_argument("--cells", type=int, default=4096)
     ap.add_argument("--replicas", type=int, default=256)
@@ -123,11 +135,16 @@ def main():
         sim.step(step)
         step += 1
     sim.sync()
-    graph = not args.no_graph and args.workload != "droplets"
+    fp = args.workload == "dense" and args.model == "fp"
+    graph = not args.no_graph and args.workload != "droplets" and not fp
     space = dict(SPACE)
     if not graph:
         space.pop(GRAPH_ROUNDS)
-    if args.workload != "droplets":
+    if fp:
+        # no collisions; its particles drift past the windows of the
+        # windowed sort, so the full-histogram sort's chunk is what counts
+        space = {mcpm.TUNE_SORT_CHUNK: [0, 1024, 2048, 4096, 8192]}
+    elif args.workload != "droplets":
         # the slabs sort by the windowed sort (a fixed chunk of particles)
         space.pop(mcpm.TUNE_SORT_CHUNK)
     best = {k: sim.get_tuning(k) for k in space if k != GRAPH_ROUNDS}
