@@ -74,7 +74,7 @@ def main():
                          "calibrating the FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
+        print(f"bench.py: measures one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU decomposition is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
 

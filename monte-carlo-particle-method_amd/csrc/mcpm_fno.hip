@@ -6,7 +6,7 @@
 // maps the nodal density of a periodic plane to the Q1 FEM solution of
 // (K + lambda^2 M) phi = M n that compute_SP_2D (FEM.cpp:249-610) solves
 // before its core subtraction (FEM.cpp:497-554).  The reference has no such
-// network; the exact solve stays the judge: a field whose FEM residual
+// network; the exact solve stays the arbiter: a field whose FEM residual
 // exceeds the caller's bound is finished by conjugate gradients started from
 // the network's field.
 //
