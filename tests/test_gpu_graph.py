@@ -144,3 +144,40 @@ def test_graph_steps_at_the_interface_size():
     for k in ("count", "start", "n", "omega_max", "F2"):
         np.testing.assert_array_equal(ce[k], cg[k], err_msg=k)
     np.testing.assert_array_equal(e.energy_momentum(), g.energy_momentum())
+
+
+@pytest.mark.parametrize("precision", [mcpm.FP32, mcpm.FP64])
+def test_graph_steps_between_diffuse_walls_without_field(precision):
+    """The dense slab's geometry (diffuse walls at T = 1.0 and 1.5, 0.005
+    sigma cells, no mean field) at 40 particles per cell: wall re-emission
+    inside a captured step (the general push), the same steps bit for bit."""
+    p, arrs, meta = W.dense_slab(n_density=0.3, ppc=40, seed=76)
+    p.precision = precision
+    _say(f"graph dense slab nc={p.n_cells} N={arrs[0].size} prec={precision}")
+    e, g = _pair(p, arrs, meta["n_gas"])
+    eager = [e.step(s) for s in range(6)]
+    g.graph_capture()
+    got = g.graph_steps(0, 6)
+    assert _sum([got]) == _sum(eager)
+    _same_state(e, g)
+
+
+def test_graph_steps_of_fp64_replicas_without_field():
+    common, reps, arrs, metas = W.replicas(n_rep=4, N_lo=20_000, N_hi=40_000, cells_lo=128,
+                                           cells_hi=200, precision=mcpm.FP64,
+                                           long_range=mcpm.LR_NONE)
+    counts = [a[0].size for a in arrs]
+    _say(f"graph replicas FP64 R=4 N={sum(counts)}")
+    sims = []
+    for _ in range(2):
+        b = mcpm.Simulation.batch(common, reps)
+        b.upload_batch(counts, *[np.concatenate([a[q] for a in arrs]) for q in range(4)])
+        b.cell_update()
+        b.init_omega_max_batch([m["n_gas"] for m in metas])
+        sims.append(b)
+    e, g = sims
+    eager = [e.step(s) for s in range(4)]
+    g.graph_capture()
+    got = g.graph_steps(0, 4)
+    assert _sum([got]) == _sum(eager)
+    _same_state(e, g)
