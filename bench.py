@@ -544,12 +544,13 @@ def random_access_roofline(n, rec, valid, acc, revisits, nbands, coll_ms, launch
     share one memory system).  Per step, in k_check + k_round: 2 priority
     stores per valid operation outside band 0 (band 0 is stored by k_gen) and
     per revisit, 2 priority-word reads (k_check) and 2 mark-byte reads
-    (k_round) per visit (operations + revisits), 2 record reads per operation
+    (k_round; one mark byte per 32 particles) per visit (operations +
+    revisits), 2 record reads per operation
     (its one winning visit), 2 record writes per accepted collision, and the
     16-byte operation records streamed in (twice per visit, once per band
     admission) and out (each revisit) at the HBM peak.  The atomics and
-    word re-reads of particles that several operations share are left out
-    (a few per cent of the operations)."""
+    word re-reads of particles that several operations share, or whose mark
+    byte another particle of the group set, are left out."""
     import mcpm
     rates = mcpm.probe_random_access(n, rec)
     visits = valid + revisits

@@ -565,7 +565,8 @@ int mcpm_timing_get(mcpm_ctx *ctx, int32_t which, double *avg_ms, int64_t *calls
 /* No counterpart in the reference: a measurement of the current device for
    the roofline of the reservation rounds, whose accesses all go to
    unpredictable particles.  Over n_records records of record_bytes (16 or 32),
-   n_records 4-byte words and n_records bytes, with indices hashed uniformly,
+   n_records 4-byte words and n_records / 32 bytes (the rounds' marks, one
+   byte per 32 particles), with indices hashed uniformly,
    the best of five launches of min(n_records, 2^24) accesses each, in
    accesses per second: record gathers, word gathers, 32-bit atomic minimums
    (result unused), record stores, word stores and byte gathers.  Allocates

@@ -31,6 +31,13 @@ enum : int {
   CNT_TOTAL = CNT_BSTART + MAX_BANDS * NREG + 1
 };
 
+// A shared-particle mark of the reservation rounds covers a group of
+// 2^MARK_SHIFT consecutive particles (neighbours in cell order).  A mark on
+// a group whose particle has a single operation only makes that operation
+// read its particle's priority word, which holds its own priority; the
+// smaller table stays in an XCD's L2 while the rounds gather it.
+constexpr int MARK_SHIFT = 5;
+
 // State of a run of graph-replayed steps, in device memory.  The kernels of
 // a captured step read the step number, its priority key and its round tags
 // from here instead of from their arguments, and return at once while stop

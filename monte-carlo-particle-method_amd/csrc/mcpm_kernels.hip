@@ -1523,11 +1523,11 @@ __global__ __launch_bounds__(256) void k_check(int r, int nbands, uint32_t seg,
       if (rec[u].x == OP_NONE) continue;
       if (ti[u] != rec[u].z) {
         atomicMin(&tab[rec[u].x], rec[u].z);
-        cflag[rec[u].x] = mark;
+        cflag[rec[u].x >> MARK_SHIFT] = mark;
       }
       if (tj[u] != rec[u].z) {
         atomicMin(&tab[rec[u].y], rec[u].z);
-        cflag[rec[u].y] = mark;
+        cflag[rec[u].y >> MARK_SHIFT] = mark;
       }
     }
   }
@@ -1601,8 +1601,8 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
     for (int u = 0; u < U; ++u) {
       fi[u] = fj[u] = false;
       if (rec[u].x != OP_NONE) {
-        fi[u] = cflag[rec[u].x] == mark;
-        fj[u] = cflag[rec[u].y] == mark;
+        fi[u] = cflag[rec[u].x >> MARK_SHIFT] == mark;
+        fj[u] = cflag[rec[u].y >> MARK_SHIFT] == mark;
       }
     }
     R pa[U], pb[U];
@@ -1746,11 +1746,11 @@ __global__ __launch_bounds__(1024) void k_round_tail(int r0, int nbands, uint32_
       if (rec.x == OP_NONE) continue;
       if (tab[rec.x] != rec.z) {
         atomicMin(&tab[rec.x], rec.z);
-        cflag[rec.x] = mark;
+        cflag[rec.x >> MARK_SHIFT] = mark;
       }
       if (tab[rec.y] != rec.z) {
         atomicMin(&tab[rec.y], rec.z);
-        cflag[rec.y] = mark;
+        cflag[rec.y >> MARK_SHIFT] = mark;
       }
     }
     __syncthreads();
@@ -1759,8 +1759,8 @@ __global__ __launch_bounds__(1024) void k_round_tail(int r0, int nbands, uint32_
       const int x = region_of(g);
       const uint4 rec = in[(size_t)x * seg + (g - s_pre[x])];
       if (rec.x == OP_NONE) continue;
-      const bool oki = cflag[rec.x] != mark || tab[rec.x] == rec.z;
-      const bool okj = cflag[rec.y] != mark || tab[rec.y] == rec.z;
+      const bool oki = cflag[rec.x >> MARK_SHIFT] != mark || tab[rec.x] == rec.z;
+      const bool okj = cflag[rec.y >> MARK_SHIFT] != mark || tab[rec.y] == rec.z;
       if (oki && okj) {
         const R a = P[rec.x], bb = P[rec.y];
         const bool acc =

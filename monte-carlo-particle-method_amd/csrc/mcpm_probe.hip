@@ -1,6 +1,7 @@
 // Random-access rates of the device, measured on arrays shaped like the
-// collision schedule's: particle records, one 4-byte priority word and one
-// mark byte per particle.  The reservation rounds are bound by
+// collision schedule's: particle records, one 4-byte priority word per
+// particle and one mark byte per 2^MARK_SHIFT particles.  The reservation
+// rounds are bound by
 // these rates (every access of an operation goes to an unpredictable
 // particle), not by streaming bandwidth, so the rates are the ceiling
 // their timings are set against.
@@ -9,6 +10,7 @@
 #include <cstdint>
 
 #include "mcpm.h"
+#include "mcpm_kernels.h"
 
 namespace mcpm {
 namespace {
@@ -98,6 +100,7 @@ extern "C" int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes,
     return MCPM_ERR_ARG;
   const uint32_t n = (uint32_t)n_records, w = (uint32_t)record_bytes / 16u;
   const uint32_t nops = n < (1u << 24) ? n : (1u << 24);
+  const uint32_t nb = n >> MARK_SHIFT;  // mark bytes
   uint4 *rec = nullptr, *sink = nullptr;
   uint32_t *words = nullptr;
   uint8_t *bytes = nullptr;
@@ -108,7 +111,7 @@ extern "C" int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes,
   double best[NKIND] = {1e30, 1e30, 1e30, 1e30, 1e30, 1e30};
   if (hipMalloc((void **)&rec, (size_t)n * w * sizeof(uint4)) != hipSuccess ||
       hipMalloc((void **)&words, (size_t)n * sizeof(uint32_t)) != hipSuccess ||
-      hipMalloc((void **)&bytes, (size_t)n) != hipSuccess ||
+      hipMalloc((void **)&bytes, (size_t)nb) != hipSuccess ||
       hipMalloc((void **)&sink, 2 * sizeof(uint4)) != hipSuccess ||
       hipStreamCreate(&s) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
       hipEventCreate(&e1) != hipSuccess) {
@@ -117,7 +120,7 @@ extern "C" int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes,
   }
   if (hipMemsetAsync(rec, 0x11, (size_t)n * w * sizeof(uint4), s) != hipSuccess ||
       hipMemsetAsync(words, 0xFF, (size_t)n * sizeof(uint32_t), s) != hipSuccess ||
-      hipMemsetAsync(bytes, 0, (size_t)n, s) != hipSuccess) {
+      hipMemsetAsync(bytes, 0, (size_t)nb, s) != hipSuccess) {
     st = MCPM_ERR_HIP;
     goto done;
   }
@@ -138,7 +141,7 @@ extern "C" int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes,
       else if (kind == 4)
         hipLaunchKernelGGL(k_probe_scatter_word, dim3(8192), dim3(256), 0, s, words, n, nops);
       else
-        hipLaunchKernelGGL(k_probe_gather_byte, dim3(8192), dim3(256), 0, s, bytes, n, nops,
+        hipLaunchKernelGGL(k_probe_gather_byte, dim3(8192), dim3(256), 0, s, bytes, nb, nops,
                            (uint32_t *)sink);
       hipEventRecord(e1, s);
       if (hipEventSynchronize(e1) != hipSuccess) {
