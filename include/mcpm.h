@@ -274,10 +274,13 @@ int mcpm_step(mcpm_ctx *ctx, int64_t step, mcpm_collision_stats *stats);
    twice the last collision call's, at least N).
    mcpm_graph_steps runs steps step0 .. step0 + nsteps - 1 by replaying the
    graphs; the results are bit for bit those of as many mcpm_step calls.
-   It looks at the device state every 64 steps.  A step whose candidates
+   It looks at the device state after 8 steps, then after twice as many
+   each time up to 64, and after 8 again following a stop (the replays
+   queued behind a stopped step run as no-ops).  A step whose candidates
    exceed the capacity is recaptured with twice the capacity and run again;
    a step whose particle left its sort window finishes its sort on the host
-   (full histogram) and the run continues, recaptured; stats are sums over
+   (full histogram), the windows widen (the graphs read their width from
+   device memory) and the same graphs replay on; stats are sums over
    the steps (rounds: summed; revisits: operations the one-workgroup tail
    handled).  Allocates only when it captures. */
 int mcpm_graph_capture(mcpm_ctx *ctx, int64_t op_capacity, int32_t rounds);

@@ -171,7 +171,7 @@ struct mcpm_ctx {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   int64_t g_cap = 0, g_n = -1;
   int64_t last_m = 0;  // candidates of the last collision call
-  int g_rounds = 0, g_wdelta = 0;
+  int g_rounds = 0;
   // timing
   bool timing = false;
   std::vector<std::pair<hipEvent_t, hipEvent_t>> ev[T_NCLASS];
@@ -1276,7 +1276,6 @@ int graph_capture_t(mcpm_ctx *c, int64_t cap, int rounds) {
   }
   c->g_cap = cap;
   c->g_rounds = rounds;
-  c->g_wdelta = c->wdelta;
   const int cur0 = c->cur;
   for (int par = 0; par < 2; ++par) {
     c->cur = par;
@@ -2271,8 +2270,8 @@ int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_
   if (stats) memset(stats, 0, sizeof *stats);
   if (c->g_cap <= 0) return MCPM_ERR_STATE;  // nothing captured
   if (!graph_ok(c)) return MCPM_ERR_STATE;
-  // lists reallocated, other particles or wider sort windows since the capture
-  if (!c->gexec[0] || !c->gexec[1] || c->n != c->g_n || c->wdelta != c->g_wdelta)
+  // lists reallocated or other particles since the capture
+  if (!c->gexec[0] || !c->gexec[1] || c->n != c->g_n)
     if (int st = graph_capture(c, c->g_cap, c->g_rounds)) return st;
   if (!c->ys_valid) {
     const int st = c->p.precision == MCPM_FP32 ? ys_fill_t<Rec<float>>(c) : ys_fill_t<Rec<double>>(c);
@@ -2284,7 +2283,13 @@ int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_
   h.lo = (uint32_t)step0;
   h.hi = (uint32_t)((uint64_t)step0 >> 32);
   h.mark = c->round_base;
-  constexpr int64_t BATCH = 64;  // replays between looks at the run state
+  h.wdelta = (uint32_t)c->wdelta;
+  // replays queued between looks at the run state: the replays behind a
+  // step that stops run as no-ops (about 0.2 ms each), so a run starts with
+  // a short batch, doubles it after every batch that ran through, up to 64,
+  // and starts short again after a stop
+  constexpr int64_t BATCH0 = 8, BATCH_MAX = 64;
+  int64_t batch = BATCH0;
   bool eager = false;
   while (done < nsteps) {
     if (eager) {
@@ -2301,7 +2306,7 @@ int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_
     const StepDev up = h;
     HIPCHK(hipMemcpyAsync(c->sd, &up, sizeof up, hipMemcpyHostToDevice, c->s));
     const int cur0 = c->cur;
-    const int64_t todo = std::min(BATCH, nsteps - done);
+    const int64_t todo = std::min(batch, nsteps - done);
     for (int64_t k = 0; k < todo; ++k) HIPCHK(hipGraphLaunch(c->gexec[(cur0 + k) & 1], c->s));
     HIPCHK(hipMemcpyAsync(&h, c->sd, sizeof h, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
@@ -2314,14 +2319,14 @@ int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_
     tail += (int64_t)h.tail_ops;
     c->round_base = h.mark;
     int64_t next = (int64_t)(((uint64_t)h.hi << 32) | h.lo);
+    batch = h.stop ? BATCH0 : std::min(2 * batch, BATCH_MAX);
     if (h.stop & STOP_ROUNDS) return MCPM_ERR_CAPACITY;
     if (h.stop & STOP_WINDOW) {
       // step `next` ran its collisions and field; a particle left its sort
-      // window, so its push and sort run here (full histogram), and the
-      // wider windows need a new capture
-      // (the failed windowed sort left the cell bounds unusable: the full
-      // histogram sorts without them; the windows widen as after an eager
-      // step's failed windowed sort)
+      // window, so its push and sort run here (full histogram; the failed
+      // windowed sort left the cell bounds unusable), and the windows widen
+      // as after an eager step's failed windowed sort: the graphs read the
+      // width from the run state, so they replay as captured
       c->kick_pending = c->p.long_range == MCPM_LR_SCREENED_POISSON;
       const bool we = c->w_enabled;
       c->w_enabled = false;
@@ -2334,10 +2339,17 @@ int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_
       else ++c->w_fallbacks;
       ++done;
       ++next;
-      if (graph_ok(c)) {
-        if (int st = graph_capture(c, c->g_cap, c->g_rounds)) return st;
-      } else {
+      if (!graph_ok(c)) {
         eager = true;
+      } else {
+        if (!c->gexec[0] || !c->gexec[1])
+          if (int st = graph_capture(c, c->g_cap, c->g_rounds)) return st;
+        // the two-digit sort (many cells) leaves no sorted y for the draw
+        if (!c->ys_valid) {
+          const int st = c->p.precision == MCPM_FP32 ? ys_fill_t<Rec<float>>(c)
+                                                     : ys_fill_t<Rec<double>>(c);
+          if (st) return st;
+        }
       }
     } else if (h.stop & STOP_CAPACITY) {
       // nothing of step `next` ran: longer operation lists
@@ -2347,6 +2359,7 @@ int mcpm_graph_steps(mcpm_ctx *c, int64_t step0, int64_t nsteps, mcpm_collision_
     h.lo = (uint32_t)next;
     h.hi = (uint32_t)((uint64_t)next >> 32);
     h.mark = c->round_base;
+    h.wdelta = (uint32_t)c->wdelta;
   }
   if (done > 0) {
     // as after as many mcpm_step calls

@@ -51,7 +51,7 @@ struct StepDev {
   uint32_t stop;     // STOP_* bits
   uint32_t rounds;   // rounds the step used (k_round_tail)
   uint32_t steps;    // steps completed in the run
-  uint32_t pad;
+  uint32_t wdelta;   // slack cells of the sort windows (widened by the host between runs)
   unsigned long long cand, valid, acc, rsum, tail_ops;  // totals over the completed steps
 };
 enum : uint32_t { STOP_WINDOW = 1u, STOP_CAPACITY = 2u, STOP_ROUNDS = 4u };

@@ -491,6 +491,7 @@ __global__ __launch_bounds__(256) void k_wpush(
   if (sd) {
     step_lo = sd->lo;
     step_hi = sd->hi;
+    wdelta = (int)sd->wdelta;
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int nch = (int)*nchp;

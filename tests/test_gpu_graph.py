@@ -4,7 +4,8 @@ densities, omega_max and the field bit for bit, and the same collision
 counts.  Covered: the interface slab in FP32 and FP64 with eager steps
 between replays, batched replicas, a capacity too small for a step (the run
 recaptures and repeats the step), and particles that leave their sort
-windows (the host finishes that step's sort and the run goes on)."""
+windows (the host finishes that step's sort and the run goes on), also on
+more than 16384 cells."""
 import numpy as np
 import pytest
 
@@ -179,5 +180,35 @@ def test_graph_steps_of_fp64_replicas_without_field():
     eager = [e.step(s) for s in range(4)]
     g.graph_capture()
     got = g.graph_steps(0, 4)
+    assert _sum([got]) == _sum(eager)
+    _same_state(e, g)
+
+
+def test_graph_particles_leaving_their_windows_on_many_cells():
+    """64 replicas, more than 16384 cells in all: the full-histogram sort that
+    finishes a step whose particles left their windows is then the two-digit
+    sort, which writes no sorted y array, and the next replayed step draws
+    its candidates from the y array refilled after it."""
+    common, reps, arrs, metas = W.replicas(n_rep=64, N_lo=6000, N_hi=10000, cells_lo=260,
+                                           cells_hi=300)
+    assert sum(r["n_cells"] for r in reps) > 16384
+    arrs = [[a.copy() for a in t] for t in arrs]
+    for t in arrs[::8]:
+        t[2][::1000] = 1500.0  # about 20 cells per step at dt = 0.02
+    counts = [a[0].size for a in arrs]
+    _say(f"graph replicas window overflow R=64 N={sum(counts)}")
+    sims = []
+    for _ in range(2):
+        b = mcpm.Simulation.batch(common, reps)
+        b.upload_batch(counts, *[np.concatenate([a[q] for a in arrs]) for q in range(4)])
+        b.cell_update()
+        b.init_omega_max_batch([m["n_gas"] for m in metas])
+        sims.append(b)
+    e, g = sims
+    eager = [e.step(s) for s in range(6)]
+    g.graph_capture()
+    redone = g.get_tuning(mcpm.TUNE_SORT_WINDOW_REDONE)
+    got = g.graph_steps(0, 6)
+    assert g.get_tuning(mcpm.TUNE_SORT_WINDOW_REDONE) > redone  # the case happened
     assert _sum([got]) == _sum(eager)
     _same_state(e, g)
