@@ -723,6 +723,9 @@ __global__ __launch_bounds__(256) void k_wscan(const int32_t *__restrict__ wlo,
   const bool periodic = kq.boundary == MCPM_BC_PERIODIC;
   uint32_t run = 0;
   for (int im = periodic ? -1 : 0; im <= (periodic ? 1 : 0); ++im) {
+    // every window lies in [-WDELTA_MAX, nc + WDELTA_MAX): the images
+    // below 0 and past nc - 1 hold chunks only for cells near the ends
+    if ((im < 0 && kl < nc - WMAX) || (im > 0 && kl >= WMAX)) continue;
     const int32_t u = kl + im * nc;
     // first chunk whose window ends past u, first whose window starts past u
     const int blo = wave_lower_bound(b0, b1, [&](int m) { return wlo[m] + wlen[m] > u; });
@@ -1074,6 +1077,10 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
   const int esmc = model != COLL_DSMC;
   const uint32_t x0 = op_region(lo, M);
   uint32_t c = model == COLL_ESMC4 ? scell : 0u;
+  // the replica of this thread's candidates: candidates run in cell order,
+  // replica after replica, so it walks forward from the replica of the
+  // chunk's first cell (rep_of_cand's answer, without its search)
+  int32_t rq = kp.nrep ? kp.cell_rep[scell] : 0;
   for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x) {
     const uint32_t k = lo + b0 + threadIdx.x;
     if (k >= M) break;
@@ -1082,8 +1089,13 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
       while (cstart[c + 1] <= k) ++c;
       band = band_new4(rank_new4(c, k - cstart[c], cstart, M), M, nbands);
     } else {
-      const uint32_t pr = kp.nrep ? cand_priority_rep(kp, rep_of_cand(kp, k), k, esmc)
-                                  : cand_priority(k, prio_key, esmc);
+      uint32_t pr;
+      if (kp.nrep) {
+        while (kp.rcand_off[rq + 1] <= k) ++rq;
+        pr = cand_priority_rep(kp, rq, k, esmc);
+      } else {
+        pr = cand_priority(k, prio_key, esmc);
+      }
       band = op_band(pr, band_mul, nbands);
     }
     const uint32_t q = band * NREG + region_from(k, x0, M);
