@@ -598,13 +598,20 @@ int mcpm_probe_random_access(int64_t n_records, int32_t record_bytes, mcpm_acces
                          full histogram when a particle left its window; 0: the
                          full histogram always
    MCPM_TUNE_SORT_WINDOW_RUNS, MCPM_TUNE_SORT_WINDOW_REDONE (read only):
-                         windowed sorts run and redone since creation */
+                         windowed sorts run and redone since creation
+   MCPM_TUNE_ROUND_UNROLL operations per thread of a reservation round, 2 or
+                         4; 0 (default): 4 when a priority band's operations
+                         fit in one pass of the round grid at 2 per thread,
+                         else 2 (DSMC: 4)
+   A captured step (mcpm_graph_capture) keeps the values it was captured
+   with; capture again for new ones. */
 #define MCPM_TUNE_BANDS      0
 #define MCPM_TUNE_ROUND_GRID 1
 #define MCPM_TUNE_SORT_CHUNK 2
 #define MCPM_TUNE_SORT_WINDOW 3
 #define MCPM_TUNE_SORT_WINDOW_RUNS 4
 #define MCPM_TUNE_SORT_WINDOW_REDONE 5
+#define MCPM_TUNE_ROUND_UNROLL 6
 int mcpm_set_tuning(mcpm_ctx *ctx, int32_t knob, int64_t value);
 int mcpm_get_tuning(mcpm_ctx *ctx, int32_t knob, int64_t *value);
 

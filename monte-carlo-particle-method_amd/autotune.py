@@ -1,8 +1,9 @@
 #!/usr/bin/env python3
 """Autotuner of the launch shapes: priority bands of the collision rounds,
 the round grid (workgroups per round), the sort chunk (particles per
-workgroup of the full-histogram sort) and the grid rounds of a captured step
-(the rounds before the one-workgroup tail).
+workgroup of the full-histogram sort), the operations per thread of a round
+and the grid rounds of a captured step (the rounds before the one-workgroup
+tail).
 
 None of these changes a result, only the time.  The state of the system keeps
 evolving while the tuner runs and the step cost drifts with it (omega_max
@@ -34,10 +35,12 @@ SPACE = {
     mcpm.TUNE_BANDS: [8, 10, 12, 14, 16],
     mcpm.TUNE_ROUND_GRID: [512, 1024, 1536, 2048],
     mcpm.TUNE_SORT_CHUNK: [0, 2048, 4096, 8192],
+    mcpm.TUNE_ROUND_UNROLL: [2, 4],  # 0 (the start): chosen from the band size
     GRAPH_ROUNDS: [2, 4, 6, 8],  # grid rounds beyond the band count
 }
 NAMES = {mcpm.TUNE_BANDS: "bands", mcpm.TUNE_ROUND_GRID: "round_grid",
-         mcpm.TUNE_SORT_CHUNK: "sort_chunk", GRAPH_ROUNDS: "graph_rounds_past_bands"}
+         mcpm.TUNE_SORT_CHUNK: "sort_chunk", mcpm.TUNE_ROUND_UNROLL: "round_unroll",
+         GRAPH_ROUNDS: "graph_rounds_past_bands"}
 
 
 def tune(space, best, apply, time_block, pairs=2, margin=0.01, log=None, names=None):

@@ -117,6 +117,7 @@ struct mcpm_ctx {
   // pinned host memory, which the host polls without touching the stream
   uint32_t *hleft = nullptr, *dleft = nullptr;
   int round_grid = 1024;
+  int round_unroll = 0;    // k_round's operations per thread: 0 = from the band size, 2 or 4
   int sort_chunk = 0;
   OpLog log{};                           // one allocation: debug log
   int64_t op_cap = 0;
@@ -823,6 +824,22 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   return MCPM_OK;
 }
 
+// k_round's operations per thread: the knob's value, else (ESMC) 4 when a
+// band's operations fit in one pass of the round grid at 2 per thread (the
+// whole grid is then resident, and more operations per thread put more loads
+// in flight) and 2 when they do not (fewer registers, more wavefronts per
+// SIMD); DSMC runs all its operations in one band: 4.
+int round_unroll_for(const mcpm_ctx *c, int model, int64_t m, int nbands, unsigned grid) {
+  if (c->round_unroll) return c->round_unroll;
+  if (model == COLL_DSMC) return 4;
+  return m / std::max(nbands, 1) > (int64_t)grid * 256 * 2 ? 2 : 4;
+}
+template <typename R, int MODEL>
+decltype(&k_round<R, MODEL, true, 4>) round_kernel(bool plain, int u) {
+  if (plain) return u == 2 ? k_round<R, MODEL, true, 2> : k_round<R, MODEL, true, 4>;
+  return u == 2 ? k_round<R, MODEL, false, 2> : k_round<R, MODEL, false, 4>;
+}
+
 // variant: for COLL_DSMC the mcpm_model run (DSMC_VHS, DSMC_HS, CBA_VHS).
 template <typename R, int MODEL>
 int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int variant) {
@@ -865,6 +882,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
   // mark is the low byte of round_base, which advances once per round (and
   // across calls), so consecutive rounds never share a mark
   uint32_t *const tabs[2] = {c->rtab, c->rtab + (size_t)c->p.capacity};
+  const int unroll = round_unroll_for(c, MODEL, M, nbands, grid);
   if (M > 0) {
     Timer tm(c, T_GEN);
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
@@ -903,8 +921,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
                            (const uint4 *)c->opBand, tabs[r & 1], c->cflag, mark,
                            (const uint32_t *)c->counters, c->dleft, (const StepDev *)nullptr);
         // no replicas and no log: the kernel without their code (fewer registers)
-        auto round_k = (c->kp.nrep == 0 && !lg.p) ? k_round<R, MODEL, true>
-                                                  : k_round<R, MODEL, false>;
+        auto round_k = round_kernel<R, MODEL>(c->kp.nrep == 0 && !lg.p, unroll);
         hipLaunchKernelGGL(round_k, dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
                            P, in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
                            (const uint32_t *)tabs[r & 1], tabs[(r + 1) & 1],
@@ -1229,13 +1246,14 @@ int graph_step_launch(mcpm_ctx *c) {
   LAUNCHCHK();
   const uint32_t seg = (cap + NREG - 1) / NREG;
   const unsigned grid = (unsigned)((std::max(8, c->round_grid) + 7) / 8 * 8);
+  const int unroll = round_unroll_for(c, COLL_ESMC, c->last_m, nbands, grid);
   const OpLog nolog{};
   for (int r = 0; r < c->g_rounds; ++r) {
     const uint4 *in = (const uint4 *)((r & 1) ? c->opB : c->opA);
     hipLaunchKernelGGL((k_check<COLL_ESMC>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
                        (const uint4 *)c->opBand, tabs[r & 1], c->cflag, 0u,
                        (const uint32_t *)c->counters, (uint32_t *)nullptr, csd);
-    auto round_k = c->kp.nrep == 0 ? k_round<R, COLL_ESMC, true> : k_round<R, COLL_ESMC, false>;
+    auto round_k = round_kernel<R, COLL_ESMC>(c->kp.nrep == 0, unroll);
     hipLaunchKernelGGL(round_k, dim3(grid), dim3(256), 0, c->s, r, nbands, seg, P,
                        in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,
                        (const uint32_t *)tabs[r & 1], tabs[(r + 1) & 1],
@@ -2695,6 +2713,10 @@ int mcpm_set_tuning(mcpm_ctx *c, int32_t knob, int64_t v) {
       c->w_enabled = v == 1;
       c->w_fallbacks = 0;
       return MCPM_OK;
+    case MCPM_TUNE_ROUND_UNROLL:
+      if (v != 0 && v != 2 && v != 4) return MCPM_ERR_ARG;
+      c->round_unroll = (int)v;
+      return MCPM_OK;
     default: return MCPM_ERR_ARG;
   }
 }
@@ -2708,6 +2730,7 @@ int mcpm_get_tuning(mcpm_ctx *c, int32_t knob, int64_t *v) {
     case MCPM_TUNE_SORT_WINDOW: *v = c->w_enabled ? 1 : 0; return MCPM_OK;
     case MCPM_TUNE_SORT_WINDOW_RUNS: *v = c->w_runs; return MCPM_OK;
     case MCPM_TUNE_SORT_WINDOW_REDONE: *v = c->w_redone; return MCPM_OK;
+    case MCPM_TUNE_ROUND_UNROLL: *v = c->round_unroll; return MCPM_OK;
     default: return MCPM_ERR_ARG;
   }
 }

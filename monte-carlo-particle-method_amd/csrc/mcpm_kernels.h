@@ -200,8 +200,10 @@ template <int MODEL>
 __global__ void k_check(int r, int nbands, uint32_t seg, const uint4 *in, const uint4 *bands,
                         uint32_t *tab, uint8_t *cflag, uint32_t tag8, const uint32_t *counters,
                         uint32_t *host_left, const StepDev *sd);
-// PLAIN: no batched replicas and no debug log (their code left out).
-template <typename R, int MODEL, bool PLAIN>
+// PLAIN: no batched replicas and no debug log (their code left out).  U:
+// operations per thread per tile (2 or 4): more of them put more loads in
+// flight per wavefront, fewer leave room for more wavefronts.
+template <typename R, int MODEL, bool PLAIN, int U>
 __global__ void k_round(int r, int nbands, uint32_t seg, R *P, const uint4 *in,
                         const uint4 *bands, uint4 *out, const uint32_t *tab_cur,
                         uint32_t *tab_next, const uint8_t *cflag, uint32_t tag8,

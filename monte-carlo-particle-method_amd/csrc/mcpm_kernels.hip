@@ -1546,7 +1546,7 @@ __global__ __launch_bounds__(256) void k_check(int r, int nbands, uint32_t seg,
   }
 }
 
-template <typename R, int MODEL, bool PLAIN>
+template <typename R, int MODEL, bool PLAIN, int U>
 __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
                                                R *__restrict__ P,
                                                const uint4 *__restrict__ in,
@@ -1597,7 +1597,6 @@ __global__ __launch_bounds__(256) void k_round(int r, int nbands, uint32_t seg,
     tab_next[rec.y] = rec.z;
   }
   // U operations per thread per tile: their loads are issued together
-  constexpr int U = 4;
   const uint32_t tile = blockDim.x * U;
   uint32_t acc_local = 0;
   for (uint32_t base = lb * tile; base < count; base += nbx * tile) {
@@ -2322,6 +2321,12 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
   }
 }
 
+#define MCPM_INST_ROUND(R, M, PL, U)                                                       \
+  template __global__ void k_round<R, M, PL, U>(                                           \
+      int, int, uint32_t, R *, const uint4 *, const uint4 *, uint4 *, const uint32_t *,    \
+      uint32_t *, const uint8_t *, uint32_t, uint32_t *, const double *, const double *,   \
+      const double *, double *, KParams, uint32_t, uint32_t, FPParams, OpLog,              \
+      const StepDev *);
 #define MCPM_INST_COLL(R, M)                                                               \
   template __global__ void k_gen<R, M>(const R *, const R::real_type *, const uint32_t *,  \
                                        const uint32_t *,                                   \
@@ -2329,20 +2334,10 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                        uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
                                        const uint32_t *, int32_t,                          \
                                        uint4 *, uint32_t *, uint32_t *, const StepDev *);  \
-  template __global__ void k_round<R, M, false>(int, int, uint32_t, R *, const uint4 *,           \
-                                         const uint4 *, uint4 *, const uint32_t *,         \
-                                         uint32_t *, const uint8_t *, uint32_t,            \
-                                         uint32_t *,                                       \
-                                         const double *, const double *, const double *,   \
-                                         double *, KParams, uint32_t, uint32_t, FPParams,  \
-                                         OpLog, const StepDev *); \
-  template __global__ void k_round<R, M, true>(int, int, uint32_t, R *, const uint4 *,           \
-                                         const uint4 *, uint4 *, const uint32_t *,         \
-                                         uint32_t *, const uint8_t *, uint32_t,            \
-                                         uint32_t *,                                       \
-                                         const double *, const double *, const double *,   \
-                                         double *, KParams, uint32_t, uint32_t, FPParams,  \
-                                         OpLog, const StepDev *);
+  MCPM_INST_ROUND(R, M, false, 2)                                                          \
+  MCPM_INST_ROUND(R, M, false, 4)                                                          \
+  MCPM_INST_ROUND(R, M, true, 2)                                                           \
+  MCPM_INST_ROUND(R, M, true, 4)
 
 template __global__ void k_check<COLL_ESMC>(int, int, uint32_t, const uint4 *, const uint4 *,
                                              uint32_t *, uint8_t *, uint32_t, const uint32_t *,

@@ -385,6 +385,7 @@ TIMER_COLLIDE, TIMER_SORT, TIMER_GEN, TIMER_FIELD, TIMER_STEP = 0, 1, 2, 3, 4
 TIMER_FP_VELOCITY, TIMER_FP_CLOSURES = 5, 6
 TUNE_BANDS, TUNE_ROUND_GRID, TUNE_SORT_CHUNK = 0, 1, 2
 TUNE_SORT_WINDOW, TUNE_SORT_WINDOW_RUNS, TUNE_SORT_WINDOW_REDONE = 3, 4, 5
+TUNE_ROUND_UNROLL = 6
 
 
 class Simulation:

@@ -4,8 +4,8 @@ Candidates without a partner (the point r_i + sigma dir beyond a wall,
 collisions.cpp:668-671, or an empty partner cell) stay in the band lists as
 empty slots; every band's admission must skip them.  A walled slab makes
 them common (about one candidate in ten).  Pairs, decisions and velocities
-must not depend on the number of bands or the round grid: compared bit for
-bit with the oracle."""
+must not depend on the number of bands, the round grid or the operations per
+thread of a round: compared bit for bit with the oracle."""
 import numpy as np
 import pytest
 
@@ -16,9 +16,10 @@ import workloads as W
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("bands,grid", [(1, 8), (3, 64), (12, 2048), (16, 1024)])
-def test_esmc_bands_with_partnerless_candidates(bands, grid):
-    print(f"bands={bands} grid={grid}", flush=True)
+@pytest.mark.parametrize("bands,grid,unroll", [(1, 8, 4), (3, 64, 2), (12, 2048, 2),
+                                               (16, 1024, 4), (12, 1024, 0)])
+def test_esmc_bands_with_partnerless_candidates(bands, grid, unroll):
+    print(f"bands={bands} grid={grid} unroll={unroll}", flush=True)
     p, arrs, meta = W.heat_conduction(n_cells=100, ppc=30, seed=71)
     g = mcpm.Simulation(p)
     o = orc.OracleSim(p)
@@ -27,7 +28,9 @@ def test_esmc_bands_with_partnerless_candidates(bands, grid):
         s.cell_update()
     g.set_tuning(mcpm.TUNE_BANDS, bands)
     g.set_tuning(mcpm.TUNE_ROUND_GRID, grid)
+    g.set_tuning(mcpm.TUNE_ROUND_UNROLL, unroll)
     assert g.get_tuning(mcpm.TUNE_BANDS) == bands
+    assert g.get_tuning(mcpm.TUNE_ROUND_UNROLL) == unroll
     o.init_omega_max(meta["n_gas"])
     g.set_omega_max(o.cells()["omega_max"])
     g.debug_collision_log(True)
@@ -50,6 +53,6 @@ def test_tuning_rejects_bad_values():
     p, arrs, meta = W.heat_conduction(n_cells=20, ppc=10, seed=72)
     g = mcpm.Simulation(p)
     for knob, bad in ((mcpm.TUNE_BANDS, 0), (mcpm.TUNE_BANDS, 17), (mcpm.TUNE_ROUND_GRID, 12),
-                      (mcpm.TUNE_SORT_CHUNK, 1000), (99, 1)):
+                      (mcpm.TUNE_SORT_CHUNK, 1000), (mcpm.TUNE_ROUND_UNROLL, 3), (99, 1)):
         with pytest.raises(mcpm.McpmError):
             g.set_tuning(knob, bad)

@@ -42,12 +42,16 @@ def _sum(stats, keys=("candidates", "valid", "accepted")):
     return {k: sum(s[k] for s in stats) for k in keys}
 
 
-@pytest.mark.parametrize("precision", [mcpm.FP32, mcpm.FP64])
-def test_graph_steps_equal_eager_steps(precision):
+@pytest.mark.parametrize("precision,eager_unroll,graph_unroll",
+                         [(mcpm.FP32, 4, 2), (mcpm.FP64, 2, 4)])
+def test_graph_steps_equal_eager_steps(precision, eager_unroll, graph_unroll):
+    """Also across the operations per thread of a round (2 or 4)."""
     p, arrs, meta = W.interface(N=300_000, n_cells=512, length=100.0, precision=precision,
                                 seed=71)
-    _say(f"graph interface N=300000 nc=512 prec={precision}")
+    _say(f"graph interface N=300000 nc=512 prec={precision} unroll {eager_unroll}/{graph_unroll}")
     e, g = _pair(p, arrs, meta["n_gas"])
+    e.set_tuning(mcpm.TUNE_ROUND_UNROLL, eager_unroll)
+    g.set_tuning(mcpm.TUNE_ROUND_UNROLL, graph_unroll)
     eager = [e.step(s) for s in range(9)]
     g.graph_capture()
     parts = [g.graph_steps(0, 3)]
