@@ -1228,8 +1228,11 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
   const uint32_t x0 = op_region(lo, M);
   uint32_t c = c0;  // this thread's cell, walking forward with its candidates
   uint32_t nvalid = 0;
-  // U candidates per thread at a time: their position loads are issued together
+  // U candidates per thread at a time: their position loads are issued
+  // together; the passes of 256 U candidates tile the chunk exactly
   constexpr int U = 4;
+  static_assert(GEN_CHUNK % (256 * U) == 0, "candidate passes must tile the chunk");
+  const uint32_t kend = lo + GEN_CHUNK < M ? lo + GEN_CHUNK : M;
   for (uint32_t b0 = 0; b0 < GEN_CHUNK; b0 += blockDim.x * U) {
     uint32_t kk[U], cc[U], tt[U], ii[U];
     bool live[U], drawn[U];
@@ -1240,13 +1243,15 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
     for (int u = 0; u < U; ++u) {
       const uint32_t k = lo + b0 + u * blockDim.x + threadIdx.x;
       kk[u] = k;
-      live[u] = k < M;
+      live[u] = k < kend;
       drawn[u] = false;
       ii[u] = 0;
       tt[u] = 0;
       cc[u] = c;
       if (!live[u]) continue;
-      while (cs(c + 1) <= k) ++c;  // empty cells have cstart[c] == cstart[c + 1]
+      // empty cells have cstart[c] == cstart[c + 1]; the chunk's candidates
+      // lie in its cells c0..c1
+      while (c < c1 && cs(c + 1) <= k) ++c;
       cc[u] = c;
       tt[u] = k - cs(c);
       const uint32_t Nc = ccnt((int32_t)c);
