@@ -501,6 +501,17 @@ __global__ __launch_bounds__(256) void k_wpush(
   if (SCATTER && *(volatile const uint32_t *)wflag != 0u) return;
   uint32_t *cnt = lcnt[wave];
   const uint32_t beg = wbeg[b], end = wbeg[b + 1], ntot = wbeg[nch];
+  // the first tiles' loads go out with the window's (every chunk holds at
+  // least one particle: a block without particles has no pieces)
+  constexpr int U = 4;  // tiles of 64 in flight
+  R nx[U];
+  if (beg < end) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const uint32_t p = beg + u * 64 + lane;
+      nx[u] = in[p < end ? p : end - 1];
+    }
+  }
   // the chunk's replica: its constants, force, cell offset and slot base
   const int32_t q = kp.nrep ? wrep[b] : 0;
   const KParams &kq = kp.nrep ? kp.rkp[q] : kp;
@@ -543,13 +554,6 @@ __global__ __launch_bounds__(256) void k_wpush(
   }
   __builtin_amdgcn_wave_barrier();
   const uint64_t lt = lanemask_lt();
-  constexpr int U = 4;  // tiles of 64 in flight
-  R nx[U];
-#pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const uint32_t p = beg + u * 64 + lane;
-    nx[u] = in[p < end ? p : end - 1];
-  }
   for (uint32_t t0 = beg; t0 < end; t0 += 64 * U) {
     R rr[U];
 #pragma unroll
