@@ -919,7 +919,7 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
         const uint32_t mark = c->round_base & 0xFFu;
         hipLaunchKernelGGL((k_check<MODEL>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
                            (const uint4 *)c->opBand, tabs[r & 1], c->cflag, mark,
-                           (const uint32_t *)c->counters, c->dleft, (const StepDev *)nullptr);
+                           c->counters, c->dleft, (const StepDev *)nullptr);
         // no replicas and no log: the kernel without their code (fewer registers)
         auto round_k = round_kernel<R, MODEL>(c->kp.nrep == 0 && !lg.p, unroll);
         hipLaunchKernelGGL(round_k, dim3(grid), dim3(256), 0, c->s, r, nbands, seg,
@@ -1252,7 +1252,7 @@ int graph_step_launch(mcpm_ctx *c) {
     const uint4 *in = (const uint4 *)((r & 1) ? c->opB : c->opA);
     hipLaunchKernelGGL((k_check<COLL_ESMC>), dim3(grid), dim3(256), 0, c->s, r, nbands, seg, in,
                        (const uint4 *)c->opBand, tabs[r & 1], c->cflag, 0u,
-                       (const uint32_t *)c->counters, (uint32_t *)nullptr, csd);
+                       c->counters, (uint32_t *)nullptr, csd);
     auto round_k = round_kernel<R, COLL_ESMC>(c->kp.nrep == 0, unroll);
     hipLaunchKernelGGL(round_k, dim3(grid), dim3(256), 0, c->s, r, nbands, seg, P,
                        in, (const uint4 *)c->opBand, (r & 1) ? c->opA : c->opB,

@@ -28,7 +28,13 @@ enum : int {
   MAX_BANDS = 16,
   CNT_BAND = CNT_LIST + (MAX_ROUNDS + 2) * NREG,  // [b NREG + x]: operations of band b, region x
   CNT_BSTART = CNT_BAND + MAX_BANDS * NREG,       // start of list [b NREG + x] in the band lists
-  CNT_TOTAL = CNT_BSTART + MAX_BANDS * NREG + 1
+  // the draw's valid-operation count, spread over NSPREAD slots one cache
+  // line apart (one address serializes its atomics: ~30k per draw at 20M
+  // particles took 175 us), folded into CNT_OPS by round 0's k_check
+  NSPREAD = 64,
+  SPREAD_STRIDE = 32,
+  CNT_SPREAD = (CNT_BSTART + MAX_BANDS * NREG + 1 + SPREAD_STRIDE - 1) / SPREAD_STRIDE * SPREAD_STRIDE,
+  CNT_TOTAL = CNT_SPREAD + NSPREAD * SPREAD_STRIDE
 };
 
 // A shared-particle mark of the reservation rounds covers a group of
@@ -198,7 +204,7 @@ __global__ void k_gen(const R *P, const typename R::real_type *ys, const uint32_
 // round's operations.
 template <int MODEL>
 __global__ void k_check(int r, int nbands, uint32_t seg, const uint4 *in, const uint4 *bands,
-                        uint32_t *tab, uint8_t *cflag, uint32_t tag8, const uint32_t *counters,
+                        uint32_t *tab, uint8_t *cflag, uint32_t tag8, uint32_t *counters,
                         uint32_t *host_left, const StepDev *sd);
 // PLAIN: no batched replicas and no debug log (their code left out).  U:
 // operations per thread per tile (2 or 4): more of them put more loads in

@@ -1337,7 +1337,8 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
     }
   }
   for (int off = 32; off > 0; off >>= 1) nvalid += __shfl_down(nvalid, off);
-  if (lane_id() == 0 && nvalid) atomicAdd(&counters[CNT_OPS], nvalid);
+  if (lane_id() == 0 && nvalid)
+    atomicAdd(&counters[CNT_SPREAD + (bx % NSPREAD) * SPREAD_STRIDE], nvalid);
 }
 
 // The collision itself, collisions.cpp:680-716: Y at the midpoint cell,
@@ -1498,11 +1499,18 @@ __global__ __launch_bounds__(256) void k_check(int r, int nbands, uint32_t seg,
                                                const uint4 *__restrict__ bands,
                                                uint32_t *__restrict__ tab,
                                                uint8_t *__restrict__ cflag, uint32_t tag8,
-                                               const uint32_t *__restrict__ counters,
+                                               uint32_t *__restrict__ counters,
                                                uint32_t *__restrict__ host_left,
                                                const StepDev *__restrict__ sd) {
   if (sd_stopped(sd)) return;
   if (sd) tag8 = sd->mark + (uint32_t)r;
+  // round 0: the draw's valid-operation count, from its slots
+  if (r == 0 && blockIdx.x == 0 && threadIdx.x < NSPREAD) {
+    static_assert(NSPREAD == 64, "one wavefront sums the slots");
+    uint32_t v = counters[CNT_SPREAD + threadIdx.x * SPREAD_STRIDE];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    if (threadIdx.x == 0) counters[CNT_OPS] = v;
+  }
   // publish the losers left after round r - 1 (final now) to the host
   if (host_left && blockIdx.x == 0 && threadIdx.x == 0 && r > 0) {
     uint32_t left = 0;
@@ -2349,13 +2357,13 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
   MCPM_INST_ROUND(R, M, true, 4)
 
 template __global__ void k_check<COLL_ESMC>(int, int, uint32_t, const uint4 *, const uint4 *,
-                                             uint32_t *, uint8_t *, uint32_t, const uint32_t *,
+                                             uint32_t *, uint8_t *, uint32_t, uint32_t *,
                                              uint32_t *, const StepDev *);
 template __global__ void k_check<COLL_DSMC>(int, int, uint32_t, const uint4 *, const uint4 *,
-                                             uint32_t *, uint8_t *, uint32_t, const uint32_t *,
+                                             uint32_t *, uint8_t *, uint32_t, uint32_t *,
                                              uint32_t *, const StepDev *);
 template __global__ void k_check<COLL_ESMC4>(int, int, uint32_t, const uint4 *, const uint4 *,
-                                              uint32_t *, uint8_t *, uint32_t, const uint32_t *,
+                                              uint32_t *, uint8_t *, uint32_t, uint32_t *,
                                               uint32_t *, const StepDev *);
 
 #define MCPM_INST_WP(R, S, L)                                                              \
