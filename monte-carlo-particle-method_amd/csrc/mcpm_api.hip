@@ -649,8 +649,11 @@ int wsort_launch(mcpm_ctx *c, int mode, int64_t step, const Kick &kick, const do
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,
                      nullptr, c->err, c->wflag, nullptr, nullptr, c->wdelta, (real *)nullptr,
                      nullptr, nullptr, sd);
-  hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 3) / 4)), dim3(256), 0, c->s, c->wlo,
-                     c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc, sd);
+  // about 32k wavefronts: one cell each on short cell arrays, up to 16
+  const int cpw = std::max(1, std::min(16, c->nc / 32768));
+  hipLaunchKernelGGL(k_wscan, dim3((unsigned)((c->nc + 4 * cpw - 1) / (4 * cpw))),
+                     dim3(256), 0, c->s, c->wlo,
+                     c->wlen, c->wtab, nchp, c->rchunk, c->kp, c->cnt, c->nden, c->yc, cpw, sd);
   if (int st = excl_scan(c, c->cnt, c->start, c->nc, sd)) return st;
   hipLaunchKernelGGL(scatter_k, dim3(gw), dim3(256), 0, c->s, in, out, c->kp, kick,
                      drift, mode, slo, shi, c->wbeg, c->wrep, nchp, c->wlo, c->wlen, c->wtab,

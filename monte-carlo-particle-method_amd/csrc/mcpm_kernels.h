@@ -131,9 +131,11 @@ __global__ void k_wfill(const int32_t *bcell, const int32_t *brep, int nblk,
                         const uint32_t *start, const uint32_t *pbase, const int32_t *rblk,
                         int nrep1, uint32_t n, uint32_t *wbeg, int32_t *wrep, uint32_t *nchp,
                         uint32_t *rchunk, const StepDev *sd);
+// cpw: consecutive cells per wavefront (more than one on long cell arrays,
+// where one wavefront per cell ran generation after generation of waves)
 __global__ void k_wscan(const int32_t *wlo, const int32_t *wlen, uint32_t *wtab,
                         const uint32_t *nchp, const uint32_t *rchunk, KParams kp, uint32_t *cnt,
-                        double *nden, double *yc, const StepDev *sd);
+                        double *nden, double *yc, int cpw, const StepDev *sd);
 // Most bins of one sort pass (one LDS histogram of 64 KB per chunk); more
 // cells sort by two digits of the cell key, up to MAX_CELLS.
 constexpr int SORT_BINS = 16384;

@@ -698,7 +698,17 @@ __device__ __forceinline__ int wave_lower_bound(int a, int z, Pred pred) {
   return z;
 }
 
-// Per cell (one wavefront each): the chunks whose windows hold it, in
+// The same from a lower bound a that is usually close: the 64 indices from a
+// in one load per lane, the search beyond them only when none holds.
+template <typename Pred>
+__device__ __forceinline__ int wave_first_from(int a, int z, Pred pred) {
+  const int i = a + (int)lane_id();
+  const uint64_t m = __ballot(i < z && pred(i));
+  if (m) return a + __ffsll((unsigned long long)m) - 1;
+  return a + 64 < z ? wave_lower_bound(a + 64, z, pred) : z;
+}
+
+// Per cell (cpw consecutive cells per wavefront): the chunks whose windows hold it, in
 // chunk order (a contiguous run for each of the cell's unwrapped numbers
 // k - nc, k, k + nc of a periodic slab, in that order), their counts
 // replaced by the running offset; the cell's count, density and contact
@@ -712,51 +722,70 @@ __global__ __launch_bounds__(256) void k_wscan(const int32_t *__restrict__ wlo,
                                                const uint32_t *__restrict__ rchunk, KParams kp,
                                                uint32_t *__restrict__ cnt,
                                                double *__restrict__ nden,
-                                               double *__restrict__ yc,
+                                               double *__restrict__ yc, int cpw,
                                                const StepDev *__restrict__ sd) {
   if (sd_stopped(sd)) return;
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
-  if (k >= kp.n_cells) return;
-  const int32_t q = kp.nrep ? kp.cell_rep[k] : 0;
-  const KParams &kq = kp.nrep ? kp.rkp[q] : kp;
-  const int32_t off = kp.nrep ? kp.rcell_off[q] : 0;
-  const int32_t nc = kq.n_cells;
-  const int32_t kl = k - off;
-  const int b0 = kp.nrep ? (int)rchunk[q] : 0, b1 = kp.nrep ? (int)rchunk[q + 1] : (int)*nchp;
-  const bool periodic = kq.boundary == MCPM_BC_PERIODIC;
-  uint32_t run = 0;
-  for (int im = periodic ? -1 : 0; im <= (periodic ? 1 : 0); ++im) {
-    // every window lies in [-WDELTA_MAX, nc + WDELTA_MAX): the images
-    // below 0 and past nc - 1 hold chunks only for cells near the ends
-    if ((im < 0 && kl < nc - WMAX) || (im > 0 && kl >= WMAX)) continue;
-    const int32_t u = kl + im * nc;
-    // first chunk whose window ends past u, first whose window starts past u
-    const int blo = wave_lower_bound(b0, b1, [&](int m) { return wlo[m] + wlen[m] > u; });
-    const int bhi = wave_lower_bound(blo, b1, [&](int m) { return wlo[m] > u; });
-    for (int bb = blo; bb < bhi; bb += 64) {
-      const int b = bb + lane;
-      uint32_t v = 0;
-      int32_t w = -1;
-      if (b < bhi) {
-        w = u - wlo[b];
-        if (w >= 0 && w < wlen[b]) v = wtab[(size_t)b * WMAX + w];
-        else w = -1;
+  const int k0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * cpw;
+  const int k1 = k0 + cpw < kp.n_cells ? k0 + cpw : kp.n_cells;
+  // the previous cell's replica and chunk bounds (image 0): both bounds only
+  // move forward from one cell to the next of a replica
+  int32_t pq = -1;
+  int pblo = 0, pbhi = 0;
+  for (int k = k0; k < k1; ++k) {
+    const int32_t q = kp.nrep ? kp.cell_rep[k] : 0;
+    const KParams &kq = kp.nrep ? kp.rkp[q] : kp;
+    const int32_t off = kp.nrep ? kp.rcell_off[q] : 0;
+    const int32_t nc = kq.n_cells;
+    const int32_t kl = k - off;
+    const int b0 = kp.nrep ? (int)rchunk[q] : 0, b1 = kp.nrep ? (int)rchunk[q + 1] : (int)*nchp;
+    const bool periodic = kq.boundary == MCPM_BC_PERIODIC;
+    uint32_t run = 0;
+    for (int im = periodic ? -1 : 0; im <= (periodic ? 1 : 0); ++im) {
+      // every window lies in [-WDELTA_MAX, nc + WDELTA_MAX): the images
+      // below 0 and past nc - 1 hold chunks only for cells near the ends
+      if ((im < 0 && kl < nc - WMAX) || (im > 0 && kl >= WMAX)) continue;
+      const int32_t u = kl + im * nc;
+      // first chunk whose window ends past u, first whose window starts past u
+      auto ends_past = [&](int m) { return wlo[m] + wlen[m] > u; };
+      auto starts_past = [&](int m) { return wlo[m] > u; };
+      int blo, bhi;
+      if (im == 0 && q == pq) {
+        blo = wave_first_from(pblo, b1, ends_past);
+        bhi = wave_first_from(pbhi > blo ? pbhi : blo, b1, starts_past);
+      } else {
+        blo = wave_lower_bound(b0, b1, ends_past);
+        bhi = wave_lower_bound(blo, b1, starts_past);
       }
-      uint32_t x = v;  // inclusive scan over the wavefront
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
+      if (im == 0) {
+        pq = q;
+        pblo = blo;
+        pbhi = bhi;
       }
-      if (w >= 0) wtab[(size_t)b * WMAX + w] = run + x - v;
-      run += __shfl(x, 63);
+      for (int bb = blo; bb < bhi; bb += 64) {
+        const int b = bb + lane;
+        uint32_t v = 0;
+        int32_t w = -1;
+        if (b < bhi) {
+          w = u - wlo[b];
+          if (w >= 0 && w < wlen[b]) v = wtab[(size_t)b * WMAX + w];
+          else w = -1;
+        }
+        uint32_t x = v;  // inclusive scan over the wavefront
+        for (int o = 1; o < 64; o <<= 1) {
+          const uint32_t y = __shfl_up(x, o);
+          if (lane >= o) x += y;
+        }
+        if (w >= 0) wtab[(size_t)b * WMAX + w] = run + x - v;
+        run += __shfl(x, 63);
+      }
     }
-  }
-  if (lane == 0) {
-    cnt[k] = run;
-    const double nd = cell_weight(kq, k) * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
-    nden[k] = nd;
-    yc[k] = contact_y(nd, kq.b, kq.y_model);
+    if (lane == 0) {
+      cnt[k] = run;
+      const double nd = cell_weight(kq, k) * kq.Fn * (double)run / kq.volume;  // Main_Functions.cpp:2601
+      nden[k] = nd;
+      yc[k] = contact_y(nd, kq.b, kq.y_model);
+    }
   }
 }
 
