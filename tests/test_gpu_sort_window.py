@@ -1,8 +1,9 @@
 """The windowed sort (per-chunk histograms over a few cells) against the
 full-histogram sort and the oracle: the same stable order bit for bit, on a
 walled slab, a periodic slab (windows wrap), batched replicas (chunks never
-span two replicas) and when a particle leaves its window (the sort is redone
-by the full histogram)."""
+span two replicas; also past 65536 cells, where a wavefront scans several
+cells) and when a particle leaves its window (the sort is redone by the full
+histogram)."""
 import numpy as np
 import pytest
 
@@ -78,6 +79,35 @@ def test_windowed_sort_on_batched_replicas():
         b = mcpm.Simulation.batch(common, reps)
         b.set_tuning(mcpm.TUNE_SORT_WINDOW, window)
         counts = [a[0].size for a in arrs]
+        b.upload_batch(counts, *[np.concatenate([a[q] for a in arrs]) for q in range(4)])
+        b.cell_update()
+        b.init_omega_max_batch([m["n_gas"] for m in metas])
+        bs.append(b)
+    for s in range(3):
+        assert bs[0].step(s)["accepted"] == bs[1].step(s)["accepted"]
+    for a, c in zip(bs[0].download(), bs[1].download()):
+        np.testing.assert_array_equal(a, c)
+    for k in ("count", "start", "n", "F2"):
+        np.testing.assert_array_equal(bs[0].cells()[k], bs[1].cells()[k])
+    runs, redone = _runs(bs[0])
+    assert runs >= 3 and redone == 0, (runs, redone)
+
+
+def test_windowed_sort_on_more_than_65536_cells():
+    """192 replicas, about 73k cells: the per-cell scan of the windowed sort
+    then takes several consecutive cells per wavefront (each cell's chunk
+    bounds searched from the previous cell's, across replica boundaries too);
+    the same order and cells as the full-histogram sort, bit for bit."""
+    common, reps, arrs, metas = W.replicas(n_rep=192, seed0=40, N_lo=7000, N_hi=9000,
+                                           cells_lo=360, cells_hi=400, length=60.0)
+    assert sum(r["n_cells"] for r in reps) > 65536
+    counts = [a[0].size for a in arrs]
+    print(f"windowed sort R=192 N={sum(counts)} cells={sum(r['n_cells'] for r in reps)}",
+          flush=True)
+    bs = []
+    for window in (1, 0):
+        b = mcpm.Simulation.batch(common, reps)
+        b.set_tuning(mcpm.TUNE_SORT_WINDOW, window)
         b.upload_batch(counts, *[np.concatenate([a[q] for a in arrs]) for q in range(4)])
         b.cell_update()
         b.init_omega_max_batch([m["n_gas"] for m in metas])
