@@ -111,6 +111,7 @@ struct mcpm_ctx {
   uint4 *opBand = nullptr;               // then the priority-band lists
   uint32_t *gen_table = nullptr;         // per-chunk list sizes of the candidate draw
   uint32_t *gen_cells = nullptr;         // each chunk's first and last cell (in gen_table)
+  uint32_t gen_tstride = 0;              // gen_table[q gen_tstride + chunk]: list q's counts
   int32_t gen_reach = -1;                // cells a partner can lie from its candidate's (-1: any)
   int nbands = 12;
   // round termination: round r + 1 writes the losers left after round r into
@@ -814,8 +815,10 @@ int ensure_ops(mcpm_ctx *c, int64_t m) {
   if (c->gen_table) HIPCHK(hipFree(c->gen_table));
   c->gen_table = nullptr;
   const size_t nch_cap = (size_t)cap / GEN_CHUNK + 2;
-  if ((st = dalloc(&c->gen_table, nch_cap * (NLIST + 2)))) return st;
-  c->gen_cells = c->gen_table + nch_cap * NLIST;
+  const size_t tstride = (nch_cap + 7) / 8 * 8;
+  if ((st = dalloc(&c->gen_table, tstride * NLIST + 2 * nch_cap))) return st;
+  c->gen_cells = c->gen_table + tstride * NLIST;
+  c->gen_tstride = (uint32_t)tstride;
   uint32_t *lg = nullptr;
   st = dalloc(&lg, (size_t)cap * 3 + ((size_t)cap + 3) / 4);
   if (st) return st;
@@ -890,18 +893,19 @@ int collide_t(mcpm_ctx *c, int64_t step, mcpm_collision_stats *stats, int varian
     Timer tm(c, T_GEN);
     const uint32_t nchunks = (M + GEN_CHUNK - 1) / GEN_CHUNK;
     hipLaunchKernelGGL(k_cand_count, dim3(nchunks), dim3(256), 0, c->s, M, pk.v[0], (int)MODEL,
-                       band_mul, nbands, c->gen_table, c->kp, c->cstart, nc, c->gen_cells,
+                       band_mul, nbands, c->gen_table, c->gen_tstride, c->kp, c->cstart, nc,
+                       c->gen_cells,
                        (const StepDev *)nullptr);
     hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
-                       nchunks, nbands * NREG, c->counters, (const uint32_t *)nullptr,
-                       (const StepDev *)nullptr);
-    hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
+                       c->gen_tstride, nchunks, nbands * NREG, c->counters,
+                       (const uint32_t *)nullptr, (const StepDev *)nullptr);
     const typename R::real_type *ys =
         c->ys_valid ? (const typename R::real_type *)c->ys : nullptr;
     hipLaunchKernelGGL((k_gen<R, MODEL>), dim3(nchunks), dim3(256), 0, c->s, P, ys, c->cnt,
                        c->start,
                        c->cstart, nc, M, c->kp, slo, shi, pk.v[0], band_mul, nbands,
-                       c->gen_table, (const uint32_t *)c->gen_cells, c->gen_reach, c->opBand,
+                       c->gen_table, c->gen_tstride, (const uint32_t *)c->gen_cells, c->gen_reach,
+                       c->opBand,
                        tabs[0], c->counters, (const StepDev *)nullptr);
     LAUNCHCHK();
   }
@@ -1237,14 +1241,16 @@ int graph_step_launch(mcpm_ctx *c) {
   const unsigned nch_cap = (unsigned)((cap + GEN_CHUNK - 1) / GEN_CHUNK);
   uint32_t *const tabs[2] = {c->rtab, c->rtab + (size_t)c->p.capacity};
   hipLaunchKernelGGL(k_cand_count, dim3(nch_cap), dim3(256), 0, c->s, 0u, 0u, (int)COLL_ESMC,
-                     band_mul, nbands, c->gen_table, c->kp, c->cstart, nc, c->gen_cells, csd);
-  hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table, 0u,
-                     nbands * NREG, c->counters, (const uint32_t *)c->cstart + nc, csd);
-  hipLaunchKernelGGL(k_list_starts, dim3(1), dim3(64), 0, c->s, c->counters, nbands * NREG);
+                     band_mul, nbands, c->gen_table, c->gen_tstride, c->kp, c->cstart, nc,
+                     c->gen_cells, csd);
+  hipLaunchKernelGGL(k_cand_scan, dim3(nbands * NREG), dim3(1024), 0, c->s, c->gen_table,
+                     c->gen_tstride, 0u, nbands * NREG, c->counters,
+                     (const uint32_t *)c->cstart + nc, csd);
   using real = typename R::real_type;
   hipLaunchKernelGGL((k_gen<R, COLL_ESMC>), dim3(nch_cap), dim3(256), 0, c->s, P,
                      (const real *)c->ys, c->cnt, c->start, c->cstart, nc, 0u, c->kp, 0u, 0u, 0u,
-                     band_mul, nbands, c->gen_table, (const uint32_t *)c->gen_cells, c->gen_reach,
+                     band_mul, nbands, c->gen_table, c->gen_tstride, (const uint32_t *)c->gen_cells,
+                     c->gen_reach,
                      c->opBand, tabs[0], c->counters, csd);
   LAUNCHCHK();
   const uint32_t seg = (cap + NREG - 1) / NREG;

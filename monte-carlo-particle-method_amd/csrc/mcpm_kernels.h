@@ -22,6 +22,7 @@ enum : int {
   CNT_OPS = 0,     // valid candidate operations of the current step
   CNT_ACC = 2,     // accepted collisions (u64, 8-byte aligned)
   CNT_LOG = 4,     // entries in the debug collision log
+  CNT_SCANNED = 6, // candidate lists whose chunk counts k_cand_scan has scanned
   NREG = 8,        // operation regions: contiguous candidate ranges, one per XCD
   CNT_LIST = 8,    // CNT_LIST + r NREG + x: losers of round r - 1 in region x (r >= 1)
   MAX_ROUNDS = 4096,
@@ -184,22 +185,24 @@ __device__ __forceinline__ uint32_t band_new4(uint32_t rank, uint32_t M, int nba
 // only on its index, so the lists are sized before anything is drawn.
 constexpr uint32_t GEN_CHUNK = 2048;
 constexpr int NLIST = MAX_BANDS * NREG;
+// table[q tstride + chunk]: the chunk's candidates in list q, then (after
+// k_cand_scan) their offset in the list.
 __global__ void k_cand_count(uint32_t M, uint32_t prio_key, int model, uint32_t band_mul,
-                             int nbands, uint32_t *table, KParams kp, const uint32_t *cstart,
-                             int nc, uint32_t *gcells, const StepDev *sd);
+                             int nbands, uint32_t *table, uint32_t tstride, KParams kp,
+                             const uint32_t *cstart, int nc, uint32_t *gcells, const StepDev *sd);
 // cells of one candidate chunk whose candidate starts the draw keeps in LDS,
 // and partner cells whose counts and starts it keeps there
 constexpr int GEN_CELLS = 512;
 constexpr int GEN_WIN = 1024;
-__global__ void k_cand_scan(uint32_t *table, uint32_t nchunks, int nl, uint32_t *counters,
-                            const uint32_t *Mp, const StepDev *sd);
-__global__ void k_list_starts(uint32_t *counters, int nl);
+__global__ void k_cand_scan(uint32_t *table, uint32_t tstride, uint32_t nchunks, int nl,
+                            uint32_t *counters, const uint32_t *Mp, const StepDev *sd);
 template <typename R, int MODEL>
 __global__ void k_gen(const R *P, const typename R::real_type *ys, const uint32_t *cnt,
                       const uint32_t *start,
                       const uint32_t *cstart, int nc, uint32_t M, KParams kp, uint32_t step_lo,
                       uint32_t step_hi, uint32_t prio_key, uint32_t band_mul, int nbands,
-                      const uint32_t *table, const uint32_t *gcells, int32_t reach,
+                      const uint32_t *table, uint32_t tstride, const uint32_t *gcells,
+                      int32_t reach,
                       uint4 *bands, uint32_t *tab0, uint32_t *counters, const StepDev *sd);
 // A reservation round: k_check resolves the particles that several of the
 // round's operations share, k_round commits the winners and admits the next

@@ -1083,7 +1083,8 @@ __device__ __forceinline__ uint32_t first_cell_of(uint32_t lo, const uint32_t *c
 
 __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_key, int model,
                                                     uint32_t band_mul, int nbands,
-                                                    uint32_t *__restrict__ table, KParams kp,
+                                                    uint32_t *__restrict__ table,
+                                                    uint32_t tstride, KParams kp,
                                                     const uint32_t *__restrict__ cstart,
                                                     int nc, uint32_t *__restrict__ gcells,
                                                     const StepDev *__restrict__ sd) {
@@ -1135,54 +1136,86 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
     atomicAdd(&scount[q], 1u);
   }
   __syncthreads();
-  if ((int)threadIdx.x < nl) table[(size_t)blockIdx.x * NLIST + threadIdx.x] = scount[threadIdx.x];
+  if ((int)threadIdx.x < nl) table[(size_t)threadIdx.x * tstride + blockIdx.x] = scount[threadIdx.x];
 }
 
-// Exclusive scan of list q's chunk counts (one workgroup per list); the
-// list's total goes to the band counters.
+// Exclusive scan of list q's chunk counts (one workgroup per list, the
+// list's counts contiguous: table[q tstride + chunk], eight per thread per
+// pass); the list's total goes to the band counters, and the last list to
+// finish writes where every (band, region) list starts.
 __global__ __launch_bounds__(1024) void k_cand_scan(uint32_t *__restrict__ table,
-                                                    uint32_t nchunks, int nl,
+                                                    uint32_t tstride, uint32_t nchunks, int nl,
                                                     uint32_t *__restrict__ counters,
                                                     const uint32_t *__restrict__ Mp,
                                                     const StepDev *__restrict__ sd) {
+  constexpr int PER = 8;
   __shared__ uint32_t sw[16];
-  __shared__ uint32_t carry;
+  __shared__ uint32_t s_last;
   const int q = blockIdx.x;
   if (q >= nl || sd_stopped(sd)) return;
   if (Mp) nchunks = (*Mp + GEN_CHUNK - 1) / GEN_CHUNK;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint32_t b0 = 0; b0 < nchunks; b0 += blockDim.x) {
-    const uint32_t ch = b0 + threadIdx.x;
-    const uint32_t v = ch < nchunks ? table[(size_t)ch * NLIST + q] : 0u;
-    // inclusive scan in the wavefront
-    uint32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      uint32_t y = __shfl_up(x, o);
-      if ((int)lane_id() >= o) x += y;
+  uint32_t *row = table + (size_t)q * tstride;
+  const int lane = (int)lane_id(), wave = (int)(threadIdx.x >> 6);
+  const int nw = (int)(blockDim.x >> 6);
+  uint32_t carry = 0;
+  for (uint32_t b0 = 0; b0 < nchunks; b0 += blockDim.x * PER) {
+    const uint32_t e0 = b0 + threadIdx.x * PER;
+    uint32_t v[PER];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      v[i] = e0 + i < nchunks ? row[e0 + i] : 0u;
+      sum += v[i];
     }
-    if (lane_id() == 63) sw[threadIdx.x >> 6] = x;
+    uint32_t x = sum;  // inclusive scan of the threads' sums in the wavefront
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) sw[wave] = x;
     __syncthreads();
-    uint32_t wbase = 0;
-    for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) wbase += sw[w];
-    const uint32_t base = carry;
-    if (ch < nchunks) table[(size_t)ch * NLIST + q] = base + wbase + x - v;
-    __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) carry = base + wbase + x;
-    __syncthreads();
+    uint32_t wbase = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+      if (w < wave) wbase += sw[w];
+      tot += sw[w];
+    }
+    uint32_t run = carry + wbase + x - sum;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+      if (e0 + i < nchunks) row[e0 + i] = run;
+      run += v[i];
+    }
+    carry += tot;
+    __syncthreads();  // sw is written again by the next pass
   }
-  if (threadIdx.x == 0) counters[CNT_BAND + q] = carry;
-}
-
-// Exclusive prefix of the list totals: where each (band, region) list starts.
-__global__ void k_list_starts(uint32_t *__restrict__ counters, int nl) {
-  if (threadIdx.x != 0) return;
-  uint32_t run = 0;
-  for (int q = 0; q < nl; ++q) {
-    counters[CNT_BSTART + q] = run;
-    run += counters[CNT_BAND + q];
+  if (threadIdx.x == 0) {
+    counters[CNT_BAND + q] = carry;
+    __threadfence();
+    s_last = atomicAdd(&counters[CNT_SCANNED], 1u) == (uint32_t)nl - 1u;
   }
-  counters[CNT_BSTART + nl] = run;
+  __syncthreads();
+  // the last list: every list's total is in; their exclusive prefix, two
+  // lists per lane (nl <= NLIST = 128)
+  if (s_last && threadIdx.x < 64) {
+    __threadfence();
+    const int q0 = 2 * lane, q1 = 2 * lane + 1;
+    const uint32_t t0 = q0 < nl ? __hip_atomic_load(&counters[CNT_BAND + q0], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT)
+                                : 0u;
+    const uint32_t t1 = q1 < nl ? __hip_atomic_load(&counters[CNT_BAND + q1], __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT)
+                                : 0u;
+    const uint32_t pair = t0 + t1;
+    uint32_t x = pair;
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o);
+      if (lane >= o) x += y;
+    }
+    const uint32_t before = x - pair;
+    if (q0 < nl) counters[CNT_BSTART + q0] = before;
+    if (q1 < nl) counters[CNT_BSTART + q1] = before + t0;
+    if (lane == 63) counters[CNT_BSTART + nl] = x;
+  }
 }
 
 // Candidate generation, one chunk of GEN_CHUNK candidate indices per
@@ -1201,6 +1234,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
                                              uint32_t step_hi, uint32_t prio_key,
                                              uint32_t band_mul, int nbands,
                                              const uint32_t *__restrict__ table,
+                                             uint32_t tstride,
                                              const uint32_t *__restrict__ gcells, int32_t reach,
                                              uint4 *__restrict__ bands,
                                              uint32_t *__restrict__ tab0,
@@ -1239,7 +1273,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
   const bool lw = reach >= 0 && wn <= GEN_WIN;
   if ((int)threadIdx.x < nl)  // list start + this chunk's offset inside the list
     spos[threadIdx.x] = counters[CNT_BSTART + threadIdx.x] +
-                        table[(size_t)bx * NLIST + threadIdx.x];
+                        table[(size_t)threadIdx.x * tstride + bx];
   if (lcs)
     for (int q = threadIdx.x; q < ncs; q += blockDim.x) s_cs[q] = cstart[c0 + q];
   if (lw)
@@ -2378,7 +2412,7 @@ __global__ __launch_bounds__(64) void k_cba_hs(R *__restrict__ P,
                                        const uint32_t *,                                   \
                                        const uint32_t *, int, uint32_t, KParams, uint32_t, \
                                        uint32_t, uint32_t, uint32_t, int, const uint32_t *,\
-                                       const uint32_t *, int32_t,                          \
+                                       uint32_t, const uint32_t *, int32_t,                \
                                        uint4 *, uint32_t *, uint32_t *, const StepDev *);  \
   MCPM_INST_ROUND(R, M, false, 2)                                                          \
   MCPM_INST_ROUND(R, M, false, 4)                                                          \
