@@ -1070,17 +1070,6 @@ __device__ __forceinline__ uint32_t cand_priority_rep(const KParams &kp, int32_t
 }
 
 // Size of every (band, region) list per chunk of candidates.
-// First cell of the candidates of chunk lo: the last c with cstart[c] <= lo.
-__device__ __forceinline__ uint32_t first_cell_of(uint32_t lo, const uint32_t *cstart, int nc) {
-  int a = 0, b = nc;  // cstart[a] <= lo < cstart[b] (cstart[nc] = M > lo)
-  while (b - a > 1) {
-    const int mid = (a + b) >> 1;
-    if (cstart[mid] <= lo) a = mid;
-    else b = mid;
-  }
-  return (uint32_t)a;
-}
-
 __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_key, int model,
                                                     uint32_t band_mul, int nbands,
                                                     uint32_t *__restrict__ table,
@@ -1099,15 +1088,23 @@ __global__ __launch_bounds__(256) void k_cand_count(uint32_t M, uint32_t prio_ke
   }
   const int nl = nbands * NREG;
   if ((int)threadIdx.x < NLIST) scount[threadIdx.x] = 0;
+  __syncthreads();  // the counts are zero before any thread adds to them
   const uint32_t lo = blockIdx.x * GEN_CHUNK;
-  // the chunk's first and last cell, for the draw
-  if (threadIdx.x == 0) {
-    scell = first_cell_of(lo, cstart, nc);
-    const uint32_t hi = lo + GEN_CHUNK < M ? lo + GEN_CHUNK : M;
-    gcells[2 * blockIdx.x] = scell;
-    gcells[2 * blockIdx.x + 1] = first_cell_of(hi - 1u, cstart, nc);
+  // the chunk's first and last cell, for the draw: the last c with
+  // cstart[c] <= lo (resp. hi - 1), searched by wavefront 0 over cstart[0..nc]
+  // (cstart[0] = 0 <= lo < M = cstart[nc])
+  if (threadIdx.x < 64) {
+    const uint32_t last = (lo + GEN_CHUNK < M ? lo + GEN_CHUNK : M) - 1u;
+    const int a = wave_lower_bound(1, nc + 1, [&](int cc) { return cstart[cc] > lo; }) - 1;
+    const int b = wave_lower_bound(a + 1, nc + 1, [&](int cc) { return cstart[cc] > last; }) - 1;
+    if (threadIdx.x == 0) {
+      scell = (uint32_t)a;
+      gcells[2 * blockIdx.x] = (uint32_t)a;
+      gcells[2 * blockIdx.x + 1] = (uint32_t)b;
+    }
   }
-  __syncthreads();
+  // only new4's ranks and the replicas' priorities need the first cell here
+  if (model == COLL_ESMC4 || kp.nrep) __syncthreads();
   const int esmc = model != COLL_DSMC;
   const uint32_t x0 = op_region(lo, M);
   uint32_t c = model == COLL_ESMC4 ? scell : 0u;
