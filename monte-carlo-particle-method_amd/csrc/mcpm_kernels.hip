@@ -1261,6 +1261,11 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
   // and the cells its partners can lie in, [wlo, wlo + wn) (their counts
   // and starts in LDS when they fit); lookups outside fall back to memory
   const uint32_t c0 = gcells[2 * bx], c1 = gcells[2 * bx + 1];
+  // a chunk inside one replica (the usual case): that replica's number for
+  // every candidate, instead of a lookup per candidate
+  const int32_t rq0 = kp.nrep ? kp.cell_rep[c0] : 0;
+  const bool one_rep = kp.nrep && kp.cell_rep[c1] == rq0;
+  auto rep_of = [&](uint32_t cc) { return one_rep ? rq0 : kp.cell_rep[cc]; };
   const int ncs = (int)(c1 - c0) + 2;
   const bool lcs = ncs <= GEN_CELLS + 1;
   int32_t wlo = (int32_t)c0 - reach, whi = (int32_t)c1 + reach + 1;
@@ -1322,8 +1327,9 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
       if (MODEL == COLL_DSMC || Nc == 0) continue;
       // the candidate's draws: i in c, the direction, j's uniform
       // (collisions.cpp:648-679); a replica's own key and local cell
-      const KParams &kq = kp.nrep ? kp.rkp[kp.cell_rep[c]] : kp;
-      const uint32_t cl = kp.nrep ? c - (uint32_t)kp.rcell_off[kp.cell_rep[c]] : c;
+      const int32_t rqc = kp.nrep ? rep_of(c) : 0;
+      const KParams &kq = kp.nrep ? kp.rkp[rqc] : kp;
+      const uint32_t cl = kp.nrep ? c - (uint32_t)kp.rcell_off[rqc] : c;
       const mcpm_u32x4 w =
           mcpm_philox(tt[u], cl, step_lo, mcpm_ctr3(MCPM_TAG_CAND, step_hi, 0u), kq.key0, kq.key1);
       ii[u] = cstr((int32_t)c) + mcpm_uint_below(w.v[0], Nc);
@@ -1361,7 +1367,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
             inside = partner_cell(ri[u], d0[u], d1[u], kp, &J);
           }
         } else {
-          const int32_t rq = kp.cell_rep[cu];
+          const int32_t rq = rep_of(cu);
           inside = partner_cell(ri[u], d0[u], d1[u], kp.rkp[rq], &J);
           J += kp.rcell_off[rq];
         }
@@ -1378,7 +1384,7 @@ __global__ __launch_bounds__(256) void k_gen(const R *__restrict__ P,
         pr = (M - cs(cu + 1)) + t;  // rank_new4
         band = band_new4(pr, M, nbands);
       } else {
-        pr = kp.nrep ? cand_priority_rep(kp, kp.cell_rep[cu], k, MODEL == COLL_ESMC)
+        pr = kp.nrep ? cand_priority_rep(kp, rep_of(cu), k, MODEL == COLL_ESMC)
                      : cand_priority(k, prio_key, MODEL == COLL_ESMC);
         band = op_band(pr, band_mul, nbands);
       }
