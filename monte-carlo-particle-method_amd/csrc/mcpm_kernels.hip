@@ -1543,18 +1543,20 @@ __device__ __forceinline__ bool commit_dsmc(uint32_t i, uint32_t j, uint32_t k, 
 //    each operation stores its priority at both particles of tab (plain
 //    stores; one of the values stored at a particle survives);
 //  * k_check: an operation that does not find its own priority at a
-//    particle lowers the word with atomicMin and marks the particle in
-//    cflag with the round's tag.  After it tab holds, at every particle,
-//    the smallest priority of the round's operations on it, and a particle
-//    left unmarked has exactly one operation on it;
-//  * k_round: an operation wins when each of its particles is unmarked or
-//    holds its priority.  Winners commit; losers store their priority into
-//    the next round's table and are compacted into the next list, and band
-//    r+1 is admitted there.
+//    particle lowers the word with atomicMin and marks the particle's group
+//    (2^MARK_SHIFT consecutive particles share a mark byte of cflag) with
+//    the round's tag.  After it tab holds, at every particle, the smallest
+//    priority of the round's operations on it, and a particle of an
+//    unmarked group has exactly one operation on it;
+//  * k_round: an operation wins when each of its particles is in an
+//    unmarked group or holds its priority.  Winners commit; losers store
+//    their priority into the next round's table and are compacted into the
+//    next list, and band r+1 is admitted there.
 // Every word a round reads was stored in that round, so the tables are never
-// cleared; a stale mark equal to the round's tag only sends an operation to
-// read its word, which then holds its own priority.  Atomics are left to
-// the particles that several operations share.
+// cleared; a stale mark equal to the round's tag, or one that another
+// particle of the group set, only sends an operation to read its word, which
+// then holds its own priority.  Atomics are left to the particles that
+// several operations share.
 // Region x of the operations is processed by the workgroups with
 // blockIdx % 8 == x, which the hardware places on XCD x: each XCD keeps
 // working on the same run of cells, whose particles and reservations stay in
